@@ -1,0 +1,364 @@
+"""cirrus_amd -- MI355X-native MGPCG pressure projection on adaptive octrees.
+
+Python mirror of the C ABI in ``include/cirrus_mgpcg.h`` (ctypes over
+``libcirrus_mgpcg.so``), shaped like the reference's solver API:
+
+* :class:`Problem` -- a device-resident HAGrid (all LEAF/NONLEAF/GHOST tiles,
+  cell types, channels x, b/r, p, Ap, z, tmp), the analogue of
+  ``HADeviceGrid<PoissonTile<float>>`` (src/HAGrid.h:205).
+* :class:`GMGSolver` -- ``GMGSolver(one_over_alpha, prolong_coeff)`` with
+  ``solve``, ``VCycle`` and ``dampedJacobiSolve`` (src/GMGSolver.h:17-52).
+* free functions mirroring src/GMGSolver.h:7-15 and src/PoissonSolver.h:8-31.
+* :class:`Scene` -- the synthetic benchmark workloads.
+
+The library must be built (``python -m cirrus_amd.build``); nothing here falls
+back to a CPU path: a missing library or a failed HIP call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcirrus_mgpcg.so")
+
+LEAF, NONLEAF, GHOST = 1, 2, 4
+DIRICHLET, INTERIOR, NEUMANN = 1, 2, 4
+CH_X, CH_B, CH_R, CH_P, CH_AP, CH_Z, CH_TMP = 0, 1, 1, 2, 3, 4, 5
+NUM_CHANNELS = 6
+SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
+SCENE_UNIFORM, SCENE_SPHERE_BAND, SCENE_PLANE, SCENE_VORTEX = 0, 1, 2, 3
+(KERNEL_GS_SWEEP, KERNEL_APPLY, KERNEL_RESTRICT, KERNEL_PROLONG, KERNEL_JACOBI, KERNEL_UPDATE_XR, KERNEL_COARSE,
+ KERNEL_EMPTY) = range(8)
+
+EXPORTED_SYMBOLS = (
+    "cg_params_default", "cg_last_error", "cg_device_count", "cg_problem_create", "cg_problem_destroy", "cg_problem_info",
+    "cg_problem_topology", "cg_set_channel", "cg_get_channel", "cg_solve", "cg_damped_jacobi_solve",
+    "cg_vcycle", "cg_full_negative_laplacian", "cg_negative_laplacian_same_level", "cg_gauss_seidel_color",
+    "cg_jacobi_sweep", "cg_residual_restrict", "cg_prolongate", "cg_dot", "cg_recenter_leaf_voxels",
+    "cg_calc_cell_types_from_leafs", "cg_benchmark_kernel", "cg_scene_create", "cg_scene_destroy", "cg_scene_tree", "cg_scene_velocity",
+)
+
+
+class TreeDesc(ctypes.Structure):
+    _fields_ = [("h0", ctypes.c_double), ("num_tiles", ctypes.c_int32),
+                ("level", ctypes.c_void_p), ("coord", ctypes.c_void_p),
+                ("type", ctypes.c_void_p), ("cell_type", ctypes.c_void_p)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("max_iters", ctypes.c_int32), ("rel_tol", ctypes.c_double), ("level_iters", ctypes.c_int32),
+                ("coarsest_iters", ctypes.c_int32), ("smoother", ctypes.c_int32), ("omega", ctypes.c_float),
+                ("sync_stride", ctypes.c_int32), ("pure_neumann", ctypes.c_int32),
+                ("one_over_alpha", ctypes.c_float), ("prolong_coeff", ctypes.c_float), ("use_graph", ctypes.c_int32)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int32), ("converged", ctypes.c_int32), ("rel_res", ctypes.c_double),
+                ("bb", ctypes.c_double), ("rr", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SceneParams(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("nroot", ctypes.c_int32 * 3), ("h0", ctypes.c_double),
+        ("uniform_level", ctypes.c_int32), ("has_solid", ctypes.c_int32), ("solid_c", ctypes.c_double * 3),
+        ("solid_r", ctypes.c_double), ("band_c", ctypes.c_double * 3), ("band_r", ctypes.c_double),
+        ("band_w", ctypes.c_double * 2), ("band_level", ctypes.c_int32 * 3), ("plane_z", ctypes.c_double),
+        ("plane_level", ctypes.c_int32 * 2), ("n_tubes", ctypes.c_int32), ("tube_r_lo", ctypes.c_double),
+        ("tube_r_hi", ctypes.c_double), ("vortex_thresh", ctypes.c_double), ("vortex_level", ctypes.c_int32 * 2),
+        ("face_bc", ctypes.c_int32 * 6), ("seed", ctypes.c_uint64), ("num_layers", ctypes.c_int32)]
+
+
+class CirrusError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libcirrus_mgpcg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CirrusError(f"{LIB_PATH} not built: run `python -m cirrus_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, f32, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_double
+        P = ctypes.POINTER
+        sig = {
+            "cg_params_default": (None, [P(Params)]),
+            "cg_last_error": (ctypes.c_char_p, []),
+            "cg_device_count": (i32, []),
+            "cg_problem_create": (i32, [P(TreeDesc), P(vp)]),
+            "cg_problem_destroy": (i32, [vp]),
+            "cg_problem_info": (i32, [vp, P(i32), P(i32), P(i32)]),
+            "cg_problem_topology": (i32, [vp, vp, vp]),
+            "cg_set_channel": (i32, [vp, i32, vp]),
+            "cg_get_channel": (i32, [vp, i32, vp]),
+            "cg_solve": (i32, [vp, P(Params), P(Result)]),
+            "cg_damped_jacobi_solve": (i32, [vp, i32, f64, f32, P(Result)]),
+            "cg_vcycle": (i32, [vp, i32, i32, i32, i32, i32, f32]),
+            "cg_full_negative_laplacian": (i32, [vp, i32, i32, i32]),
+            "cg_negative_laplacian_same_level": (i32, [vp, i32, i32, i32, i32, i32, i32]),
+            "cg_gauss_seidel_color": (i32, [vp, i32, i32, i32, i32]),
+            "cg_jacobi_sweep": (i32, [vp, i32, i32, i32, f32]),
+            "cg_residual_restrict": (i32, [vp, i32, i32, i32, i32, f32]),
+            "cg_prolongate": (i32, [vp, i32, i32, i32, f32]),
+            "cg_dot": (i32, [vp, i32, i32, i32, P(f64)]),
+            "cg_recenter_leaf_voxels": (i32, [vp, i32]),
+            "cg_calc_cell_types_from_leafs": (i32, [P(TreeDesc), vp]),
+            "cg_benchmark_kernel": (i32, [vp, i32, i32, i32, P(f64), P(ctypes.c_int64)]),
+            "cg_scene_create": (i32, [P(SceneParams), P(vp)]),
+            "cg_scene_destroy": (i32, [vp]),
+            "cg_scene_tree": (i32, [vp, P(TreeDesc), P(vp)]),
+            "cg_scene_velocity": (i32, [vp, P(vp)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def device_count():
+    """Number of visible HIP devices (0 without a GPU)."""
+    return int(lib().cg_device_count())
+
+
+def _check(st, what):
+    if st != 0:
+        msg = lib().cg_last_error()
+        raise CirrusError(f"{what} failed (status {st}): {msg.decode() if msg else ''}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def scene_params(**kw):
+    """SceneParams with the plain configuration's defaults: uniform 64^3 (level 3),
+    solid sphere r=0.2 at the centre (NEUMANN), top face DIRICHLET, other walls
+    NEUMANN, seed 0."""
+    p = SceneParams()
+    p.kind = SCENE_UNIFORM
+    p.nroot[:] = [1, 1, 1]
+    p.h0 = 0.125
+    p.uniform_level = 3
+    p.has_solid = 1
+    p.solid_c[:] = [0.5, 0.5, 0.5]
+    p.solid_r = 0.2
+    p.band_c[:] = [0.5, 0.5, 0.5]
+    p.band_r = 0.15
+    p.band_w[:] = [0.05, 0.15]
+    p.band_level[:] = [5, 4, 3]
+    p.plane_z = 0.5
+    p.plane_level[:] = [3, 2]
+    p.n_tubes = 8
+    p.tube_r_lo, p.tube_r_hi = 0.02, 0.06
+    p.vortex_thresh = 0.5
+    p.vortex_level[:] = [6, 4]
+    p.face_bc[:] = [NEUMANN] * 5 + [DIRICHLET]
+    p.seed = 0
+    p.num_layers = 10
+    for k, v in kw.items():
+        f = getattr(p, k)
+        if isinstance(f, ctypes.Array):
+            f[:] = list(v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Tree:
+    """A host-side HAGrid description: tile levels, coordinates, types and cell types."""
+
+    def __init__(self, h0, level, coord, type_, cell_type):
+        self.h0 = float(h0)
+        self.level = np.ascontiguousarray(level, np.int32)
+        self.coord = np.ascontiguousarray(coord, np.int32).reshape(-1, 3)
+        self.type = np.ascontiguousarray(type_, np.uint8)
+        self.cell_type = np.ascontiguousarray(cell_type, np.uint8).reshape(-1, 512)
+        self.n = len(self.level)
+
+    def desc(self):
+        d = TreeDesc()
+        d.h0 = self.h0
+        d.num_tiles = self.n
+        d.level = self.level.ctypes.data
+        d.coord = self.coord.ctypes.data
+        d.type = self.type.ctypes.data
+        d.cell_type = self.cell_type.ctypes.data
+        return d
+
+
+def calc_cell_types_from_leafs(tree):
+    """CalcCellTypesFromLeafs (src/PoissonSolver.cu:7-61) on a host tree (in place)."""
+    d = tree.desc()
+    ct = np.ascontiguousarray(tree.cell_type)
+    _check(lib().cg_calc_cell_types_from_leafs(ctypes.byref(d), _ptr(ct)), "cg_calc_cell_types_from_leafs")
+    tree.cell_type = ct
+    return ct
+
+
+class Scene:
+    """A synthetic workload: the octree, its cell types, the face velocities and
+    the divergence right-hand side, built on the host by the library."""
+
+    def __init__(self, params=None, **kw):
+        self.params = params if params is not None else scene_params(**kw)
+        h = ctypes.c_void_p()
+        _check(lib().cg_scene_create(ctypes.byref(self.params), ctypes.byref(h)), "cg_scene_create")
+        self._h = h
+        d = TreeDesc()
+        rhs = ctypes.c_void_p()
+        _check(lib().cg_scene_tree(h, ctypes.byref(d), ctypes.byref(rhs)), "cg_scene_tree")
+        n = d.num_tiles
+
+        def arr(ptr, ctype, count):
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), shape=(count,)).copy()
+
+        self.tree = Tree(d.h0, arr(d.level, ctypes.c_int32, n), arr(d.coord, ctypes.c_int32, 3 * n),
+                         arr(d.type, ctypes.c_uint8, n), arr(d.cell_type, ctypes.c_uint8, 512 * n))
+        self.rhs = arr(rhs, ctypes.c_float, 512 * n).reshape(n, 512)
+        uvw = ctypes.c_void_p()
+        _check(lib().cg_scene_velocity(h, ctypes.byref(uvw)), "cg_scene_velocity")
+        self.uvw = arr(uvw, ctypes.c_float, 3 * 512 * n).reshape(n, 3, 512)
+        lib().cg_scene_destroy(h)
+        self._h = None
+
+
+class Problem:
+    """A device-resident hierarchy (cg_problem); arrays are in the tree's tile order."""
+
+    def __init__(self, tree):
+        self.tree = tree
+        h = ctypes.c_void_p()
+        d = tree.desc()
+        _check(lib().cg_problem_create(ctypes.byref(d), ctypes.byref(h)), "cg_problem_create")
+        self._h = h
+        n, nl, ml = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        lib().cg_problem_info(h, ctypes.byref(n), ctypes.byref(nl), ctypes.byref(ml))
+        self.n, self.n_leaf, self.max_level = n.value, nl.value, ml.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cg_problem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def topology(self):
+        nbr = np.zeros((self.n, 6), np.int32)
+        par = np.zeros(self.n, np.int32)
+        _check(lib().cg_problem_topology(self._h, _ptr(nbr), _ptr(par)), "cg_problem_topology")
+        return nbr, par
+
+    def set(self, ch, a):
+        a = np.ascontiguousarray(a, np.float32).reshape(self.n, 512)
+        _check(lib().cg_set_channel(self._h, ch, _ptr(a)), "cg_set_channel")
+
+    def get(self, ch):
+        a = np.zeros((self.n, 512), np.float32)
+        _check(lib().cg_get_channel(self._h, ch, _ptr(a)), "cg_get_channel")
+        return a
+
+    def benchmark_kernel(self, kernel, level=0, reps=50):
+        """(average launch duration in us, tiles per launch) of one kernel of
+        the solve, timed with hipEvents on the problem's stream."""
+        us = ctypes.c_double()
+        nt = ctypes.c_int64()
+        _check(lib().cg_benchmark_kernel(self._h, kernel, level, reps, ctypes.byref(us), ctypes.byref(nt)),
+               "cg_benchmark_kernel")
+        return us.value, nt.value
+
+    def dot(self, a, b, types=LEAF):
+        out = ctypes.c_double()
+        _check(lib().cg_dot(self._h, a, b, types, ctypes.byref(out)), "cg_dot")
+        return out.value
+
+
+# --- operator entry points (src/GMGSolver.h:7-15, src/GMGSolver.cu) -----------
+def FullNegativeLaplacian(prob, x_channel, Ax_channel, calc_diag=False):
+    _check(lib().cg_full_negative_laplacian(prob.handle, x_channel, Ax_channel, int(calc_diag)), "FullNegativeLaplacian")
+
+
+def NegativeLaplacianSameLevel(prob, level, subtree_level, tile_types, x_channel, Ax_channel, calc_diag=False):
+    _check(lib().cg_negative_laplacian_same_level(prob.handle, level, subtree_level, tile_types, x_channel,
+                                                  Ax_channel, int(calc_diag)), "NegativeLaplacianSameLevel")
+
+
+def GaussSeidelColor(prob, level, x_channel, rhs_channel, color):
+    _check(lib().cg_gauss_seidel_color(prob.handle, level, x_channel, rhs_channel, color), "GaussSeidelColor")
+
+
+def JacobiSweep(prob, level, x_channel, rhs_channel, omega):
+    _check(lib().cg_jacobi_sweep(prob.handle, level, x_channel, rhs_channel, omega), "JacobiSweep")
+
+
+def ResidualAndRestrict(prob, level, x_channel, rhs_channel, coarse_channel, one_over_alpha=1.0):
+    _check(lib().cg_residual_restrict(prob.handle, level, x_channel, rhs_channel, coarse_channel, one_over_alpha),
+           "ResidualAndRestrict")
+
+
+def ProlongateAndUpdate(prob, level, coarse_channel, fine_channel, prolong_coeff=1.0):
+    _check(lib().cg_prolongate(prob.handle, level, coarse_channel, fine_channel, prolong_coeff), "ProlongateAndUpdate")
+
+
+def ReCenterLeafVoxels(prob, channel):
+    _check(lib().cg_recenter_leaf_voxels(prob.handle, channel), "ReCenterLeafVoxels")
+
+
+def default_params(**kw):
+    p = Params()
+    lib().cg_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class GMGSolver:
+    """GMGSolver (src/GMGSolver.h:17-52) over a :class:`Problem`."""
+
+    def __init__(self, one_over_alpha=1.0, prolong_coeff=1.0):
+        self.one_over_alpha = float(one_over_alpha)
+        self.prolong_coeff = float(prolong_coeff)
+
+    def solve(self, prob, verbose=False, max_iters=1000, relative_tolerance=1e-6, level_iters=1, coarsest_iters=10,
+              sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, use_graph=True):
+        """Returns (iters, relative residual) like the reference; the full result
+        is kept in ``self.last``."""
+        p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
+                           coarsest_iters=coarsest_iters, smoother=smoother, omega=omega,
+                           sync_stride=max(1, sync_stride), pure_neumann=int(is_pure_neumann),
+                           one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff,
+                           use_graph=int(use_graph))
+        r = Result()
+        _check(lib().cg_solve(prob.handle, ctypes.byref(p), ctypes.byref(r)), "GMGSolver.solve")
+        self.last = r.as_dict()
+        if verbose:
+            print(f"MGPCG {r.iters} iters, rel residual {r.rel_res:.3e}, {r.seconds * 1e3:.3f} ms")
+        return r.iters, r.rel_res
+
+    def VCycle(self, prob, x_channel, f_channel, level_iters=1, coarsest_iters=10, smoother=SMOOTH_RBGS,
+               omega=2.0 / 3.0):
+        _check(lib().cg_vcycle(prob.handle, x_channel, f_channel, level_iters, coarsest_iters, smoother, omega),
+               "GMGSolver.VCycle")
+
+    def dampedJacobiSolve(self, prob, verbose=False, max_iters=100, relative_tolerance=1e-6, omega=2.0 / 3.0):
+        r = Result()
+        _check(lib().cg_damped_jacobi_solve(prob.handle, max_iters, relative_tolerance, omega, ctypes.byref(r)),
+               "GMGSolver.dampedJacobiSolve")
+        self.last = r.as_dict()
+        return r.iters, r.rel_res

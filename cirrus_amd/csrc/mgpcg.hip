@@ -1,0 +1,989 @@
+// Host side of the MI355X MGPCG library: problem upload, worklists, the
+// V-cycle and CG launch sequences, hipGraph replay of one CG iteration with
+// device-side convergence flags, and the C ABI of include/cirrus_mgpcg.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "cirrus_mgpcg.h"
+#include "kernels.hpp"
+#include "tree.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Fail {
+  cg_status st;
+};
+[[noreturn]] void fail(cg_status st, const std::string& msg) {
+  g_err = msg;
+  throw Fail{st};
+}
+#define HIPC(expr)                                                                                     \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess)                                                                              \
+      fail(e_ == hipErrorOutOfMemory ? CG_ERR_NOMEM : CG_ERR_HIP,                                     \
+           std::string(#expr) + ": " + hipGetErrorString(e_));                                         \
+  } while (0)
+
+template <class F>
+cg_status guard(F f) {
+  try {
+    f();
+    return CG_OK;
+  } catch (const Fail& e) {
+    return e.st;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return CG_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return CG_ERR_INVALID;
+  }
+}
+
+struct DList {
+  int off = 0, n = 0;
+};
+
+}  // namespace
+
+using namespace cg;
+
+struct cg_problem {
+  int n = 0;               // tiles
+  int max_level = 0;
+  int n_leaf = 0, n_ghost = 0;
+  std::vector<int> ext_of;  // internal -> caller index
+  std::vector<int> int_of;  // caller -> internal index
+  std::vector<DevTile> tiles;
+  std::vector<int32_t> gchild;   // 8 ghost-child tile indices per row, then their 8 ghost slots
+  std::vector<int32_t> lists;  // host copy of all worklists
+  DList leaf, gather, applyl, ghosts;
+  std::vector<DList> act, lvl;
+  CoarsePlan coarse;             // levels 0..coarse.C resident in LDS (C = -1: none)
+  std::vector<CoarseSlot> coarse_slots;
+  CoarseSlot* d_coarse = nullptr;
+
+  // device
+  DevTile* d_tiles = nullptr;
+  int32_t* d_gchild = nullptr;
+  uint8_t* d_ctype = nullptr;
+  float* d_ch[CG_NUM_CHANNELS] = {};
+  float* d_zb = nullptr;
+  float* d_gsum = nullptr;
+  float* d_gext = nullptr;
+  ProbState* d_ps = nullptr;
+  double* d_part = nullptr;
+  double* d_part2 = nullptr;
+  double* d_scalar = nullptr;
+  int32_t* d_lists = nullptr;
+  int32_t* d_tmp_list = nullptr;
+  int tmp_list_cap = 0;
+  hipStream_t stream = nullptr;
+  int* h_pin = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipGraphExec_t graph = nullptr;
+  std::string graph_key;
+
+  const int32_t* L(const DList& d) const { return d_lists + d.off; }
+  Ctx ctx(float* z, float* r) const {
+    Ctx c;
+    c.tiles = d_tiles;
+    c.gchild = d_gchild;
+    c.gcslot = d_gchild + gchild.size() / 2;
+    c.ctype = d_ctype;
+    c.x = d_ch[CG_CH_X];
+    c.r = r;
+    c.p = d_ch[CG_CH_P];
+    c.ap = d_ch[CG_CH_AP];
+    c.z = z;
+    c.rhsv = d_ch[CG_CH_TMP];
+    c.zb = d_zb;
+    c.gsum = d_gsum;
+    c.gext = d_gext;
+    c.ps = d_ps;
+    c.part = d_part;
+    c.part2 = d_part2;
+    return c;
+  }
+  Ctx ctx() const { return ctx(d_ch[CG_CH_Z], d_ch[CG_CH_R]); }
+  ~cg_problem();
+};
+
+cg_problem::~cg_problem() {
+  if (graph) (void)hipGraphExecDestroy(graph);
+  for (auto e : ev) if (e) (void)hipEventDestroy(e);
+  if (h_pin) (void)hipHostFree(h_pin);
+  void* ptrs[] = {d_tiles, d_gchild, d_ctype, d_zb, d_gsum, d_gext, d_ps, d_part, d_part2, d_scalar, d_lists, d_tmp_list,
+                  d_coarse};
+  for (void* q : ptrs) if (q) (void)hipFree(q);
+  for (auto q : d_ch) if (q) (void)hipFree(q);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+constexpr int kSwizzle = F_SWIZZLE;
+constexpr int kCoarseTiles = 64;
+
+inline unsigned grid_for(int n, int flags) { return (flags & F_SWIZZLE) ? unsigned((n + 7) & ~7) : unsigned(n); }
+
+// ---------------------------------------------------------------------------
+// Problem construction
+// ---------------------------------------------------------------------------
+void build_problem(const cg_tree_desc* d, cg_problem& P) {
+  if (!d || d->num_tiles <= 0 || !d->level || !d->coord || !d->type || !d->cell_type)
+    fail(CG_ERR_INVALID, "cg_problem_create: empty or incomplete tree description");
+  if (!(d->h0 > 0)) fail(CG_ERR_INVALID, "cg_problem_create: h0 must be positive");
+  const int n = d->num_tiles;
+  P.n = n;
+  std::vector<uint64_t> mk(n);
+  for (int t = 0; t < n; t++) {
+    int l = d->level[t];
+    const int32_t* c = d->coord + 3 * t;
+    if (l < 0 || l > 30) fail(CG_ERR_INVALID, "tile " + std::to_string(t) + ": level out of range");
+    if (!key_ok(c[0], c[1], c[2])) fail(CG_ERR_INVALID, "tile " + std::to_string(t) + ": coordinate out of range");
+    uint8_t ty = d->type[t];
+    if (ty != LEAF && ty != NONLEAF && ty != GHOST) fail(CG_ERR_INVALID, "tile " + std::to_string(t) + ": bad tile type");
+    mk[t] = morton3(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]));
+  }
+  P.ext_of.resize(n);
+  for (int t = 0; t < n; t++) P.ext_of[t] = t;
+  std::sort(P.ext_of.begin(), P.ext_of.end(), [&](int a, int b) {
+    if (d->level[a] != d->level[b]) return d->level[a] < d->level[b];
+    return mk[a] < mk[b];
+  });
+  P.int_of.assign(n, -1);
+  for (int i = 0; i < n; i++) P.int_of[P.ext_of[i]] = i;
+  std::unordered_map<uint64_t, int> idx;
+  idx.reserve(size_t(n) * 2);
+  std::vector<int> lev(n);
+  std::vector<std::array<int32_t, 3>> cd(n);
+  std::vector<uint8_t> ty(n);
+  for (int i = 0; i < n; i++) {
+    int e = P.ext_of[i];
+    lev[i] = d->level[e];
+    cd[i] = {d->coord[3 * e], d->coord[3 * e + 1], d->coord[3 * e + 2]};
+    ty[i] = d->type[e];
+    if (!idx.emplace(tile_key(lev[i], cd[i][0], cd[i][1], cd[i][2]), i).second)
+      fail(CG_ERR_INVALID, "duplicate tile at level " + std::to_string(lev[i]));
+  }
+  auto find = [&](int l, int x, int y, int z) -> int {
+    if (l < 0 || !key_ok(x, y, z)) return -1;
+    auto it = idx.find(tile_key(l, x, y, z));
+    return it == idx.end() ? -1 : it->second;
+  };
+  P.max_level = 0;
+  for (int i = 0; i < n; i++) P.max_level = std::max(P.max_level, lev[i]);
+  P.tiles.assign(n, DevTile());
+  int nleaf = 0, nghost = 0;
+  std::vector<int> leaf_slot(n, -1), gslot(n, -1);
+  for (int i = 0; i < n; i++) {
+    if (ty[i] == LEAF) leaf_slot[i] = nleaf++;
+    if (ty[i] == GHOST) gslot[i] = nghost++;
+  }
+  P.n_leaf = nleaf;
+  P.n_ghost = nghost;
+  if (nleaf == 0) fail(CG_ERR_INVALID, "tree has no LEAF tiles");
+  std::vector<std::array<int32_t, 8>> gch;
+  for (int i = 0; i < n; i++) {
+    DevTile& T = P.tiles[i];
+    std::memset(&T, 0, sizeof(T));
+    const int l = lev[i];
+    for (int f = 0; f < 6; f++) {
+      int q[3] = {cd[i][0], cd[i][1], cd[i][2]};
+      q[f % 3] += (f < 3) ? -1 : 1;
+      int nb = find(l, q[0], q[1], q[2]);
+      T.nbr[f] = nb;
+      T.nbr_type[f] = nb >= 0 ? ty[nb] : 0;
+    }
+    T.parent = -1;
+    if (l > 0) {
+      int p = find(l - 1, cd[i][0] >> 1, cd[i][1] >> 1, cd[i][2] >> 1);
+      if (p < 0) fail(CG_ERR_INVALID, "tile at level " + std::to_string(l) + " has no parent tile");
+      if (ty[i] == GHOST && ty[p] != LEAF) fail(CG_ERR_INVALID, "a GHOST tile's parent must be a LEAF");
+      if (ty[i] != GHOST && ty[p] != NONLEAF) fail(CG_ERR_INVALID, "a LEAF/NONLEAF tile's parent must be a NONLEAF");
+      T.parent = p;
+    }
+    T.octant = uint8_t((cd[i][0] & 1) | ((cd[i][1] & 1) << 1) | ((cd[i][2] & 1) << 2));
+    T.prob = 0;
+    T.leaf_slot = leaf_slot[i];
+    T.gslot = gslot[i];
+    T.gchild = -1;
+    T.h = float(d->h0 / double(1u << l));
+    T.level = uint8_t(l);
+    T.type = ty[i];
+    if (ty[i] == NONLEAF) {
+      for (int o = 0; o < 8; o++) {
+        int ch = find(l + 1, 2 * cd[i][0] + (o & 1), 2 * cd[i][1] + ((o >> 1) & 1), 2 * cd[i][2] + ((o >> 2) & 1));
+        if (ch < 0 || ty[ch] == GHOST) fail(CG_ERR_INVALID, "a NONLEAF tile must have 8 LEAF/NONLEAF children");
+      }
+    }
+    if (ty[i] == LEAF) {
+      std::array<int32_t, 8> g;
+      bool any = false;
+      for (int o = 0; o < 8; o++) {
+        int ch = find(l + 1, 2 * cd[i][0] + (o & 1), 2 * cd[i][1] + ((o >> 1) & 1), 2 * cd[i][2] + ((o >> 2) & 1));
+        g[o] = (ch >= 0 && ty[ch] == GHOST) ? ch : -1;
+        if (ch >= 0 && ty[ch] != GHOST) fail(CG_ERR_INVALID, "a LEAF tile can only have GHOST children");
+        any |= g[o] >= 0;
+      }
+      if (any) {
+        T.gchild = int32_t(gch.size());
+        gch.push_back(g);
+      }
+    }
+  }
+  P.gchild.assign(gch.size() * 16, -1);
+  for (size_t r = 0; r < gch.size(); r++)
+    for (int o = 0; o < 8; o++) {
+      P.gchild[r * 8 + o] = gch[r][o];
+      P.gchild[gch.size() * 8 + r * 8 + o] = gch[r][o] >= 0 ? gslot[gch[r][o]] : -1;
+    }
+
+  // worklists (canonical order)
+  auto& W = P.lists;
+  W.clear();
+  auto add_list = [&](auto pred) {
+    DList dl;
+    dl.off = int(W.size());
+    for (int i = 0; i < n; i++) if (pred(i)) W.push_back(i);
+    dl.n = int(W.size()) - dl.off;
+    return dl;
+  };
+  P.leaf = add_list([&](int i) { return ty[i] == LEAF; });
+  P.gather = add_list([&](int i) { return ty[i] == LEAF && P.tiles[i].gchild >= 0; });
+  P.applyl = add_list([&](int i) { return ty[i] == LEAF || ty[i] == GHOST; });
+  P.ghosts = add_list([&](int i) { return ty[i] == GHOST; });
+  P.act.assign(P.max_level + 1, DList());
+  P.lvl.assign(P.max_level + 1, DList());
+  for (int l = 0; l <= P.max_level; l++) {
+    P.act[l] = add_list([&](int i) { return lev[i] == l && ty[i] != GHOST; });
+    P.lvl[l] = add_list([&](int i) { return lev[i] == l; });
+  }
+  // Coarse levels kept in LDS by one workgroup: the deepest C whose levels
+  // 0..C hold at most CL_MAX_TILES tiles in total (-1 if level 0 alone is larger).
+  int C = -1, total = 0;
+  while (C + 1 <= P.max_level && C + 1 < CL_MAX_LEVELS && total + P.lvl[C + 1].n <= CL_MAX_TILES) {
+    C++;
+    total += P.lvl[C].n;
+  }
+  if (const char* e = std::getenv("CIRRUS_COARSE_LEVELS")) C = std::min(C, std::atoi(e) - 1);
+  std::memset(&P.coarse, 0, sizeof(P.coarse));
+  P.coarse.C = C;
+  P.coarse_slots.clear();
+  if (C >= 0) {
+    std::vector<int> slot_of(n, -1);
+    for (int l = 0; l <= C; l++) {
+      P.coarse.lvl_begin[l] = int(P.coarse_slots.size());
+      for (int q = 0; q < P.lvl[l].n; q++) {
+        int t = W[P.lvl[l].off + q];
+        slot_of[t] = int(P.coarse_slots.size());
+        CoarseSlot cs;
+        std::memset(&cs, 0, sizeof(cs));
+        cs.tile = t;
+        P.coarse_slots.push_back(cs);
+      }
+    }
+    P.coarse.lvl_begin[C + 1] = int(P.coarse_slots.size());
+    P.coarse.nslots = int(P.coarse_slots.size());
+    for (auto& cs : P.coarse_slots) {
+      const DevTile& T = P.tiles[cs.tile];
+      for (int f = 0; f < 6; f++) cs.nbr[f] = T.nbr[f] < 0 ? -1 : slot_of[T.nbr[f]];
+      cs.parent = T.parent < 0 ? -1 : slot_of[T.parent];
+      cs.level = T.level;
+      cs.h = T.h;
+      cs.type = T.type;
+      cs.octant = T.octant;
+    }
+  }
+  // device buffers
+  HIPC(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+  const size_t cells = size_t(n) * 512;
+  HIPC(hipMalloc(&P.d_tiles, sizeof(DevTile) * n));
+  HIPC(hipMalloc(&P.d_gchild, sizeof(int32_t) * std::max<size_t>(16, P.gchild.size())));
+  HIPC(hipMalloc(&P.d_ctype, cells));
+  for (int c = 0; c < CG_NUM_CHANNELS; c++) {
+    HIPC(hipMalloc(&P.d_ch[c], cells * sizeof(float)));
+    HIPC(hipMemsetAsync(P.d_ch[c], 0, cells * sizeof(float), P.stream));
+  }
+  HIPC(hipMalloc(&P.d_zb, cells * sizeof(float)));
+  HIPC(hipMemsetAsync(P.d_zb, 0, cells * sizeof(float), P.stream));
+  size_t g64 = size_t(std::max(1, nghost)) * 64 * sizeof(float);
+  HIPC(hipMalloc(&P.d_gsum, g64));
+  HIPC(hipMalloc(&P.d_gext, g64));
+  HIPC(hipMemsetAsync(P.d_gsum, 0, g64, P.stream));
+  HIPC(hipMemsetAsync(P.d_gext, 0, g64, P.stream));
+  HIPC(hipMalloc(&P.d_ps, sizeof(ProbState)));
+  HIPC(hipMemsetAsync(P.d_ps, 0, sizeof(ProbState), P.stream));
+  size_t np = size_t(std::max(n, 64));
+  HIPC(hipMalloc(&P.d_part, np * sizeof(double)));
+  HIPC(hipMalloc(&P.d_part2, np * sizeof(double)));
+  HIPC(hipMalloc(&P.d_scalar, 8 * sizeof(double)));
+  HIPC(hipMalloc(&P.d_lists, sizeof(int32_t) * W.size()));
+  if (!P.coarse_slots.empty()) {
+    HIPC(hipMalloc(&P.d_coarse, sizeof(CoarseSlot) * P.coarse_slots.size()));
+    HIPC(hipMemcpyAsync(P.d_coarse, P.coarse_slots.data(), sizeof(CoarseSlot) * P.coarse_slots.size(),
+                        hipMemcpyHostToDevice, P.stream));
+    P.coarse.slots = P.d_coarse;
+  }
+  HIPC(hipMemcpyAsync(P.d_tiles, P.tiles.data(), sizeof(DevTile) * n, hipMemcpyHostToDevice, P.stream));
+  if (!P.gchild.empty())
+    HIPC(hipMemcpyAsync(P.d_gchild, P.gchild.data(), sizeof(int32_t) * P.gchild.size(), hipMemcpyHostToDevice, P.stream));
+  HIPC(hipMemcpyAsync(P.d_lists, W.data(), sizeof(int32_t) * W.size(), hipMemcpyHostToDevice, P.stream));
+  std::vector<uint8_t> ct(cells);
+  for (int i = 0; i < n; i++) std::memcpy(&ct[size_t(i) * 512], d->cell_type + size_t(P.ext_of[i]) * 512, 512);
+  HIPC(hipMemcpyAsync(P.d_ctype, ct.data(), cells, hipMemcpyHostToDevice, P.stream));
+  HIPC(hipHostMalloc(&P.h_pin, 2 * sizeof(int), hipHostMallocDefault));
+  HIPC(hipEventCreateWithFlags(&P.ev[0], hipEventDisableTiming));
+  HIPC(hipEventCreateWithFlags(&P.ev[1], hipEventDisableTiming));
+  HIPC(hipStreamSynchronize(P.stream));
+}
+
+void check_channel(int ch) {
+  if (ch < 0 || ch >= CG_NUM_CHANNELS) fail(CG_ERR_INVALID, "channel index out of range");
+}
+
+// upload a ProbState for a single-problem run
+void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann) {
+  ProbState s;
+  std::memset(&s, 0, sizeof(s));
+  s.tol2 = tol * tol;
+  s.max_iters = max_iters;
+  s.leaf_begin = 0;
+  s.leaf_count = P.n_leaf;
+  s.pure_neumann = pure_neumann;
+  HIPC(hipStreamSynchronize(P.stream));
+  HIPC(hipMemcpy(P.d_ps, &s, sizeof(s), hipMemcpyHostToDevice));
+}
+
+// ---------------------------------------------------------------------------
+// Launch sequences
+// ---------------------------------------------------------------------------
+struct VParams {
+  int nu = 1, coarsest = 10, smoother = CG_SMOOTH_RBGS;
+  float omega = 2.f / 3.f, ooa = 1.f, pcoef = 1.f;
+  int check_done = 1;
+};
+
+void launch_gs(cg_problem& P, const Ctx& c, int l, float* Z, const float* R, int color, int flags) {
+  const DList& a = P.act[l];
+  if (a.n == 0) return;
+  k_gs_color<<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
+}
+
+// One V-cycle Z <- M^-1 R (GMGSolver::VCycle, GMGSolver.cu:383-447): the zero
+// initial guess, rhs = R on LEAF tiles and zero ghost values of the reference's
+// init pass are implicit in the kernels' flags.
+void enqueue_vcycle(cg_problem& P, float* Z, float* R, const VParams& v) {
+  const Ctx c = P.ctx(Z, R);
+  const int base = F_VCYCLE_RHS | kSwizzle | (v.check_done ? F_CHECK_DONE : 0);
+  const int L = P.max_level, C = P.coarse.C;  // levels > max(C, 0) use grid-wide launches
+  float* ZB = P.d_zb;
+  for (int l = L; l > std::max(C, 0); l--) {
+    if (v.smoother == CG_SMOOTH_RBGS) {
+      for (int s = 0; s < v.nu; s++) {
+        launch_gs(P, c, l, Z, R, 0, base | F_GHOST_ZERO | (s == 0 ? F_ZERO_X : 0));
+        launch_gs(P, c, l, Z, R, 1, base | F_GHOST_ZERO);
+      }
+    } else {
+      const DList& a = P.act[l];
+      const float* src = Z;
+      for (int s = 0; s < v.nu; s++) {
+        float* dst = ((v.nu - 1 - s) % 2 == 0) ? Z : ZB;
+        int fl = base | F_GHOST_ZERO | (s == 0 ? F_ZERO_X : 0);
+        if (a.n) k_jacobi<<<grid_for(a.n, fl), 64, 0, P.stream>>>(c, P.L(a), a.n, l, src, Z, dst, R, v.omega, fl);
+        src = dst;
+      }
+    }
+    const DList& al = P.lvl[l];
+    int fl = base | F_VCYCLE_OUT | F_GHOST_ZERO | (v.nu == 0 ? F_ZERO_X : 0);
+    if (al.n) k_restrict<<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, Z, R, nullptr, v.ooa, fl);
+  }
+  if (C >= 0) {
+    VCycleArgs va{v.nu, v.coarsest, v.smoother, v.omega, v.ooa, v.pcoef};
+    k_coarse_lds<<<1, CL_THREADS, 0, P.stream>>>(c, P.coarse, va, Z, R, base & ~F_SWIZZLE);
+  } else {
+    // level 0 too large for LDS: bottom sweeps as grid-wide launches
+    const int it0 = v.coarsest / 2;
+    if (it0 == 0 && P.act[0].n) k_fill_tiles<<<P.act[0].n, 64, 0, P.stream>>>(P.L(P.act[0]), P.act[0].n, Z, 0.f);
+    for (int i = 0; i < it0; i++) {
+      launch_gs(P, c, 0, Z, R, 0, base | ((i == 0) ? F_ZERO_X : 0));
+      launch_gs(P, c, 0, Z, R, 1, base);
+    }
+    for (int i = 0; i < it0; i++) {
+      launch_gs(P, c, 0, Z, R, 1, base);
+      launch_gs(P, c, 0, Z, R, 0, base);
+    }
+  }
+  for (int l = std::max(C, 0) + 1; l <= L; l++) {
+    const DList& al = P.lvl[l];
+    const bool jac = v.smoother != CG_SMOOTH_RBGS;
+    float* XO = (jac && (v.nu % 2 == 1)) ? ZB : Z;
+    int fl = base | (v.nu == 0 ? F_ZERO_X : 0);
+    if (al.n) k_prolong<<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, Z, Z, XO, Z, v.pcoef, fl);
+    if (!jac) {
+      for (int s = 0; s < v.nu; s++) {
+        launch_gs(P, c, l, Z, R, 1, base);
+        launch_gs(P, c, l, Z, R, 0, base);
+      }
+    } else {
+      const DList& a = P.act[l];
+      const float* src = XO;
+      for (int s = 0; s < v.nu; s++) {
+        float* dst = ((v.nu - 1 - s) % 2 == 0) ? Z : ZB;
+        if (a.n) k_jacobi<<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, l, src, Z, dst, R, v.omega, base);
+        src = dst;
+      }
+    }
+  }
+}
+
+// One CG iteration of the rotated loop (GMGSolver.cu:573-653 shifted so that it
+// starts with the V-cycle): [re-centre r], z = M^-1 r, gamma/beta, p, A p and
+// alpha, x/r and the convergence test.
+void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann) {
+  const Ctx c = P.ctx();
+  const int fl = kSwizzle | F_CHECK_DONE;
+  const unsigned np = 1;
+  if (pure_neumann) {
+    k_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, fl);
+    k_finalize<<<np, 64, 0, P.stream>>>(c, FIN_MEAN, fl);
+    k_subtract_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, fl);
+  }
+  enqueue_vcycle(P, c.z, c.r, v);
+  k_cg_dot_rz<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
+  k_finalize<<<np, 64, 0, P.stream>>>(c, FIN_RZ, fl);
+  k_cg_update_p<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
+  k_apply<<<grid_for(P.applyl.n, fl), 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, c.p, c.ap, -1, 0, 1, fl);
+  if (P.gather.n) k_ghost_gather<<<grid_for(P.gather.n, fl), 64, 0, P.stream>>>(c, P.L(P.gather), P.gather.n, fl);
+  k_finalize<<<np, 64, 0, P.stream>>>(c, FIN_PAP, fl);
+  k_cg_update_xr<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
+  k_finalize<<<np, 64, 0, P.stream>>>(c, FIN_RR, fl);
+}
+
+void ensure_tmp_list(cg_problem& P, const std::vector<int32_t>& l) {
+  if (int(l.size()) > P.tmp_list_cap) {
+    if (P.d_tmp_list) HIPC(hipFree(P.d_tmp_list));
+    P.d_tmp_list = nullptr;
+    P.tmp_list_cap = std::max<int>(int(l.size()), 64);
+    HIPC(hipMalloc(&P.d_tmp_list, sizeof(int32_t) * P.tmp_list_cap));
+  }
+  if (!l.empty()) HIPC(hipMemcpyAsync(P.d_tmp_list, l.data(), sizeof(int32_t) * l.size(), hipMemcpyHostToDevice, P.stream));
+}
+
+double dot_types(cg_problem& P, const float* A, const float* B, int types) {
+  std::vector<int32_t> l;
+  for (int i = 0; i < P.n; i++) if (P.tiles[i].type & types) l.push_back(i);
+  if (l.empty()) return 0.0;
+  ensure_tmp_list(P, l);
+  const Ctx c = P.ctx();
+  k_dot_tiles<<<unsigned(l.size()), 64, 0, P.stream>>>(c, P.d_tmp_list, int(l.size()), A, B, P.d_part);
+  k_sum_partials<<<1, 64, 0, P.stream>>>(P.d_part, int(l.size()), P.d_scalar);
+  double out = 0;
+  HIPC(hipMemcpyAsync(&out, P.d_scalar, sizeof(double), hipMemcpyDeviceToHost, P.stream));
+  HIPC(hipStreamSynchronize(P.stream));
+  return out;
+}
+
+void full_negative_laplacian(cg_problem& P, float* X, float* AX, int diag) {
+  const Ctx c = P.ctx();
+  if (P.ghosts.n) k_propagate_ghosts<<<P.ghosts.n, 64, 0, P.stream>>>(c, P.L(P.ghosts), P.ghosts.n, X, X);
+  k_apply<<<P.applyl.n, 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, X, AX, -1, diag, 0, 0);
+  if (P.ghosts.n) k_accumulate_ghosts<<<P.ghosts.n, 64, 0, P.stream>>>(c, P.L(P.ghosts), P.ghosts.n, AX);
+}
+
+ProbState read_state(cg_problem& P) {
+  ProbState s;
+  HIPC(hipMemcpyAsync(&s, P.d_ps, sizeof(s), hipMemcpyDeviceToHost, P.stream));
+  HIPC(hipStreamSynchronize(P.stream));
+  return s;
+}
+
+void solve(cg_problem& P, const cg_params& prm, cg_result& res) {
+  if (prm.max_iters < 0 || !(prm.rel_tol >= 0) || prm.level_iters < 0 || prm.coarsest_iters < 0)
+    fail(CG_ERR_INVALID, "cg_solve: invalid parameters");
+  if (prm.smoother != CG_SMOOTH_RBGS && prm.smoother != CG_SMOOTH_JACOBI) fail(CG_ERR_INVALID, "cg_solve: unknown smoother");
+  VParams v;
+  v.nu = prm.level_iters;
+  v.coarsest = prm.coarsest_iters;
+  v.smoother = prm.smoother;
+  v.omega = prm.omega;
+  v.ooa = prm.one_over_alpha;
+  v.pcoef = prm.prolong_coeff;
+  const int stride = std::max(1, prm.sync_stride);
+  auto t0 = std::chrono::steady_clock::now();
+  set_prob_state(P, prm.rel_tol, prm.max_iters, prm.pure_neumann);
+  const Ctx c = P.ctx();
+  k_cg_init<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, kSwizzle);
+  k_finalize<<<1, 64, 0, P.stream>>>(c, FIN_BB, 0);
+  HIPC(hipGetLastError());
+
+  char keybuf[160];
+  std::snprintf(keybuf, sizeof(keybuf), "%d/%d/%d/%a/%a/%a/%d", v.nu, v.coarsest, v.smoother, double(v.omega), double(v.ooa),
+                double(v.pcoef), prm.pure_neumann);
+  const bool use_graph = prm.use_graph != 0;
+  if (use_graph && P.graph_key != keybuf) {
+    if (P.graph) HIPC(hipGraphExecDestroy(P.graph));
+    P.graph = nullptr;
+    hipGraph_t g;
+    HIPC(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
+    enqueue_cg_iteration(P, v, prm.pure_neumann);
+    HIPC(hipStreamEndCapture(P.stream, &g));
+    HIPC(hipGraphInstantiate(&P.graph, g, nullptr, nullptr, 0));
+    HIPC(hipGraphDestroy(g));
+    P.graph_key = keybuf;
+  }
+  auto one_iter = [&]() {
+    if (use_graph) HIPC(hipGraphLaunch(P.graph, P.stream));
+    else enqueue_cg_iteration(P, v, prm.pure_neumann);
+  };
+  // Poll the done flag one chunk behind the enqueue front, so the device never
+  // waits for the host; iterations after convergence exit at their first load.
+  int enq = 0, slot = 0;
+  bool have_prev = false;
+  while (true) {
+    if (prm.max_iters == 0) break;
+    for (int s = 0; s < stride; s++) { one_iter(); enq++; }
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(&P.h_pin[slot], &P.d_ps->done, sizeof(int), hipMemcpyDeviceToHost, P.stream));
+    HIPC(hipEventRecord(P.ev[slot], P.stream));
+    if (have_prev) {
+      HIPC(hipEventSynchronize(P.ev[1 - slot]));
+      if (P.h_pin[1 - slot]) break;
+    }
+    if (enq >= prm.max_iters) {
+      HIPC(hipEventSynchronize(P.ev[slot]));
+      break;
+    }
+    have_prev = true;
+    slot = 1 - slot;
+  }
+  HIPC(hipStreamSynchronize(P.stream));
+  auto t1 = std::chrono::steady_clock::now();
+  ProbState s = read_state(P);
+  res.iters = s.iter;
+  res.converged = s.converged;
+  res.bb = s.bb;
+  res.rr = s.rr;
+  res.rel_res = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
+  res.seconds = std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+struct cg_scene {
+  Scene s;
+  std::vector<int32_t> coord_flat;
+};
+
+extern "C" {
+
+const char* cg_last_error(void) { return g_err.c_str(); }
+
+int32_t cg_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void cg_params_default(cg_params* p) {
+  if (!p) return;
+  p->max_iters = 1000;
+  p->rel_tol = 1e-6;
+  p->level_iters = 1;
+  p->coarsest_iters = 10;
+  p->smoother = CG_SMOOTH_RBGS;
+  p->omega = 2.0f / 3.0f;
+  p->sync_stride = 1;
+  p->pure_neumann = 0;
+  p->one_over_alpha = 1.0f;
+  p->prolong_coeff = 1.0f;
+  p->use_graph = 1;
+}
+
+cg_status cg_problem_create(const cg_tree_desc* desc, cg_problem** out) {
+  if (!out) { g_err = "cg_problem_create: null output"; return CG_ERR_INVALID; }
+  *out = nullptr;
+  auto P = std::make_unique<cg_problem>();
+  cg_status st = guard([&] { build_problem(desc, *P); });
+  if (st == CG_OK) *out = P.release();
+  return st;
+}
+
+cg_status cg_problem_destroy(cg_problem* p) {
+  delete p;
+  return CG_OK;
+}
+
+cg_status cg_problem_info(const cg_problem* p, int32_t* num_tiles, int32_t* num_leaf_tiles, int32_t* max_level) {
+  if (!p) { g_err = "null problem"; return CG_ERR_INVALID; }
+  if (num_tiles) *num_tiles = p->n;
+  if (num_leaf_tiles) *num_leaf_tiles = p->n_leaf;
+  if (max_level) *max_level = p->max_level;
+  return CG_OK;
+}
+
+cg_status cg_problem_topology(const cg_problem* p, int32_t* nbr, int32_t* parent) {
+  if (!p) { g_err = "null problem"; return CG_ERR_INVALID; }
+  for (int i = 0; i < p->n; i++) {
+    int e = p->ext_of[i];
+    if (nbr)
+      for (int f = 0; f < 6; f++) nbr[6 * e + f] = p->tiles[i].nbr[f] < 0 ? -1 : p->ext_of[p->tiles[i].nbr[f]];
+    if (parent) parent[e] = p->tiles[i].parent < 0 ? -1 : p->ext_of[p->tiles[i].parent];
+  }
+  return CG_OK;
+}
+
+cg_status cg_set_channel(cg_problem* p, int32_t channel, const float* host) {
+  return guard([&] {
+    if (!p || !host) fail(CG_ERR_INVALID, "cg_set_channel: null argument");
+    check_channel(channel);
+    std::vector<float> buf(size_t(p->n) * 512);
+    for (int i = 0; i < p->n; i++) std::memcpy(&buf[size_t(i) * 512], host + size_t(p->ext_of[i]) * 512, 512 * sizeof(float));
+    HIPC(hipMemcpyAsync(p->d_ch[channel], buf.data(), buf.size() * sizeof(float), hipMemcpyHostToDevice, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+cg_status cg_get_channel(cg_problem* p, int32_t channel, float* host) {
+  return guard([&] {
+    if (!p || !host) fail(CG_ERR_INVALID, "cg_get_channel: null argument");
+    check_channel(channel);
+    std::vector<float> buf(size_t(p->n) * 512);
+    HIPC(hipMemcpyAsync(buf.data(), p->d_ch[channel], buf.size() * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+    for (int i = 0; i < p->n; i++) std::memcpy(host + size_t(p->ext_of[i]) * 512, &buf[size_t(i) * 512], 512 * sizeof(float));
+  });
+}
+
+cg_status cg_solve(cg_problem* p, const cg_params* prm, cg_result* res) {
+  return guard([&] {
+    if (!p || !prm || !res) fail(CG_ERR_INVALID, "cg_solve: null argument");
+    solve(*p, *prm, *res);
+  });
+}
+
+cg_status cg_damped_jacobi_solve(cg_problem* p, int32_t max_iters, double rel_tol, float omega, cg_result* res) {
+  return guard([&] {
+    if (!p || !res) fail(cg_status(CG_ERR_INVALID), "cg_damped_jacobi_solve: null argument");
+    auto t0 = std::chrono::steady_clock::now();
+    cg_problem& P = *p;
+    const Ctx c = P.ctx();
+    float* X = P.d_ch[CG_CH_X];
+    float* B = P.d_ch[CG_CH_B];
+    float* AX = P.d_ch[CG_CH_P];
+    float* D = P.d_ch[CG_CH_AP];
+    std::vector<int32_t> all(P.n);
+    for (int i = 0; i < P.n; i++) all[i] = i;
+    ensure_tmp_list(P, all);
+    k_fill_tiles<<<P.n, 64, 0, P.stream>>>(P.d_tmp_list, P.n, X, 0.f);
+    double bn = std::sqrt(dot_types(P, B, B, LEAF | NONLEAF | GHOST));
+    double thr = std::max(rel_tol * bn, 2.2250738585072014e-308);
+    int i = 0;
+    double rn = bn;
+    for (i = 0; i < max_iters; i++) {
+      full_negative_laplacian(P, X, AX, 0);
+      k_leaf_residual<<<P.leaf.n, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, B, AX);
+      rn = std::sqrt(dot_types(P, AX, AX, LEAF));
+      if (rn < thr) break;
+      full_negative_laplacian(P, X, D, 1);
+      k_leaf_jacobi_update<<<P.leaf.n, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, X, AX, D, omega);
+    }
+    HIPC(hipStreamSynchronize(P.stream));
+    res->iters = i;
+    res->converged = rn < thr;
+    res->rel_res = bn > 0 ? rn / bn : 0.0;
+    res->bb = bn * bn;
+    res->rr = rn * rn;
+    res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+cg_status cg_vcycle(cg_problem* p, int32_t x_channel, int32_t f_channel, int32_t level_iters, int32_t coarsest_iters,
+                    int32_t smoother, float omega) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    check_channel(x_channel);
+    check_channel(f_channel);
+    if (x_channel == f_channel || x_channel == CG_CH_TMP || f_channel == CG_CH_TMP)
+      fail(CG_ERR_INVALID, "cg_vcycle: x and f must differ and may not be the scratch channel CG_CH_TMP");
+    VParams v;
+    v.nu = level_iters;
+    v.coarsest = coarsest_iters;
+    v.smoother = smoother;
+    v.omega = omega;
+    v.check_done = 0;
+    enqueue_vcycle(*p, p->d_ch[x_channel], p->d_ch[f_channel], v);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+cg_status cg_full_negative_laplacian(cg_problem* p, int32_t x_channel, int32_t ax_channel, int32_t calc_diag) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    check_channel(x_channel);
+    check_channel(ax_channel);
+    full_negative_laplacian(*p, p->d_ch[x_channel], p->d_ch[ax_channel], calc_diag);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+cg_status cg_negative_laplacian_same_level(cg_problem* p, int32_t level, int32_t subtree_level, int32_t tile_types,
+                                           int32_t x_channel, int32_t ax_channel, int32_t calc_diag) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    check_channel(x_channel);
+    check_channel(ax_channel);
+    std::vector<int32_t> l;
+    for (int i = 0; i < p->n; i++) {
+      const DevTile& T = p->tiles[i];
+      if (level >= 0 && T.level != level) continue;
+      if (subtree_type(T.type, T.level, subtree_level) & tile_types) l.push_back(i);
+    }
+    if (l.empty()) return;
+    ensure_tmp_list(*p, l);
+    const Ctx c = p->ctx();
+    k_apply<<<unsigned(l.size()), 64, 0, p->stream>>>(c, p->d_tmp_list, int(l.size()), p->d_ch[x_channel], p->d_ch[ax_channel],
+                                                       subtree_level, calc_diag, 0, 0);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+cg_status cg_gauss_seidel_color(cg_problem* p, int32_t level, int32_t x_channel, int32_t rhs_channel, int32_t color) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    check_channel(x_channel);
+    check_channel(rhs_channel);
+    if (level < 0 || level > p->max_level) fail(CG_ERR_INVALID, "level out of range");
+    launch_gs(*p, p->ctx(), level, p->d_ch[x_channel], p->d_ch[rhs_channel], color & 1, 0);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+cg_status cg_jacobi_sweep(cg_problem* p, int32_t level, int32_t x_channel, int32_t rhs_channel, float omega) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    check_channel(x_channel);
+    check_channel(rhs_channel);
+    if (level < 0 || level > p->max_level) fail(CG_ERR_INVALID, "level out of range");
+    const DList& a = p->act[level];
+    if (!a.n) return;
+    float* X = p->d_ch[x_channel];
+    k_jacobi<<<a.n, 64, 0, p->stream>>>(p->ctx(), p->L(a), a.n, level, X, X, p->d_zb, p->d_ch[rhs_channel], omega, 0);
+    k_copy_tiles<<<a.n, 64, 0, p->stream>>>(p->L(a), a.n, p->d_zb, X);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+cg_status cg_residual_restrict(cg_problem* p, int32_t level, int32_t x_channel, int32_t rhs_channel, int32_t coarse_channel,
+                               float one_over_alpha) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    check_channel(x_channel);
+    check_channel(rhs_channel);
+    check_channel(coarse_channel);
+    if (level < 1 || level > p->max_level) fail(CG_ERR_INVALID, "level out of range");
+    const DList& al = p->lvl[level];
+    if (!al.n) return;
+    k_restrict<<<al.n, 64, 0, p->stream>>>(p->ctx(), p->L(al), al.n, level, p->d_ch[x_channel], p->d_ch[rhs_channel],
+                                           p->d_ch[coarse_channel], one_over_alpha, 0);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+cg_status cg_prolongate(cg_problem* p, int32_t level, int32_t coarse_channel, int32_t fine_channel, float prolong_coeff) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    check_channel(coarse_channel);
+    check_channel(fine_channel);
+    if (level < 1 || level > p->max_level) fail(CG_ERR_INVALID, "level out of range");
+    const DList& al = p->lvl[level];
+    if (!al.n) return;
+    float* F = p->d_ch[fine_channel];
+    k_prolong<<<al.n, 64, 0, p->stream>>>(p->ctx(), p->L(al), al.n, level, p->d_ch[coarse_channel], F, F, F, prolong_coeff, 0);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+cg_status cg_dot(cg_problem* p, int32_t a_channel, int32_t b_channel, int32_t tile_types, double* out) {
+  return guard([&] {
+    if (!p || !out) fail(CG_ERR_INVALID, "null argument");
+    check_channel(a_channel);
+    check_channel(b_channel);
+    *out = dot_types(*p, p->d_ch[a_channel], p->d_ch[b_channel], tile_types);
+  });
+}
+
+cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    check_channel(channel);
+    set_prob_state(*p, 0.0, 0, 1);
+    const Ctx c = p->ctx();
+    k_mean<<<p->leaf.n, 64, 0, p->stream>>>(c, p->L(p->leaf), p->leaf.n, p->d_ch[channel], 0);
+    k_finalize<<<1, 64, 0, p->stream>>>(c, FIN_MEAN, 0);
+    k_subtract_mean<<<p->leaf.n, 64, 0, p->stream>>>(c, p->L(p->leaf), p->leaf.n, p->d_ch[channel], 0);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int32_t reps, double* avg_us, int64_t* tiles) {
+  return guard([&] {
+    if (!p || !avg_us || reps <= 0) fail(CG_ERR_INVALID, "cg_benchmark_kernel: bad argument");
+    cg_problem& P = *p;
+    if ((kernel == CG_KERNEL_GS_SWEEP || kernel == CG_KERNEL_JACOBI) && (level < 0 || level > P.max_level))
+      fail(CG_ERR_INVALID, "level out of range");
+    if ((kernel == CG_KERNEL_RESTRICT || kernel == CG_KERNEL_PROLONG) && (level < 1 || level > P.max_level))
+      fail(CG_ERR_INVALID, "level out of range");
+    set_prob_state(P, 0.0, 1 << 30, 0);
+    HIPC(hipMemsetAsync(P.d_part, 0, sizeof(double) * std::max(P.n, 64), P.stream));
+    const Ctx c = P.ctx();
+    const int base = F_VCYCLE_RHS | kSwizzle | F_CHECK_DONE;
+    int64_t nt = 0;
+    auto launch = [&]() {
+      switch (kernel) {
+        case CG_KERNEL_GS_SWEEP:
+          nt = P.act[level].n;
+          launch_gs(P, c, level, c.z, c.r, 0, base | F_GHOST_ZERO);
+          break;
+        case CG_KERNEL_JACOBI: {
+          const DList& a = P.act[level];
+          nt = a.n;
+          if (a.n) k_jacobi<<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, level, c.z, c.z, c.zb, c.r, 0.5f, base);
+          break;
+        }
+        case CG_KERNEL_APPLY:
+          nt = P.applyl.n;
+          k_apply<<<grid_for(P.applyl.n, base), 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, c.p, c.ap, -1, 0, 1, base);
+          break;
+        case CG_KERNEL_RESTRICT: {
+          const DList& al = P.lvl[level];
+          nt = al.n;
+          int fl = base | F_VCYCLE_OUT | F_GHOST_ZERO;
+          k_restrict<<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, level, c.z, c.r, nullptr, 1.f, fl);
+          break;
+        }
+        case CG_KERNEL_PROLONG: {
+          const DList& al = P.lvl[level];
+          nt = al.n;
+          k_prolong<<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, level, c.z, c.z, c.zb, c.zb, 1.f, base);
+          break;
+        }
+        case CG_KERNEL_EMPTY:
+          nt = level;
+          k_empty<<<unsigned(std::max(1, level)), 64, 0, P.stream>>>(nullptr);
+          break;
+        case CG_KERNEL_COARSE: {
+          nt = P.coarse.nslots;
+          VCycleArgs va{2, 10, 0, 2.f / 3.f, 1.f, 1.f};
+          if (P.coarse.C >= 0) k_coarse_lds<<<1, CL_THREADS, 0, P.stream>>>(c, P.coarse, va, c.z, c.r, base & ~F_SWIZZLE);
+          break;
+        }
+        case CG_KERNEL_UPDATE_XR:
+          nt = P.leaf.n;
+          k_cg_update_xr<<<grid_for(P.leaf.n, base), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, base);
+          break;
+        default:
+          fail(CG_ERR_INVALID, "unknown kernel id");
+      }
+    };
+    launch();  // warm-up
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    HIPC(hipEventRecord(e0, P.stream));
+    for (int i = 0; i < reps; i++) launch();
+    HIPC(hipEventRecord(e1, P.stream));
+    HIPC(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPC(hipGetLastError());
+    *avg_us = double(ms) * 1e3 / reps;
+    if (tiles) *tiles = nt;
+  });
+}
+
+cg_status cg_calc_cell_types_from_leafs(const cg_tree_desc* desc, uint8_t* cell_type_inout) {
+  return guard([&] {
+    if (!desc || !cell_type_inout || desc->num_tiles <= 0) fail(CG_ERR_INVALID, "null argument");
+    Tree T;
+    for (int t = 0; t < desc->num_tiles; t++) {
+      const int32_t* c = desc->coord + 3 * t;
+      if (!key_ok(c[0], c[1], c[2])) fail(CG_ERR_INVALID, "coordinate out of range");
+      T.add(desc->level[t], c[0], c[1], c[2], desc->type[t]);
+    }
+    std::memcpy(T.ctype.data(), cell_type_inout, size_t(desc->num_tiles) * 512);
+    T.cell_types_from_leafs();
+    std::memcpy(cell_type_inout, T.ctype.data(), size_t(desc->num_tiles) * 512);
+  });
+}
+
+cg_status cg_scene_create(const cg_scene_params* sp, cg_scene** out) {
+  if (!out || !sp) { g_err = "null argument"; return CG_ERR_INVALID; }
+  *out = nullptr;
+  auto S = std::make_unique<cg_scene>();
+  cg_status st = guard([&] {
+    if (sp->nroot[0] <= 0 || sp->nroot[1] <= 0 || sp->nroot[2] <= 0 || sp->num_layers <= 0 || sp->num_layers > 20 || !(sp->h0 > 0))
+      fail(CG_ERR_INVALID, "cg_scene_create: invalid scene parameters");
+    scene_build(*sp, S->s);
+    const Tree& T = S->s.tree;
+    S->coord_flat.resize(size_t(T.size()) * 3);
+    for (int t = 0; t < T.size(); t++) for (int d = 0; d < 3; d++) S->coord_flat[3 * t + d] = T.coord[t][d];
+  });
+  if (st == CG_OK) *out = S.release();
+  return st;
+}
+
+cg_status cg_scene_destroy(cg_scene* s) {
+  delete s;
+  return CG_OK;
+}
+
+cg_status cg_scene_tree(const cg_scene* s, cg_tree_desc* desc, const float** rhs) {
+  if (!s || !desc) { g_err = "null argument"; return CG_ERR_INVALID; }
+  const Tree& T = s->s.tree;
+  desc->h0 = T.h0;
+  desc->num_tiles = T.size();
+  desc->level = T.level.data();
+  desc->coord = s->coord_flat.data();
+  desc->type = T.type.data();
+  desc->cell_type = T.ctype.data();
+  if (rhs) *rhs = s->s.rhs.data();
+  return CG_OK;
+}
+
+cg_status cg_scene_velocity(const cg_scene* s, const float** uvw) {
+  if (!s || !uvw) { g_err = "null argument"; return CG_ERR_INVALID; }
+  *uvw = s->s.uvw.data();
+  return CG_OK;
+}
+
+}  // extern "C"

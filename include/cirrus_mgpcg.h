@@ -1,0 +1,213 @@
+/*
+ * cirrus_mgpcg.h -- C ABI of the MI355X-native MGPCG pressure-projection
+ * library (libcirrus_mgpcg.so, HIP for gfx950).
+ *
+ * The library solves the volume-weighted Poisson problem of an adaptive octree
+ * of 8^3-cell tiles (the reference's HAGrid of PoissonTiles) with a
+ * multigrid-preconditioned conjugate gradient, on one GPU.  Plain C: opaque
+ * handles, pointers and sizes; no exceptions cross the boundary; every call
+ * returns a cg_status and cg_last_error() describes the last failure of the
+ * calling thread.
+ *
+ * Conventions shared with the reference:
+ *   tile types   CG_LEAF=1, CG_NONLEAF=2, CG_GHOST=4           (src/HATileInfo.h:19)
+ *   cell types   CG_DIRICHLET=1, CG_INTERIOR=2, CG_NEUMANN=4   (src/PoissonTile.h:6)
+ *   cell order   i-major inside a tile, offset = i<<6 | j<<3 | k (src/HAAccessor.h:32-42)
+ *   cell size    h(level) = h0 / 2^level                        (src/HAAccessor.h:72-74)
+ *   channels     CG_CH_X=0 (pressure), CG_CH_B=CG_CH_R=1 (rhs / residual),
+ *                CG_CH_P=2, CG_CH_AP=3, CG_CH_Z=4, CG_CH_TMP=5 (src/PoissonTile.h:12-24)
+ * Per-tile host arrays (channels, cell types) are passed in the order in
+ * which the tiles were given to cg_problem_create.
+ */
+#ifndef CIRRUS_MGPCG_H
+#define CIRRUS_MGPCG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { CG_LEAF = 1, CG_NONLEAF = 2, CG_GHOST = 4 };
+enum { CG_DIRICHLET = 1, CG_INTERIOR = 2, CG_NEUMANN = 4 };
+enum { CG_CH_X = 0, CG_CH_B = 1, CG_CH_R = 1, CG_CH_P = 2, CG_CH_AP = 3, CG_CH_Z = 4, CG_CH_TMP = 5, CG_NUM_CHANNELS = 6 };
+enum { CG_SMOOTH_RBGS = 0, CG_SMOOTH_JACOBI = 1 };
+
+typedef enum {
+  CG_OK = 0,
+  CG_ERR_INVALID = 1,   /* bad argument or malformed tree */
+  CG_ERR_HIP = 2,       /* a HIP runtime call failed */
+  CG_ERR_NOMEM = 3,     /* device or host allocation failed */
+  CG_ERR_STATE = 4      /* call not valid in the handle's current state */
+} cg_status;
+
+/* Opaque device-resident hierarchy of one octree (tiles, tables, channels). */
+typedef struct cg_problem cg_problem;
+
+/* A complete HAGrid as the solver receives it: every LEAF, NONLEAF and GHOST
+ * tile with its cell types (what GMGSolver::solve reads from HADeviceGrid,
+ * src/GMGSolver.cu:531, after CalcCellTypesFromLeafs, src/PoissonSolver.cu:7).
+ * Neighbour tables (src/PoissonGrid.cu:121-134) and parent links are derived
+ * from the coordinates. */
+typedef struct {
+  double h0;                 /* level-0 cell size (the reference uses 1/8, src/FluidEuler.h:415) */
+  int32_t num_tiles;
+  const int32_t* level;      /* [num_tiles] */
+  const int32_t* coord;      /* [3*num_tiles] tile coordinate at its level */
+  const uint8_t* type;       /* [num_tiles] CG_LEAF / CG_NONLEAF / CG_GHOST */
+  const uint8_t* cell_type;  /* [512*num_tiles] */
+} cg_tree_desc;
+
+/* Solver parameters; cg_params_default() gives the reference's project() call
+ * solve(grid, false, 1000, 1e-6, 1, 10, 1, pure) with GMGSolver(1, 1)
+ * (src/FluidEuler.h:591,598). */
+typedef struct {
+  int32_t max_iters;        /* 1000 */
+  double rel_tol;           /* 1e-6: stop when |r|^2 < rel_tol^2 |b|^2 (recursive r) */
+  int32_t level_iters;      /* smoothing sweeps per level, pre and post (1) */
+  int32_t coarsest_iters;   /* red-black GS iterations on level 0 (10) */
+  int32_t smoother;         /* CG_SMOOTH_RBGS (reference) or CG_SMOOTH_JACOBI */
+  float omega;              /* damped-Jacobi weight (2/3) */
+  int32_t sync_stride;      /* host polls the device convergence flag every this many iterations (1) */
+  int32_t pure_neumann;     /* re-centre r before each V-cycle (src/GMGSolver.cu:556,621) */
+  float one_over_alpha;     /* restriction scale (1) */
+  float prolong_coeff;      /* prolongation scale (1) */
+  int32_t use_graph;        /* capture one CG iteration in a hipGraph and replay it (1) */
+} cg_params;
+
+typedef struct {
+  int32_t iters;            /* CG iterations run */
+  int32_t converged;        /* 1 if |r| < rel_tol |b| was reached */
+  double rel_res;           /* sqrt(|r|^2/|b|^2) of the recursive residual */
+  double bb;                /* |b|^2 over LEAF tiles */
+  double rr;                /* final |r|^2 */
+  double seconds;           /* wall time of the solve, host clock, device synchronised */
+} cg_result;
+
+void cg_params_default(cg_params* p);
+const char* cg_last_error(void);
+/* Number of visible HIP devices (0 when there is none); no reference
+ * counterpart, used by callers to check for a GPU before creating problems. */
+int32_t cg_device_count(void);
+
+/* Upload a tree; allocates the device hierarchy.  Replaces building an
+ * HADeviceGrid (src/HAGrid.h:387-561) plus CalculateNeighborTiles
+ * (src/PoissonGrid.cu:121-134). */
+cg_status cg_problem_create(const cg_tree_desc* desc, cg_problem** out);
+cg_status cg_problem_destroy(cg_problem* p);
+/* Tile counts and the deepest level. */
+cg_status cg_problem_info(const cg_problem* p, int32_t* num_tiles, int32_t* num_leaf_tiles, int32_t* max_level);
+/* Derived topology in the caller's tile order: nbr[6*t+f] is the caller index
+ * of the same-level face neighbour f (x-,y-,z-,x+,y+,z+) or -1, exactly the
+ * reference's tile.mNeighbors (src/PoissonGrid.cu:121-134); parent[t] likewise. */
+cg_status cg_problem_topology(const cg_problem* p, int32_t* nbr, int32_t* parent);
+
+/* Channel transfer (host <-> device), caller's tile order, 512 floats per tile. */
+cg_status cg_set_channel(cg_problem* p, int32_t channel, const float* host);
+cg_status cg_get_channel(cg_problem* p, int32_t channel, float* host);
+
+/* GMGSolver::solve (src/GMGSolver.cu:531-666): solves A x = b with b in
+ * CG_CH_B, pressure returned in CG_CH_X on LEAF tiles.  Channels 1-5 are
+ * clobbered, as in the reference. */
+cg_status cg_solve(cg_problem* p, const cg_params* prm, cg_result* res);
+
+/* GMGSolver::dampedJacobiSolve (src/GMGSolver.cu:449-529). */
+cg_status cg_damped_jacobi_solve(cg_problem* p, int32_t max_iters, double rel_tol, float omega, cg_result* res);
+
+/* GMGSolver::VCycle (src/GMGSolver.cu:383-447): x_channel <- M^-1 f_channel
+ * on LEAF tiles (one V-cycle with the given smoother). */
+cg_status cg_vcycle(cg_problem* p, int32_t x_channel, int32_t f_channel, int32_t level_iters,
+                    int32_t coarsest_iters, int32_t smoother, float omega);
+
+/* FullNegativeLaplacian (src/GMGSolver.h:13, src/GMGSolver.cu:203-213):
+ * ghost values from parents, the 7-point operator on LEAF|GHOST tiles, ghost
+ * rows summed into their LEAF parents.  calc_diag computes the diagonal. */
+cg_status cg_full_negative_laplacian(cg_problem* p, int32_t x_channel, int32_t ax_channel, int32_t calc_diag);
+
+/* NegativeLaplacianSameLevel128 (src/GMGSolver.h:7, src/GMGSolver.cu:145-200)
+ * on the tiles of `level` (all levels if level < 0) whose subtree type at
+ * subtree_level is in tile_types. */
+cg_status cg_negative_laplacian_same_level(cg_problem* p, int32_t level, int32_t subtree_level, int32_t tile_types,
+                                           int32_t x_channel, int32_t ax_channel, int32_t calc_diag);
+
+/* One red-black Gauss-Seidel colour sweep on the LEAF|NONLEAF tiles of level
+ * (NegativeLaplacianAndGaussSeidelSameLevel128, src/GMGSolver.cu:216-274). */
+cg_status cg_gauss_seidel_color(cg_problem* p, int32_t level, int32_t x_channel, int32_t rhs_channel, int32_t color);
+/* One damped-Jacobi sweep on the LEAF|NONLEAF tiles of level, the update of
+ * src/GMGSolver.cu:508-525 with the level's operator. */
+cg_status cg_jacobi_sweep(cg_problem* p, int32_t level, int32_t x_channel, int32_t rhs_channel, float omega);
+/* ResidualAndRestrict (src/GMGSolver.cu:284-343) from level to level-1. */
+cg_status cg_residual_restrict(cg_problem* p, int32_t level, int32_t x_channel, int32_t rhs_channel,
+                               int32_t coarse_channel, float one_over_alpha);
+/* ProlongateAndUpdate128 (src/GMGSolver.h:15, src/GMGSolver.cu:345-380) from level-1 to level. */
+cg_status cg_prolongate(cg_problem* p, int32_t level, int32_t coarse_channel, int32_t fine_channel, float prolong_coeff);
+
+/* Dot over all cells of the tiles of type in tile_types, FP64 accumulation in
+ * a fixed order (Dot, src/PoissonGrid.cu:217-228). */
+cg_status cg_dot(cg_problem* p, int32_t a_channel, int32_t b_channel, int32_t tile_types, double* out);
+/* ReCenterLeafVoxels (src/PoissonSolver.cu:356-381). */
+cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel);
+
+/* Average device duration (microseconds, hipEvents on the problem's stream)
+ * of `reps` back-to-back launches of one kernel of the solve on its real
+ * worklist, for roofline measurement; no reference counterpart.
+ * kernel: CG_KERNEL_GS_SWEEP (one colour sweep of `level`, V-cycle mode),
+ * CG_KERNEL_APPLY (CG operator on LEAF+GHOST tiles), CG_KERNEL_RESTRICT and
+ * CG_KERNEL_PROLONG (from/to `level`), CG_KERNEL_JACOBI (one sweep of `level`),
+ * CG_KERNEL_UPDATE_XR (x/r update and |r|^2), CG_KERNEL_COARSE (the LDS-resident
+ * coarse levels of a V-cycle, RB-GS 2+2), CG_KERNEL_EMPTY (an empty kernel of
+ * `level` workgroups: the launch floor).  Channels are overwritten. */
+enum { CG_KERNEL_GS_SWEEP = 0, CG_KERNEL_APPLY = 1, CG_KERNEL_RESTRICT = 2, CG_KERNEL_PROLONG = 3,
+       CG_KERNEL_JACOBI = 4, CG_KERNEL_UPDATE_XR = 5, CG_KERNEL_COARSE = 6, CG_KERNEL_EMPTY = 7 };
+cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int32_t reps, double* avg_us,
+                              int64_t* tiles);
+
+/* CalcCellTypesFromLeafs (src/PoissonSolver.cu:7-61) on a host tree: fills
+ * the NONLEAF and GHOST cell types of desc from its LEAF cell types.
+ * cell_type_inout has 512*num_tiles entries in desc's tile order. */
+cg_status cg_calc_cell_types_from_leafs(const cg_tree_desc* desc, uint8_t* cell_type_inout);
+
+/* ---- Synthetic scenes (the benchmark workloads) --------------------------
+ * A scene is an octree built the reference's way (IterativeRefine with 2:1
+ * face balance and ghost spawning, src/PoissonGrid.h:35-162,317-326 and
+ * src/PoissonGrid.cu:30-60), typed (walls/solid), and a seeded face-velocity
+ * field whose divergence is the right-hand side (src/PoissonSolver.cu:66-142). */
+enum { CG_SCENE_UNIFORM = 0, CG_SCENE_SPHERE_BAND = 1, CG_SCENE_PLANE = 2, CG_SCENE_VORTEX = 3 };
+typedef struct {
+  int32_t kind;
+  int32_t nroot[3];          /* root tiles per axis at level 0 */
+  double h0;
+  int32_t uniform_level;
+  int32_t has_solid;         /* cells whose centre is inside the sphere are NEUMANN */
+  double solid_c[3];
+  double solid_r;
+  double band_c[3];
+  double band_r;
+  double band_w[2];
+  int32_t band_level[3];
+  double plane_z;
+  int32_t plane_level[2];
+  int32_t n_tubes;
+  double tube_r_lo, tube_r_hi;
+  double vortex_thresh;
+  int32_t vortex_level[2];
+  int32_t face_bc[6];        /* x-,x+,y-,y+,z-,z+ : CG_NEUMANN or CG_DIRICHLET */
+  uint64_t seed;
+  int32_t num_layers;        /* maximum number of levels (10) */
+} cg_scene_params;
+
+typedef struct cg_scene cg_scene;
+/* Build the tree, cell types and RHS on the host. */
+cg_status cg_scene_create(const cg_scene_params* sp, cg_scene** out);
+cg_status cg_scene_destroy(cg_scene* s);
+/* Borrowed views valid until cg_scene_destroy: the tree (tiles in level,
+ * Morton order) and the divergence RHS (512 floats per tile). */
+cg_status cg_scene_tree(const cg_scene* s, cg_tree_desc* desc, const float** rhs);
+/* The face velocities (3 x 512 floats per tile, minus faces, after ghost
+ * propagation) from which the RHS was formed. */
+cg_status cg_scene_velocity(const cg_scene* s, const float** uvw);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CIRRUS_MGPCG_H */

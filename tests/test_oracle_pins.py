@@ -1,0 +1,338 @@
+"""Pins the oracle (the CPU transcription) to what the mathematics fixes.
+
+The reference ships no tests or golden vectors, so each part of the oracle is
+checked against an independent statement: a leaf-partition finite-volume
+operator (fv_reference), dense-grid closed forms, invariants (symmetry,
+consistency on linear fields, R = P^T, 2:1 balance, ghost rules), brute-force
+counts and a sparse direct solve.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import pyoracle as po
+from fv_reference import LeafCells
+
+SCENES = {
+    "uniform32": dict(uniform_level=2),
+    "plane": dict(kind=po.SCENE_PLANE, plane_level=[3, 2], has_solid=0),
+    "band": dict(kind=po.SCENE_SPHERE_BAND, solid_r=0.15, band_level=[4, 3, 2]),
+    "vortex": dict(kind=po.SCENE_VORTEX, nroot=[1, 1, 2], vortex_level=[3, 2], seed=5),
+}
+_cache = {}
+
+
+def scene(name, fp64=True):
+    key = (name, fp64)
+    if key not in _cache:
+        o = po.Oracle(fp64=fp64, **SCENES[name])
+        lev, crd, typ, nbr = o.topology()
+        ct = o.celltypes()
+        lc = LeafCells(lev, crd, typ, ct, o.params.h0, list(o.params.nroot))
+        _cache[key] = (o, lev, crd, typ, nbr, ct, lc)
+    return _cache[key]
+
+
+def leaf_interior_vector(lc, rng):
+    v = rng.standard_normal(lc.n)
+    v[lc.ct != po.INTERIOR] = 0
+    return v
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_operator_matches_independent_fv(name):
+    o, lev, crd, typ, nbr, ct, lc = scene(name)
+    A = lc.matrix()
+    rng = np.random.default_rng(1)
+    x = leaf_interior_vector(lc, rng)
+    o.set(po.CH_X, lc.to_tiles(x, o.n))
+    o.full_negative_laplacian(po.CH_X, po.CH_AP)
+    ax = lc.from_tiles(o.get(po.CH_AP))
+    ref = A @ x
+    m = lc.ct == po.INTERIOR
+    err = np.abs(ax[m] - ref[m]).max() / np.abs(ref[m]).max()
+    assert err < 1e-12, err
+    # diagonal
+    o.full_negative_laplacian(po.CH_X, po.CH_AP, True)
+    d = lc.from_tiles(o.get(po.CH_AP))
+    assert np.abs(d[m] - A.diagonal()[m]).max() / A.diagonal()[m].max() < 1e-12
+
+
+@pytest.mark.parametrize("name", ["band", "plane"])
+def test_operator_symmetric_positive(name):
+    o, lev, crd, typ, nbr, ct, lc = scene(name)
+    rng = np.random.default_rng(2)
+    x, y = leaf_interior_vector(lc, rng), leaf_interior_vector(lc, rng)
+
+    def A(v):
+        o.set(po.CH_X, lc.to_tiles(v, o.n))
+        o.full_negative_laplacian(po.CH_X, po.CH_AP)
+        return lc.from_tiles(o.get(po.CH_AP))
+
+    ax, ay = A(x), A(y)
+    assert abs(ax @ y - x @ ay) <= 1e-12 * abs(ax @ y)
+    assert ax @ x > 0 and ay @ y > 0
+
+
+def test_operator_exact_on_linear_field_across_coarse_fine_faces():
+    """A u = 0 for u = z at INTERIOR cells whose neighbours are all INTERIOR,
+    including cells on both sides of the z = 1/2 coarse/fine plane: only the
+    2/3 coarse/fine coefficient with the ghost = parent rule balances the
+    fluxes there."""
+    o, lev, crd, typ, nbr, ct, lc = scene("plane")
+    zc = (lc.g[:, 2] + 0.5) * lc.h
+    o.set(po.CH_X, lc.to_tiles(zc, o.n))
+    o.full_negative_laplacian(po.CH_X, po.CH_AP)
+    ax = lc.from_tiles(o.get(po.CH_AP))
+    # cells whose every face neighbour (any level) is INTERIOR
+    A = lc.matrix().tocsr()
+    ok = lc.ct == po.INTERIOR
+    nb_ok = np.ones(lc.n, bool)
+    for r in np.nonzero(ok)[0]:
+        cols = A.indices[A.indptr[r]:A.indptr[r + 1]]
+        nb_ok[r] = np.all(lc.ct[cols] == po.INTERIOR) and len(cols) >= 7
+    m = ok & nb_ok
+    interface = m & (np.abs(zc - 0.5) < 0.02)
+    assert interface.sum() > 100
+    assert np.abs(ax[m]).max() < 1e-12
+
+
+def test_uniform_operator_closed_form():
+    """On a uniform grid A x = h * sum_faces (x_c - x_n) with x = 0 outside and
+    faces touching NEUMANN cells removed: a dense-array statement."""
+    o = po.Oracle(fp64=True, uniform_level=2, has_solid=1, solid_r=0.3)
+    lev, crd, typ, nbr = o.topology()
+    ct = o.celltypes()
+    N, h = 32, 0.125 / 4
+    dense_ct = np.zeros((N, N, N), np.uint8)
+    tiles = np.nonzero(typ == po.LEAF)[0]
+    ci = np.arange(512)
+    for t in tiles:
+        i, j, k = crd[t] * 8
+        dense_ct[i:i + 8, j:j + 8, k:k + 8] = ct[t].reshape(8, 8, 8)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((N, N, N)) * (dense_ct == po.INTERIOR)
+    xt = np.zeros((o.n, 512))
+    for t in tiles:
+        i, j, k = crd[t] * 8
+        xt[t] = X[i:i + 8, j:j + 8, k:k + 8].ravel()
+    o.set(po.CH_X, xt)
+    o.full_negative_laplacian(po.CH_X, po.CH_AP)
+    got = o.get(po.CH_AP)
+    Xp = np.pad(X, 1)
+    Cp = np.pad(dense_ct, 1, constant_values=po.DIRICHLET)
+    acc = np.zeros_like(X)
+    neu0 = (dense_ct & po.NEUMANN) != 0
+    for a in range(3):
+        for s in (-1, 1):
+            sl = [slice(1, N + 1)] * 3
+            sl[a] = slice(1 + s, N + 1 + s)
+            xn, cn = Xp[tuple(sl)], Cp[tuple(sl)]
+            cut = neu0 | ((cn & po.NEUMANN) != 0)
+            acc += np.where(cut, 0.0, X - xn)
+    ref = np.where(dense_ct == po.INTERIOR, h * acc, 0.0)
+    for t in tiles:
+        i, j, k = crd[t] * 8
+        assert np.allclose(got[t], ref[i:i + 8, j:j + 8, k:k + 8].ravel(), rtol=1e-12, atol=1e-14)
+
+
+def test_gauss_seidel_textbook_update():
+    """One red sweep on the finest level = x_c + (b_c - (A x)_c) / A_cc on red
+    INTERIOR cells, black cells unchanged."""
+    o, lev, crd, typ, nbr, ct, lc = scene("uniform32")
+    A = lc.matrix()
+    rng = np.random.default_rng(4)
+    x, b = leaf_interior_vector(lc, rng), rng.standard_normal(lc.n)
+    o.set(po.CH_Z, lc.to_tiles(x, o.n))
+    o.set(po.CH_TMP, lc.to_tiles(b, o.n))
+    L = o.max_level
+    o.gs_color(L, po.CH_Z, po.CH_TMP, 0)
+    got = lc.from_tiles(o.get(po.CH_Z))
+    red = (lc.g.sum(axis=1) % 2 == 0) & (lc.ct == po.INTERIOR)
+    ref = x.copy()
+    ref[red] = x[red] + (b[red] - (A @ x)[red]) / A.diagonal()[red]
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["band", "vortex"])
+def test_restriction_is_transpose_of_prolongation(name):
+    o, lev, crd, typ, nbr, ct, lc = scene(name)
+    rng = np.random.default_rng(5)
+    for level in range(1, o.max_level + 1):
+        f = rng.standard_normal((o.n, 512)) * (ct == po.INTERIOR) * ((lev == level) & (typ != po.GHOST))[:, None]
+        c = rng.standard_normal((o.n, 512)) * (lev == level - 1)[:, None]
+        o.set(po.CH_TMP, f)
+        o.set(po.CH_Z, np.zeros((o.n, 512)))
+        o.set(po.CH_AP, np.zeros((o.n, 512)))
+        o.residual_restrict(level, po.CH_Z, po.CH_TMP, po.CH_AP)
+        Rf = o.get(po.CH_AP)
+        o.set(po.CH_AP, c)
+        o.prolongate(level, po.CH_AP, po.CH_Z)
+        Pc = o.get(po.CH_Z)
+        lhs = np.sum(Rf[lev == level - 1] * c[lev == level - 1])
+        rhs = np.sum(f[lev == level] * Pc[lev == level])
+        assert abs(lhs - rhs) <= 1e-11 * max(1.0, abs(lhs))
+
+
+def test_restriction_maps_fine_cell_to_its_parent_cell():
+    o, lev, crd, typ, nbr, ct, lc = scene("band")
+    level = o.max_level
+    t = np.nonzero((lev == level) & (typ == po.LEAF))[0][7]
+    f = np.zeros((o.n, 512))
+    f[t, 5 * 64 + 2 * 8 + 3] = 1.0
+    o.set(po.CH_TMP, f)
+    o.set(po.CH_Z, np.zeros((o.n, 512)))
+    o.set(po.CH_AP, np.zeros((o.n, 512)))
+    o.residual_restrict(level, po.CH_Z, po.CH_TMP, po.CH_AP)
+    got = o.get(po.CH_AP)
+    g = crd[t] * 8 + np.array([5, 2, 3])
+    pg = g // 2
+    pt = np.nonzero((lev == level - 1) & np.all(crd == pg // 8, axis=1))[0][0]
+    pc = pg % 8
+    assert got[pt, pc[0] * 64 + pc[1] * 8 + pc[2]] == 1.0
+    assert np.count_nonzero(got) == 1
+
+
+@pytest.mark.parametrize("smoother,nu", [(po.SMOOTH_RBGS, 1), (po.SMOOTH_RBGS, 2), (po.SMOOTH_JACOBI, 2)])
+def test_vcycle_symmetric_positive(smoother, nu):
+    """The V-cycle is a symmetric positive preconditioner (mirrored smoothing,
+    R = P^T, symmetric bottom solve), as CG requires."""
+    o, lev, crd, typ, nbr, ct, lc = scene("band")
+    rng = np.random.default_rng(6)
+    r1, r2 = leaf_interior_vector(lc, rng), leaf_interior_vector(lc, rng)
+
+    def V(r):
+        o.set(po.CH_R, lc.to_tiles(r, o.n))
+        o.vcycle(po.CH_Z, po.CH_R, po.CH_AP, nu, 10, smoother, 2.0 / 3.0)
+        return lc.from_tiles(o.get(po.CH_Z))
+
+    v1, v2 = V(r1), V(r2)
+    assert abs(v1 @ r2 - r1 @ v2) <= 1e-10 * abs(v1 @ r2)
+    assert v1 @ r1 > 0 and v2 @ r2 > 0
+
+
+@pytest.mark.parametrize("name", ["band", "vortex", "plane"])
+def test_fp64_mgpcg_matches_direct_solve(name):
+    o, lev, crd, typ, nbr, ct, lc = scene(name)
+    o32 = po.Oracle(fp64=False, **SCENES[name])
+    b = o32.get(po.CH_B)
+    o.set(po.CH_B, b)
+    r = o.solve(max_iters=300, tol=1e-13, level_iters=2, coarsest_iters=10)
+    assert r["rel_res"] < 1e-13
+    x = lc.from_tiles(o.get(po.CH_X))
+    m = lc.ct == po.INTERIOR
+    A = lc.matrix()[m][:, m]
+    xd = spla.spsolve(A.tocsc(), lc.from_tiles(b)[m])
+    assert np.linalg.norm(x[m] - xd) / np.linalg.norm(xd) < 1e-9
+
+
+def test_dot_close_to_exact_sum():
+    o, lev, crd, typ, nbr, ct, lc = scene("band", fp64=False)
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal((o.n, 512)).astype(np.float32)
+    b = rng.standard_normal((o.n, 512)).astype(np.float32)
+    o.set(po.CH_P, a.astype(np.float64))
+    o.set(po.CH_Z, b.astype(np.float64))
+    d = o.dot(po.CH_P, po.CH_Z, po.LEAF)
+    leaf = typ == po.LEAF
+    exact = math.fsum((a[leaf].astype(np.float64) * b[leaf].astype(np.float64)).ravel())
+    assert abs(d - exact) <= 1e-13 * np.sum(np.abs(a[leaf].astype(np.float64) * b[leaf]))
+
+
+def test_uniform_tree_counts_and_no_ghosts():
+    o = po.Oracle()
+    lev, crd, typ, nbr = o.topology()
+    assert list(np.bincount(lev)) == [1, 8, 64, 512]
+    assert (typ[lev == 3] == po.LEAF).all() and (typ[lev < 3] == po.NONLEAF).all()
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_topology_invariants(name):
+    o, lev, crd, typ, nbr, ct, lc = scene(name)
+    keys = {(int(l), *map(int, c)): i for i, (l, c) in enumerate(zip(lev, crd))}
+    # neighbour table symmetric and consistent with coordinates
+    for t in range(o.n):
+        for f in range(6):
+            n = nbr[t, f]
+            q = crd[t].copy()
+            q[f % 3] += -1 if f < 3 else 1
+            assert n == keys.get((int(lev[t]), *map(int, q)), -1)
+            if n >= 0:
+                assert nbr[n, (f + 3) % 6] == t
+    # every ghost has a LEAF parent; every non-root active tile a NONLEAF parent
+    for t in range(o.n):
+        if lev[t] == 0:
+            continue
+        p = keys[(int(lev[t]) - 1, *map(int, crd[t] // 2))]
+        assert typ[p] == (po.LEAF if typ[t] == po.GHOST else po.NONLEAF)
+    # ghosts are exactly the empty face positions of active tiles whose parent is a LEAF
+    want = set()
+    for t in range(o.n):
+        if lev[t] == 0 or typ[t] == po.GHOST:
+            continue
+        for f in range(6):
+            q = crd[t].copy()
+            q[f % 3] += -1 if f < 3 else 1
+            k = (int(lev[t]), *map(int, q))
+            pk = (int(lev[t]) - 1, *map(int, q // 2))
+            if (k not in keys or typ[keys[k]] == po.GHOST) and pk in keys and typ[keys[pk]] == po.LEAF:
+                want.add(k)
+    have = {(int(lev[t]), *map(int, crd[t])) for t in range(o.n) if typ[t] == po.GHOST}
+    assert want == have
+    # 2:1 face balance of the leaves
+    leaf = {(int(lev[t]), *map(int, crd[t])) for t in range(o.n) if typ[t] == po.LEAF}
+    for (l, x, y, z) in leaf:
+        for f in range(6):
+            q = np.array([x, y, z])
+            q[f % 3] += -1 if f < 3 else 1
+            for up in range(2, l + 1):
+                assert (l - up, *map(int, q >> up)) not in leaf
+
+
+def test_solid_cells_match_brute_force_count():
+    o = po.Oracle()
+    lev, crd, typ, nbr = o.topology()
+    ct = o.celltypes()
+    N, h = 64, 1.0 / 64
+    idx = np.arange(N)
+    X, Y, Z = np.meshgrid(idx, idx, idx, indexing="ij")
+    inside = ((X + 0.5) * h - 0.5) ** 2 + ((Y + 0.5) * h - 0.5) ** 2 + ((Z + 0.5) * h - 0.5) ** 2 <= 0.2 ** 2
+    boundary = (X == 0) | (X == N - 1) | (Y == 0) | (Y == N - 1) | (Z == 0) | (Z == N - 1)
+    n_neu = sum(int(np.sum(ct[t] == po.NEUMANN)) for t in np.nonzero(typ == po.LEAF)[0])
+    n_dir = sum(int(np.sum(ct[t] == po.DIRICHLET)) for t in np.nonzero(typ == po.LEAF)[0])
+    top_only = (Z == N - 1) & ~((X == 0) | (X == N - 1) | (Y == 0) | (Y == N - 1))
+    assert n_dir == int(top_only.sum())
+    assert n_neu == int((inside & ~boundary).sum() + (boundary & ~top_only).sum())
+
+
+def test_rhs_sum_equals_boundary_flux():
+    """Divergence theorem on the plain scene: the sum of b over INTERIOR cells
+    equals h^2 times the net face velocity through faces between INTERIOR and
+    non-INTERIOR cells (the velocities come from the oracle's channels)."""
+    o = po.Oracle(fp64=True)
+    lev, crd, typ, nbr = o.topology()
+    ct, b = o.celltypes(), o.get(po.CH_B)
+    N, h = 64, 1.0 / 64
+    dct = np.zeros((N, N, N), np.uint8)
+    U = np.zeros((3, N, N, N))
+    B = np.zeros((N, N, N))
+    for t in np.nonzero(typ == po.LEAF)[0]:
+        i, j, k = crd[t] * 8
+        dct[i:i + 8, j:j + 8, k:k + 8] = ct[t].reshape(8, 8, 8)
+        B[i:i + 8, j:j + 8, k:k + 8] = b[t].reshape(8, 8, 8)
+        for a in range(3):
+            U[a, i:i + 8, j:j + 8, k:k + 8] = o.get(po.CH_U + a)[t].reshape(8, 8, 8)
+    interior = dct == po.INTERIOR
+    flux = 0.0
+    for a in range(3):
+        ua = U[a]
+        # face between cell c-1 and c along axis a carries ua[c] (minus face of c)
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[a] = slice(0, N - 1)
+        hi[a] = slice(1, N)
+        cm, cp, uf = interior[tuple(lo)], interior[tuple(hi)], ua[tuple(hi)]
+        flux += np.sum(uf[cm & ~cp]) - np.sum(uf[~cm & cp])
+    assert abs(B[interior].sum() - h * h * flux) < 1e-9 * max(1.0, h * h * np.abs(U).sum())
