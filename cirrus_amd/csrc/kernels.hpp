@@ -3,7 +3,11 @@
 // Layout in HBM (structure of arrays, one problem or many in one pool):
 //   channel c of tile t, cell (i,j,k):  ch[c][t*512 + (i<<6 | j<<3 | k)]  (FP32)
 //   cell types:                        ctype[t*512 + ...]                (u8)
-//   per-tile table DevTile (64 B), per-problem scalars ProbState.
+//   worklists:                          arrays of 128-byte TileRec (one per tile
+//                                       and phase), so a workgroup finds its
+//                                       tile, neighbours and children in one
+//                                       scalar load
+//   per-problem scalars:                ProbState
 // Execution: one 64-lane wavefront per 8^3 tile.  Lane l = (i,j) = (l>>3, l&7)
 // owns the k-pencil of 8 cells at offsets l*8..l*8+7 (32 contiguous bytes), so a
 // wavefront reads or writes a whole channel of a tile as 2 KB of coalesced
@@ -27,19 +31,21 @@ namespace cg {
 enum : uint8_t { T_LEAF = 1, T_NONLEAF = 2, T_GHOST = 4 };
 enum : uint8_t { C_DIR = 1, C_INT = 2, C_NEU = 4 };
 
-struct alignas(16) DevTile {
+struct alignas(16) TileRec {
   int32_t nbr[6];     // same-level face neighbours x-,y-,z-,x+,y+,z+ (-1: none)
   int32_t parent;     // -1 on level 0
   int32_t prob;       // problem index in the pool
   int32_t leaf_slot;  // LEAF: slot of its dot partial; -1 otherwise
-  int32_t gslot;      // GHOST: slot in the 64-value ghost buffers; -1 otherwise
-  int32_t gchild;     // LEAF with ghost children: row of the 8-entry ghost-child table; -1 otherwise
+  int32_t gslot;      // GHOST: slot in the 64-value ghost buffer; -1 otherwise
+  int32_t self;       // the tile's index
   float h;            // cell size of the tile's level
-  uint8_t level, type, octant, pad0;
+  uint8_t level, type, octant, has_gchild;
   uint8_t nbr_type[6];
-  uint8_t pad1[6];
+  uint8_t pad[6];
+  int32_t gch[8];     // LEAF: ghost child per octant (-1: none)
+  int32_t gcs[8];     // ... and its ghost slot
 };
-static_assert(sizeof(DevTile) == 64, "DevTile is 64 bytes");
+static_assert(sizeof(TileRec) == 128, "TileRec is 128 bytes");
 
 struct alignas(16) ProbState {
   double gamma, gamma_old, alpha, beta;
@@ -50,21 +56,17 @@ struct alignas(16) ProbState {
   int32_t leaf_begin, leaf_count, pure_neumann;
 };
 
-// Channel pointers and tables passed to every kernel by value.
+// Channel pointers passed to every kernel by value.
 struct Ctx {
-  const DevTile* tiles;
-  const int32_t* gchild;  // 8 ghost-child tile indices (or -1) per row
-  const int32_t* gcslot;  // the same children's ghost slots (or -1)
   const uint8_t* ctype;
   float* x;      // pressure / CG x
   float* r;      // rhs b, then residual r
   float* p;      // CG direction
   float* ap;     // A p
   float* z;      // preconditioned residual (V-cycle x)
-  float* rhsv;   // V-cycle rhs of NONLEAF tiles
+  float* rhsv;   // V-cycle rhs: r on LEAF tiles (plus ghost sums), restricted residual on NONLEAF
   float* zb;     // damped-Jacobi ping-pong buffer
   float* gsum;   // 64 per ghost: 2x2x2 sums of the ghost rows of A p
-  float* gext;   // 64 per ghost: restricted ghost residual (V-cycle)
   ProbState* ps;
   double* part;  // per-leaf-slot dot partials
   double* part2; // second partial stream (mean counts)
@@ -78,8 +80,8 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __host__ __device__ __forceinline__ uint8_t subtree_type(uint8_t type, int level, int sl) {
   return (level == sl && type == T_NONLEAF) ? T_LEAF : type;
 }
-// Face coefficient (times h^2 is 1/h^2 on LEAF-LEAF faces, (2/3)/h^2 on
-// LEAF-GHOST faces, 0 otherwise) before the NEUMANN cut: GMGSolver.cu:26-39.
+// Face coefficient before the NEUMANN cut (GMGSolver.cu:26-39): 1/h^2 on
+// LEAF-LEAF faces, (2/3)/h^2 on LEAF-GHOST faces, 0 otherwise.
 __host__ __device__ __forceinline__ float base_coeff(uint8_t t0, uint8_t t1, float cll, float clg) {
   bool both = (t0 & T_LEAF) && (t1 & T_LEAF);
   bool lg = ((t0 & T_LEAF) && (t1 & T_GHOST)) || ((t0 & T_GHOST) && (t1 & T_LEAF));
@@ -108,31 +110,37 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Sum of a problem's per-tile partials in the fixed order: lane q adds
-// partials q, q+64, q+128, ... in sequence (loads issued 8 at a time), then the
-// wavefront butterfly.
-__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int begin, int count) {
-  double a = 0.0;
-  int q = lane_id();
-  for (; q + 7 * 64 < count; q += 8 * 64) {
-    double v[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) v[u] = part[begin + q + u * 64];
-#pragma unroll
-    for (int u = 0; u < 8; u++) a += v[u];
-  }
-  for (; q < count; q += 64) a += part[begin + q];
-  return wave_sum(a);
-}
-
 // Block -> worklist position.  Consecutive workgroups go round-robin to the 8
 // XCDs; giving each XCD a contiguous (Morton-ordered) run of tiles keeps the
 // face halos of a tile in the L2 of the XCD that reads them.
-__device__ __forceinline__ int work_index(int n, int xcd_swizzle) {
+__device__ __forceinline__ int work_index(int xcd_swizzle) {
   int b = blockIdx.x;
   if (!xcd_swizzle) return b;
   int per = (int(gridDim.x) + 7) >> 3;
   return (b & 7) * per + (b >> 3);
+}
+
+// Each tile is worked by exactly one wavefront with its own LDS region, so
+// staging only needs ordering inside the wavefront.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// Kernel flags
+// ---------------------------------------------------------------------------
+enum : int {
+  F_ZERO_X = 1,      // X is implicitly zero (first sweep)
+  F_GHOST_ZERO = 2,  // ghost X is zero (V-cycle downstroke)
+  F_VCYCLE = 4,      // V-cycle mode: rhs from rhsv; restriction into rhsv (ghosts +=)
+  F_SWIZZLE = 16,    // XCD-aware tile order
+  F_CHECK_DONE = 32, // skip tiles of converged problems
+};
+
+__device__ __forceinline__ bool prob_done(const Ctx& c, const TileRec& T, int flags) {
+  return (flags & F_CHECK_DONE) && c.ps[T.prob].done;
 }
 
 // ---------------------------------------------------------------------------
@@ -152,17 +160,17 @@ struct Stencil {
   float h;
 };
 
-// Stage tile t's X values (zeros where `zero` says so) and cell types.
+// Stage tile T's X values (zeros where the flags say so) and cell types.
 //   zero_all:   treat X as 0 everywhere (first sweep from a zero guess)
 //   ghost_zero: treat X of GHOST tiles as 0 (downstroke of the V-cycle)
 //   sl:         subtree level of the tile-type view (-1: plain types)
 // X holds active tiles, XG ghost tiles (they differ only for the damped-Jacobi
 // ping-pong, where ghost values stay in the primary buffer).
-__device__ __forceinline__ void stage_tile(const Ctx& c, int t, const DevTile& T, const float* __restrict__ X,
+__device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const float* __restrict__ X,
                                            const float* __restrict__ XG, bool zero_all, bool ghost_zero, int sl,
                                            TileLDS& L, Stencil& S) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
-  const size_t base = size_t(t) * 512 + lane * 8;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
   bool zero_own = zero_all || (ghost_zero && T.type == T_GHOST);
   if (zero_own) { for (int k = 0; k < 8; k++) S.own[k] = 0.f; }
   else load8((T.type == T_GHOST ? XG : X) + base, S.own);
@@ -230,152 +238,152 @@ __device__ __forceinline__ void stage_tile(const Ctx& c, int t, const DevTile& T
   }
 }
 
-// A x (and its diagonal) on the lane's 8 cells after stage_tile + barrier.
-// The six faces are visited in the reference's order x-, x+, y-, y+, z-, z+
-// (GMGSolver.cu:62-73).
+__device__ __forceinline__ void unpack8(float4 a, float4 b, float v[8]) {
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
+// The 7-point row of one cell: faces visited in the reference's order x-, x+,
+// y-, y+, z-, z+ (GMGSolver.cu:62-73); returns A x and D (times h^3, 0 off
+// INTERIOR cells).
+__device__ __forceinline__ void cell_row(float x0, uint8_t c0, const float xn[6], const uint8_t cn[6], const float bc[6],
+                                         float h, float& ax, float& dg) {
+  const bool n0 = c0 & C_NEU;
+  float s = 0.f, d = 0.f;
+#pragma unroll
+  for (int e = 0; e < 6; e++) {
+    const float cf = (n0 || (cn[e] & C_NEU)) ? 0.f : bc[e];
+    s = __builtin_fmaf(cf, x0 - xn[e], s);
+    d = d + cf;
+  }
+  const bool in = c0 & C_INT;
+  ax = in ? ((s * h) * h) * h : 0.f;
+  dg = in ? ((d * h) * h) * h : 0.f;
+}
+
+// A x (and its diagonal) on all 8 cells of the lane after stage_tile + wave_sync.
 template <bool WANT_AX, bool WANT_D>
 __device__ __forceinline__ void apply_pencil(const TileLDS& L, const Stencil& S, float ax[8], float dg[8]) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const int pc = (i + 1) * 10 + (j + 1);
   float xim[8], xip[8], xjm[8], xjp[8];
-  {
-    float4 a = L.xs[0][pc - 10], b = L.xs[1][pc - 10];
-    xim[0] = a.x; xim[1] = a.y; xim[2] = a.z; xim[3] = a.w; xim[4] = b.x; xim[5] = b.y; xim[6] = b.z; xim[7] = b.w;
-    a = L.xs[0][pc + 10]; b = L.xs[1][pc + 10];
-    xip[0] = a.x; xip[1] = a.y; xip[2] = a.z; xip[3] = a.w; xip[4] = b.x; xip[5] = b.y; xip[6] = b.z; xip[7] = b.w;
-    a = L.xs[0][pc - 1]; b = L.xs[1][pc - 1];
-    xjm[0] = a.x; xjm[1] = a.y; xjm[2] = a.z; xjm[3] = a.w; xjm[4] = b.x; xjm[5] = b.y; xjm[6] = b.z; xjm[7] = b.w;
-    a = L.xs[0][pc + 1]; b = L.xs[1][pc + 1];
-    xjp[0] = a.x; xjp[1] = a.y; xjp[2] = a.z; xjp[3] = a.w; xjp[4] = b.x; xjp[5] = b.y; xjp[6] = b.z; xjp[7] = b.w;
-  }
+  unpack8(L.xs[0][pc - 10], L.xs[1][pc - 10], xim);
+  unpack8(L.xs[0][pc + 10], L.xs[1][pc + 10], xip);
+  unpack8(L.xs[0][pc - 1], L.xs[1][pc - 1], xjm);
+  unpack8(L.xs[0][pc + 1], L.xs[1][pc + 1], xjp);
   const uint64_t cim = L.cs[pc - 10], cip = L.cs[pc + 10], cjm = L.cs[pc - 1], cjp = L.cs[pc + 1];
   const float bim = (i == 0) ? S.b_f[0] : S.b_int;
   const float bip = (i == 7) ? S.b_f[3] : S.b_int;
   const float bjm = (j == 0) ? S.b_f[1] : S.b_int;
   const float bjp = (j == 7) ? S.b_f[4] : S.b_int;
-  const float h = S.h;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    const uint8_t c0 = ct_at(S.ct, k);
-    const float x0 = S.own[k];
-    const uint8_t ckm = (k == 0) ? S.ckm : ct_at(S.ct, k - 1);
-    const uint8_t ckp = (k == 7) ? S.ckp : ct_at(S.ct, k + 1);
-    const float xkm = (k == 0) ? S.xkm : S.own[k - 1];
-    const float xkp = (k == 7) ? S.xkp : S.own[k + 1];
-    const float bkm = (k == 0) ? S.b_f[2] : S.b_int;
-    const float bkp = (k == 7) ? S.b_f[5] : S.b_int;
-    const bool n0 = c0 & C_NEU;
-    const float c1 = (n0 || (ct_at(cim, k) & C_NEU)) ? 0.f : bim;
-    const float c2 = (n0 || (ct_at(cip, k) & C_NEU)) ? 0.f : bip;
-    const float c3 = (n0 || (ct_at(cjm, k) & C_NEU)) ? 0.f : bjm;
-    const float c4 = (n0 || (ct_at(cjp, k) & C_NEU)) ? 0.f : bjp;
-    const float c5 = (n0 || (ckm & C_NEU)) ? 0.f : bkm;
-    const float c6 = (n0 || (ckp & C_NEU)) ? 0.f : bkp;
-    const bool in = c0 & C_INT;
-    if (WANT_AX) {
-      float s = 0.f;
-      s = __builtin_fmaf(c1, x0 - xim[k], s);
-      s = __builtin_fmaf(c2, x0 - xip[k], s);
-      s = __builtin_fmaf(c3, x0 - xjm[k], s);
-      s = __builtin_fmaf(c4, x0 - xjp[k], s);
-      s = __builtin_fmaf(c5, x0 - xkm, s);
-      s = __builtin_fmaf(c6, x0 - xkp, s);
-      ax[k] = in ? ((s * h) * h) * h : 0.f;
-    }
-    if (WANT_D) {
-      float d = 0.f;
-      d = d + c1; d = d + c2; d = d + c3; d = d + c4; d = d + c5; d = d + c6;
-      dg[k] = in ? ((d * h) * h) * h : 0.f;
-    }
+    const float xn[6] = {xim[k], xip[k], xjm[k], xjp[k], (k == 0) ? S.xkm : S.own[k - 1], (k == 7) ? S.xkp : S.own[k + 1]};
+    const uint8_t cn[6] = {ct_at(cim, k), ct_at(cip, k), ct_at(cjm, k), ct_at(cjp, k),
+                           (k == 0) ? S.ckm : ct_at(S.ct, k - 1), (k == 7) ? S.ckp : ct_at(S.ct, k + 1)};
+    const float bc[6] = {bim, bip, bjm, bjp, (k == 0) ? S.b_f[2] : S.b_int, (k == 7) ? S.b_f[5] : S.b_int};
+    float a, d;
+    cell_row(S.own[k], ct_at(S.ct, k), xn, cn, bc, S.h, a, d);
+    if (WANT_AX) ax[k] = a;
+    if (WANT_D) dg[k] = d;
   }
 }
 
-// rhs of the V-cycle on a LEAF|NONLEAF tile: NONLEAF tiles hold the restricted
-// residual; LEAF tiles use r plus, in octants that have a ghost child, that
-// ghost's restricted residual (the reference's rhs += ghost sum,
-// GMGSolver.cu:330-331, on top of rhs = r from the V-cycle init, :393-400).
-__device__ __forceinline__ void vcycle_rhs(const Ctx& c, int t, const DevTile& T, const float* __restrict__ f, float b[8]) {
+// Cells of one parity of a k-pencil held as two float4 (k 0..3, 4..7).
+__device__ __forceinline__ float4 even4(float4 a, float4 b) { return make_float4(a.x, a.z, b.x, b.z); }
+__device__ __forceinline__ float4 odd4(float4 a, float4 b) { return make_float4(a.y, a.w, b.y, b.w); }
+__device__ __forceinline__ float4 sel4(bool p, float4 if_false, float4 if_true) {
+  return make_float4(p ? if_true.x : if_false.x, p ? if_true.y : if_false.y, p ? if_true.z : if_false.z,
+                     p ? if_true.w : if_false.w);
+}
+__device__ __forceinline__ float comp4(float4 v, int m) { return m == 0 ? v.x : (m == 1 ? v.y : (m == 2 ? v.z : v.w)); }
+
+// One red-black colour update on the lane's four cells of that colour: cell
+// (i,j,k) is updated when (i+j+k) % 2 == color, i.e. k = 2m + p with the lane's
+// parity p.  The operands of those four rows are gathered with whole-vector
+// selects on p (no per-lane register indexing), so only the four rows are formed.
+__device__ __forceinline__ void gs_pencil(const TileLDS& L, const Stencil& S, const float b[8], int color, float4& out0,
+                                          float4& out1) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
-  const size_t base = size_t(t) * 512 + lane * 8;
-  if (T.type == T_NONLEAF) { load8(c.rhsv + base, b); return; }
-  load8(f + base, b);
-  if (T.gchild < 0) return;
-  for (int h = 0; h < 2; h++) {
-    int o = (i >> 2) | ((j >> 2) << 1) | (h << 2);
-    int gs = c.gcslot[T.gchild * 8 + o];
-    if (gs < 0) continue;
-    const float4 e = *reinterpret_cast<const float4*>(c.gext + size_t(gs) * 64 + (i & 3) * 16 + (j & 3) * 4);
-    b[4 * h + 0] = b[4 * h + 0] + e.x;
-    b[4 * h + 1] = b[4 * h + 1] + e.y;
-    b[4 * h + 2] = b[4 * h + 2] + e.z;
-    b[4 * h + 3] = b[4 * h + 3] + e.w;
+  const int pc = (i + 1) * 10 + (j + 1);
+  const bool p = ((color + i + j) & 1) != 0;
+  auto pick = [&](float4 a, float4 c) { return sel4(p, even4(a, c), odd4(a, c)); };
+  const float4 XIM = pick(L.xs[0][pc - 10], L.xs[1][pc - 10]);
+  const float4 XIP = pick(L.xs[0][pc + 10], L.xs[1][pc + 10]);
+  const float4 XJM = pick(L.xs[0][pc - 1], L.xs[1][pc - 1]);
+  const float4 XJP = pick(L.xs[0][pc + 1], L.xs[1][pc + 1]);
+  const float4 o0 = make_float4(S.own[0], S.own[1], S.own[2], S.own[3]);
+  const float4 o1 = make_float4(S.own[4], S.own[5], S.own[6], S.own[7]);
+  const float4 EV = even4(o0, o1), OD = odd4(o0, o1);
+  const float4 X0 = sel4(p, EV, OD);
+  // k-1 and k+1 neighbours of cell 2m+p
+  const float4 XKM = sel4(p, make_float4(S.xkm, OD.x, OD.y, OD.z), EV);
+  const float4 XKP = sel4(p, OD, make_float4(EV.y, EV.z, EV.w, S.xkp));
+  const float4 B = sel4(p, make_float4(b[0], b[2], b[4], b[6]), make_float4(b[1], b[3], b[5], b[7]));
+  const uint64_t cim = L.cs[pc - 10], cip = L.cs[pc + 10], cjm = L.cs[pc - 1], cjp = L.cs[pc + 1];
+  const float bim = (i == 0) ? S.b_f[0] : S.b_int;
+  const float bip = (i == 7) ? S.b_f[3] : S.b_int;
+  const float bjm = (j == 0) ? S.b_f[1] : S.b_int;
+  const float bjp = (j == 7) ? S.b_f[4] : S.b_int;
+  const int sh = p ? 8 : 0;
+  float nv[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int k0 = 2 * m;  // the cell is k0 + p
+    const int bit = 8 * k0 + sh;
+    const uint8_t c0 = uint8_t((S.ct >> bit) & 0xff);
+    const uint8_t ckm = (!p && m == 0) ? S.ckm : uint8_t((S.ct >> (bit - 8)) & 0xff);
+    const uint8_t ckp = (p && m == 3) ? S.ckp : uint8_t((S.ct >> (bit + 8)) & 0xff);
+    const float bkm = (!p && m == 0) ? S.b_f[2] : S.b_int;
+    const float bkp = (p && m == 3) ? S.b_f[5] : S.b_int;
+    const float x0 = comp4(X0, m);
+    const float xn[6] = {comp4(XIM, m), comp4(XIP, m), comp4(XJM, m), comp4(XJP, m), comp4(XKM, m), comp4(XKP, m)};
+    const uint8_t cn[6] = {uint8_t((cim >> bit) & 0xff), uint8_t((cip >> bit) & 0xff), uint8_t((cjm >> bit) & 0xff),
+                           uint8_t((cjp >> bit) & 0xff), ckm, ckp};
+    const float bc[6] = {bim, bip, bjm, bjp, bkm, bkp};
+    float ax, dg;
+    cell_row(x0, c0, xn, cn, bc, S.h, ax, dg);
+    float delta = 0.f;
+    if ((c0 & C_INT) && dg != 0.f) delta = (comp4(B, m) - ax) / dg;
+    nv[m] = x0 + delta;
   }
+  // put the four new values back at their parity positions
+  out0 = sel4(p, make_float4(nv[0], o0.y, nv[1], o0.w), make_float4(o0.x, nv[0], o0.z, nv[1]));
+  out1 = sel4(p, make_float4(nv[2], o1.y, nv[3], o1.w), make_float4(o1.x, nv[2], o1.z, nv[3]));
 }
 
 // ---------------------------------------------------------------------------
-// Kernel flags
+// Per-tile bodies (one wavefront each)
 // ---------------------------------------------------------------------------
-enum : int {
-  F_ZERO_X = 1,      // X is implicitly zero (first sweep)
-  F_GHOST_ZERO = 2,  // ghost X is zero (V-cycle downstroke)
-  F_VCYCLE_RHS = 4,  // rhs via vcycle_rhs (else the rhs channel)
-  F_VCYCLE_OUT = 8,  // restriction writes rhsv / gext (else the coarse channel)
-  F_SWIZZLE = 16,    // XCD-aware tile order
-  F_CHECK_DONE = 32, // skip tiles of converged problems
-};
-
-__device__ __forceinline__ bool prob_done(const Ctx& c, const DevTile& T, int flags) {
-  return (flags & F_CHECK_DONE) && c.ps[T.prob].done;
+__device__ __forceinline__ void load_rhs(const Ctx& c, const TileRec& T, const float* F, int flags, float b[8]) {
+  const float* src = (flags & F_VCYCLE) ? c.rhsv : F;
+  load8(src + size_t(T.self) * 512 + lane_id() * 8, b);
 }
 
-// Each tile is worked by exactly one wavefront with its own LDS region, so
-// staging only needs ordering inside the wavefront.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// ---------------------------------------------------------------------------
-// Per-tile bodies (one wavefront each), shared by the one-tile-per-workgroup
-// kernels and the coarse-level kernel.
-// ---------------------------------------------------------------------------
 // Red-black Gauss-Seidel colour sweep of a LEAF|NONLEAF tile (GMGSolver.cu:216-269).
 // An INTERIOR cell with D == 0 is left unchanged.
-__device__ __forceinline__ void tile_gs(const Ctx& c, int t, const DevTile& T, int level, float* X, const float* F,
-                                        int color, int flags, TileLDS& L) {
+__device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int level, float* X, const float* F, int color,
+                                        int flags, TileLDS& L) {
   Stencil S;
-  stage_tile(c, t, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, level, L, S);
+  stage_tile(c, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, level, L, S);
   float b[8];
-  if (flags & F_VCYCLE_RHS) vcycle_rhs(c, t, T, F, b);
-  else load8(F + size_t(t) * 512 + lane_id() * 8, b);
+  load_rhs(c, T, F, flags, b);
   if (prob_done(c, T, flags)) return;
   wave_sync();
-  float ax[8], dg[8];
-  apply_pencil<true, true>(L, S, ax, dg);
-  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
-  float out[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    out[k] = S.own[k];
-    if (((i + j + k) & 1) == color) {
-      float delta = 0.f;
-      if ((ct_at(S.ct, k) & C_INT) && dg[k] != 0.f) delta = (b[k] - ax[k]) / dg[k];
-      out[k] = S.own[k] + delta;
-    }
-  }
-  store8(X + size_t(t) * 512 + lane * 8, out);
+  float4 out0, out1;
+  gs_pencil(L, S, b, color, out0, out1);
+  float4* dst = reinterpret_cast<float4*>(X + size_t(T.self) * 512 + lane_id() * 8);
+  dst[0] = out0;
+  dst[1] = out1;
 }
 
 // Damped-Jacobi sweep (the update of GMGSolver.cu:508-525 with the level's
 // operator): Y = X + omega (b - A X) / D on INTERIOR cells, 0 where D == 0.
-__device__ __forceinline__ void tile_jacobi(const Ctx& c, int t, const DevTile& T, int level, const float* X,
-                                            const float* XG, float* Y, const float* F, float omega, int flags, TileLDS& L) {
+__device__ __forceinline__ void tile_jacobi(const Ctx& c, const TileRec& T, int level, const float* X, const float* XG,
+                                            float* Y, const float* F, float omega, int flags, TileLDS& L) {
   Stencil S;
-  stage_tile(c, t, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO, level, L, S);
+  stage_tile(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO, level, L, S);
   float b[8];
-  if (flags & F_VCYCLE_RHS) vcycle_rhs(c, t, T, F, b);
-  else load8(F + size_t(t) * 512 + lane_id() * 8, b);
+  load_rhs(c, T, F, flags, b);
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float ax[8], dg[8];
@@ -389,12 +397,12 @@ __device__ __forceinline__ void tile_jacobi(const Ctx& c, int t, const DevTile& 
       out[k] = (dg[k] == 0.f) ? 0.f : S.own[k] + (omega * res) / dg[k];
     }
   }
-  store8(Y + size_t(t) * 512 + lane_id() * 8, out);
+  store8(Y + size_t(T.self) * 512 + lane_id() * 8, out);
 }
 
-// 2x2x2 sums of a tile's pencil values, in the reference's (ii, jj, kk) order
-// (GMGSolver.cu:319-328): lane q = (a,b,c) = (q>>4, (q>>2)&3, q&3) returns the
-// sum of fine cells (2a+ii, 2b+jj, 2c+kk).  The values must be in LDS first.
+// 2x2x2 sums of a tile's values staged in LDS, in the reference's (ii, jj, kk)
+// order (GMGSolver.cu:319-328): lane q = (a,b,c) = (q>>4, (q>>2)&3, q&3) returns
+// the sum of fine cells (2a+ii, 2b+jj, 2c+kk).
 __device__ __forceinline__ float octet_sum(const float* Ls, int q) {
   int a = q >> 4, bq = (q >> 2) & 3, cq = q & 3;
   float s = 0.f;
@@ -403,28 +411,24 @@ __device__ __forceinline__ float octet_sum(const float* Ls, int q) {
       for (int kk = 0; kk < 2; kk++) s += Ls[((2 * a + ii) * 8 + (2 * bq + jj)) * 8 + 2 * cq + kk];
   return s;
 }
-__device__ __forceinline__ size_t parent_cell(const DevTile& T, int q) {
+__device__ __forceinline__ size_t parent_cell(const TileRec& T, int q) {
   int a = q >> 4, bq = (q >> 2) & 3, cq = q & 3;
   int ox = T.octant & 1, oy = (T.octant >> 1) & 1, oz = (T.octant >> 2) & 1;
   return size_t(T.parent) * 512 + ((ox * 4 + a) << 6) + ((oy * 4 + bq) << 3) + (oz * 4 + cq);
 }
 
 // Residual and restriction of one tile of `level` to level-1
-// (GMGSolver.cu:284-339).  V-cycle output: NONLEAF/LEAF fine tiles write the
-// parent's rhsv (= sum * one_over_alpha); GHOST fine tiles write their 64 sums
-// to gext (added to the parent LEAF's r when the parent level is smoothed).
-// Channel output: parent channel = sum * one_over_alpha, or += sum for ghosts.
-__device__ __forceinline__ void tile_restrict(const Ctx& c, int t, const DevTile& T, int level, const float* X,
-                                              const float* F, float* CO, float one_over_alpha, int flags, TileLDS& L,
-                                              float* res) {
+// (GMGSolver.cu:284-339).  Output channel CO (rhsv in V-cycle mode): the
+// parent cell = sum * one_over_alpha, or += sum for a GHOST fine tile (whose
+// LEAF parent's rhs already holds r).  In V-cycle mode a ghost's rhs is 0.
+__device__ __forceinline__ void tile_restrict(const Ctx& c, const TileRec& T, int level, const float* X, const float* F,
+                                              float* CO, float one_over_alpha, int flags, TileLDS& L, float* res) {
   Stencil S;
-  stage_tile(c, t, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, level, L, S);
+  stage_tile(c, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, level, L, S);
   const int lane = lane_id();
   float b[8];
-  if (flags & F_VCYCLE_RHS) {
-    if (T.type == T_GHOST) { for (int k = 0; k < 8; k++) b[k] = 0.f; }
-    else vcycle_rhs(c, t, T, F, b);
-  } else load8(F + size_t(t) * 512 + lane * 8, b);
+  if ((flags & F_VCYCLE) && T.type == T_GHOST) { for (int k = 0; k < 8; k++) b[k] = 0.f; }
+  else load_rhs(c, T, F, flags, b);
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float ax[8], dg[8];
@@ -433,24 +437,20 @@ __device__ __forceinline__ void tile_restrict(const Ctx& c, int t, const DevTile
   wave_sync();
   float s = octet_sum(res, lane);
   const bool ghost = (subtree_type(T.type, T.level, level) & T_GHOST);
-  if (flags & F_VCYCLE_OUT) {
-    if (ghost) c.gext[size_t(T.gslot) * 64 + lane] = s;
-    else c.rhsv[parent_cell(T, lane)] = s * one_over_alpha;
-  } else {
-    size_t pc = parent_cell(T, lane);
-    if (ghost) CO[pc] = CO[pc] + s;
-    else CO[pc] = s * one_over_alpha;
-  }
+  float* out = (flags & F_VCYCLE) ? c.rhsv : CO;
+  size_t pcell = parent_cell(T, lane);
+  if (ghost) out[pcell] = out[pcell] + s;
+  else out[pcell] = s * one_over_alpha;
 }
 
 // Prolongation and update of one tile of `level` (GMGSolver.cu:345-376):
 // ghosts take the parent value, others add prolong_coeff * parent value, on
 // INTERIOR cells; other cells become 0.  Ghost results go to XG (the primary
 // buffer) and active tiles to XO (a Jacobi ping-pong target).
-__device__ __forceinline__ void tile_prolong(const Ctx& c, int t, const DevTile& T, int level, const float* XC,
-                                             const float* XF, float* XO, float* XG, float pcoef, int flags) {
+__device__ __forceinline__ void tile_prolong(const Ctx& c, const TileRec& T, int level, const float* XC, const float* XF,
+                                             float* XO, float* XG, float pcoef, int flags) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
-  const size_t base = size_t(t) * 512 + lane * 8;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
   int ox = T.octant & 1, oy = (T.octant >> 1) & 1, oz = (T.octant >> 2) & 1;
   const float4 pv = *reinterpret_cast<const float4*>(XC + size_t(T.parent) * 512 + ((ox * 4 + (i >> 1)) << 6) +
                                                      ((oy * 4 + (j >> 1)) << 3) + oz * 4);
@@ -474,37 +474,34 @@ __device__ __forceinline__ void tile_prolong(const Ctx& c, int t, const DevTile&
 // ---------------------------------------------------------------------------
 // One-tile-per-workgroup kernels over a worklist
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_gs_color(Ctx c, const int32_t* __restrict__ list, int n, int level,
-                                                 float* X, const float* F, int color, int flags) {
-  int w = work_index(n, flags & F_SWIZZLE);
-  if (w >= n) return;
-  const int t = list[w];
+#define TILE_KERNEL_PROLOGUE                     \
+  int w = work_index(flags & F_SWIZZLE);         \
+  if (w >= n) return;                            \
+  const TileRec T = list[w];
+
+__global__ __launch_bounds__(64) void k_gs_color(Ctx c, const TileRec* __restrict__ list, int n, int level, float* X,
+                                                 const float* F, int color, int flags) {
+  TILE_KERNEL_PROLOGUE
   __shared__ TileLDS L;
-  tile_gs(c, t, c.tiles[t], level, X, F, color, flags, L);
+  tile_gs(c, T, level, X, F, color, flags, L);
 }
-__global__ __launch_bounds__(64) void k_jacobi(Ctx c, const int32_t* __restrict__ list, int n, int level, const float* X,
+__global__ __launch_bounds__(64) void k_jacobi(Ctx c, const TileRec* __restrict__ list, int n, int level, const float* X,
                                                const float* XG, float* Y, const float* F, float omega, int flags) {
-  int w = work_index(n, flags & F_SWIZZLE);
-  if (w >= n) return;
-  const int t = list[w];
+  TILE_KERNEL_PROLOGUE
   __shared__ TileLDS L;
-  tile_jacobi(c, t, c.tiles[t], level, X, XG, Y, F, omega, flags, L);
+  tile_jacobi(c, T, level, X, XG, Y, F, omega, flags, L);
 }
-__global__ __launch_bounds__(64) void k_restrict(Ctx c, const int32_t* __restrict__ list, int n, int level, const float* X,
+__global__ __launch_bounds__(64) void k_restrict(Ctx c, const TileRec* __restrict__ list, int n, int level, const float* X,
                                                  const float* F, float* CO, float one_over_alpha, int flags) {
-  int w = work_index(n, flags & F_SWIZZLE);
-  if (w >= n) return;
-  const int t = list[w];
+  TILE_KERNEL_PROLOGUE
   __shared__ TileLDS L;
   __shared__ float res[512];
-  tile_restrict(c, t, c.tiles[t], level, X, F, CO, one_over_alpha, flags, L, res);
+  tile_restrict(c, T, level, X, F, CO, one_over_alpha, flags, L, res);
 }
-__global__ __launch_bounds__(64) void k_prolong(Ctx c, const int32_t* __restrict__ list, int n, int level,
+__global__ __launch_bounds__(64) void k_prolong(Ctx c, const TileRec* __restrict__ list, int n, int level,
                                                 const float* XC, const float* XF, float* XO, float* XG, float pcoef, int flags) {
-  int w = work_index(n, flags & F_SWIZZLE);
-  if (w >= n) return;
-  const int t = list[w];
-  tile_prolong(c, t, c.tiles[t], level, XC, XF, XO, XG, pcoef, flags);
+  TILE_KERNEL_PROLOGUE
+  tile_prolong(c, T, level, XC, XF, XO, XG, pcoef, flags);
 }
 
 // ---------------------------------------------------------------------------
@@ -513,8 +510,7 @@ __global__ __launch_bounds__(64) void k_prolong(Ctx c, const int32_t* __restrict
 // of those levels (active and ghost, at most CL_MAX_TILES) resident in LDS and
 // one thread per cell.  The host picks C so that these levels are tiny; one
 // launch then replaces the ~10 launches per level a grid-wide phase each would
-// need.  The per-cell arithmetic is the same as the tile kernels' (faces in
-// the order x-, x+, y-, y+, z-, z+, fmaf, ((s*h)*h)*h, (b-Ax)/D), so results
+// need.  The per-cell arithmetic is the same as the tile kernels', so results
 // are identical; the bottom is GMGSolver.cu:432-433.
 // ---------------------------------------------------------------------------
 constexpr int CL_MAX_TILES = 16;
@@ -527,9 +523,9 @@ struct alignas(16) CoarseSlot {
   int32_t level;
   float h;
   uint8_t type, octant, pad[2];
-  int32_t pad2;
+  float bc[7];       // base coefficients at the slot's own level: faces x-,y-,z-,x+,y+,z+ and inside
 };
-static_assert(sizeof(CoarseSlot) == 48, "CoarseSlot is 48 bytes");
+static_assert(sizeof(CoarseSlot) == 72 || sizeof(CoarseSlot) == 80, "CoarseSlot layout");
 struct CoarsePlan {
   int C;                              // deepest level held in LDS
   int nslots;
@@ -548,44 +544,35 @@ struct CoarseLDS {
   CoarseSlot S[CL_MAX_TILES];
 };
 
-// A x and D of cell (i,j,k) of slot s with the subtree view at level sl.
-__device__ __forceinline__ void cl_stencil(const CoarseLDS& M, const float (*Z)[512], int s, int i, int j, int k, int sl,
-                                           float& ax, float& dg) {
+// A x and D of cell (i,j,k) of slot s (subtree view at the slot's level); the
+// same arithmetic as cell_row, accumulated face by face to keep registers low.
+__device__ __forceinline__ void cl_stencil(const CoarseLDS& M, int s, int i, int j, int k, float& ax, float& dg) {
   const CoarseSlot& S = M.S[s];
   const int c = (i << 6) | (j << 3) | k;
-  const uint8_t to = subtree_type(S.type, S.level, sl);
-  const float ooh = 1.0f / S.h;
-  const float cll = ooh * ooh, clg = ((2.0f * ooh) / 3.0f) * ooh;
   const uint8_t c0 = M.ct[s][c];
-  const float x0 = Z[s][c];
+  const float x0 = M.z[s][c];
   const bool n0 = c0 & C_NEU;
   float sum = 0.f, d = 0.f;
   const int di[6] = {-1, 1, 0, 0, 0, 0}, dj[6] = {0, 0, -1, 1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
   const int fi[6] = {0, 3, 1, 4, 2, 5};
 #pragma unroll
   for (int e = 0; e < 6; e++) {
-    int ni = i + di[e], nj = j + dj[e], nk = k + dk[e];
-    float xv;
-    uint8_t cv, tn;
+    const int ni = i + di[e], nj = j + dj[e], nk = k + dk[e];
+    float xv, bcv;
+    uint8_t cv;
     if (ni >= 0 && ni < 8 && nj >= 0 && nj < 8 && nk >= 0 && nk < 8) {
       const int nc = (ni << 6) | (nj << 3) | nk;
-      xv = Z[s][nc];
+      xv = M.z[s][nc];
       cv = M.ct[s][nc];
-      tn = to;
+      bcv = S.bc[6];
     } else {
       const int q = S.nbr[fi[e]];
-      if (q < 0) {
-        xv = 0.f;
-        cv = C_DIR;
-        tn = to;
-      } else {
-        const int nc = ((ni & 7) << 6) | ((nj & 7) << 3) | (nk & 7);
-        xv = Z[q][nc];
-        cv = M.ct[q][nc];
-        tn = subtree_type(M.S[q].type, M.S[q].level, sl);
-      }
+      const int nc = ((ni & 7) << 6) | ((nj & 7) << 3) | (nk & 7);
+      xv = q < 0 ? 0.f : M.z[q][nc];
+      cv = q < 0 ? C_DIR : M.ct[q][nc];
+      bcv = S.bc[fi[e]];
     }
-    const float cf = (n0 || (cv & C_NEU)) ? 0.f : base_coeff(to, tn, cll, clg);
+    const float cf = (n0 || (cv & C_NEU)) ? 0.f : bcv;
     sum = __builtin_fmaf(cf, x0 - xv, sum);
     d = d + cf;
   }
@@ -595,43 +582,20 @@ __device__ __forceinline__ void cl_stencil(const CoarseLDS& M, const float (*Z)[
   dg = in ? ((d * h) * h) * h : 0.f;
 }
 
-__global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, CoarsePlan cp, VCycleArgs v, float* Z, const float* R,
-                                                           int flags) {
+__global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, CoarsePlan cp, VCycleArgs v, float* Z, int flags) {
   __shared__ CoarseLDS M;
   if ((flags & F_CHECK_DONE) && c.ps[0].done) return;
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int q = tid; q < cp.nslots; q += nt) M.S[q] = cp.slots[q];
   __syncthreads();
-  // load cell types and the rhs; z starts at zero
+  // cell types and rhs (V-cycle rhs buffer for active tiles, 0 for ghosts); z = 0
   for (int idx = tid; idx < cp.nslots * 64; idx += nt) {
     const int s = idx >> 6, lane = idx & 63;
     const CoarseSlot& S = M.S[s];
     const size_t base = size_t(S.tile) * 512 + lane * 8;
-    const uint64_t ct = load_ct8(c.ctype + base);
-    *reinterpret_cast<uint64_t*>(&M.ct[s][lane * 8]) = ct;
+    *reinterpret_cast<uint64_t*>(&M.ct[s][lane * 8]) = load_ct8(c.ctype + base);
     float b[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (S.type == T_LEAF) {
-      load8(R + base, b);
-      if (S.level == cp.C) {
-        // ghost children below the LDS levels were restricted into gext
-        const DevTile T = c.tiles[S.tile];
-        if (T.gchild >= 0) {
-          const int i = lane >> 3, j = lane & 7;
-          for (int hh = 0; hh < 2; hh++) {
-            int o = (i >> 2) | ((j >> 2) << 1) | (hh << 2);
-            int gs = c.gcslot[T.gchild * 8 + o];
-            if (gs < 0) continue;
-            const float4 e = *reinterpret_cast<const float4*>(c.gext + size_t(gs) * 64 + (i & 3) * 16 + (j & 3) * 4);
-            b[4 * hh + 0] = b[4 * hh + 0] + e.x;
-            b[4 * hh + 1] = b[4 * hh + 1] + e.y;
-            b[4 * hh + 2] = b[4 * hh + 2] + e.z;
-            b[4 * hh + 3] = b[4 * hh + 3] + e.w;
-          }
-        }
-      }
-    } else if (S.type == T_NONLEAF && S.level == cp.C) {
-      load8(c.rhsv + base, b);
-    }
+    if (S.type != T_GHOST) load8(c.rhsv + base, b);
     for (int k = 0; k < 8; k++) {
       M.b[s][lane * 8 + k] = b[k];
       M.z[s][lane * 8 + k] = 0.f;
@@ -647,7 +611,7 @@ __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, CoarsePlan cp,
       if (M.S[s].type == T_GHOST) continue;
       const int i = r >> 5, j = (r >> 2) & 7, k = 2 * (r & 3) + ((color + i + j) & 1);
       float ax, dg;
-      cl_stencil(M, M.z, s, i, j, k, l, ax, dg);
+      cl_stencil(M, s, i, j, k, ax, dg);
       const int cc = (i << 6) | (j << 3) | k;
       float delta = 0.f;
       if ((M.ct[s][cc] & C_INT) && dg != 0.f) delta = (M.b[s][cc] - ax) / dg;
@@ -662,7 +626,7 @@ __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, CoarsePlan cp,
       const int s = s0 + (idx >> 9), cc = idx & 511;
       if (M.S[s].type == T_GHOST) continue;
       float ax, dg;
-      cl_stencil(M, M.z, s, cc >> 6, (cc >> 3) & 7, cc & 7, l, ax, dg);
+      cl_stencil(M, s, cc >> 6, (cc >> 3) & 7, cc & 7, ax, dg);
       float out = M.z[s][cc];
       if (M.ct[s][cc] & C_INT) out = (dg == 0.f) ? 0.f : M.z[s][cc] + (v.omega * (M.b[s][cc] - ax)) / dg;
       M.z2[s][cc] = out;
@@ -693,7 +657,7 @@ __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, CoarsePlan cp,
           for (int kk = 0; kk < 2; kk++) {
             const int fi = 2 * a + ii, fj = 2 * bq + jj, fk = 2 * cq + kk;
             float ax, dg;
-            cl_stencil(M, M.z, s, fi, fj, fk, l, ax, dg);
+            cl_stencil(M, s, fi, fj, fk, ax, dg);
             const float rhs = (M.S[s].type == T_GHOST) ? 0.f : M.b[s][(fi << 6) | (fj << 3) | fk];
             sum += rhs - ax;
           }
@@ -739,41 +703,36 @@ __global__ void k_empty(int* p) {
 }
 
 // ---------------------------------------------------------------------------
-// CG kernels (rotated loop: V-cycle, gamma/beta, p, A p + alpha, x/r + check)
+// CG kernels (rotated loop: V-cycle, gamma/beta, p, A p + alpha, x/r + check).
+// Kernels write per-tile FP64 partials; k_finalize reduces them per problem.
 // ---------------------------------------------------------------------------
-// Init: x = 0, p = 0 on LEAF tiles; |b|^2; threshold; zero-rhs early exit
+// Init: x = 0, p = 0 on LEAF tiles, V-cycle rhs = b; partials of |b|^2
 // (GMGSolver.cu:537-555).
-__global__ __launch_bounds__(64) void k_cg_init(Ctx c, const int32_t* __restrict__ list, int n, int flags) {
-  int w = work_index(n, flags & F_SWIZZLE);
-  if (w >= n) return;
-  const int t = list[w];
-  const DevTile T = c.tiles[t];
+__global__ __launch_bounds__(64) void k_cg_init(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
+  TILE_KERNEL_PROLOGUE
   const int lane = lane_id();
-  const size_t base = size_t(t) * 512 + lane * 8;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
   float zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   store8(c.x + base, zero);
   store8(c.p + base, zero);
   float rv[8];
   load8(c.r + base, rv);
+  store8(c.rhsv + base, rv);
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(rv[k]) * double(rv[k]);
   a = wave_sum(a);
   if (lane == 0) c.part[T.leaf_slot] = a;
 }
 
-// gamma = r.z over LEAF tiles; beta = gamma / gamma_old (0 on the first
-// iteration, where p = z: GMGSolver.cu:564-568, 629-638).
-__global__ __launch_bounds__(64) void k_cg_dot_rz(Ctx c, const int32_t* __restrict__ list, int n, int flags) {
-  int w = work_index(n, flags & F_SWIZZLE);
-  if (w >= n) return;
-  const int t = list[w];
-  const DevTile T = c.tiles[t];
-  if (prob_done(c, T, flags)) return;
+// partials of gamma = r.z over LEAF tiles (GMGSolver.cu:634)
+__global__ __launch_bounds__(64) void k_cg_dot_rz(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
+  TILE_KERNEL_PROLOGUE
   const int lane = lane_id();
-  const size_t base = size_t(t) * 512 + lane * 8;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
   float rv[8], zv[8];
   load8(c.r + base, rv);
   load8(c.z + base, zv);
+  if (prob_done(c, T, flags)) return;
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(rv[k]) * double(zv[k]);
   a = wave_sum(a);
@@ -783,29 +742,26 @@ __global__ __launch_bounds__(64) void k_cg_dot_rz(Ctx c, const int32_t* __restri
 // p = z + beta p on INTERIOR cells of LEAF tiles (GMGSolver.cu:642-653), then
 // the new p copied into the tile's ghost children (their x is the parent's
 // INTERIOR value, PoissonGrid.cu:585-605), ready for the operator.
-__global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const int32_t* __restrict__ list, int n, int flags) {
-  int w = work_index(n, flags & F_SWIZZLE);
-  if (w >= n) return;
-  const int t = list[w];
-  const DevTile T = c.tiles[t];
-  if (prob_done(c, T, flags)) return;
+__global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
+  TILE_KERNEL_PROLOGUE
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
-  const size_t base = size_t(t) * 512 + lane * 8;
-  const float beta = float(c.ps[T.prob].beta);
+  const size_t base = size_t(T.self) * 512 + lane * 8;
   float pv[8], zv[8];
   load8(c.p + base, pv);
   load8(c.z + base, zv);
   const uint64_t ct = load_ct8(c.ctype + base);
+  if (prob_done(c, T, flags)) return;
+  const float beta = float(c.ps[T.prob].beta);
   for (int k = 0; k < 8; k++)
     if (ct_at(ct, k) & C_INT) pv[k] = __builtin_fmaf(beta, pv[k], zv[k]);
   store8(c.p + base, pv);
-  if (T.gchild < 0) return;
+  if (!T.has_gchild) return;
   __shared__ float Lp[512];
   __shared__ uint8_t Lc[512];
   for (int k = 0; k < 8; k++) { Lp[lane * 8 + k] = pv[k]; Lc[lane * 8 + k] = ct_at(ct, k); }
-  __syncthreads();
+  wave_sync();
   for (int o = 0; o < 8; o++) {
-    int g = c.gchild[T.gchild * 8 + o];
+    int g = T.gch[o];
     if (g < 0) continue;
     int ox = o & 1, oy = (o >> 1) & 1, oz = (o >> 2) & 1;
     int src = ((ox * 4 + (i >> 1)) * 8 + (oy * 4 + (j >> 1))) * 8 + oz * 4;
@@ -815,36 +771,33 @@ __global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const int32_t* __rest
   }
 }
 
-// A p on LEAF and GHOST tiles (subtree level -1).  LEAF tiles write A p and,
-// when they have no ghost children, their p.Ap partial; GHOST tiles write the
-// 2x2x2 sums of their rows for the parent (AccumulateToParents128 with all
-// cell types, PoissonGrid.cu:664-710).  Channel mode (F_VCYCLE_OUT clear)
-// writes ghost rows to AX instead and takes no dot.
-__global__ __launch_bounds__(64) void k_apply(Ctx c, const int32_t* __restrict__ list, int n, const float* __restrict__ X,
+// A p on LEAF and GHOST tiles with the subtree view sl.  In CG mode (dot=1)
+// LEAF tiles write A p and, when they have no ghost children, their p.Ap
+// partial; GHOST tiles write the 2x2x2 sums of their rows for the parent
+// (AccumulateToParents128 with all cell types, PoissonGrid.cu:664-710).  In
+// channel mode (dot=0) every listed tile writes its rows to AX.
+__global__ __launch_bounds__(64) void k_apply(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ X,
                                               float* __restrict__ AX, int sl, int diag, int dot, int flags) {
-  int w = work_index(n, flags & F_SWIZZLE);
-  if (w >= n) return;
-  const int t = list[w];
-  const DevTile T = c.tiles[t];
-  if (prob_done(c, T, flags)) return;
+  TILE_KERNEL_PROLOGUE
   __shared__ TileLDS L;
   __shared__ float gl[512];
   Stencil S;
-  stage_tile(c, t, T, X, X, false, false, sl, L, S);
+  stage_tile(c, T, X, X, false, false, sl, L, S);
+  if (prob_done(c, T, flags)) return;
   wave_sync();
   float ax[8], dg[8];
   if (diag) apply_pencil<false, true>(L, S, dg, ax);
   else apply_pencil<true, false>(L, S, ax, dg);
   const int lane = lane_id();
-  const size_t base = size_t(t) * 512 + lane * 8;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
   if (T.type == T_GHOST && dot) {
     for (int k = 0; k < 8; k++) gl[lane * 8 + k] = (ct_at(S.ct, k) & (C_INT | C_DIR | C_NEU)) ? ax[k] : 0.f;
-    __syncthreads();
+    wave_sync();
     c.gsum[size_t(T.gslot) * 64 + lane] = octet_sum(gl, lane);
     return;
   }
   store8(AX + base, ax);
-  if (!dot || T.type != T_LEAF || T.gchild >= 0) return;
+  if (!dot || T.type != T_LEAF || T.has_gchild) return;
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(S.own[k]) * double(ax[k]);
   a = wave_sum(a);
@@ -852,28 +805,27 @@ __global__ __launch_bounds__(64) void k_apply(Ctx c, const int32_t* __restrict__
 }
 
 // LEAF tiles with ghost children: A p += ghost row sums (the += of
-// AccumulateToParents128), then the p.Ap partial; the last tile of the problem
-// forms alpha = gamma / p.Ap (GMGSolver.cu:580-582).
-__global__ __launch_bounds__(64) void k_ghost_gather(Ctx c, const int32_t* __restrict__ list, int n, int flags) {
-  int w = work_index(n, flags & F_SWIZZLE);
-  if (w >= n) return;
-  const int t = list[w];
-  const DevTile T = c.tiles[t];
-  if (prob_done(c, T, flags)) return;
+// AccumulateToParents128), then the p.Ap partial (GMGSolver.cu:580).
+__global__ __launch_bounds__(64) void k_ghost_gather(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
+  TILE_KERNEL_PROLOGUE
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
-  const size_t base = size_t(t) * 512 + lane * 8;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
   float av[8], pv[8];
   load8(c.ap + base, av);
   load8(c.p + base, pv);
+  float4 e[2];
+  int gs[2];
   for (int h = 0; h < 2; h++) {
-    int o = (i >> 2) | ((j >> 2) << 1) | (h << 2);
-    int gs = c.gcslot[T.gchild * 8 + o];
-    if (gs < 0) continue;
-    const float4 e = *reinterpret_cast<const float4*>(c.gsum + size_t(gs) * 64 + (i & 3) * 16 + (j & 3) * 4);
-    av[4 * h + 0] = av[4 * h + 0] + e.x;
-    av[4 * h + 1] = av[4 * h + 1] + e.y;
-    av[4 * h + 2] = av[4 * h + 2] + e.z;
-    av[4 * h + 3] = av[4 * h + 3] + e.w;
+    gs[h] = list[w].gcs[(i >> 2) | ((j >> 2) << 1) | (h << 2)];  // lane-dependent index: read from memory
+    if (gs[h] >= 0) e[h] = *reinterpret_cast<const float4*>(c.gsum + size_t(gs[h]) * 64 + (i & 3) * 16 + (j & 3) * 4);
+  }
+  if (prob_done(c, T, flags)) return;
+  for (int h = 0; h < 2; h++) {
+    if (gs[h] < 0) continue;
+    av[4 * h + 0] = av[4 * h + 0] + e[h].x;
+    av[4 * h + 1] = av[4 * h + 1] + e[h].y;
+    av[4 * h + 2] = av[4 * h + 2] + e[h].z;
+    av[4 * h + 3] = av[4 * h + 3] + e[h].w;
   }
   store8(c.ap + base, av);
   double a = 0.0;
@@ -883,24 +835,19 @@ __global__ __launch_bounds__(64) void k_ghost_gather(Ctx c, const int32_t* __res
 }
 
 // x += alpha p, r -= alpha A p on INTERIOR cells of LEAF tiles
-// (GMGSolver.cu:587-609), |r|^2, and the convergence test of
-// GMGSolver.cu:611-619 (or the iteration limit) setting the problem's done flag.
-__global__ __launch_bounds__(64) void k_cg_update_xr(Ctx c, const int32_t* __restrict__ list, int n, int flags) {
-  int w = work_index(n, flags & F_SWIZZLE);
-  if (w >= n) return;
-  const int t = list[w];
-  const DevTile T = c.tiles[t];
-  if (prob_done(c, T, flags)) return;
+// (GMGSolver.cu:587-609); r is also the next V-cycle's rhs; partials of |r|^2.
+__global__ __launch_bounds__(64) void k_cg_update_xr(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
+  TILE_KERNEL_PROLOGUE
   const int lane = lane_id();
-  const size_t base = size_t(t) * 512 + lane * 8;
-  ProbState* ps = c.ps + T.prob;
-  const float alpha = float(ps->alpha);
+  const size_t base = size_t(T.self) * 512 + lane * 8;
   float xv[8], rv[8], pv[8], av[8];
   load8(c.x + base, xv);
   load8(c.r + base, rv);
   load8(c.p + base, pv);
   load8(c.ap + base, av);
   const uint64_t ct = load_ct8(c.ctype + base);
+  if (prob_done(c, T, flags)) return;
+  const float alpha = float(c.ps[T.prob].alpha);
   for (int k = 0; k < 8; k++) {
     if (ct_at(ct, k) & C_INT) {
       xv[k] = __builtin_fmaf(alpha, pv[k], xv[k]);
@@ -909,23 +856,46 @@ __global__ __launch_bounds__(64) void k_cg_update_xr(Ctx c, const int32_t* __res
   }
   store8(c.x + base, xv);
   store8(c.r + base, rv);
+  store8(c.rhsv + base, rv);
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(rv[k]) * double(rv[k]);
   a = wave_sum(a);
   if (lane == 0) c.part[T.leaf_slot] = a;
 }
 
-// Per-problem finalisation of a reduction over the LEAF-tile partials (one
-// wavefront per problem, launched after the kernel that wrote the partials).
+// Sum of a problem's per-tile partials in the fixed order (lane q adds
+// partials q, q+64, ... in sequence, then the butterfly), staged through LDS by
+// the whole workgroup so the loads are issued at once.
+constexpr int FIN_THREADS = 256;
+constexpr int FIN_CHUNK = 4096;
+__device__ __forceinline__ double block_sum_partials(const double* __restrict__ part, int begin, int count, double* lds) {
+  const int tid = threadIdx.x;
+  double a = 0.0;
+  for (int c0 = 0; c0 < count; c0 += FIN_CHUNK) {
+    const int m = min(FIN_CHUNK, count - c0);
+    __syncthreads();
+    for (int q = tid; q < m; q += FIN_THREADS) lds[q] = part[begin + c0 + q];
+    __syncthreads();
+    if (tid < 64) {
+      // lane tid continues its sequence tid, tid+64, ... across chunks (FIN_CHUNK is a multiple of 64)
+      for (int q = tid; q < m; q += 64) a += lds[q];
+    }
+  }
+  if (tid < 64) a = wave_sum(a);
+  return a;
+}
+
+// Per-problem finalisation of a reduction over the LEAF-tile partials.
 enum : int { FIN_BB = 0, FIN_RZ = 1, FIN_PAP = 2, FIN_RR = 3, FIN_MEAN = 4 };
-__global__ __launch_bounds__(64) void k_finalize(Ctx c, int mode, int flags) {
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctx c, int mode, int flags) {
+  __shared__ double lds[FIN_CHUNK];
   ProbState* ps = c.ps + blockIdx.x;
   if (mode != FIN_BB && (flags & F_CHECK_DONE) && ps->done) return;
   if (mode == FIN_MEAN && !ps->pure_neumann) return;
-  const double s = sum_partials(c.part, ps->leaf_begin, ps->leaf_count);
+  const double s = block_sum_partials(c.part, ps->leaf_begin, ps->leaf_count, lds);
   double q = 0.0;
-  if (mode == FIN_MEAN) q = sum_partials(c.part2, ps->leaf_begin, ps->leaf_count);
-  if (lane_id() != 0) return;
+  if (mode == FIN_MEAN) q = block_sum_partials(c.part2, ps->leaf_begin, ps->leaf_count, lds);
+  if (threadIdx.x != 0) return;
   switch (mode) {
     case FIN_BB: {  // GMGSolver.cu:542-554
       ps->bb = s;
@@ -964,16 +934,13 @@ __global__ __launch_bounds__(64) void k_finalize(Ctx c, int mode, int flags) {
 // Pure-Neumann re-centring (MeanAsync + ReCenterLeafVoxels,
 // PoissonGrid.cu:231-325, PoissonSolver.cu:356-368): mean of r over all cells
 // of LEAF tiles divided by the INTERIOR count, subtracted on INTERIOR cells.
-__global__ __launch_bounds__(64) void k_mean(Ctx c, const int32_t* __restrict__ list, int n, const float* __restrict__ X, int flags) {
-  int w = work_index(n, flags & F_SWIZZLE);
-  if (w >= n) return;
-  const int t = list[w];
-  const DevTile T = c.tiles[t];
+__global__ __launch_bounds__(64) void k_mean(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ X, int flags) {
+  TILE_KERNEL_PROLOGUE
   if (prob_done(c, T, flags)) return;
   ProbState* ps = c.ps + T.prob;
   if (!ps->pure_neumann) return;
   const int lane = lane_id();
-  const size_t base = size_t(t) * 512 + lane * 8;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
   float v[8];
   load8(X + base, v);
   const uint64_t ct = load_ct8(c.ctype + base);
@@ -983,35 +950,32 @@ __global__ __launch_bounds__(64) void k_mean(Ctx c, const int32_t* __restrict__ 
   q = wave_sum(q);
   if (lane == 0) { c.part[T.leaf_slot] = a; c.part2[T.leaf_slot] = q; }
 }
-__global__ __launch_bounds__(64) void k_subtract_mean(Ctx c, const int32_t* __restrict__ list, int n, float* __restrict__ X, int flags) {
-  int w = work_index(n, flags & F_SWIZZLE);
-  if (w >= n) return;
-  const int t = list[w];
-  const DevTile T = c.tiles[t];
+__global__ __launch_bounds__(64) void k_subtract_mean(Ctx c, const TileRec* __restrict__ list, int n, float* __restrict__ X,
+                                                      int write_rhsv, int flags) {
+  TILE_KERNEL_PROLOGUE
   if (prob_done(c, T, flags)) return;
   ProbState* ps = c.ps + T.prob;
   if (!ps->pure_neumann) return;
   const double mean = ps->mean;
-  const size_t base = size_t(t) * 512 + lane_id() * 8;
+  const size_t base = size_t(T.self) * 512 + lane_id() * 8;
   float v[8];
   load8(X + base, v);
   const uint64_t ct = load_ct8(c.ctype + base);
   for (int k = 0; k < 8; k++) if (ct_at(ct, k) & C_INT) v[k] = float(double(v[k]) - mean);
   store8(X + base, v);
+  if (write_rhsv) store8(c.rhsv + base, v);
 }
 
 // ---------------------------------------------------------------------------
 // Channel-level kernels for the operator API
 // ---------------------------------------------------------------------------
 // Ghost x from the parent's INTERIOR value (PoissonGrid.cu:585-605).
-__global__ __launch_bounds__(64) void k_propagate_ghosts(Ctx c, const int32_t* __restrict__ list, int n,
+__global__ __launch_bounds__(64) void k_propagate_ghosts(Ctx c, const TileRec* __restrict__ list, int n,
                                                          const float* __restrict__ XC, float* __restrict__ XF) {
-  int w = blockIdx.x;
-  if (w >= n) return;
-  const int t = list[w];
-  const DevTile T = c.tiles[t];
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
-  const size_t base = size_t(t) * 512 + lane * 8;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
   int ox = T.octant & 1, oy = (T.octant >> 1) & 1, oz = (T.octant >> 2) & 1;
   const size_t pb = size_t(T.parent) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) + oz * 4;
   const float4 pv = *reinterpret_cast<const float4*>(XC + pb);
@@ -1027,30 +991,27 @@ __global__ __launch_bounds__(64) void k_propagate_ghosts(Ctx c, const int32_t* _
 }
 // Ghost rows summed into the parent channel (AccumulateToParents128 with
 // coefficient 1, additive, all cell types; PoissonGrid.cu:664-719).
-__global__ __launch_bounds__(64) void k_accumulate_ghosts(Ctx c, const int32_t* __restrict__ list, int n, float* __restrict__ AX) {
-  int w = blockIdx.x;
-  if (w >= n) return;
-  const int t = list[w];
-  const DevTile T = c.tiles[t];
+__global__ __launch_bounds__(64) void k_accumulate_ghosts(Ctx c, const TileRec* __restrict__ list, int n, float* __restrict__ AX) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
   const int lane = lane_id();
-  const size_t base = size_t(t) * 512 + lane * 8;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
   __shared__ float gl[512];
   float v[8];
   load8(AX + base, v);
   const uint64_t ct = load_ct8(c.ctype + base);
   for (int k = 0; k < 8; k++) gl[lane * 8 + k] = (ct_at(ct, k) & (C_INT | C_DIR | C_NEU)) ? v[k] : 0.f;
-  __syncthreads();
+  wave_sync();
   float s = octet_sum(gl, lane);
-  size_t pc = parent_cell(T, lane);
-  AX[pc] = AX[pc] + s * 1.0f;
+  size_t pcell = parent_cell(T, lane);
+  AX[pcell] = AX[pcell] + s * 1.0f;
 }
 // Per-tile dot partials over all cells of the listed tiles, then one total.
-__global__ __launch_bounds__(64) void k_dot_tiles(Ctx c, const int32_t* __restrict__ list, int n, const float* __restrict__ A,
+__global__ __launch_bounds__(64) void k_dot_tiles(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ A,
                                                   const float* __restrict__ Bv, double* __restrict__ part) {
-  int w = blockIdx.x;
-  if (w >= n) return;
-  const int t = list[w];
-  const size_t base = size_t(t) * 512 + lane_id() * 8;
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const size_t base = size_t(T.self) * 512 + lane_id() * 8;
   float a[8], b[8];
   load8(A + base, a);
   load8(Bv + base, b);
@@ -1059,36 +1020,36 @@ __global__ __launch_bounds__(64) void k_dot_tiles(Ctx c, const int32_t* __restri
   s = wave_sum(s);
   if (lane_id() == 0) part[w] = s;
 }
-__global__ __launch_bounds__(64) void k_sum_partials(const double* __restrict__ part, int n, double* __restrict__ out) {
-  double s = sum_partials(part, 0, n);
-  if (lane_id() == 0) *out = s;
+__global__ __launch_bounds__(FIN_THREADS) void k_sum_partials(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double lds[FIN_CHUNK];
+  double s = block_sum_partials(part, 0, n, lds);
+  if (threadIdx.x == 0) *out = s;
 }
-
 // Copy whole tiles of a list from one channel to another.
-__global__ __launch_bounds__(64) void k_copy_tiles(const int32_t* __restrict__ list, int n, const float* __restrict__ src,
+__global__ __launch_bounds__(64) void k_copy_tiles(const TileRec* __restrict__ list, int n, const float* __restrict__ src,
                                                    float* __restrict__ dst) {
-  int w = blockIdx.x;
-  if (w >= n) return;
-  const size_t base = size_t(list[w]) * 512 + lane_id() * 8;
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const size_t base = size_t(T.self) * 512 + lane_id() * 8;
   float v[8];
   load8(src + base, v);
   store8(dst + base, v);
 }
 // Fill whole tiles of a list with a value.
-__global__ __launch_bounds__(64) void k_fill_tiles(const int32_t* __restrict__ list, int n, float* __restrict__ dst, float val) {
-  int w = blockIdx.x;
-  if (w >= n) return;
-  const size_t base = size_t(list[w]) * 512 + lane_id() * 8;
+__global__ __launch_bounds__(64) void k_fill_tiles(const TileRec* __restrict__ list, int n, float* __restrict__ dst, float val) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const size_t base = size_t(T.self) * 512 + lane_id() * 8;
   float v[8] = {val, val, val, val, val, val, val, val};
   store8(dst + base, v);
 }
 // dampedJacobiSolve pieces (GMGSolver.cu:479-525) on LEAF tiles:
 // R = B - R on INTERIOR cells; then X += omega R / D (X = 0 where D == 0).
-__global__ __launch_bounds__(64) void k_leaf_residual(Ctx c, const int32_t* __restrict__ list, int n,
+__global__ __launch_bounds__(64) void k_leaf_residual(Ctx c, const TileRec* __restrict__ list, int n,
                                                       const float* __restrict__ B, float* __restrict__ R) {
-  int w = blockIdx.x;
-  if (w >= n) return;
-  const size_t base = size_t(list[w]) * 512 + lane_id() * 8;
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const size_t base = size_t(T.self) * 512 + lane_id() * 8;
   float b[8], r[8];
   load8(B + base, b);
   load8(R + base, r);
@@ -1096,11 +1057,11 @@ __global__ __launch_bounds__(64) void k_leaf_residual(Ctx c, const int32_t* __re
   for (int k = 0; k < 8; k++) if (ct_at(ct, k) & C_INT) r[k] = b[k] - r[k];
   store8(R + base, r);
 }
-__global__ __launch_bounds__(64) void k_leaf_jacobi_update(Ctx c, const int32_t* __restrict__ list, int n, float* __restrict__ X,
+__global__ __launch_bounds__(64) void k_leaf_jacobi_update(Ctx c, const TileRec* __restrict__ list, int n, float* __restrict__ X,
                                                            const float* __restrict__ R, const float* __restrict__ D, float omega) {
-  int w = blockIdx.x;
-  if (w >= n) return;
-  const size_t base = size_t(list[w]) * 512 + lane_id() * 8;
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const size_t base = size_t(T.self) * 512 + lane_id() * 8;
   float x[8], r[8], d[8];
   load8(X + base, x);
   load8(R + base, r);

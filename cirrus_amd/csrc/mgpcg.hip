@@ -4,15 +4,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <map>
 #include <memory>
 #include <string>
-#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -56,7 +55,7 @@ cg_status guard(F f) {
 }
 
 struct DList {
-  int off = 0, n = 0;
+  int off = 0, n = 0;  // in TileRec units
 };
 
 }  // namespace
@@ -64,47 +63,39 @@ struct DList {
 using namespace cg;
 
 struct cg_problem {
-  int n = 0;               // tiles
+  int n = 0;  // tiles
   int max_level = 0;
   int n_leaf = 0, n_ghost = 0;
-  std::vector<int> ext_of;  // internal -> caller index
-  std::vector<int> int_of;  // caller -> internal index
-  std::vector<DevTile> tiles;
-  std::vector<int32_t> gchild;   // 8 ghost-child tile indices per row, then their 8 ghost slots
-  std::vector<int32_t> lists;  // host copy of all worklists
+  std::vector<int> ext_of;     // internal -> caller index
+  std::vector<TileRec> tiles;  // one record per tile (internal order)
+  std::vector<TileRec> lists;  // all worklists, concatenated
   DList leaf, gather, applyl, ghosts;
   std::vector<DList> act, lvl;
-  CoarsePlan coarse;             // levels 0..coarse.C resident in LDS (C = -1: none)
+  CoarsePlan coarse;  // levels 0..coarse.C resident in LDS (C = -1: none)
   std::vector<CoarseSlot> coarse_slots;
-  CoarseSlot* d_coarse = nullptr;
 
   // device
-  DevTile* d_tiles = nullptr;
-  int32_t* d_gchild = nullptr;
   uint8_t* d_ctype = nullptr;
   float* d_ch[CG_NUM_CHANNELS] = {};
   float* d_zb = nullptr;
   float* d_gsum = nullptr;
-  float* d_gext = nullptr;
   ProbState* d_ps = nullptr;
   double* d_part = nullptr;
   double* d_part2 = nullptr;
   double* d_scalar = nullptr;
-  int32_t* d_lists = nullptr;
-  int32_t* d_tmp_list = nullptr;
+  TileRec* d_lists = nullptr;
+  TileRec* d_tmp_list = nullptr;
   int tmp_list_cap = 0;
+  CoarseSlot* d_coarse = nullptr;
   hipStream_t stream = nullptr;
   int* h_pin = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipGraphExec_t graph = nullptr;
   std::string graph_key;
 
-  const int32_t* L(const DList& d) const { return d_lists + d.off; }
+  const TileRec* L(const DList& d) const { return d_lists + d.off; }
   Ctx ctx(float* z, float* r) const {
     Ctx c;
-    c.tiles = d_tiles;
-    c.gchild = d_gchild;
-    c.gcslot = d_gchild + gchild.size() / 2;
     c.ctype = d_ctype;
     c.x = d_ch[CG_CH_X];
     c.r = r;
@@ -114,7 +105,6 @@ struct cg_problem {
     c.rhsv = d_ch[CG_CH_TMP];
     c.zb = d_zb;
     c.gsum = d_gsum;
-    c.gext = d_gext;
     c.ps = d_ps;
     c.part = d_part;
     c.part2 = d_part2;
@@ -128,8 +118,7 @@ cg_problem::~cg_problem() {
   if (graph) (void)hipGraphExecDestroy(graph);
   for (auto e : ev) if (e) (void)hipEventDestroy(e);
   if (h_pin) (void)hipHostFree(h_pin);
-  void* ptrs[] = {d_tiles, d_gchild, d_ctype, d_zb, d_gsum, d_gext, d_ps, d_part, d_part2, d_scalar, d_lists, d_tmp_list,
-                  d_coarse};
+  void* ptrs[] = {d_ctype, d_zb, d_gsum, d_ps, d_part, d_part2, d_scalar, d_lists, d_tmp_list, d_coarse};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
   if (stream) (void)hipStreamDestroy(stream);
@@ -138,9 +127,22 @@ cg_problem::~cg_problem() {
 namespace {
 
 constexpr int kSwizzle = F_SWIZZLE;
-constexpr int kCoarseTiles = 64;
 
 inline unsigned grid_for(int n, int flags) { return (flags & F_SWIZZLE) ? unsigned((n + 7) & ~7) : unsigned(n); }
+
+// Base face coefficients of a tile at its own level's subtree view (the
+// device computes the same values the same way in stage_tile).
+void slot_coefficients(const TileRec& T, float bc[7]) {
+  const float ooh = 1.0f / T.h;
+  const float cll = ooh * ooh;
+  const float clg = ((2.0f * ooh) / 3.0f) * ooh;
+  const uint8_t to = subtree_type(T.type, T.level, T.level);
+  bc[6] = base_coeff(to, to, cll, clg);
+  for (int f = 0; f < 6; f++) {
+    const uint8_t tn = (T.nbr[f] < 0) ? to : subtree_type(T.nbr_type[f], T.level, T.level);
+    bc[f] = base_coeff(to, tn, cll, clg);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Problem construction
@@ -161,14 +163,13 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
     if (ty != LEAF && ty != NONLEAF && ty != GHOST) fail(CG_ERR_INVALID, "tile " + std::to_string(t) + ": bad tile type");
     mk[t] = morton3(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]));
   }
+  // canonical order: level, then Morton code of the tile coordinate
   P.ext_of.resize(n);
   for (int t = 0; t < n; t++) P.ext_of[t] = t;
   std::sort(P.ext_of.begin(), P.ext_of.end(), [&](int a, int b) {
     if (d->level[a] != d->level[b]) return d->level[a] < d->level[b];
     return mk[a] < mk[b];
   });
-  P.int_of.assign(n, -1);
-  for (int i = 0; i < n; i++) P.int_of[P.ext_of[i]] = i;
   std::unordered_map<uint64_t, int> idx;
   idx.reserve(size_t(n) * 2);
   std::vector<int> lev(n);
@@ -189,7 +190,6 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
   };
   P.max_level = 0;
   for (int i = 0; i < n; i++) P.max_level = std::max(P.max_level, lev[i]);
-  P.tiles.assign(n, DevTile());
   int nleaf = 0, nghost = 0;
   std::vector<int> leaf_slot(n, -1), gslot(n, -1);
   for (int i = 0; i < n; i++) {
@@ -199,9 +199,9 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
   P.n_leaf = nleaf;
   P.n_ghost = nghost;
   if (nleaf == 0) fail(CG_ERR_INVALID, "tree has no LEAF tiles");
-  std::vector<std::array<int32_t, 8>> gch;
+  P.tiles.assign(n, TileRec());
   for (int i = 0; i < n; i++) {
-    DevTile& T = P.tiles[i];
+    TileRec& T = P.tiles[i];
     std::memset(&T, 0, sizeof(T));
     const int l = lev[i];
     for (int f = 0; f < 6; f++) {
@@ -223,37 +223,25 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
     T.prob = 0;
     T.leaf_slot = leaf_slot[i];
     T.gslot = gslot[i];
-    T.gchild = -1;
+    T.self = i;
     T.h = float(d->h0 / double(1u << l));
     T.level = uint8_t(l);
     T.type = ty[i];
-    if (ty[i] == NONLEAF) {
+    for (int o = 0; o < 8; o++) { T.gch[o] = -1; T.gcs[o] = -1; }
+    if (ty[i] != GHOST) {
       for (int o = 0; o < 8; o++) {
         int ch = find(l + 1, 2 * cd[i][0] + (o & 1), 2 * cd[i][1] + ((o >> 1) & 1), 2 * cd[i][2] + ((o >> 2) & 1));
-        if (ch < 0 || ty[ch] == GHOST) fail(CG_ERR_INVALID, "a NONLEAF tile must have 8 LEAF/NONLEAF children");
-      }
-    }
-    if (ty[i] == LEAF) {
-      std::array<int32_t, 8> g;
-      bool any = false;
-      for (int o = 0; o < 8; o++) {
-        int ch = find(l + 1, 2 * cd[i][0] + (o & 1), 2 * cd[i][1] + ((o >> 1) & 1), 2 * cd[i][2] + ((o >> 2) & 1));
-        g[o] = (ch >= 0 && ty[ch] == GHOST) ? ch : -1;
-        if (ch >= 0 && ty[ch] != GHOST) fail(CG_ERR_INVALID, "a LEAF tile can only have GHOST children");
-        any |= g[o] >= 0;
-      }
-      if (any) {
-        T.gchild = int32_t(gch.size());
-        gch.push_back(g);
+        if (ty[i] == NONLEAF && (ch < 0 || ty[ch] == GHOST))
+          fail(CG_ERR_INVALID, "a NONLEAF tile must have 8 LEAF/NONLEAF children");
+        if (ty[i] == LEAF && ch >= 0) {
+          if (ty[ch] != GHOST) fail(CG_ERR_INVALID, "a LEAF tile can only have GHOST children");
+          T.gch[o] = ch;
+          T.gcs[o] = gslot[ch];
+          T.has_gchild = 1;
+        }
       }
     }
   }
-  P.gchild.assign(gch.size() * 16, -1);
-  for (size_t r = 0; r < gch.size(); r++)
-    for (int o = 0; o < 8; o++) {
-      P.gchild[r * 8 + o] = gch[r][o];
-      P.gchild[gch.size() * 8 + r * 8 + o] = gch[r][o] >= 0 ? gslot[gch[r][o]] : -1;
-    }
 
   // worklists (canonical order)
   auto& W = P.lists;
@@ -261,12 +249,12 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
   auto add_list = [&](auto pred) {
     DList dl;
     dl.off = int(W.size());
-    for (int i = 0; i < n; i++) if (pred(i)) W.push_back(i);
+    for (int i = 0; i < n; i++) if (pred(i)) W.push_back(P.tiles[i]);
     dl.n = int(W.size()) - dl.off;
     return dl;
   };
   P.leaf = add_list([&](int i) { return ty[i] == LEAF; });
-  P.gather = add_list([&](int i) { return ty[i] == LEAF && P.tiles[i].gchild >= 0; });
+  P.gather = add_list([&](int i) { return ty[i] == LEAF && P.tiles[i].has_gchild; });
   P.applyl = add_list([&](int i) { return ty[i] == LEAF || ty[i] == GHOST; });
   P.ghosts = add_list([&](int i) { return ty[i] == GHOST; });
   P.act.assign(P.max_level + 1, DList());
@@ -275,6 +263,7 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
     P.act[l] = add_list([&](int i) { return lev[i] == l && ty[i] != GHOST; });
     P.lvl[l] = add_list([&](int i) { return lev[i] == l; });
   }
+
   // Coarse levels kept in LDS by one workgroup: the deepest C whose levels
   // 0..C hold at most CL_MAX_TILES tiles in total (-1 if level 0 alone is larger).
   int C = -1, total = 0;
@@ -291,7 +280,7 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
     for (int l = 0; l <= C; l++) {
       P.coarse.lvl_begin[l] = int(P.coarse_slots.size());
       for (int q = 0; q < P.lvl[l].n; q++) {
-        int t = W[P.lvl[l].off + q];
+        int t = W[P.lvl[l].off + q].self;
         slot_of[t] = int(P.coarse_slots.size());
         CoarseSlot cs;
         std::memset(&cs, 0, sizeof(cs));
@@ -302,20 +291,20 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
     P.coarse.lvl_begin[C + 1] = int(P.coarse_slots.size());
     P.coarse.nslots = int(P.coarse_slots.size());
     for (auto& cs : P.coarse_slots) {
-      const DevTile& T = P.tiles[cs.tile];
+      const TileRec& T = P.tiles[cs.tile];
       for (int f = 0; f < 6; f++) cs.nbr[f] = T.nbr[f] < 0 ? -1 : slot_of[T.nbr[f]];
       cs.parent = T.parent < 0 ? -1 : slot_of[T.parent];
       cs.level = T.level;
       cs.h = T.h;
       cs.type = T.type;
       cs.octant = T.octant;
+      slot_coefficients(T, cs.bc);
     }
   }
+
   // device buffers
   HIPC(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
   const size_t cells = size_t(n) * 512;
-  HIPC(hipMalloc(&P.d_tiles, sizeof(DevTile) * n));
-  HIPC(hipMalloc(&P.d_gchild, sizeof(int32_t) * std::max<size_t>(16, P.gchild.size())));
   HIPC(hipMalloc(&P.d_ctype, cells));
   for (int c = 0; c < CG_NUM_CHANNELS; c++) {
     HIPC(hipMalloc(&P.d_ch[c], cells * sizeof(float)));
@@ -325,26 +314,21 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
   HIPC(hipMemsetAsync(P.d_zb, 0, cells * sizeof(float), P.stream));
   size_t g64 = size_t(std::max(1, nghost)) * 64 * sizeof(float);
   HIPC(hipMalloc(&P.d_gsum, g64));
-  HIPC(hipMalloc(&P.d_gext, g64));
   HIPC(hipMemsetAsync(P.d_gsum, 0, g64, P.stream));
-  HIPC(hipMemsetAsync(P.d_gext, 0, g64, P.stream));
   HIPC(hipMalloc(&P.d_ps, sizeof(ProbState)));
   HIPC(hipMemsetAsync(P.d_ps, 0, sizeof(ProbState), P.stream));
   size_t np = size_t(std::max(n, 64));
   HIPC(hipMalloc(&P.d_part, np * sizeof(double)));
   HIPC(hipMalloc(&P.d_part2, np * sizeof(double)));
   HIPC(hipMalloc(&P.d_scalar, 8 * sizeof(double)));
-  HIPC(hipMalloc(&P.d_lists, sizeof(int32_t) * W.size()));
+  HIPC(hipMalloc(&P.d_lists, sizeof(TileRec) * W.size()));
+  HIPC(hipMemcpyAsync(P.d_lists, W.data(), sizeof(TileRec) * W.size(), hipMemcpyHostToDevice, P.stream));
   if (!P.coarse_slots.empty()) {
     HIPC(hipMalloc(&P.d_coarse, sizeof(CoarseSlot) * P.coarse_slots.size()));
     HIPC(hipMemcpyAsync(P.d_coarse, P.coarse_slots.data(), sizeof(CoarseSlot) * P.coarse_slots.size(),
                         hipMemcpyHostToDevice, P.stream));
     P.coarse.slots = P.d_coarse;
   }
-  HIPC(hipMemcpyAsync(P.d_tiles, P.tiles.data(), sizeof(DevTile) * n, hipMemcpyHostToDevice, P.stream));
-  if (!P.gchild.empty())
-    HIPC(hipMemcpyAsync(P.d_gchild, P.gchild.data(), sizeof(int32_t) * P.gchild.size(), hipMemcpyHostToDevice, P.stream));
-  HIPC(hipMemcpyAsync(P.d_lists, W.data(), sizeof(int32_t) * W.size(), hipMemcpyHostToDevice, P.stream));
   std::vector<uint8_t> ct(cells);
   for (int i = 0; i < n; i++) std::memcpy(&ct[size_t(i) * 512], d->cell_type + size_t(P.ext_of[i]) * 512, 512);
   HIPC(hipMemcpyAsync(P.d_ctype, ct.data(), cells, hipMemcpyHostToDevice, P.stream));
@@ -358,7 +342,7 @@ void check_channel(int ch) {
   if (ch < 0 || ch >= CG_NUM_CHANNELS) fail(CG_ERR_INVALID, "channel index out of range");
 }
 
-// upload a ProbState for a single-problem run
+// upload the ProbState of a single-problem run (synchronous)
 void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann) {
   ProbState s;
   std::memset(&s, 0, sizeof(s));
@@ -386,12 +370,13 @@ void launch_gs(cg_problem& P, const Ctx& c, int l, float* Z, const float* R, int
   k_gs_color<<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
 }
 
-// One V-cycle Z <- M^-1 R (GMGSolver::VCycle, GMGSolver.cu:383-447): the zero
-// initial guess, rhs = R on LEAF tiles and zero ghost values of the reference's
-// init pass are implicit in the kernels' flags.
+// One V-cycle Z <- M^-1 R (GMGSolver::VCycle, GMGSolver.cu:383-447).  The
+// V-cycle rhs of LEAF tiles (rhsv) must hold R on entry.  The zero initial
+// guess and the zero ghost values of the reference's init pass are implicit in
+// the kernels' flags.
 void enqueue_vcycle(cg_problem& P, float* Z, float* R, const VParams& v) {
   const Ctx c = P.ctx(Z, R);
-  const int base = F_VCYCLE_RHS | kSwizzle | (v.check_done ? F_CHECK_DONE : 0);
+  const int base = F_VCYCLE | kSwizzle | (v.check_done ? F_CHECK_DONE : 0);
   const int L = P.max_level, C = P.coarse.C;  // levels > max(C, 0) use grid-wide launches
   float* ZB = P.d_zb;
   for (int l = L; l > std::max(C, 0); l--) {
@@ -411,12 +396,12 @@ void enqueue_vcycle(cg_problem& P, float* Z, float* R, const VParams& v) {
       }
     }
     const DList& al = P.lvl[l];
-    int fl = base | F_VCYCLE_OUT | F_GHOST_ZERO | (v.nu == 0 ? F_ZERO_X : 0);
+    int fl = base | F_GHOST_ZERO | (v.nu == 0 ? F_ZERO_X : 0);
     if (al.n) k_restrict<<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, Z, R, nullptr, v.ooa, fl);
   }
   if (C >= 0) {
     VCycleArgs va{v.nu, v.coarsest, v.smoother, v.omega, v.ooa, v.pcoef};
-    k_coarse_lds<<<1, CL_THREADS, 0, P.stream>>>(c, P.coarse, va, Z, R, base & ~F_SWIZZLE);
+    k_coarse_lds<<<1, CL_THREADS, 0, P.stream>>>(c, P.coarse, va, Z, base & ~F_SWIZZLE);
   } else {
     // level 0 too large for LDS: bottom sweeps as grid-wide launches
     const int it0 = v.coarsest / 2;
@@ -462,38 +447,45 @@ void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann) {
   const unsigned np = 1;
   if (pure_neumann) {
     k_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, fl);
-    k_finalize<<<np, 64, 0, P.stream>>>(c, FIN_MEAN, fl);
-    k_subtract_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, fl);
+    k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_MEAN, fl);
+    k_subtract_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, 1, fl);
   }
   enqueue_vcycle(P, c.z, c.r, v);
   k_cg_dot_rz<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
-  k_finalize<<<np, 64, 0, P.stream>>>(c, FIN_RZ, fl);
+  k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_RZ, fl);
   k_cg_update_p<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
   k_apply<<<grid_for(P.applyl.n, fl), 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, c.p, c.ap, -1, 0, 1, fl);
   if (P.gather.n) k_ghost_gather<<<grid_for(P.gather.n, fl), 64, 0, P.stream>>>(c, P.L(P.gather), P.gather.n, fl);
-  k_finalize<<<np, 64, 0, P.stream>>>(c, FIN_PAP, fl);
+  k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_PAP, fl);
   k_cg_update_xr<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
-  k_finalize<<<np, 64, 0, P.stream>>>(c, FIN_RR, fl);
+  k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_RR, fl);
 }
 
-void ensure_tmp_list(cg_problem& P, const std::vector<int32_t>& l) {
-  if (int(l.size()) > P.tmp_list_cap) {
+// A temporary worklist of the records of the given tiles.
+const TileRec* tmp_list(cg_problem& P, const std::vector<int>& tiles) {
+  if (int(tiles.size()) > P.tmp_list_cap) {
     if (P.d_tmp_list) HIPC(hipFree(P.d_tmp_list));
     P.d_tmp_list = nullptr;
-    P.tmp_list_cap = std::max<int>(int(l.size()), 64);
-    HIPC(hipMalloc(&P.d_tmp_list, sizeof(int32_t) * P.tmp_list_cap));
+    P.tmp_list_cap = std::max<int>(int(tiles.size()), 64);
+    HIPC(hipMalloc(&P.d_tmp_list, sizeof(TileRec) * P.tmp_list_cap));
   }
-  if (!l.empty()) HIPC(hipMemcpyAsync(P.d_tmp_list, l.data(), sizeof(int32_t) * l.size(), hipMemcpyHostToDevice, P.stream));
+  std::vector<TileRec> recs(tiles.size());
+  for (size_t q = 0; q < tiles.size(); q++) recs[q] = P.tiles[tiles[q]];
+  if (!recs.empty()) {
+    HIPC(hipStreamSynchronize(P.stream));
+    HIPC(hipMemcpy(P.d_tmp_list, recs.data(), sizeof(TileRec) * recs.size(), hipMemcpyHostToDevice));
+  }
+  return P.d_tmp_list;
 }
 
 double dot_types(cg_problem& P, const float* A, const float* B, int types) {
-  std::vector<int32_t> l;
+  std::vector<int> l;
   for (int i = 0; i < P.n; i++) if (P.tiles[i].type & types) l.push_back(i);
   if (l.empty()) return 0.0;
-  ensure_tmp_list(P, l);
+  const TileRec* dl = tmp_list(P, l);
   const Ctx c = P.ctx();
-  k_dot_tiles<<<unsigned(l.size()), 64, 0, P.stream>>>(c, P.d_tmp_list, int(l.size()), A, B, P.d_part);
-  k_sum_partials<<<1, 64, 0, P.stream>>>(P.d_part, int(l.size()), P.d_scalar);
+  k_dot_tiles<<<unsigned(l.size()), 64, 0, P.stream>>>(c, dl, int(l.size()), A, B, P.d_part);
+  k_sum_partials<<<1, FIN_THREADS, 0, P.stream>>>(P.d_part, int(l.size()), P.d_scalar);
   double out = 0;
   HIPC(hipMemcpyAsync(&out, P.d_scalar, sizeof(double), hipMemcpyDeviceToHost, P.stream));
   HIPC(hipStreamSynchronize(P.stream));
@@ -530,7 +522,7 @@ void solve(cg_problem& P, const cg_params& prm, cg_result& res) {
   set_prob_state(P, prm.rel_tol, prm.max_iters, prm.pure_neumann);
   const Ctx c = P.ctx();
   k_cg_init<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, kSwizzle);
-  k_finalize<<<1, 64, 0, P.stream>>>(c, FIN_BB, 0);
+  k_finalize<<<1, FIN_THREADS, 0, P.stream>>>(c, FIN_BB, 0);
   HIPC(hipGetLastError());
 
   char keybuf[160];
@@ -683,7 +675,7 @@ cg_status cg_solve(cg_problem* p, const cg_params* prm, cg_result* res) {
 
 cg_status cg_damped_jacobi_solve(cg_problem* p, int32_t max_iters, double rel_tol, float omega, cg_result* res) {
   return guard([&] {
-    if (!p || !res) fail(cg_status(CG_ERR_INVALID), "cg_damped_jacobi_solve: null argument");
+    if (!p || !res) fail(CG_ERR_INVALID, "cg_damped_jacobi_solve: null argument");
     auto t0 = std::chrono::steady_clock::now();
     cg_problem& P = *p;
     const Ctx c = P.ctx();
@@ -691,10 +683,10 @@ cg_status cg_damped_jacobi_solve(cg_problem* p, int32_t max_iters, double rel_to
     float* B = P.d_ch[CG_CH_B];
     float* AX = P.d_ch[CG_CH_P];
     float* D = P.d_ch[CG_CH_AP];
-    std::vector<int32_t> all(P.n);
+    std::vector<int> all(P.n);
     for (int i = 0; i < P.n; i++) all[i] = i;
-    ensure_tmp_list(P, all);
-    k_fill_tiles<<<P.n, 64, 0, P.stream>>>(P.d_tmp_list, P.n, X, 0.f);
+    const TileRec* dl = tmp_list(P, all);
+    k_fill_tiles<<<P.n, 64, 0, P.stream>>>(dl, P.n, X, 0.f);
     double bn = std::sqrt(dot_types(P, B, B, LEAF | NONLEAF | GHOST));
     double thr = std::max(rel_tol * bn, 2.2250738585072014e-308);
     int i = 0;
@@ -725,12 +717,15 @@ cg_status cg_vcycle(cg_problem* p, int32_t x_channel, int32_t f_channel, int32_t
     check_channel(f_channel);
     if (x_channel == f_channel || x_channel == CG_CH_TMP || f_channel == CG_CH_TMP)
       fail(CG_ERR_INVALID, "cg_vcycle: x and f must differ and may not be the scratch channel CG_CH_TMP");
+    if (smoother != CG_SMOOTH_RBGS && smoother != CG_SMOOTH_JACOBI) fail(CG_ERR_INVALID, "cg_vcycle: unknown smoother");
     VParams v;
     v.nu = level_iters;
     v.coarsest = coarsest_iters;
     v.smoother = smoother;
     v.omega = omega;
     v.check_done = 0;
+    // the V-cycle rhs of LEAF tiles starts as f (GMGSolver.cu:393-400)
+    k_copy_tiles<<<p->leaf.n, 64, 0, p->stream>>>(p->L(p->leaf), p->leaf.n, p->d_ch[f_channel], p->d_ch[CG_CH_TMP]);
     enqueue_vcycle(*p, p->d_ch[x_channel], p->d_ch[f_channel], v);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
@@ -754,16 +749,16 @@ cg_status cg_negative_laplacian_same_level(cg_problem* p, int32_t level, int32_t
     if (!p) fail(CG_ERR_INVALID, "null problem");
     check_channel(x_channel);
     check_channel(ax_channel);
-    std::vector<int32_t> l;
+    std::vector<int> l;
     for (int i = 0; i < p->n; i++) {
-      const DevTile& T = p->tiles[i];
+      const TileRec& T = p->tiles[i];
       if (level >= 0 && T.level != level) continue;
       if (subtree_type(T.type, T.level, subtree_level) & tile_types) l.push_back(i);
     }
     if (l.empty()) return;
-    ensure_tmp_list(*p, l);
+    const TileRec* dl = tmp_list(*p, l);
     const Ctx c = p->ctx();
-    k_apply<<<unsigned(l.size()), 64, 0, p->stream>>>(c, p->d_tmp_list, int(l.size()), p->d_ch[x_channel], p->d_ch[ax_channel],
+    k_apply<<<unsigned(l.size()), 64, 0, p->stream>>>(c, dl, int(l.size()), p->d_ch[x_channel], p->d_ch[ax_channel],
                                                        subtree_level, calc_diag, 0, 0);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
@@ -846,8 +841,8 @@ cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel) {
     set_prob_state(*p, 0.0, 0, 1);
     const Ctx c = p->ctx();
     k_mean<<<p->leaf.n, 64, 0, p->stream>>>(c, p->L(p->leaf), p->leaf.n, p->d_ch[channel], 0);
-    k_finalize<<<1, 64, 0, p->stream>>>(c, FIN_MEAN, 0);
-    k_subtract_mean<<<p->leaf.n, 64, 0, p->stream>>>(c, p->L(p->leaf), p->leaf.n, p->d_ch[channel], 0);
+    k_finalize<<<1, FIN_THREADS, 0, p->stream>>>(c, FIN_MEAN, 0);
+    k_subtract_mean<<<p->leaf.n, 64, 0, p->stream>>>(c, p->L(p->leaf), p->leaf.n, p->d_ch[channel], 0, 0);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
   });
@@ -862,9 +857,8 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
     if ((kernel == CG_KERNEL_RESTRICT || kernel == CG_KERNEL_PROLONG) && (level < 1 || level > P.max_level))
       fail(CG_ERR_INVALID, "level out of range");
     set_prob_state(P, 0.0, 1 << 30, 0);
-    HIPC(hipMemsetAsync(P.d_part, 0, sizeof(double) * std::max(P.n, 64), P.stream));
     const Ctx c = P.ctx();
-    const int base = F_VCYCLE_RHS | kSwizzle | F_CHECK_DONE;
+    const int base = F_VCYCLE | kSwizzle | F_CHECK_DONE;
     int64_t nt = 0;
     auto launch = [&]() {
       switch (kernel) {
@@ -885,7 +879,7 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
         case CG_KERNEL_RESTRICT: {
           const DList& al = P.lvl[level];
           nt = al.n;
-          int fl = base | F_VCYCLE_OUT | F_GHOST_ZERO;
+          int fl = base | F_GHOST_ZERO;
           k_restrict<<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, level, c.z, c.r, nullptr, 1.f, fl);
           break;
         }
@@ -895,19 +889,19 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
           k_prolong<<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, level, c.z, c.z, c.zb, c.zb, 1.f, base);
           break;
         }
-        case CG_KERNEL_EMPTY:
-          nt = level;
-          k_empty<<<unsigned(std::max(1, level)), 64, 0, P.stream>>>(nullptr);
+        case CG_KERNEL_UPDATE_XR:
+          nt = P.leaf.n;
+          k_cg_update_xr<<<grid_for(P.leaf.n, base), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, base);
           break;
         case CG_KERNEL_COARSE: {
           nt = P.coarse.nslots;
           VCycleArgs va{2, 10, 0, 2.f / 3.f, 1.f, 1.f};
-          if (P.coarse.C >= 0) k_coarse_lds<<<1, CL_THREADS, 0, P.stream>>>(c, P.coarse, va, c.z, c.r, base & ~F_SWIZZLE);
+          if (P.coarse.C >= 0) k_coarse_lds<<<1, CL_THREADS, 0, P.stream>>>(c, P.coarse, va, c.z, base & ~F_SWIZZLE);
           break;
         }
-        case CG_KERNEL_UPDATE_XR:
-          nt = P.leaf.n;
-          k_cg_update_xr<<<grid_for(P.leaf.n, base), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, base);
+        case CG_KERNEL_EMPTY:
+          nt = level;
+          k_empty<<<unsigned(std::max(1, level)), 64, 0, P.stream>>>(nullptr);
           break;
         default:
           fail(CG_ERR_INVALID, "unknown kernel id");
