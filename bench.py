@@ -13,6 +13,17 @@ red-black Gauss-Seidel, 2 pre + 2 post sweeps, 10 bottom iterations; solved to
 
 Prints one JSON line.  The CPU baseline is the oracle (the OpenMP CPU
 transcription) timed on this host's cores on a bounded sample of the same solve.
+
+HBM traffic of the dominant kernel comes from two hardware-counter runs of the
+kernels-only mode (one counter block each), scaled by what the same counter
+reads for a plain copy of known size:
+
+    rocprofv3 --pmc FETCH_SIZE -d OUT/fetch -o pmc --output-format csv -- python3 bench.py --kernels-only
+    rocprofv3 --pmc WRITE_SIZE -d OUT/write -o pmc --output-format csv -- python3 bench.py --kernels-only
+    python bench.py --summarize-pmc OUT/fetch/pmc_counter_collection.csv OUT/write/pmc_counter_collection.csv
+
+The last command writes profiles/k_gs_color_adaptive256_hbm_traffic.txt, whose
+traffic figure the benchmark line reports.
 """
 import argparse
 import json
@@ -35,6 +46,8 @@ WORKLOADS = {
                 "uniform 64^3, solid sphere r=0.2, FP32, damped Jacobi 2+2 (w=2/3), tol 1e-6"),
 }
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gs_color_adaptive256_hbm_traffic.txt")
+KERNEL_REPS = 20
 
 
 def build_if_missing():
@@ -69,6 +82,143 @@ def cpu_baseline(scene_kw, solver_kw, budget_s=20.0, max_solves=8):
     }
 
 
+def gs_alg_bytes(tiles):
+    """Algorithmic bytes of one red-black colour sweep: x and rhs read, x written
+    for half the cells, the cell type read (2s + t per cell, s = 4, t = 1)."""
+    return tiles * 512 * (2 * 4 + 1)
+
+
+def kernels_only(P, ca):
+    """The dominant kernel and the copy reference, KERNEL_REPS launches each
+    (the workload for hardware-counter runs)."""
+    L = P.max_level
+    us, tiles = P.benchmark_kernel(ca.KERNEL_GS_SWEEP, L, reps=KERNEL_REPS)
+    cus, ctiles = P.benchmark_kernel(ca.KERNEL_COPY, 0, reps=KERNEL_REPS)
+    print(json.dumps({"gs_level": L, "gs_tiles": tiles, "gs_us": us, "copy_tiles": ctiles, "copy_us": cus}), flush=True)
+
+
+def _pmc_per_dispatch(path, counter):
+    """Mean counter value per dispatch, by kernel name and grid size."""
+    import csv
+    import collections
+    acc = collections.defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r.get("Counter_Name") != counter:
+                continue
+            name = r["Kernel_Name"].split("(")[0]
+            grid = int(r.get("Grid_Size", r.get("Grid_Size_X", 0)) or 0)
+            acc[(name, grid)].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def summarize_pmc(fetch_csv, write_csv):
+    """Scale FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the dominant kernel by
+    the copy kernel's reading of its known traffic; write the profile summary."""
+    fetch = _pmc_per_dispatch(fetch_csv, "FETCH_SIZE")
+    write = _pmc_per_dispatch(write_csv, "WRITE_SIZE")
+
+    def pick(d, name):
+        ks = [k for k in d if k[0].endswith(name)]
+        if not ks:
+            raise SystemExit(f"no {name} dispatches in the counter file")
+        k = max(ks, key=lambda k: k[1])
+        return k, d[k]
+
+    (gk, gf), (_, gw) = pick(fetch, "k_gs_color"), pick(write, "k_gs_color")
+    (ck, cf), (_, cw) = pick(fetch, "k_copy_tiles"), pick(write, "k_copy_tiles")
+    copy_tiles = ck[1] // 64
+    copy_bytes = copy_tiles * 512 * 4
+    f_scale = copy_bytes / (cf * 1024.0)
+    w_scale = copy_bytes / (cw * 1024.0)
+    gs_tiles = gk[1] // 64
+    rd = gf * 1024.0 * f_scale
+    wr = gw * 1024.0 * w_scale
+    alg = gs_alg_bytes(gs_tiles)
+    lines = [
+        "HBM traffic of k_gs_color (red-black colour sweep, finest level) on the adaptive 256^3 workload",
+        "",
+        "commands:",
+        "  rocprofv3 --pmc FETCH_SIZE -d OUT/fetch -o pmc --output-format csv -- python3 bench.py --kernels-only",
+        "  rocprofv3 --pmc WRITE_SIZE -d OUT/write -o pmc --output-format csv -- python3 bench.py --kernels-only",
+        "  python bench.py --summarize-pmc OUT/fetch/pmc_counter_collection.csv OUT/write/pmc_counter_collection.csv",
+        "",
+        f"copy reference k_copy_tiles: {copy_tiles} tiles, {copy_bytes} B read and {copy_bytes} B written per dispatch",
+        f"  FETCH_SIZE {cf:.1f} KiB/dispatch -> read scale {f_scale:.4f}",
+        f"  WRITE_SIZE {cw:.1f} KiB/dispatch -> write scale {w_scale:.4f}",
+        f"k_gs_color: {gs_tiles} tiles (grid {gk[1]} threads)",
+        f"  FETCH_SIZE {gf:.1f} KiB/dispatch -> {rd:.0f} B read",
+        f"  WRITE_SIZE {gw:.1f} KiB/dispatch -> {wr:.0f} B written",
+        f"  algorithmic bytes per launch {alg} (2s+t per cell)",
+        f"  traffic / algorithmic = {(rd + wr) / alg:.3f}",
+        "",
+        f"traffic_bytes_per_launch = {rd + wr:.0f}",
+        f"traffic_tiles = {gs_tiles}",
+    ]
+    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
+    with open(TRAFFIC_FILE, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+STATS_FILE = os.path.join(ROOT, "profiles", "mgpcg_adaptive256_kernel_stats.txt")
+
+
+def summarize_trace(trace_csv, stats_csv):
+    """Per-kernel and per-(kernel, grid) durations of a --kernel-trace --stats run;
+    writes the profile summary."""
+    import csv
+    import collections
+    lines = [
+        "Kernel durations, adaptive 256^3 MGPCG workload (one MI355X)",
+        "",
+        "command:",
+        "  rocprofv3 --kernel-trace --stats -d OUT -o run --output-format csv -- "
+        "python3 bench.py --steps 5 --warmup 1 --no-cpu-baseline",
+        "  python bench.py --summarize-trace OUT/run_kernel_trace.csv OUT/run_kernel_stats.csv",
+        "",
+        "rocprofv3 --stats (all launches of each kernel):",
+        f"  {'kernel':24s} {'calls':>7s} {'avg us':>9s} {'total ms':>9s} {'%':>6s}",
+    ]
+    with open(stats_csv) as f:
+        for r in csv.DictReader(f):
+            name = r["Name"].split("(")[0].replace("cg::", "")
+            lines.append(f"  {name:24s} {int(r['Calls']):7d} {float(r['AverageNs']) / 1e3:9.3f} "
+                         f"{float(r['TotalDurationNs']) / 1e6:9.3f} {float(r['Percentage']):6.2f}")
+    acc = collections.defaultdict(list)
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"].split("(")[0].replace("cg::", "")
+            wg = int(r.get("Workgroup_Size_X", r.get("Workgroup_Size", 1)) or 1)
+            grid = int(r.get("Grid_Size_X", r.get("Grid_Size", 0)) or 0)
+            acc[(name, grid // max(1, wg))].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    lines += ["", "kernel trace by launch size (workgroups = tiles for the tile kernels):",
+              f"  {'kernel':24s} {'WGs':>6s} {'calls':>6s} {'mean us':>8s} {'median us':>9s}"]
+    for (name, wgs), v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+        v = sorted(v)
+        lines.append(f"  {name:24s} {wgs:6d} {len(v):6d} {sum(v) / len(v):8.3f} {v[len(v) // 2]:9.3f}")
+    os.makedirs(os.path.dirname(STATS_FILE), exist_ok=True)
+    with open(STATS_FILE, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def read_traffic(tiles):
+    """Traffic per launch from the committed counter summary, if it covers this launch."""
+    try:
+        kv = {}
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if " = " in line:
+                    k, v = line.split(" = ", 1)
+                    kv[k.strip()] = v.strip()
+        if int(kv["traffic_tiles"]) == tiles:
+            return float(kv["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,7 +227,17 @@ def main():
     ap.add_argument("--workload", default="adaptive256", choices=list(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="run only the dominant kernel and the copy reference (for counter runs)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
+    ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_CSV", "STATS_CSV"))
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
+    if args.summarize_trace:
+        summarize_trace(*args.summarize_trace)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every solve here runs on a single MI355X; "
               "run it with --gpus 1 (independent problems would be split across GPUs by a front end).",
@@ -93,6 +253,9 @@ def main():
     scene = ca.Scene(**scene_kw)
     t_build = time.perf_counter() - t0
     P = ca.Problem(scene.tree)
+    if args.kernels_only:
+        kernels_only(P, ca)
+        return
     S = ca.GMGSolver(1.0, 1.0)
 
     def step():
@@ -126,13 +289,15 @@ def main():
     # (x and rhs read, x written for half the cells; s=4 B, t=1 B)
     L = P.max_level
     us, tiles = P.benchmark_kernel(ca.KERNEL_GS_SWEEP, L, reps=200)
-    alg_bytes = tiles * 512 * (2 * 4 + 1)
+    alg_bytes = gs_alg_bytes(tiles)
+    traffic = read_traffic(tiles)
     achieved = alg_bytes / (us * 1e-6) / 1e9
     per_kernel = {}
     for name, kid, lvl in [("apply", ca.KERNEL_APPLY, 0), ("restrict_L", ca.KERNEL_RESTRICT, L),
                            ("prolong_L", ca.KERNEL_PROLONG, L), ("update_xr", ca.KERNEL_UPDATE_XR, 0),
                            ("gs_level1", ca.KERNEL_GS_SWEEP, 1), ("coarse_lds", ca.KERNEL_COARSE, 0),
-                           ("empty_1wg", ca.KERNEL_EMPTY, 1), ("empty_2240wg", ca.KERNEL_EMPTY, 2240)]:
+                           ("empty_1wg", ca.KERNEL_EMPTY, 1), ("empty_2240wg", ca.KERNEL_EMPTY, 2240),
+                           ("copy_all", ca.KERNEL_COPY, 0)]:
         u, nt = P.benchmark_kernel(kid, lvl, reps=100)
         per_kernel[name] = {"us": round(u, 3), "tiles": nt}
 
@@ -155,7 +320,7 @@ def main():
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "build_s": t_build,
         "roofline": {"bound": "hbm", "kernel": f"k_gs_color level {L} ({tiles} tiles)", "achieved": achieved,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "avg_launch_us": us, "alg_bytes_per_launch": alg_bytes},
         "kernels_us": per_kernel,
     }

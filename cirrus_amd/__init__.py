@@ -29,7 +29,7 @@ NUM_CHANNELS = 6
 SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
 SCENE_UNIFORM, SCENE_SPHERE_BAND, SCENE_PLANE, SCENE_VORTEX = 0, 1, 2, 3
 (KERNEL_GS_SWEEP, KERNEL_APPLY, KERNEL_RESTRICT, KERNEL_PROLONG, KERNEL_JACOBI, KERNEL_UPDATE_XR, KERNEL_COARSE,
- KERNEL_EMPTY) = range(8)
+ KERNEL_EMPTY, KERNEL_COPY) = range(9)
 
 EXPORTED_SYMBOLS = (
     "cg_params_default", "cg_last_error", "cg_device_count", "cg_problem_create", "cg_problem_destroy", "cg_problem_info",
@@ -341,7 +341,7 @@ class GMGSolver:
         is kept in ``self.last``."""
         p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
                            coarsest_iters=coarsest_iters, smoother=smoother, omega=omega,
-                           sync_stride=max(1, sync_stride), pure_neumann=int(is_pure_neumann),
+                           sync_stride=int(sync_stride), pure_neumann=int(is_pure_neumann),
                            one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff,
                            use_graph=int(use_graph))
         r = Result()

@@ -54,6 +54,7 @@ struct alignas(16) ProbState {
   int32_t iter, done, converged, max_iters;
   uint32_t counter;
   int32_t leaf_begin, leaf_count, pure_neumann;
+  int32_t check_stride;  // test convergence after iterations k*check_stride; <= 0: never
 };
 
 // Channel pointers passed to every kernel by value.
@@ -917,11 +918,12 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctx c, int mode, int f
       ps->pAp = s;
       ps->alpha = ps->gamma / s;
       break;
-    case FIN_RR: {  // GMGSolver.cu:611-619
+    case FIN_RR: {  // GMGSolver.cu:611-619 (checked every sync_stride iterations)
       ps->rr = s;
       int it = ps->iter + 1;
       ps->iter = it;
-      if (s < ps->thr) { ps->converged = 1; ps->done = 1; }
+      const int cs = ps->check_stride;
+      if (cs > 0 && it % cs == 0 && s < ps->thr) { ps->converged = 1; ps->done = 1; }
       else if (it >= ps->max_iters) ps->done = 1;
       break;
     }

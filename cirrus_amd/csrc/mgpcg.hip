@@ -69,7 +69,7 @@ struct cg_problem {
   std::vector<int> ext_of;     // internal -> caller index
   std::vector<TileRec> tiles;  // one record per tile (internal order)
   std::vector<TileRec> lists;  // all worklists, concatenated
-  DList leaf, gather, applyl, ghosts;
+  DList leaf, gather, applyl, ghosts, all;
   std::vector<DList> act, lvl;
   CoarsePlan coarse;  // levels 0..coarse.C resident in LDS (C = -1: none)
   std::vector<CoarseSlot> coarse_slots;
@@ -257,6 +257,7 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
   P.gather = add_list([&](int i) { return ty[i] == LEAF && P.tiles[i].has_gchild; });
   P.applyl = add_list([&](int i) { return ty[i] == LEAF || ty[i] == GHOST; });
   P.ghosts = add_list([&](int i) { return ty[i] == GHOST; });
+  P.all = add_list([&](int) { return true; });
   P.act.assign(P.max_level + 1, DList());
   P.lvl.assign(P.max_level + 1, DList());
   for (int l = 0; l <= P.max_level; l++) {
@@ -343,7 +344,7 @@ void check_channel(int ch) {
 }
 
 // upload the ProbState of a single-problem run (synchronous)
-void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann) {
+void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann, int check_stride = 1) {
   ProbState s;
   std::memset(&s, 0, sizeof(s));
   s.tol2 = tol * tol;
@@ -351,6 +352,7 @@ void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann) 
   s.leaf_begin = 0;
   s.leaf_count = P.n_leaf;
   s.pure_neumann = pure_neumann;
+  s.check_stride = check_stride;
   HIPC(hipStreamSynchronize(P.stream));
   HIPC(hipMemcpy(P.d_ps, &s, sizeof(s), hipMemcpyHostToDevice));
 }
@@ -519,7 +521,7 @@ void solve(cg_problem& P, const cg_params& prm, cg_result& res) {
   v.pcoef = prm.prolong_coeff;
   const int stride = std::max(1, prm.sync_stride);
   auto t0 = std::chrono::steady_clock::now();
-  set_prob_state(P, prm.rel_tol, prm.max_iters, prm.pure_neumann);
+  set_prob_state(P, prm.rel_tol, prm.max_iters, prm.pure_neumann, prm.sync_stride);
   const Ctx c = P.ctx();
   k_cg_init<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, kSwizzle);
   k_finalize<<<1, FIN_THREADS, 0, P.stream>>>(c, FIN_BB, 0);
@@ -572,7 +574,9 @@ void solve(cg_problem& P, const cg_params& prm, cg_result& res) {
   res.converged = s.converged;
   res.bb = s.bb;
   res.rr = s.rr;
-  res.rel_res = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
+  // GMGSolver.cu:553-556, 655-665: (0, 0) for b = 0; relative residual -1
+  // when convergence is never tested (sync_stride <= 0)
+  res.rel_res = s.bb > 0 ? (prm.sync_stride > 0 ? std::sqrt(s.rr / s.bb) : -1.0) : 0.0;
   res.seconds = std::chrono::duration<double>(t1 - t0).count();
 }
 
@@ -683,10 +687,7 @@ cg_status cg_damped_jacobi_solve(cg_problem* p, int32_t max_iters, double rel_to
     float* B = P.d_ch[CG_CH_B];
     float* AX = P.d_ch[CG_CH_P];
     float* D = P.d_ch[CG_CH_AP];
-    std::vector<int> all(P.n);
-    for (int i = 0; i < P.n; i++) all[i] = i;
-    const TileRec* dl = tmp_list(P, all);
-    k_fill_tiles<<<P.n, 64, 0, P.stream>>>(dl, P.n, X, 0.f);
+    k_fill_tiles<<<P.n, 64, 0, P.stream>>>(P.L(P.all), P.n, X, 0.f);
     double bn = std::sqrt(dot_types(P, B, B, LEAF | NONLEAF | GHOST));
     double thr = std::max(rel_tol * bn, 2.2250738585072014e-308);
     int i = 0;
@@ -899,6 +900,10 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
           if (P.coarse.C >= 0) k_coarse_lds<<<1, CL_THREADS, 0, P.stream>>>(c, P.coarse, va, c.z, base & ~F_SWIZZLE);
           break;
         }
+        case CG_KERNEL_COPY:
+          nt = P.n;
+          k_copy_tiles<<<P.n, 64, 0, P.stream>>>(P.L(P.all), P.n, c.x, c.ap);
+          break;
         case CG_KERNEL_EMPTY:
           nt = level;
           k_empty<<<unsigned(std::max(1, level)), 64, 0, P.stream>>>(nullptr);

@@ -68,7 +68,8 @@ typedef struct {
   int32_t coarsest_iters;   /* red-black GS iterations on level 0 (10) */
   int32_t smoother;         /* CG_SMOOTH_RBGS (reference) or CG_SMOOTH_JACOBI */
   float omega;              /* damped-Jacobi weight (2/3) */
-  int32_t sync_stride;      /* host polls the device convergence flag every this many iterations (1) */
+  int32_t sync_stride;      /* the residual is tested after every sync_stride-th iteration (1); <= 0:
+                               never, run max_iters and report rel_res = -1 (src/GMGSolver.cu:611,655-665) */
   int32_t pure_neumann;     /* re-centre r before each V-cycle (src/GMGSolver.cu:556,621) */
   float one_over_alpha;     /* restriction scale (1) */
   float prolong_coeff;      /* prolongation scale (1) */
@@ -156,9 +157,12 @@ cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel);
  * CG_KERNEL_PROLONG (from/to `level`), CG_KERNEL_JACOBI (one sweep of `level`),
  * CG_KERNEL_UPDATE_XR (x/r update and |r|^2), CG_KERNEL_COARSE (the LDS-resident
  * coarse levels of a V-cycle, RB-GS 2+2), CG_KERNEL_EMPTY (an empty kernel of
- * `level` workgroups: the launch floor).  Channels are overwritten. */
+ * `level` workgroups: the launch floor), CG_KERNEL_COPY (copy of one channel
+ * over all tiles: 4096 bytes moved per tile, the reference point that scales
+ * hardware byte counters).  Channels are overwritten. */
 enum { CG_KERNEL_GS_SWEEP = 0, CG_KERNEL_APPLY = 1, CG_KERNEL_RESTRICT = 2, CG_KERNEL_PROLONG = 3,
-       CG_KERNEL_JACOBI = 4, CG_KERNEL_UPDATE_XR = 5, CG_KERNEL_COARSE = 6, CG_KERNEL_EMPTY = 7 };
+       CG_KERNEL_JACOBI = 4, CG_KERNEL_UPDATE_XR = 5, CG_KERNEL_COARSE = 6, CG_KERNEL_EMPTY = 7,
+       CG_KERNEL_COPY = 8 };
 cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int32_t reps, double* avg_us,
                               int64_t* tiles);
 

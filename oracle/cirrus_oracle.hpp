@@ -15,13 +15,45 @@
 // Every function cites the reference file:line it restates.  Two deliberate
 // deviations, both shared with the GPU path and stated where they occur:
 //   * dot products use one fixed, deterministic FP64 summation order
-//     (det_sum below) instead of cub's (cub's order is not pinned by anything
-//     in the reference, so that choice stays "parity unpinned");
+//     (tile_partial / sum_partials below) instead of cub's (cub's order is not
+//     pinned by anything in the reference, so that choice stays "parity
+//     unpinned");
 //   * an INTERIOR cell whose diagonal D is 0 gets no update (the reference
 //     divides by zero there: GMGSolver.cu:244-265).
 // FP contraction is off (-ffp-contract=off); where the reference's compiler
 // would fuse a multiply-add (x += a*b), std::fma is written explicitly so both
 // this file and the GPU kernels round identically.
+//
+// How each part is pinned (tests/test_oracle_pins.py, tests/test_abi_cpu.py),
+// since the reference ships no tests or golden vectors and cannot be built here:
+//   grid build (IterativeRefine, RefineLeafsOneStep, SpawnGhostTiles,
+//     CalculateNeighborTiles)      2:1 face balance, ghost-placement rule and
+//                                  neighbour symmetry checked tile by tile;
+//                                  closed-form uniform counts
+//   SceneCellType                  brute-force count of solid / wall cells
+//   CalcCellTypesFromLeafs         dense 2x2x2 block reduction per level;
+//                                  ghost cells equal their parent cell
+//   SceneRHS                       divergence theorem (sum of b = net wall flux)
+//   operator (NegLapCoeff, LoadLapTile, NegLapSameLevel, FullNegativeLaplacian)
+//                                  an independent leaf-partition finite-volume
+//                                  assembly; symmetry and positivity; exactness
+//                                  on linear fields across coarse/fine faces;
+//                                  dense closed form on uniform grids; A 1 = 0
+//                                  without Dirichlet cells
+//   GSColor, JacobiSweep           textbook updates with the independent matrix
+//   ResidualAndRestrict, ProlongateAndUpdate
+//                                  R = P^T; a unit fine value lands in its parent cell
+//   VCycle                         symmetric positive preconditioner
+//   solve (T=double)               residual of the independent system, a sparse
+//                                  direct solve on 32^3, the pure-Neumann form
+//   Dot                            value vs an exactly rounded sum
+//   ReCenterLeafVoxels             interior mean removed, other cells untouched
+//   dampedJacobiSolve              closed-form first iterate, residual decreases
+// Parity unpinned: the exact FP32 bit patterns of the reference's own GPU run
+// (they depend on its compiler's contraction choices and on cub's reduction
+// order), hence its exact iteration counts; the hash-table slot order
+// (hashCoord / tileIdx) is internal and reaches no output, since every export
+// uses the canonical (level, Morton) tile order.
 // ============================================================================
 #pragma once
 #include <algorithm>
@@ -1091,8 +1123,12 @@ struct Solver {
   struct Result { int iters; double rel_res; double bb; double rr; };
 
   // GMGSolver::solve (GMGSolver.cu:531-666)
+  // sync_stride: the residual is tested after iterations k*sync_stride
+  // (GMGSolver.cu:611); <= 0 never tests and reports a relative residual of -1
+  // (GMGSolver.cu:655-665).  b = 0 returns (0, 0) in every mode (with
+  // sync_stride <= 0 the reference would go on and divide 0 by 0).
   Result solve(int max_iters, double tol, int level_iters, int coarsest_iters, bool pure_neumann,
-               int smoother = SMOOTH_RBGS, T omega = T(2) / T(3)) {
+               int smoother = SMOOTH_RBGS, T omega = T(2) / T(3), int sync_stride = 1) {
     ForLeafCells([](TileData<T>& t, int c) { t.ch[CH_X][c] = T(0); });
     double bb = Dot(CH_R, CH_R, LEAF);
     if (bb == 0.0) return {0, 0.0, 0.0, 0.0};
@@ -1112,8 +1148,10 @@ struct Solver {
           t.ch[CH_R][c] = std::fma(-alpha, t.ch[CH_AP][c], t.ch[CH_R][c]);
         }
       });
-      rr = Dot(CH_R, CH_R, LEAF);
-      if (rr < thr) return {i + 1, std::sqrt(rr / bb), bb, rr};
+      if (sync_stride > 0 && (i + 1) % sync_stride == 0) {
+        rr = Dot(CH_R, CH_R, LEAF);
+        if (rr < thr) return {i + 1, std::sqrt(rr / bb), bb, rr};
+      }
       if (pure_neumann) ReCenterLeafVoxels(CH_R);
       VCycle(CH_Z, CH_R, CH_AP, level_iters, coarsest_iters, smoother, omega);
       double gamma_old = gamma;
@@ -1124,7 +1162,7 @@ struct Solver {
       });
     }
     rr = Dot(CH_R, CH_R, LEAF);
-    return {max_iters, std::sqrt(rr / bb), bb, rr};
+    return {max_iters, sync_stride > 0 ? std::sqrt(rr / bb) : -1.0, bb, rr};
   }
 
   // GMGSolver::dampedJacobiSolve (GMGSolver.cu:449-529)

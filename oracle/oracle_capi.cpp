@@ -196,12 +196,12 @@ void orc_recenter(void* h, int ch) {
 // GMGSolver::solve.  out[0]=relative residual (recursive r), out[1]=|b|^2,
 // out[2]=|r|^2, out[3]=wall seconds of the solve.  Returns the iteration count.
 int orc_solve(void* h, int max_iters, double tol, int level_iters, int coarsest_iters, int pure_neumann,
-              int smoother, double omega, double* out) {
+              int smoother, double omega, int sync_stride, double* out) {
   return with(h, [&](auto& P) {
     using T = std::remove_reference_t<decltype(P.g().h0)>;
     Solver<T> S(P);
     auto t0 = std::chrono::steady_clock::now();
-    auto r = S.solve(max_iters, tol, level_iters, coarsest_iters, pure_neumann != 0, smoother, T(omega));
+    auto r = S.solve(max_iters, tol, level_iters, coarsest_iters, pure_neumann != 0, smoother, T(omega), sync_stride);
     auto t1 = std::chrono::steady_clock::now();
     if (out) { out[0] = r.rel_res; out[1] = r.bb; out[2] = r.rr; out[3] = std::chrono::duration<double>(t1 - t0).count(); }
     return r.iters;

@@ -110,7 +110,7 @@ def lib():
         L.orc_dot.argtypes = [vp, i, i, i]
         L.orc_recenter.argtypes = [vp, i]
         L.orc_solve.restype = i
-        L.orc_solve.argtypes = [vp, i, d, i, i, i, i, d, vp]
+        L.orc_solve.argtypes = [vp, i, d, i, i, i, i, d, i, vp]
         L.orc_damped_jacobi_solve.restype = i
         L.orc_damped_jacobi_solve.argtypes = [vp, i, d, d, vp]
         L.orc_set_threads.argtypes = [i]
@@ -206,10 +206,10 @@ class Oracle:
         lib().orc_recenter(self.h, ch)
 
     def solve(self, max_iters=1000, tol=1e-6, level_iters=1, coarsest_iters=10, pure_neumann=False,
-              smoother=SMOOTH_RBGS, omega=2.0 / 3.0):
+              smoother=SMOOTH_RBGS, omega=2.0 / 3.0, sync_stride=1):
         out = np.zeros(4, np.float64)
         it = lib().orc_solve(self.h, max_iters, tol, level_iters, coarsest_iters, int(pure_neumann),
-                             smoother, omega, _ptr(out))
+                             smoother, omega, int(sync_stride), _ptr(out))
         return {"iters": it, "rel_res": out[0], "bb": out[1], "rr": out[2], "seconds": out[3]}
 
     def damped_jacobi_solve(self, max_iters, tol, omega):

@@ -1,0 +1,155 @@
+"""CPU checks of the C ABI: the library loads and exports every function the
+header declares, the host-side builders (synthetic scenes, cell typing) equal
+the oracle bit for bit, and malformed trees are rejected before any device
+work.  Nothing here launches a kernel."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import cirrus_amd as ca
+import pyoracle as po
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cirrus_mgpcg.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cg_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_the_mirrored_symbols():
+    assert header_functions() == sorted(ca.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_every_header_function():
+    out = subprocess.run(["nm", "-D", "--defined-only", ca.LIB_PATH], check=True, capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [f for f in header_functions() if f not in exported]
+    assert not missing, missing
+    lib = ca.lib()  # loads with every signature bound
+    for f in header_functions():
+        assert getattr(lib, f) is not None
+
+
+def test_params_default_matches_the_reference_defaults():
+    p = ca.default_params()
+    # GMGSolver::solve as called by FluidEuler::project (src/FluidEuler.h:598)
+    assert (p.max_iters, p.level_iters, p.coarsest_iters, p.sync_stride) == (1000, 1, 10, 1)
+    assert p.rel_tol == 1e-6 and p.one_over_alpha == 1.0 and p.prolong_coeff == 1.0
+
+
+SCENES = {
+    "uniform32": dict(uniform_level=2),
+    "plain64": dict(),
+    "plane": dict(kind=po.SCENE_PLANE, plane_level=[3, 2]),
+    "band": dict(kind=po.SCENE_SPHERE_BAND, solid_r=0.15, band_level=[4, 3, 2]),
+    "vortex2root": dict(kind=po.SCENE_VORTEX, nroot=[1, 1, 2], vortex_level=[4, 2], seed=3),
+    "mixed_bc": dict(kind=po.SCENE_SPHERE_BAND, solid_r=0.2, band_level=[4, 3, 2], seed=11,
+                     face_bc=[po.DIRICHLET, po.NEUMANN, po.NEUMANN, po.NEUMANN, po.DIRICHLET, po.NEUMANN]),
+}
+
+
+def _by_key(level, coord):
+    return {(int(l), *map(int, c)): i for i, (l, c) in enumerate(zip(level, coord))}
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_scene_builder_matches_oracle(name):
+    """The library's host scene builder (refinement with 2:1 balance, ghosts, BC
+    typing, velocities, divergence) reproduces the oracle's scene exactly."""
+    o = po.Oracle(**SCENES[name])
+    lev, crd, typ, _ = o.topology()
+    ct = o.celltypes()
+    b = o.get(po.CH_B).astype(np.float32)
+    uvw = np.stack([o.get(po.CH_U), o.get(po.CH_V), o.get(po.CH_W)], axis=1).astype(np.float32)
+    S = ca.Scene(**SCENES[name])
+    T = S.tree
+    assert T.n == o.n
+    ko = _by_key(lev, crd)
+    ks = _by_key(T.level, T.coord)
+    assert ko.keys() == ks.keys()
+    perm = np.array([ks[k] for k in ko])  # oracle order -> scene order
+    idx = np.array(list(ko.values()))
+    assert np.array_equal(T.type[perm], typ[idx])
+    assert np.array_equal(T.cell_type[perm], ct[idx])
+    assert np.array_equal(S.rhs[perm].view(np.uint32), b[idx].view(np.uint32))
+    assert np.array_equal(S.uvw[perm].view(np.uint32), uvw[idx].view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["band", "vortex2root", "mixed_bc"])
+def test_calc_cell_types_from_leafs_matches_oracle(name):
+    o = po.Oracle(**SCENES[name])
+    lev, crd, typ, _ = o.topology()
+    ct = o.celltypes()
+    scrubbed = ct.copy()
+    scrubbed[typ != po.LEAF] = 0
+    tree = ca.Tree(o.params.h0, lev, crd, typ, scrubbed)
+    got = ca.calc_cell_types_from_leafs(tree)
+    assert np.array_equal(got, ct)
+
+
+def _small_tree():
+    o = po.Oracle(kind=po.SCENE_PLANE, plane_level=[2, 1])
+    lev, crd, typ, _ = o.topology()
+    return o.params.h0, lev, crd, typ, o.celltypes()
+
+
+def _create_status(h0, lev, crd, typ, ct):
+    import ctypes
+    tree = ca.Tree(h0, lev, crd, typ, ct)
+    d = tree.desc()
+    h = ctypes.c_void_p()
+    st = ca.lib().cg_problem_create(ctypes.byref(d), ctypes.byref(h))
+    if st == 0:
+        ca.lib().cg_problem_destroy(h)
+    return st, ca.lib().cg_last_error().decode()
+
+
+CG_ERR_INVALID = 1
+
+
+def test_problem_create_rejects_a_duplicate_tile():
+    h0, lev, crd, typ, ct = _small_tree()
+    lev2, crd2 = np.append(lev, lev[-1]), np.vstack([crd, crd[-1]])
+    st, msg = _create_status(h0, lev2, crd2, np.append(typ, typ[-1]), np.vstack([ct, ct[-1]]))
+    assert st == CG_ERR_INVALID and "duplicate" in msg
+
+
+def test_problem_create_rejects_a_missing_parent():
+    h0, lev, crd, typ, ct = _small_tree()
+    keep = lev != 0
+    st, msg = _create_status(h0, lev[keep], crd[keep], typ[keep], ct[keep])
+    assert st == CG_ERR_INVALID and "parent" in msg
+
+
+def test_problem_create_rejects_a_nonleaf_without_children():
+    h0, lev, crd, typ, ct = _small_tree()
+    deepest = lev == lev.max()
+    keep = ~deepest
+    typ2 = typ[keep].copy()
+    st, msg = _create_status(h0, lev[keep], crd[keep], typ2, ct[keep])
+    assert st == CG_ERR_INVALID and "children" in msg
+
+
+def test_problem_create_rejects_bad_types_and_empty_trees():
+    h0, lev, crd, typ, ct = _small_tree()
+    typ2 = typ.copy()
+    typ2[0] = 3
+    st, msg = _create_status(h0, lev, crd, typ2, ct)
+    assert st == CG_ERR_INVALID and "type" in msg
+    st, msg = _create_status(h0, lev[:0], crd[:0], typ[:0], ct[:0])
+    assert st == CG_ERR_INVALID
+    st, msg = _create_status(-1.0, lev, crd, typ, ct)
+    assert st == CG_ERR_INVALID and "h0" in msg
+
+
+def test_scene_create_rejects_invalid_parameters():
+    with pytest.raises(ca.CirrusError):
+        ca.Scene(nroot=[0, 1, 1])
+    with pytest.raises(ca.CirrusError):
+        ca.Scene(h0=0.0)

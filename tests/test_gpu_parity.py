@@ -155,15 +155,17 @@ def test_dot_bitexact(name):
         assert P.dot(ca.CH_P, ca.CH_Z, types) == o.dot(po.CH_P, po.CH_Z, types)
 
 
-def _solve_both(name, smoother, nu, use_graph=True, sync_stride=1, pure=False, scene=None):
+def _solve_both(name, smoother, nu, use_graph=True, sync_stride=1, pure=False, scene=None, max_iters=200):
     if scene is None:
         o, P, typ, nbr, b = pair(name)
     else:
         o, P, typ, nbr, b = scene
     both_set(o, P, po.CH_B, b)
-    ro = o.solve(max_iters=200, tol=1e-6, level_iters=nu, coarsest_iters=10, pure_neumann=pure, smoother=smoother)
+    ro = o.solve(max_iters=max_iters, tol=1e-6, level_iters=nu, coarsest_iters=10, pure_neumann=pure, smoother=smoother,
+                 sync_stride=sync_stride)
     S = ca.GMGSolver(1.0, 1.0)
-    it, rel = S.solve(P, False, 200, 1e-6, nu, 10, sync_stride, pure, smoother=smoother, use_graph=use_graph)
+    it, rel = S.solve(P, False, max_iters, 1e-6, nu, 10, sync_stride, pure, smoother=smoother, use_graph=use_graph)
+    assert (it, rel) == (S.last["iters"], S.last["rel_res"])
     S.last["b"] = b
     return o, P, typ, ro, S.last
 
@@ -180,10 +182,29 @@ def test_solve_bitexact(name, smoother, nu):
 
 
 @pytest.mark.parametrize("use_graph,stride", [(False, 1), (True, 4), (False, 3)])
-def test_solve_launch_modes_identical(use_graph, stride):
+def test_solve_launch_modes_and_sync_stride(use_graph, stride):
+    """Graph replay or direct launches; the residual tested every `stride`
+    iterations, as GMGSolver::solve does (the count is a multiple of it)."""
     o, P, typ, ro, rd = _solve_both("band", ca.SMOOTH_RBGS, 2, use_graph=use_graph, sync_stride=stride)
-    assert rd["iters"] == ro["iters"] and rd["rr"] == ro["rr"]
+    assert rd["iters"] == ro["iters"] and rd["rr"] == ro["rr"] and rd["rel_res"] == ro["rel_res"]
+    assert rd["iters"] % stride == 0
     assert_same(P.get(ca.CH_X), o.get(po.CH_X), "pressure", mask=typ == po.LEAF)
+
+
+def test_solve_without_convergence_test():
+    """sync_stride <= 0: exactly max_iters iterations, relative residual -1."""
+    o, P, typ, ro, rd = _solve_both("plane", ca.SMOOTH_RBGS, 1, sync_stride=0, max_iters=7)
+    assert rd["iters"] == ro["iters"] == 7 and rd["rel_res"] == ro["rel_res"] == -1.0
+    assert rd["rr"] == ro["rr"]
+    assert_same(P.get(ca.CH_X), o.get(po.CH_X), "pressure", mask=typ == po.LEAF)
+
+
+def test_zero_rhs_returns_immediately():
+    o, P, typ, nbr, b = pair("plane")
+    P.set(ca.CH_B, np.zeros_like(b))
+    S = ca.GMGSolver()
+    assert S.solve(P, False, 100, 1e-6, 1, 10, 1, False) == (0, 0.0)
+    assert not P.get(ca.CH_X)[typ == po.LEAF].any()
 
 
 def test_solve_repeatable():

@@ -214,9 +214,27 @@ def test_vcycle_symmetric_positive(smoother, nu):
 
 
 @pytest.mark.parametrize("name", ["band", "vortex", "plane"])
-def test_fp64_mgpcg_matches_direct_solve(name):
+def test_fp64_mgpcg_solves_independent_system(name):
+    """The FP64 MGPCG pressure satisfies the independently assembled
+    finite-volume system: ||b - A_fv x|| / ||b|| at the solver's tolerance."""
     o, lev, crd, typ, nbr, ct, lc = scene(name)
     o32 = po.Oracle(fp64=False, **SCENES[name])
+    b = o32.get(po.CH_B)
+    o.set(po.CH_B, b)
+    r = o.solve(max_iters=300, tol=1e-12, level_iters=2, coarsest_iters=10)
+    assert r["rel_res"] < 1e-12
+    x = lc.from_tiles(o.get(po.CH_X))
+    m = lc.ct == po.INTERIOR
+    A = lc.matrix()[m][:, m]
+    bl = lc.from_tiles(b)[m]
+    assert np.linalg.norm(bl - A @ x[m]) / np.linalg.norm(bl) < 1e-10
+
+
+def test_fp64_mgpcg_matches_direct_solve():
+    """On the 32^3 uniform scene the FP64 MGPCG pressure equals a sparse direct
+    solve of the independently assembled system."""
+    o, lev, crd, typ, nbr, ct, lc = scene("uniform32")
+    o32 = po.Oracle(fp64=False, **SCENES["uniform32"])
     b = o32.get(po.CH_B)
     o.set(po.CH_B, b)
     r = o.solve(max_iters=300, tol=1e-13, level_iters=2, coarsest_iters=10)
@@ -336,3 +354,140 @@ def test_rhs_sum_equals_boundary_flux():
         cm, cp, uf = interior[tuple(lo)], interior[tuple(hi)], ua[tuple(hi)]
         flux += np.sum(uf[cm & ~cp]) - np.sum(uf[~cm & cp])
     assert abs(B[interior].sum() - h * h * flux) < 1e-9 * max(1.0, h * h * np.abs(U).sum())
+
+
+def test_jacobi_textbook_update():
+    """One damped-Jacobi sweep on the finest level = x + w (b - A x) / A_cc on
+    INTERIOR cells, with A assembled independently."""
+    o, lev, crd, typ, nbr, ct, lc = scene("uniform32")
+    A = lc.matrix()
+    rng = np.random.default_rng(8)
+    x, b = leaf_interior_vector(lc, rng), rng.standard_normal(lc.n)
+    o.set(po.CH_Z, lc.to_tiles(x, o.n))
+    o.set(po.CH_TMP, lc.to_tiles(b, o.n))
+    w = 2.0 / 3.0
+    o.jacobi_sweep(o.max_level, po.CH_Z, po.CH_TMP, w)
+    got = lc.from_tiles(o.get(po.CH_Z))
+    m = lc.ct == po.INTERIOR
+    ref = x.copy()
+    ref[m] = x[m] + w * (b[m] - (A @ x)[m]) / A.diagonal()[m]
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-14)
+
+
+def _reduce_types(fine):
+    """2x2x2 block reduction of a dense cell-type array: DIRICHLET if any child
+    is, else INTERIOR if any child is, else NEUMANN."""
+    n = fine.shape[0] // 2
+    blk = fine.reshape(n, 2, n, 2, n, 2).transpose(0, 2, 4, 1, 3, 5).reshape(n, n, n, 8)
+    anyd = (blk == po.DIRICHLET).any(axis=-1)
+    anyi = (blk == po.INTERIOR).any(axis=-1)
+    return np.where(anyd, po.DIRICHLET, np.where(anyi, po.INTERIOR, po.NEUMANN)).astype(np.uint8)
+
+
+def test_nonleaf_cell_types_are_dense_block_reductions():
+    """On the uniform 64^3 scene every NONLEAF level equals the 2x2x2 block
+    reduction of the level below, assembled as dense arrays."""
+    o = po.Oracle()
+    lev, crd, typ, nbr = o.topology()
+    ct = o.celltypes()
+
+    def dense(level):
+        n = 8 << level
+        d = np.zeros((n, n, n), np.uint8)
+        for t in np.nonzero(lev == level)[0]:
+            i, j, k = crd[t] * 8
+            d[i:i + 8, j:j + 8, k:k + 8] = ct[t].reshape(8, 8, 8)
+        return d
+
+    fine = dense(3)
+    for level in (2, 1, 0):
+        want = _reduce_types(fine)
+        assert np.array_equal(dense(level), want), level
+        fine = want
+
+
+def test_ghost_cell_types_copy_the_parent_cell():
+    o, lev, crd, typ, nbr, ct, lc = scene("band")
+    keys = {(int(l), *map(int, c)): i for i, (l, c) in enumerate(zip(lev, crd))}
+    ci = np.arange(512)
+    loc = np.stack([ci >> 6, (ci >> 3) & 7, ci & 7], axis=1)
+    for t in np.nonzero(typ == po.GHOST)[0]:
+        g = crd[t] * 8 + loc
+        pg = g // 2
+        p = keys[(int(lev[t]) - 1, *map(int, crd[t] // 2))]
+        pc = pg - crd[p] * 8
+        assert np.array_equal(ct[t], ct[p][pc[:, 0] * 64 + pc[:, 1] * 8 + pc[:, 2]])
+
+
+def test_recenter_removes_the_interior_mean():
+    """ReCenterLeafVoxels: the sum over all LEAF cells divided by the INTERIOR
+    count is subtracted from INTERIOR cells, so their sum becomes 0."""
+    o, lev, crd, typ, nbr, ct, lc = scene("band")
+    rng = np.random.default_rng(9)
+    v = leaf_interior_vector(lc, rng) + 0.25
+    v[lc.ct != po.INTERIOR] = 0
+    o.set(po.CH_TMP, lc.to_tiles(v, o.n))
+    o.recenter(po.CH_TMP)
+    got = lc.from_tiles(o.get(po.CH_TMP))
+    m = lc.ct == po.INTERIOR
+    assert abs(got[m].sum()) < 1e-9 * m.sum()
+    d = v[m] - got[m]
+    assert np.allclose(d, v[m].sum() / m.sum(), rtol=1e-12)
+    assert np.array_equal(got[~m], v[~m])
+
+
+def test_damped_jacobi_solve_first_iterate_and_progress():
+    """dampedJacobiSolve from x = 0: the first iterate is w b / A_cc on INTERIOR
+    cells; more iterations lower the residual."""
+    o, lev, crd, typ, nbr, ct, lc = scene("uniform32")
+    A = lc.matrix()
+    b = lc.to_tiles(leaf_interior_vector(lc, np.random.default_rng(10)), o.n)
+    w = 2.0 / 3.0
+    o.set(po.CH_B, b)
+    o.damped_jacobi_solve(1, 1e-30, w)
+    x = lc.from_tiles(o.get(po.CH_X))
+    m = lc.ct == po.INTERIOR
+    bl = lc.from_tiles(b)
+    assert np.allclose(x[m], w * bl[m] / A.diagonal()[m], rtol=1e-12)
+    res = []
+    for it in (10, 40):
+        o.set(po.CH_B, b)
+        res.append(o.damped_jacobi_solve(it, 1e-30, w)["rel_res"])
+    assert res[1] < res[0] < 1.0
+
+
+ALL_NEUMANN = dict(kind=po.SCENE_SPHERE_BAND, solid_r=0.15, band_level=[4, 3, 2], face_bc=[po.NEUMANN] * 6)
+
+
+def test_operator_annihilates_constants_without_dirichlet_cells():
+    """With every wall NEUMANN there is no DIRICHLET cell, and A 1 = 0 on LEAF
+    INTERIOR cells across coarse/fine faces too."""
+    o = po.Oracle(fp64=True, **ALL_NEUMANN)
+    lev, crd, typ, nbr = o.topology()
+    ct = o.celltypes()
+    assert not (ct[typ == po.LEAF] == po.DIRICHLET).any()
+    one = (ct == po.INTERIOR).astype(np.float64) * (typ == po.LEAF)[:, None]
+    o.set(po.CH_X, one)
+    o.full_negative_laplacian(po.CH_X, po.CH_AP)
+    got = o.get(po.CH_AP)[typ == po.LEAF]
+    assert np.abs(got).max() < 1e-13
+
+
+def test_pure_neumann_fp64_solve_satisfies_independent_system():
+    """Pure-Neumann MGPCG with re-centring (ReCenterLeafVoxels each iteration)."""
+    o = po.Oracle(fp64=True, **ALL_NEUMANN)
+    lev, crd, typ, nbr = o.topology()
+    ct = o.celltypes()
+    lc = LeafCells(lev, crd, typ, ct, o.params.h0, list(o.params.nroot))
+    b = po.Oracle(fp64=False, **ALL_NEUMANN).get(po.CH_B)
+    o.set(po.CH_B, b)
+    r = o.solve(max_iters=300, tol=1e-11, level_iters=2, coarsest_iters=10, pure_neumann=True)
+    assert r["rel_res"] < 1e-11
+    x = lc.from_tiles(o.get(po.CH_X))
+    m = lc.ct == po.INTERIOR
+    bl = lc.from_tiles(b)[m]
+    # A is singular (A 1 = 0) and b is not exactly mean-free (coarse/fine faces
+    # see ghost velocities), so the re-centred solve satisfies A x = b - mean(b)
+    A = lc.matrix()[m][:, m]
+    bc = bl - bl.mean()
+    assert np.linalg.norm(bc - A @ x[m]) / np.linalg.norm(bl) < 1e-8
