@@ -295,7 +295,7 @@ def main():
     per_kernel = {}
     for name, kid, lvl in [("apply", ca.KERNEL_APPLY, 0), ("restrict_L", ca.KERNEL_RESTRICT, L),
                            ("prolong_L", ca.KERNEL_PROLONG, L), ("update_xr", ca.KERNEL_UPDATE_XR, 0),
-                           ("gs_level1", ca.KERNEL_GS_SWEEP, 1), ("coarse_lds", ca.KERNEL_COARSE, 0),
+                           ("gs_level1", ca.KERNEL_GS_SWEEP, 1), ("coarse_lds", ca.KERNEL_COARSE, 10),
                            ("empty_1wg", ca.KERNEL_EMPTY, 1), ("empty_2240wg", ca.KERNEL_EMPTY, 2240),
                            ("copy_all", ca.KERNEL_COPY, 0)]:
         u, nt = P.benchmark_kernel(kid, lvl, reps=100)

@@ -3,7 +3,7 @@
 // Layout in HBM (structure of arrays, one problem or many in one pool):
 //   channel c of tile t, cell (i,j,k):  ch[c][t*512 + (i<<6 | j<<3 | k)]  (FP32)
 //   cell types:                        ctype[t*512 + ...]                (u8)
-//   worklists:                          arrays of 128-byte TileRec (one per tile
+//   worklists:                          arrays of 160-byte TileRec (one per tile
 //                                       and phase), so a workgroup finds its
 //                                       tile, neighbours and children in one
 //                                       scalar load
@@ -39,13 +39,22 @@ struct alignas(16) TileRec {
   int32_t gslot;      // GHOST: slot in the 64-value ghost buffer; -1 otherwise
   int32_t self;       // the tile's index
   float h;            // cell size of the tile's level
-  uint8_t level, type, octant, has_gchild;
-  uint8_t nbr_type[6];
-  uint8_t pad[6];
+  uint32_t meta;      // level | type << 8 | octant << 16 | has_gchild << 24
+  uint32_t ntypes;    // type of face neighbour f in bits 4f..4f+3 (0: none)
+  float bc[7];        // face coefficients of the smoothing view (subtree at the
+                      // tile's own level): faces x-,y-,z-,x+,y+,z+, then inside
+  int32_t pad0;
   int32_t gch[8];     // LEAF: ghost child per octant (-1: none)
   int32_t gcs[8];     // ... and its ghost slot
+  int32_t pad1[2];
 };
-static_assert(sizeof(TileRec) == 128, "TileRec is 128 bytes");
+static_assert(sizeof(TileRec) == 160, "TileRec is 160 bytes");
+// Fields packed in dwords so that a workgroup fetches them with scalar loads.
+__host__ __device__ __forceinline__ int tr_level(const TileRec& T) { return int(T.meta & 0xff); }
+__host__ __device__ __forceinline__ uint8_t tr_type(const TileRec& T) { return uint8_t((T.meta >> 8) & 0xff); }
+__host__ __device__ __forceinline__ int tr_octant(const TileRec& T) { return int((T.meta >> 16) & 0xff); }
+__host__ __device__ __forceinline__ bool tr_has_gchild(const TileRec& T) { return (T.meta >> 24) != 0; }
+__host__ __device__ __forceinline__ uint8_t tr_ntype(const TileRec& T, int f) { return uint8_t((T.ntypes >> (4 * f)) & 0xf); }
 
 struct alignas(16) ProbState {
   double gamma, gamma_old, alpha, beta;
@@ -164,54 +173,53 @@ struct Stencil {
 // Stage tile T's X values (zeros where the flags say so) and cell types.
 //   zero_all:   treat X as 0 everywhere (first sweep from a zero guess)
 //   ghost_zero: treat X of GHOST tiles as 0 (downstroke of the V-cycle)
-//   sl:         subtree level of the tile-type view (-1: plain types)
+//   bcv:        the 7 face coefficients of the view (faces x-..z+, inside)
 // X holds active tiles, XG ghost tiles (they differ only for the damped-Jacobi
 // ping-pong, where ghost values stay in the primary buffer).
 __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const float* __restrict__ X,
-                                           const float* __restrict__ XG, bool zero_all, bool ghost_zero, int sl,
-                                           TileLDS& L, Stencil& S) {
+                                           const float* __restrict__ XG, bool zero_all, bool ghost_zero,
+                                           const float bcv[7], TileLDS& L, Stencil& S) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t base = size_t(T.self) * 512 + lane * 8;
-  bool zero_own = zero_all || (ghost_zero && T.type == T_GHOST);
-  if (zero_own) { for (int k = 0; k < 8; k++) S.own[k] = 0.f; }
-  else load8((T.type == T_GHOST ? XG : X) + base, S.own);
+  const bool own_ghost = tr_type(T) == T_GHOST;
+  if (zero_all || (ghost_zero && own_ghost)) { for (int k = 0; k < 8; k++) S.own[k] = 0.f; }
+  else load8((own_ghost ? XG : X) + base, S.own);
   S.ct = load_ct8(c.ctype + base);
   const int pc = (i + 1) * 10 + (j + 1);
   L.xs[0][pc] = make_float4(S.own[0], S.own[1], S.own[2], S.own[3]);
   L.xs[1][pc] = make_float4(S.own[4], S.own[5], S.own[6], S.own[7]);
   L.cs[pc] = S.ct;
-
-  // halo pencils: x- (i==0), x+ (i==7), y- (j==0), y+ (j==7)
-  auto halo = [&](int f, int src_off, int dst) {
-    int n = T.nbr[f];
-    float v[8];
-    uint64_t w;
-    if (n < 0) {
-      for (int k = 0; k < 8; k++) v[k] = 0.f;
-      w = 0x0101010101010101ull;  // DIRICHLET outside the grid (GMGSolver.cu:120-122)
-    } else {
-      const size_t nb = size_t(n) * 512 + src_off;
-      bool gh = T.nbr_type[f] == T_GHOST;
-      bool z = zero_all || (ghost_zero && gh);
-      if (z) { for (int k = 0; k < 8; k++) v[k] = 0.f; }
-      else load8((gh ? XG : X) + nb, v);
+  // halo ring: lanes 0..31 fetch one neighbour pencil each
+  //   g = 0: x- (pencil (7,t) of nbr 0), 1: x+ ((0,t) of nbr 3), 2: y- ((t,7) of nbr 1), 3: y+ ((t,0) of nbr 4)
+  if (lane < 32) {
+    const int g = lane >> 3, t = lane & 7;
+    const int f = (g == 0) ? 0 : (g == 1) ? 3 : (g == 2) ? 1 : 4;
+    const int n = (g == 0) ? T.nbr[0] : (g == 1) ? T.nbr[3] : (g == 2) ? T.nbr[1] : T.nbr[4];
+    const int src = (g == 0) ? (56 + t) : (g == 1) ? t : (g == 2) ? (t * 8 + 7) : (t * 8);
+    const int dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
+    const bool gh = tr_ntype(T, f) == T_GHOST;
+    float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+    uint64_t w = 0x0101010101010101ull;  // DIRICHLET outside the grid (GMGSolver.cu:120-122)
+    if (n >= 0) {
+      const size_t nb = size_t(n) * 512 + src * 8;
+      if (!(zero_all || (ghost_zero && gh))) {
+        const float4* q = reinterpret_cast<const float4*>((gh ? XG : X) + nb);
+        v0 = q[0];
+        v1 = q[1];
+      }
       w = load_ct8(c.ctype + nb);
     }
-    L.xs[0][dst] = make_float4(v[0], v[1], v[2], v[3]);
-    L.xs[1][dst] = make_float4(v[4], v[5], v[6], v[7]);
+    L.xs[0][dst] = v0;
+    L.xs[1][dst] = v1;
     L.cs[dst] = w;
-  };
-  if (i == 0) halo(0, (7 * 8 + j) * 8, 0 * 10 + (j + 1));
-  if (i == 7) halo(3, (0 * 8 + j) * 8, 9 * 10 + (j + 1));
-  if (j == 0) halo(1, (i * 8 + 7) * 8, (i + 1) * 10 + 0);
-  if (j == 7) halo(4, (i * 8 + 0) * 8, (i + 1) * 10 + 9);
+  }
   // k halos
   {
     int n = T.nbr[2];
     if (n < 0) { S.xkm = 0.f; S.ckm = C_DIR; }
     else {
       size_t o = size_t(n) * 512 + lane * 8 + 7;
-      bool gh = T.nbr_type[2] == T_GHOST;
+      bool gh = tr_ntype(T, 2) == T_GHOST;
       bool z = zero_all || (ghost_zero && gh);
       S.xkm = z ? 0.f : (gh ? XG : X)[o];
       S.ckm = c.ctype[o];
@@ -220,22 +228,29 @@ __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const
     if (n < 0) { S.xkp = 0.f; S.ckp = C_DIR; }
     else {
       size_t o = size_t(n) * 512 + lane * 8;
-      bool gh = T.nbr_type[5] == T_GHOST;
+      bool gh = tr_ntype(T, 5) == T_GHOST;
       bool z = zero_all || (ghost_zero && gh);
       S.xkp = z ? 0.f : (gh ? XG : X)[o];
       S.ckp = c.ctype[o];
     }
   }
-  // coefficients (GMGSolver.cu:26-39): 1/h^2 and (2/h/3)/h as the reference forms them
   S.h = T.h;
-  float ooh = 1.0f / T.h;
-  float cll = ooh * ooh;
-  float clg = ((2.0f * ooh) / 3.0f) * ooh;
-  uint8_t to = subtree_type(T.type, T.level, sl);
-  S.b_int = base_coeff(to, to, cll, clg);
+  S.b_int = bcv[6];
+  for (int f = 0; f < 6; f++) S.b_f[f] = bcv[f];
+}
+
+// Face coefficients of tile T seen with the subtree view at level sl
+// (GMGSolver.cu:26-39): 1/h^2 and (2/h/3)/h as the reference forms them; an
+// absent neighbour counts as the tile's own type (GMGSolver.cu:118-122).
+__host__ __device__ __forceinline__ void view_coeffs(const TileRec& T, int sl, float bcv[7]) {
+  const float ooh = 1.0f / T.h;
+  const float cll = ooh * ooh;
+  const float clg = ((2.0f * ooh) / 3.0f) * ooh;
+  const uint8_t to = subtree_type(tr_type(T), tr_level(T), sl);
+  bcv[6] = base_coeff(to, to, cll, clg);
   for (int f = 0; f < 6; f++) {
-    uint8_t tn = (T.nbr[f] < 0) ? to : subtree_type(T.nbr_type[f], T.level, sl);
-    S.b_f[f] = base_coeff(to, tn, cll, clg);
+    const uint8_t tn = (T.nbr[f] < 0) ? to : subtree_type(tr_ntype(T, f), tr_level(T), sl);
+    bcv[f] = base_coeff(to, tn, cll, clg);
   }
 }
 
@@ -248,11 +263,11 @@ __device__ __forceinline__ void unpack8(float4 a, float4 b, float v[8]) {
 // INTERIOR cells).
 __device__ __forceinline__ void cell_row(float x0, uint8_t c0, const float xn[6], const uint8_t cn[6], const float bc[6],
                                          float h, float& ax, float& dg) {
-  const bool n0 = c0 & C_NEU;
   float s = 0.f, d = 0.f;
 #pragma unroll
   for (int e = 0; e < 6; e++) {
-    const float cf = (n0 || (cn[e] & C_NEU)) ? 0.f : bc[e];
+    // a NEUMANN cell on either side cuts the face (one integer test, no mask logic)
+    const float cf = ((c0 | cn[e]) & C_NEU) ? 0.f : bc[e];
     s = __builtin_fmaf(cf, x0 - xn[e], s);
     d = d + cf;
   }
@@ -261,17 +276,33 @@ __device__ __forceinline__ void cell_row(float x0, uint8_t c0, const float xn[6]
   dg = in ? ((d * h) * h) * h : 0.f;
 }
 
-// A x (and its diagonal) on all 8 cells of the lane after stage_tile + wave_sync.
-template <bool WANT_AX, bool WANT_D>
-__device__ __forceinline__ void apply_pencil(const TileLDS& L, const Stencil& S, float ax[8], float dg[8]) {
+// The four i/j-neighbour pencils of a lane (values and cell types).
+struct NbrPencils {
+  float4 im0, im1, ip0, ip1, jm0, jm1, jp0, jp1;
+  uint64_t cim, cip, cjm, cjp;
+};
+__device__ __forceinline__ NbrPencils lds_nbrs(const TileLDS& L) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const int pc = (i + 1) * 10 + (j + 1);
+  NbrPencils N;
+  N.im0 = L.xs[0][pc - 10]; N.im1 = L.xs[1][pc - 10];
+  N.ip0 = L.xs[0][pc + 10]; N.ip1 = L.xs[1][pc + 10];
+  N.jm0 = L.xs[0][pc - 1];  N.jm1 = L.xs[1][pc - 1];
+  N.jp0 = L.xs[0][pc + 1];  N.jp1 = L.xs[1][pc + 1];
+  N.cim = L.cs[pc - 10]; N.cip = L.cs[pc + 10]; N.cjm = L.cs[pc - 1]; N.cjp = L.cs[pc + 1];
+  return N;
+}
+
+// A x (and its diagonal) on all 8 cells of the lane, given its neighbour pencils.
+template <bool WANT_AX, bool WANT_D>
+__device__ __forceinline__ void apply_pencil(const NbrPencils& N, const Stencil& S, float ax[8], float dg[8]) {
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   float xim[8], xip[8], xjm[8], xjp[8];
-  unpack8(L.xs[0][pc - 10], L.xs[1][pc - 10], xim);
-  unpack8(L.xs[0][pc + 10], L.xs[1][pc + 10], xip);
-  unpack8(L.xs[0][pc - 1], L.xs[1][pc - 1], xjm);
-  unpack8(L.xs[0][pc + 1], L.xs[1][pc + 1], xjp);
-  const uint64_t cim = L.cs[pc - 10], cip = L.cs[pc + 10], cjm = L.cs[pc - 1], cjp = L.cs[pc + 1];
+  unpack8(N.im0, N.im1, xim);
+  unpack8(N.ip0, N.ip1, xip);
+  unpack8(N.jm0, N.jm1, xjm);
+  unpack8(N.jp0, N.jp1, xjp);
+  const uint64_t cim = N.cim, cip = N.cip, cjm = N.cjm, cjp = N.cjp;
   const float bim = (i == 0) ? S.b_f[0] : S.b_int;
   const float bip = (i == 7) ? S.b_f[3] : S.b_int;
   const float bjm = (j == 0) ? S.b_f[1] : S.b_int;
@@ -302,16 +333,15 @@ __device__ __forceinline__ float comp4(float4 v, int m) { return m == 0 ? v.x : 
 // (i,j,k) is updated when (i+j+k) % 2 == color, i.e. k = 2m + p with the lane's
 // parity p.  The operands of those four rows are gathered with whole-vector
 // selects on p (no per-lane register indexing), so only the four rows are formed.
-__device__ __forceinline__ void gs_pencil(const TileLDS& L, const Stencil& S, const float b[8], int color, float4& out0,
-                                          float4& out1) {
+__device__ __forceinline__ void gs_pencil(const NbrPencils& N, const Stencil& S, const float b[8], int color,
+                                          float4& out0, float4& out1) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
-  const int pc = (i + 1) * 10 + (j + 1);
   const bool p = ((color + i + j) & 1) != 0;
   auto pick = [&](float4 a, float4 c) { return sel4(p, even4(a, c), odd4(a, c)); };
-  const float4 XIM = pick(L.xs[0][pc - 10], L.xs[1][pc - 10]);
-  const float4 XIP = pick(L.xs[0][pc + 10], L.xs[1][pc + 10]);
-  const float4 XJM = pick(L.xs[0][pc - 1], L.xs[1][pc - 1]);
-  const float4 XJP = pick(L.xs[0][pc + 1], L.xs[1][pc + 1]);
+  const float4 XIM = pick(N.im0, N.im1);
+  const float4 XIP = pick(N.ip0, N.ip1);
+  const float4 XJM = pick(N.jm0, N.jm1);
+  const float4 XJP = pick(N.jp0, N.jp1);
   const float4 o0 = make_float4(S.own[0], S.own[1], S.own[2], S.own[3]);
   const float4 o1 = make_float4(S.own[4], S.own[5], S.own[6], S.own[7]);
   const float4 EV = even4(o0, o1), OD = odd4(o0, o1);
@@ -320,7 +350,7 @@ __device__ __forceinline__ void gs_pencil(const TileLDS& L, const Stencil& S, co
   const float4 XKM = sel4(p, make_float4(S.xkm, OD.x, OD.y, OD.z), EV);
   const float4 XKP = sel4(p, OD, make_float4(EV.y, EV.z, EV.w, S.xkp));
   const float4 B = sel4(p, make_float4(b[0], b[2], b[4], b[6]), make_float4(b[1], b[3], b[5], b[7]));
-  const uint64_t cim = L.cs[pc - 10], cip = L.cs[pc + 10], cjm = L.cs[pc - 1], cjp = L.cs[pc + 1];
+  const uint64_t cim = N.cim, cip = N.cip, cjm = N.cjm, cjp = N.cjp;
   const float bim = (i == 0) ? S.b_f[0] : S.b_int;
   const float bip = (i == 7) ? S.b_f[3] : S.b_int;
   const float bjm = (j == 0) ? S.b_f[1] : S.b_int;
@@ -365,13 +395,13 @@ __device__ __forceinline__ void load_rhs(const Ctx& c, const TileRec& T, const f
 __device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int level, float* X, const float* F, int color,
                                         int flags, TileLDS& L) {
   Stencil S;
-  stage_tile(c, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, level, L, S);
+  stage_tile(c, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
   float b[8];
   load_rhs(c, T, F, flags, b);
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float4 out0, out1;
-  gs_pencil(L, S, b, color, out0, out1);
+  gs_pencil(lds_nbrs(L), S, b, color, out0, out1);
   float4* dst = reinterpret_cast<float4*>(X + size_t(T.self) * 512 + lane_id() * 8);
   dst[0] = out0;
   dst[1] = out1;
@@ -382,13 +412,13 @@ __device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int leve
 __device__ __forceinline__ void tile_jacobi(const Ctx& c, const TileRec& T, int level, const float* X, const float* XG,
                                             float* Y, const float* F, float omega, int flags, TileLDS& L) {
   Stencil S;
-  stage_tile(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO, level, L, S);
+  stage_tile(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
   float b[8];
   load_rhs(c, T, F, flags, b);
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float ax[8], dg[8];
-  apply_pencil<true, true>(L, S, ax, dg);
+  apply_pencil<true, true>(lds_nbrs(L), S, ax, dg);
   float out[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) {
@@ -414,7 +444,7 @@ __device__ __forceinline__ float octet_sum(const float* Ls, int q) {
 }
 __device__ __forceinline__ size_t parent_cell(const TileRec& T, int q) {
   int a = q >> 4, bq = (q >> 2) & 3, cq = q & 3;
-  int ox = T.octant & 1, oy = (T.octant >> 1) & 1, oz = (T.octant >> 2) & 1;
+  const int oct = tr_octant(T), ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
   return size_t(T.parent) * 512 + ((ox * 4 + a) << 6) + ((oy * 4 + bq) << 3) + (oz * 4 + cq);
 }
 
@@ -425,19 +455,19 @@ __device__ __forceinline__ size_t parent_cell(const TileRec& T, int q) {
 __device__ __forceinline__ void tile_restrict(const Ctx& c, const TileRec& T, int level, const float* X, const float* F,
                                               float* CO, float one_over_alpha, int flags, TileLDS& L, float* res) {
   Stencil S;
-  stage_tile(c, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, level, L, S);
+  stage_tile(c, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
   const int lane = lane_id();
   float b[8];
-  if ((flags & F_VCYCLE) && T.type == T_GHOST) { for (int k = 0; k < 8; k++) b[k] = 0.f; }
+  if ((flags & F_VCYCLE) && tr_type(T) == T_GHOST) { for (int k = 0; k < 8; k++) b[k] = 0.f; }
   else load_rhs(c, T, F, flags, b);
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float ax[8], dg[8];
-  apply_pencil<true, false>(L, S, ax, dg);
+  apply_pencil<true, false>(lds_nbrs(L), S, ax, dg);
   for (int k = 0; k < 8; k++) res[lane * 8 + k] = b[k] - ax[k];
   wave_sync();
   float s = octet_sum(res, lane);
-  const bool ghost = (subtree_type(T.type, T.level, level) & T_GHOST);
+  const bool ghost = tr_type(T) == T_GHOST;
   float* out = (flags & F_VCYCLE) ? c.rhsv : CO;
   size_t pcell = parent_cell(T, lane);
   if (ghost) out[pcell] = out[pcell] + s;
@@ -452,12 +482,12 @@ __device__ __forceinline__ void tile_prolong(const Ctx& c, const TileRec& T, int
                                              float* XO, float* XG, float pcoef, int flags) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t base = size_t(T.self) * 512 + lane * 8;
-  int ox = T.octant & 1, oy = (T.octant >> 1) & 1, oz = (T.octant >> 2) & 1;
+  const int oct = tr_octant(T), ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
   const float4 pv = *reinterpret_cast<const float4*>(XC + size_t(T.parent) * 512 + ((ox * 4 + (i >> 1)) << 6) +
                                                      ((oy * 4 + (j >> 1)) << 3) + oz * 4);
   const float cz[4] = {pv.x, pv.y, pv.z, pv.w};
   const uint64_t ct = load_ct8(c.ctype + base);
-  const bool ghost = subtree_type(T.type, T.level, level) & T_GHOST;
+  const bool ghost = tr_type(T) == T_GHOST;
   float fx[8];
   if (ghost || (flags & F_ZERO_X)) { for (int k = 0; k < 8; k++) fx[k] = 0.f; }
   else load8(XF + base, fx);
@@ -514,9 +544,12 @@ __global__ __launch_bounds__(64) void k_prolong(Ctx c, const TileRec* __restrict
 // need.  The per-cell arithmetic is the same as the tile kernels', so results
 // are identical; the bottom is GMGSolver.cu:432-433.
 // ---------------------------------------------------------------------------
-constexpr int CL_MAX_TILES = 16;
+constexpr int CL_MAX_TILES = 12;     // tiles on levels 0..C together
+constexpr int CL_MAX_PER_LEVEL = 8;  // tiles on any one of those levels
 constexpr int CL_MAX_LEVELS = 8;
 constexpr int CL_THREADS = 1024;
+constexpr int CL_CPT = CL_MAX_PER_LEVEL * 256 / CL_THREADS;  // cells of one colour per thread
+constexpr int CL_ZERO = CL_MAX_TILES * 512;                 // LDS index of a cell that stays 0
 struct alignas(16) CoarseSlot {
   int32_t tile;      // tile index
   int32_t nbr[6];    // slot of the same-level face neighbour (x-,y-,z-,x+,y+,z+), -1: none
@@ -538,163 +571,198 @@ struct VCycleArgs {
   float omega, ooa, pcoef;
 };
 struct CoarseLDS {
-  float z[CL_MAX_TILES][512];
-  float z2[CL_MAX_TILES][512];
-  float b[CL_MAX_TILES][512];
-  uint8_t ct[CL_MAX_TILES][512];
+  float z[CL_MAX_TILES * 512 + 4];  // V-cycle x of every slot (slot * 512 + cell), then CL_ZERO = 0
+  float b[CL_MAX_TILES][512];       // V-cycle rhs
+  float t[CL_MAX_TILES][512];       // residuals (restriction) or new values (damped Jacobi)
+  alignas(16) uint8_t ct[CL_MAX_TILES][512];
   CoarseSlot S[CL_MAX_TILES];
 };
 
-// A x and D of cell (i,j,k) of slot s (subtree view at the slot's level); the
-// same arithmetic as cell_row, accumulated face by face to keep registers low.
-__device__ __forceinline__ void cl_stencil(const CoarseLDS& M, int s, int i, int j, int k, float& ax, float& dg) {
+// The static part of one cell's row, prepared once per level phase and kept in
+// registers: LDS indices of the cell and its six neighbours (faces x-, x+, y-,
+// y+, z-, z+; CL_ZERO outside the grid), the face coefficients after the
+// NEUMANN cut, the diagonal and the rhs.  A pass then costs seven independent
+// LDS reads and the fma chain of cell_row, with the same operations in the same
+// order (GMGSolver.cu:56-76).
+struct CellPrep {
+  int own;      // slot * 512 + cell, -1: no cell
+  int nb[6];
+  float cf[6];
+  float dg, h, b;
+  bool in;
+};
+
+__device__ __forceinline__ void cl_prep(const CoarseLDS& M, int s, int cc, CellPrep& P) {
   const CoarseSlot& S = M.S[s];
-  const int c = (i << 6) | (j << 3) | k;
-  const uint8_t c0 = M.ct[s][c];
-  const float x0 = M.z[s][c];
-  const bool n0 = c0 & C_NEU;
-  float sum = 0.f, d = 0.f;
-  const int di[6] = {-1, 1, 0, 0, 0, 0}, dj[6] = {0, 0, -1, 1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
-  const int fi[6] = {0, 3, 1, 4, 2, 5};
+  const int i = cc >> 6, j = (cc >> 3) & 7, k = cc & 7;
+  const uint8_t c0 = M.ct[s][cc];
+  const bool inside[6] = {i > 0, i < 7, j > 0, j < 7, k > 0, k < 7};
+  const int step[6] = {-64, 64, -8, 8, -1, 1};
+  const int face[6] = {0, 3, 1, 4, 2, 5};
+  P.own = s * 512 + cc;
+  float d = 0.f;
 #pragma unroll
   for (int e = 0; e < 6; e++) {
-    const int ni = i + di[e], nj = j + dj[e], nk = k + dk[e];
-    float xv, bcv;
-    uint8_t cv;
-    if (ni >= 0 && ni < 8 && nj >= 0 && nj < 8 && nk >= 0 && nk < 8) {
-      const int nc = (ni << 6) | (nj << 3) | nk;
-      xv = M.z[s][nc];
-      cv = M.ct[s][nc];
-      bcv = S.bc[6];
-    } else {
-      const int q = S.nbr[fi[e]];
-      const int nc = ((ni & 7) << 6) | ((nj & 7) << 3) | (nk & 7);
-      xv = q < 0 ? 0.f : M.z[q][nc];
-      cv = q < 0 ? C_DIR : M.ct[q][nc];
-      bcv = S.bc[fi[e]];
-    }
-    const float cf = (n0 || (cv & C_NEU)) ? 0.f : bcv;
-    sum = __builtin_fmaf(cf, x0 - xv, sum);
-    d = d + cf;
+    const int q = inside[e] ? s : S.nbr[face[e]];
+    const int nc = inside[e] ? cc + step[e] : cc - 7 * step[e];
+    const uint8_t cn = q < 0 ? uint8_t(C_DIR) : M.ct[q < 0 ? 0 : q][nc];
+    const float bc = inside[e] ? S.bc[6] : S.bc[face[e]];
+    P.nb[e] = q < 0 ? CL_ZERO : q * 512 + nc;
+    P.cf[e] = ((c0 | cn) & C_NEU) ? 0.f : bc;
+    d = d + P.cf[e];
   }
-  const float h = S.h;
-  const bool in = c0 & C_INT;
-  ax = in ? ((sum * h) * h) * h : 0.f;
-  dg = in ? ((d * h) * h) * h : 0.f;
+  P.in = c0 & C_INT;
+  P.h = S.h;
+  P.dg = P.in ? ((d * P.h) * P.h) * P.h : 0.f;
+  P.b = M.b[s][cc];
 }
 
+// A x of a prepared cell (cell_row's arithmetic); x0 returns the cell's value.
+__device__ __forceinline__ float cl_ax(const CoarseLDS& M, const CellPrep& P, float& x0) {
+  x0 = M.z[P.own];
+  float xn[6];
+#pragma unroll
+  for (int e = 0; e < 6; e++) xn[e] = M.z[P.nb[e]];
+  float sum = 0.f;
+#pragma unroll
+  for (int e = 0; e < 6; e++) sum = __builtin_fmaf(P.cf[e], x0 - xn[e], sum);
+  return P.in ? ((sum * P.h) * P.h) * P.h : 0.f;
+}
+
+// The coarse levels 0..C of one V-cycle in one 1024-thread workgroup with every
+// tile of those levels in LDS (the host keeps levels with GHOST tiles out of C
+// and at most CL_MAX_PER_LEVEL tiles on a level): smoothing, restriction and
+// prolongation per level and the bottom sweeps on level 0.  Each thread owns up
+// to CL_CPT cells of each colour of the level being worked and keeps their
+// prepared rows in registers.  One launch replaces ~10 launches per coarse
+// level; the per-cell arithmetic is that of the tile kernels
+// (GMGSolver.cu:216-376, bottom :432-433).
 __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, CoarsePlan cp, VCycleArgs v, float* Z, int flags) {
   __shared__ CoarseLDS M;
   if ((flags & F_CHECK_DONE) && c.ps[0].done) return;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int q = tid; q < cp.nslots; q += nt) M.S[q] = cp.slots[q];
+  const int tid = threadIdx.x;
+  if (tid < cp.nslots) M.S[tid] = cp.slots[tid];
+  if (tid < 4) M.z[CL_ZERO + tid] = 0.f;
   __syncthreads();
-  // cell types and rhs (V-cycle rhs buffer for active tiles, 0 for ghosts); z = 0
-  for (int idx = tid; idx < cp.nslots * 64; idx += nt) {
-    const int s = idx >> 6, lane = idx & 63;
-    const CoarseSlot& S = M.S[s];
-    const size_t base = size_t(S.tile) * 512 + lane * 8;
-    *reinterpret_cast<uint64_t*>(&M.ct[s][lane * 8]) = load_ct8(c.ctype + base);
-    float b[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (S.type != T_GHOST) load8(c.rhsv + base, b);
-    for (int k = 0; k < 8; k++) {
-      M.b[s][lane * 8 + k] = b[k];
-      M.z[s][lane * 8 + k] = 0.f;
-    }
+  // cell types and rhs; z = 0
+  for (int q = tid; q < cp.nslots * 128; q += CL_THREADS) {
+    const int s = q >> 7, o = (q & 127) * 4;
+    const size_t g = size_t(M.S[s].tile) * 512 + o;
+    *reinterpret_cast<float4*>(&M.b[s][o]) = *reinterpret_cast<const float4*>(c.rhsv + g);
+    *reinterpret_cast<float4*>(&M.z[s * 512 + o]) = make_float4(0.f, 0.f, 0.f, 0.f);
+    *reinterpret_cast<uint32_t*>(&M.ct[s][o]) = *reinterpret_cast<const uint32_t*>(c.ctype + g);
   }
   __syncthreads();
-  const bool jac = v.smoother != 0;
-  // one red-black colour pass over the active slots of level l
-  auto gs = [&](int l, int color) {
-    const int s0 = cp.lvl_begin[l], ns = cp.lvl_begin[l + 1] - s0;
-    for (int idx = tid; idx < ns * 256; idx += nt) {
-      const int s = s0 + (idx >> 8), r = idx & 255;
-      if (M.S[s].type == T_GHOST) continue;
-      const int i = r >> 5, j = (r >> 2) & 7, k = 2 * (r & 3) + ((color + i + j) & 1);
-      float ax, dg;
-      cl_stencil(M, s, i, j, k, ax, dg);
-      const int cc = (i << 6) | (j << 3) | k;
-      float delta = 0.f;
-      if ((M.ct[s][cc] & C_INT) && dg != 0.f) delta = (M.b[s][cc] - ax) / dg;
-      M.z[s][cc] = M.z[s][cc] + delta;
+  CellPrep P[2][CL_CPT];  // [colour][m]
+  // rows of this thread's cells of level l (cell m of colour col: q = tid + m * CL_THREADS)
+  auto prep = [&](int l) {
+    const int s0 = cp.lvl_begin[l], n = (cp.lvl_begin[l + 1] - s0) * 256;
+#pragma unroll
+    for (int col = 0; col < 2; col++)
+#pragma unroll
+      for (int m = 0; m < CL_CPT; m++) {
+        const int q = tid + m * CL_THREADS;
+        P[col][m].own = -1;
+        if (q < n) {
+          const int s = s0 + (q >> 8), r = q & 255;
+          const int i = r >> 5, j = (r >> 2) & 7;
+          cl_prep(M, s, (i << 6) | (j << 3) | (2 * (r & 3) + ((col + i + j) & 1)), P[col][m]);
+        }
+      }
+  };
+  // one red-black colour pass of the prepared level
+  auto gs = [&](int col) {
+#pragma unroll
+    for (int m = 0; m < CL_CPT; m++) {
+      const CellPrep& Q = P[col][m];
+      if (Q.own >= 0) {
+        float x0;
+        const float ax = cl_ax(M, Q, x0);
+        float delta = 0.f;
+        if (Q.in && Q.dg != 0.f) delta = (Q.b - ax) / Q.dg;
+        M.z[Q.own] = x0 + delta;
+      }
     }
     __syncthreads();
   };
-  // one damped-Jacobi sweep over the active slots of level l
-  auto jacobi = [&](int l) {
-    const int s0 = cp.lvl_begin[l], ns = cp.lvl_begin[l + 1] - s0;
-    for (int idx = tid; idx < ns * 512; idx += nt) {
-      const int s = s0 + (idx >> 9), cc = idx & 511;
-      if (M.S[s].type == T_GHOST) continue;
-      float ax, dg;
-      cl_stencil(M, s, cc >> 6, (cc >> 3) & 7, cc & 7, ax, dg);
-      float out = M.z[s][cc];
-      if (M.ct[s][cc] & C_INT) out = (dg == 0.f) ? 0.f : M.z[s][cc] + (v.omega * (M.b[s][cc] - ax)) / dg;
-      M.z2[s][cc] = out;
-    }
+  // one damped-Jacobi sweep of the prepared level
+  auto jacobi = [&]() {
+    float out[2][CL_CPT];
+#pragma unroll
+    for (int col = 0; col < 2; col++)
+#pragma unroll
+      for (int m = 0; m < CL_CPT; m++) {
+        const CellPrep& Q = P[col][m];
+        out[col][m] = 0.f;
+        if (Q.own >= 0) {
+          float x0;
+          const float ax = cl_ax(M, Q, x0);
+          out[col][m] = x0;
+          if (Q.in) out[col][m] = (Q.dg == 0.f) ? 0.f : x0 + (v.omega * (Q.b - ax)) / Q.dg;
+        }
+      }
     __syncthreads();
-    for (int idx = tid; idx < ns * 512; idx += nt) {
-      const int s = s0 + (idx >> 9), cc = idx & 511;
-      if (M.S[s].type != T_GHOST) M.z[s][cc] = M.z2[s][cc];
-    }
+#pragma unroll
+    for (int col = 0; col < 2; col++)
+#pragma unroll
+      for (int m = 0; m < CL_CPT; m++)
+        if (P[col][m].own >= 0) M.z[P[col][m].own] = out[col][m];
     __syncthreads();
   };
-  auto smooth = [&](int l, int order) {
+  auto smooth = [&](int order) {
     for (int it = 0; it < v.nu; it++) {
-      if (jac) jacobi(l);
-      else { gs(l, order); gs(l, 1 - order); }
+      if (v.smoother != 0) jacobi();
+      else { gs(order); gs(1 - order); }
     }
   };
   for (int l = cp.C; l >= 1; l--) {
-    smooth(l, 0);
-    // residual of level l restricted to level l-1: one thread per coarse cell
+    prep(l);
+    smooth(0);
+    // residuals of level l, then their 2x2x2 sums into the rhs of level l-1
+#pragma unroll
+    for (int col = 0; col < 2; col++)
+#pragma unroll
+      for (int m = 0; m < CL_CPT; m++) {
+        const CellPrep& Q = P[col][m];
+        if (Q.own >= 0) {
+          float x0;
+          const float ax = cl_ax(M, Q, x0);
+          (&M.t[0][0])[Q.own] = Q.b - ax;
+        }
+      }
+    __syncthreads();
     const int s0 = cp.lvl_begin[l], ns = cp.lvl_begin[l + 1] - s0;
-    for (int idx = tid; idx < ns * 64; idx += nt) {
-      const int s = s0 + (idx >> 6), q = idx & 63;
-      const int a = q >> 4, bq = (q >> 2) & 3, cq = q & 3;
-      float sum = 0.f;
-      for (int ii = 0; ii < 2; ii++)
-        for (int jj = 0; jj < 2; jj++)
-          for (int kk = 0; kk < 2; kk++) {
-            const int fi = 2 * a + ii, fj = 2 * bq + jj, fk = 2 * cq + kk;
-            float ax, dg;
-            cl_stencil(M, s, fi, fj, fk, ax, dg);
-            const float rhs = (M.S[s].type == T_GHOST) ? 0.f : M.b[s][(fi << 6) | (fj << 3) | fk];
-            sum += rhs - ax;
-          }
+    for (int q = tid; q < ns * 64; q += CL_THREADS) {
+      const int s = s0 + (q >> 6), lq = q & 63;
+      const float sum = octet_sum(M.t[s], lq);
       const CoarseSlot& S = M.S[s];
+      const int a = lq >> 4, bq = (lq >> 2) & 3, cq = lq & 3;
       const int ox = S.octant & 1, oy = (S.octant >> 1) & 1, oz = (S.octant >> 2) & 1;
-      const int pc = ((ox * 4 + a) << 6) | ((oy * 4 + bq) << 3) | (oz * 4 + cq);
-      if (S.type == T_GHOST) M.b[S.parent][pc] = M.b[S.parent][pc] + sum;
-      else M.b[S.parent][pc] = sum * v.ooa;
+      M.b[S.parent][((ox * 4 + a) << 6) | ((oy * 4 + bq) << 3) | (oz * 4 + cq)] = sum * v.ooa;
     }
     __syncthreads();
   }
+  prep(0);
   const int it0 = v.coarsest / 2;
-  for (int i = 0; i < it0; i++) { gs(0, 0); gs(0, 1); }
-  for (int i = 0; i < it0; i++) { gs(0, 1); gs(0, 0); }
+  for (int it = 0; it < it0; it++) { gs(0); gs(1); }
+  for (int it = 0; it < it0; it++) { gs(1); gs(0); }
   for (int l = 1; l <= cp.C; l++) {
     const int s0 = cp.lvl_begin[l], ns = cp.lvl_begin[l + 1] - s0;
-    for (int idx = tid; idx < ns * 512; idx += nt) {
-      const int s = s0 + (idx >> 9), cc = idx & 511;
+    for (int q = tid; q < ns * 512; q += CL_THREADS) {
+      const int s = s0 + (q >> 9), cc = q & 511;
       const CoarseSlot& S = M.S[s];
       const int i = cc >> 6, j = (cc >> 3) & 7, k = cc & 7;
       const int ox = S.octant & 1, oy = (S.octant >> 1) & 1, oz = (S.octant >> 2) & 1;
-      const float cz = M.z[S.parent][((ox * 4 + (i >> 1)) << 6) | ((oy * 4 + (j >> 1)) << 3) | (oz * 4 + (k >> 1))];
-      const bool in = M.ct[s][cc] & C_INT;
-      if (S.type == T_GHOST) M.z[s][cc] = in ? cz : 0.f;
-      else M.z[s][cc] = in ? __builtin_fmaf(v.pcoef, cz, M.z[s][cc]) : 0.f;
+      const float cz = M.z[S.parent * 512 + (((ox * 4 + (i >> 1)) << 6) | ((oy * 4 + (j >> 1)) << 3) | (oz * 4 + (k >> 1)))];
+      M.z[s * 512 + cc] = (M.ct[s][cc] & C_INT) ? __builtin_fmaf(v.pcoef, cz, M.z[s * 512 + cc]) : 0.f;
     }
     __syncthreads();
-    smooth(l, 1);
+    prep(l);
+    smooth(1);
   }
-  // write back the active tiles' z
-  for (int idx = tid; idx < cp.nslots * 64; idx += nt) {
-    const int s = idx >> 6, lane = idx & 63;
-    if (M.S[s].type == T_GHOST) continue;
-    float o[8];
-    for (int k = 0; k < 8; k++) o[k] = M.z[s][lane * 8 + k];
-    store8(Z + size_t(M.S[s].tile) * 512 + lane * 8, o);
+  for (int q = tid; q < cp.nslots * 128; q += CL_THREADS) {
+    const int s = q >> 7, o = (q & 127) * 4;
+    *reinterpret_cast<float4*>(Z + size_t(M.S[s].tile) * 512 + o) = *reinterpret_cast<const float4*>(&M.z[s * 512 + o]);
   }
 }
 
@@ -756,7 +824,7 @@ __global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const TileRec* __rest
   for (int k = 0; k < 8; k++)
     if (ct_at(ct, k) & C_INT) pv[k] = __builtin_fmaf(beta, pv[k], zv[k]);
   store8(c.p + base, pv);
-  if (!T.has_gchild) return;
+  if (!tr_has_gchild(T)) return;
   __shared__ float Lp[512];
   __shared__ uint8_t Lc[512];
   for (int k = 0; k < 8; k++) { Lp[lane * 8 + k] = pv[k]; Lc[lane * 8 + k] = ct_at(ct, k); }
@@ -783,22 +851,24 @@ __global__ __launch_bounds__(64) void k_apply(Ctx c, const TileRec* __restrict__
   __shared__ TileLDS L;
   __shared__ float gl[512];
   Stencil S;
-  stage_tile(c, T, X, X, false, false, sl, L, S);
+  float bcv[7];
+  view_coeffs(T, sl, bcv);
+  stage_tile(c, T, X, X, false, false, bcv, L, S);
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float ax[8], dg[8];
-  if (diag) apply_pencil<false, true>(L, S, dg, ax);
-  else apply_pencil<true, false>(L, S, ax, dg);
+  if (diag) apply_pencil<false, true>(lds_nbrs(L), S, dg, ax);
+  else apply_pencil<true, false>(lds_nbrs(L), S, ax, dg);
   const int lane = lane_id();
   const size_t base = size_t(T.self) * 512 + lane * 8;
-  if (T.type == T_GHOST && dot) {
+  if (tr_type(T) == T_GHOST && dot) {
     for (int k = 0; k < 8; k++) gl[lane * 8 + k] = (ct_at(S.ct, k) & (C_INT | C_DIR | C_NEU)) ? ax[k] : 0.f;
     wave_sync();
     c.gsum[size_t(T.gslot) * 64 + lane] = octet_sum(gl, lane);
     return;
   }
   store8(AX + base, ax);
-  if (!dot || T.type != T_LEAF || T.has_gchild) return;
+  if (!dot || tr_type(T) != T_LEAF || tr_has_gchild(T)) return;
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(S.own[k]) * double(ax[k]);
   a = wave_sum(a);
@@ -978,7 +1048,7 @@ __global__ __launch_bounds__(64) void k_propagate_ghosts(Ctx c, const TileRec* _
   TILE_KERNEL_PROLOGUE
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t base = size_t(T.self) * 512 + lane * 8;
-  int ox = T.octant & 1, oy = (T.octant >> 1) & 1, oz = (T.octant >> 2) & 1;
+  const int oct = tr_octant(T), ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
   const size_t pb = size_t(T.parent) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) + oz * 4;
   const float4 pv = *reinterpret_cast<const float4*>(XC + pb);
   const uint32_t pc = *reinterpret_cast<const uint32_t*>(c.ctype + pb);

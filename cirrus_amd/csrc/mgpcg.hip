@@ -130,20 +130,6 @@ constexpr int kSwizzle = F_SWIZZLE;
 
 inline unsigned grid_for(int n, int flags) { return (flags & F_SWIZZLE) ? unsigned((n + 7) & ~7) : unsigned(n); }
 
-// Base face coefficients of a tile at its own level's subtree view (the
-// device computes the same values the same way in stage_tile).
-void slot_coefficients(const TileRec& T, float bc[7]) {
-  const float ooh = 1.0f / T.h;
-  const float cll = ooh * ooh;
-  const float clg = ((2.0f * ooh) / 3.0f) * ooh;
-  const uint8_t to = subtree_type(T.type, T.level, T.level);
-  bc[6] = base_coeff(to, to, cll, clg);
-  for (int f = 0; f < 6; f++) {
-    const uint8_t tn = (T.nbr[f] < 0) ? to : subtree_type(T.nbr_type[f], T.level, T.level);
-    bc[f] = base_coeff(to, tn, cll, clg);
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Problem construction
 // ---------------------------------------------------------------------------
@@ -209,7 +195,7 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
       q[f % 3] += (f < 3) ? -1 : 1;
       int nb = find(l, q[0], q[1], q[2]);
       T.nbr[f] = nb;
-      T.nbr_type[f] = nb >= 0 ? ty[nb] : 0;
+      T.ntypes |= uint32_t(nb >= 0 ? ty[nb] : 0) << (4 * f);
     }
     T.parent = -1;
     if (l > 0) {
@@ -219,14 +205,13 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
       if (ty[i] != GHOST && ty[p] != NONLEAF) fail(CG_ERR_INVALID, "a LEAF/NONLEAF tile's parent must be a NONLEAF");
       T.parent = p;
     }
-    T.octant = uint8_t((cd[i][0] & 1) | ((cd[i][1] & 1) << 1) | ((cd[i][2] & 1) << 2));
+    const uint32_t octant = uint32_t((cd[i][0] & 1) | ((cd[i][1] & 1) << 1) | ((cd[i][2] & 1) << 2));
     T.prob = 0;
     T.leaf_slot = leaf_slot[i];
     T.gslot = gslot[i];
     T.self = i;
     T.h = float(d->h0 / double(1u << l));
-    T.level = uint8_t(l);
-    T.type = ty[i];
+    T.meta = uint32_t(l) | (uint32_t(ty[i]) << 8) | (octant << 16);
     for (int o = 0; o < 8; o++) { T.gch[o] = -1; T.gcs[o] = -1; }
     if (ty[i] != GHOST) {
       for (int o = 0; o < 8; o++) {
@@ -237,11 +222,14 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
           if (ty[ch] != GHOST) fail(CG_ERR_INVALID, "a LEAF tile can only have GHOST children");
           T.gch[o] = ch;
           T.gcs[o] = gslot[ch];
-          T.has_gchild = 1;
+          T.meta |= 1u << 24;
         }
       }
     }
   }
+
+  // smoothing-view coefficients (subtree at each tile's own level)
+  for (auto& T : P.tiles) view_coeffs(T, tr_level(T), T.bc);
 
   // worklists (canonical order)
   auto& W = P.lists;
@@ -254,7 +242,7 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
     return dl;
   };
   P.leaf = add_list([&](int i) { return ty[i] == LEAF; });
-  P.gather = add_list([&](int i) { return ty[i] == LEAF && P.tiles[i].has_gchild; });
+  P.gather = add_list([&](int i) { return ty[i] == LEAF && tr_has_gchild(P.tiles[i]); });
   P.applyl = add_list([&](int i) { return ty[i] == LEAF || ty[i] == GHOST; });
   P.ghosts = add_list([&](int i) { return ty[i] == GHOST; });
   P.all = add_list([&](int) { return true; });
@@ -266,9 +254,14 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
   }
 
   // Coarse levels kept in LDS by one workgroup: the deepest C whose levels
-  // 0..C hold at most CL_MAX_TILES tiles in total (-1 if level 0 alone is larger).
+  // 0..C hold at most CL_MAX_TILES tiles in total, at most CL_MAX_PER_LEVEL on
+  // any level, and no GHOST tile (-1 if level 0 alone is larger).  (A GHOST
+  // tile needs a LEAF tile one level up beside a NONLEAF one, i.e. at least 14
+  // tiles on two levels, so the size bound alone already excludes them; the
+  // check keeps the kernel's assumption explicit.)
   int C = -1, total = 0;
-  while (C + 1 <= P.max_level && C + 1 < CL_MAX_LEVELS && total + P.lvl[C + 1].n <= CL_MAX_TILES) {
+  while (C + 1 <= P.max_level && C + 1 < CL_MAX_LEVELS && total + P.lvl[C + 1].n <= CL_MAX_TILES &&
+         P.lvl[C + 1].n <= CL_MAX_PER_LEVEL && P.lvl[C + 1].n == P.act[C + 1].n) {
     C++;
     total += P.lvl[C].n;
   }
@@ -295,11 +288,11 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
       const TileRec& T = P.tiles[cs.tile];
       for (int f = 0; f < 6; f++) cs.nbr[f] = T.nbr[f] < 0 ? -1 : slot_of[T.nbr[f]];
       cs.parent = T.parent < 0 ? -1 : slot_of[T.parent];
-      cs.level = T.level;
+      cs.level = tr_level(T);
       cs.h = T.h;
-      cs.type = T.type;
-      cs.octant = T.octant;
-      slot_coefficients(T, cs.bc);
+      cs.type = tr_type(T);
+      cs.octant = uint8_t(tr_octant(T));
+      for (int f = 0; f < 7; f++) cs.bc[f] = T.bc[f];
     }
   }
 
@@ -371,7 +364,6 @@ void launch_gs(cg_problem& P, const Ctx& c, int l, float* Z, const float* R, int
   if (a.n == 0) return;
   k_gs_color<<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
 }
-
 // One V-cycle Z <- M^-1 R (GMGSolver::VCycle, GMGSolver.cu:383-447).  The
 // V-cycle rhs of LEAF tiles (rhsv) must hold R on entry.  The zero initial
 // guess and the zero ghost values of the reference's init pass are implicit in
@@ -482,7 +474,7 @@ const TileRec* tmp_list(cg_problem& P, const std::vector<int>& tiles) {
 
 double dot_types(cg_problem& P, const float* A, const float* B, int types) {
   std::vector<int> l;
-  for (int i = 0; i < P.n; i++) if (P.tiles[i].type & types) l.push_back(i);
+  for (int i = 0; i < P.n; i++) if (tr_type(P.tiles[i]) & types) l.push_back(i);
   if (l.empty()) return 0.0;
   const TileRec* dl = tmp_list(P, l);
   const Ctx c = P.ctx();
@@ -753,8 +745,8 @@ cg_status cg_negative_laplacian_same_level(cg_problem* p, int32_t level, int32_t
     std::vector<int> l;
     for (int i = 0; i < p->n; i++) {
       const TileRec& T = p->tiles[i];
-      if (level >= 0 && T.level != level) continue;
-      if (subtree_type(T.type, T.level, subtree_level) & tile_types) l.push_back(i);
+      if (level >= 0 && tr_level(T) != level) continue;
+      if (subtree_type(tr_type(T), tr_level(T), subtree_level) & tile_types) l.push_back(i);
     }
     if (l.empty()) return;
     const TileRec* dl = tmp_list(*p, l);
@@ -857,6 +849,7 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
       fail(CG_ERR_INVALID, "level out of range");
     if ((kernel == CG_KERNEL_RESTRICT || kernel == CG_KERNEL_PROLONG) && (level < 1 || level > P.max_level))
       fail(CG_ERR_INVALID, "level out of range");
+    if ((kernel == CG_KERNEL_COARSE || kernel == CG_KERNEL_EMPTY) && level < 0) fail(CG_ERR_INVALID, "negative count");
     set_prob_state(P, 0.0, 1 << 30, 0);
     const Ctx c = P.ctx();
     const int base = F_VCYCLE | kSwizzle | F_CHECK_DONE;
@@ -896,7 +889,7 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
           break;
         case CG_KERNEL_COARSE: {
           nt = P.coarse.nslots;
-          VCycleArgs va{2, 10, 0, 2.f / 3.f, 1.f, 1.f};
+          VCycleArgs va{2, level, 0, 2.f / 3.f, 1.f, 1.f};  // `level` = bottom iterations
           if (P.coarse.C >= 0) k_coarse_lds<<<1, CL_THREADS, 0, P.stream>>>(c, P.coarse, va, c.z, base & ~F_SWIZZLE);
           break;
         }

@@ -156,7 +156,7 @@ cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel);
  * CG_KERNEL_APPLY (CG operator on LEAF+GHOST tiles), CG_KERNEL_RESTRICT and
  * CG_KERNEL_PROLONG (from/to `level`), CG_KERNEL_JACOBI (one sweep of `level`),
  * CG_KERNEL_UPDATE_XR (x/r update and |r|^2), CG_KERNEL_COARSE (the LDS-resident
- * coarse levels of a V-cycle, RB-GS 2+2), CG_KERNEL_EMPTY (an empty kernel of
+ * coarse levels of a V-cycle, RB-GS 2+2, `level` bottom iterations), CG_KERNEL_EMPTY (an empty kernel of
  * `level` workgroups: the launch floor), CG_KERNEL_COPY (copy of one channel
  * over all tiles: 4096 bytes moved per tile, the reference point that scales
  * hardware byte counters).  Channels are overwritten. */
