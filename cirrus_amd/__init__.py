@@ -29,10 +29,11 @@ NUM_CHANNELS = 6
 SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
 SCENE_UNIFORM, SCENE_SPHERE_BAND, SCENE_PLANE, SCENE_VORTEX = 0, 1, 2, 3
 (KERNEL_GS_SWEEP, KERNEL_APPLY, KERNEL_RESTRICT, KERNEL_PROLONG, KERNEL_JACOBI, KERNEL_UPDATE_XR, KERNEL_COARSE,
- KERNEL_EMPTY, KERNEL_COPY) = range(9)
+ KERNEL_EMPTY, KERNEL_COPY, KERNEL_GS_PROLONG) = range(10)
 
 EXPORTED_SYMBOLS = (
-    "cg_params_default", "cg_last_error", "cg_device_count", "cg_problem_create", "cg_problem_destroy", "cg_problem_info",
+    "cg_params_default", "cg_last_error", "cg_device_count", "cg_problem_create", "cg_problem_create_batch",
+    "cg_problem_num_problems", "cg_solve_batch", "cg_problem_destroy", "cg_problem_info",
     "cg_problem_topology", "cg_set_channel", "cg_get_channel", "cg_solve", "cg_damped_jacobi_solve",
     "cg_vcycle", "cg_full_negative_laplacian", "cg_negative_laplacian_same_level", "cg_gauss_seidel_color",
     "cg_jacobi_sweep", "cg_residual_restrict", "cg_prolongate", "cg_dot", "cg_recenter_leaf_voxels",
@@ -93,6 +94,9 @@ def lib():
             "cg_last_error": (ctypes.c_char_p, []),
             "cg_device_count": (i32, []),
             "cg_problem_create": (i32, [P(TreeDesc), P(vp)]),
+            "cg_problem_create_batch": (i32, [P(TreeDesc), i32, P(vp)]),
+            "cg_problem_num_problems": (i32, [vp, P(i32)]),
+            "cg_solve_batch": (i32, [vp, P(Params), P(Result)]),
             "cg_problem_destroy": (i32, [vp]),
             "cg_problem_info": (i32, [vp, P(i32), P(i32), P(i32)]),
             "cg_problem_topology": (i32, [vp, vp, vp]),
@@ -232,17 +236,33 @@ class Scene:
 
 
 class Problem:
-    """A device-resident hierarchy (cg_problem); arrays are in the tree's tile order."""
+    """A device-resident hierarchy (cg_problem); arrays are in the tree's tile order.
 
-    def __init__(self, tree):
-        self.tree = tree
+    ``Problem.batch(trees)`` holds several independent trees in one handle
+    (solved together by :meth:`GMGSolver.solve_batch`); its arrays concatenate
+    the trees in order, and :meth:`split` cuts such an array per tree."""
+
+    def __init__(self, tree, _batch=None):
+        trees = [tree] if _batch is None else list(_batch)
+        self.tree = trees[0]
+        self.trees = trees
         h = ctypes.c_void_p()
-        d = tree.desc()
-        _check(lib().cg_problem_create(ctypes.byref(d), ctypes.byref(h)), "cg_problem_create")
+        descs = (TreeDesc * len(trees))(*[t.desc() for t in trees])
+        _check(lib().cg_problem_create_batch(descs, len(trees), ctypes.byref(h)), "cg_problem_create")
         self._h = h
         n, nl, ml = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         lib().cg_problem_info(h, ctypes.byref(n), ctypes.byref(nl), ctypes.byref(ml))
         self.n, self.n_leaf, self.max_level = n.value, nl.value, ml.value
+        self.num_problems = len(trees)
+        self.offsets = np.cumsum([0] + [t.n for t in trees])
+
+    @classmethod
+    def batch(cls, trees):
+        return cls(None, _batch=trees)
+
+    def split(self, a):
+        """Per-tree views of a [tiles, ...] array of this handle."""
+        return [a[self.offsets[q]:self.offsets[q + 1]] for q in range(self.num_problems)]
 
     def close(self):
         if getattr(self, "_h", None):
@@ -355,6 +375,20 @@ class GMGSolver:
                omega=2.0 / 3.0):
         _check(lib().cg_vcycle(prob.handle, x_channel, f_channel, level_iters, coarsest_iters, smoother, omega),
                "GMGSolver.VCycle")
+
+    def solve_batch(self, prob, max_iters=1000, relative_tolerance=1e-6, level_iters=1, coarsest_iters=10,
+                    sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, use_graph=True):
+        """All problems of a batched :class:`Problem`; returns one result dict
+        per problem (iters, converged, rel_res, bb, rr, seconds)."""
+        p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
+                           coarsest_iters=coarsest_iters, smoother=smoother, omega=omega,
+                           sync_stride=int(sync_stride), pure_neumann=int(is_pure_neumann),
+                           one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff,
+                           use_graph=int(use_graph))
+        res = (Result * prob.num_problems)()
+        _check(lib().cg_solve_batch(prob.handle, ctypes.byref(p), res), "GMGSolver.solve_batch")
+        self.last_batch = [r.as_dict() for r in res]
+        return self.last_batch
 
     def dampedJacobiSolve(self, prob, verbose=False, max_iters=100, relative_tolerance=1e-6, omega=2.0 / 3.0):
         r = Result()

@@ -3,7 +3,7 @@
 // Layout in HBM (structure of arrays, one problem or many in one pool):
 //   channel c of tile t, cell (i,j,k):  ch[c][t*512 + (i<<6 | j<<3 | k)]  (FP32)
 //   cell types:                        ctype[t*512 + ...]                (u8)
-//   worklists:                          arrays of 160-byte TileRec (one per tile
+//   worklists:                          arrays of 176-byte TileRec (one per tile
 //                                       and phase), so a workgroup finds its
 //                                       tile, neighbours and children in one
 //                                       scalar load
@@ -43,12 +43,12 @@ struct alignas(16) TileRec {
   uint32_t ntypes;    // type of face neighbour f in bits 4f..4f+3 (0: none)
   float bc[7];        // face coefficients of the smoothing view (subtree at the
                       // tile's own level): faces x-,y-,z-,x+,y+,z+, then inside
-  int32_t pad0;
+  int32_t nbr_parent[6];  // parent tile of face neighbour f (-1: no neighbour or level 0)
   int32_t gch[8];     // LEAF: ghost child per octant (-1: none)
   int32_t gcs[8];     // ... and its ghost slot
-  int32_t pad1[2];
+  int32_t pad1;
 };
-static_assert(sizeof(TileRec) == 160, "TileRec is 160 bytes");
+static_assert(sizeof(TileRec) == 176, "TileRec is 176 bytes");
 // Fields packed in dwords so that a workgroup fetches them with scalar loads.
 __host__ __device__ __forceinline__ int tr_level(const TileRec& T) { return int(T.meta & 0xff); }
 __host__ __device__ __forceinline__ uint8_t tr_type(const TileRec& T) { return uint8_t((T.meta >> 8) & 0xff); }
@@ -78,6 +78,7 @@ struct Ctx {
   float* zb;     // damped-Jacobi ping-pong buffer
   float* gsum;   // 64 per ghost: 2x2x2 sums of the ghost rows of A p
   ProbState* ps;
+  int32_t* dflag; // per-problem copy of ProbState::done, polled by the host
   double* part;  // per-leaf-slot dot partials
   double* part2; // second partial stream (mean counts)
 };
@@ -111,6 +112,11 @@ __device__ __forceinline__ void store8(float* __restrict__ p, const float v[8]) 
 __device__ __forceinline__ uint64_t load_ct8(const uint8_t* __restrict__ p) {
   return *reinterpret_cast<const uint64_t*>(p);
 }
+
+// A wavefront-uniform value (keeps record fields in scalar registers).
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// v[g] for a lane-dependent g in 0..3, as register selects
+__device__ __forceinline__ int pick4(int g, int a, int b, int c, int d) { return g == 0 ? a : (g == 1 ? b : (g == 2 ? c : d)); }
 
 // Wavefront sum of a double in the fixed xor-butterfly order (masks 32..1);
 // every lane ends with the same bits.
@@ -194,7 +200,7 @@ __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const
   if (lane < 32) {
     const int g = lane >> 3, t = lane & 7;
     const int f = (g == 0) ? 0 : (g == 1) ? 3 : (g == 2) ? 1 : 4;
-    const int n = (g == 0) ? T.nbr[0] : (g == 1) ? T.nbr[3] : (g == 2) ? T.nbr[1] : T.nbr[4];
+    const int n = pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
     const int src = (g == 0) ? (56 + t) : (g == 1) ? t : (g == 2) ? (t * 8 + 7) : (t * 8);
     const int dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
     const bool gh = tr_ntype(T, f) == T_GHOST;
@@ -502,6 +508,128 @@ __device__ __forceinline__ void tile_prolong(const Ctx& c, const TileRec& T, int
   }
 }
 
+// Prolongation fused into the first colour pass of the up-sweep on a level
+// (GMGSolver.cu:345-376 then :216-269): every value the pass reads -- own cells
+// and the face halos -- is formed on the fly as the prolongation would leave
+// it (x + pcoef * parent on INTERIOR cells of active tiles, the parent value on
+// INTERIOR cells of GHOST tiles, 0 elsewhere), so no separate launch is needed.
+// XD holds the down-sweep values, XO receives the level's result; GHOST tiles
+// in the list only store their prolongated values.
+__device__ __forceinline__ float4 parent_quad(const float* XC, int parent, int oct, int i, int j) {
+  const int ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
+  return *reinterpret_cast<const float4*>(XC + size_t(parent) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) +
+                                         oz * 4);
+}
+// prolongated values of one pencil: active (fma with the down-sweep values) or ghost
+__device__ __forceinline__ void prolong_pencil(float4 xa, float4 xb, float4 pq, uint64_t ct, bool ghost, float pcoef,
+                                               float4& oa, float4& ob) {
+  const float cz[4] = {pq.x, pq.y, pq.z, pq.w};
+  float xo[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
+  float o[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const bool in = ct_at(ct, k) & C_INT;
+    o[k] = in ? (ghost ? cz[k >> 1] : __builtin_fmaf(pcoef, cz[k >> 1], xo[k])) : 0.f;
+  }
+  oa = make_float4(o[0], o[1], o[2], o[3]);
+  ob = make_float4(o[4], o[5], o[6], o[7]);
+}
+
+// Prolongated k-halo value of lane (i,j): cell (i,j,KN) of face neighbour F.
+template <int F, int KN>
+__device__ __forceinline__ void khalo_prolong(const Ctx& c, const TileRec& T, const float* XD, const float* XO, float pcoef,
+                                              int oct, int lane, float& xv, uint8_t& cv) {
+  const int n = T.nbr[F];
+  if (n < 0) { xv = 0.f; cv = C_DIR; return; }
+  const int i = lane >> 3, j = lane & 7;
+  const size_t o = size_t(n) * 512 + lane * 8 + KN;
+  cv = c.ctype[o];
+  const bool gh = tr_ntype(T, F) == T_GHOST;
+  const int noct = oct ^ 4;
+  const int ox = noct & 1, oy = (noct >> 1) & 1, oz = (noct >> 2) & 1;
+  const float pz = XO[size_t(T.nbr_parent[F]) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) + oz * 4 +
+                      (KN >> 1)];
+  const float xo = gh ? 0.f : XD[o];
+  xv = (cv & C_INT) ? (gh ? pz : __builtin_fmaf(pcoef, pz, xo)) : 0.f;
+}
+
+__device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, const float* XD, float* XO, float pcoef,
+                                                int color, int flags, TileLDS& L) {
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
+  const int oct = tr_octant(T);
+  const bool own_ghost = tr_type(T) == T_GHOST;
+  const uint64_t ct = load_ct8(c.ctype + base);
+  const float4 pq = parent_quad(XO, T.parent, oct, i, j);
+  float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), xb = xa;
+  if (!own_ghost) {
+    const float4* q = reinterpret_cast<const float4*>(XD + base);
+    xa = q[0];
+    xb = q[1];
+  }
+  float4 oa, ob;
+  prolong_pencil(xa, xb, pq, ct, own_ghost, pcoef, oa, ob);
+  if (own_ghost) {  // GHOST tile: its prolongated values only
+    if (prob_done(c, T, flags)) return;
+    float4* dst = reinterpret_cast<float4*>(XO + base);
+    dst[0] = oa;
+    dst[1] = ob;
+    return;
+  }
+  Stencil S;
+  S.own[0] = oa.x; S.own[1] = oa.y; S.own[2] = oa.z; S.own[3] = oa.w;
+  S.own[4] = ob.x; S.own[5] = ob.y; S.own[6] = ob.z; S.own[7] = ob.w;
+  S.ct = ct;
+  const int pc = (i + 1) * 10 + (j + 1);
+  L.xs[0][pc] = oa;
+  L.xs[1][pc] = ob;
+  L.cs[pc] = ct;
+  // halo ring, lanes 0..31 (as stage_tile), prolongated in the neighbour's frame
+  if (lane < 32) {
+    const int g = lane >> 3, t = lane & 7;
+    const int f = (g == 0) ? 0 : (g == 1) ? 3 : (g == 2) ? 1 : 4;
+    // the record's fields as uniform values, so that the per-lane choice below is
+    // a register select and not an indexed access to the record
+    const int n = pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
+    const int pn = pick4(g, uni(T.nbr_parent[0]), uni(T.nbr_parent[3]), uni(T.nbr_parent[1]), uni(T.nbr_parent[4]));
+    const int ni = (g == 0) ? 7 : (g == 1) ? 0 : t, nj = (g == 2) ? 7 : (g == 3) ? 0 : t;  // pencil in the neighbour
+    const int dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
+    const int noct = oct ^ (1 << (f % 3));
+    float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;
+    uint64_t w = 0x0101010101010101ull;  // DIRICHLET outside the grid (GMGSolver.cu:120-122)
+    if (n >= 0) {
+      const size_t nb = size_t(n) * 512 + (ni * 8 + nj) * 8;
+      const bool gh = tr_ntype(T, f) == T_GHOST;
+      w = load_ct8(c.ctype + nb);
+      float4 ya = make_float4(0.f, 0.f, 0.f, 0.f), yb = ya;
+      if (!gh) {
+        const float4* q = reinterpret_cast<const float4*>(XD + nb);
+        ya = q[0];
+        yb = q[1];
+      }
+      prolong_pencil(ya, yb, parent_quad(XO, pn, noct, ni, nj), w, gh, pcoef, va, vb);
+    }
+    L.xs[0][dst] = va;
+    L.xs[1][dst] = vb;
+    L.cs[dst] = w;
+  }
+  // k halos: cell (i,j,7) of the z- neighbour and (i,j,0) of the z+ neighbour
+  khalo_prolong<2, 7>(c, T, XD, XO, pcoef, oct, lane, S.xkm, S.ckm);
+  khalo_prolong<5, 0>(c, T, XD, XO, pcoef, oct, lane, S.xkp, S.ckp);
+  S.h = T.h;
+  S.b_int = T.bc[6];
+  for (int f = 0; f < 6; f++) S.b_f[f] = T.bc[f];
+  float b[8];
+  load8(c.rhsv + base, b);
+  if (prob_done(c, T, flags)) return;
+  wave_sync();
+  float4 out0, out1;
+  gs_pencil(lds_nbrs(L), S, b, color, out0, out1);
+  float4* dst = reinterpret_cast<float4*>(XO + base);
+  dst[0] = out0;
+  dst[1] = out1;
+}
+
 // ---------------------------------------------------------------------------
 // One-tile-per-workgroup kernels over a worklist
 // ---------------------------------------------------------------------------
@@ -515,6 +643,12 @@ __global__ __launch_bounds__(64) void k_gs_color(Ctx c, const TileRec* __restric
   TILE_KERNEL_PROLOGUE
   __shared__ TileLDS L;
   tile_gs(c, T, level, X, F, color, flags, L);
+}
+__global__ __launch_bounds__(64) void k_gs_prolong(Ctx c, const TileRec* __restrict__ list, int n, const float* XD,
+                                                   float* XO, float pcoef, int color, int flags) {
+  TILE_KERNEL_PROLOGUE
+  __shared__ TileLDS L;
+  tile_gs_prolong(c, T, XD, XO, pcoef, color, flags, L);
 }
 __global__ __launch_bounds__(64) void k_jacobi(Ctx c, const TileRec* __restrict__ list, int n, int level, const float* X,
                                                const float* XG, float* Y, const float* F, float omega, int flags) {
@@ -561,6 +695,7 @@ struct alignas(16) CoarseSlot {
 };
 static_assert(sizeof(CoarseSlot) == 72 || sizeof(CoarseSlot) == 80, "CoarseSlot layout");
 struct CoarsePlan {
+  int prob;                           // problem index in the pool
   int C;                              // deepest level held in LDS
   int nslots;
   int lvl_begin[CL_MAX_LEVELS + 1];   // slots sorted by level
@@ -637,10 +772,14 @@ __device__ __forceinline__ float cl_ax(const CoarseLDS& M, const CellPrep& P, fl
 // prepared rows in registers.  One launch replaces ~10 launches per coarse
 // level; the per-cell arithmetic is that of the tile kernels
 // (GMGSolver.cu:216-376, bottom :432-433).
-__global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, CoarsePlan cp, VCycleArgs v, float* Z, int flags) {
+__global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, const CoarsePlan* __restrict__ plans, VCycleArgs v,
+                                                           float* Z, int flags) {
   __shared__ CoarseLDS M;
-  if ((flags & F_CHECK_DONE) && c.ps[0].done) return;
+  __shared__ CoarsePlan cp;  // one workgroup per problem; in LDS, as its level table is indexed per level
   const int tid = threadIdx.x;
+  if (tid == 0) cp = plans[blockIdx.x];
+  __syncthreads();
+  if ((flags & F_CHECK_DONE) && c.ps[cp.prob].done) return;
   if (tid < cp.nslots) M.S[tid] = cp.slots[tid];
   if (tid < 4) M.z[CL_ZERO + tid] = 0.f;
   __syncthreads();
@@ -977,6 +1116,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctx c, int mode, int f
       ps->converged = (s == 0.0) ? 1 : 0;
       ps->done = (s == 0.0 || ps->max_iters <= 0) ? 1 : 0;
       ps->gamma = 0.0;
+      c.dflag[blockIdx.x] = ps->done;
       break;
     }
     case FIN_RZ:  // gamma_old = gamma; gamma = r.z; beta = gamma / gamma_old (GMGSolver.cu:629-638)
@@ -995,6 +1135,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctx c, int mode, int f
       const int cs = ps->check_stride;
       if (cs > 0 && it % cs == 0 && s < ps->thr) { ps->converged = 1; ps->done = 1; }
       else if (it >= ps->max_iters) ps->done = 1;
+      c.dflag[blockIdx.x] = ps->done;
       break;
     }
     case FIN_MEAN:

@@ -62,16 +62,24 @@ struct DList {
 
 using namespace cg;
 
+// A pool of one or more independent problems (octrees) solved together: tiles
+// of all problems share the channel arrays, each worklist spans all problems,
+// and every problem has its own ProbState, leaf-slot range and coarse plan.
 struct cg_problem {
-  int n = 0;  // tiles
+  int nprob = 0;
+  int n = 0;  // tiles of all problems
   int max_level = 0;
   int n_leaf = 0, n_ghost = 0;
-  std::vector<int> ext_of;     // internal -> caller index
-  std::vector<TileRec> tiles;  // one record per tile (internal order)
-  std::vector<TileRec> lists;  // all worklists, concatenated
+  std::vector<int> ext_of;       // internal -> caller index (problems concatenated)
+  std::vector<int> leaf_begin;   // per problem: first leaf slot
+  std::vector<int> leaf_count;   // per problem: LEAF tiles
+  std::vector<int> coarse_C;     // per problem: deepest LDS-resident level (-1: none)
+  std::vector<TileRec> tiles;    // one record per tile (internal order)
+  std::vector<TileRec> lists;    // all worklists, concatenated
   DList leaf, gather, applyl, ghosts, all;
-  std::vector<DList> act, lvl;
-  CoarsePlan coarse;  // levels 0..coarse.C resident in LDS (C = -1: none)
+  std::vector<DList> act, lvl;   // per level, all problems (operator API)
+  std::vector<DList> actg, lvlg; // per level, tiles the V-cycle smooths with grid-wide launches
+  std::vector<CoarsePlan> plans; // LDS-resident coarse levels, one per problem with C >= 0
   std::vector<CoarseSlot> coarse_slots;
 
   // device
@@ -80,6 +88,8 @@ struct cg_problem {
   float* d_zb = nullptr;
   float* d_gsum = nullptr;
   ProbState* d_ps = nullptr;
+  int32_t* d_dflag = nullptr;
+  CoarsePlan* d_plans = nullptr;
   double* d_part = nullptr;
   double* d_part2 = nullptr;
   double* d_scalar = nullptr;
@@ -106,6 +116,7 @@ struct cg_problem {
     c.zb = d_zb;
     c.gsum = d_gsum;
     c.ps = d_ps;
+    c.dflag = d_dflag;
     c.part = d_part;
     c.part2 = d_part2;
     return c;
@@ -118,7 +129,7 @@ cg_problem::~cg_problem() {
   if (graph) (void)hipGraphExecDestroy(graph);
   for (auto e : ev) if (e) (void)hipEventDestroy(e);
   if (h_pin) (void)hipHostFree(h_pin);
-  void* ptrs[] = {d_ctype, d_zb, d_gsum, d_ps, d_part, d_part2, d_scalar, d_lists, d_tmp_list, d_coarse};
+  void* ptrs[] = {d_ctype, d_zb, d_gsum, d_ps, d_dflag, d_plans, d_part, d_part2, d_scalar, d_lists, d_tmp_list, d_coarse};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
   if (stream) (void)hipStreamDestroy(stream);
@@ -133,105 +144,164 @@ inline unsigned grid_for(int n, int flags) { return (flags & F_SWIZZLE) ? unsign
 // ---------------------------------------------------------------------------
 // Problem construction
 // ---------------------------------------------------------------------------
-void build_problem(const cg_tree_desc* d, cg_problem& P) {
+// One tree of the pool: validation, canonical order (level, then Morton code of
+// the tile coordinate), neighbour / parent / ghost-child links and tile records.
+// Indices in the records are global (offset by `base`).
+void build_tree(const cg_tree_desc* d, int prob, int base, int leaf_base, int ghost_base, cg_problem& P,
+                std::vector<int>& lev, std::vector<uint8_t>& ty, int& nleaf, int& nghost) {
   if (!d || d->num_tiles <= 0 || !d->level || !d->coord || !d->type || !d->cell_type)
     fail(CG_ERR_INVALID, "cg_problem_create: empty or incomplete tree description");
   if (!(d->h0 > 0)) fail(CG_ERR_INVALID, "cg_problem_create: h0 must be positive");
   const int n = d->num_tiles;
-  P.n = n;
   std::vector<uint64_t> mk(n);
   for (int t = 0; t < n; t++) {
     int l = d->level[t];
     const int32_t* c = d->coord + 3 * t;
     if (l < 0 || l > 30) fail(CG_ERR_INVALID, "tile " + std::to_string(t) + ": level out of range");
     if (!key_ok(c[0], c[1], c[2])) fail(CG_ERR_INVALID, "tile " + std::to_string(t) + ": coordinate out of range");
-    uint8_t ty = d->type[t];
-    if (ty != LEAF && ty != NONLEAF && ty != GHOST) fail(CG_ERR_INVALID, "tile " + std::to_string(t) + ": bad tile type");
+    uint8_t tt = d->type[t];
+    if (tt != LEAF && tt != NONLEAF && tt != GHOST) fail(CG_ERR_INVALID, "tile " + std::to_string(t) + ": bad tile type");
     mk[t] = morton3(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]));
   }
-  // canonical order: level, then Morton code of the tile coordinate
-  P.ext_of.resize(n);
-  for (int t = 0; t < n; t++) P.ext_of[t] = t;
-  std::sort(P.ext_of.begin(), P.ext_of.end(), [&](int a, int b) {
+  std::vector<int> order(n);
+  for (int t = 0; t < n; t++) order[t] = t;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
     if (d->level[a] != d->level[b]) return d->level[a] < d->level[b];
     return mk[a] < mk[b];
   });
   std::unordered_map<uint64_t, int> idx;
   idx.reserve(size_t(n) * 2);
-  std::vector<int> lev(n);
   std::vector<std::array<int32_t, 3>> cd(n);
-  std::vector<uint8_t> ty(n);
   for (int i = 0; i < n; i++) {
-    int e = P.ext_of[i];
-    lev[i] = d->level[e];
+    const int e = order[i];
+    P.ext_of[base + i] = base + e;
+    lev[base + i] = d->level[e];
     cd[i] = {d->coord[3 * e], d->coord[3 * e + 1], d->coord[3 * e + 2]};
-    ty[i] = d->type[e];
-    if (!idx.emplace(tile_key(lev[i], cd[i][0], cd[i][1], cd[i][2]), i).second)
-      fail(CG_ERR_INVALID, "duplicate tile at level " + std::to_string(lev[i]));
+    ty[base + i] = d->type[e];
+    if (!idx.emplace(tile_key(lev[base + i], cd[i][0], cd[i][1], cd[i][2]), i).second)
+      fail(CG_ERR_INVALID, "duplicate tile at level " + std::to_string(lev[base + i]));
   }
   auto find = [&](int l, int x, int y, int z) -> int {
     if (l < 0 || !key_ok(x, y, z)) return -1;
     auto it = idx.find(tile_key(l, x, y, z));
     return it == idx.end() ? -1 : it->second;
   };
-  P.max_level = 0;
-  for (int i = 0; i < n; i++) P.max_level = std::max(P.max_level, lev[i]);
-  int nleaf = 0, nghost = 0;
   std::vector<int> leaf_slot(n, -1), gslot(n, -1);
+  nleaf = nghost = 0;
   for (int i = 0; i < n; i++) {
-    if (ty[i] == LEAF) leaf_slot[i] = nleaf++;
-    if (ty[i] == GHOST) gslot[i] = nghost++;
+    if (ty[base + i] == LEAF) leaf_slot[i] = leaf_base + nleaf++;
+    if (ty[base + i] == GHOST) gslot[i] = ghost_base + nghost++;
   }
-  P.n_leaf = nleaf;
-  P.n_ghost = nghost;
   if (nleaf == 0) fail(CG_ERR_INVALID, "tree has no LEAF tiles");
-  P.tiles.assign(n, TileRec());
   for (int i = 0; i < n; i++) {
-    TileRec& T = P.tiles[i];
+    TileRec& T = P.tiles[base + i];
     std::memset(&T, 0, sizeof(T));
-    const int l = lev[i];
+    const int l = lev[base + i];
+    const uint8_t my = ty[base + i];
     for (int f = 0; f < 6; f++) {
       int q[3] = {cd[i][0], cd[i][1], cd[i][2]};
       q[f % 3] += (f < 3) ? -1 : 1;
-      int nb = find(l, q[0], q[1], q[2]);
-      T.nbr[f] = nb;
-      T.ntypes |= uint32_t(nb >= 0 ? ty[nb] : 0) << (4 * f);
+      const int nb = find(l, q[0], q[1], q[2]);
+      T.nbr[f] = nb < 0 ? -1 : base + nb;
+      T.ntypes |= uint32_t(nb >= 0 ? ty[base + nb] : 0) << (4 * f);
     }
     T.parent = -1;
     if (l > 0) {
-      int p = find(l - 1, cd[i][0] >> 1, cd[i][1] >> 1, cd[i][2] >> 1);
+      const int p = find(l - 1, cd[i][0] >> 1, cd[i][1] >> 1, cd[i][2] >> 1);
       if (p < 0) fail(CG_ERR_INVALID, "tile at level " + std::to_string(l) + " has no parent tile");
-      if (ty[i] == GHOST && ty[p] != LEAF) fail(CG_ERR_INVALID, "a GHOST tile's parent must be a LEAF");
-      if (ty[i] != GHOST && ty[p] != NONLEAF) fail(CG_ERR_INVALID, "a LEAF/NONLEAF tile's parent must be a NONLEAF");
-      T.parent = p;
+      if (my == GHOST && ty[base + p] != LEAF) fail(CG_ERR_INVALID, "a GHOST tile's parent must be a LEAF");
+      if (my != GHOST && ty[base + p] != NONLEAF) fail(CG_ERR_INVALID, "a LEAF/NONLEAF tile's parent must be a NONLEAF");
+      T.parent = base + p;
     }
     const uint32_t octant = uint32_t((cd[i][0] & 1) | ((cd[i][1] & 1) << 1) | ((cd[i][2] & 1) << 2));
-    T.prob = 0;
+    T.prob = prob;
     T.leaf_slot = leaf_slot[i];
     T.gslot = gslot[i];
-    T.self = i;
+    T.self = base + i;
     T.h = float(d->h0 / double(1u << l));
-    T.meta = uint32_t(l) | (uint32_t(ty[i]) << 8) | (octant << 16);
+    T.meta = uint32_t(l) | (uint32_t(my) << 8) | (octant << 16);
     for (int o = 0; o < 8; o++) { T.gch[o] = -1; T.gcs[o] = -1; }
-    if (ty[i] != GHOST) {
+    if (my != GHOST) {
       for (int o = 0; o < 8; o++) {
-        int ch = find(l + 1, 2 * cd[i][0] + (o & 1), 2 * cd[i][1] + ((o >> 1) & 1), 2 * cd[i][2] + ((o >> 2) & 1));
-        if (ty[i] == NONLEAF && (ch < 0 || ty[ch] == GHOST))
+        const int ch = find(l + 1, 2 * cd[i][0] + (o & 1), 2 * cd[i][1] + ((o >> 1) & 1), 2 * cd[i][2] + ((o >> 2) & 1));
+        if (my == NONLEAF && (ch < 0 || ty[base + ch] == GHOST))
           fail(CG_ERR_INVALID, "a NONLEAF tile must have 8 LEAF/NONLEAF children");
-        if (ty[i] == LEAF && ch >= 0) {
-          if (ty[ch] != GHOST) fail(CG_ERR_INVALID, "a LEAF tile can only have GHOST children");
-          T.gch[o] = ch;
+        if (my == LEAF && ch >= 0) {
+          if (ty[base + ch] != GHOST) fail(CG_ERR_INVALID, "a LEAF tile can only have GHOST children");
+          T.gch[o] = base + ch;
           T.gcs[o] = gslot[ch];
           T.meta |= 1u << 24;
         }
       }
     }
+    // smoothing-view coefficients (subtree at the tile's own level)
+    view_coeffs(T, l, T.bc);
+  }
+  for (int i = 0; i < n; i++) {
+    TileRec& T = P.tiles[base + i];
+    for (int f = 0; f < 6; f++) T.nbr_parent[f] = T.nbr[f] < 0 ? -1 : P.tiles[T.nbr[f]].parent;
+  }
+}
+
+void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
+  if (!descs || nprob <= 0) fail(CG_ERR_INVALID, "cg_problem_create: no tree description");
+  P.nprob = nprob;
+  std::vector<int> tbase(nprob + 1, 0);
+  for (int q = 0; q < nprob; q++) {
+    if (descs[q].num_tiles <= 0) fail(CG_ERR_INVALID, "cg_problem_create: empty or incomplete tree description");
+    tbase[q + 1] = tbase[q] + descs[q].num_tiles;
+  }
+  const int n = tbase[nprob];
+  P.n = n;
+  P.ext_of.assign(n, 0);
+  P.tiles.assign(n, TileRec());
+  std::vector<int> lev(n);
+  std::vector<uint8_t> ty(n);
+  P.leaf_begin.assign(nprob, 0);
+  P.leaf_count.assign(nprob, 0);
+  P.coarse_C.assign(nprob, -1);
+  int leaf_total = 0, ghost_total = 0;
+  for (int q = 0; q < nprob; q++) {
+    int nl = 0, ng = 0;
+    build_tree(&descs[q], q, tbase[q], leaf_total, ghost_total, P, lev, ty, nl, ng);
+    P.leaf_begin[q] = leaf_total;
+    P.leaf_count[q] = nl;
+    leaf_total += nl;
+    ghost_total += ng;
+  }
+  P.n_leaf = leaf_total;
+  P.n_ghost = ghost_total;
+  P.max_level = 0;
+  for (int i = 0; i < n; i++) P.max_level = std::max(P.max_level, lev[i]);
+  const int L = P.max_level;
+
+  // Coarse levels kept in LDS by one workgroup per problem: the deepest C whose
+  // levels 0..C hold at most CL_MAX_TILES tiles in total, at most
+  // CL_MAX_PER_LEVEL on any level, and no GHOST tile (-1 if level 0 alone is
+  // larger).  (A GHOST tile needs a LEAF tile one level up beside a NONLEAF one,
+  // i.e. at least 14 tiles on two levels, so the size bound alone already
+  // excludes them; the check keeps the kernel's assumption explicit.)
+  std::vector<std::vector<int>> cnt(nprob, std::vector<int>(L + 1, 0)), cnt_act(nprob, std::vector<int>(L + 1, 0));
+  for (int i = 0; i < n; i++) {
+    const int q = P.tiles[i].prob;
+    cnt[q][lev[i]]++;
+    if (ty[i] != GHOST) cnt_act[q][lev[i]]++;
+  }
+  // CIRRUS_COARSE_LEVELS=k caps the LDS-resident levels at k (0: none), for measurement
+  int forced = CL_MAX_LEVELS;
+  if (const char* e = std::getenv("CIRRUS_COARSE_LEVELS")) forced = std::atoi(e) - 1;
+  for (int q = 0; q < nprob; q++) {
+    int C = -1, total = 0;
+    while (C + 1 <= L && C + 1 < CL_MAX_LEVELS && cnt[q][C + 1] > 0 && total + cnt[q][C + 1] <= CL_MAX_TILES &&
+           cnt[q][C + 1] <= CL_MAX_PER_LEVEL && cnt[q][C + 1] == cnt_act[q][C + 1]) {
+      C++;
+      total += cnt[q][C];
+    }
+    C = std::max(-1, std::min(C, forced));
+    P.coarse_C[q] = C;
   }
 
-  // smoothing-view coefficients (subtree at each tile's own level)
-  for (auto& T : P.tiles) view_coeffs(T, tr_level(T), T.bc);
-
-  // worklists (canonical order)
+  // worklists (canonical order, problems one after another)
   auto& W = P.lists;
   W.clear();
   auto add_list = [&](auto pred) {
@@ -241,50 +311,51 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
     dl.n = int(W.size()) - dl.off;
     return dl;
   };
+  auto grid_level = [&](int i) { return lev[i] > P.coarse_C[P.tiles[i].prob]; };
   P.leaf = add_list([&](int i) { return ty[i] == LEAF; });
   P.gather = add_list([&](int i) { return ty[i] == LEAF && tr_has_gchild(P.tiles[i]); });
   P.applyl = add_list([&](int i) { return ty[i] == LEAF || ty[i] == GHOST; });
   P.ghosts = add_list([&](int i) { return ty[i] == GHOST; });
   P.all = add_list([&](int) { return true; });
-  P.act.assign(P.max_level + 1, DList());
-  P.lvl.assign(P.max_level + 1, DList());
-  for (int l = 0; l <= P.max_level; l++) {
+  P.act.assign(L + 1, DList());
+  P.lvl.assign(L + 1, DList());
+  P.actg.assign(L + 1, DList());
+  P.lvlg.assign(L + 1, DList());
+  for (int l = 0; l <= L; l++) {
     P.act[l] = add_list([&](int i) { return lev[i] == l && ty[i] != GHOST; });
     P.lvl[l] = add_list([&](int i) { return lev[i] == l; });
+    P.actg[l] = add_list([&](int i) { return lev[i] == l && ty[i] != GHOST && grid_level(i); });
+    P.lvlg[l] = add_list([&](int i) { return lev[i] == l && grid_level(i); });
   }
 
-  // Coarse levels kept in LDS by one workgroup: the deepest C whose levels
-  // 0..C hold at most CL_MAX_TILES tiles in total, at most CL_MAX_PER_LEVEL on
-  // any level, and no GHOST tile (-1 if level 0 alone is larger).  (A GHOST
-  // tile needs a LEAF tile one level up beside a NONLEAF one, i.e. at least 14
-  // tiles on two levels, so the size bound alone already excludes them; the
-  // check keeps the kernel's assumption explicit.)
-  int C = -1, total = 0;
-  while (C + 1 <= P.max_level && C + 1 < CL_MAX_LEVELS && total + P.lvl[C + 1].n <= CL_MAX_TILES &&
-         P.lvl[C + 1].n <= CL_MAX_PER_LEVEL && P.lvl[C + 1].n == P.act[C + 1].n) {
-    C++;
-    total += P.lvl[C].n;
-  }
-  if (const char* e = std::getenv("CIRRUS_COARSE_LEVELS")) C = std::min(C, std::atoi(e) - 1);
-  std::memset(&P.coarse, 0, sizeof(P.coarse));
-  P.coarse.C = C;
+  // coarse plans and slots
+  P.plans.clear();
   P.coarse_slots.clear();
-  if (C >= 0) {
-    std::vector<int> slot_of(n, -1);
+  std::vector<int> slot_of(n, -1);
+  std::vector<std::vector<int>> by_level(L + 1);
+  for (int q = 0; q < nprob; q++) {
+    const int C = P.coarse_C[q];
+    if (C < 0) continue;
+    CoarsePlan cp;
+    std::memset(&cp, 0, sizeof(cp));
+    cp.prob = q;
+    cp.C = C;
+    const int first = int(P.coarse_slots.size());
     for (int l = 0; l <= C; l++) {
-      P.coarse.lvl_begin[l] = int(P.coarse_slots.size());
-      for (int q = 0; q < P.lvl[l].n; q++) {
-        int t = W[P.lvl[l].off + q].self;
-        slot_of[t] = int(P.coarse_slots.size());
+      cp.lvl_begin[l] = int(P.coarse_slots.size()) - first;
+      for (int i = tbase[q]; i < tbase[q + 1]; i++) {
+        if (lev[i] != l) continue;
+        slot_of[i] = int(P.coarse_slots.size()) - first;
         CoarseSlot cs;
         std::memset(&cs, 0, sizeof(cs));
-        cs.tile = t;
+        cs.tile = i;
         P.coarse_slots.push_back(cs);
       }
     }
-    P.coarse.lvl_begin[C + 1] = int(P.coarse_slots.size());
-    P.coarse.nslots = int(P.coarse_slots.size());
-    for (auto& cs : P.coarse_slots) {
+    cp.lvl_begin[C + 1] = int(P.coarse_slots.size()) - first;
+    cp.nslots = cp.lvl_begin[C + 1];
+    for (int s = first; s < int(P.coarse_slots.size()); s++) {
+      CoarseSlot& cs = P.coarse_slots[s];
       const TileRec& T = P.tiles[cs.tile];
       for (int f = 0; f < 6; f++) cs.nbr[f] = T.nbr[f] < 0 ? -1 : slot_of[T.nbr[f]];
       cs.parent = T.parent < 0 ? -1 : slot_of[T.parent];
@@ -294,6 +365,8 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
       cs.octant = uint8_t(tr_octant(T));
       for (int f = 0; f < 7; f++) cs.bc[f] = T.bc[f];
     }
+    cp.slots = reinterpret_cast<const CoarseSlot*>(size_t(first));  // offset, patched after upload
+    P.plans.push_back(cp);
   }
 
   // device buffers
@@ -306,27 +379,34 @@ void build_problem(const cg_tree_desc* d, cg_problem& P) {
   }
   HIPC(hipMalloc(&P.d_zb, cells * sizeof(float)));
   HIPC(hipMemsetAsync(P.d_zb, 0, cells * sizeof(float), P.stream));
-  size_t g64 = size_t(std::max(1, nghost)) * 64 * sizeof(float);
+  size_t g64 = size_t(std::max(1, ghost_total)) * 64 * sizeof(float);
   HIPC(hipMalloc(&P.d_gsum, g64));
   HIPC(hipMemsetAsync(P.d_gsum, 0, g64, P.stream));
-  HIPC(hipMalloc(&P.d_ps, sizeof(ProbState)));
-  HIPC(hipMemsetAsync(P.d_ps, 0, sizeof(ProbState), P.stream));
+  HIPC(hipMalloc(&P.d_ps, sizeof(ProbState) * nprob));
+  HIPC(hipMemsetAsync(P.d_ps, 0, sizeof(ProbState) * nprob, P.stream));
+  HIPC(hipMalloc(&P.d_dflag, sizeof(int32_t) * nprob));
+  HIPC(hipMemsetAsync(P.d_dflag, 0, sizeof(int32_t) * nprob, P.stream));
   size_t np = size_t(std::max(n, 64));
   HIPC(hipMalloc(&P.d_part, np * sizeof(double)));
   HIPC(hipMalloc(&P.d_part2, np * sizeof(double)));
   HIPC(hipMalloc(&P.d_scalar, 8 * sizeof(double)));
   HIPC(hipMalloc(&P.d_lists, sizeof(TileRec) * W.size()));
   HIPC(hipMemcpyAsync(P.d_lists, W.data(), sizeof(TileRec) * W.size(), hipMemcpyHostToDevice, P.stream));
-  if (!P.coarse_slots.empty()) {
+  if (!P.plans.empty()) {
     HIPC(hipMalloc(&P.d_coarse, sizeof(CoarseSlot) * P.coarse_slots.size()));
     HIPC(hipMemcpyAsync(P.d_coarse, P.coarse_slots.data(), sizeof(CoarseSlot) * P.coarse_slots.size(),
                         hipMemcpyHostToDevice, P.stream));
-    P.coarse.slots = P.d_coarse;
+    for (auto& cp : P.plans) cp.slots = P.d_coarse + size_t(cp.slots);
+    HIPC(hipMalloc(&P.d_plans, sizeof(CoarsePlan) * P.plans.size()));
+    HIPC(hipMemcpyAsync(P.d_plans, P.plans.data(), sizeof(CoarsePlan) * P.plans.size(), hipMemcpyHostToDevice, P.stream));
   }
   std::vector<uint8_t> ct(cells);
-  for (int i = 0; i < n; i++) std::memcpy(&ct[size_t(i) * 512], d->cell_type + size_t(P.ext_of[i]) * 512, 512);
+  for (int i = 0; i < n; i++) {
+    const int q = P.tiles[i].prob;
+    std::memcpy(&ct[size_t(i) * 512], descs[q].cell_type + size_t(P.ext_of[i] - tbase[q]) * 512, 512);
+  }
   HIPC(hipMemcpyAsync(P.d_ctype, ct.data(), cells, hipMemcpyHostToDevice, P.stream));
-  HIPC(hipHostMalloc(&P.h_pin, 2 * sizeof(int), hipHostMallocDefault));
+  HIPC(hipHostMalloc(&P.h_pin, 2 * sizeof(int) * nprob, hipHostMallocDefault));
   HIPC(hipEventCreateWithFlags(&P.ev[0], hipEventDisableTiming));
   HIPC(hipEventCreateWithFlags(&P.ev[1], hipEventDisableTiming));
   HIPC(hipStreamSynchronize(P.stream));
@@ -336,18 +416,22 @@ void check_channel(int ch) {
   if (ch < 0 || ch >= CG_NUM_CHANNELS) fail(CG_ERR_INVALID, "channel index out of range");
 }
 
-// upload the ProbState of a single-problem run (synchronous)
+// upload every problem's ProbState (synchronous)
 void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann, int check_stride = 1) {
-  ProbState s;
-  std::memset(&s, 0, sizeof(s));
-  s.tol2 = tol * tol;
-  s.max_iters = max_iters;
-  s.leaf_begin = 0;
-  s.leaf_count = P.n_leaf;
-  s.pure_neumann = pure_neumann;
-  s.check_stride = check_stride;
+  std::vector<ProbState> st(P.nprob);
+  for (int q = 0; q < P.nprob; q++) {
+    ProbState& s = st[q];
+    std::memset(&s, 0, sizeof(s));
+    s.tol2 = tol * tol;
+    s.max_iters = max_iters;
+    s.leaf_begin = P.leaf_begin[q];
+    s.leaf_count = P.leaf_count[q];
+    s.pure_neumann = pure_neumann;
+    s.check_stride = check_stride;
+  }
   HIPC(hipStreamSynchronize(P.stream));
-  HIPC(hipMemcpy(P.d_ps, &s, sizeof(s), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(P.d_ps, st.data(), sizeof(ProbState) * P.nprob, hipMemcpyHostToDevice));
+  HIPC(hipMemset(P.d_dflag, 0, sizeof(int32_t) * P.nprob));
 }
 
 // ---------------------------------------------------------------------------
@@ -359,8 +443,7 @@ struct VParams {
   int check_done = 1;
 };
 
-void launch_gs(cg_problem& P, const Ctx& c, int l, float* Z, const float* R, int color, int flags) {
-  const DList& a = P.act[l];
+void launch_gs(cg_problem& P, const DList& a, const Ctx& c, int l, float* Z, const float* R, int color, int flags) {
   if (a.n == 0) return;
   k_gs_color<<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
 }
@@ -368,19 +451,21 @@ void launch_gs(cg_problem& P, const Ctx& c, int l, float* Z, const float* R, int
 // V-cycle rhs of LEAF tiles (rhsv) must hold R on entry.  The zero initial
 // guess and the zero ghost values of the reference's init pass are implicit in
 // the kernels' flags.
-void enqueue_vcycle(cg_problem& P, float* Z, float* R, const VParams& v) {
-  const Ctx c = P.ctx(Z, R);
+// Z is the down-sweep buffer, ZO receives z (they are the same buffer for the
+// damped-Jacobi smoother, which ping-pongs with the scratch buffer instead).
+void enqueue_vcycle(cg_problem& P, float* Z, float* ZO, float* R, const VParams& v) {
+  const Ctx c = P.ctx(ZO, R);
   const int base = F_VCYCLE | kSwizzle | (v.check_done ? F_CHECK_DONE : 0);
-  const int L = P.max_level, C = P.coarse.C;  // levels > max(C, 0) use grid-wide launches
+  const int L = P.max_level;  // a problem's levels above its coarse_C use the grid-wide lists actg / lvlg
   float* ZB = P.d_zb;
-  for (int l = L; l > std::max(C, 0); l--) {
+  for (int l = L; l >= 1; l--) {
+    const DList& a = P.actg[l];
     if (v.smoother == CG_SMOOTH_RBGS) {
       for (int s = 0; s < v.nu; s++) {
-        launch_gs(P, c, l, Z, R, 0, base | F_GHOST_ZERO | (s == 0 ? F_ZERO_X : 0));
-        launch_gs(P, c, l, Z, R, 1, base | F_GHOST_ZERO);
+        launch_gs(P, a, c, l, Z, R, 0, base | F_GHOST_ZERO | (s == 0 ? F_ZERO_X : 0));
+        launch_gs(P, a, c, l, Z, R, 1, base | F_GHOST_ZERO);
       }
     } else {
-      const DList& a = P.act[l];
       const float* src = Z;
       for (int s = 0; s < v.nu; s++) {
         float* dst = ((v.nu - 1 - s) % 2 == 0) ? Z : ZB;
@@ -389,39 +474,47 @@ void enqueue_vcycle(cg_problem& P, float* Z, float* R, const VParams& v) {
         src = dst;
       }
     }
-    const DList& al = P.lvl[l];
+    const DList& al = P.lvlg[l];
     int fl = base | F_GHOST_ZERO | (v.nu == 0 ? F_ZERO_X : 0);
     if (al.n) k_restrict<<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, Z, R, nullptr, v.ooa, fl);
   }
-  if (C >= 0) {
+  if (!P.plans.empty()) {
     VCycleArgs va{v.nu, v.coarsest, v.smoother, v.omega, v.ooa, v.pcoef};
-    k_coarse_lds<<<1, CL_THREADS, 0, P.stream>>>(c, P.coarse, va, Z, base & ~F_SWIZZLE);
-  } else {
-    // level 0 too large for LDS: bottom sweeps as grid-wide launches
+    k_coarse_lds<<<unsigned(P.plans.size()), CL_THREADS, 0, P.stream>>>(c, P.d_plans, va, ZO, base & ~F_SWIZZLE);
+  }
+  {
+    // problems whose level 0 is too large for LDS: bottom sweeps as grid-wide launches
+    const DList& a0 = P.actg[0];
     const int it0 = v.coarsest / 2;
-    if (it0 == 0 && P.act[0].n) k_fill_tiles<<<P.act[0].n, 64, 0, P.stream>>>(P.L(P.act[0]), P.act[0].n, Z, 0.f);
+    if (it0 == 0 && a0.n) k_fill_tiles<<<a0.n, 64, 0, P.stream>>>(P.L(a0), a0.n, ZO, 0.f);
     for (int i = 0; i < it0; i++) {
-      launch_gs(P, c, 0, Z, R, 0, base | ((i == 0) ? F_ZERO_X : 0));
-      launch_gs(P, c, 0, Z, R, 1, base);
+      launch_gs(P, a0, c, 0, ZO, R, 0, base | ((i == 0) ? F_ZERO_X : 0));
+      launch_gs(P, a0, c, 0, ZO, R, 1, base);
     }
     for (int i = 0; i < it0; i++) {
-      launch_gs(P, c, 0, Z, R, 1, base);
-      launch_gs(P, c, 0, Z, R, 0, base);
+      launch_gs(P, a0, c, 0, ZO, R, 1, base);
+      launch_gs(P, a0, c, 0, ZO, R, 0, base);
     }
   }
-  for (int l = std::max(C, 0) + 1; l <= L; l++) {
-    const DList& al = P.lvl[l];
+  for (int l = 1; l <= L; l++) {
+    const DList& al = P.lvlg[l];
+    const DList& a = P.actg[l];
     const bool jac = v.smoother != CG_SMOOTH_RBGS;
-    float* XO = (jac && (v.nu % 2 == 1)) ? ZB : Z;
-    int fl = base | (v.nu == 0 ? F_ZERO_X : 0);
-    if (al.n) k_prolong<<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, Z, Z, XO, Z, v.pcoef, fl);
-    if (!jac) {
-      for (int s = 0; s < v.nu; s++) {
-        launch_gs(P, c, l, Z, R, 1, base);
-        launch_gs(P, c, l, Z, R, 0, base);
+    if (!jac && v.nu > 0) {
+      // prolongation fused into the first (black) pass; the level's result goes to ZO
+      if (al.n) k_gs_prolong<<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, Z, ZO, v.pcoef, 1, base);
+      launch_gs(P, a, c, l, ZO, R, 0, base);
+      for (int s = 1; s < v.nu; s++) {
+        launch_gs(P, a, c, l, ZO, R, 1, base);
+        launch_gs(P, a, c, l, ZO, R, 0, base);
       }
-    } else {
-      const DList& a = P.act[l];
+      continue;
+    }
+    float* XO = (jac && (v.nu % 2 == 1)) ? ZB : ZO;
+    int fl = base | (v.nu == 0 ? F_ZERO_X : 0);
+    if (al.n) k_prolong<<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, ZO, Z, XO, ZO, v.pcoef, fl);
+    if (!jac) continue;
+    {
       const float* src = XO;
       for (int s = 0; s < v.nu; s++) {
         float* dst = ((v.nu - 1 - s) % 2 == 0) ? Z : ZB;
@@ -438,13 +531,14 @@ void enqueue_vcycle(cg_problem& P, float* Z, float* R, const VParams& v) {
 void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann) {
   const Ctx c = P.ctx();
   const int fl = kSwizzle | F_CHECK_DONE;
-  const unsigned np = 1;
+  const unsigned np = unsigned(P.nprob);  // one finalize workgroup per problem
   if (pure_neumann) {
     k_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, fl);
     k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_MEAN, fl);
     k_subtract_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, 1, fl);
   }
-  enqueue_vcycle(P, c.z, c.r, v);
+  // RB-GS: down-sweep in the scratch buffer, z assembled in c.z by the up-sweep
+  enqueue_vcycle(P, v.smoother == CG_SMOOTH_RBGS ? P.d_zb : c.z, c.z, c.r, v);
   k_cg_dot_rz<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
   k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_RZ, fl);
   k_cg_update_p<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
@@ -493,14 +587,16 @@ void full_negative_laplacian(cg_problem& P, float* X, float* AX, int diag) {
   if (P.ghosts.n) k_accumulate_ghosts<<<P.ghosts.n, 64, 0, P.stream>>>(c, P.L(P.ghosts), P.ghosts.n, AX);
 }
 
-ProbState read_state(cg_problem& P) {
-  ProbState s;
-  HIPC(hipMemcpyAsync(&s, P.d_ps, sizeof(s), hipMemcpyDeviceToHost, P.stream));
+std::vector<ProbState> read_states(cg_problem& P) {
+  std::vector<ProbState> s(P.nprob);
+  HIPC(hipMemcpyAsync(s.data(), P.d_ps, sizeof(ProbState) * P.nprob, hipMemcpyDeviceToHost, P.stream));
   HIPC(hipStreamSynchronize(P.stream));
   return s;
 }
 
-void solve(cg_problem& P, const cg_params& prm, cg_result& res) {
+// All problems of the pool, one graph replay per CG iteration for all of them;
+// a converged problem's tiles drop out of every kernel at their first load.
+void solve(cg_problem& P, const cg_params& prm, cg_result* res) {
   if (prm.max_iters < 0 || !(prm.rel_tol >= 0) || prm.level_iters < 0 || prm.coarsest_iters < 0)
     fail(CG_ERR_INVALID, "cg_solve: invalid parameters");
   if (prm.smoother != CG_SMOOTH_RBGS && prm.smoother != CG_SMOOTH_JACOBI) fail(CG_ERR_INVALID, "cg_solve: unknown smoother");
@@ -516,7 +612,7 @@ void solve(cg_problem& P, const cg_params& prm, cg_result& res) {
   set_prob_state(P, prm.rel_tol, prm.max_iters, prm.pure_neumann, prm.sync_stride);
   const Ctx c = P.ctx();
   k_cg_init<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, kSwizzle);
-  k_finalize<<<1, FIN_THREADS, 0, P.stream>>>(c, FIN_BB, 0);
+  k_finalize<<<unsigned(P.nprob), FIN_THREADS, 0, P.stream>>>(c, FIN_BB, 0);
   HIPC(hipGetLastError());
 
   char keybuf[160];
@@ -538,19 +634,24 @@ void solve(cg_problem& P, const cg_params& prm, cg_result& res) {
     if (use_graph) HIPC(hipGraphLaunch(P.graph, P.stream));
     else enqueue_cg_iteration(P, v, prm.pure_neumann);
   };
-  // Poll the done flag one chunk behind the enqueue front, so the device never
+  // Poll the done flags one chunk behind the enqueue front, so the device never
   // waits for the host; iterations after convergence exit at their first load.
+  const int npb = P.nprob;
+  auto all_done = [&](int sl) {
+    for (int q = 0; q < npb; q++) if (!P.h_pin[sl * npb + q]) return false;
+    return true;
+  };
   int enq = 0, slot = 0;
   bool have_prev = false;
   while (true) {
     if (prm.max_iters == 0) break;
     for (int s = 0; s < stride; s++) { one_iter(); enq++; }
     HIPC(hipGetLastError());
-    HIPC(hipMemcpyAsync(&P.h_pin[slot], &P.d_ps->done, sizeof(int), hipMemcpyDeviceToHost, P.stream));
+    HIPC(hipMemcpyAsync(&P.h_pin[slot * npb], P.d_dflag, sizeof(int32_t) * npb, hipMemcpyDeviceToHost, P.stream));
     HIPC(hipEventRecord(P.ev[slot], P.stream));
     if (have_prev) {
       HIPC(hipEventSynchronize(P.ev[1 - slot]));
-      if (P.h_pin[1 - slot]) break;
+      if (all_done(1 - slot)) break;
     }
     if (enq >= prm.max_iters) {
       HIPC(hipEventSynchronize(P.ev[slot]));
@@ -561,15 +662,19 @@ void solve(cg_problem& P, const cg_params& prm, cg_result& res) {
   }
   HIPC(hipStreamSynchronize(P.stream));
   auto t1 = std::chrono::steady_clock::now();
-  ProbState s = read_state(P);
-  res.iters = s.iter;
-  res.converged = s.converged;
-  res.bb = s.bb;
-  res.rr = s.rr;
-  // GMGSolver.cu:553-556, 655-665: (0, 0) for b = 0; relative residual -1
-  // when convergence is never tested (sync_stride <= 0)
-  res.rel_res = s.bb > 0 ? (prm.sync_stride > 0 ? std::sqrt(s.rr / s.bb) : -1.0) : 0.0;
-  res.seconds = std::chrono::duration<double>(t1 - t0).count();
+  const std::vector<ProbState> st = read_states(P);
+  for (int q = 0; q < P.nprob; q++) {
+    const ProbState& s = st[q];
+    cg_result& r = res[q];
+    r.iters = s.iter;
+    r.converged = s.converged;
+    r.bb = s.bb;
+    r.rr = s.rr;
+    // GMGSolver.cu:553-556, 655-665: (0, 0) for b = 0; relative residual -1
+    // when convergence is never tested (sync_stride <= 0)
+    r.rel_res = s.bb > 0 ? (prm.sync_stride > 0 ? std::sqrt(s.rr / s.bb) : -1.0) : 0.0;
+    r.seconds = std::chrono::duration<double>(t1 - t0).count();
+  }
 }
 
 }  // namespace
@@ -607,13 +712,21 @@ void cg_params_default(cg_params* p) {
   p->use_graph = 1;
 }
 
-cg_status cg_problem_create(const cg_tree_desc* desc, cg_problem** out) {
+cg_status cg_problem_create_batch(const cg_tree_desc* descs, int32_t num_problems, cg_problem** out) {
   if (!out) { g_err = "cg_problem_create: null output"; return CG_ERR_INVALID; }
   *out = nullptr;
   auto P = std::make_unique<cg_problem>();
-  cg_status st = guard([&] { build_problem(desc, *P); });
+  cg_status st = guard([&] { build_pool(descs, num_problems, *P); });
   if (st == CG_OK) *out = P.release();
   return st;
+}
+
+cg_status cg_problem_create(const cg_tree_desc* desc, cg_problem** out) { return cg_problem_create_batch(desc, 1, out); }
+
+cg_status cg_problem_num_problems(const cg_problem* p, int32_t* num_problems) {
+  if (!p || !num_problems) { g_err = "null argument"; return CG_ERR_INVALID; }
+  *num_problems = p->nprob;
+  return CG_OK;
 }
 
 cg_status cg_problem_destroy(cg_problem* p) {
@@ -665,13 +778,22 @@ cg_status cg_get_channel(cg_problem* p, int32_t channel, float* host) {
 cg_status cg_solve(cg_problem* p, const cg_params* prm, cg_result* res) {
   return guard([&] {
     if (!p || !prm || !res) fail(CG_ERR_INVALID, "cg_solve: null argument");
-    solve(*p, *prm, *res);
+    if (p->nprob != 1) fail(CG_ERR_INVALID, "cg_solve: the handle holds several problems; use cg_solve_batch");
+    solve(*p, *prm, res);
+  });
+}
+
+cg_status cg_solve_batch(cg_problem* p, const cg_params* prm, cg_result* results) {
+  return guard([&] {
+    if (!p || !prm || !results) fail(CG_ERR_INVALID, "cg_solve_batch: null argument");
+    solve(*p, *prm, results);
   });
 }
 
 cg_status cg_damped_jacobi_solve(cg_problem* p, int32_t max_iters, double rel_tol, float omega, cg_result* res) {
   return guard([&] {
     if (!p || !res) fail(CG_ERR_INVALID, "cg_damped_jacobi_solve: null argument");
+    if (p->nprob != 1) fail(CG_ERR_INVALID, "cg_damped_jacobi_solve: one problem per handle");
     auto t0 = std::chrono::steady_clock::now();
     cg_problem& P = *p;
     const Ctx c = P.ctx();
@@ -719,7 +841,8 @@ cg_status cg_vcycle(cg_problem* p, int32_t x_channel, int32_t f_channel, int32_t
     v.check_done = 0;
     // the V-cycle rhs of LEAF tiles starts as f (GMGSolver.cu:393-400)
     k_copy_tiles<<<p->leaf.n, 64, 0, p->stream>>>(p->L(p->leaf), p->leaf.n, p->d_ch[f_channel], p->d_ch[CG_CH_TMP]);
-    enqueue_vcycle(*p, p->d_ch[x_channel], p->d_ch[f_channel], v);
+    float* X = p->d_ch[x_channel];
+    enqueue_vcycle(*p, smoother == CG_SMOOTH_RBGS ? p->d_zb : X, X, p->d_ch[f_channel], v);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
   });
@@ -764,7 +887,7 @@ cg_status cg_gauss_seidel_color(cg_problem* p, int32_t level, int32_t x_channel,
     check_channel(x_channel);
     check_channel(rhs_channel);
     if (level < 0 || level > p->max_level) fail(CG_ERR_INVALID, "level out of range");
-    launch_gs(*p, p->ctx(), level, p->d_ch[x_channel], p->d_ch[rhs_channel], color & 1, 0);
+    launch_gs(*p, p->act[level], p->ctx(), level, p->d_ch[x_channel], p->d_ch[rhs_channel], color & 1, 0);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
   });
@@ -847,7 +970,8 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
     cg_problem& P = *p;
     if ((kernel == CG_KERNEL_GS_SWEEP || kernel == CG_KERNEL_JACOBI) && (level < 0 || level > P.max_level))
       fail(CG_ERR_INVALID, "level out of range");
-    if ((kernel == CG_KERNEL_RESTRICT || kernel == CG_KERNEL_PROLONG) && (level < 1 || level > P.max_level))
+    if ((kernel == CG_KERNEL_RESTRICT || kernel == CG_KERNEL_PROLONG || kernel == CG_KERNEL_GS_PROLONG) &&
+        (level < 1 || level > P.max_level))
       fail(CG_ERR_INVALID, "level out of range");
     if ((kernel == CG_KERNEL_COARSE || kernel == CG_KERNEL_EMPTY) && level < 0) fail(CG_ERR_INVALID, "negative count");
     set_prob_state(P, 0.0, 1 << 30, 0);
@@ -858,7 +982,7 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
       switch (kernel) {
         case CG_KERNEL_GS_SWEEP:
           nt = P.act[level].n;
-          launch_gs(P, c, level, c.z, c.r, 0, base | F_GHOST_ZERO);
+          launch_gs(P, P.act[level], c, level, c.z, c.r, 0, base | F_GHOST_ZERO);
           break;
         case CG_KERNEL_JACOBI: {
           const DList& a = P.act[level];
@@ -883,14 +1007,21 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
           k_prolong<<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, level, c.z, c.z, c.zb, c.zb, 1.f, base);
           break;
         }
+        case CG_KERNEL_GS_PROLONG: {
+          const DList& al = P.lvl[level];
+          nt = al.n;
+          k_gs_prolong<<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, c.zb, c.z, 1.f, 1, base);
+          break;
+        }
         case CG_KERNEL_UPDATE_XR:
           nt = P.leaf.n;
           k_cg_update_xr<<<grid_for(P.leaf.n, base), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, base);
           break;
         case CG_KERNEL_COARSE: {
-          nt = P.coarse.nslots;
+          nt = int64_t(P.coarse_slots.size());
           VCycleArgs va{2, level, 0, 2.f / 3.f, 1.f, 1.f};  // `level` = bottom iterations
-          if (P.coarse.C >= 0) k_coarse_lds<<<1, CL_THREADS, 0, P.stream>>>(c, P.coarse, va, c.z, base & ~F_SWIZZLE);
+          if (!P.plans.empty())
+            k_coarse_lds<<<unsigned(P.plans.size()), CL_THREADS, 0, P.stream>>>(c, P.d_plans, va, c.z, base & ~F_SWIZZLE);
           break;
         }
         case CG_KERNEL_COPY:

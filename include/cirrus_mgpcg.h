@@ -95,6 +95,15 @@ int32_t cg_device_count(void);
  * HADeviceGrid (src/HAGrid.h:387-561) plus CalculateNeighborTiles
  * (src/PoissonGrid.cu:121-134). */
 cg_status cg_problem_create(const cg_tree_desc* desc, cg_problem** out);
+/* Batched form, no reference counterpart: num_problems independent trees in
+ * one handle, solved together by cg_solve_batch (every kernel of a CG
+ * iteration covers the tiles of all problems; each problem keeps its own
+ * scalars and convergence test, so its result is bit-identical to solving it
+ * alone).  Per-tile host arrays of the handle concatenate the problems in
+ * order, each in the tile order of its description; topology indices refer to
+ * that concatenated order. */
+cg_status cg_problem_create_batch(const cg_tree_desc* descs, int32_t num_problems, cg_problem** out);
+cg_status cg_problem_num_problems(const cg_problem* p, int32_t* num_problems);
 cg_status cg_problem_destroy(cg_problem* p);
 /* Tile counts and the deepest level. */
 cg_status cg_problem_info(const cg_problem* p, int32_t* num_tiles, int32_t* num_leaf_tiles, int32_t* max_level);
@@ -111,6 +120,8 @@ cg_status cg_get_channel(cg_problem* p, int32_t channel, float* host);
  * CG_CH_B, pressure returned in CG_CH_X on LEAF tiles.  Channels 1-5 are
  * clobbered, as in the reference. */
 cg_status cg_solve(cg_problem* p, const cg_params* prm, cg_result* res);
+/* All problems of a batched handle; results[num_problems]. */
+cg_status cg_solve_batch(cg_problem* p, const cg_params* prm, cg_result* results);
 
 /* GMGSolver::dampedJacobiSolve (src/GMGSolver.cu:449-529). */
 cg_status cg_damped_jacobi_solve(cg_problem* p, int32_t max_iters, double rel_tol, float omega, cg_result* res);
@@ -159,10 +170,11 @@ cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel);
  * coarse levels of a V-cycle, RB-GS 2+2, `level` bottom iterations), CG_KERNEL_EMPTY (an empty kernel of
  * `level` workgroups: the launch floor), CG_KERNEL_COPY (copy of one channel
  * over all tiles: 4096 bytes moved per tile, the reference point that scales
- * hardware byte counters).  Channels are overwritten. */
+ * hardware byte counters), CG_KERNEL_GS_PROLONG (prolongation to `level` fused
+ * with the first up-sweep colour pass).  Channels are overwritten. */
 enum { CG_KERNEL_GS_SWEEP = 0, CG_KERNEL_APPLY = 1, CG_KERNEL_RESTRICT = 2, CG_KERNEL_PROLONG = 3,
        CG_KERNEL_JACOBI = 4, CG_KERNEL_UPDATE_XR = 5, CG_KERNEL_COARSE = 6, CG_KERNEL_EMPTY = 7,
-       CG_KERNEL_COPY = 8 };
+       CG_KERNEL_COPY = 8, CG_KERNEL_GS_PROLONG = 9 };
 cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int32_t reps, double* avg_us,
                               int64_t* tiles);
 
