@@ -79,6 +79,97 @@ def batch_scene_params(q, seed=0):
     return kw, target
 
 
+def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=64, nscenes=8, budget=24 << 30, rho=0.95, steps=50):
+    """Serving form: requests arrive as a Poisson process of rate `lam` for
+    `duration` seconds; request j goes to resident instance j mod conc, whose
+    hierarchy and pressure stay on the device.  Each instance runs sequences of
+    `steps` correlated solves: its right-hand side is updated on the device as
+    b = rho b + sqrt(1 - rho^2) rms(b0) u (u seeded unit-variance noise) and the
+    solve is warm-started from its previous pressure; every steps-th request
+    starts a new sequence (cold).  Requests that find their instance busy wait;
+    latency is arrival -> result.  Instances are built like the batched form."""
+    import collections
+    trees_b0 = []
+    for q in range(nscenes):
+        sc = ca.Scene(**batch_scene_params(q, seed=1)[0])
+        trees_b0.append((sc.tree, np.ascontiguousarray(sc.rhs, np.float32)))
+    S = ca.Server(budget, conc, level_iters=solver_kw["level_iters"], coarsest_iters=solver_kw["coarsest_iters"],
+                  smoother=solver_kw["smoother"])
+    insts, sig = [], []
+    for w in range(conc):
+        tree, b0 = trees_b0[w % nscenes]
+        try:
+            insts.append(S.admit(tree))
+        except ca.CirrusBusy:
+            break
+        sig.append(float(np.sqrt(1 - rho * rho) * np.sqrt(np.mean(b0[b0 != 0] ** 2))))
+    W = len(insts)
+    # bring every instance to a warm state (first, cold solve from its host rhs)
+    for w, i in enumerate(insts):
+        S.submit(i, trees_b0[w % nscenes][1], warm=False, tag=-1 - w, download=False)
+    left = W
+    while left:
+        S.step()
+        while S.next_result(with_x=False) is not None:
+            left -= 1
+    rng = np.random.default_rng(7)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, int(lam * duration * 1.2) + 16))
+    arrivals = arrivals[arrivals < duration]
+    fifo = [collections.deque() for _ in range(W)]
+    busy = [False] * W
+    count = [1] * W  # solves done per instance (the warm-up one included)
+    lat, iters, act = [], [], []
+    nxt = 0
+    t0 = time.perf_counter()
+    deadline = duration + 20.0
+    while True:
+        now = time.perf_counter() - t0
+        while nxt < len(arrivals) and arrivals[nxt] <= now:
+            fifo[nxt % W].append(nxt)
+            nxt += 1
+        for w in range(W):
+            if not busy[w] and fifo[w]:
+                j = fifo[w][0]
+                warm = count[w] % steps != 0
+                if warm:
+                    S.perturb_rhs(insts[w], rho, sig[w], 1000003 * w + count[w])
+                    S.submit(insts[w], None, warm=True, tag=j, download=False)
+                else:
+                    S.submit(insts[w], trees_b0[w % nscenes][1], warm=False, tag=j, download=False)
+                busy[w] = True
+        act.append(S.step())
+        t_done = time.perf_counter() - t0
+        while True:
+            r = S.next_result(with_x=False)
+            if r is None:
+                break
+            j, inst, res, _ = r
+            w = j % W
+            fifo[w].popleft()
+            busy[w] = False
+            count[w] += 1
+            lat.append(t_done - arrivals[j])
+            iters.append(res["iters"])
+        if nxt >= len(arrivals) and not any(busy) and not any(fifo):
+            break
+        if now > deadline:
+            break
+    t_end = time.perf_counter() - t0
+    lat_ms = np.array(lat) * 1e3
+    leaf = [int((trees_b0[w % nscenes][0].type == 1).sum()) for w in range(W)]
+    return {
+        "workload": f"Poisson requests at {lam:g}/s for {duration:g} s over {W} resident instances "
+                    f"(lognormal sizes, median {int(np.median(leaf))} leaf tiles), warm-started correlated "
+                    f"sequences of {steps}, FP32, RB-GS 2+2, tol 1e-6 of |b|",
+        "instances": W, "requests": int(len(arrivals)), "completed": len(lat),
+        "solves_per_s": len(lat) / t_end, "offered_per_s": lam,
+        "p50_ms": float(np.percentile(lat_ms, 50)) if len(lat) else None,
+        "p99_ms": float(np.percentile(lat_ms, 99)) if len(lat) else None,
+        "iterations_median": float(np.median(iters)) if iters else None,
+        "active_mean": float(np.mean(act)) if act else 0.0, "wall_s": t_end,
+    }
+
+
 def run_batched(ca, nprob, reps, solver_kw, use_graph=True):
     """Solve nprob independent problems together; returns the form's record."""
     t0 = time.perf_counter()
@@ -297,6 +388,10 @@ def main():
     ap.add_argument("--batch", type=int, default=32,
                     help="problems in the batched form measured beside the plain solve (0: skip)")
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="length of the Poisson request window of the served form (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-instances", type=int, default=64)
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only the dominant kernel and the copy reference (for counter runs)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
@@ -398,6 +493,10 @@ def main():
         bf = run_batched(ca, args.batch, args.batch_reps, solver_kw, use_graph=not args.no_graph)
         bf["vs_plain_solves_per_s"] = bf["solves_per_s"] / out["value"]
         out["forms"] = {"batched": bf}
+    if args.serve_seconds > 0:
+        sf = run_served(ca, solver_kw, duration=args.serve_seconds, lam=args.serve_rate, conc=args.serve_instances)
+        sf["vs_plain_solves_per_s"] = sf["solves_per_s"] / out["value"]
+        out.setdefault("forms", {})["served"] = sf
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene_kw, solver_kw)
     print(json.dumps(out), flush=True)

@@ -38,6 +38,8 @@ EXPORTED_SYMBOLS = (
     "cg_vcycle", "cg_full_negative_laplacian", "cg_negative_laplacian_same_level", "cg_gauss_seidel_color",
     "cg_jacobi_sweep", "cg_residual_restrict", "cg_prolongate", "cg_dot", "cg_recenter_leaf_voxels",
     "cg_calc_cell_types_from_leafs", "cg_benchmark_kernel", "cg_scene_create", "cg_scene_destroy", "cg_scene_tree", "cg_scene_velocity",
+    "cg_server_create", "cg_server_destroy", "cg_server_admit", "cg_server_release", "cg_server_submit", "cg_server_step",
+    "cg_server_next_result", "cg_server_info", "cg_server_perturb_rhs",
 )
 
 
@@ -73,8 +75,16 @@ class SceneParams(ctypes.Structure):
         ("face_bc", ctypes.c_int32 * 6), ("seed", ctypes.c_uint64), ("num_layers", ctypes.c_int32)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_int64), ("max_problems", ctypes.c_int32), ("params", Params)]
+
+
 class CirrusError(RuntimeError):
     pass
+
+
+class CirrusBusy(CirrusError):
+    """No capacity now (HBM budget or problem slots); retry later."""
 
 
 _lib = None
@@ -115,6 +125,15 @@ def lib():
             "cg_recenter_leaf_voxels": (i32, [vp, i32]),
             "cg_calc_cell_types_from_leafs": (i32, [P(TreeDesc), vp]),
             "cg_benchmark_kernel": (i32, [vp, i32, i32, i32, P(f64), P(ctypes.c_int64)]),
+            "cg_server_create": (i32, [P(ServerConfig), P(vp)]),
+            "cg_server_destroy": (i32, [vp]),
+            "cg_server_admit": (i32, [vp, P(TreeDesc), P(i32)]),
+            "cg_server_release": (i32, [vp, i32]),
+            "cg_server_submit": (i32, [vp, i32, vp, i32, ctypes.c_int64]),
+            "cg_server_perturb_rhs": (i32, [vp, i32, f32, f32, ctypes.c_uint64]),
+            "cg_server_step": (i32, [vp, P(i32)]),
+            "cg_server_next_result": (i32, [vp, i32, P(i32), P(ctypes.c_int64), P(i32), P(Result), vp]),
+            "cg_server_info": (i32, [vp, P(ctypes.c_int64), P(ctypes.c_int64), P(i32)]),
             "cg_scene_create": (i32, [P(SceneParams), P(vp)]),
             "cg_scene_destroy": (i32, [vp]),
             "cg_scene_tree": (i32, [vp, P(TreeDesc), P(vp)]),
@@ -133,10 +152,14 @@ def device_count():
     return int(lib().cg_device_count())
 
 
+CG_ERR_BUSY = 5
+
+
 def _check(st, what):
     if st != 0:
         msg = lib().cg_last_error()
-        raise CirrusError(f"{what} failed (status {st}): {msg.decode() if msg else ''}")
+        cls = CirrusBusy if st == CG_ERR_BUSY else CirrusError
+        raise cls(f"{what} failed (status {st}): {msg.decode() if msg else ''}")
 
 
 def _ptr(a):
@@ -396,3 +419,79 @@ class GMGSolver:
                "GMGSolver.dampedJacobiSolve")
         self.last = r.as_dict()
         return r.iters, r.rel_res
+
+
+class Server:
+    """The serving loop (cg_server_*): resident instances, requests with warm
+    starts, one CG iteration over all running requests per :meth:`step`."""
+
+    def __init__(self, budget_bytes, max_problems, **params):
+        cfg = ServerConfig()
+        cfg.budget_bytes = int(budget_bytes)
+        cfg.max_problems = int(max_problems)
+        lib().cg_params_default(ctypes.byref(cfg.params))
+        for k, v in params.items():
+            setattr(cfg.params, k, v)
+        h = ctypes.c_void_p()
+        _check(lib().cg_server_create(ctypes.byref(cfg), ctypes.byref(h)), "cg_server_create")
+        self._h = h
+        self.ntiles = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cg_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, tree):
+        """Make a tree resident; returns its instance id (raises CirrusBusy when it does not fit)."""
+        inst = ctypes.c_int32()
+        d = tree.desc()
+        _check(lib().cg_server_admit(self._h, ctypes.byref(d), ctypes.byref(inst)), "cg_server_admit")
+        self.ntiles[inst.value] = tree.n
+        return inst.value
+
+    def release(self, inst):
+        _check(lib().cg_server_release(self._h, inst), "cg_server_release")
+        self.ntiles.pop(inst, None)
+
+    def submit(self, inst, rhs, warm=False, tag=0, download=True):
+        """Queue a request; rhs None uses the right-hand side already on the device."""
+        flags = (1 if warm else 0) | (0 if download else 2)
+        if rhs is None:
+            _check(lib().cg_server_submit(self._h, inst, None, flags, int(tag)), "cg_server_submit")
+            return
+        rhs = np.ascontiguousarray(rhs, np.float32).reshape(self.ntiles[inst], 512)
+        _check(lib().cg_server_submit(self._h, inst, _ptr(rhs), flags, int(tag)), "cg_server_submit")
+
+    def perturb_rhs(self, inst, rho, sigma, seed):
+        _check(lib().cg_server_perturb_rhs(self._h, inst, float(rho), float(sigma), int(seed)), "cg_server_perturb_rhs")
+
+    def step(self):
+        n = ctypes.c_int32()
+        _check(lib().cg_server_step(self._h, ctypes.byref(n)), "cg_server_step")
+        return n.value
+
+    def next_result(self, wait=False, with_x=True):
+        """(tag, instance, result dict, pressure or None) of the oldest finished
+        request, or None if none is ready."""
+        nmax = max(self.ntiles.values()) if self.ntiles else 1
+        x = np.zeros((max(nmax, 1), 512), np.float32) if with_x else None
+        got, tag, inst = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
+        res = Result()
+        _check(lib().cg_server_next_result(self._h, int(wait), ctypes.byref(got), ctypes.byref(tag), ctypes.byref(inst),
+                                           ctypes.byref(res), _ptr(x) if with_x else None), "cg_server_next_result")
+        if not got.value:
+            return None
+        xs = x[:self.ntiles[inst.value]].copy() if with_x else None
+        return tag.value, inst.value, res.as_dict(), xs
+
+    def info(self):
+        cap, free, ninst = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        _check(lib().cg_server_info(self._h, ctypes.byref(cap), ctypes.byref(free), ctypes.byref(ninst)), "cg_server_info")
+        return {"capacity_tiles": cap.value, "free_tiles": free.value, "instances": ninst.value}

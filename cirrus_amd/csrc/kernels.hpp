@@ -79,6 +79,7 @@ struct Ctx {
   float* gsum;   // 64 per ghost: 2x2x2 sums of the ghost rows of A p
   ProbState* ps;
   int32_t* dflag; // per-problem copy of ProbState::done, polled by the host
+  const int32_t* pmap;  // finalize workgroup -> problem (null: identity)
   double* part;  // per-leaf-slot dot partials
   double* part2; // second partial stream (mean counts)
 };
@@ -932,6 +933,71 @@ __global__ __launch_bounds__(64) void k_cg_init(Ctx c, const TileRec* __restrict
   if (lane == 0) c.part[T.leaf_slot] = a;
 }
 
+// Warm-started init: r = b - A x0 on LEAF tiles (A x0 in AP, formed by the
+// operator kernels), p = 0, V-cycle rhs = r; partials of |b|^2, against which
+// the tolerance is measured.
+__global__ __launch_bounds__(64) void k_cg_init_warm(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ B,
+                                                     int flags) {
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id();
+  const size_t base = size_t(T.self) * 512 + lane * 8;
+  float bv[8], av[8], rv[8];
+  load8(B + base, bv);
+  load8(c.ap + base, av);
+  for (int k = 0; k < 8; k++) rv[k] = bv[k] - av[k];
+  float zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  store8(c.p + base, zero);
+  store8(c.r + base, rv);
+  store8(c.rhsv + base, rv);
+  double a = 0.0;
+  for (int k = 0; k < 8; k++) a += double(bv[k]) * double(bv[k]);
+  a = wave_sum(a);
+  if (lane == 0) c.part[T.leaf_slot] = a;
+}
+
+// Seeded update of a right-hand side, for a synthetic client of the serving
+// loop: b = rho * b + sigma * u on INTERIOR cells of the listed (LEAF) tiles,
+// u uniform on [-sqrt(3), sqrt(3)] (unit variance) from a counter hash of
+// (seed, tile, cell).
+__device__ __forceinline__ uint32_t hash32(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return uint32_t(x);
+}
+__global__ __launch_bounds__(64) void k_perturb_rhs(Ctx c, const TileRec* __restrict__ list, int n, float* __restrict__ B,
+                                                    float rho, float sigma, uint64_t seed) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id();
+  const size_t base = size_t(T.self) * 512 + lane * 8;
+  float b[8];
+  load8(B + base, b);
+  const uint64_t ct = load_ct8(c.ctype + base);
+  for (int k = 0; k < 8; k++) {
+    if (!(ct_at(ct, k) & C_INT)) continue;
+    const uint32_t h = hash32(seed * 0x9e3779b97f4a7c15ull + uint64_t(base + k));
+    const float u = (float(h) * 2.3283064365386963e-10f * 2.0f - 1.0f) * 1.7320508075688772f;
+    b[k] = rho * b[k] + sigma * u;
+  }
+  store8(B + base, b);
+}
+
+// Copy runs of tile records into an assembled worklist: run q copies
+// counts[q] records from srcs[q] to dst + offs[q] (grid-stride over records).
+__global__ __launch_bounds__(256) void k_gather_recs(const TileRec* const* __restrict__ srcs, const int32_t* __restrict__ offs,
+                                                     const int32_t* __restrict__ counts, int nruns, TileRec* __restrict__ dst) {
+  constexpr int W = sizeof(TileRec) / 16;
+  for (int q = blockIdx.y; q < nruns; q += gridDim.y) {
+    const int4* src = reinterpret_cast<const int4*>(srcs[q]);
+    int4* out = reinterpret_cast<int4*>(dst + offs[q]);
+    const int words = counts[q] * W;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < words; k += gridDim.x * blockDim.x) out[k] = src[k];
+  }
+}
+
 // partials of gamma = r.z over LEAF tiles (GMGSolver.cu:634)
 __global__ __launch_bounds__(64) void k_cg_dot_rz(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
   TILE_KERNEL_PROLOGUE
@@ -1099,7 +1165,7 @@ __device__ __forceinline__ double block_sum_partials(const double* __restrict__ 
 enum : int { FIN_BB = 0, FIN_RZ = 1, FIN_PAP = 2, FIN_RR = 3, FIN_MEAN = 4 };
 __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctx c, int mode, int flags) {
   __shared__ double lds[FIN_CHUNK];
-  ProbState* ps = c.ps + blockIdx.x;
+  ProbState* ps = c.ps + (c.pmap ? c.pmap[blockIdx.x] : int(blockIdx.x));
   if (mode != FIN_BB && (flags & F_CHECK_DONE) && ps->done) return;
   if (mode == FIN_MEAN && !ps->pure_neumann) return;
   const double s = block_sum_partials(c.part, ps->leaf_begin, ps->leaf_count, lds);

@@ -66,6 +66,7 @@ using namespace cg;
 // of all problems share the channel arrays, each worklist spans all problems,
 // and every problem has its own ProbState, leaf-slot range and coarse plan.
 struct cg_problem {
+  bool owner = true;  // false: a view whose device arrays belong to someone else
   int nprob = 0;
   int n = 0;  // tiles of all problems
   int max_level = 0;
@@ -89,6 +90,7 @@ struct cg_problem {
   float* d_gsum = nullptr;
   ProbState* d_ps = nullptr;
   int32_t* d_dflag = nullptr;
+  const int32_t* d_pmap = nullptr;  // finalize workgroup -> problem (pools whose problems are a subset)
   CoarsePlan* d_plans = nullptr;
   double* d_part = nullptr;
   double* d_part2 = nullptr;
@@ -117,6 +119,7 @@ struct cg_problem {
     c.gsum = d_gsum;
     c.ps = d_ps;
     c.dflag = d_dflag;
+    c.pmap = d_pmap;
     c.part = d_part;
     c.part2 = d_part2;
     return c;
@@ -126,6 +129,7 @@ struct cg_problem {
 };
 
 cg_problem::~cg_problem() {
+  if (!owner) return;
   if (graph) (void)hipGraphExecDestroy(graph);
   for (auto e : ev) if (e) (void)hipEventDestroy(e);
   if (h_pin) (void)hipHostFree(h_pin);
@@ -144,15 +148,28 @@ inline unsigned grid_for(int n, int flags) { return (flags & F_SWIZZLE) ? unsign
 // ---------------------------------------------------------------------------
 // Problem construction
 // ---------------------------------------------------------------------------
-// One tree of the pool: validation, canonical order (level, then Morton code of
-// the tile coordinate), neighbour / parent / ghost-child links and tile records.
-// Indices in the records are global (offset by `base`).
-void build_tree(const cg_tree_desc* d, int prob, int base, int leaf_base, int ghost_base, cg_problem& P,
-                std::vector<int>& lev, std::vector<uint8_t>& ty, int& nleaf, int& nghost) {
+// One tree: validation, canonical order (level, then Morton code of the tile
+// coordinate), neighbour / parent / ghost-child links and tile records.  Output
+// arrays are local to the tree (tiles[i], ext[i] = caller index of tile i,
+// lev[i], ty[i]); indices stored in the records are global (offset by `base`,
+// leaf and ghost slots by leaf_base / ghost_base).
+struct TreeBuild {
+  std::vector<TileRec> tiles;
+  std::vector<int> ext, lev;
+  std::vector<uint8_t> ty;
+  int nleaf = 0, nghost = 0;
+};
+void build_tree(const cg_tree_desc* d, int prob, int base, int leaf_base, int ghost_base, TreeBuild& B) {
   if (!d || d->num_tiles <= 0 || !d->level || !d->coord || !d->type || !d->cell_type)
     fail(CG_ERR_INVALID, "cg_problem_create: empty or incomplete tree description");
   if (!(d->h0 > 0)) fail(CG_ERR_INVALID, "cg_problem_create: h0 must be positive");
   const int n = d->num_tiles;
+  B.tiles.assign(n, TileRec());
+  B.ext.assign(n, 0);
+  B.lev.assign(n, 0);
+  B.ty.assign(n, 0);
+  std::vector<int>& lev = B.lev;
+  std::vector<uint8_t>& ty = B.ty;
   std::vector<uint64_t> mk(n);
   for (int t = 0; t < n; t++) {
     int l = d->level[t];
@@ -174,12 +191,12 @@ void build_tree(const cg_tree_desc* d, int prob, int base, int leaf_base, int gh
   std::vector<std::array<int32_t, 3>> cd(n);
   for (int i = 0; i < n; i++) {
     const int e = order[i];
-    P.ext_of[base + i] = base + e;
-    lev[base + i] = d->level[e];
+    B.ext[i] = e;
+    lev[i] = d->level[e];
     cd[i] = {d->coord[3 * e], d->coord[3 * e + 1], d->coord[3 * e + 2]};
-    ty[base + i] = d->type[e];
-    if (!idx.emplace(tile_key(lev[base + i], cd[i][0], cd[i][1], cd[i][2]), i).second)
-      fail(CG_ERR_INVALID, "duplicate tile at level " + std::to_string(lev[base + i]));
+    ty[i] = d->type[e];
+    if (!idx.emplace(tile_key(lev[i], cd[i][0], cd[i][1], cd[i][2]), i).second)
+      fail(CG_ERR_INVALID, "duplicate tile at level " + std::to_string(lev[i]));
   }
   auto find = [&](int l, int x, int y, int z) -> int {
     if (l < 0 || !key_ok(x, y, z)) return -1;
@@ -187,30 +204,32 @@ void build_tree(const cg_tree_desc* d, int prob, int base, int leaf_base, int gh
     return it == idx.end() ? -1 : it->second;
   };
   std::vector<int> leaf_slot(n, -1), gslot(n, -1);
-  nleaf = nghost = 0;
+  int nleaf = 0, nghost = 0;
   for (int i = 0; i < n; i++) {
-    if (ty[base + i] == LEAF) leaf_slot[i] = leaf_base + nleaf++;
-    if (ty[base + i] == GHOST) gslot[i] = ghost_base + nghost++;
+    if (ty[i] == LEAF) leaf_slot[i] = leaf_base + nleaf++;
+    if (ty[i] == GHOST) gslot[i] = ghost_base + nghost++;
   }
+  B.nleaf = nleaf;
+  B.nghost = nghost;
   if (nleaf == 0) fail(CG_ERR_INVALID, "tree has no LEAF tiles");
   for (int i = 0; i < n; i++) {
-    TileRec& T = P.tiles[base + i];
+    TileRec& T = B.tiles[i];
     std::memset(&T, 0, sizeof(T));
-    const int l = lev[base + i];
-    const uint8_t my = ty[base + i];
+    const int l = lev[i];
+    const uint8_t my = ty[i];
     for (int f = 0; f < 6; f++) {
       int q[3] = {cd[i][0], cd[i][1], cd[i][2]};
       q[f % 3] += (f < 3) ? -1 : 1;
       const int nb = find(l, q[0], q[1], q[2]);
       T.nbr[f] = nb < 0 ? -1 : base + nb;
-      T.ntypes |= uint32_t(nb >= 0 ? ty[base + nb] : 0) << (4 * f);
+      T.ntypes |= uint32_t(nb >= 0 ? ty[nb] : 0) << (4 * f);
     }
     T.parent = -1;
     if (l > 0) {
       const int p = find(l - 1, cd[i][0] >> 1, cd[i][1] >> 1, cd[i][2] >> 1);
       if (p < 0) fail(CG_ERR_INVALID, "tile at level " + std::to_string(l) + " has no parent tile");
-      if (my == GHOST && ty[base + p] != LEAF) fail(CG_ERR_INVALID, "a GHOST tile's parent must be a LEAF");
-      if (my != GHOST && ty[base + p] != NONLEAF) fail(CG_ERR_INVALID, "a LEAF/NONLEAF tile's parent must be a NONLEAF");
+      if (my == GHOST && ty[p] != LEAF) fail(CG_ERR_INVALID, "a GHOST tile's parent must be a LEAF");
+      if (my != GHOST && ty[p] != NONLEAF) fail(CG_ERR_INVALID, "a LEAF/NONLEAF tile's parent must be a NONLEAF");
       T.parent = base + p;
     }
     const uint32_t octant = uint32_t((cd[i][0] & 1) | ((cd[i][1] & 1) << 1) | ((cd[i][2] & 1) << 2));
@@ -224,10 +243,10 @@ void build_tree(const cg_tree_desc* d, int prob, int base, int leaf_base, int gh
     if (my != GHOST) {
       for (int o = 0; o < 8; o++) {
         const int ch = find(l + 1, 2 * cd[i][0] + (o & 1), 2 * cd[i][1] + ((o >> 1) & 1), 2 * cd[i][2] + ((o >> 2) & 1));
-        if (my == NONLEAF && (ch < 0 || ty[base + ch] == GHOST))
+        if (my == NONLEAF && (ch < 0 || ty[ch] == GHOST))
           fail(CG_ERR_INVALID, "a NONLEAF tile must have 8 LEAF/NONLEAF children");
         if (my == LEAF && ch >= 0) {
-          if (ty[base + ch] != GHOST) fail(CG_ERR_INVALID, "a LEAF tile can only have GHOST children");
+          if (ty[ch] != GHOST) fail(CG_ERR_INVALID, "a LEAF tile can only have GHOST children");
           T.gch[o] = base + ch;
           T.gcs[o] = gslot[ch];
           T.meta |= 1u << 24;
@@ -238,8 +257,68 @@ void build_tree(const cg_tree_desc* d, int prob, int base, int leaf_base, int gh
     view_coeffs(T, l, T.bc);
   }
   for (int i = 0; i < n; i++) {
-    TileRec& T = P.tiles[base + i];
-    for (int f = 0; f < 6; f++) T.nbr_parent[f] = T.nbr[f] < 0 ? -1 : P.tiles[T.nbr[f]].parent;
+    TileRec& T = B.tiles[i];
+    for (int f = 0; f < 6; f++) T.nbr_parent[f] = T.nbr[f] < 0 ? -1 : B.tiles[T.nbr[f] - base].parent;
+  }
+}
+
+// Deepest level C of a tree kept in LDS by the coarse kernel: levels 0..C hold
+// at most CL_MAX_TILES tiles in total, at most CL_MAX_PER_LEVEL on any level,
+// and no GHOST tile (-1 if level 0 alone is larger).  (A GHOST tile needs a LEAF
+// tile one level up beside a NONLEAF one, i.e. at least 14 tiles on two levels,
+// so the size bound alone already excludes them; the check keeps the kernel's
+// assumption explicit.)  CIRRUS_COARSE_LEVELS=k caps it at k levels, for
+// measurement.
+int coarse_depth(const std::vector<int>& lev, const std::vector<uint8_t>& ty) {
+  int L = 0;
+  for (int l : lev) L = std::max(L, l);
+  std::vector<int> cnt(L + 1, 0), cnt_act(L + 1, 0);
+  for (size_t i = 0; i < lev.size(); i++) {
+    cnt[lev[i]]++;
+    if (ty[i] != GHOST) cnt_act[lev[i]]++;
+  }
+  int C = -1, total = 0;
+  while (C + 1 <= L && C + 1 < CL_MAX_LEVELS && total + cnt[C + 1] <= CL_MAX_TILES && cnt[C + 1] <= CL_MAX_PER_LEVEL &&
+         cnt[C + 1] == cnt_act[C + 1]) {
+    C++;
+    total += cnt[C];
+  }
+  if (const char* e = std::getenv("CIRRUS_COARSE_LEVELS")) C = std::max(-1, std::min(C, std::atoi(e) - 1));
+  return C;
+}
+
+// Coarse plan of tiles [first, first + count) of `tiles` (global indices start at
+// base): slots of levels 0..C in level order; `slots` is filled in by the caller.
+void coarse_plan(const std::vector<TileRec>& tiles, int base, int C, int prob, CoarsePlan& cp, std::vector<CoarseSlot>& out) {
+  std::memset(&cp, 0, sizeof(cp));
+  cp.prob = prob;
+  cp.C = C;
+  const int n = int(tiles.size());
+  std::unordered_map<int, int> slot_of;
+  const int first = int(out.size());
+  for (int l = 0; l <= C; l++) {
+    cp.lvl_begin[l] = int(out.size()) - first;
+    for (int i = 0; i < n; i++) {
+      if (tr_level(tiles[i]) != l) continue;
+      slot_of[base + i] = int(out.size()) - first;
+      CoarseSlot cs;
+      std::memset(&cs, 0, sizeof(cs));
+      cs.tile = base + i;
+      out.push_back(cs);
+    }
+  }
+  cp.lvl_begin[C + 1] = int(out.size()) - first;
+  cp.nslots = cp.lvl_begin[C + 1];
+  for (int s = first; s < int(out.size()); s++) {
+    CoarseSlot& cs = out[s];
+    const TileRec& T = tiles[cs.tile - base];
+    for (int f = 0; f < 6; f++) cs.nbr[f] = T.nbr[f] < 0 ? -1 : slot_of.at(T.nbr[f]);
+    cs.parent = T.parent < 0 ? -1 : slot_of.at(T.parent);
+    cs.level = tr_level(T);
+    cs.h = T.h;
+    cs.type = tr_type(T);
+    cs.octant = uint8_t(tr_octant(T));
+    for (int f = 0; f < 7; f++) cs.bc[f] = T.bc[f];
   }
 }
 
@@ -262,44 +341,25 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   P.coarse_C.assign(nprob, -1);
   int leaf_total = 0, ghost_total = 0;
   for (int q = 0; q < nprob; q++) {
-    int nl = 0, ng = 0;
-    build_tree(&descs[q], q, tbase[q], leaf_total, ghost_total, P, lev, ty, nl, ng);
+    TreeBuild B;
+    build_tree(&descs[q], q, tbase[q], leaf_total, ghost_total, B);
+    for (int i = 0; i < descs[q].num_tiles; i++) {
+      P.tiles[tbase[q] + i] = B.tiles[i];
+      P.ext_of[tbase[q] + i] = tbase[q] + B.ext[i];
+      lev[tbase[q] + i] = B.lev[i];
+      ty[tbase[q] + i] = B.ty[i];
+    }
     P.leaf_begin[q] = leaf_total;
-    P.leaf_count[q] = nl;
-    leaf_total += nl;
-    ghost_total += ng;
+    P.leaf_count[q] = B.nleaf;
+    P.coarse_C[q] = coarse_depth(B.lev, B.ty);
+    leaf_total += B.nleaf;
+    ghost_total += B.nghost;
   }
   P.n_leaf = leaf_total;
   P.n_ghost = ghost_total;
   P.max_level = 0;
   for (int i = 0; i < n; i++) P.max_level = std::max(P.max_level, lev[i]);
   const int L = P.max_level;
-
-  // Coarse levels kept in LDS by one workgroup per problem: the deepest C whose
-  // levels 0..C hold at most CL_MAX_TILES tiles in total, at most
-  // CL_MAX_PER_LEVEL on any level, and no GHOST tile (-1 if level 0 alone is
-  // larger).  (A GHOST tile needs a LEAF tile one level up beside a NONLEAF one,
-  // i.e. at least 14 tiles on two levels, so the size bound alone already
-  // excludes them; the check keeps the kernel's assumption explicit.)
-  std::vector<std::vector<int>> cnt(nprob, std::vector<int>(L + 1, 0)), cnt_act(nprob, std::vector<int>(L + 1, 0));
-  for (int i = 0; i < n; i++) {
-    const int q = P.tiles[i].prob;
-    cnt[q][lev[i]]++;
-    if (ty[i] != GHOST) cnt_act[q][lev[i]]++;
-  }
-  // CIRRUS_COARSE_LEVELS=k caps the LDS-resident levels at k (0: none), for measurement
-  int forced = CL_MAX_LEVELS;
-  if (const char* e = std::getenv("CIRRUS_COARSE_LEVELS")) forced = std::atoi(e) - 1;
-  for (int q = 0; q < nprob; q++) {
-    int C = -1, total = 0;
-    while (C + 1 <= L && C + 1 < CL_MAX_LEVELS && cnt[q][C + 1] > 0 && total + cnt[q][C + 1] <= CL_MAX_TILES &&
-           cnt[q][C + 1] <= CL_MAX_PER_LEVEL && cnt[q][C + 1] == cnt_act[q][C + 1]) {
-      C++;
-      total += cnt[q][C];
-    }
-    C = std::max(-1, std::min(C, forced));
-    P.coarse_C[q] = C;
-  }
 
   // worklists (canonical order, problems one after another)
   auto& W = P.lists;
@@ -331,40 +391,12 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   // coarse plans and slots
   P.plans.clear();
   P.coarse_slots.clear();
-  std::vector<int> slot_of(n, -1);
-  std::vector<std::vector<int>> by_level(L + 1);
   for (int q = 0; q < nprob; q++) {
-    const int C = P.coarse_C[q];
-    if (C < 0) continue;
+    if (P.coarse_C[q] < 0) continue;
     CoarsePlan cp;
-    std::memset(&cp, 0, sizeof(cp));
-    cp.prob = q;
-    cp.C = C;
     const int first = int(P.coarse_slots.size());
-    for (int l = 0; l <= C; l++) {
-      cp.lvl_begin[l] = int(P.coarse_slots.size()) - first;
-      for (int i = tbase[q]; i < tbase[q + 1]; i++) {
-        if (lev[i] != l) continue;
-        slot_of[i] = int(P.coarse_slots.size()) - first;
-        CoarseSlot cs;
-        std::memset(&cs, 0, sizeof(cs));
-        cs.tile = i;
-        P.coarse_slots.push_back(cs);
-      }
-    }
-    cp.lvl_begin[C + 1] = int(P.coarse_slots.size()) - first;
-    cp.nslots = cp.lvl_begin[C + 1];
-    for (int s = first; s < int(P.coarse_slots.size()); s++) {
-      CoarseSlot& cs = P.coarse_slots[s];
-      const TileRec& T = P.tiles[cs.tile];
-      for (int f = 0; f < 6; f++) cs.nbr[f] = T.nbr[f] < 0 ? -1 : slot_of[T.nbr[f]];
-      cs.parent = T.parent < 0 ? -1 : slot_of[T.parent];
-      cs.level = tr_level(T);
-      cs.h = T.h;
-      cs.type = tr_type(T);
-      cs.octant = uint8_t(tr_octant(T));
-      for (int f = 0; f < 7; f++) cs.bc[f] = T.bc[f];
-    }
+    std::vector<TileRec> mine(P.tiles.begin() + tbase[q], P.tiles.begin() + tbase[q + 1]);
+    coarse_plan(mine, tbase[q], P.coarse_C[q], q, cp, P.coarse_slots);
     cp.slots = reinterpret_cast<const CoarseSlot*>(size_t(first));  // offset, patched after upload
     P.plans.push_back(cp);
   }
@@ -678,6 +710,8 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res) {
 }
 
 }  // namespace
+
+#include "serve.hpp"
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -1067,6 +1101,75 @@ cg_status cg_calc_cell_types_from_leafs(const cg_tree_desc* desc, uint8_t* cell_
     T.cell_types_from_leafs();
     std::memcpy(cell_type_inout, T.ctype.data(), size_t(desc->num_tiles) * 512);
   });
+}
+
+cg_status cg_server_create(const cg_server_config* cfg, cg_server** out) {
+  if (!out) { g_err = "cg_server_create: null output"; return CG_ERR_INVALID; }
+  *out = nullptr;
+  auto S = std::make_unique<cg_server>();
+  cg_status st = guard([&] { server_create(cfg, *S); });
+  if (st == CG_OK) *out = S.release();
+  return st;
+}
+
+cg_status cg_server_destroy(cg_server* s) {
+  delete s;
+  return CG_OK;
+}
+
+cg_status cg_server_admit(cg_server* s, const cg_tree_desc* desc, int32_t* instance) {
+  return guard([&] {
+    if (!s) fail(CG_ERR_INVALID, "null server");
+    server_admit(*s, desc, instance);
+  });
+}
+
+cg_status cg_server_release(cg_server* s, int32_t instance) {
+  return guard([&] {
+    if (!s) fail(CG_ERR_INVALID, "null server");
+    server_release(*s, instance);
+  });
+}
+
+cg_status cg_server_submit(cg_server* s, int32_t instance, const float* rhs, int32_t flags, int64_t tag) {
+  return guard([&] {
+    if (!s) fail(CG_ERR_INVALID, "null server");
+    server_submit(*s, instance, rhs, flags, tag);
+  });
+}
+
+cg_status cg_server_perturb_rhs(cg_server* s, int32_t instance, float rho, float sigma, uint64_t seed) {
+  return guard([&] {
+    if (!s) fail(CG_ERR_INVALID, "null server");
+    server_perturb(*s, instance, rho, sigma, seed);
+  });
+}
+
+cg_status cg_server_step(cg_server* s, int32_t* num_active) {
+  return guard([&] {
+    if (!s) fail(CG_ERR_INVALID, "null server");
+    server_step(*s, num_active);
+  });
+}
+
+cg_status cg_server_next_result(cg_server* s, int32_t wait, int32_t* got, int64_t* tag, int32_t* instance, cg_result* res,
+                                float* x) {
+  return guard([&] {
+    if (!s || !got) fail(CG_ERR_INVALID, "null argument");
+    *got = server_next(*s, wait, tag, instance, res, x) ? 1 : 0;
+  });
+}
+
+cg_status cg_server_info(const cg_server* s, int64_t* capacity_tiles, int64_t* free_tiles, int32_t* num_instances) {
+  if (!s) { g_err = "null server"; return CG_ERR_INVALID; }
+  if (capacity_tiles) *capacity_tiles = s->cap_tiles;
+  if (free_tiles) {
+    int64_t f = 0;
+    for (auto& kv : s->tiles_alloc.free_) f += kv.second;
+    *free_tiles = f;
+  }
+  if (num_instances) *num_instances = s->max_probs - int32_t(s->free_slots.size());
+  return CG_OK;
 }
 
 cg_status cg_scene_create(const cg_scene_params* sp, cg_scene** out) {

@@ -1,0 +1,537 @@
+// Serving engine of the MGPCG library (host side; included by mgpcg.hip).
+//
+// Instances are resident hierarchies: each admitted tree gets a slab of tiles,
+// leaf slots and ghost slots in device arrays sized from an HBM budget, its own
+// tile records on the device and a pinned staging buffer.  A request is a right
+// hand side for an instance (optionally warm-started from the instance's
+// current pressure).  Every cg_server_step enqueues one CG iteration over the
+// instances whose request is running -- their worklists assembled on the device
+// whenever that set changes -- so requests that arrive while others run join at
+// the next iteration boundary (continuous batching) and each result is the same
+// as solving the request alone.
+#pragma once
+
+#include <deque>
+#include <map>
+
+namespace {
+
+// First-fit allocator of index ranges [0, cap).
+struct RangeAlloc {
+  std::map<int64_t, int64_t> free_;  // start -> length
+  void init(int64_t cap) {
+    free_.clear();
+    if (cap > 0) free_[0] = cap;
+  }
+  int64_t alloc(int64_t n) {
+    if (n <= 0) return 0;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < n) continue;
+      const int64_t at = it->first, len = it->second;
+      free_.erase(it);
+      if (len > n) free_[at + n] = len - n;
+      return at;
+    }
+    return -1;
+  }
+  void release(int64_t at, int64_t n) {
+    if (n <= 0) return;
+    auto it = free_.emplace(at, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+enum InstState { INST_IDLE = 0, INST_QUEUED = 1, INST_ACTIVE = 2, INST_DONE = 3 };
+
+struct Instance {
+  int slot = -1;
+  int64_t tile_begin = 0, leaf_begin = 0, ghost_begin = 0;
+  int ntiles = 0, nleaf = 0, nghost = 0;
+  int max_level = 0, C = -1;
+  std::vector<int> ext;  // internal tile i -> caller index
+  TileRec* d_recs = nullptr;
+  DList leaf, gather, applyl, ghosts, all;
+  std::vector<DList> actg, lvlg;
+  CoarseSlot* d_slots = nullptr;
+  CoarsePlan plan;
+  float* h_buf = nullptr;  // pinned: rhs on the way in, pressure on the way out
+  hipEvent_t ev = nullptr;
+  int state = INST_IDLE;
+  int warm = 0;
+  int dev_rhs = 0;      // the right-hand side is already on the device
+  int no_download = 0;  // the pressure stays on the device
+  int64_t tag = 0;
+  std::chrono::steady_clock::time_point t_submit;
+  cg_result res{};
+  const TileRec* L(const DList& d) const { return d_recs + d.off; }
+};
+
+}  // namespace
+
+struct cg_server {
+  cg_params prm{};
+  VParams v;
+  int max_probs = 0;
+  int64_t cap_tiles = 0, cap_leaf = 0, cap_ghost = 0;
+  RangeAlloc tiles_alloc, leaf_alloc, ghost_alloc;
+  std::vector<int> free_slots;
+  std::vector<std::unique_ptr<Instance>> inst;  // by slot
+  std::vector<int> active;                      // slots whose request runs, assembly order
+  std::vector<int> snap_active[2];              // active set of the iteration each snapshot follows
+  bool dirty = false;
+  std::deque<int> done_q;                       // slots with a finished request, in completion order
+  int64_t step_no = 0;
+  bool snap_pending[2] = {false, false};
+
+  // device
+  cg_problem view;  // the kernels' view of the assembled active set (owner = false)
+  uint8_t* d_ctype = nullptr;
+  float* d_ch[CG_NUM_CHANNELS] = {};
+  float* d_zb = nullptr;
+  float* d_gsum = nullptr;
+  ProbState* d_ps = nullptr;
+  int32_t* d_dflag = nullptr;
+  int32_t* d_dflag_init = nullptr;
+  int32_t* d_pmap = nullptr;
+  int32_t* d_newmap = nullptr;
+  double* d_part = nullptr;
+  double* d_part2 = nullptr;
+  TileRec* d_asm = nullptr;
+  int64_t asm_cap = 0;
+  CoarsePlan* d_plans = nullptr;
+  const TileRec** d_srcs = nullptr;
+  int32_t* d_offs = nullptr;
+  int32_t* d_counts = nullptr;
+  int runs_cap = 0;
+  hipStream_t stream = nullptr;
+  // pinned host memory: ProbState snapshots, per-slot ProbState staging, per-step small tables
+  ProbState* h_snap[2] = {nullptr, nullptr};
+  hipEvent_t ev_snap[2] = {nullptr, nullptr};
+  ProbState* h_ps_stage = nullptr;
+  char* h_tab[2] = {nullptr, nullptr};
+  size_t tab_cap = 0;
+  ~cg_server();
+};
+
+cg_server::~cg_server() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& I : inst) {
+    if (!I) continue;
+    if (I->d_recs) (void)hipFree(I->d_recs);
+    if (I->d_slots) (void)hipFree(I->d_slots);
+    if (I->h_buf) (void)hipHostFree(I->h_buf);
+    if (I->ev) (void)hipEventDestroy(I->ev);
+  }
+  void* ptrs[] = {d_ctype, d_zb, d_gsum, d_ps, d_dflag, d_dflag_init, d_pmap, d_newmap, d_part, d_part2, d_asm,
+                  d_plans, (void*)d_srcs, d_offs, d_counts};
+  for (void* q : ptrs) if (q) (void)hipFree(q);
+  for (auto q : d_ch) if (q) (void)hipFree(q);
+  for (auto e : ev_snap) if (e) (void)hipEventDestroy(e);
+  for (auto q : h_snap) if (q) (void)hipHostFree(q);
+  for (auto q : h_tab) if (q) (void)hipHostFree(q);
+  if (h_ps_stage) (void)hipHostFree(h_ps_stage);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+// Device bytes one tile costs: 6 channels + the V-cycle scratch (FP32), cell
+// types, ghost row sums, dot partials and room for its records in the
+// assembled worklists.
+constexpr int kRecKindsPerTile = 5;  // leaf, gather, applyl, actg, lvlg: at most 5 records per tile
+constexpr int64_t kBytesPerTile =
+    (CG_NUM_CHANNELS + 1) * 512 * 4 + 512 + 64 * 4 + 2 * 8 + kRecKindsPerTile * int64_t(sizeof(TileRec));
+
+void server_create(const cg_server_config* cfg, cg_server& S) {
+  if (!cfg || cfg->budget_bytes <= 0 || cfg->max_problems <= 0) fail(CG_ERR_INVALID, "cg_server_create: bad configuration");
+  const cg_params& prm = cfg->params;
+  if (prm.max_iters < 0 || !(prm.rel_tol >= 0) || prm.level_iters < 0 || prm.coarsest_iters < 0)
+    fail(CG_ERR_INVALID, "cg_server_create: invalid solver parameters");
+  if (prm.smoother != CG_SMOOTH_RBGS && prm.smoother != CG_SMOOTH_JACOBI) fail(CG_ERR_INVALID, "cg_server_create: unknown smoother");
+  S.prm = prm;
+  S.v.nu = prm.level_iters;
+  S.v.coarsest = prm.coarsest_iters;
+  S.v.smoother = prm.smoother;
+  S.v.omega = prm.omega;
+  S.v.ooa = prm.one_over_alpha;
+  S.v.pcoef = prm.prolong_coeff;
+  S.max_probs = cfg->max_problems;
+  S.cap_tiles = std::max<int64_t>(64, cfg->budget_bytes / kBytesPerTile);
+  if (S.cap_tiles * 512 > int64_t(INT32_MAX)) S.cap_tiles = int64_t(INT32_MAX) / 512;  // cell offsets stay 32-bit safe
+  S.cap_leaf = S.cap_tiles;
+  S.cap_ghost = S.cap_tiles;
+  S.tiles_alloc.init(S.cap_tiles);
+  S.leaf_alloc.init(S.cap_leaf);
+  S.ghost_alloc.init(S.cap_ghost);
+  S.inst.resize(S.max_probs);
+  for (int q = S.max_probs - 1; q >= 0; q--) S.free_slots.push_back(q);
+
+  HIPC(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+  const size_t cells = size_t(S.cap_tiles) * 512;
+  HIPC(hipMalloc(&S.d_ctype, cells));
+  for (int c = 0; c < CG_NUM_CHANNELS; c++) HIPC(hipMalloc(&S.d_ch[c], cells * sizeof(float)));
+  HIPC(hipMalloc(&S.d_zb, cells * sizeof(float)));
+  HIPC(hipMalloc(&S.d_gsum, size_t(S.cap_ghost) * 64 * sizeof(float)));
+  HIPC(hipMalloc(&S.d_ps, sizeof(ProbState) * S.max_probs));
+  HIPC(hipMemset(S.d_ps, 0, sizeof(ProbState) * S.max_probs));
+  HIPC(hipMalloc(&S.d_dflag, sizeof(int32_t) * S.max_probs));
+  HIPC(hipMalloc(&S.d_dflag_init, sizeof(int32_t) * S.max_probs));
+  HIPC(hipMalloc(&S.d_pmap, sizeof(int32_t) * S.max_probs));
+  HIPC(hipMalloc(&S.d_newmap, sizeof(int32_t) * S.max_probs));
+  HIPC(hipMalloc(&S.d_part, sizeof(double) * S.cap_leaf));
+  HIPC(hipMalloc(&S.d_part2, sizeof(double) * S.cap_leaf));
+  S.asm_cap = S.cap_tiles * kRecKindsPerTile;
+  HIPC(hipMalloc(&S.d_asm, sizeof(TileRec) * S.asm_cap));
+  HIPC(hipMalloc(&S.d_plans, sizeof(CoarsePlan) * S.max_probs));
+  S.runs_cap = S.max_probs * (3 + 2 * 32);
+  HIPC(hipMalloc(&S.d_srcs, sizeof(TileRec*) * S.runs_cap));
+  HIPC(hipMalloc(&S.d_offs, sizeof(int32_t) * S.runs_cap));
+  HIPC(hipMalloc(&S.d_counts, sizeof(int32_t) * S.runs_cap));
+  for (int k = 0; k < 2; k++) {
+    HIPC(hipHostMalloc(&S.h_snap[k], sizeof(ProbState) * S.max_probs, hipHostMallocDefault));
+    HIPC(hipEventCreateWithFlags(&S.ev_snap[k], hipEventDisableTiming));
+  }
+  HIPC(hipHostMalloc(&S.h_ps_stage, sizeof(ProbState) * S.max_probs, hipHostMallocDefault));
+  // per-step tables: runs (srcs, offs, counts), active map, new map, coarse plans
+  S.tab_cap = size_t(S.runs_cap) * (sizeof(TileRec*) + 8) + size_t(S.max_probs) * (8 + sizeof(CoarsePlan)) + 4096;
+  for (int k = 0; k < 2; k++) HIPC(hipHostMalloc(&S.h_tab[k], S.tab_cap, hipHostMallocDefault));
+
+  // the kernels' view: shares the server's arrays
+  cg_problem& P = S.view;
+  P.owner = false;
+  P.stream = S.stream;
+  P.d_ctype = S.d_ctype;
+  for (int c = 0; c < CG_NUM_CHANNELS; c++) P.d_ch[c] = S.d_ch[c];
+  P.d_zb = S.d_zb;
+  P.d_gsum = S.d_gsum;
+  P.d_ps = S.d_ps;
+  P.d_dflag = S.d_dflag;
+  P.d_pmap = S.d_pmap;
+  P.d_part = S.d_part;
+  P.d_part2 = S.d_part2;
+  P.d_lists = S.d_asm;
+  P.d_plans = S.d_plans;
+  P.nprob = 0;
+}
+
+void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
+  if (!d || !out) fail(CG_ERR_INVALID, "cg_server_admit: null argument");
+  if (d->num_tiles <= 0) fail(CG_ERR_INVALID, "cg_server_admit: empty tree");
+  if (S.free_slots.empty()) fail(CG_ERR_BUSY, "cg_server_admit: no free problem slot");
+  const int n = d->num_tiles;
+  const int64_t tb = S.tiles_alloc.alloc(n);
+  if (tb < 0) fail(CG_ERR_BUSY, "cg_server_admit: HBM budget exhausted");
+  // leaf and ghost counts are known only after the build: reserve the tile count
+  // first, trim after
+  const int64_t lb = S.leaf_alloc.alloc(n), gb = S.ghost_alloc.alloc(n);
+  if (lb < 0 || gb < 0) {
+    S.tiles_alloc.release(tb, n);
+    if (lb >= 0) S.leaf_alloc.release(lb, n);
+    if (gb >= 0) S.ghost_alloc.release(gb, n);
+    fail(CG_ERR_BUSY, "cg_server_admit: HBM budget exhausted");
+  }
+  const int slot = S.free_slots.back();
+  auto I = std::make_unique<Instance>();
+  TreeBuild B;
+  try {
+    build_tree(d, slot, int(tb), int(lb), int(gb), B);
+  } catch (...) {
+    S.tiles_alloc.release(tb, n);
+    S.leaf_alloc.release(lb, n);
+    S.ghost_alloc.release(gb, n);
+    throw;
+  }
+  S.leaf_alloc.release(lb + B.nleaf, n - B.nleaf);
+  S.ghost_alloc.release(gb + B.nghost, n - B.nghost);
+  S.free_slots.pop_back();
+  I->slot = slot;
+  I->tile_begin = tb;
+  I->leaf_begin = lb;
+  I->ghost_begin = gb;
+  I->ntiles = n;
+  I->nleaf = B.nleaf;
+  I->nghost = B.nghost;
+  I->ext = B.ext;
+  I->C = coarse_depth(B.lev, B.ty);
+  I->max_level = 0;
+  for (int l : B.lev) I->max_level = std::max(I->max_level, l);
+  // records of the instance's lists, back to back
+  std::vector<TileRec> W;
+  auto add_list = [&](auto pred) {
+    DList dl;
+    dl.off = int(W.size());
+    for (int i = 0; i < n; i++) if (pred(i)) W.push_back(B.tiles[i]);
+    dl.n = int(W.size()) - dl.off;
+    return dl;
+  };
+  auto grid_level = [&](int i) { return B.lev[i] > I->C; };
+  I->leaf = add_list([&](int i) { return B.ty[i] == LEAF; });
+  I->gather = add_list([&](int i) { return B.ty[i] == LEAF && tr_has_gchild(B.tiles[i]); });
+  I->applyl = add_list([&](int i) { return B.ty[i] == LEAF || B.ty[i] == GHOST; });
+  I->ghosts = add_list([&](int i) { return B.ty[i] == GHOST; });
+  I->all = add_list([&](int) { return true; });
+  I->actg.assign(I->max_level + 1, DList());
+  I->lvlg.assign(I->max_level + 1, DList());
+  for (int l = 0; l <= I->max_level; l++) {
+    I->actg[l] = add_list([&](int i) { return B.lev[i] == l && B.ty[i] != GHOST && grid_level(i); });
+    I->lvlg[l] = add_list([&](int i) { return B.lev[i] == l && grid_level(i); });
+  }
+  HIPC(hipMalloc(&I->d_recs, sizeof(TileRec) * W.size()));
+  HIPC(hipMemcpy(I->d_recs, W.data(), sizeof(TileRec) * W.size(), hipMemcpyHostToDevice));
+  if (I->C >= 0) {
+    std::vector<CoarseSlot> slots;
+    coarse_plan(B.tiles, int(tb), I->C, slot, I->plan, slots);
+    HIPC(hipMalloc(&I->d_slots, sizeof(CoarseSlot) * slots.size()));
+    HIPC(hipMemcpy(I->d_slots, slots.data(), sizeof(CoarseSlot) * slots.size(), hipMemcpyHostToDevice));
+    I->plan.slots = I->d_slots;
+  }
+  // cell types in internal order; x = 0 so that a first warm start is a cold one
+  std::vector<uint8_t> ct(size_t(n) * 512);
+  for (int i = 0; i < n; i++) std::memcpy(&ct[size_t(i) * 512], d->cell_type + size_t(B.ext[i]) * 512, 512);
+  HIPC(hipMemcpy(S.d_ctype + size_t(tb) * 512, ct.data(), ct.size(), hipMemcpyHostToDevice));
+  HIPC(hipMemset(S.d_ch[CG_CH_X] + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float)));
+  HIPC(hipHostMalloc(&I->h_buf, size_t(n) * 512 * sizeof(float), hipHostMallocDefault));
+  HIPC(hipEventCreateWithFlags(&I->ev, hipEventDisableTiming));
+  *out = slot;
+  S.inst[slot] = std::move(I);
+}
+
+Instance& instance(cg_server& S, int32_t id) {
+  if (id < 0 || id >= S.max_probs || !S.inst[id]) fail(CG_ERR_INVALID, "unknown instance");
+  return *S.inst[id];
+}
+
+void server_release(cg_server& S, int32_t id) {
+  Instance& I = instance(S, id);
+  if (I.state == INST_QUEUED || I.state == INST_ACTIVE) fail(CG_ERR_STATE, "cg_server_release: the instance has a request running");
+  HIPC(hipStreamSynchronize(S.stream));  // no kernel may still read its slab
+  if (I.state == INST_DONE)
+    for (auto it = S.done_q.begin(); it != S.done_q.end(); ++it)
+      if (*it == id) { S.done_q.erase(it); break; }
+  S.tiles_alloc.release(I.tile_begin, I.ntiles);
+  S.leaf_alloc.release(I.leaf_begin, I.nleaf);
+  S.ghost_alloc.release(I.ghost_begin, I.nghost);
+  HIPC(hipFree(I.d_recs));
+  I.d_recs = nullptr;
+  if (I.d_slots) HIPC(hipFree(I.d_slots));
+  I.d_slots = nullptr;
+  HIPC(hipHostFree(I.h_buf));
+  I.h_buf = nullptr;
+  HIPC(hipEventDestroy(I.ev));
+  I.ev = nullptr;
+  S.inst[id].reset();
+  S.free_slots.push_back(id);
+}
+
+void server_submit(cg_server& S, int32_t id, const float* rhs, int32_t flags, int64_t tag) {
+  Instance& I = instance(S, id);
+  if (I.state != INST_IDLE) fail(CG_ERR_STATE, "cg_server_submit: the instance has a request running or an unread result");
+  if (rhs)
+    for (int i = 0; i < I.ntiles; i++) std::memcpy(I.h_buf + size_t(i) * 512, rhs + size_t(I.ext[i]) * 512, 512 * sizeof(float));
+  I.dev_rhs = rhs ? 0 : 1;
+  I.no_download = (flags & CG_SUBMIT_NO_DOWNLOAD) ? 1 : 0;
+  I.warm = (flags & CG_SUBMIT_WARM) ? 1 : 0;
+  I.tag = tag;
+  I.t_submit = std::chrono::steady_clock::now();
+  I.state = INST_QUEUED;
+}
+
+// Start queued requests: ProbState, rhs upload, x0 (zero for a cold start),
+// r0 = b - A x0 on the instance's own lists, |b|^2.
+void server_start_queued(cg_server& S, int k) {
+  std::vector<int> fresh;
+  for (int q = 0; q < S.max_probs; q++)
+    if (S.inst[q] && S.inst[q]->state == INST_QUEUED) fresh.push_back(q);
+  if (fresh.empty()) return;
+  const hipStream_t st = S.stream;
+  int32_t* hmap = reinterpret_cast<int32_t*>(S.h_tab[k] + S.tab_cap - 2048);  // tail of this step's table
+  const int nf = std::min<int>(int(fresh.size()), 2048 / 4);
+  Ctx c = S.view.ctx();
+  for (int f = 0; f < nf; f++) {
+    Instance& I = *S.inst[fresh[f]];
+    ProbState& ps = S.h_ps_stage[I.slot];
+    std::memset(&ps, 0, sizeof(ps));
+    ps.tol2 = S.prm.rel_tol * S.prm.rel_tol;
+    ps.max_iters = S.prm.max_iters;
+    ps.leaf_begin = int32_t(I.leaf_begin);
+    ps.leaf_count = I.nleaf;
+    ps.pure_neumann = S.prm.pure_neumann;
+    ps.check_stride = S.prm.sync_stride;
+    HIPC(hipMemcpyAsync(S.d_ps + I.slot, &ps, sizeof(ps), hipMemcpyHostToDevice, st));
+    if (!I.dev_rhs)
+      HIPC(hipMemcpyAsync(S.d_ch[CG_CH_B] + size_t(I.tile_begin) * 512, I.h_buf, size_t(I.ntiles) * 512 * sizeof(float),
+                          hipMemcpyHostToDevice, st));
+    float* X = S.d_ch[CG_CH_X];
+    if (!I.warm) k_fill_tiles<<<I.all.n, 64, 0, st>>>(I.L(I.all), I.all.n, X, 0.f);
+    if (I.ghosts.n) k_propagate_ghosts<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, X, X);
+    k_apply<<<I.applyl.n, 64, 0, st>>>(c, I.L(I.applyl), I.applyl.n, X, c.ap, -1, 0, 0, 0);
+    if (I.ghosts.n) k_accumulate_ghosts<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, c.ap);
+    k_cg_init_warm<<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_ch[CG_CH_B], 0);
+    hmap[f] = I.slot;
+    I.state = INST_ACTIVE;
+    S.active.push_back(I.slot);
+    S.dirty = true;
+  }
+  HIPC(hipMemcpyAsync(S.d_newmap, hmap, sizeof(int32_t) * nf, hipMemcpyHostToDevice, st));
+  Ctx ci = c;
+  ci.pmap = S.d_newmap;
+  ci.dflag = S.d_dflag_init;
+  k_finalize<<<unsigned(nf), FIN_THREADS, 0, st>>>(ci, FIN_BB, 0);
+  HIPC(hipGetLastError());
+}
+
+// Assemble the worklists of the active set on the device (records copied from
+// each instance's own lists), with its problem map and coarse plans.
+void server_assemble(cg_server& S, int k) {
+  cg_problem& P = S.view;
+  const int na = int(S.active.size());
+  P.nprob = na;
+  P.plans.clear();
+  if (na == 0) return;
+  int L = 0;
+  for (int q : S.active) L = std::max(L, S.inst[q]->max_level);
+  P.max_level = L;
+  char* tab = S.h_tab[k];
+  auto** srcs = reinterpret_cast<const TileRec**>(tab);
+  auto* offs = reinterpret_cast<int32_t*>(tab + sizeof(TileRec*) * S.runs_cap);
+  auto* cnts = offs + S.runs_cap;
+  auto* pmap = cnts + S.runs_cap;
+  auto* plans = reinterpret_cast<CoarsePlan*>(reinterpret_cast<char*>(pmap + S.max_probs) + 64 -
+                                              (reinterpret_cast<uintptr_t>(pmap + S.max_probs) & 63));
+  int nr = 0;
+  int64_t pos = 0;
+  auto kind = [&](auto get) {
+    DList out;
+    out.off = int(pos);
+    for (int q : S.active) {
+      const Instance& I = *S.inst[q];
+      const DList* dl = get(I);
+      if (!dl || dl->n == 0) continue;
+      if (nr >= S.runs_cap) fail(CG_ERR_BUSY, "cg_server_step: too many worklist runs");
+      srcs[nr] = I.L(*dl);
+      offs[nr] = int32_t(pos);
+      cnts[nr] = dl->n;
+      nr++;
+      pos += dl->n;
+    }
+    out.n = int(pos) - out.off;
+    return out;
+  };
+  P.leaf = kind([](const Instance& I) { return &I.leaf; });
+  P.gather = kind([](const Instance& I) { return &I.gather; });
+  P.applyl = kind([](const Instance& I) { return &I.applyl; });
+  P.actg.assign(L + 1, DList());
+  P.lvlg.assign(L + 1, DList());
+  for (int l = 0; l <= L; l++) {
+    P.actg[l] = kind([l](const Instance& I) { return l <= I.max_level ? &I.actg[l] : nullptr; });
+    P.lvlg[l] = kind([l](const Instance& I) { return l <= I.max_level ? &I.lvlg[l] : nullptr; });
+  }
+  if (pos > S.asm_cap) fail(CG_ERR_BUSY, "cg_server_step: worklists exceed their reserved space");
+  for (int a = 0; a < na; a++) pmap[a] = S.active[a];
+  int np = 0;
+  for (int q : S.active)
+    if (S.inst[q]->C >= 0) plans[np++] = S.inst[q]->plan;
+  for (int a = 0; a < np; a++) P.plans.push_back(plans[a]);
+  const hipStream_t st = S.stream;
+  HIPC(hipMemcpyAsync(S.d_srcs, srcs, sizeof(TileRec*) * nr, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(S.d_offs, offs, sizeof(int32_t) * nr, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(S.d_counts, cnts, sizeof(int32_t) * nr, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(S.d_pmap, pmap, sizeof(int32_t) * na, hipMemcpyHostToDevice, st));
+  if (np) HIPC(hipMemcpyAsync(S.d_plans, plans, sizeof(CoarsePlan) * np, hipMemcpyHostToDevice, st));
+  if (nr) k_gather_recs<<<dim3(64, unsigned(std::min(nr, 65535))), 256, 0, st>>>(S.d_srcs, S.d_offs, S.d_counts, nr, S.d_asm);
+  HIPC(hipGetLastError());
+  S.dirty = false;
+}
+
+// Finished requests of the iteration snapshot k: results, pressure download.
+void server_collect(cg_server& S, int k) {
+  if (!S.snap_pending[k]) return;
+  HIPC(hipEventSynchronize(S.ev_snap[k]));
+  S.snap_pending[k] = false;
+  const ProbState* st = S.h_snap[k];
+  const auto now = std::chrono::steady_clock::now();
+  for (int q : S.snap_active[k]) {
+    if (!S.inst[q]) continue;
+    Instance& I = *S.inst[q];
+    if (I.state != INST_ACTIVE || !st[q].done) continue;
+    const ProbState& s = st[q];
+    I.res.iters = s.iter;
+    I.res.converged = s.converged;
+    I.res.bb = s.bb;
+    I.res.rr = s.rr;
+    I.res.rel_res = s.bb > 0 ? (S.prm.sync_stride > 0 ? std::sqrt(s.rr / s.bb) : -1.0) : 0.0;
+    I.res.seconds = std::chrono::duration<double>(now - I.t_submit).count();
+    if (!I.no_download)
+      HIPC(hipMemcpyAsync(I.h_buf, S.d_ch[CG_CH_X] + size_t(I.tile_begin) * 512, size_t(I.ntiles) * 512 * sizeof(float),
+                          hipMemcpyDeviceToHost, S.stream));
+    HIPC(hipEventRecord(I.ev, S.stream));
+    I.state = INST_DONE;
+    S.done_q.push_back(q);
+    S.active.erase(std::find(S.active.begin(), S.active.end(), q));
+    S.dirty = true;
+  }
+}
+
+// One step: an iteration over the current active set, then (one iteration
+// behind, so the device keeps working) finished requests, new requests and a
+// reassembly of the worklists for the next step.
+void server_step(cg_server& S, int32_t* n_active) {
+  const int k = int(S.step_no & 1);
+  cg_problem& P = S.view;
+  if (!S.active.empty() && !S.dirty) {
+    enqueue_cg_iteration(P, S.v, S.prm.pure_neumann);
+    HIPC(hipGetLastError());
+    int maxslot = 0;
+    for (int q : S.active) maxslot = std::max(maxslot, q + 1);
+    HIPC(hipMemcpyAsync(S.h_snap[k], S.d_ps, sizeof(ProbState) * maxslot, hipMemcpyDeviceToHost, S.stream));
+    HIPC(hipEventRecord(S.ev_snap[k], S.stream));
+    S.snap_active[k] = S.active;
+    S.snap_pending[k] = true;
+  }
+  server_collect(S, 1 - k);
+  server_start_queued(S, k);
+  if (S.dirty) server_assemble(S, k);
+  S.step_no++;
+  if (n_active) *n_active = int32_t(S.active.size());
+}
+
+// Seeded on-device update of an idle instance's right-hand side (synthetic client).
+void server_perturb(cg_server& S, int32_t id, float rho, float sigma, uint64_t seed) {
+  Instance& I = instance(S, id);
+  if (I.state != INST_IDLE) fail(CG_ERR_STATE, "cg_server_perturb_rhs: the instance has a request running or an unread result");
+  k_perturb_rhs<<<I.leaf.n, 64, 0, S.stream>>>(S.view.ctx(), I.L(I.leaf), I.leaf.n, S.d_ch[CG_CH_B], rho, sigma, seed);
+  HIPC(hipGetLastError());
+}
+
+// Pop a finished request whose pressure has arrived (or wait for it if `wait`).
+bool server_next(cg_server& S, int wait, int64_t* tag, int32_t* id, cg_result* res, float* x) {
+  if (S.done_q.empty()) return false;
+  Instance& I = *S.inst[S.done_q.front()];
+  if (wait) HIPC(hipEventSynchronize(I.ev));
+  else {
+    const hipError_t e = hipEventQuery(I.ev);
+    if (e == hipErrorNotReady) return false;
+    HIPC(e);
+  }
+  S.done_q.pop_front();
+  if (tag) *tag = I.tag;
+  if (id) *id = I.slot;
+  if (res) *res = I.res;
+  if (x && !I.no_download)
+    for (int i = 0; i < I.ntiles; i++) std::memcpy(x + size_t(I.ext[i]) * 512, I.h_buf + size_t(i) * 512, 512 * sizeof(float));
+  I.state = INST_IDLE;
+  return true;
+}
+
+}  // namespace

@@ -38,7 +38,8 @@ typedef enum {
   CG_ERR_INVALID = 1,   /* bad argument or malformed tree */
   CG_ERR_HIP = 2,       /* a HIP runtime call failed */
   CG_ERR_NOMEM = 3,     /* device or host allocation failed */
-  CG_ERR_STATE = 4      /* call not valid in the handle's current state */
+  CG_ERR_STATE = 4,     /* call not valid in the handle's current state */
+  CG_ERR_BUSY = 5       /* no capacity now (HBM budget or problem slots); retry later */
 } cg_status;
 
 /* Opaque device-resident hierarchy of one octree (tiles, tables, channels). */
@@ -182,6 +183,47 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
  * the NONLEAF and GHOST cell types of desc from its LEAF cell types.
  * cell_type_inout has 512*num_tiles entries in desc's tile order. */
 cg_status cg_calc_cell_types_from_leafs(const cg_tree_desc* desc, uint8_t* cell_type_inout);
+
+/* ---- Serving (no reference counterpart) ------------------------------------
+ * A request loop over resident hierarchies.  cg_server_admit places a tree in
+ * device memory sized from the HBM budget (CG_ERR_BUSY when it does not fit:
+ * release instances and retry); it stays resident until released, and its
+ * pressure stays on the device as the next request's warm start.
+ * cg_server_submit queues a right-hand side for an idle instance;
+ * cg_server_step enqueues one CG iteration over every running request and, one
+ * iteration behind, starts queued requests and retires finished ones, so
+ * requests join and leave at iteration boundaries while the others keep
+ * running (continuous batching).  A finished request is read with
+ * cg_server_next_result; each result equals the same request solved alone.
+ * Warm start: x0 = the instance's last pressure, r0 = b - A x0, tolerance
+ * relative to |b| (a cold start uses x0 = 0, as GMGSolver::solve).  One handle
+ * must not be used from two threads at once. */
+typedef struct cg_server cg_server;
+typedef struct {
+  int64_t budget_bytes;      /* device memory for channels, cell types and worklists */
+  int32_t max_problems;      /* resident instances at most */
+  cg_params params;          /* solver settings for every request */
+} cg_server_config;
+cg_status cg_server_create(const cg_server_config* cfg, cg_server** out);
+cg_status cg_server_destroy(cg_server* s);
+cg_status cg_server_admit(cg_server* s, const cg_tree_desc* desc, int32_t* instance);
+cg_status cg_server_release(cg_server* s, int32_t instance);
+/* rhs: 512 floats per tile in the tile order of the admitted description, or
+ * null to use the right-hand side already in the instance's device memory.
+ * flags: CG_SUBMIT_WARM (start from the instance's last pressure),
+ * CG_SUBMIT_NO_DOWNLOAD (the pressure stays on the device only). */
+enum { CG_SUBMIT_WARM = 1, CG_SUBMIT_NO_DOWNLOAD = 2 };
+cg_status cg_server_submit(cg_server* s, int32_t instance, const float* rhs, int32_t flags, int64_t tag);
+/* Synthetic client: b = rho b + sigma u on the INTERIOR cells of an idle
+ * instance, u seeded unit-variance noise, on the device (for benchmarks). */
+cg_status cg_server_perturb_rhs(cg_server* s, int32_t instance, float rho, float sigma, uint64_t seed);
+cg_status cg_server_step(cg_server* s, int32_t* num_active);
+/* *got = 1 and the oldest finished request (its tag, instance, result and, if x
+ * is not null, pressure in the caller's tile order) if one is ready; with wait
+ * set, waits for that request's pressure to arrive. */
+cg_status cg_server_next_result(cg_server* s, int32_t wait, int32_t* got, int64_t* tag, int32_t* instance, cg_result* res,
+                                float* x);
+cg_status cg_server_info(const cg_server* s, int64_t* capacity_tiles, int64_t* free_tiles, int32_t* num_instances);
 
 /* ---- Synthetic scenes (the benchmark workloads) --------------------------
  * A scene is an octree built the reference's way (IterativeRefine with 2:1
