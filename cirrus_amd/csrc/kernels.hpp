@@ -69,6 +69,8 @@ struct alignas(16) ProbState {
 // Channel pointers passed to every kernel by value.
 struct Ctx {
   const uint8_t* ctype;
+  const uint8_t* fmask;  // per cell: bit e (faces x-, x+, y-, y+, z-, z+) set if the face is not cut by a
+                         // NEUMANN cell on either side, bit 6 set on INTERIOR cells (see k_face_mask)
   float* x;      // pressure / CG x
   float* r;      // rhs b, then residual r
   float* p;      // CG direction
@@ -165,19 +167,20 @@ __device__ __forceinline__ bool prob_done(const Ctx& c, const TileRec& T, int fl
 // ---------------------------------------------------------------------------
 struct TileLDS {
   float4 xs[2][100];   // pencils (i,j), i,j in -1..8, index (i+1)*10 + (j+1); [0]: k 0..3, [1]: k 4..7
-  uint64_t cs[100];    // cell types of those pencils
 };
+
+// Face-mask bits of a cell (Ctx::fmask).
+enum : int { M_INT = 64 };
 
 struct Stencil {
   float own[8];
-  uint64_t ct;         // own cell types
+  uint64_t m;          // own face masks, one byte per cell
   float xkm, xkp;      // k-halo values (z- neighbour cell (i,j,7), z+ neighbour cell (i,j,0))
-  uint8_t ckm, ckp;
   float b_int, b_f[6]; // base coefficients: inside the tile and across face f
   float h;
 };
 
-// Stage tile T's X values (zeros where the flags say so) and cell types.
+// Stage tile T's X values (zeros where the flags say so) and face masks.
 //   zero_all:   treat X as 0 everywhere (first sweep from a zero guess)
 //   ghost_zero: treat X of GHOST tiles as 0 (downstroke of the V-cycle)
 //   bcv:        the 7 face coefficients of the view (faces x-..z+, inside)
@@ -191,11 +194,10 @@ __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const
   const bool own_ghost = tr_type(T) == T_GHOST;
   if (zero_all || (ghost_zero && own_ghost)) { for (int k = 0; k < 8; k++) S.own[k] = 0.f; }
   else load8((own_ghost ? XG : X) + base, S.own);
-  S.ct = load_ct8(c.ctype + base);
+  S.m = load_ct8(c.fmask + base);
   const int pc = (i + 1) * 10 + (j + 1);
   L.xs[0][pc] = make_float4(S.own[0], S.own[1], S.own[2], S.own[3]);
   L.xs[1][pc] = make_float4(S.own[4], S.own[5], S.own[6], S.own[7]);
-  L.cs[pc] = S.ct;
   // halo ring: lanes 0..31 fetch one neighbour pencil each
   //   g = 0: x- (pencil (7,t) of nbr 0), 1: x+ ((0,t) of nbr 3), 2: y- ((t,7) of nbr 1), 3: y+ ((t,0) of nbr 4)
   if (lane < 32) {
@@ -205,41 +207,23 @@ __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const
     const int src = (g == 0) ? (56 + t) : (g == 1) ? t : (g == 2) ? (t * 8 + 7) : (t * 8);
     const int dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
     const bool gh = tr_ntype(T, f) == T_GHOST;
-    float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
-    uint64_t w = 0x0101010101010101ull;  // DIRICHLET outside the grid (GMGSolver.cu:120-122)
-    if (n >= 0) {
-      const size_t nb = size_t(n) * 512 + src * 8;
-      if (!(zero_all || (ghost_zero && gh))) {
-        const float4* q = reinterpret_cast<const float4*>((gh ? XG : X) + nb);
-        v0 = q[0];
-        v1 = q[1];
-      }
-      w = load_ct8(c.ctype + nb);
+    float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;  // x = 0 outside the grid (GMGSolver.cu:120-122)
+    if (n >= 0 && !(zero_all || (ghost_zero && gh))) {
+      const float4* q = reinterpret_cast<const float4*>((gh ? XG : X) + size_t(n) * 512 + src * 8);
+      v0 = q[0];
+      v1 = q[1];
     }
     L.xs[0][dst] = v0;
     L.xs[1][dst] = v1;
-    L.cs[dst] = w;
   }
   // k halos
   {
     int n = T.nbr[2];
-    if (n < 0) { S.xkm = 0.f; S.ckm = C_DIR; }
-    else {
-      size_t o = size_t(n) * 512 + lane * 8 + 7;
-      bool gh = tr_ntype(T, 2) == T_GHOST;
-      bool z = zero_all || (ghost_zero && gh);
-      S.xkm = z ? 0.f : (gh ? XG : X)[o];
-      S.ckm = c.ctype[o];
-    }
+    bool gh = tr_ntype(T, 2) == T_GHOST;
+    S.xkm = (n < 0 || zero_all || (ghost_zero && gh)) ? 0.f : (gh ? XG : X)[size_t(n) * 512 + lane * 8 + 7];
     n = T.nbr[5];
-    if (n < 0) { S.xkp = 0.f; S.ckp = C_DIR; }
-    else {
-      size_t o = size_t(n) * 512 + lane * 8;
-      bool gh = tr_ntype(T, 5) == T_GHOST;
-      bool z = zero_all || (ghost_zero && gh);
-      S.xkp = z ? 0.f : (gh ? XG : X)[o];
-      S.ckp = c.ctype[o];
-    }
+    gh = tr_ntype(T, 5) == T_GHOST;
+    S.xkp = (n < 0 || zero_all || (ghost_zero && gh)) ? 0.f : (gh ? XG : X)[size_t(n) * 512 + lane * 8];
   }
   S.h = T.h;
   S.b_int = bcv[6];
@@ -268,17 +252,18 @@ __device__ __forceinline__ void unpack8(float4 a, float4 b, float v[8]) {
 // The 7-point row of one cell: faces visited in the reference's order x-, x+,
 // y-, y+, z-, z+ (GMGSolver.cu:62-73); returns A x and D (times h^3, 0 off
 // INTERIOR cells).
-__device__ __forceinline__ void cell_row(float x0, uint8_t c0, const float xn[6], const uint8_t cn[6], const float bc[6],
-                                         float h, float& ax, float& dg) {
+// The NEUMANN cuts and the INTERIOR test come from the cell's face mask m: a
+// NEUMANN cell on either side of a face cuts it (coefficient 0).
+__device__ __forceinline__ void cell_row_m(float x0, uint32_t m, const float xn[6], const float bc[6], float h, float& ax,
+                                           float& dg) {
   float s = 0.f, d = 0.f;
 #pragma unroll
   for (int e = 0; e < 6; e++) {
-    // a NEUMANN cell on either side cuts the face (one integer test, no mask logic)
-    const float cf = ((c0 | cn[e]) & C_NEU) ? 0.f : bc[e];
+    const float cf = (m & (1u << e)) ? bc[e] : 0.f;
     s = __builtin_fmaf(cf, x0 - xn[e], s);
     d = d + cf;
   }
-  const bool in = c0 & C_INT;
+  const bool in = m & M_INT;
   ax = in ? ((s * h) * h) * h : 0.f;
   dg = in ? ((d * h) * h) * h : 0.f;
 }
@@ -286,7 +271,6 @@ __device__ __forceinline__ void cell_row(float x0, uint8_t c0, const float xn[6]
 // The four i/j-neighbour pencils of a lane (values and cell types).
 struct NbrPencils {
   float4 im0, im1, ip0, ip1, jm0, jm1, jp0, jp1;
-  uint64_t cim, cip, cjm, cjp;
 };
 __device__ __forceinline__ NbrPencils lds_nbrs(const TileLDS& L) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
@@ -296,7 +280,6 @@ __device__ __forceinline__ NbrPencils lds_nbrs(const TileLDS& L) {
   N.ip0 = L.xs[0][pc + 10]; N.ip1 = L.xs[1][pc + 10];
   N.jm0 = L.xs[0][pc - 1];  N.jm1 = L.xs[1][pc - 1];
   N.jp0 = L.xs[0][pc + 1];  N.jp1 = L.xs[1][pc + 1];
-  N.cim = L.cs[pc - 10]; N.cip = L.cs[pc + 10]; N.cjm = L.cs[pc - 1]; N.cjp = L.cs[pc + 1];
   return N;
 }
 
@@ -309,7 +292,6 @@ __device__ __forceinline__ void apply_pencil(const NbrPencils& N, const Stencil&
   unpack8(N.ip0, N.ip1, xip);
   unpack8(N.jm0, N.jm1, xjm);
   unpack8(N.jp0, N.jp1, xjp);
-  const uint64_t cim = N.cim, cip = N.cip, cjm = N.cjm, cjp = N.cjp;
   const float bim = (i == 0) ? S.b_f[0] : S.b_int;
   const float bip = (i == 7) ? S.b_f[3] : S.b_int;
   const float bjm = (j == 0) ? S.b_f[1] : S.b_int;
@@ -317,11 +299,9 @@ __device__ __forceinline__ void apply_pencil(const NbrPencils& N, const Stencil&
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const float xn[6] = {xim[k], xip[k], xjm[k], xjp[k], (k == 0) ? S.xkm : S.own[k - 1], (k == 7) ? S.xkp : S.own[k + 1]};
-    const uint8_t cn[6] = {ct_at(cim, k), ct_at(cip, k), ct_at(cjm, k), ct_at(cjp, k),
-                           (k == 0) ? S.ckm : ct_at(S.ct, k - 1), (k == 7) ? S.ckp : ct_at(S.ct, k + 1)};
     const float bc[6] = {bim, bip, bjm, bjp, (k == 0) ? S.b_f[2] : S.b_int, (k == 7) ? S.b_f[5] : S.b_int};
     float a, d;
-    cell_row(S.own[k], ct_at(S.ct, k), xn, cn, bc, S.h, a, d);
+    cell_row_m(S.own[k], ct_at(S.m, k), xn, bc, S.h, a, d);
     if (WANT_AX) ax[k] = a;
     if (WANT_D) dg[k] = d;
   }
@@ -357,7 +337,6 @@ __device__ __forceinline__ void gs_pencil(const NbrPencils& N, const Stencil& S,
   const float4 XKM = sel4(p, make_float4(S.xkm, OD.x, OD.y, OD.z), EV);
   const float4 XKP = sel4(p, OD, make_float4(EV.y, EV.z, EV.w, S.xkp));
   const float4 B = sel4(p, make_float4(b[0], b[2], b[4], b[6]), make_float4(b[1], b[3], b[5], b[7]));
-  const uint64_t cim = N.cim, cip = N.cip, cjm = N.cjm, cjp = N.cjp;
   const float bim = (i == 0) ? S.b_f[0] : S.b_int;
   const float bip = (i == 7) ? S.b_f[3] : S.b_int;
   const float bjm = (j == 0) ? S.b_f[1] : S.b_int;
@@ -367,21 +346,16 @@ __device__ __forceinline__ void gs_pencil(const NbrPencils& N, const Stencil& S,
 #pragma unroll
   for (int m = 0; m < 4; m++) {
     const int k0 = 2 * m;  // the cell is k0 + p
-    const int bit = 8 * k0 + sh;
-    const uint8_t c0 = uint8_t((S.ct >> bit) & 0xff);
-    const uint8_t ckm = (!p && m == 0) ? S.ckm : uint8_t((S.ct >> (bit - 8)) & 0xff);
-    const uint8_t ckp = (p && m == 3) ? S.ckp : uint8_t((S.ct >> (bit + 8)) & 0xff);
+    const uint32_t mb = uint32_t(S.m >> (8 * k0 + sh)) & 0xff;
     const float bkm = (!p && m == 0) ? S.b_f[2] : S.b_int;
     const float bkp = (p && m == 3) ? S.b_f[5] : S.b_int;
     const float x0 = comp4(X0, m);
     const float xn[6] = {comp4(XIM, m), comp4(XIP, m), comp4(XJM, m), comp4(XJP, m), comp4(XKM, m), comp4(XKP, m)};
-    const uint8_t cn[6] = {uint8_t((cim >> bit) & 0xff), uint8_t((cip >> bit) & 0xff), uint8_t((cjm >> bit) & 0xff),
-                           uint8_t((cjp >> bit) & 0xff), ckm, ckp};
     const float bc[6] = {bim, bip, bjm, bjp, bkm, bkp};
     float ax, dg;
-    cell_row(x0, c0, xn, cn, bc, S.h, ax, dg);
+    cell_row_m(x0, mb, xn, bc, S.h, ax, dg);
     float delta = 0.f;
-    if ((c0 & C_INT) && dg != 0.f) delta = (comp4(B, m) - ax) / dg;
+    if ((mb & M_INT) && dg != 0.f) delta = (comp4(B, m) - ax) / dg;
     nv[m] = x0 + delta;
   }
   // put the four new values back at their parity positions
@@ -430,7 +404,7 @@ __device__ __forceinline__ void tile_jacobi(const Ctx& c, const TileRec& T, int 
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     out[k] = S.own[k];
-    if (ct_at(S.ct, k) & C_INT) {
+    if (ct_at(S.m, k) & M_INT) {
       float res = b[k] - ax[k];
       out[k] = (dg[k] == 0.f) ? 0.f : S.own[k] + (omega * res) / dg[k];
     }
@@ -521,15 +495,16 @@ __device__ __forceinline__ float4 parent_quad(const float* XC, int parent, int o
   return *reinterpret_cast<const float4*>(XC + size_t(parent) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) +
                                          oz * 4);
 }
-// prolongated values of one pencil: active (fma with the down-sweep values) or ghost
-__device__ __forceinline__ void prolong_pencil(float4 xa, float4 xb, float4 pq, uint64_t ct, bool ghost, float pcoef,
+// prolongated values of one pencil: active (fma with the down-sweep values) or
+// ghost; m: the pencil's face masks (their INTERIOR bits)
+__device__ __forceinline__ void prolong_pencil(float4 xa, float4 xb, float4 pq, uint64_t m, bool ghost, float pcoef,
                                                float4& oa, float4& ob) {
   const float cz[4] = {pq.x, pq.y, pq.z, pq.w};
   float xo[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
   float o[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    const bool in = ct_at(ct, k) & C_INT;
+    const bool in = ct_at(m, k) & M_INT;
     o[k] = in ? (ghost ? cz[k >> 1] : __builtin_fmaf(pcoef, cz[k >> 1], xo[k])) : 0.f;
   }
   oa = make_float4(o[0], o[1], o[2], o[3]);
@@ -539,19 +514,19 @@ __device__ __forceinline__ void prolong_pencil(float4 xa, float4 xb, float4 pq, 
 // Prolongated k-halo value of lane (i,j): cell (i,j,KN) of face neighbour F.
 template <int F, int KN>
 __device__ __forceinline__ void khalo_prolong(const Ctx& c, const TileRec& T, const float* XD, const float* XO, float pcoef,
-                                              int oct, int lane, float& xv, uint8_t& cv) {
+                                              int oct, int lane, float& xv) {
   const int n = T.nbr[F];
-  if (n < 0) { xv = 0.f; cv = C_DIR; return; }
+  if (n < 0) { xv = 0.f; return; }
   const int i = lane >> 3, j = lane & 7;
   const size_t o = size_t(n) * 512 + lane * 8 + KN;
-  cv = c.ctype[o];
+  const uint8_t mv = c.fmask[o];
   const bool gh = tr_ntype(T, F) == T_GHOST;
   const int noct = oct ^ 4;
   const int ox = noct & 1, oy = (noct >> 1) & 1, oz = (noct >> 2) & 1;
   const float pz = XO[size_t(T.nbr_parent[F]) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) + oz * 4 +
                       (KN >> 1)];
   const float xo = gh ? 0.f : XD[o];
-  xv = (cv & C_INT) ? (gh ? pz : __builtin_fmaf(pcoef, pz, xo)) : 0.f;
+  xv = (mv & M_INT) ? (gh ? pz : __builtin_fmaf(pcoef, pz, xo)) : 0.f;
 }
 
 __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, const float* XD, float* XO, float pcoef,
@@ -560,7 +535,7 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
   const size_t base = size_t(T.self) * 512 + lane * 8;
   const int oct = tr_octant(T);
   const bool own_ghost = tr_type(T) == T_GHOST;
-  const uint64_t ct = load_ct8(c.ctype + base);
+  const uint64_t mown = load_ct8(c.fmask + base);
   const float4 pq = parent_quad(XO, T.parent, oct, i, j);
   float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), xb = xa;
   if (!own_ghost) {
@@ -569,7 +544,7 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
     xb = q[1];
   }
   float4 oa, ob;
-  prolong_pencil(xa, xb, pq, ct, own_ghost, pcoef, oa, ob);
+  prolong_pencil(xa, xb, pq, mown, own_ghost, pcoef, oa, ob);
   if (own_ghost) {  // GHOST tile: its prolongated values only
     if (prob_done(c, T, flags)) return;
     float4* dst = reinterpret_cast<float4*>(XO + base);
@@ -580,11 +555,10 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
   Stencil S;
   S.own[0] = oa.x; S.own[1] = oa.y; S.own[2] = oa.z; S.own[3] = oa.w;
   S.own[4] = ob.x; S.own[5] = ob.y; S.own[6] = ob.z; S.own[7] = ob.w;
-  S.ct = ct;
+  S.m = mown;
   const int pc = (i + 1) * 10 + (j + 1);
   L.xs[0][pc] = oa;
   L.xs[1][pc] = ob;
-  L.cs[pc] = ct;
   // halo ring, lanes 0..31 (as stage_tile), prolongated in the neighbour's frame
   if (lane < 32) {
     const int g = lane >> 3, t = lane & 7;
@@ -596,12 +570,11 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
     const int ni = (g == 0) ? 7 : (g == 1) ? 0 : t, nj = (g == 2) ? 7 : (g == 3) ? 0 : t;  // pencil in the neighbour
     const int dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
     const int noct = oct ^ (1 << (f % 3));
-    float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;
-    uint64_t w = 0x0101010101010101ull;  // DIRICHLET outside the grid (GMGSolver.cu:120-122)
+    float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;  // x = 0 outside the grid (GMGSolver.cu:120-122)
     if (n >= 0) {
       const size_t nb = size_t(n) * 512 + (ni * 8 + nj) * 8;
       const bool gh = tr_ntype(T, f) == T_GHOST;
-      w = load_ct8(c.ctype + nb);
+      const uint64_t w = load_ct8(c.fmask + nb);
       float4 ya = make_float4(0.f, 0.f, 0.f, 0.f), yb = ya;
       if (!gh) {
         const float4* q = reinterpret_cast<const float4*>(XD + nb);
@@ -612,11 +585,10 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
     }
     L.xs[0][dst] = va;
     L.xs[1][dst] = vb;
-    L.cs[dst] = w;
   }
   // k halos: cell (i,j,7) of the z- neighbour and (i,j,0) of the z+ neighbour
-  khalo_prolong<2, 7>(c, T, XD, XO, pcoef, oct, lane, S.xkm, S.ckm);
-  khalo_prolong<5, 0>(c, T, XD, XO, pcoef, oct, lane, S.xkp, S.ckp);
+  khalo_prolong<2, 7>(c, T, XD, XO, pcoef, oct, lane, S.xkm);
+  khalo_prolong<5, 0>(c, T, XD, XO, pcoef, oct, lane, S.xkp);
   S.h = T.h;
   S.b_int = T.bc[6];
   for (int f = 0; f < 6; f++) S.b_f[f] = T.bc[f];
@@ -718,7 +690,7 @@ struct CoarseLDS {
 // registers: LDS indices of the cell and its six neighbours (faces x-, x+, y-,
 // y+, z-, z+; CL_ZERO outside the grid), the face coefficients after the
 // NEUMANN cut, the diagonal and the rhs.  A pass then costs seven independent
-// LDS reads and the fma chain of cell_row, with the same operations in the same
+// LDS reads and the fma chain of cell_row_m, with the same operations in the same
 // order (GMGSolver.cu:56-76).
 struct CellPrep {
   int own;      // slot * 512 + cell, -1: no cell
@@ -753,7 +725,7 @@ __device__ __forceinline__ void cl_prep(const CoarseLDS& M, int s, int cc, CellP
   P.b = M.b[s][cc];
 }
 
-// A x of a prepared cell (cell_row's arithmetic); x0 returns the cell's value.
+// A x of a prepared cell (cell_row_m's arithmetic); x0 returns the cell's value.
 __device__ __forceinline__ float cl_ax(const CoarseLDS& M, const CellPrep& P, float& x0) {
   x0 = M.z[P.own];
   float xn[6];
@@ -955,6 +927,46 @@ __global__ __launch_bounds__(64) void k_cg_init_warm(Ctx c, const TileRec* __res
   if (lane == 0) c.part[T.leaf_slot] = a;
 }
 
+// Face masks of every cell of the listed tiles (Ctx::fmask): bit e of cell c is
+// set when face e (x-, x+, y-, y+, z-, z+) is not cut, i.e. neither c nor its
+// neighbour across e is NEUMANN (GMGSolver.cu:26-39; a neighbour outside the
+// grid is DIRICHLET, GMGSolver.cu:120-122); bit 6 when c is INTERIOR.  Cell
+// types do not change during a solve, so the masks are formed once.
+__global__ __launch_bounds__(64) void k_face_mask(const uint8_t* __restrict__ ctype, uint8_t* __restrict__ fmask,
+                                                  const TileRec* __restrict__ list, int n) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  __shared__ uint64_t cs[100];
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
+  const uint64_t own = load_ct8(ctype + base);
+  const int pc = (i + 1) * 10 + (j + 1);
+  cs[pc] = own;
+  const uint64_t dir = 0x0101010101010101ull;
+  if (lane < 32) {
+    const int g = lane >> 3, t = lane & 7;
+    const int nb = pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
+    const int src = (g == 0) ? (56 + t) : (g == 1) ? t : (g == 2) ? (t * 8 + 7) : (t * 8);
+    const int dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
+    cs[dst] = nb < 0 ? dir : load_ct8(ctype + size_t(nb) * 512 + src * 8);
+  }
+  const uint8_t ckm = T.nbr[2] < 0 ? uint8_t(C_DIR) : ctype[size_t(T.nbr[2]) * 512 + lane * 8 + 7];
+  const uint8_t ckp = T.nbr[5] < 0 ? uint8_t(C_DIR) : ctype[size_t(T.nbr[5]) * 512 + lane * 8];
+  wave_sync();
+  const uint64_t cim = cs[pc - 10], cip = cs[pc + 10], cjm = cs[pc - 1], cjp = cs[pc + 1];
+  uint64_t out = 0;
+  for (int k = 0; k < 8; k++) {
+    const uint8_t c0 = ct_at(own, k);
+    const uint8_t cn[6] = {ct_at(cim, k), ct_at(cip, k), ct_at(cjm, k), ct_at(cjp, k), k == 0 ? ckm : ct_at(own, k - 1),
+                           k == 7 ? ckp : ct_at(own, k + 1)};
+    uint32_t m = (c0 & C_INT) ? uint32_t(M_INT) : 0u;
+    for (int e = 0; e < 6; e++)
+      if (!((c0 | cn[e]) & C_NEU)) m |= 1u << e;
+    out |= uint64_t(m) << (8 * k);
+  }
+  *reinterpret_cast<uint64_t*>(fmask + base) = out;
+}
+
 // Seeded update of a right-hand side, for a synthetic client of the serving
 // loop: b = rho * b + sigma * u on INTERIOR cells of the listed (LEAF) tiles,
 // u uniform on [-sqrt(3), sqrt(3)] (unit variance) from a counter hash of
@@ -1067,7 +1079,8 @@ __global__ __launch_bounds__(64) void k_apply(Ctx c, const TileRec* __restrict__
   const int lane = lane_id();
   const size_t base = size_t(T.self) * 512 + lane * 8;
   if (tr_type(T) == T_GHOST && dot) {
-    for (int k = 0; k < 8; k++) gl[lane * 8 + k] = (ct_at(S.ct, k) & (C_INT | C_DIR | C_NEU)) ? ax[k] : 0.f;
+    const uint64_t ct = load_ct8(c.ctype + base);
+    for (int k = 0; k < 8; k++) gl[lane * 8 + k] = (ct_at(ct, k) & (C_INT | C_DIR | C_NEU)) ? ax[k] : 0.f;
     wave_sync();
     c.gsum[size_t(T.gslot) * 64 + lane] = octet_sum(gl, lane);
     return;

@@ -85,6 +85,7 @@ struct cg_problem {
 
   // device
   uint8_t* d_ctype = nullptr;
+  uint8_t* d_fmask = nullptr;
   float* d_ch[CG_NUM_CHANNELS] = {};
   float* d_zb = nullptr;
   float* d_gsum = nullptr;
@@ -109,6 +110,7 @@ struct cg_problem {
   Ctx ctx(float* z, float* r) const {
     Ctx c;
     c.ctype = d_ctype;
+    c.fmask = d_fmask;
     c.x = d_ch[CG_CH_X];
     c.r = r;
     c.p = d_ch[CG_CH_P];
@@ -133,7 +135,8 @@ cg_problem::~cg_problem() {
   if (graph) (void)hipGraphExecDestroy(graph);
   for (auto e : ev) if (e) (void)hipEventDestroy(e);
   if (h_pin) (void)hipHostFree(h_pin);
-  void* ptrs[] = {d_ctype, d_zb, d_gsum, d_ps, d_dflag, d_plans, d_part, d_part2, d_scalar, d_lists, d_tmp_list, d_coarse};
+  void* ptrs[] = {d_ctype, d_fmask, d_zb, d_gsum, d_ps, d_dflag, d_plans, d_part, d_part2, d_scalar, d_lists, d_tmp_list,
+                  d_coarse};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
   if (stream) (void)hipStreamDestroy(stream);
@@ -405,6 +408,7 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   HIPC(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
   const size_t cells = size_t(n) * 512;
   HIPC(hipMalloc(&P.d_ctype, cells));
+  HIPC(hipMalloc(&P.d_fmask, cells));
   for (int c = 0; c < CG_NUM_CHANNELS; c++) {
     HIPC(hipMalloc(&P.d_ch[c], cells * sizeof(float)));
     HIPC(hipMemsetAsync(P.d_ch[c], 0, cells * sizeof(float), P.stream));
@@ -438,6 +442,8 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
     std::memcpy(&ct[size_t(i) * 512], descs[q].cell_type + size_t(P.ext_of[i] - tbase[q]) * 512, 512);
   }
   HIPC(hipMemcpyAsync(P.d_ctype, ct.data(), cells, hipMemcpyHostToDevice, P.stream));
+  k_face_mask<<<P.all.n, 64, 0, P.stream>>>(P.d_ctype, P.d_fmask, P.L(P.all), P.all.n);
+  HIPC(hipGetLastError());
   HIPC(hipHostMalloc(&P.h_pin, 2 * sizeof(int) * nprob, hipHostMallocDefault));
   HIPC(hipEventCreateWithFlags(&P.ev[0], hipEventDisableTiming));
   HIPC(hipEventCreateWithFlags(&P.ev[1], hipEventDisableTiming));

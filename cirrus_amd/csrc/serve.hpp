@@ -97,6 +97,7 @@ struct cg_server {
   // device
   cg_problem view;  // the kernels' view of the assembled active set (owner = false)
   uint8_t* d_ctype = nullptr;
+  uint8_t* d_fmask = nullptr;
   float* d_ch[CG_NUM_CHANNELS] = {};
   float* d_zb = nullptr;
   float* d_gsum = nullptr;
@@ -133,7 +134,7 @@ cg_server::~cg_server() {
     if (I->h_buf) (void)hipHostFree(I->h_buf);
     if (I->ev) (void)hipEventDestroy(I->ev);
   }
-  void* ptrs[] = {d_ctype, d_zb, d_gsum, d_ps, d_dflag, d_dflag_init, d_pmap, d_newmap, d_part, d_part2, d_asm,
+  void* ptrs[] = {d_ctype, d_fmask, d_zb, d_gsum, d_ps, d_dflag, d_dflag_init, d_pmap, d_newmap, d_part, d_part2, d_asm,
                   d_plans, (void*)d_srcs, d_offs, d_counts};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
@@ -147,11 +148,11 @@ cg_server::~cg_server() {
 namespace {
 
 // Device bytes one tile costs: 6 channels + the V-cycle scratch (FP32), cell
-// types, ghost row sums, dot partials and room for its records in the
-// assembled worklists.
+// types and face masks, ghost row sums, dot partials and room for its records
+// in the assembled worklists.
 constexpr int kRecKindsPerTile = 5;  // leaf, gather, applyl, actg, lvlg: at most 5 records per tile
 constexpr int64_t kBytesPerTile =
-    (CG_NUM_CHANNELS + 1) * 512 * 4 + 512 + 64 * 4 + 2 * 8 + kRecKindsPerTile * int64_t(sizeof(TileRec));
+    (CG_NUM_CHANNELS + 1) * 512 * 4 + 2 * 512 + 64 * 4 + 2 * 8 + kRecKindsPerTile * int64_t(sizeof(TileRec));
 
 void server_create(const cg_server_config* cfg, cg_server& S) {
   if (!cfg || cfg->budget_bytes <= 0 || cfg->max_problems <= 0) fail(CG_ERR_INVALID, "cg_server_create: bad configuration");
@@ -180,6 +181,7 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   HIPC(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
   const size_t cells = size_t(S.cap_tiles) * 512;
   HIPC(hipMalloc(&S.d_ctype, cells));
+  HIPC(hipMalloc(&S.d_fmask, cells));
   for (int c = 0; c < CG_NUM_CHANNELS; c++) HIPC(hipMalloc(&S.d_ch[c], cells * sizeof(float)));
   HIPC(hipMalloc(&S.d_zb, cells * sizeof(float)));
   HIPC(hipMalloc(&S.d_gsum, size_t(S.cap_ghost) * 64 * sizeof(float)));
@@ -212,6 +214,7 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   P.owner = false;
   P.stream = S.stream;
   P.d_ctype = S.d_ctype;
+  P.d_fmask = S.d_fmask;
   for (int c = 0; c < CG_NUM_CHANNELS; c++) P.d_ch[c] = S.d_ch[c];
   P.d_zb = S.d_zb;
   P.d_gsum = S.d_gsum;
@@ -300,6 +303,9 @@ void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
   std::vector<uint8_t> ct(size_t(n) * 512);
   for (int i = 0; i < n; i++) std::memcpy(&ct[size_t(i) * 512], d->cell_type + size_t(B.ext[i]) * 512, 512);
   HIPC(hipMemcpy(S.d_ctype + size_t(tb) * 512, ct.data(), ct.size(), hipMemcpyHostToDevice));
+  // face masks on the server's stream, ahead of the instance's first request
+  k_face_mask<<<I->all.n, 64, 0, S.stream>>>(S.d_ctype, S.d_fmask, I->L(I->all), I->all.n);
+  HIPC(hipGetLastError());
   HIPC(hipMemset(S.d_ch[CG_CH_X] + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float)));
   HIPC(hipHostMalloc(&I->h_buf, size_t(n) * 512 * sizeof(float), hipHostMallocDefault));
   HIPC(hipEventCreateWithFlags(&I->ev, hipEventDisableTiming));
