@@ -27,6 +27,7 @@ DIRICHLET, INTERIOR, NEUMANN = 1, 2, 4
 CH_X, CH_B, CH_R, CH_P, CH_AP, CH_Z, CH_TMP = 0, 1, 1, 2, 3, 4, 5
 NUM_CHANNELS = 6
 SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
+STORE_F32, STORE_BF16 = 0, 1
 SCENE_UNIFORM, SCENE_SPHERE_BAND, SCENE_PLANE, SCENE_VORTEX = 0, 1, 2, 3
 (KERNEL_GS_SWEEP, KERNEL_APPLY, KERNEL_RESTRICT, KERNEL_PROLONG, KERNEL_JACOBI, KERNEL_UPDATE_XR, KERNEL_COARSE,
  KERNEL_EMPTY, KERNEL_COPY, KERNEL_GS_PROLONG) = range(10)
@@ -40,6 +41,7 @@ EXPORTED_SYMBOLS = (
     "cg_calc_cell_types_from_leafs", "cg_benchmark_kernel", "cg_scene_create", "cg_scene_destroy", "cg_scene_tree", "cg_scene_velocity",
     "cg_server_create", "cg_server_destroy", "cg_server_admit", "cg_server_release", "cg_server_submit", "cg_server_step",
     "cg_server_next_result", "cg_server_info", "cg_server_perturb_rhs",
+    "cg_refine_params_default", "cg_solve_refined", "cg_get_solution_f64",
 )
 
 
@@ -53,12 +55,26 @@ class Params(ctypes.Structure):
     _fields_ = [("max_iters", ctypes.c_int32), ("rel_tol", ctypes.c_double), ("level_iters", ctypes.c_int32),
                 ("coarsest_iters", ctypes.c_int32), ("smoother", ctypes.c_int32), ("omega", ctypes.c_float),
                 ("sync_stride", ctypes.c_int32), ("pure_neumann", ctypes.c_int32),
-                ("one_over_alpha", ctypes.c_float), ("prolong_coeff", ctypes.c_float), ("use_graph", ctypes.c_int32)]
+                ("one_over_alpha", ctypes.c_float), ("prolong_coeff", ctypes.c_float), ("use_graph", ctypes.c_int32),
+                ("storage", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int32), ("converged", ctypes.c_int32), ("rel_res", ctypes.c_double),
                 ("bb", ctypes.c_double), ("rr", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RefineParams(ctypes.Structure):
+    _fields_ = [("rel_tol", ctypes.c_double), ("inner_rel_tol", ctypes.c_double), ("max_outer", ctypes.c_int32)]
+
+
+class RefineResult(ctypes.Structure):
+    _fields_ = [("outer", ctypes.c_int32), ("iters", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("rel_res", ctypes.c_double), ("bb", ctypes.c_double), ("rr", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -134,6 +150,9 @@ def lib():
             "cg_server_step": (i32, [vp, P(i32)]),
             "cg_server_next_result": (i32, [vp, i32, P(i32), P(ctypes.c_int64), P(i32), P(Result), vp]),
             "cg_server_info": (i32, [vp, P(ctypes.c_int64), P(ctypes.c_int64), P(i32)]),
+            "cg_refine_params_default": (None, [P(RefineParams)]),
+            "cg_solve_refined": (i32, [vp, P(Params), P(RefineParams), P(RefineResult)]),
+            "cg_get_solution_f64": (i32, [vp, vp]),
             "cg_scene_create": (i32, [P(SceneParams), P(vp)]),
             "cg_scene_destroy": (i32, [vp]),
             "cg_scene_tree": (i32, [vp, P(TreeDesc), P(vp)]),
@@ -317,6 +336,12 @@ class Problem:
         _check(lib().cg_get_channel(self._h, ch, _ptr(a)), "cg_get_channel")
         return a
 
+    def get_f64(self):
+        """The FP64 x of the last refined solve, [tiles, 512] float64."""
+        a = np.zeros((self.n, 512), np.float64)
+        _check(lib().cg_get_solution_f64(self._h, _ptr(a)), "cg_get_solution_f64")
+        return a
+
     def benchmark_kernel(self, kernel, level=0, reps=50):
         """(average launch duration in us, tiles per launch) of one kernel of
         the solve, timed with hipEvents on the problem's stream."""
@@ -379,14 +404,15 @@ class GMGSolver:
         self.prolong_coeff = float(prolong_coeff)
 
     def solve(self, prob, verbose=False, max_iters=1000, relative_tolerance=1e-6, level_iters=1, coarsest_iters=10,
-              sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, use_graph=True):
+              sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, use_graph=True,
+              storage=STORE_F32):
         """Returns (iters, relative residual) like the reference; the full result
         is kept in ``self.last``."""
         p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
                            coarsest_iters=coarsest_iters, smoother=smoother, omega=omega,
                            sync_stride=int(sync_stride), pure_neumann=int(is_pure_neumann),
                            one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff,
-                           use_graph=int(use_graph))
+                           use_graph=int(use_graph), storage=int(storage))
         r = Result()
         _check(lib().cg_solve(prob.handle, ctypes.byref(p), ctypes.byref(r)), "GMGSolver.solve")
         self.last = r.as_dict()
@@ -400,18 +426,39 @@ class GMGSolver:
                "GMGSolver.VCycle")
 
     def solve_batch(self, prob, max_iters=1000, relative_tolerance=1e-6, level_iters=1, coarsest_iters=10,
-                    sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, use_graph=True):
+                    sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, use_graph=True,
+                    storage=STORE_F32):
         """All problems of a batched :class:`Problem`; returns one result dict
         per problem (iters, converged, rel_res, bb, rr, seconds)."""
         p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
                            coarsest_iters=coarsest_iters, smoother=smoother, omega=omega,
                            sync_stride=int(sync_stride), pure_neumann=int(is_pure_neumann),
                            one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff,
-                           use_graph=int(use_graph))
+                           use_graph=int(use_graph), storage=int(storage))
         res = (Result * prob.num_problems)()
         _check(lib().cg_solve_batch(prob.handle, ctypes.byref(p), res), "GMGSolver.solve_batch")
         self.last_batch = [r.as_dict() for r in res]
         return self.last_batch
+
+    def solve_refined(self, prob, relative_tolerance=1e-8, inner_tolerance=1e-4, max_outer=20, max_iters=1000,
+                      level_iters=1, coarsest_iters=10, sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS,
+                      omega=2.0 / 3.0, use_graph=True, storage=STORE_BF16):
+        """Mixed-precision form: iterative refinement with FP64 x and residual
+        around the MGPCG (cg_solve_refined); one result dict per problem
+        (outer, iters, converged, rel_res, bb, rr, seconds).  The FP64 x is
+        :meth:`Problem.get_f64`; CH_X holds it rounded to FP32."""
+        p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
+                           coarsest_iters=coarsest_iters, smoother=smoother, omega=omega,
+                           sync_stride=int(sync_stride), pure_neumann=int(is_pure_neumann),
+                           one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff,
+                           use_graph=int(use_graph), storage=int(storage))
+        rp = RefineParams()
+        lib().cg_refine_params_default(ctypes.byref(rp))
+        rp.rel_tol, rp.inner_rel_tol, rp.max_outer = float(relative_tolerance), float(inner_tolerance), int(max_outer)
+        res = (RefineResult * prob.num_problems)()
+        _check(lib().cg_solve_refined(prob.handle, ctypes.byref(p), ctypes.byref(rp), res), "GMGSolver.solve_refined")
+        self.last_refined = [r.as_dict() for r in res]
+        return self.last_refined
 
     def dampedJacobiSolve(self, prob, verbose=False, max_iters=100, relative_tolerance=1e-6, omega=2.0 / 3.0):
         r = Result()

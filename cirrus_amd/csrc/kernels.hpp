@@ -30,6 +30,7 @@ namespace cg {
 
 enum : uint8_t { T_LEAF = 1, T_NONLEAF = 2, T_GHOST = 4 };
 enum : uint8_t { C_DIR = 1, C_INT = 2, C_NEU = 4 };
+typedef __bf16 bf16;
 
 struct alignas(16) TileRec {
   int32_t nbr[6];     // same-level face neighbours x-,y-,z-,x+,y+,z+ (-1: none)
@@ -77,7 +78,10 @@ struct Ctx {
   float* ap;     // A p
   float* z;      // preconditioned residual (V-cycle x)
   float* rhsv;   // V-cycle rhs: r on LEAF tiles (plus ghost sums), restricted residual on NONLEAF
-  float* zb;     // damped-Jacobi ping-pong buffer
+  float* zb;     // damped-Jacobi ping-pong buffer; RB-GS down-sweep values
+  bf16* zh;      // z, rhsv and zb stored as bfloat16 (mixed-precision form; null until used)
+  bf16* rhsvh;
+  bf16* zbh;
   float* gsum;   // 64 per ghost: 2x2x2 sums of the ghost rows of A p
   ProbState* ps;
   int32_t* dflag; // per-problem copy of ProbState::done, polled by the host
@@ -85,6 +89,14 @@ struct Ctx {
   double* part;  // per-leaf-slot dot partials
   double* part2; // second partial stream (mean counts)
 };
+
+// The V-cycle vectors of the storage type V.
+template <class V> __device__ __forceinline__ V* vc_z(const Ctx& c);
+template <class V> __device__ __forceinline__ V* vc_rhs(const Ctx& c);
+template <> __device__ __forceinline__ float* vc_z<float>(const Ctx& c) { return c.z; }
+template <> __device__ __forceinline__ bf16* vc_z<bf16>(const Ctx& c) { return c.zh; }
+template <> __device__ __forceinline__ float* vc_rhs<float>(const Ctx& c) { return c.rhsv; }
+template <> __device__ __forceinline__ bf16* vc_rhs<bf16>(const Ctx& c) { return c.rhsvh; }
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -112,6 +124,59 @@ __device__ __forceinline__ void store8(float* __restrict__ p, const float v[8]) 
   reinterpret_cast<float4*>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
   reinterpret_cast<float4*>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
 }
+// V-cycle vectors may be stored as bfloat16 (8-bit significand, the FP32
+// exponent range) and are always computed on in FP32: a stored value widens
+// exactly, and every store rounds to nearest even (v_cvt_pk_bf16_f32).  One
+// pencil of 8 values is then one 16-byte access.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float to_f(float v) { return v; }
+__device__ __forceinline__ float to_f(bf16 v) { return float(v); }
+template <class V> __device__ __forceinline__ V from_f(float v);
+template <> __device__ __forceinline__ float from_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ bf16 from_f<bf16>(float v) { return bf16(v); }
+// the value a store of v to V-typed memory leaves (identity for FP32)
+template <class V> __device__ __forceinline__ float vround(float v) { return to_f(from_f<V>(v)); }
+__device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+__device__ __forceinline__ uint32_t bf_pack(float a, float b) { return __builtin_bit_cast(uint32_t, bf16x2{bf16(a), bf16(b)}); }
+__device__ __forceinline__ void load8(const bf16* __restrict__ p, float v[8]) {
+  const uint4 q = *reinterpret_cast<const uint4*>(p);
+  v[0] = bf_lo(q.x); v[1] = bf_hi(q.x); v[2] = bf_lo(q.y); v[3] = bf_hi(q.y);
+  v[4] = bf_lo(q.z); v[5] = bf_hi(q.z); v[6] = bf_lo(q.w); v[7] = bf_hi(q.w);
+}
+__device__ __forceinline__ void store8(bf16* __restrict__ p, const float v[8]) {
+  *reinterpret_cast<uint4*>(p) = make_uint4(bf_pack(v[0], v[1]), bf_pack(v[2], v[3]), bf_pack(v[4], v[5]), bf_pack(v[6], v[7]));
+}
+// a pencil as two float4 (k 0..3, 4..7)
+__device__ __forceinline__ void load_pencil(const float* __restrict__ p, float4& a, float4& b) {
+  const float4* q = reinterpret_cast<const float4*>(p);
+  a = q[0];
+  b = q[1];
+}
+__device__ __forceinline__ void load_pencil(const bf16* __restrict__ p, float4& a, float4& b) {
+  const uint4 q = *reinterpret_cast<const uint4*>(p);
+  a = make_float4(bf_lo(q.x), bf_hi(q.x), bf_lo(q.y), bf_hi(q.y));
+  b = make_float4(bf_lo(q.z), bf_hi(q.z), bf_lo(q.w), bf_hi(q.w));
+}
+__device__ __forceinline__ void store_pencil(float* __restrict__ p, float4 a, float4 b) {
+  float4* q = reinterpret_cast<float4*>(p);
+  q[0] = a;
+  q[1] = b;
+}
+__device__ __forceinline__ void store_pencil(bf16* __restrict__ p, float4 a, float4 b) {
+  *reinterpret_cast<uint4*>(p) = make_uint4(bf_pack(a.x, a.y), bf_pack(a.z, a.w), bf_pack(b.x, b.y), bf_pack(b.z, b.w));
+}
+// four consecutive values (a parent quad)
+__device__ __forceinline__ float4 load4(const float* __restrict__ p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ float4 load4(const bf16* __restrict__ p) {
+  const uint2 q = *reinterpret_cast<const uint2*>(p);
+  return make_float4(bf_lo(q.x), bf_hi(q.x), bf_lo(q.y), bf_hi(q.y));
+}
+__device__ __forceinline__ void store4(float* __restrict__ p, float4 a) { *reinterpret_cast<float4*>(p) = a; }
+__device__ __forceinline__ void store4(bf16* __restrict__ p, float4 a) {
+  *reinterpret_cast<uint2*>(p) = make_uint2(bf_pack(a.x, a.y), bf_pack(a.z, a.w));
+}
+
 __device__ __forceinline__ uint64_t load_ct8(const uint8_t* __restrict__ p) {
   return *reinterpret_cast<const uint64_t*>(p);
 }
@@ -186,8 +251,9 @@ struct Stencil {
 //   bcv:        the 7 face coefficients of the view (faces x-..z+, inside)
 // X holds active tiles, XG ghost tiles (they differ only for the damped-Jacobi
 // ping-pong, where ghost values stay in the primary buffer).
-__device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const float* __restrict__ X,
-                                           const float* __restrict__ XG, bool zero_all, bool ghost_zero,
+template <class V>
+__device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const V* __restrict__ X,
+                                           const V* __restrict__ XG, bool zero_all, bool ghost_zero,
                                            const float bcv[7], TileLDS& L, Stencil& S) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t base = size_t(T.self) * 512 + lane * 8;
@@ -208,11 +274,7 @@ __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const
     const int dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
     const bool gh = tr_ntype(T, f) == T_GHOST;
     float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;  // x = 0 outside the grid (GMGSolver.cu:120-122)
-    if (n >= 0 && !(zero_all || (ghost_zero && gh))) {
-      const float4* q = reinterpret_cast<const float4*>((gh ? XG : X) + size_t(n) * 512 + src * 8);
-      v0 = q[0];
-      v1 = q[1];
-    }
+    if (n >= 0 && !(zero_all || (ghost_zero && gh))) load_pencil((gh ? XG : X) + size_t(n) * 512 + src * 8, v0, v1);
     L.xs[0][dst] = v0;
     L.xs[1][dst] = v1;
   }
@@ -220,10 +282,10 @@ __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const
   {
     int n = T.nbr[2];
     bool gh = tr_ntype(T, 2) == T_GHOST;
-    S.xkm = (n < 0 || zero_all || (ghost_zero && gh)) ? 0.f : (gh ? XG : X)[size_t(n) * 512 + lane * 8 + 7];
+    S.xkm = (n < 0 || zero_all || (ghost_zero && gh)) ? 0.f : to_f((gh ? XG : X)[size_t(n) * 512 + lane * 8 + 7]);
     n = T.nbr[5];
     gh = tr_ntype(T, 5) == T_GHOST;
-    S.xkp = (n < 0 || zero_all || (ghost_zero && gh)) ? 0.f : (gh ? XG : X)[size_t(n) * 512 + lane * 8];
+    S.xkp = (n < 0 || zero_all || (ghost_zero && gh)) ? 0.f : to_f((gh ? XG : X)[size_t(n) * 512 + lane * 8]);
   }
   S.h = T.h;
   S.b_int = bcv[6];
@@ -366,14 +428,16 @@ __device__ __forceinline__ void gs_pencil(const NbrPencils& N, const Stencil& S,
 // ---------------------------------------------------------------------------
 // Per-tile bodies (one wavefront each)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void load_rhs(const Ctx& c, const TileRec& T, const float* F, int flags, float b[8]) {
-  const float* src = (flags & F_VCYCLE) ? c.rhsv : F;
+template <class V>
+__device__ __forceinline__ void load_rhs(const Ctx& c, const TileRec& T, const V* F, int flags, float b[8]) {
+  const V* src = (flags & F_VCYCLE) ? vc_rhs<V>(c) : F;
   load8(src + size_t(T.self) * 512 + lane_id() * 8, b);
 }
 
 // Red-black Gauss-Seidel colour sweep of a LEAF|NONLEAF tile (GMGSolver.cu:216-269).
 // An INTERIOR cell with D == 0 is left unchanged.
-__device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int level, float* X, const float* F, int color,
+template <class V>
+__device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int level, V* X, const V* F, int color,
                                         int flags, TileLDS& L) {
   Stencil S;
   stage_tile(c, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
@@ -383,15 +447,14 @@ __device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int leve
   wave_sync();
   float4 out0, out1;
   gs_pencil(lds_nbrs(L), S, b, color, out0, out1);
-  float4* dst = reinterpret_cast<float4*>(X + size_t(T.self) * 512 + lane_id() * 8);
-  dst[0] = out0;
-  dst[1] = out1;
+  store_pencil(X + size_t(T.self) * 512 + lane_id() * 8, out0, out1);
 }
 
 // Damped-Jacobi sweep (the update of GMGSolver.cu:508-525 with the level's
 // operator): Y = X + omega (b - A X) / D on INTERIOR cells, 0 where D == 0.
-__device__ __forceinline__ void tile_jacobi(const Ctx& c, const TileRec& T, int level, const float* X, const float* XG,
-                                            float* Y, const float* F, float omega, int flags, TileLDS& L) {
+template <class V>
+__device__ __forceinline__ void tile_jacobi(const Ctx& c, const TileRec& T, int level, const V* X, const V* XG,
+                                            V* Y, const V* F, float omega, int flags, TileLDS& L) {
   Stencil S;
   stage_tile(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
   float b[8];
@@ -433,8 +496,9 @@ __device__ __forceinline__ size_t parent_cell(const TileRec& T, int q) {
 // (GMGSolver.cu:284-339).  Output channel CO (rhsv in V-cycle mode): the
 // parent cell = sum * one_over_alpha, or += sum for a GHOST fine tile (whose
 // LEAF parent's rhs already holds r).  In V-cycle mode a ghost's rhs is 0.
-__device__ __forceinline__ void tile_restrict(const Ctx& c, const TileRec& T, int level, const float* X, const float* F,
-                                              float* CO, float one_over_alpha, int flags, TileLDS& L, float* res) {
+template <class V>
+__device__ __forceinline__ void tile_restrict(const Ctx& c, const TileRec& T, int level, const V* X, const V* F,
+                                              V* CO, float one_over_alpha, int flags, TileLDS& L, float* res) {
   Stencil S;
   stage_tile(c, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
   const int lane = lane_id();
@@ -449,23 +513,23 @@ __device__ __forceinline__ void tile_restrict(const Ctx& c, const TileRec& T, in
   wave_sync();
   float s = octet_sum(res, lane);
   const bool ghost = tr_type(T) == T_GHOST;
-  float* out = (flags & F_VCYCLE) ? c.rhsv : CO;
+  V* out = (flags & F_VCYCLE) ? vc_rhs<V>(c) : CO;
   size_t pcell = parent_cell(T, lane);
-  if (ghost) out[pcell] = out[pcell] + s;
-  else out[pcell] = s * one_over_alpha;
+  if (ghost) out[pcell] = from_f<V>(to_f(out[pcell]) + s);
+  else out[pcell] = from_f<V>(s * one_over_alpha);
 }
 
 // Prolongation and update of one tile of `level` (GMGSolver.cu:345-376):
 // ghosts take the parent value, others add prolong_coeff * parent value, on
 // INTERIOR cells; other cells become 0.  Ghost results go to XG (the primary
 // buffer) and active tiles to XO (a Jacobi ping-pong target).
-__device__ __forceinline__ void tile_prolong(const Ctx& c, const TileRec& T, int level, const float* XC, const float* XF,
-                                             float* XO, float* XG, float pcoef, int flags) {
+template <class V>
+__device__ __forceinline__ void tile_prolong(const Ctx& c, const TileRec& T, int level, const V* XC, const V* XF,
+                                             V* XO, V* XG, float pcoef, int flags) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t base = size_t(T.self) * 512 + lane * 8;
   const int oct = tr_octant(T), ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
-  const float4 pv = *reinterpret_cast<const float4*>(XC + size_t(T.parent) * 512 + ((ox * 4 + (i >> 1)) << 6) +
-                                                     ((oy * 4 + (j >> 1)) << 3) + oz * 4);
+  const float4 pv = load4(XC + size_t(T.parent) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) + oz * 4);
   const float cz[4] = {pv.x, pv.y, pv.z, pv.w};
   const uint64_t ct = load_ct8(c.ctype + base);
   const bool ghost = tr_type(T) == T_GHOST;
@@ -490,13 +554,15 @@ __device__ __forceinline__ void tile_prolong(const Ctx& c, const TileRec& T, int
 // INTERIOR cells of GHOST tiles, 0 elsewhere), so no separate launch is needed.
 // XD holds the down-sweep values, XO receives the level's result; GHOST tiles
 // in the list only store their prolongated values.
-__device__ __forceinline__ float4 parent_quad(const float* XC, int parent, int oct, int i, int j) {
+template <class V>
+__device__ __forceinline__ float4 parent_quad(const V* XC, int parent, int oct, int i, int j) {
   const int ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
-  return *reinterpret_cast<const float4*>(XC + size_t(parent) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) +
-                                         oz * 4);
+  return load4(XC + size_t(parent) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) + oz * 4);
 }
 // prolongated values of one pencil: active (fma with the down-sweep values) or
-// ghost; m: the pencil's face masks (their INTERIOR bits)
+// ghost; m: the pencil's face masks (their INTERIOR bits).  Values are rounded
+// as a store to V-typed memory would leave them.
+template <class V>
 __device__ __forceinline__ void prolong_pencil(float4 xa, float4 xb, float4 pq, uint64_t m, bool ghost, float pcoef,
                                                float4& oa, float4& ob) {
   const float cz[4] = {pq.x, pq.y, pq.z, pq.w};
@@ -505,15 +571,15 @@ __device__ __forceinline__ void prolong_pencil(float4 xa, float4 xb, float4 pq, 
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const bool in = ct_at(m, k) & M_INT;
-    o[k] = in ? (ghost ? cz[k >> 1] : __builtin_fmaf(pcoef, cz[k >> 1], xo[k])) : 0.f;
+    o[k] = in ? vround<V>(ghost ? cz[k >> 1] : __builtin_fmaf(pcoef, cz[k >> 1], xo[k])) : 0.f;
   }
   oa = make_float4(o[0], o[1], o[2], o[3]);
   ob = make_float4(o[4], o[5], o[6], o[7]);
 }
 
 // Prolongated k-halo value of lane (i,j): cell (i,j,KN) of face neighbour F.
-template <int F, int KN>
-__device__ __forceinline__ void khalo_prolong(const Ctx& c, const TileRec& T, const float* XD, const float* XO, float pcoef,
+template <int F, int KN, class V>
+__device__ __forceinline__ void khalo_prolong(const Ctx& c, const TileRec& T, const V* XD, const V* XO, float pcoef,
                                               int oct, int lane, float& xv) {
   const int n = T.nbr[F];
   if (n < 0) { xv = 0.f; return; }
@@ -523,13 +589,14 @@ __device__ __forceinline__ void khalo_prolong(const Ctx& c, const TileRec& T, co
   const bool gh = tr_ntype(T, F) == T_GHOST;
   const int noct = oct ^ 4;
   const int ox = noct & 1, oy = (noct >> 1) & 1, oz = (noct >> 2) & 1;
-  const float pz = XO[size_t(T.nbr_parent[F]) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) + oz * 4 +
-                      (KN >> 1)];
-  const float xo = gh ? 0.f : XD[o];
-  xv = (mv & M_INT) ? (gh ? pz : __builtin_fmaf(pcoef, pz, xo)) : 0.f;
+  const float pz = to_f(XO[size_t(T.nbr_parent[F]) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) +
+                           oz * 4 + (KN >> 1)]);
+  const float xo = gh ? 0.f : to_f(XD[o]);
+  xv = (mv & M_INT) ? vround<V>(gh ? pz : __builtin_fmaf(pcoef, pz, xo)) : 0.f;
 }
 
-__device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, const float* XD, float* XO, float pcoef,
+template <class V>
+__device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, const V* XD, V* XO, float pcoef,
                                                 int color, int flags, TileLDS& L) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t base = size_t(T.self) * 512 + lane * 8;
@@ -538,18 +605,12 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
   const uint64_t mown = load_ct8(c.fmask + base);
   const float4 pq = parent_quad(XO, T.parent, oct, i, j);
   float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), xb = xa;
-  if (!own_ghost) {
-    const float4* q = reinterpret_cast<const float4*>(XD + base);
-    xa = q[0];
-    xb = q[1];
-  }
+  if (!own_ghost) load_pencil(XD + base, xa, xb);
   float4 oa, ob;
-  prolong_pencil(xa, xb, pq, mown, own_ghost, pcoef, oa, ob);
+  prolong_pencil<V>(xa, xb, pq, mown, own_ghost, pcoef, oa, ob);
   if (own_ghost) {  // GHOST tile: its prolongated values only
     if (prob_done(c, T, flags)) return;
-    float4* dst = reinterpret_cast<float4*>(XO + base);
-    dst[0] = oa;
-    dst[1] = ob;
+    store_pencil(XO + base, oa, ob);
     return;
   }
   Stencil S;
@@ -576,12 +637,8 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
       const bool gh = tr_ntype(T, f) == T_GHOST;
       const uint64_t w = load_ct8(c.fmask + nb);
       float4 ya = make_float4(0.f, 0.f, 0.f, 0.f), yb = ya;
-      if (!gh) {
-        const float4* q = reinterpret_cast<const float4*>(XD + nb);
-        ya = q[0];
-        yb = q[1];
-      }
-      prolong_pencil(ya, yb, parent_quad(XO, pn, noct, ni, nj), w, gh, pcoef, va, vb);
+      if (!gh) load_pencil(XD + nb, ya, yb);
+      prolong_pencil<V>(ya, yb, parent_quad(XO, pn, noct, ni, nj), w, gh, pcoef, va, vb);
     }
     L.xs[0][dst] = va;
     L.xs[1][dst] = vb;
@@ -593,14 +650,12 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
   S.b_int = T.bc[6];
   for (int f = 0; f < 6; f++) S.b_f[f] = T.bc[f];
   float b[8];
-  load8(c.rhsv + base, b);
+  load8(vc_rhs<V>(c) + base, b);
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float4 out0, out1;
   gs_pencil(lds_nbrs(L), S, b, color, out0, out1);
-  float4* dst = reinterpret_cast<float4*>(XO + base);
-  dst[0] = out0;
-  dst[1] = out1;
+  store_pencil(XO + base, out0, out1);
 }
 
 // ---------------------------------------------------------------------------
@@ -611,33 +666,38 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
   if (w >= n) return;                            \
   const TileRec T = list[w];
 
-__global__ __launch_bounds__(64) void k_gs_color(Ctx c, const TileRec* __restrict__ list, int n, int level, float* X,
-                                                 const float* F, int color, int flags) {
+template <class V>
+__global__ __launch_bounds__(64) void k_gs_color(Ctx c, const TileRec* __restrict__ list, int n, int level, V* X,
+                                                 const V* F, int color, int flags) {
   TILE_KERNEL_PROLOGUE
   __shared__ TileLDS L;
   tile_gs(c, T, level, X, F, color, flags, L);
 }
-__global__ __launch_bounds__(64) void k_gs_prolong(Ctx c, const TileRec* __restrict__ list, int n, const float* XD,
-                                                   float* XO, float pcoef, int color, int flags) {
+template <class V>
+__global__ __launch_bounds__(64) void k_gs_prolong(Ctx c, const TileRec* __restrict__ list, int n, const V* XD,
+                                                   V* XO, float pcoef, int color, int flags) {
   TILE_KERNEL_PROLOGUE
   __shared__ TileLDS L;
   tile_gs_prolong(c, T, XD, XO, pcoef, color, flags, L);
 }
-__global__ __launch_bounds__(64) void k_jacobi(Ctx c, const TileRec* __restrict__ list, int n, int level, const float* X,
-                                               const float* XG, float* Y, const float* F, float omega, int flags) {
+template <class V>
+__global__ __launch_bounds__(64) void k_jacobi(Ctx c, const TileRec* __restrict__ list, int n, int level, const V* X,
+                                               const V* XG, V* Y, const V* F, float omega, int flags) {
   TILE_KERNEL_PROLOGUE
   __shared__ TileLDS L;
   tile_jacobi(c, T, level, X, XG, Y, F, omega, flags, L);
 }
-__global__ __launch_bounds__(64) void k_restrict(Ctx c, const TileRec* __restrict__ list, int n, int level, const float* X,
-                                                 const float* F, float* CO, float one_over_alpha, int flags) {
+template <class V>
+__global__ __launch_bounds__(64) void k_restrict(Ctx c, const TileRec* __restrict__ list, int n, int level, const V* X,
+                                                 const V* F, V* CO, float one_over_alpha, int flags) {
   TILE_KERNEL_PROLOGUE
   __shared__ TileLDS L;
   __shared__ float res[512];
   tile_restrict(c, T, level, X, F, CO, one_over_alpha, flags, L, res);
 }
+template <class V>
 __global__ __launch_bounds__(64) void k_prolong(Ctx c, const TileRec* __restrict__ list, int n, int level,
-                                                const float* XC, const float* XF, float* XO, float* XG, float pcoef, int flags) {
+                                                const V* XC, const V* XF, V* XO, V* XG, float pcoef, int flags) {
   TILE_KERNEL_PROLOGUE
   tile_prolong(c, T, level, XC, XF, XO, XG, pcoef, flags);
 }
@@ -745,8 +805,9 @@ __device__ __forceinline__ float cl_ax(const CoarseLDS& M, const CellPrep& P, fl
 // prepared rows in registers.  One launch replaces ~10 launches per coarse
 // level; the per-cell arithmetic is that of the tile kernels
 // (GMGSolver.cu:216-376, bottom :432-433).
+template <class V>
 __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, const CoarsePlan* __restrict__ plans, VCycleArgs v,
-                                                           float* Z, int flags) {
+                                                           V* Z, int flags) {
   __shared__ CoarseLDS M;
   __shared__ CoarsePlan cp;  // one workgroup per problem; in LDS, as its level table is indexed per level
   const int tid = threadIdx.x;
@@ -760,7 +821,7 @@ __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, const CoarsePl
   for (int q = tid; q < cp.nslots * 128; q += CL_THREADS) {
     const int s = q >> 7, o = (q & 127) * 4;
     const size_t g = size_t(M.S[s].tile) * 512 + o;
-    *reinterpret_cast<float4*>(&M.b[s][o]) = *reinterpret_cast<const float4*>(c.rhsv + g);
+    *reinterpret_cast<float4*>(&M.b[s][o]) = load4(vc_rhs<V>(c) + g);
     *reinterpret_cast<float4*>(&M.z[s * 512 + o]) = make_float4(0.f, 0.f, 0.f, 0.f);
     *reinterpret_cast<uint32_t*>(&M.ct[s][o]) = *reinterpret_cast<const uint32_t*>(c.ctype + g);
   }
@@ -792,7 +853,7 @@ __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, const CoarsePl
         const float ax = cl_ax(M, Q, x0);
         float delta = 0.f;
         if (Q.in && Q.dg != 0.f) delta = (Q.b - ax) / Q.dg;
-        M.z[Q.own] = x0 + delta;
+        M.z[Q.own] = vround<V>(x0 + delta);
       }
     }
     __syncthreads();
@@ -818,7 +879,7 @@ __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, const CoarsePl
     for (int col = 0; col < 2; col++)
 #pragma unroll
       for (int m = 0; m < CL_CPT; m++)
-        if (P[col][m].own >= 0) M.z[P[col][m].own] = out[col][m];
+        if (P[col][m].own >= 0) M.z[P[col][m].own] = vround<V>(out[col][m]);
     __syncthreads();
   };
   auto smooth = [&](int order) {
@@ -850,7 +911,7 @@ __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, const CoarsePl
       const CoarseSlot& S = M.S[s];
       const int a = lq >> 4, bq = (lq >> 2) & 3, cq = lq & 3;
       const int ox = S.octant & 1, oy = (S.octant >> 1) & 1, oz = (S.octant >> 2) & 1;
-      M.b[S.parent][((ox * 4 + a) << 6) | ((oy * 4 + bq) << 3) | (oz * 4 + cq)] = sum * v.ooa;
+      M.b[S.parent][((ox * 4 + a) << 6) | ((oy * 4 + bq) << 3) | (oz * 4 + cq)] = vround<V>(sum * v.ooa);
     }
     __syncthreads();
   }
@@ -866,7 +927,7 @@ __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, const CoarsePl
       const int i = cc >> 6, j = (cc >> 3) & 7, k = cc & 7;
       const int ox = S.octant & 1, oy = (S.octant >> 1) & 1, oz = (S.octant >> 2) & 1;
       const float cz = M.z[S.parent * 512 + (((ox * 4 + (i >> 1)) << 6) | ((oy * 4 + (j >> 1)) << 3) | (oz * 4 + (k >> 1)))];
-      M.z[s * 512 + cc] = (M.ct[s][cc] & C_INT) ? __builtin_fmaf(v.pcoef, cz, M.z[s * 512 + cc]) : 0.f;
+      M.z[s * 512 + cc] = (M.ct[s][cc] & C_INT) ? vround<V>(__builtin_fmaf(v.pcoef, cz, M.z[s * 512 + cc])) : 0.f;
     }
     __syncthreads();
     prep(l);
@@ -874,7 +935,7 @@ __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, const CoarsePl
   }
   for (int q = tid; q < cp.nslots * 128; q += CL_THREADS) {
     const int s = q >> 7, o = (q & 127) * 4;
-    *reinterpret_cast<float4*>(Z + size_t(M.S[s].tile) * 512 + o) = *reinterpret_cast<const float4*>(&M.z[s * 512 + o]);
+    store4(Z + size_t(M.S[s].tile) * 512 + o, *reinterpret_cast<const float4*>(&M.z[s * 512 + o]));
   }
 }
 
@@ -889,6 +950,7 @@ __global__ void k_empty(int* p) {
 // ---------------------------------------------------------------------------
 // Init: x = 0, p = 0 on LEAF tiles, V-cycle rhs = b; partials of |b|^2
 // (GMGSolver.cu:537-555).
+template <class V>
 __global__ __launch_bounds__(64) void k_cg_init(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
   TILE_KERNEL_PROLOGUE
   const int lane = lane_id();
@@ -898,7 +960,7 @@ __global__ __launch_bounds__(64) void k_cg_init(Ctx c, const TileRec* __restrict
   store8(c.p + base, zero);
   float rv[8];
   load8(c.r + base, rv);
-  store8(c.rhsv + base, rv);
+  store8(vc_rhs<V>(c) + base, rv);
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(rv[k]) * double(rv[k]);
   a = wave_sum(a);
@@ -908,6 +970,7 @@ __global__ __launch_bounds__(64) void k_cg_init(Ctx c, const TileRec* __restrict
 // Warm-started init: r = b - A x0 on LEAF tiles (A x0 in AP, formed by the
 // operator kernels), p = 0, V-cycle rhs = r; partials of |b|^2, against which
 // the tolerance is measured.
+template <class V>
 __global__ __launch_bounds__(64) void k_cg_init_warm(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ B,
                                                      int flags) {
   TILE_KERNEL_PROLOGUE
@@ -920,7 +983,7 @@ __global__ __launch_bounds__(64) void k_cg_init_warm(Ctx c, const TileRec* __res
   float zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   store8(c.p + base, zero);
   store8(c.r + base, rv);
-  store8(c.rhsv + base, rv);
+  store8(vc_rhs<V>(c) + base, rv);
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(bv[k]) * double(bv[k]);
   a = wave_sum(a);
@@ -1011,13 +1074,14 @@ __global__ __launch_bounds__(256) void k_gather_recs(const TileRec* const* __res
 }
 
 // partials of gamma = r.z over LEAF tiles (GMGSolver.cu:634)
+template <class V>
 __global__ __launch_bounds__(64) void k_cg_dot_rz(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
   TILE_KERNEL_PROLOGUE
   const int lane = lane_id();
   const size_t base = size_t(T.self) * 512 + lane * 8;
   float rv[8], zv[8];
   load8(c.r + base, rv);
-  load8(c.z + base, zv);
+  load8(vc_z<V>(c) + base, zv);
   if (prob_done(c, T, flags)) return;
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(rv[k]) * double(zv[k]);
@@ -1028,13 +1092,14 @@ __global__ __launch_bounds__(64) void k_cg_dot_rz(Ctx c, const TileRec* __restri
 // p = z + beta p on INTERIOR cells of LEAF tiles (GMGSolver.cu:642-653), then
 // the new p copied into the tile's ghost children (their x is the parent's
 // INTERIOR value, PoissonGrid.cu:585-605), ready for the operator.
+template <class V>
 __global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
   TILE_KERNEL_PROLOGUE
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t base = size_t(T.self) * 512 + lane * 8;
   float pv[8], zv[8];
   load8(c.p + base, pv);
-  load8(c.z + base, zv);
+  load8(vc_z<V>(c) + base, zv);
   const uint64_t ct = load_ct8(c.ctype + base);
   if (prob_done(c, T, flags)) return;
   const float beta = float(c.ps[T.prob].beta);
@@ -1125,6 +1190,7 @@ __global__ __launch_bounds__(64) void k_ghost_gather(Ctx c, const TileRec* __res
 
 // x += alpha p, r -= alpha A p on INTERIOR cells of LEAF tiles
 // (GMGSolver.cu:587-609); r is also the next V-cycle's rhs; partials of |r|^2.
+template <class V>
 __global__ __launch_bounds__(64) void k_cg_update_xr(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
   TILE_KERNEL_PROLOGUE
   const int lane = lane_id();
@@ -1145,7 +1211,7 @@ __global__ __launch_bounds__(64) void k_cg_update_xr(Ctx c, const TileRec* __res
   }
   store8(c.x + base, xv);
   store8(c.r + base, rv);
-  store8(c.rhsv + base, rv);
+  store8(vc_rhs<V>(c) + base, rv);
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(rv[k]) * double(rv[k]);
   a = wave_sum(a);
@@ -1242,6 +1308,7 @@ __global__ __launch_bounds__(64) void k_mean(Ctx c, const TileRec* __restrict__ 
   q = wave_sum(q);
   if (lane == 0) { c.part[T.leaf_slot] = a; c.part2[T.leaf_slot] = q; }
 }
+template <class V>
 __global__ __launch_bounds__(64) void k_subtract_mean(Ctx c, const TileRec* __restrict__ list, int n, float* __restrict__ X,
                                                       int write_rhsv, int flags) {
   TILE_KERNEL_PROLOGUE
@@ -1255,7 +1322,7 @@ __global__ __launch_bounds__(64) void k_subtract_mean(Ctx c, const TileRec* __re
   const uint64_t ct = load_ct8(c.ctype + base);
   for (int k = 0; k < 8; k++) if (ct_at(ct, k) & C_INT) v[k] = float(double(v[k]) - mean);
   store8(X + base, v);
-  if (write_rhsv) store8(c.rhsv + base, v);
+  if (write_rhsv) store8(vc_rhs<V>(c) + base, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -1328,7 +1395,8 @@ __global__ __launch_bounds__(64) void k_copy_tiles(const TileRec* __restrict__ l
   store8(dst + base, v);
 }
 // Fill whole tiles of a list with a value.
-__global__ __launch_bounds__(64) void k_fill_tiles(const TileRec* __restrict__ list, int n, float* __restrict__ dst, float val) {
+template <class V = float>
+__global__ __launch_bounds__(64) void k_fill_tiles(const TileRec* __restrict__ list, int n, V* __restrict__ dst, float val) {
   const int flags = 0;
   TILE_KERNEL_PROLOGUE
   const size_t base = size_t(T.self) * 512 + lane_id() * 8;
@@ -1362,6 +1430,221 @@ __global__ __launch_bounds__(64) void k_leaf_jacobi_update(Ctx c, const TileRec*
   for (int k = 0; k < 8; k++)
     if (ct_at(ct, k) & C_INT) x[k] = (d[k] == 0.f) ? 0.f : x[k] + (omega * r[k]) / d[k];
   store8(X + base, x);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision form: the FP64 outer loop of the iterative refinement (no
+// reference counterpart).  x64 and the residual r64 are FP64 arrays beside the
+// FP32 channels; the operator is FullNegativeLaplacian (GMGSolver.cu:203-213)
+// evaluated in FP64 with the reference's per-face order, so the CPU
+// transcription in FP64 gives the same bits.  Each inner solve is the FP32
+// MGPCG on the FP32-rounded residual; its x is added to x64.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load8d(const double* __restrict__ p, double v[8]) {
+  const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+  for (int h = 0; h < 4; h++) {
+    const double2 t = q[h];
+    v[2 * h] = t.x;
+    v[2 * h + 1] = t.y;
+  }
+}
+__device__ __forceinline__ void store8d(double* __restrict__ p, const double v[8]) {
+  double2* q = reinterpret_cast<double2*>(p);
+#pragma unroll
+  for (int h = 0; h < 4; h++) q[h] = make_double2(v[2 * h], v[2 * h + 1]);
+}
+// Face coefficient before the NEUMANN cut in FP64 (GMGSolver.cu:26-39).
+__device__ __forceinline__ double base_coeff64(uint8_t t0, uint8_t t1, double ooh) {
+  const bool both = (t0 & T_LEAF) && (t1 & T_LEAF);
+  const bool lg = ((t0 & T_LEAF) && (t1 & T_GHOST)) || ((t0 & T_GHOST) && (t1 & T_LEAF));
+  return both ? ooh * ooh : (lg ? ((2.0 * ooh) / 3.0) * ooh : 0.0);
+}
+
+// Ghost x64 from the parent's INTERIOR value (PoissonGrid.cu:585-605).
+__global__ __launch_bounds__(64) void k_propagate_ghosts64(Ctx c, const TileRec* __restrict__ list, int n, double* __restrict__ X) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
+  const int oct = tr_octant(T), ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
+  const size_t pb = size_t(T.parent) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) + oz * 4;
+  const double2 p0 = reinterpret_cast<const double2*>(X + pb)[0], p1 = reinterpret_cast<const double2*>(X + pb)[1];
+  const double cv[4] = {p0.x, p0.y, p1.x, p1.y};
+  const uint32_t pc = *reinterpret_cast<const uint32_t*>(c.ctype + pb);
+  const uint64_t ct = load_ct8(c.ctype + base);
+  double out[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const bool pin = uint8_t(pc >> (8 * (k >> 1))) & C_INT;
+    out[k] = ((ct_at(ct, k) & C_INT) && pin) ? cv[k >> 1] : 0.0;
+  }
+  store8d(X + base, out);
+}
+
+// A x64 on every cell of the listed LEAF and GHOST tiles with their own types
+// (NegativeLaplacianSameLevel128 with no subtree view, GMGSolver.cu:145-200),
+// in FP64.  Neighbour pencils are read straight from memory (this runs once per
+// refinement step, not per CG iteration).
+__global__ __launch_bounds__(64) void k_apply64(Ctx c, const TileRec* __restrict__ list, int n, const double* __restrict__ X,
+                                                double* __restrict__ AX) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t self = size_t(T.self) * 512;
+  const size_t base = self + lane * 8;
+  const double h = double(T.h);
+  const double ooh = 1.0 / h;
+  const uint8_t to = tr_type(T);
+  double bf[6];
+#pragma unroll
+  for (int f = 0; f < 6; f++) bf[f] = base_coeff64(to, T.nbr[f] < 0 ? to : tr_ntype(T, f), ooh);
+  const double bint = base_coeff64(to, to, ooh);
+  double own[8], xim[8], xip[8], xjm[8], xjp[8];
+  load8d(X + base, own);
+  // pencil offsets of the four i/j neighbours (-1: outside the grid, x = 0)
+  auto nb_pencil = [&](bool inside, int own_off, int f, int nb_off, double v[8]) {
+    const int nt = T.nbr[f];
+    if (inside) load8d(X + self + own_off * 8, v);
+    else if (nt >= 0) load8d(X + size_t(nt) * 512 + nb_off * 8, v);
+    else {
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = 0.0;
+    }
+  };
+  nb_pencil(i > 0, lane - 8, 0, 56 + j, xim);
+  nb_pencil(i < 7, lane + 8, 3, j, xip);
+  nb_pencil(j > 0, lane - 1, 1, i * 8 + 7, xjm);
+  nb_pencil(j < 7, lane + 1, 4, i * 8, xjp);
+  const double xkm = T.nbr[2] >= 0 ? X[size_t(T.nbr[2]) * 512 + lane * 8 + 7] : 0.0;
+  const double xkp = T.nbr[5] >= 0 ? X[size_t(T.nbr[5]) * 512 + lane * 8] : 0.0;
+  const uint64_t m = load_ct8(c.fmask + base);
+  const double bim = (i == 0) ? bf[0] : bint, bip = (i == 7) ? bf[3] : bint;
+  const double bjm = (j == 0) ? bf[1] : bint, bjp = (j == 7) ? bf[4] : bint;
+  double out[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const double xn[6] = {xim[k], xip[k], xjm[k], xjp[k], k == 0 ? xkm : own[k - 1], k == 7 ? xkp : own[k + 1]};
+    const double bc[6] = {bim, bip, bjm, bjp, k == 0 ? bf[2] : bint, k == 7 ? bf[5] : bint};
+    const uint32_t mk = ct_at(m, k);
+    double sum = 0.0;
+#pragma unroll
+    for (int e = 0; e < 6; e++) sum = __builtin_fma((mk & (1u << e)) ? bc[e] : 0.0, own[k] - xn[e], sum);
+    out[k] = (mk & M_INT) ? ((sum * h) * h) * h : 0.0;
+  }
+  store8d(AX + base, out);
+}
+
+// Ghost rows of A x64 summed into the parent (AccumulateToParents128 with
+// coefficient 1, additive, all cell types; PoissonGrid.cu:664-719).
+__global__ __launch_bounds__(64) void k_accumulate_ghosts64(Ctx c, const TileRec* __restrict__ list, int n, double* __restrict__ AX) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id();
+  const size_t base = size_t(T.self) * 512 + lane * 8;
+  __shared__ double gl[512];
+  double v[8];
+  load8d(AX + base, v);
+  const uint64_t ct = load_ct8(c.ctype + base);
+  for (int k = 0; k < 8; k++) gl[lane * 8 + k] = (ct_at(ct, k) & (C_INT | C_DIR | C_NEU)) ? v[k] : 0.0;
+  wave_sync();
+  const int a = lane >> 4, bq = (lane >> 2) & 3, cq = lane & 3;
+  double sum = 0.0;
+  for (int ii = 0; ii < 2; ii++)
+    for (int jj = 0; jj < 2; jj++)
+      for (int kk = 0; kk < 2; kk++) sum += gl[((2 * a + ii) * 8 + (2 * bq + jj)) * 8 + 2 * cq + kk];
+  const size_t pcell = parent_cell(T, lane);
+  AX[pcell] = AX[pcell] + sum * 1.0;
+}
+
+// |b|^2 partials of the LEAF tiles (per leaf slot).
+__global__ __launch_bounds__(64) void k_ir_norm(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ B) {
+  const int flags = F_SWIZZLE;
+  TILE_KERNEL_PROLOGUE
+  const size_t base = size_t(T.self) * 512 + lane_id() * 8;
+  float b[8];
+  load8(B + base, b);
+  double a = 0.0;
+  for (int k = 0; k < 8; k++) a += double(b[k]) * double(b[k]);
+  a = wave_sum(a);
+  if (lane_id() == 0) c.part[T.leaf_slot] = a;
+}
+// r64 = b - A x64 on every cell of the LEAF tiles (R holds A x64 on entry).
+//   mode 0: the next inner solve's rhs r = float(r64), partials of |r64|^2;
+//   mode 1 (pure Neumann, before re-centring): partials of sum r64 and of the
+//           INTERIOR count (MeanAsync, PoissonGrid.cu:231-325).
+__global__ __launch_bounds__(64) void k_ir_residual(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ B,
+                                                    double* __restrict__ R, int mode) {
+  const int flags = F_SWIZZLE;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id();
+  const size_t base = size_t(T.self) * 512 + lane * 8;
+  float b[8];
+  double r[8];
+  load8(B + base, b);
+  load8d(R + base, r);
+  for (int k = 0; k < 8; k++) r[k] = double(b[k]) - r[k];
+  store8d(R + base, r);
+  double a = 0.0, q = 0.0;
+  if (mode == 0) {
+    float rf[8];
+    for (int k = 0; k < 8; k++) { rf[k] = float(r[k]); a += r[k] * r[k]; }
+    store8(c.r + base, rf);
+  } else {
+    const uint64_t ct = load_ct8(c.ctype + base);
+    for (int k = 0; k < 8; k++) { a += r[k]; q += (ct_at(ct, k) & C_INT) ? 1.0 : 0.0; }
+    q = wave_sum(q);
+  }
+  a = wave_sum(a);
+  if (lane == 0) {
+    c.part[T.leaf_slot] = a;
+    if (mode != 0) c.part2[T.leaf_slot] = q;
+  }
+}
+// Pure Neumann: r64 -= mean on INTERIOR cells (ReCenterLeafVoxels,
+// PoissonSolver.cu:356-368, in FP64), then as k_ir_residual mode 0.
+__global__ __launch_bounds__(64) void k_ir_recenter(Ctx c, const TileRec* __restrict__ list, int n, double* __restrict__ R) {
+  const int flags = F_SWIZZLE;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id();
+  const size_t base = size_t(T.self) * 512 + lane * 8;
+  const double mean = c.ps[T.prob].mean;
+  double r[8];
+  load8d(R + base, r);
+  const uint64_t ct = load_ct8(c.ctype + base);
+  for (int k = 0; k < 8; k++) if (ct_at(ct, k) & C_INT) r[k] = r[k] - mean;
+  store8d(R + base, r);
+  float rf[8];
+  double a = 0.0;
+  for (int k = 0; k < 8; k++) { rf[k] = float(r[k]); a += r[k] * r[k]; }
+  store8(c.r + base, rf);
+  a = wave_sum(a);
+  if (lane == 0) c.part[T.leaf_slot] = a;
+}
+// x64 += x (the inner solve's correction) on LEAF tiles; finish: x = float(x64).
+__global__ __launch_bounds__(64) void k_ir_accumulate(Ctx c, const TileRec* __restrict__ list, int n, double* __restrict__ X64,
+                                                      int finish) {
+  const int flags = F_SWIZZLE;
+  TILE_KERNEL_PROLOGUE
+  const size_t base = size_t(T.self) * 512 + lane_id() * 8;
+  double xd[8];
+  float xf[8];
+  load8d(X64 + base, xd);
+  if (finish) {
+    for (int k = 0; k < 8; k++) xf[k] = float(xd[k]);
+    store8(c.x + base, xf);
+    return;
+  }
+  load8(c.x + base, xf);
+  for (int k = 0; k < 8; k++) xd[k] = xd[k] + double(xf[k]);
+  store8d(X64 + base, xd);
+}
+// Per-problem totals of the leaf-slot partials, in the fixed order.
+__global__ __launch_bounds__(FIN_THREADS) void k_prob_sums(Ctx c, double* __restrict__ out) {
+  __shared__ double lds[FIN_CHUNK];
+  const ProbState* ps = c.ps + blockIdx.x;
+  const double s = block_sum_partials(c.part, ps->leaf_begin, ps->leaf_count, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 }  // namespace cg

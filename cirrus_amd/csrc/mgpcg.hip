@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -88,6 +89,13 @@ struct cg_problem {
   uint8_t* d_fmask = nullptr;
   float* d_ch[CG_NUM_CHANNELS] = {};
   float* d_zb = nullptr;
+  bf16* d_zh = nullptr;     // bfloat16 z, V-cycle rhs and scratch (allocated at the first such solve)
+  bf16* d_rhsvh = nullptr;
+  bf16* d_zbh = nullptr;
+  double* d_x64 = nullptr;  // iterative refinement: FP64 x and residual, the caller's b, per-problem sums
+  double* d_r64 = nullptr;
+  float* d_bsave = nullptr;
+  double* d_psum = nullptr;
   float* d_gsum = nullptr;
   ProbState* d_ps = nullptr;
   int32_t* d_dflag = nullptr;
@@ -118,6 +126,9 @@ struct cg_problem {
     c.z = z;
     c.rhsv = d_ch[CG_CH_TMP];
     c.zb = d_zb;
+    c.zh = d_zh;
+    c.rhsvh = d_rhsvh;
+    c.zbh = d_zbh;
     c.gsum = d_gsum;
     c.ps = d_ps;
     c.dflag = d_dflag;
@@ -135,8 +146,8 @@ cg_problem::~cg_problem() {
   if (graph) (void)hipGraphExecDestroy(graph);
   for (auto e : ev) if (e) (void)hipEventDestroy(e);
   if (h_pin) (void)hipHostFree(h_pin);
-  void* ptrs[] = {d_ctype, d_fmask, d_zb, d_gsum, d_ps, d_dflag, d_plans, d_part, d_part2, d_scalar, d_lists, d_tmp_list,
-                  d_coarse};
+  void* ptrs[] = {d_ctype, d_fmask, d_zb, d_zh, d_rhsvh, d_zbh, d_x64, d_r64, d_bsave, d_psum, d_gsum, d_ps, d_dflag,
+                  d_plans, d_part, d_part2, d_scalar, d_lists, d_tmp_list, d_coarse};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
   if (stream) (void)hipStreamDestroy(stream);
@@ -455,13 +466,16 @@ void check_channel(int ch) {
 }
 
 // upload every problem's ProbState (synchronous)
-void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann, int check_stride = 1) {
+// tol_q / maxit_q: per-problem tolerance and iteration cap (null: tol / max_iters for all)
+void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann, int check_stride = 1,
+                    const double* tol_q = nullptr, const int* maxit_q = nullptr) {
   std::vector<ProbState> st(P.nprob);
   for (int q = 0; q < P.nprob; q++) {
     ProbState& s = st[q];
     std::memset(&s, 0, sizeof(s));
-    s.tol2 = tol * tol;
-    s.max_iters = max_iters;
+    const double t = tol_q ? tol_q[q] : tol;
+    s.tol2 = t * t;
+    s.max_iters = maxit_q ? maxit_q[q] : max_iters;
     s.leaf_begin = P.leaf_begin[q];
     s.leaf_count = P.leaf_count[q];
     s.pure_neumann = pure_neumann;
@@ -481,9 +495,32 @@ struct VParams {
   int check_done = 1;
 };
 
-void launch_gs(cg_problem& P, const DList& a, const Ctx& c, int l, float* Z, const float* R, int color, int flags) {
+template <class V>
+void launch_gs(cg_problem& P, const DList& a, const Ctx& c, int l, V* Z, const V* R, int color, int flags) {
   if (a.n == 0) return;
-  k_gs_color<<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
+  k_gs_color<V><<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
+}
+// The pool's V-cycle vectors of storage type V.
+template <class V> V* pool_z(cg_problem& P);
+template <class V> V* pool_zb(cg_problem& P);
+template <> float* pool_z<float>(cg_problem& P) { return P.d_ch[CG_CH_Z]; }
+template <> bf16* pool_z<bf16>(cg_problem& P) { return P.d_zh; }
+template <> float* pool_zb<float>(cg_problem& P) { return P.d_zb; }
+template <> bf16* pool_zb<bf16>(cg_problem& P) { return P.d_zbh; }
+// bfloat16 V-cycle vectors, allocated once per pool when first needed
+void ensure_bf16(cg_problem& P) {
+  if (P.d_zh) return;
+  if (!P.owner) fail(CG_ERR_INVALID, "bfloat16 storage was not configured for this pool");
+  const size_t cells = size_t(P.n) * 512;
+  bf16** bufs[3] = {&P.d_zh, &P.d_rhsvh, &P.d_zbh};
+  for (bf16** b : bufs) {
+    HIPC(hipMalloc(b, cells * sizeof(bf16)));
+    HIPC(hipMemsetAsync(*b, 0, cells * sizeof(bf16), P.stream));
+  }
+  HIPC(hipStreamSynchronize(P.stream));
+  if (P.graph) HIPC(hipGraphExecDestroy(P.graph));
+  P.graph = nullptr;
+  P.graph_key.clear();
 }
 // One V-cycle Z <- M^-1 R (GMGSolver::VCycle, GMGSolver.cu:383-447).  The
 // V-cycle rhs of LEAF tiles (rhsv) must hold R on entry.  The zero initial
@@ -491,11 +528,14 @@ void launch_gs(cg_problem& P, const DList& a, const Ctx& c, int l, float* Z, con
 // the kernels' flags.
 // Z is the down-sweep buffer, ZO receives z (they are the same buffer for the
 // damped-Jacobi smoother, which ping-pongs with the scratch buffer instead).
-void enqueue_vcycle(cg_problem& P, float* Z, float* ZO, float* R, const VParams& v) {
-  const Ctx c = P.ctx(ZO, R);
+// V is the storage type of Z, ZO, the scratch buffer and the V-cycle rhs (R is
+// only a placeholder: in V-cycle mode the rhs comes from the V-cycle rhs array).
+template <class V>
+void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v) {
+  const Ctx c = P.ctx();
   const int base = F_VCYCLE | kSwizzle | (v.check_done ? F_CHECK_DONE : 0);
   const int L = P.max_level;  // a problem's levels above its coarse_C use the grid-wide lists actg / lvlg
-  float* ZB = P.d_zb;
+  V* ZB = pool_zb<V>(P);
   for (int l = L; l >= 1; l--) {
     const DList& a = P.actg[l];
     if (v.smoother == CG_SMOOTH_RBGS) {
@@ -504,27 +544,27 @@ void enqueue_vcycle(cg_problem& P, float* Z, float* ZO, float* R, const VParams&
         launch_gs(P, a, c, l, Z, R, 1, base | F_GHOST_ZERO);
       }
     } else {
-      const float* src = Z;
+      const V* src = Z;
       for (int s = 0; s < v.nu; s++) {
-        float* dst = ((v.nu - 1 - s) % 2 == 0) ? Z : ZB;
+        V* dst = ((v.nu - 1 - s) % 2 == 0) ? Z : ZB;
         int fl = base | F_GHOST_ZERO | (s == 0 ? F_ZERO_X : 0);
-        if (a.n) k_jacobi<<<grid_for(a.n, fl), 64, 0, P.stream>>>(c, P.L(a), a.n, l, src, Z, dst, R, v.omega, fl);
+        if (a.n) k_jacobi<V><<<grid_for(a.n, fl), 64, 0, P.stream>>>(c, P.L(a), a.n, l, src, Z, dst, R, v.omega, fl);
         src = dst;
       }
     }
     const DList& al = P.lvlg[l];
     int fl = base | F_GHOST_ZERO | (v.nu == 0 ? F_ZERO_X : 0);
-    if (al.n) k_restrict<<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, Z, R, nullptr, v.ooa, fl);
+    if (al.n) k_restrict<V><<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, Z, R, nullptr, v.ooa, fl);
   }
   if (!P.plans.empty()) {
     VCycleArgs va{v.nu, v.coarsest, v.smoother, v.omega, v.ooa, v.pcoef};
-    k_coarse_lds<<<unsigned(P.plans.size()), CL_THREADS, 0, P.stream>>>(c, P.d_plans, va, ZO, base & ~F_SWIZZLE);
+    k_coarse_lds<V><<<unsigned(P.plans.size()), CL_THREADS, 0, P.stream>>>(c, P.d_plans, va, ZO, base & ~F_SWIZZLE);
   }
   {
     // problems whose level 0 is too large for LDS: bottom sweeps as grid-wide launches
     const DList& a0 = P.actg[0];
     const int it0 = v.coarsest / 2;
-    if (it0 == 0 && a0.n) k_fill_tiles<<<a0.n, 64, 0, P.stream>>>(P.L(a0), a0.n, ZO, 0.f);
+    if (it0 == 0 && a0.n) k_fill_tiles<V><<<a0.n, 64, 0, P.stream>>>(P.L(a0), a0.n, ZO, 0.f);
     for (int i = 0; i < it0; i++) {
       launch_gs(P, a0, c, 0, ZO, R, 0, base | ((i == 0) ? F_ZERO_X : 0));
       launch_gs(P, a0, c, 0, ZO, R, 1, base);
@@ -540,7 +580,7 @@ void enqueue_vcycle(cg_problem& P, float* Z, float* ZO, float* R, const VParams&
     const bool jac = v.smoother != CG_SMOOTH_RBGS;
     if (!jac && v.nu > 0) {
       // prolongation fused into the first (black) pass; the level's result goes to ZO
-      if (al.n) k_gs_prolong<<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, Z, ZO, v.pcoef, 1, base);
+      if (al.n) k_gs_prolong<V><<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, Z, ZO, v.pcoef, 1, base);
       launch_gs(P, a, c, l, ZO, R, 0, base);
       for (int s = 1; s < v.nu; s++) {
         launch_gs(P, a, c, l, ZO, R, 1, base);
@@ -548,15 +588,15 @@ void enqueue_vcycle(cg_problem& P, float* Z, float* ZO, float* R, const VParams&
       }
       continue;
     }
-    float* XO = (jac && (v.nu % 2 == 1)) ? ZB : ZO;
+    V* XO = (jac && (v.nu % 2 == 1)) ? ZB : ZO;
     int fl = base | (v.nu == 0 ? F_ZERO_X : 0);
-    if (al.n) k_prolong<<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, ZO, Z, XO, ZO, v.pcoef, fl);
+    if (al.n) k_prolong<V><<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, ZO, Z, XO, ZO, v.pcoef, fl);
     if (!jac) continue;
     {
-      const float* src = XO;
+      const V* src = XO;
       for (int s = 0; s < v.nu; s++) {
-        float* dst = ((v.nu - 1 - s) % 2 == 0) ? Z : ZB;
-        if (a.n) k_jacobi<<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, l, src, Z, dst, R, v.omega, base);
+        V* dst = ((v.nu - 1 - s) % 2 == 0) ? Z : ZB;
+        if (a.n) k_jacobi<V><<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, l, src, Z, dst, R, v.omega, base);
         src = dst;
       }
     }
@@ -566,6 +606,7 @@ void enqueue_vcycle(cg_problem& P, float* Z, float* ZO, float* R, const VParams&
 // One CG iteration of the rotated loop (GMGSolver.cu:573-653 shifted so that it
 // starts with the V-cycle): [re-centre r], z = M^-1 r, gamma/beta, p, A p and
 // alpha, x/r and the convergence test.
+template <class V>
 void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann) {
   const Ctx c = P.ctx();
   const int fl = kSwizzle | F_CHECK_DONE;
@@ -573,18 +614,23 @@ void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann) {
   if (pure_neumann) {
     k_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, fl);
     k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_MEAN, fl);
-    k_subtract_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, 1, fl);
+    k_subtract_mean<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, 1, fl);
   }
-  // RB-GS: down-sweep in the scratch buffer, z assembled in c.z by the up-sweep
-  enqueue_vcycle(P, v.smoother == CG_SMOOTH_RBGS ? P.d_zb : c.z, c.z, c.r, v);
-  k_cg_dot_rz<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
+  // RB-GS: down-sweep in the scratch buffer, z assembled in z by the up-sweep
+  V* Z = pool_z<V>(P);
+  enqueue_vcycle<V>(P, v.smoother == CG_SMOOTH_RBGS ? pool_zb<V>(P) : Z, Z, nullptr, v);
+  k_cg_dot_rz<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
   k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_RZ, fl);
-  k_cg_update_p<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
+  k_cg_update_p<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
   k_apply<<<grid_for(P.applyl.n, fl), 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, c.p, c.ap, -1, 0, 1, fl);
   if (P.gather.n) k_ghost_gather<<<grid_for(P.gather.n, fl), 64, 0, P.stream>>>(c, P.L(P.gather), P.gather.n, fl);
   k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_PAP, fl);
-  k_cg_update_xr<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
+  k_cg_update_xr<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
   k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_RR, fl);
+}
+void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann, int storage) {
+  if (storage == CG_STORE_BF16) enqueue_cg_iteration<bf16>(P, v, pure_neumann);
+  else enqueue_cg_iteration<float>(P, v, pure_neumann);
 }
 
 // A temporary worklist of the records of the given tiles.
@@ -634,10 +680,17 @@ std::vector<ProbState> read_states(cg_problem& P) {
 
 // All problems of the pool, one graph replay per CG iteration for all of them;
 // a converged problem's tiles drop out of every kernel at their first load.
-void solve(cg_problem& P, const cg_params& prm, cg_result* res) {
+void check_params(const cg_params& prm) {
   if (prm.max_iters < 0 || !(prm.rel_tol >= 0) || prm.level_iters < 0 || prm.coarsest_iters < 0)
     fail(CG_ERR_INVALID, "cg_solve: invalid parameters");
   if (prm.smoother != CG_SMOOTH_RBGS && prm.smoother != CG_SMOOTH_JACOBI) fail(CG_ERR_INVALID, "cg_solve: unknown smoother");
+  if (prm.storage != CG_STORE_F32 && prm.storage != CG_STORE_BF16) fail(CG_ERR_INVALID, "cg_solve: unknown storage type");
+}
+
+// tol_q / maxit_q: per-problem tolerances and iteration caps (null: prm's for all).
+void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* tol_q = nullptr, const int* maxit_q = nullptr) {
+  check_params(prm);
+  if (prm.storage == CG_STORE_BF16) ensure_bf16(P);
   VParams v;
   v.nu = prm.level_iters;
   v.coarsest = prm.coarsest_iters;
@@ -647,22 +700,27 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res) {
   v.pcoef = prm.prolong_coeff;
   const int stride = std::max(1, prm.sync_stride);
   auto t0 = std::chrono::steady_clock::now();
-  set_prob_state(P, prm.rel_tol, prm.max_iters, prm.pure_neumann, prm.sync_stride);
+  set_prob_state(P, prm.rel_tol, prm.max_iters, prm.pure_neumann, prm.sync_stride, tol_q, maxit_q);
+  int max_iters = prm.max_iters;
+  if (maxit_q) for (int q = 0; q < P.nprob; q++) max_iters = std::max(max_iters, maxit_q[q]);
   const Ctx c = P.ctx();
-  k_cg_init<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, kSwizzle);
+  if (prm.storage == CG_STORE_BF16)
+    k_cg_init<bf16><<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, kSwizzle);
+  else
+    k_cg_init<float><<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, kSwizzle);
   k_finalize<<<unsigned(P.nprob), FIN_THREADS, 0, P.stream>>>(c, FIN_BB, 0);
   HIPC(hipGetLastError());
 
   char keybuf[160];
-  std::snprintf(keybuf, sizeof(keybuf), "%d/%d/%d/%a/%a/%a/%d", v.nu, v.coarsest, v.smoother, double(v.omega), double(v.ooa),
-                double(v.pcoef), prm.pure_neumann);
+  std::snprintf(keybuf, sizeof(keybuf), "%d/%d/%d/%a/%a/%a/%d/%d", v.nu, v.coarsest, v.smoother, double(v.omega), double(v.ooa),
+                double(v.pcoef), prm.pure_neumann, prm.storage);
   const bool use_graph = prm.use_graph != 0;
   if (use_graph && P.graph_key != keybuf) {
     if (P.graph) HIPC(hipGraphExecDestroy(P.graph));
     P.graph = nullptr;
     hipGraph_t g;
     HIPC(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
-    enqueue_cg_iteration(P, v, prm.pure_neumann);
+    enqueue_cg_iteration(P, v, prm.pure_neumann, prm.storage);
     HIPC(hipStreamEndCapture(P.stream, &g));
     HIPC(hipGraphInstantiate(&P.graph, g, nullptr, nullptr, 0));
     HIPC(hipGraphDestroy(g));
@@ -670,7 +728,7 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res) {
   }
   auto one_iter = [&]() {
     if (use_graph) HIPC(hipGraphLaunch(P.graph, P.stream));
-    else enqueue_cg_iteration(P, v, prm.pure_neumann);
+    else enqueue_cg_iteration(P, v, prm.pure_neumann, prm.storage);
   };
   // Poll the done flags one chunk behind the enqueue front, so the device never
   // waits for the host; iterations after convergence exit at their first load.
@@ -682,7 +740,7 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res) {
   int enq = 0, slot = 0;
   bool have_prev = false;
   while (true) {
-    if (prm.max_iters == 0) break;
+    if (max_iters == 0) break;
     for (int s = 0; s < stride; s++) { one_iter(); enq++; }
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(&P.h_pin[slot * npb], P.d_dflag, sizeof(int32_t) * npb, hipMemcpyDeviceToHost, P.stream));
@@ -691,7 +749,7 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res) {
       HIPC(hipEventSynchronize(P.ev[1 - slot]));
       if (all_done(1 - slot)) break;
     }
-    if (enq >= prm.max_iters) {
+    if (enq >= max_iters) {
       HIPC(hipEventSynchronize(P.ev[slot]));
       break;
     }
@@ -711,6 +769,114 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res) {
     // GMGSolver.cu:553-556, 655-665: (0, 0) for b = 0; relative residual -1
     // when convergence is never tested (sync_stride <= 0)
     r.rel_res = s.bb > 0 ? (prm.sync_stride > 0 ? std::sqrt(s.rr / s.bb) : -1.0) : 0.0;
+    r.seconds = std::chrono::duration<double>(t1 - t0).count();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision form: iterative refinement with an FP64 outer loop
+// ---------------------------------------------------------------------------
+void ensure_f64(cg_problem& P) {
+  if (P.d_x64) return;
+  if (!P.owner) fail(CG_ERR_INVALID, "iterative refinement needs an owning problem handle");
+  const size_t cells = size_t(P.n) * 512;
+  HIPC(hipMalloc(&P.d_x64, cells * sizeof(double)));
+  HIPC(hipMalloc(&P.d_r64, cells * sizeof(double)));
+  HIPC(hipMalloc(&P.d_bsave, cells * sizeof(float)));
+  HIPC(hipMalloc(&P.d_psum, size_t(P.nprob) * sizeof(double)));
+  HIPC(hipMemsetAsync(P.d_x64, 0, cells * sizeof(double), P.stream));
+  HIPC(hipMemsetAsync(P.d_r64, 0, cells * sizeof(double), P.stream));
+  HIPC(hipStreamSynchronize(P.stream));
+}
+
+// Per-problem totals of the leaf-slot partials in c.part, to the host.
+std::vector<double> prob_sums(cg_problem& P) {
+  const Ctx c = P.ctx();
+  k_prob_sums<<<unsigned(P.nprob), FIN_THREADS, 0, P.stream>>>(c, P.d_psum);
+  std::vector<double> out(P.nprob);
+  HIPC(hipMemcpyAsync(out.data(), P.d_psum, sizeof(double) * P.nprob, hipMemcpyDeviceToHost, P.stream));
+  HIPC(hipStreamSynchronize(P.stream));
+  return out;
+}
+
+// r64 = b - A x64 on LEAF tiles in FP64 (re-centred for pure Neumann), the
+// FP32 copy in the r channel, and |r64|^2 per problem.
+std::vector<double> refine_residual(cg_problem& P, int pure_neumann) {
+  const Ctx c = P.ctx();
+  const unsigned gl = grid_for(P.leaf.n, kSwizzle);
+  if (P.ghosts.n) k_propagate_ghosts64<<<P.ghosts.n, 64, 0, P.stream>>>(c, P.L(P.ghosts), P.ghosts.n, P.d_x64);
+  k_apply64<<<P.applyl.n, 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, P.d_x64, P.d_r64);
+  if (P.ghosts.n) k_accumulate_ghosts64<<<P.ghosts.n, 64, 0, P.stream>>>(c, P.L(P.ghosts), P.ghosts.n, P.d_r64);
+  if (pure_neumann) {
+    k_ir_residual<<<gl, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.d_bsave, P.d_r64, 1);
+    k_finalize<<<unsigned(P.nprob), FIN_THREADS, 0, P.stream>>>(c, FIN_MEAN, 0);
+    k_ir_recenter<<<gl, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.d_r64);
+  } else {
+    k_ir_residual<<<gl, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.d_bsave, P.d_r64, 0);
+  }
+  return prob_sums(P);
+}
+
+// Iterative refinement to |b - A x|_2 < rel_tol |b|_2 measured in FP64, every
+// problem of the pool with its own outer loop: the residual and x are FP64; each
+// correction solves A d = float(r) with the FP32 MGPCG (V-cycle storage as
+// prm.storage) to max(inner_rel_tol, 0.5 sqrt(thr / |r|^2)), so the last step
+// does not solve further than the outer tolerance needs.
+void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& rp, cg_refine_result* res) {
+  check_params(prm);
+  if (!(rp.rel_tol >= 0) || !(rp.inner_rel_tol >= 0) || rp.max_outer < 0)
+    fail(CG_ERR_INVALID, "cg_solve_refined: invalid refinement parameters");
+  ensure_f64(P);
+  if (prm.storage == CG_STORE_BF16) ensure_bf16(P);
+  auto t0 = std::chrono::steady_clock::now();
+  const int np = P.nprob;
+  const size_t cells = size_t(P.n) * 512;
+  const Ctx c = P.ctx();
+  set_prob_state(P, 0.0, 0, prm.pure_neumann, 1);  // leaf ranges for the per-problem sums
+  HIPC(hipMemcpyAsync(P.d_bsave, c.r, cells * sizeof(float), hipMemcpyDeviceToDevice, P.stream));
+  HIPC(hipMemsetAsync(P.d_x64, 0, cells * sizeof(double), P.stream));
+  k_ir_norm<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.d_bsave);
+  const std::vector<double> bb = prob_sums(P);
+  std::vector<double> thr(np), rr(bb);
+  std::vector<int> done(np, 0), conv(np, 0), outer(np, 0), iters(np, 0);
+  for (int q = 0; q < np; q++) {
+    thr[q] = std::max(rp.rel_tol * rp.rel_tol * bb[q], std::numeric_limits<double>::min());
+    if (bb[q] == 0.0) done[q] = conv[q] = 1;  // b = 0: x = 0
+  }
+  std::vector<double> tol_q(np);
+  std::vector<int> maxit_q(np);
+  std::vector<cg_result> inner(np);
+  while (true) {
+    const std::vector<double> r2 = refine_residual(P, prm.pure_neumann);
+    bool any = false;
+    for (int q = 0; q < np; q++) {
+      if (done[q]) continue;
+      rr[q] = r2[q];
+      if (rr[q] < thr[q]) done[q] = conv[q] = 1;
+      else if (outer[q] >= rp.max_outer) done[q] = 1;
+      any = any || !done[q];
+    }
+    if (!any) break;
+    for (int q = 0; q < np; q++) {
+      tol_q[q] = done[q] ? 0.0 : std::max(rp.inner_rel_tol, 0.5 * std::sqrt(thr[q] / rr[q]));
+      maxit_q[q] = done[q] ? 0 : prm.max_iters;
+    }
+    solve(P, prm, inner.data(), tol_q.data(), maxit_q.data());
+    for (int q = 0; q < np; q++)
+      if (!done[q]) { outer[q]++; iters[q] += inner[q].iters; }
+    k_ir_accumulate<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.d_x64, 0);
+  }
+  k_ir_accumulate<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.d_x64, 1);
+  HIPC(hipStreamSynchronize(P.stream));
+  auto t1 = std::chrono::steady_clock::now();
+  for (int q = 0; q < np; q++) {
+    cg_refine_result& r = res[q];
+    r.outer = outer[q];
+    r.iters = iters[q];
+    r.converged = conv[q];
+    r.bb = bb[q];
+    r.rr = bb[q] > 0 ? rr[q] : 0.0;
+    r.rel_res = bb[q] > 0 ? std::sqrt(rr[q] / bb[q]) : 0.0;
     r.seconds = std::chrono::duration<double>(t1 - t0).count();
   }
 }
@@ -750,6 +916,32 @@ void cg_params_default(cg_params* p) {
   p->one_over_alpha = 1.0f;
   p->prolong_coeff = 1.0f;
   p->use_graph = 1;
+  p->storage = CG_STORE_F32;
+}
+
+void cg_refine_params_default(cg_refine_params* p) {
+  if (!p) return;
+  p->rel_tol = 1e-8;
+  p->inner_rel_tol = 1e-4;
+  p->max_outer = 20;
+}
+
+cg_status cg_solve_refined(cg_problem* p, const cg_params* prm, const cg_refine_params* rp, cg_refine_result* results) {
+  return guard([&] {
+    if (!p || !prm || !rp || !results) fail(CG_ERR_INVALID, "cg_solve_refined: null argument");
+    solve_refined(*p, *prm, *rp, results);
+  });
+}
+
+cg_status cg_get_solution_f64(cg_problem* p, double* host) {
+  return guard([&] {
+    if (!p || !host) fail(CG_ERR_INVALID, "cg_get_solution_f64: null argument");
+    if (!p->d_x64) fail(CG_ERR_STATE, "cg_get_solution_f64: no refined solve has run on this handle");
+    std::vector<double> buf(size_t(p->n) * 512);
+    HIPC(hipMemcpyAsync(buf.data(), p->d_x64, buf.size() * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+    for (int i = 0; i < p->n; i++) std::memcpy(host + size_t(p->ext_of[i]) * 512, &buf[size_t(i) * 512], 512 * sizeof(double));
+  });
 }
 
 cg_status cg_problem_create_batch(const cg_tree_desc* descs, int32_t num_problems, cg_problem** out) {
@@ -841,7 +1033,7 @@ cg_status cg_damped_jacobi_solve(cg_problem* p, int32_t max_iters, double rel_to
     float* B = P.d_ch[CG_CH_B];
     float* AX = P.d_ch[CG_CH_P];
     float* D = P.d_ch[CG_CH_AP];
-    k_fill_tiles<<<P.n, 64, 0, P.stream>>>(P.L(P.all), P.n, X, 0.f);
+    k_fill_tiles<float><<<P.n, 64, 0, P.stream>>>(P.L(P.all), P.n, X, 0.f);
     double bn = std::sqrt(dot_types(P, B, B, LEAF | NONLEAF | GHOST));
     double thr = std::max(rel_tol * bn, 2.2250738585072014e-308);
     int i = 0;
@@ -942,7 +1134,7 @@ cg_status cg_jacobi_sweep(cg_problem* p, int32_t level, int32_t x_channel, int32
     const DList& a = p->act[level];
     if (!a.n) return;
     float* X = p->d_ch[x_channel];
-    k_jacobi<<<a.n, 64, 0, p->stream>>>(p->ctx(), p->L(a), a.n, level, X, X, p->d_zb, p->d_ch[rhs_channel], omega, 0);
+    k_jacobi<float><<<a.n, 64, 0, p->stream>>>(p->ctx(), p->L(a), a.n, level, X, X, p->d_zb, p->d_ch[rhs_channel], omega, 0);
     k_copy_tiles<<<a.n, 64, 0, p->stream>>>(p->L(a), a.n, p->d_zb, X);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
@@ -959,7 +1151,7 @@ cg_status cg_residual_restrict(cg_problem* p, int32_t level, int32_t x_channel, 
     if (level < 1 || level > p->max_level) fail(CG_ERR_INVALID, "level out of range");
     const DList& al = p->lvl[level];
     if (!al.n) return;
-    k_restrict<<<al.n, 64, 0, p->stream>>>(p->ctx(), p->L(al), al.n, level, p->d_ch[x_channel], p->d_ch[rhs_channel],
+    k_restrict<float><<<al.n, 64, 0, p->stream>>>(p->ctx(), p->L(al), al.n, level, p->d_ch[x_channel], p->d_ch[rhs_channel],
                                            p->d_ch[coarse_channel], one_over_alpha, 0);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
@@ -975,7 +1167,7 @@ cg_status cg_prolongate(cg_problem* p, int32_t level, int32_t coarse_channel, in
     const DList& al = p->lvl[level];
     if (!al.n) return;
     float* F = p->d_ch[fine_channel];
-    k_prolong<<<al.n, 64, 0, p->stream>>>(p->ctx(), p->L(al), al.n, level, p->d_ch[coarse_channel], F, F, F, prolong_coeff, 0);
+    k_prolong<float><<<al.n, 64, 0, p->stream>>>(p->ctx(), p->L(al), al.n, level, p->d_ch[coarse_channel], F, F, F, prolong_coeff, 0);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
   });
@@ -998,7 +1190,7 @@ cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel) {
     const Ctx c = p->ctx();
     k_mean<<<p->leaf.n, 64, 0, p->stream>>>(c, p->L(p->leaf), p->leaf.n, p->d_ch[channel], 0);
     k_finalize<<<1, FIN_THREADS, 0, p->stream>>>(c, FIN_MEAN, 0);
-    k_subtract_mean<<<p->leaf.n, 64, 0, p->stream>>>(c, p->L(p->leaf), p->leaf.n, p->d_ch[channel], 0, 0);
+    k_subtract_mean<float><<<p->leaf.n, 64, 0, p->stream>>>(c, p->L(p->leaf), p->leaf.n, p->d_ch[channel], 0, 0);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
   });
@@ -1027,7 +1219,7 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
         case CG_KERNEL_JACOBI: {
           const DList& a = P.act[level];
           nt = a.n;
-          if (a.n) k_jacobi<<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, level, c.z, c.z, c.zb, c.r, 0.5f, base);
+          if (a.n) k_jacobi<float><<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, level, c.z, c.z, c.zb, c.r, 0.5f, base);
           break;
         }
         case CG_KERNEL_APPLY:
@@ -1038,30 +1230,30 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
           const DList& al = P.lvl[level];
           nt = al.n;
           int fl = base | F_GHOST_ZERO;
-          k_restrict<<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, level, c.z, c.r, nullptr, 1.f, fl);
+          k_restrict<float><<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, level, c.z, c.r, nullptr, 1.f, fl);
           break;
         }
         case CG_KERNEL_PROLONG: {
           const DList& al = P.lvl[level];
           nt = al.n;
-          k_prolong<<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, level, c.z, c.z, c.zb, c.zb, 1.f, base);
+          k_prolong<float><<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, level, c.z, c.z, c.zb, c.zb, 1.f, base);
           break;
         }
         case CG_KERNEL_GS_PROLONG: {
           const DList& al = P.lvl[level];
           nt = al.n;
-          k_gs_prolong<<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, c.zb, c.z, 1.f, 1, base);
+          k_gs_prolong<float><<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, c.zb, c.z, 1.f, 1, base);
           break;
         }
         case CG_KERNEL_UPDATE_XR:
           nt = P.leaf.n;
-          k_cg_update_xr<<<grid_for(P.leaf.n, base), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, base);
+          k_cg_update_xr<float><<<grid_for(P.leaf.n, base), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, base);
           break;
         case CG_KERNEL_COARSE: {
           nt = int64_t(P.coarse_slots.size());
           VCycleArgs va{2, level, 0, 2.f / 3.f, 1.f, 1.f};  // `level` = bottom iterations
           if (!P.plans.empty())
-            k_coarse_lds<<<unsigned(P.plans.size()), CL_THREADS, 0, P.stream>>>(c, P.d_plans, va, c.z, base & ~F_SWIZZLE);
+            k_coarse_lds<float><<<unsigned(P.plans.size()), CL_THREADS, 0, P.stream>>>(c, P.d_plans, va, c.z, base & ~F_SWIZZLE);
           break;
         }
         case CG_KERNEL_COPY:

@@ -100,6 +100,9 @@ struct cg_server {
   uint8_t* d_fmask = nullptr;
   float* d_ch[CG_NUM_CHANNELS] = {};
   float* d_zb = nullptr;
+  bf16* d_zh = nullptr;  // bfloat16 V-cycle vectors (storage CG_STORE_BF16)
+  bf16* d_rhsvh = nullptr;
+  bf16* d_zbh = nullptr;
   float* d_gsum = nullptr;
   ProbState* d_ps = nullptr;
   int32_t* d_dflag = nullptr;
@@ -134,7 +137,7 @@ cg_server::~cg_server() {
     if (I->h_buf) (void)hipHostFree(I->h_buf);
     if (I->ev) (void)hipEventDestroy(I->ev);
   }
-  void* ptrs[] = {d_ctype, d_fmask, d_zb, d_gsum, d_ps, d_dflag, d_dflag_init, d_pmap, d_newmap, d_part, d_part2, d_asm,
+  void* ptrs[] = {d_ctype, d_fmask, d_zb, d_zh, d_rhsvh, d_zbh, d_gsum, d_ps, d_dflag, d_dflag_init, d_pmap, d_newmap, d_part, d_part2, d_asm,
                   d_plans, (void*)d_srcs, d_offs, d_counts};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
@@ -149,7 +152,7 @@ namespace {
 
 // Device bytes one tile costs: 6 channels + the V-cycle scratch (FP32), cell
 // types and face masks, ghost row sums, dot partials and room for its records
-// in the assembled worklists.
+// in the assembled worklists; BF16 storage adds three bfloat16 vectors.
 constexpr int kRecKindsPerTile = 5;  // leaf, gather, applyl, actg, lvlg: at most 5 records per tile
 constexpr int64_t kBytesPerTile =
     (CG_NUM_CHANNELS + 1) * 512 * 4 + 2 * 512 + 64 * 4 + 2 * 8 + kRecKindsPerTile * int64_t(sizeof(TileRec));
@@ -157,9 +160,7 @@ constexpr int64_t kBytesPerTile =
 void server_create(const cg_server_config* cfg, cg_server& S) {
   if (!cfg || cfg->budget_bytes <= 0 || cfg->max_problems <= 0) fail(CG_ERR_INVALID, "cg_server_create: bad configuration");
   const cg_params& prm = cfg->params;
-  if (prm.max_iters < 0 || !(prm.rel_tol >= 0) || prm.level_iters < 0 || prm.coarsest_iters < 0)
-    fail(CG_ERR_INVALID, "cg_server_create: invalid solver parameters");
-  if (prm.smoother != CG_SMOOTH_RBGS && prm.smoother != CG_SMOOTH_JACOBI) fail(CG_ERR_INVALID, "cg_server_create: unknown smoother");
+  check_params(prm);
   S.prm = prm;
   S.v.nu = prm.level_iters;
   S.v.coarsest = prm.coarsest_iters;
@@ -168,7 +169,8 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   S.v.ooa = prm.one_over_alpha;
   S.v.pcoef = prm.prolong_coeff;
   S.max_probs = cfg->max_problems;
-  S.cap_tiles = std::max<int64_t>(64, cfg->budget_bytes / kBytesPerTile);
+  const int64_t per_tile = kBytesPerTile + (prm.storage == CG_STORE_BF16 ? 3 * 512 * int64_t(sizeof(bf16)) : 0);
+  S.cap_tiles = std::max<int64_t>(64, cfg->budget_bytes / per_tile);
   if (S.cap_tiles * 512 > int64_t(INT32_MAX)) S.cap_tiles = int64_t(INT32_MAX) / 512;  // cell offsets stay 32-bit safe
   S.cap_leaf = S.cap_tiles;
   S.cap_ghost = S.cap_tiles;
@@ -184,6 +186,13 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   HIPC(hipMalloc(&S.d_fmask, cells));
   for (int c = 0; c < CG_NUM_CHANNELS; c++) HIPC(hipMalloc(&S.d_ch[c], cells * sizeof(float)));
   HIPC(hipMalloc(&S.d_zb, cells * sizeof(float)));
+  if (S.prm.storage == CG_STORE_BF16) {
+    bf16** bufs[3] = {&S.d_zh, &S.d_rhsvh, &S.d_zbh};
+    for (bf16** b : bufs) {
+      HIPC(hipMalloc(b, cells * sizeof(bf16)));
+      HIPC(hipMemset(*b, 0, cells * sizeof(bf16)));
+    }
+  }
   HIPC(hipMalloc(&S.d_gsum, size_t(S.cap_ghost) * 64 * sizeof(float)));
   HIPC(hipMalloc(&S.d_ps, sizeof(ProbState) * S.max_probs));
   HIPC(hipMemset(S.d_ps, 0, sizeof(ProbState) * S.max_probs));
@@ -217,6 +226,9 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   P.d_fmask = S.d_fmask;
   for (int c = 0; c < CG_NUM_CHANNELS; c++) P.d_ch[c] = S.d_ch[c];
   P.d_zb = S.d_zb;
+  P.d_zh = S.d_zh;
+  P.d_rhsvh = S.d_rhsvh;
+  P.d_zbh = S.d_zbh;
   P.d_gsum = S.d_gsum;
   P.d_ps = S.d_ps;
   P.d_dflag = S.d_dflag;
@@ -379,11 +391,12 @@ void server_start_queued(cg_server& S, int k) {
       HIPC(hipMemcpyAsync(S.d_ch[CG_CH_B] + size_t(I.tile_begin) * 512, I.h_buf, size_t(I.ntiles) * 512 * sizeof(float),
                           hipMemcpyHostToDevice, st));
     float* X = S.d_ch[CG_CH_X];
-    if (!I.warm) k_fill_tiles<<<I.all.n, 64, 0, st>>>(I.L(I.all), I.all.n, X, 0.f);
+    if (!I.warm) k_fill_tiles<float><<<I.all.n, 64, 0, st>>>(I.L(I.all), I.all.n, X, 0.f);
     if (I.ghosts.n) k_propagate_ghosts<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, X, X);
     k_apply<<<I.applyl.n, 64, 0, st>>>(c, I.L(I.applyl), I.applyl.n, X, c.ap, -1, 0, 0, 0);
     if (I.ghosts.n) k_accumulate_ghosts<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, c.ap);
-    k_cg_init_warm<<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_ch[CG_CH_B], 0);
+    if (S.prm.storage == CG_STORE_BF16) k_cg_init_warm<bf16><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_ch[CG_CH_B], 0);
+    else k_cg_init_warm<float><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_ch[CG_CH_B], 0);
     hmap[f] = I.slot;
     I.state = INST_ACTIVE;
     S.active.push_back(I.slot);
@@ -496,7 +509,7 @@ void server_step(cg_server& S, int32_t* n_active) {
   const int k = int(S.step_no & 1);
   cg_problem& P = S.view;
   if (!S.active.empty() && !S.dirty) {
-    enqueue_cg_iteration(P, S.v, S.prm.pure_neumann);
+    enqueue_cg_iteration(P, S.v, S.prm.pure_neumann, S.prm.storage);
     HIPC(hipGetLastError());
     int maxslot = 0;
     for (int q : S.active) maxslot = std::max(maxslot, q + 1);

@@ -32,6 +32,10 @@ enum { CG_LEAF = 1, CG_NONLEAF = 2, CG_GHOST = 4 };
 enum { CG_DIRICHLET = 1, CG_INTERIOR = 2, CG_NEUMANN = 4 };
 enum { CG_CH_X = 0, CG_CH_B = 1, CG_CH_R = 1, CG_CH_P = 2, CG_CH_AP = 3, CG_CH_Z = 4, CG_CH_TMP = 5, CG_NUM_CHANNELS = 6 };
 enum { CG_SMOOTH_RBGS = 0, CG_SMOOTH_JACOBI = 1 };
+/* Storage of the V-cycle vectors (its x on every level, its rhs and scratch);
+ * arithmetic is FP32 either way.  BF16 rounds each stored value to nearest
+ * even and halves their HBM traffic; no reference counterpart. */
+enum { CG_STORE_F32 = 0, CG_STORE_BF16 = 1 };
 
 typedef enum {
   CG_OK = 0,
@@ -75,6 +79,7 @@ typedef struct {
   float one_over_alpha;     /* restriction scale (1) */
   float prolong_coeff;      /* prolongation scale (1) */
   int32_t use_graph;        /* capture one CG iteration in a hipGraph and replay it (1) */
+  int32_t storage;          /* CG_STORE_F32 (0, the reference) or CG_STORE_BF16 */
 } cg_params;
 
 typedef struct {
@@ -123,6 +128,36 @@ cg_status cg_get_channel(cg_problem* p, int32_t channel, float* host);
 cg_status cg_solve(cg_problem* p, const cg_params* prm, cg_result* res);
 /* All problems of a batched handle; results[num_problems]. */
 cg_status cg_solve_batch(cg_problem* p, const cg_params* prm, cg_result* results);
+
+/* ---- Mixed-precision form (no reference counterpart) ----------------------
+ * Iterative refinement: x and the residual r = b - A x are FP64 (the operator
+ * of cg_full_negative_laplacian evaluated in FP64); each refinement step solves
+ * A d = float(r) with the MGPCG of cg_solve (FP32 arithmetic, FP64 dot
+ * products, V-cycle storage as prm->storage) and adds d to x.  The stop test
+ * is |r|^2 < rel_tol^2 |b|^2 in FP64 over LEAF tiles, with the true residual.
+ * A step's inner tolerance is max(inner_rel_tol, 0.5 * sqrt(rel_tol^2 |b|^2 / |r|^2))
+ * and prm->max_iters caps each inner solve (prm->rel_tol is not used).  Every
+ * problem of a batched handle runs its own outer loop; results[num_problems].
+ * On return CG_CH_X holds float(x) and cg_get_solution_f64 gives x. */
+typedef struct {
+  double rel_tol;           /* 1e-8 */
+  double inner_rel_tol;     /* 1e-4 */
+  int32_t max_outer;        /* refinement steps at most (20) */
+} cg_refine_params;
+typedef struct {
+  int32_t outer;            /* refinement steps (inner solves) run */
+  int32_t iters;            /* CG iterations over all inner solves */
+  int32_t converged;        /* 1 if |b - A x| < rel_tol |b| (FP64) was reached */
+  double rel_res;           /* sqrt(|b - A x|^2 / |b|^2), FP64 */
+  double bb;                /* |b|^2 over LEAF tiles */
+  double rr;                /* final |b - A x|^2 */
+  double seconds;           /* wall time, host clock, device synchronised */
+} cg_refine_result;
+void cg_refine_params_default(cg_refine_params* p);
+cg_status cg_solve_refined(cg_problem* p, const cg_params* prm, const cg_refine_params* rp, cg_refine_result* results);
+/* The FP64 x of the last cg_solve_refined (caller's tile order, 512 per tile;
+ * LEAF tiles carry the solution). */
+cg_status cg_get_solution_f64(cg_problem* p, double* host);
 
 /* GMGSolver::dampedJacobiSolve (src/GMGSolver.cu:449-529). */
 cg_status cg_damped_jacobi_solve(cg_problem* p, int32_t max_iters, double rel_tol, float omega, cg_result* res);

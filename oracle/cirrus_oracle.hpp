@@ -49,6 +49,12 @@
 //   Dot                            value vs an exactly rounded sum
 //   ReCenterLeafVoxels             interior mean removed, other cells untouched
 //   dampedJacobiSolve              closed-form first iterate, residual decreases
+//   bf16_round                     every bfloat16 value and tie case, against
+//                                  an exact rational rounding in Python
+//   SolveRefined (no reference counterpart: the mixed-precision form)
+//                                  its FP64 residual against the independent
+//                                  assembly; the refined x against a direct
+//                                  FP64 solve
 // Parity unpinned: the exact FP32 bit patterns of the reference's own GPU run
 // (they depend on its compiler's contraction choices and on cub's reduction
 // order), hence its exact iteration counts; the hash-table slot order
@@ -66,9 +72,24 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace orc {
+
+// The value a bfloat16 store of v leaves (round to nearest, ties to even; the
+// top 16 bits of the FP32 pattern), as the GPU's conversion instruction gives
+// for finite inputs.  The mixed-precision form stores the V-cycle vectors so.
+inline float bf16_round(float v) {
+  uint32_t u;
+  std::memcpy(&u, &v, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return v;  // inf / nan (never produced by the solver)
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
 
 // HATileInfo.h:19 / PoissonTile.h:6
 enum : uint8_t { LEAF = 1, NONLEAF = 2, GHOST = 4 };
@@ -849,7 +870,14 @@ template <class T>
 struct Solver {
   Problem<T>& P;
   T one_over_alpha = T(1), prolong_coeff = T(1);  // GMGSolver.h:19-22
+  // Mixed-precision form (no reference counterpart): every value the V-cycle
+  // stores in its x and rhs channels is rounded to bfloat16 (T=float only).
+  bool vround = false;
   explicit Solver(Problem<T>& p) : P(p) {}
+  T st(T v) const {
+    if constexpr (std::is_same_v<T, float>) return vround ? bf16_round(v) : v;
+    else return v;
+  }
   Grid<T>& G() { return P.g(); }
 
   // NegativeLaplacianSameLevel128 (GMGSolver.cu:145-200)
@@ -949,7 +977,7 @@ struct Solver {
         T Ax = s.negativeLap(h, l[0], l[1], l[2], false);
         T D = s.negativeLap(h, l[0], l[1], l[2], true);
         T delta = (ct & INTERIOR) ? ((D != T(0)) ? (b - Ax) / D : T(0)) : T(0);
-        out[ti][vi] = xo + delta;
+        out[ti][vi] = st(xo + delta);
       }
     }
 #pragma omp parallel for schedule(static)
@@ -988,7 +1016,7 @@ struct Solver {
         T Ax = s.negativeLap(h, l[0], l[1], l[2], false);
         T D = s.negativeLap(h, l[0], l[1], l[2], true);
         T r = t.ch[rhsch][vi] - Ax;
-        out[ti][vi] = (D == T(0)) ? T(0) : xo + (omega * r) / D;
+        out[ti][vi] = (D == T(0)) ? T(0) : st(xo + (omega * r) / D);
       }
     }
 #pragma omp parallel for schedule(static)
@@ -1021,8 +1049,8 @@ struct Solver {
         for (int ii : {0, 1}) for (int jj : {0, 1}) for (int kk : {0, 1})
           sum += res[localOffset(fl[0] + ii, fl[1] + jj, fl[2] + kk)];
         T& dst = g.tile(ci).ch[cch][localOffset(cl)];
-        if (finfo.subtreeType(level) & GHOST) dst += sum;
-        else dst = sum * one_over_alpha;
+        if (finfo.subtreeType(level) & GHOST) dst = st(dst + sum);
+        else dst = st(sum * one_over_alpha);
       }
     }
   }
@@ -1044,7 +1072,7 @@ struct Solver {
         T fx = ft.ch[fxch][vi];
         bool in = ft.ctype[vi] & INTERIOR;
         if (ftt & GHOST) ft.ch[fxch][vi] = in ? cx : T(0);
-        else ft.ch[fxch][vi] = in ? std::fma(prolong_coeff, cx, fx) : T(0);
+        else ft.ch[fxch][vi] = in ? st(std::fma(prolong_coeff, cx, fx)) : T(0);
       }
     }
   }
@@ -1062,7 +1090,7 @@ struct Solver {
       auto& t = g.tile(info);
       for (int vi = 0; vi < 512; vi++) {
         t.ch[xch][vi] = T(0);
-        t.ch[rhsch][vi] = (info.type & GHOST) ? T(0) : t.ch[fch][vi];
+        t.ch[rhsch][vi] = (info.type & GHOST) ? T(0) : st(t.ch[fch][vi]);
       }
     }
     for (int i = g.maxLevel; i > 0; i--) {
@@ -1190,5 +1218,98 @@ struct Solver {
     return {i, rn / bnorm, bnorm * bnorm, rn * rn};
   }
 };
+
+// ---------------------------------------------------------------------------
+// Mixed-precision form (no reference counterpart)
+// ---------------------------------------------------------------------------
+// A copy of a problem with another arithmetic type (same tiles, hash tables,
+// canonical order and cell types; channel values converted).
+template <class B, class A>
+std::unique_ptr<Problem<B>> ConvertProblem(Problem<A>& a) {
+  auto P = std::make_unique<Problem<B>>();
+  P->params = a.params;
+  P->tubes = a.tubes;
+  Grid<A>& ga = a.g();
+  P->grid = std::make_unique<Grid<B>>(B(ga.h0), ga.log2);
+  Grid<B>& gb = P->g();
+  gb.maxLevel = ga.maxLevel;
+  gb.hashT = ga.hashT;
+  gb.arrays = ga.arrays;
+  gb.all = ga.all;
+  gb.count = ga.count;
+  gb.store.clear();
+  for (const auto& t : ga.store) {
+    auto d = std::make_unique<TileData<B>>();
+    for (int ch = 0; ch < NCH; ch++)
+      for (int c = 0; c < 512; c++) d->ch[ch][c] = B(t->ch[ch][c]);
+    std::memcpy(d->ctype, t->ctype, 512);
+    for (int f = 0; f < 6; f++) d->nbr[f] = t->nbr[f];
+    d->status = t->status;
+    gb.store.push_back(std::move(d));
+  }
+  P->canon = a.canon;
+  P->canonOfStore = a.canonOfStore;
+  P->leafCanon = a.leafCanon;
+  return P;
+}
+
+// Iterative refinement: x and r = b - A x in FP64 (Pd, an FP64 copy of Pf whose
+// b channel holds the FP32 b); each step solves A d = float(r) with the FP32
+// MGPCG on Pf (V-cycle stores rounded to bfloat16 when bf16 is set) to the
+// relative tolerance max(inner_tol, 0.5 sqrt(thr / |r|^2)), thr = max(tol^2 |b|^2,
+// DBL_MIN), and adds d to x.  Stops when |r|^2 < thr (FP64, LEAF tiles) or after
+// max_outer steps.  On return Pd's x channel holds x and Pf's holds float(x).
+struct RefineResult { int outer, iters, converged; double rel_res, bb, rr; };
+inline RefineResult SolveRefined(Problem<float>& Pf, Problem<double>& Pd, double tol, double inner_tol, int max_outer,
+                                 int max_iters, int level_iters, int coarsest_iters, bool pure_neumann, int smoother,
+                                 float omega, int sync_stride, bool bf16) {
+  Solver<double> Sd(Pd);
+  Solver<float> Sf(Pf);
+  Sf.vround = bf16;
+  Grid<double>& gd = Pd.g();
+  Grid<float>& gf = Pf.g();
+  for (const Info& in : gd.all) { auto& t = gd.tile(in); for (int c = 0; c < 512; c++) t.ch[CH_X][c] = 0.0; }
+  const double bb = Sd.Dot(CH_B, CH_B, LEAF);
+  RefineResult res{0, 0, 0, 0.0, bb, 0.0};
+  const auto& tl = Pd.leafCanon;
+  if (bb == 0.0) {
+    res.converged = 1;
+    for (const Info& in : tl) { auto& t = gf.tile(in); for (int c = 0; c < 512; c++) t.ch[CH_X][c] = 0.f; }
+    return res;
+  }
+  const double thr = std::max(tol * tol * bb, std::numeric_limits<double>::min());
+  while (true) {
+    Sd.FullNegativeLaplacian(CH_X, CH_AP);
+    for (const Info& in : tl) {
+      auto& t = gd.tile(in);
+      for (int c = 0; c < 512; c++) t.ch[CH_TMP][c] = t.ch[CH_B][c] - t.ch[CH_AP][c];
+    }
+    if (pure_neumann) Sd.ReCenterLeafVoxels(CH_TMP);
+    res.rr = Sd.Dot(CH_TMP, CH_TMP, LEAF);
+    if (res.rr < thr) { res.converged = 1; break; }
+    if (res.outer >= max_outer) break;
+    for (const Info& in : tl) {
+      auto& td = gd.tile(in);
+      auto& tf = gf.tile(in);
+      for (int c = 0; c < 512; c++) tf.ch[CH_R][c] = float(td.ch[CH_TMP][c]);
+    }
+    const double ti = std::max(inner_tol, 0.5 * std::sqrt(thr / res.rr));
+    auto r = Sf.solve(max_iters, ti, level_iters, coarsest_iters, pure_neumann, smoother, omega, sync_stride);
+    res.outer++;
+    res.iters += r.iters;
+    for (const Info& in : tl) {
+      auto& td = gd.tile(in);
+      auto& tf = gf.tile(in);
+      for (int c = 0; c < 512; c++) td.ch[CH_X][c] = td.ch[CH_X][c] + double(tf.ch[CH_X][c]);
+    }
+  }
+  for (const Info& in : tl) {
+    auto& td = gd.tile(in);
+    auto& tf = gf.tile(in);
+    for (int c = 0; c < 512; c++) tf.ch[CH_X][c] = float(td.ch[CH_X][c]);
+  }
+  res.rel_res = std::sqrt(res.rr / bb);
+  return res;
+}
 
 }  // namespace orc

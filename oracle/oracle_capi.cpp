@@ -15,6 +15,7 @@ struct OrcHandle {
   int fp64 = 0;
   std::unique_ptr<Problem<float>> pf;
   std::unique_ptr<Problem<double>> pd;
+  std::unique_ptr<Problem<double>> mirror;  // FP64 copy of pf for the refinement's outer loop
 };
 
 // Plain-C mirror of orc::SceneParams (same field order and types).
@@ -181,10 +182,14 @@ void orc_residual_restrict(void* h, int level, int xch, int rhsch, int cch) {
 void orc_prolongate(void* h, int level, int cxch, int fxch) {
   with(h, [&](auto& P) { Solver<std::remove_reference_t<decltype(P.g().h0)>> S(P); S.ProlongateAndUpdate(level, cxch, fxch); return 0; });
 }
-void orc_vcycle(void* h, int xch, int fch, int rhsch, int level_iters, int coarsest_iters, int smoother, double omega) {
+void orc_vcycle(void* h, int xch, int fch, int rhsch, int level_iters, int coarsest_iters, int smoother, double omega,
+                int storage) {
   with(h, [&](auto& P) {
     using T = std::remove_reference_t<decltype(P.g().h0)>;
-    Solver<T> S(P); S.VCycle(xch, fch, rhsch, level_iters, coarsest_iters, smoother, T(omega)); return 0;
+    Solver<T> S(P);
+    S.vround = storage == 1;
+    S.VCycle(xch, fch, rhsch, level_iters, coarsest_iters, smoother, T(omega));
+    return 0;
   });
 }
 double orc_dot(void* h, int ach, int bch, int types) {
@@ -195,11 +200,13 @@ void orc_recenter(void* h, int ch) {
 }
 // GMGSolver::solve.  out[0]=relative residual (recursive r), out[1]=|b|^2,
 // out[2]=|r|^2, out[3]=wall seconds of the solve.  Returns the iteration count.
+// storage 1: V-cycle stores rounded to bfloat16 (FP32 handles only).
 int orc_solve(void* h, int max_iters, double tol, int level_iters, int coarsest_iters, int pure_neumann,
-              int smoother, double omega, int sync_stride, double* out) {
+              int smoother, double omega, int sync_stride, int storage, double* out) {
   return with(h, [&](auto& P) {
     using T = std::remove_reference_t<decltype(P.g().h0)>;
     Solver<T> S(P);
+    S.vround = storage == 1;
     auto t0 = std::chrono::steady_clock::now();
     auto r = S.solve(max_iters, tol, level_iters, coarsest_iters, pure_neumann != 0, smoother, T(omega), sync_stride);
     auto t1 = std::chrono::steady_clock::now();
@@ -207,6 +214,39 @@ int orc_solve(void* h, int max_iters, double tol, int level_iters, int coarsest_
     return r.iters;
   });
 }
+// Mixed-precision iterative refinement on an FP32 handle (SolveRefined).
+// out[0]=relative FP64 residual, [1]=|b|^2, [2]=|r|^2, [3]=CG iterations,
+// [4]=converged, [5]=wall seconds.  Returns the number of refinement steps;
+// -1 on an FP64 handle.
+int orc_solve_refined(void* h, double tol, double inner_tol, int max_outer, int max_iters, int level_iters,
+                      int coarsest_iters, int pure_neumann, int smoother, double omega, int sync_stride, int storage,
+                      double* out) {
+  OrcHandle* o = static_cast<OrcHandle*>(h);
+  if (o->fp64) return -1;
+  auto t0 = std::chrono::steady_clock::now();
+  o->mirror = ConvertProblem<double>(*o->pf);
+  auto r = SolveRefined(*o->pf, *o->mirror, tol, inner_tol, max_outer, max_iters, level_iters, coarsest_iters,
+                        pure_neumann != 0, smoother, float(omega), sync_stride, storage == 1);
+  auto t1 = std::chrono::steady_clock::now();
+  if (out) {
+    out[0] = r.rel_res; out[1] = r.bb; out[2] = r.rr; out[3] = r.iters; out[4] = r.converged;
+    out[5] = std::chrono::duration<double>(t1 - t0).count();
+  }
+  return r.outer;
+}
+// The FP64 x of the last orc_solve_refined (canonical order); 0 if none ran.
+int orc_get_refined_x(void* h, double* out) {
+  OrcHandle* o = static_cast<OrcHandle*>(h);
+  if (!o->mirror) return 0;
+  auto& P = *o->mirror;
+  for (size_t i = 0; i < P.canon.size(); i++) std::memcpy(out + 512 * i, P.g().tile(P.canon[i]).ch[CH_X], 512 * sizeof(double));
+  return 1;
+}
+// Round FP32 values to bfloat16 as the V-cycle stores do (for pinning).
+void orc_bf16_round(const float* in, float* out, long n) {
+  for (long i = 0; i < n; i++) out[i] = bf16_round(in[i]);
+}
+
 int orc_damped_jacobi_solve(void* h, int max_iters, double tol, double omega, double* out) {
   return with(h, [&](auto& P) {
     using T = std::remove_reference_t<decltype(P.g().h0)>;

@@ -17,6 +17,15 @@ DIRICHLET, INTERIOR, NEUMANN = 1, 2, 4
 CH_X, CH_B, CH_R, CH_P, CH_AP, CH_Z, CH_TMP, CH_U, CH_V, CH_W = 0, 1, 1, 2, 3, 4, 5, 6, 7, 8
 SCENE_UNIFORM, SCENE_SPHERE_BAND, SCENE_PLANE, SCENE_VORTEX = 0, 1, 2, 3
 SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
+STORE_F32, STORE_BF16 = 0, 1
+
+
+def bf16_round(a):
+    """FP32 values rounded to bfloat16 as the V-cycle stores do (float32 array)."""
+    a = np.ascontiguousarray(a, np.float32)
+    out = np.empty_like(a)
+    lib().orc_bf16_round(_ptr(a), _ptr(out), a.size)
+    return out
 
 
 class SceneParams(ctypes.Structure):
@@ -105,12 +114,17 @@ def lib():
         L.orc_jacobi_sweep.argtypes = [vp, i, i, i, d]
         L.orc_residual_restrict.argtypes = [vp, i, i, i, i]
         L.orc_prolongate.argtypes = [vp, i, i, i]
-        L.orc_vcycle.argtypes = [vp, i, i, i, i, i, i, d]
+        L.orc_vcycle.argtypes = [vp, i, i, i, i, i, i, d, i]
         L.orc_dot.restype = d
         L.orc_dot.argtypes = [vp, i, i, i]
         L.orc_recenter.argtypes = [vp, i]
         L.orc_solve.restype = i
-        L.orc_solve.argtypes = [vp, i, d, i, i, i, i, d, i, vp]
+        L.orc_solve.argtypes = [vp, i, d, i, i, i, i, d, i, i, vp]
+        L.orc_solve_refined.restype = i
+        L.orc_solve_refined.argtypes = [vp, d, d, i, i, i, i, i, i, d, i, i, vp]
+        L.orc_get_refined_x.restype = i
+        L.orc_get_refined_x.argtypes = [vp, vp]
+        L.orc_bf16_round.argtypes = [vp, vp, ctypes.c_long]
         L.orc_damped_jacobi_solve.restype = i
         L.orc_damped_jacobi_solve.argtypes = [vp, i, d, d, vp]
         L.orc_set_threads.argtypes = [i]
@@ -196,8 +210,9 @@ class Oracle:
     def prolongate(self, level, cxch, fxch):
         lib().orc_prolongate(self.h, level, cxch, fxch)
 
-    def vcycle(self, xch, fch, rhsch, level_iters=1, coarsest_iters=10, smoother=SMOOTH_RBGS, omega=2.0 / 3.0):
-        lib().orc_vcycle(self.h, xch, fch, rhsch, level_iters, coarsest_iters, smoother, omega)
+    def vcycle(self, xch, fch, rhsch, level_iters=1, coarsest_iters=10, smoother=SMOOTH_RBGS, omega=2.0 / 3.0,
+               storage=STORE_F32):
+        lib().orc_vcycle(self.h, xch, fch, rhsch, level_iters, coarsest_iters, smoother, omega, int(storage))
 
     def dot(self, a, b, types=LEAF):
         return lib().orc_dot(self.h, a, b, types)
@@ -206,11 +221,26 @@ class Oracle:
         lib().orc_recenter(self.h, ch)
 
     def solve(self, max_iters=1000, tol=1e-6, level_iters=1, coarsest_iters=10, pure_neumann=False,
-              smoother=SMOOTH_RBGS, omega=2.0 / 3.0, sync_stride=1):
+              smoother=SMOOTH_RBGS, omega=2.0 / 3.0, sync_stride=1, storage=STORE_F32):
         out = np.zeros(4, np.float64)
         it = lib().orc_solve(self.h, max_iters, tol, level_iters, coarsest_iters, int(pure_neumann),
-                             smoother, omega, int(sync_stride), _ptr(out))
+                             smoother, omega, int(sync_stride), int(storage), _ptr(out))
         return {"iters": it, "rel_res": out[0], "bb": out[1], "rr": out[2], "seconds": out[3]}
+
+    def solve_refined(self, tol=1e-8, inner_tol=1e-4, max_outer=20, max_iters=1000, level_iters=1, coarsest_iters=10,
+                      pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, sync_stride=1, storage=STORE_BF16):
+        """Mixed-precision iterative refinement (FP32 handles); x64 via refined_x()."""
+        assert not self.fp64
+        out = np.zeros(6, np.float64)
+        k = lib().orc_solve_refined(self.h, tol, inner_tol, max_outer, max_iters, level_iters, coarsest_iters,
+                                    int(pure_neumann), smoother, omega, int(sync_stride), int(storage), _ptr(out))
+        return {"outer": k, "iters": int(out[3]), "converged": int(out[4]), "rel_res": out[0], "bb": out[1],
+                "rr": out[2], "seconds": out[5]}
+
+    def refined_x(self):
+        a = np.zeros((self.n, 512), np.float64)
+        assert lib().orc_get_refined_x(self.h, _ptr(a)) == 1
+        return a
 
     def damped_jacobi_solve(self, max_iters, tol, omega):
         out = np.zeros(3, np.float64)

@@ -19,7 +19,7 @@ HEADER = os.path.join(ROOT, "include", "cirrus_mgpcg.h")
 def header_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(cg_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(cg_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_header_declares_the_mirrored_symbols():
@@ -41,6 +41,10 @@ def test_params_default_matches_the_reference_defaults():
     # GMGSolver::solve as called by FluidEuler::project (src/FluidEuler.h:598)
     assert (p.max_iters, p.level_iters, p.coarsest_iters, p.sync_stride) == (1000, 1, 10, 1)
     assert p.rel_tol == 1e-6 and p.one_over_alpha == 1.0 and p.prolong_coeff == 1.0
+    assert p.storage == ca.STORE_F32  # FP32 V-cycle storage, as the reference
+    rp = ca.RefineParams()
+    ca.lib().cg_refine_params_default(rp)
+    assert (rp.rel_tol, rp.inner_rel_tol, rp.max_outer) == (1e-8, 1e-4, 20)
 
 
 SCENES = {

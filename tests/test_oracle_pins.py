@@ -491,3 +491,100 @@ def test_pure_neumann_fp64_solve_satisfies_independent_system():
     A = lc.matrix()[m][:, m]
     bc = bl - bl.mean()
     assert np.linalg.norm(bc - A @ x[m]) / np.linalg.norm(bl) < 1e-8
+
+
+# ---------------------------------------------------------------------------
+# Mixed-precision form
+# ---------------------------------------------------------------------------
+def _bf16_nearest(v):
+    """Independent bfloat16 rounding: of the two bfloat16 neighbours of each FP32
+    value (pattern truncated, and one bfloat16 step further from zero), the
+    nearer in exact float64 arithmetic, ties to the even significand."""
+    v = np.ascontiguousarray(v, np.float32)
+    u = v.view(np.uint32)
+    lo_bits = u & np.uint32(0xFFFF0000)
+    hi_bits = lo_bits + np.uint32(0x10000)
+    lo = lo_bits.view(np.float32).astype(np.float64)
+    hi = hi_bits.view(np.float32).astype(np.float64)
+    x = v.astype(np.float64)
+    dlo, dhi = np.abs(x - lo), np.abs(hi - x)
+    take_hi = (dhi < dlo) | ((dhi == dlo) & (((lo_bits >> 16) & 1) == 1))
+    return np.where(take_hi, hi_bits, lo_bits).view(np.float32)
+
+
+def test_bf16_round_every_value_and_tie():
+    pats = np.arange(0, 1 << 16, dtype=np.uint32) << 16
+    vals = pats.view(np.float32)
+    fin = np.isfinite(vals)
+    # bfloat16 values round to themselves; their midpoints (ties) go to the even neighbour
+    exact = vals[fin]
+    assert np.array_equal(po.bf16_round(exact).view(np.uint32), exact.view(np.uint32))
+    mids = (pats[fin] | np.uint32(0x8000)).view(np.float32)
+    # (the top binade's upper half rounds to infinity, outside the two-neighbour rule)
+    mids = mids[(mids.view(np.uint32) & 0x7FFFFFFF) < 0x7F7F8000]
+    assert np.array_equal(po.bf16_round(mids).view(np.uint32), _bf16_nearest(mids).view(np.uint32))
+    rng = np.random.default_rng(0)
+    r = rng.integers(0, 1 << 32, 1 << 20, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    r = r[(r.view(np.uint32) & 0x7FFFFFFF) < 0x7F7F8000]
+    assert np.array_equal(po.bf16_round(r).view(np.uint32), _bf16_nearest(r).view(np.uint32))
+
+
+def _is_bf16(a):
+    return np.all((np.ascontiguousarray(a, np.float32).view(np.uint32) & 0xFFFF) == 0)
+
+
+@pytest.mark.parametrize("smoother,nu", [(po.SMOOTH_RBGS, 2), (po.SMOOTH_JACOBI, 2)])
+def test_bf16_vcycle_stores_only_bf16_values_and_stays_close(smoother, nu):
+    """With BF16 storage every value the V-cycle leaves in its x and rhs
+    channels is a bfloat16 value (a store that skipped the rounding would leave
+    an FP32 value), and z stays within bfloat16 accuracy of the FP32 V-cycle."""
+    o = po.Oracle(fp64=False, **SCENES["band"])
+    b = o.get(po.CH_B)
+    o.vcycle(po.CH_Z, po.CH_R, po.CH_AP, nu, 10, smoother, 2.0 / 3.0, storage=po.STORE_BF16)
+    z16, rhs16 = o.get(po.CH_Z), o.get(po.CH_AP)
+    assert _is_bf16(z16) and _is_bf16(rhs16)
+    o.set(po.CH_B, b)
+    o.vcycle(po.CH_Z, po.CH_R, po.CH_AP, nu, 10, smoother, 2.0 / 3.0)
+    z32 = o.get(po.CH_Z)
+    assert not _is_bf16(z32)
+    rel = np.linalg.norm(z16 - z32) / np.linalg.norm(z32)
+    assert 0 < rel < 2e-2, rel
+
+
+def test_refined_solve_meets_the_independent_fp64_system():
+    """Iterative refinement (FP64 x and residual, BF16-storage MGPCG inside):
+    the FP64 x satisfies the independently assembled system to the outer
+    tolerance, the reported residual is that system's, and on 32^3 x equals a
+    sparse direct solve."""
+    for name in ["band", "uniform32"]:
+        o64, lev, crd, typ, nbr, ct, lc = scene(name)
+        o = po.Oracle(fp64=False, **SCENES[name])
+        b = o.get(po.CH_B)
+        r = o.solve_refined(tol=1e-9, inner_tol=1e-4, level_iters=2, coarsest_iters=10)
+        assert r["converged"] == 1 and r["outer"] >= 2 and r["rel_res"] < 1e-9
+        x = lc.from_tiles(o.refined_x())
+        m = lc.ct == po.INTERIOR
+        A = lc.matrix()[m][:, m]
+        bl = lc.from_tiles(b)[m]
+        res = np.linalg.norm(bl - A @ x[m]) / np.linalg.norm(bl)
+        assert res < 1e-9 and abs(res - r["rel_res"]) < 1e-3 * r["rel_res"], (res, r["rel_res"])
+        if name == "uniform32":
+            xd = spla.spsolve(A.tocsc(), bl)
+            assert np.linalg.norm(x[m] - xd) / np.linalg.norm(xd) < 1e-6
+        # the FP32 channel holds the FP64 x rounded
+        assert np.array_equal(lc.from_tiles(o.get(po.CH_X)), x.astype(np.float32).astype(np.float64))
+
+
+def test_refined_pure_neumann_solve():
+    o = po.Oracle(fp64=False, **ALL_NEUMANN)
+    lev, crd, typ, nbr = o.topology()
+    ct = o.celltypes()
+    lc = LeafCells(lev, crd, typ, ct, o.params.h0, list(o.params.nroot))
+    b = o.get(po.CH_B)
+    r = o.solve_refined(tol=1e-9, inner_tol=1e-4, level_iters=2, coarsest_iters=10, pure_neumann=True)
+    assert r["converged"] == 1
+    x = lc.from_tiles(o.refined_x())
+    m = lc.ct == po.INTERIOR
+    bl = lc.from_tiles(b)[m]
+    A = lc.matrix()[m][:, m]
+    assert np.linalg.norm(bl - bl.mean() - A @ x[m]) / np.linalg.norm(bl) < 1e-8
