@@ -56,12 +56,13 @@ class Params(ctypes.Structure):
                 ("coarsest_iters", ctypes.c_int32), ("smoother", ctypes.c_int32), ("omega", ctypes.c_float),
                 ("sync_stride", ctypes.c_int32), ("pure_neumann", ctypes.c_int32),
                 ("one_over_alpha", ctypes.c_float), ("prolong_coeff", ctypes.c_float), ("use_graph", ctypes.c_int32),
-                ("storage", ctypes.c_int32)]
+                ("storage", ctypes.c_int32), ("verify_f64", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int32), ("converged", ctypes.c_int32), ("rel_res", ctypes.c_double),
-                ("bb", ctypes.c_double), ("rr", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("bb", ctypes.c_double), ("rr", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("rel_res_f64", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -405,14 +406,15 @@ class GMGSolver:
 
     def solve(self, prob, verbose=False, max_iters=1000, relative_tolerance=1e-6, level_iters=1, coarsest_iters=10,
               sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, use_graph=True,
-              storage=STORE_F32):
+              storage=STORE_F32, verify_f64=False):
         """Returns (iters, relative residual) like the reference; the full result
-        is kept in ``self.last``."""
+        is kept in ``self.last`` (with verify_f64, ``rel_res_f64`` is the FP64
+        recomputed |b - A x| / |b|)."""
         p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
                            coarsest_iters=coarsest_iters, smoother=smoother, omega=omega,
                            sync_stride=int(sync_stride), pure_neumann=int(is_pure_neumann),
                            one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff,
-                           use_graph=int(use_graph), storage=int(storage))
+                           use_graph=int(use_graph), storage=int(storage), verify_f64=int(verify_f64))
         r = Result()
         _check(lib().cg_solve(prob.handle, ctypes.byref(p), ctypes.byref(r)), "GMGSolver.solve")
         self.last = r.as_dict()
@@ -427,14 +429,14 @@ class GMGSolver:
 
     def solve_batch(self, prob, max_iters=1000, relative_tolerance=1e-6, level_iters=1, coarsest_iters=10,
                     sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, use_graph=True,
-                    storage=STORE_F32):
+                    storage=STORE_F32, verify_f64=False):
         """All problems of a batched :class:`Problem`; returns one result dict
-        per problem (iters, converged, rel_res, bb, rr, seconds)."""
+        per problem (iters, converged, rel_res, bb, rr, seconds, rel_res_f64)."""
         p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
                            coarsest_iters=coarsest_iters, smoother=smoother, omega=omega,
                            sync_stride=int(sync_stride), pure_neumann=int(is_pure_neumann),
                            one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff,
-                           use_graph=int(use_graph), storage=int(storage))
+                           use_graph=int(use_graph), storage=int(storage), verify_f64=int(verify_f64))
         res = (Result * prob.num_problems)()
         _check(lib().cg_solve_batch(prob.handle, ctypes.byref(p), res), "GMGSolver.solve_batch")
         self.last_batch = [r.as_dict() for r in res]

@@ -1572,7 +1572,8 @@ __global__ __launch_bounds__(64) void k_ir_norm(Ctx c, const TileRec* __restrict
 // r64 = b - A x64 on every cell of the LEAF tiles (R holds A x64 on entry).
 //   mode 0: the next inner solve's rhs r = float(r64), partials of |r64|^2;
 //   mode 1 (pure Neumann, before re-centring): partials of sum r64 and of the
-//           INTERIOR count (MeanAsync, PoissonGrid.cu:231-325).
+//           INTERIOR count (MeanAsync, PoissonGrid.cu:231-325);
+//   mode 2: partials of |r64|^2 only (a check of a finished solve).
 __global__ __launch_bounds__(64) void k_ir_residual(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ B,
                                                     double* __restrict__ R, int mode) {
   const int flags = F_SWIZZLE;
@@ -1590,6 +1591,8 @@ __global__ __launch_bounds__(64) void k_ir_residual(Ctx c, const TileRec* __rest
     float rf[8];
     for (int k = 0; k < 8; k++) { rf[k] = float(r[k]); a += r[k] * r[k]; }
     store8(c.r + base, rf);
+  } else if (mode == 2) {
+    for (int k = 0; k < 8; k++) a += r[k] * r[k];
   } else {
     const uint64_t ct = load_ct8(c.ctype + base);
     for (int k = 0; k < 8; k++) { a += r[k]; q += (ct_at(ct, k) & C_INT) ? 1.0 : 0.0; }
@@ -1598,12 +1601,13 @@ __global__ __launch_bounds__(64) void k_ir_residual(Ctx c, const TileRec* __rest
   a = wave_sum(a);
   if (lane == 0) {
     c.part[T.leaf_slot] = a;
-    if (mode != 0) c.part2[T.leaf_slot] = q;
+    if (mode == 1) c.part2[T.leaf_slot] = q;
   }
 }
 // Pure Neumann: r64 -= mean on INTERIOR cells (ReCenterLeafVoxels,
 // PoissonSolver.cu:356-368, in FP64), then as k_ir_residual mode 0.
-__global__ __launch_bounds__(64) void k_ir_recenter(Ctx c, const TileRec* __restrict__ list, int n, double* __restrict__ R) {
+__global__ __launch_bounds__(64) void k_ir_recenter(Ctx c, const TileRec* __restrict__ list, int n, double* __restrict__ R,
+                                                    int write_r) {
   const int flags = F_SWIZZLE;
   TILE_KERNEL_PROLOGUE
   const int lane = lane_id();
@@ -1617,9 +1621,21 @@ __global__ __launch_bounds__(64) void k_ir_recenter(Ctx c, const TileRec* __rest
   float rf[8];
   double a = 0.0;
   for (int k = 0; k < 8; k++) { rf[k] = float(r[k]); a += r[k] * r[k]; }
-  store8(c.r + base, rf);
+  if (write_r) store8(c.r + base, rf);
   a = wave_sum(a);
   if (lane == 0) c.part[T.leaf_slot] = a;
+}
+// x64 = x on LEAF tiles (the FP32 pressure of a finished solve, to be checked).
+__global__ __launch_bounds__(64) void k_ir_load_x(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ X,
+                                                  double* __restrict__ X64) {
+  const int flags = F_SWIZZLE;
+  TILE_KERNEL_PROLOGUE
+  const size_t base = size_t(T.self) * 512 + lane_id() * 8;
+  float xf[8];
+  double xd[8];
+  load8(X + base, xf);
+  for (int k = 0; k < 8; k++) xd[k] = double(xf[k]);
+  store8d(X64 + base, xd);
 }
 // x64 += x (the inner solve's correction) on LEAF tiles; finish: x = float(x64).
 __global__ __launch_bounds__(64) void k_ir_accumulate(Ctx c, const TileRec* __restrict__ list, int n, double* __restrict__ X64,

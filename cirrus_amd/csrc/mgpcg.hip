@@ -687,10 +687,58 @@ void check_params(const cg_params& prm) {
   if (prm.storage != CG_STORE_F32 && prm.storage != CG_STORE_BF16) fail(CG_ERR_INVALID, "cg_solve: unknown storage type");
 }
 
+// ---------------------------------------------------------------------------
+// FP64 residuals (iterative refinement, checks of finished solves)
+// ---------------------------------------------------------------------------
+void ensure_f64(cg_problem& P) {
+  if (P.d_x64) return;
+  if (!P.owner) fail(CG_ERR_INVALID, "FP64 residuals need an owning problem handle");
+  const size_t cells = size_t(P.n) * 512;
+  HIPC(hipMalloc(&P.d_x64, cells * sizeof(double)));
+  HIPC(hipMalloc(&P.d_r64, cells * sizeof(double)));
+  HIPC(hipMalloc(&P.d_bsave, cells * sizeof(float)));
+  HIPC(hipMalloc(&P.d_psum, size_t(P.nprob) * sizeof(double)));
+  HIPC(hipMemsetAsync(P.d_x64, 0, cells * sizeof(double), P.stream));
+  HIPC(hipMemsetAsync(P.d_r64, 0, cells * sizeof(double), P.stream));
+  HIPC(hipStreamSynchronize(P.stream));
+}
+
+// Per-problem totals of the leaf-slot partials in c.part, to the host.
+std::vector<double> prob_sums(cg_problem& P) {
+  const Ctx c = P.ctx();
+  k_prob_sums<<<unsigned(P.nprob), FIN_THREADS, 0, P.stream>>>(c, P.d_psum);
+  std::vector<double> out(P.nprob);
+  HIPC(hipMemcpyAsync(out.data(), P.d_psum, sizeof(double) * P.nprob, hipMemcpyDeviceToHost, P.stream));
+  HIPC(hipStreamSynchronize(P.stream));
+  return out;
+}
+
+// r64 = B - A x64 on LEAF tiles in FP64 (re-centred for pure Neumann) and
+// |r64|^2 per problem; write_r: also the FP32 copy in the r channel.
+std::vector<double> residual64(cg_problem& P, const float* B, int pure_neumann, bool write_r) {
+  const Ctx c = P.ctx();
+  const unsigned gl = grid_for(P.leaf.n, kSwizzle);
+  if (P.ghosts.n) k_propagate_ghosts64<<<P.ghosts.n, 64, 0, P.stream>>>(c, P.L(P.ghosts), P.ghosts.n, P.d_x64);
+  k_apply64<<<P.applyl.n, 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, P.d_x64, P.d_r64);
+  if (P.ghosts.n) k_accumulate_ghosts64<<<P.ghosts.n, 64, 0, P.stream>>>(c, P.L(P.ghosts), P.ghosts.n, P.d_r64);
+  if (pure_neumann) {
+    k_ir_residual<<<gl, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, B, P.d_r64, 1);
+    k_finalize<<<unsigned(P.nprob), FIN_THREADS, 0, P.stream>>>(c, FIN_MEAN, 0);
+    k_ir_recenter<<<gl, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.d_r64, write_r ? 1 : 0);
+  } else {
+    k_ir_residual<<<gl, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, B, P.d_r64, write_r ? 0 : 2);
+  }
+  return prob_sums(P);
+}
+
 // tol_q / maxit_q: per-problem tolerances and iteration caps (null: prm's for all).
 void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* tol_q = nullptr, const int* maxit_q = nullptr) {
   check_params(prm);
   if (prm.storage == CG_STORE_BF16) ensure_bf16(P);
+  if (prm.verify_f64) {  // keep b for the FP64 check at the end
+    ensure_f64(P);
+    HIPC(hipMemcpyAsync(P.d_bsave, P.d_ch[CG_CH_R], size_t(P.n) * 512 * sizeof(float), hipMemcpyDeviceToDevice, P.stream));
+  }
   VParams v;
   v.nu = prm.level_iters;
   v.coarsest = prm.coarsest_iters;
@@ -759,9 +807,15 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
   HIPC(hipStreamSynchronize(P.stream));
   auto t1 = std::chrono::steady_clock::now();
   const std::vector<ProbState> st = read_states(P);
+  std::vector<double> rr64;
+  if (prm.verify_f64) {
+    k_ir_load_x<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.x, P.d_x64);
+    rr64 = residual64(P, P.d_bsave, prm.pure_neumann, false);
+  }
   for (int q = 0; q < P.nprob; q++) {
     const ProbState& s = st[q];
     cg_result& r = res[q];
+    r.rel_res_f64 = prm.verify_f64 ? (s.bb > 0 ? std::sqrt(rr64[q] / s.bb) : 0.0) : -1.0;
     r.iters = s.iter;
     r.converged = s.converged;
     r.bb = s.bb;
@@ -776,47 +830,6 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
 // ---------------------------------------------------------------------------
 // Mixed-precision form: iterative refinement with an FP64 outer loop
 // ---------------------------------------------------------------------------
-void ensure_f64(cg_problem& P) {
-  if (P.d_x64) return;
-  if (!P.owner) fail(CG_ERR_INVALID, "iterative refinement needs an owning problem handle");
-  const size_t cells = size_t(P.n) * 512;
-  HIPC(hipMalloc(&P.d_x64, cells * sizeof(double)));
-  HIPC(hipMalloc(&P.d_r64, cells * sizeof(double)));
-  HIPC(hipMalloc(&P.d_bsave, cells * sizeof(float)));
-  HIPC(hipMalloc(&P.d_psum, size_t(P.nprob) * sizeof(double)));
-  HIPC(hipMemsetAsync(P.d_x64, 0, cells * sizeof(double), P.stream));
-  HIPC(hipMemsetAsync(P.d_r64, 0, cells * sizeof(double), P.stream));
-  HIPC(hipStreamSynchronize(P.stream));
-}
-
-// Per-problem totals of the leaf-slot partials in c.part, to the host.
-std::vector<double> prob_sums(cg_problem& P) {
-  const Ctx c = P.ctx();
-  k_prob_sums<<<unsigned(P.nprob), FIN_THREADS, 0, P.stream>>>(c, P.d_psum);
-  std::vector<double> out(P.nprob);
-  HIPC(hipMemcpyAsync(out.data(), P.d_psum, sizeof(double) * P.nprob, hipMemcpyDeviceToHost, P.stream));
-  HIPC(hipStreamSynchronize(P.stream));
-  return out;
-}
-
-// r64 = b - A x64 on LEAF tiles in FP64 (re-centred for pure Neumann), the
-// FP32 copy in the r channel, and |r64|^2 per problem.
-std::vector<double> refine_residual(cg_problem& P, int pure_neumann) {
-  const Ctx c = P.ctx();
-  const unsigned gl = grid_for(P.leaf.n, kSwizzle);
-  if (P.ghosts.n) k_propagate_ghosts64<<<P.ghosts.n, 64, 0, P.stream>>>(c, P.L(P.ghosts), P.ghosts.n, P.d_x64);
-  k_apply64<<<P.applyl.n, 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, P.d_x64, P.d_r64);
-  if (P.ghosts.n) k_accumulate_ghosts64<<<P.ghosts.n, 64, 0, P.stream>>>(c, P.L(P.ghosts), P.ghosts.n, P.d_r64);
-  if (pure_neumann) {
-    k_ir_residual<<<gl, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.d_bsave, P.d_r64, 1);
-    k_finalize<<<unsigned(P.nprob), FIN_THREADS, 0, P.stream>>>(c, FIN_MEAN, 0);
-    k_ir_recenter<<<gl, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.d_r64);
-  } else {
-    k_ir_residual<<<gl, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.d_bsave, P.d_r64, 0);
-  }
-  return prob_sums(P);
-}
-
 // Iterative refinement to |b - A x|_2 < rel_tol |b|_2 measured in FP64, every
 // problem of the pool with its own outer loop: the residual and x are FP64; each
 // correction solves A d = float(r) with the FP32 MGPCG (V-cycle storage as
@@ -846,8 +859,10 @@ void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& 
   std::vector<double> tol_q(np);
   std::vector<int> maxit_q(np);
   std::vector<cg_result> inner(np);
+  cg_params inner_prm = prm;
+  inner_prm.verify_f64 = 0;
   while (true) {
-    const std::vector<double> r2 = refine_residual(P, prm.pure_neumann);
+    const std::vector<double> r2 = residual64(P, P.d_bsave, prm.pure_neumann, true);
     bool any = false;
     for (int q = 0; q < np; q++) {
       if (done[q]) continue;
@@ -861,7 +876,7 @@ void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& 
       tol_q[q] = done[q] ? 0.0 : std::max(rp.inner_rel_tol, 0.5 * std::sqrt(thr[q] / rr[q]));
       maxit_q[q] = done[q] ? 0 : prm.max_iters;
     }
-    solve(P, prm, inner.data(), tol_q.data(), maxit_q.data());
+    solve(P, inner_prm, inner.data(), tol_q.data(), maxit_q.data());
     for (int q = 0; q < np; q++)
       if (!done[q]) { outer[q]++; iters[q] += inner[q].iters; }
     k_ir_accumulate<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.d_x64, 0);
@@ -917,6 +932,7 @@ void cg_params_default(cg_params* p) {
   p->prolong_coeff = 1.0f;
   p->use_graph = 1;
   p->storage = CG_STORE_F32;
+  p->verify_f64 = 0;
 }
 
 void cg_refine_params_default(cg_refine_params* p) {
@@ -1053,6 +1069,7 @@ cg_status cg_damped_jacobi_solve(cg_problem* p, int32_t max_iters, double rel_to
     res->bb = bn * bn;
     res->rr = rn * rn;
     res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    res->rel_res_f64 = -1.0;
   });
 }
 

@@ -8,7 +8,10 @@
 // instances whose request is running -- their worklists assembled on the device
 // whenever that set changes -- so requests that arrive while others run join at
 // the next iteration boundary (continuous batching) and each result is the same
-// as solving the request alone.
+// as solving the request alone.  With verify_f64 a finished request's pressure
+// is checked in FP64 (|b - A x| / |b| recomputed from the instance's kept b)
+// before it is reported, and a request that fails the check is restarted warm
+// from its own pressure (at most kMaxRestarts times).
 #pragma once
 
 #include <deque>
@@ -52,7 +55,8 @@ struct RangeAlloc {
   }
 };
 
-enum InstState { INST_IDLE = 0, INST_QUEUED = 1, INST_ACTIVE = 2, INST_DONE = 3 };
+enum InstState { INST_IDLE = 0, INST_QUEUED = 1, INST_ACTIVE = 2, INST_DONE = 3, INST_VERIFY = 4 };
+constexpr int kMaxRestarts = 3;
 
 struct Instance {
   int slot = -1;
@@ -67,7 +71,10 @@ struct Instance {
   CoarsePlan plan;
   float* h_buf = nullptr;  // pinned: rhs on the way in, pressure on the way out
   hipEvent_t ev = nullptr;
+  hipEvent_t ev_ver = nullptr;  // the FP64 check's result has arrived
   int state = INST_IDLE;
+  int restarts = 0;     // warm restarts after a failed FP64 check
+  int iters_acc = 0;    // CG iterations of the request's earlier runs
   int warm = 0;
   int dev_rhs = 0;      // the right-hand side is already on the device
   int no_download = 0;  // the pressure stays on the device
@@ -100,6 +107,11 @@ struct cg_server {
   uint8_t* d_fmask = nullptr;
   float* d_ch[CG_NUM_CHANNELS] = {};
   float* d_zb = nullptr;
+  float* d_b = nullptr;  // each instance's right-hand side, kept across its requests
+  double* d_x64 = nullptr;  // FP64 check of finished requests (verify_f64): x, residual, per-slot |r|^2
+  double* d_r64 = nullptr;
+  double* d_ver = nullptr;
+  double* h_ver = nullptr;  // pinned
   bf16* d_zh = nullptr;  // bfloat16 V-cycle vectors (storage CG_STORE_BF16)
   bf16* d_rhsvh = nullptr;
   bf16* d_zbh = nullptr;
@@ -136,8 +148,10 @@ cg_server::~cg_server() {
     if (I->d_slots) (void)hipFree(I->d_slots);
     if (I->h_buf) (void)hipHostFree(I->h_buf);
     if (I->ev) (void)hipEventDestroy(I->ev);
+    if (I->ev_ver) (void)hipEventDestroy(I->ev_ver);
   }
-  void* ptrs[] = {d_ctype, d_fmask, d_zb, d_zh, d_rhsvh, d_zbh, d_gsum, d_ps, d_dflag, d_dflag_init, d_pmap, d_newmap, d_part, d_part2, d_asm,
+  if (h_ver) (void)hipHostFree(h_ver);
+  void* ptrs[] = {d_ctype, d_fmask, d_zb, d_b, d_x64, d_r64, d_ver, d_zh, d_rhsvh, d_zbh, d_gsum, d_ps, d_dflag, d_dflag_init, d_pmap, d_newmap, d_part, d_part2, d_asm,
                   d_plans, (void*)d_srcs, d_offs, d_counts};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
@@ -150,12 +164,13 @@ cg_server::~cg_server() {
 
 namespace {
 
-// Device bytes one tile costs: 6 channels + the V-cycle scratch (FP32), cell
-// types and face masks, ghost row sums, dot partials and room for its records
-// in the assembled worklists; BF16 storage adds three bfloat16 vectors.
+// Device bytes one tile costs: 6 channels, the V-cycle scratch and the kept
+// rhs (FP32), cell types and face masks, ghost row sums, dot partials and room
+// for its records in the assembled worklists; BF16 storage adds three bfloat16
+// vectors, verify_f64 two FP64 vectors.
 constexpr int kRecKindsPerTile = 5;  // leaf, gather, applyl, actg, lvlg: at most 5 records per tile
 constexpr int64_t kBytesPerTile =
-    (CG_NUM_CHANNELS + 1) * 512 * 4 + 2 * 512 + 64 * 4 + 2 * 8 + kRecKindsPerTile * int64_t(sizeof(TileRec));
+    (CG_NUM_CHANNELS + 2) * 512 * 4 + 2 * 512 + 64 * 4 + 2 * 8 + kRecKindsPerTile * int64_t(sizeof(TileRec));
 
 void server_create(const cg_server_config* cfg, cg_server& S) {
   if (!cfg || cfg->budget_bytes <= 0 || cfg->max_problems <= 0) fail(CG_ERR_INVALID, "cg_server_create: bad configuration");
@@ -169,7 +184,8 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   S.v.ooa = prm.one_over_alpha;
   S.v.pcoef = prm.prolong_coeff;
   S.max_probs = cfg->max_problems;
-  const int64_t per_tile = kBytesPerTile + (prm.storage == CG_STORE_BF16 ? 3 * 512 * int64_t(sizeof(bf16)) : 0);
+  const int64_t per_tile = kBytesPerTile + (prm.storage == CG_STORE_BF16 ? 3 * 512 * int64_t(sizeof(bf16)) : 0) +
+                           (prm.verify_f64 ? 2 * 512 * int64_t(sizeof(double)) : 0);
   S.cap_tiles = std::max<int64_t>(64, cfg->budget_bytes / per_tile);
   if (S.cap_tiles * 512 > int64_t(INT32_MAX)) S.cap_tiles = int64_t(INT32_MAX) / 512;  // cell offsets stay 32-bit safe
   S.cap_leaf = S.cap_tiles;
@@ -186,6 +202,16 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   HIPC(hipMalloc(&S.d_fmask, cells));
   for (int c = 0; c < CG_NUM_CHANNELS; c++) HIPC(hipMalloc(&S.d_ch[c], cells * sizeof(float)));
   HIPC(hipMalloc(&S.d_zb, cells * sizeof(float)));
+  HIPC(hipMalloc(&S.d_b, cells * sizeof(float)));
+  HIPC(hipMemset(S.d_b, 0, cells * sizeof(float)));
+  if (S.prm.verify_f64) {
+    HIPC(hipMalloc(&S.d_x64, cells * sizeof(double)));
+    HIPC(hipMalloc(&S.d_r64, cells * sizeof(double)));
+    HIPC(hipMemset(S.d_x64, 0, cells * sizeof(double)));
+    HIPC(hipMemset(S.d_r64, 0, cells * sizeof(double)));
+    HIPC(hipMalloc(&S.d_ver, sizeof(double) * S.max_probs));
+    HIPC(hipHostMalloc(&S.h_ver, sizeof(double) * S.max_probs, hipHostMallocDefault));
+  }
   if (S.prm.storage == CG_STORE_BF16) {
     bf16** bufs[3] = {&S.d_zh, &S.d_rhsvh, &S.d_zbh};
     for (bf16** b : bufs) {
@@ -319,8 +345,11 @@ void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
   k_face_mask<<<I->all.n, 64, 0, S.stream>>>(S.d_ctype, S.d_fmask, I->L(I->all), I->all.n);
   HIPC(hipGetLastError());
   HIPC(hipMemset(S.d_ch[CG_CH_X] + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float)));
+  HIPC(hipMemset(S.d_b + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float)));
+  if (S.d_x64) HIPC(hipMemset(S.d_x64 + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(double)));
   HIPC(hipHostMalloc(&I->h_buf, size_t(n) * 512 * sizeof(float), hipHostMallocDefault));
   HIPC(hipEventCreateWithFlags(&I->ev, hipEventDisableTiming));
+  HIPC(hipEventCreateWithFlags(&I->ev_ver, hipEventDisableTiming));
   *out = slot;
   S.inst[slot] = std::move(I);
 }
@@ -332,7 +361,8 @@ Instance& instance(cg_server& S, int32_t id) {
 
 void server_release(cg_server& S, int32_t id) {
   Instance& I = instance(S, id);
-  if (I.state == INST_QUEUED || I.state == INST_ACTIVE) fail(CG_ERR_STATE, "cg_server_release: the instance has a request running");
+  if (I.state == INST_QUEUED || I.state == INST_ACTIVE || I.state == INST_VERIFY)
+    fail(CG_ERR_STATE, "cg_server_release: the instance has a request running");
   HIPC(hipStreamSynchronize(S.stream));  // no kernel may still read its slab
   if (I.state == INST_DONE)
     for (auto it = S.done_q.begin(); it != S.done_q.end(); ++it)
@@ -348,6 +378,8 @@ void server_release(cg_server& S, int32_t id) {
   I.h_buf = nullptr;
   HIPC(hipEventDestroy(I.ev));
   I.ev = nullptr;
+  HIPC(hipEventDestroy(I.ev_ver));
+  I.ev_ver = nullptr;
   S.inst[id].reset();
   S.free_slots.push_back(id);
 }
@@ -361,6 +393,8 @@ void server_submit(cg_server& S, int32_t id, const float* rhs, int32_t flags, in
   I.no_download = (flags & CG_SUBMIT_NO_DOWNLOAD) ? 1 : 0;
   I.warm = (flags & CG_SUBMIT_WARM) ? 1 : 0;
   I.tag = tag;
+  I.restarts = 0;
+  I.iters_acc = 0;
   I.t_submit = std::chrono::steady_clock::now();
   I.state = INST_QUEUED;
 }
@@ -388,15 +422,15 @@ void server_start_queued(cg_server& S, int k) {
     ps.check_stride = S.prm.sync_stride;
     HIPC(hipMemcpyAsync(S.d_ps + I.slot, &ps, sizeof(ps), hipMemcpyHostToDevice, st));
     if (!I.dev_rhs)
-      HIPC(hipMemcpyAsync(S.d_ch[CG_CH_B] + size_t(I.tile_begin) * 512, I.h_buf, size_t(I.ntiles) * 512 * sizeof(float),
+      HIPC(hipMemcpyAsync(S.d_b + size_t(I.tile_begin) * 512, I.h_buf, size_t(I.ntiles) * 512 * sizeof(float),
                           hipMemcpyHostToDevice, st));
     float* X = S.d_ch[CG_CH_X];
     if (!I.warm) k_fill_tiles<float><<<I.all.n, 64, 0, st>>>(I.L(I.all), I.all.n, X, 0.f);
     if (I.ghosts.n) k_propagate_ghosts<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, X, X);
     k_apply<<<I.applyl.n, 64, 0, st>>>(c, I.L(I.applyl), I.applyl.n, X, c.ap, -1, 0, 0, 0);
     if (I.ghosts.n) k_accumulate_ghosts<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, c.ap);
-    if (S.prm.storage == CG_STORE_BF16) k_cg_init_warm<bf16><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_ch[CG_CH_B], 0);
-    else k_cg_init_warm<float><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_ch[CG_CH_B], 0);
+    if (S.prm.storage == CG_STORE_BF16) k_cg_init_warm<bf16><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, 0);
+    else k_cg_init_warm<float><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, 0);
     hmap[f] = I.slot;
     I.state = INST_ACTIVE;
     S.active.push_back(I.slot);
@@ -473,6 +507,43 @@ void server_assemble(cg_server& S, int k) {
   S.dirty = false;
 }
 
+// A request is reported: its pressure goes to the pinned buffer (unless it stays
+// on the device) and it joins the completion queue.
+void finish_request(cg_server& S, Instance& I) {
+  if (!I.no_download)
+    HIPC(hipMemcpyAsync(I.h_buf, S.d_ch[CG_CH_X] + size_t(I.tile_begin) * 512, size_t(I.ntiles) * 512 * sizeof(float),
+                        hipMemcpyDeviceToHost, S.stream));
+  HIPC(hipEventRecord(I.ev, S.stream));
+  I.state = INST_DONE;
+  S.done_q.push_back(I.slot);
+}
+
+// FP64 check of a finished request on the instance's own lists: x64 = x,
+// |b - A x64|^2 (re-centred for pure Neumann) into the instance's slot.
+void enqueue_verify(cg_server& S, Instance& I) {
+  const hipStream_t st = S.stream;
+  const Ctx c = S.view.ctx();
+  Ctx cv = c;  // one-problem view for the per-problem reductions
+  cv.ps = S.d_ps + I.slot;
+  cv.pmap = nullptr;
+  const unsigned gl = grid_for(I.leaf.n, kSwizzle);
+  k_ir_load_x<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_ch[CG_CH_X], S.d_x64);
+  if (I.ghosts.n) k_propagate_ghosts64<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, S.d_x64);
+  k_apply64<<<I.applyl.n, 64, 0, st>>>(c, I.L(I.applyl), I.applyl.n, S.d_x64, S.d_r64);
+  if (I.ghosts.n) k_accumulate_ghosts64<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, S.d_r64);
+  if (S.prm.pure_neumann) {
+    k_ir_residual<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, S.d_r64, 1);
+    k_finalize<<<1, FIN_THREADS, 0, st>>>(cv, FIN_MEAN, 0);
+    k_ir_recenter<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_r64, 0);
+  } else {
+    k_ir_residual<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, S.d_r64, 2);
+  }
+  k_prob_sums<<<1, FIN_THREADS, 0, st>>>(cv, S.d_ver + I.slot);
+  HIPC(hipMemcpyAsync(S.h_ver + I.slot, S.d_ver + I.slot, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPC(hipEventRecord(I.ev_ver, st));
+  I.state = INST_VERIFY;
+}
+
 // Finished requests of the iteration snapshot k: results, pressure download.
 void server_collect(cg_server& S, int k) {
   if (!S.snap_pending[k]) return;
@@ -485,20 +556,46 @@ void server_collect(cg_server& S, int k) {
     Instance& I = *S.inst[q];
     if (I.state != INST_ACTIVE || !st[q].done) continue;
     const ProbState& s = st[q];
-    I.res.iters = s.iter;
+    I.res.iters = I.iters_acc + s.iter;
     I.res.converged = s.converged;
     I.res.bb = s.bb;
     I.res.rr = s.rr;
     I.res.rel_res = s.bb > 0 ? (S.prm.sync_stride > 0 ? std::sqrt(s.rr / s.bb) : -1.0) : 0.0;
+    I.res.rel_res_f64 = -1.0;
     I.res.seconds = std::chrono::duration<double>(now - I.t_submit).count();
-    if (!I.no_download)
-      HIPC(hipMemcpyAsync(I.h_buf, S.d_ch[CG_CH_X] + size_t(I.tile_begin) * 512, size_t(I.ntiles) * 512 * sizeof(float),
-                          hipMemcpyDeviceToHost, S.stream));
-    HIPC(hipEventRecord(I.ev, S.stream));
-    I.state = INST_DONE;
-    S.done_q.push_back(q);
     S.active.erase(std::find(S.active.begin(), S.active.end(), q));
     S.dirty = true;
+    if (S.prm.verify_f64 && s.bb > 0) {
+      enqueue_verify(S, I);
+    } else {
+      if (S.prm.verify_f64) I.res.rel_res_f64 = 0.0;  // b = 0: x = 0 is exact
+      finish_request(S, I);
+    }
+  }
+}
+
+// Requests whose FP64 check has arrived: report them, or restart them warm from
+// their own pressure when the check fails (kMaxRestarts at most).
+void server_verified(cg_server& S) {
+  for (int q = 0; q < S.max_probs; q++) {
+    if (!S.inst[q] || S.inst[q]->state != INST_VERIFY) continue;
+    Instance& I = *S.inst[q];
+    const hipError_t e = hipEventQuery(I.ev_ver);
+    if (e == hipErrorNotReady) continue;
+    HIPC(e);
+    const double rr = S.h_ver[q];
+    I.res.rel_res_f64 = std::sqrt(rr / I.res.bb);
+    const bool pass = rr < S.prm.rel_tol * S.prm.rel_tol * I.res.bb;
+    if (pass || I.restarts >= kMaxRestarts || I.res.converged == 0) {
+      I.res.converged = pass ? 1 : 0;
+      finish_request(S, I);
+    } else {
+      I.restarts++;
+      I.iters_acc = I.res.iters;
+      I.warm = 1;
+      I.dev_rhs = 1;
+      I.state = INST_QUEUED;
+    }
   }
 }
 
@@ -519,6 +616,7 @@ void server_step(cg_server& S, int32_t* n_active) {
     S.snap_pending[k] = true;
   }
   server_collect(S, 1 - k);
+  if (S.prm.verify_f64) server_verified(S);
   server_start_queued(S, k);
   if (S.dirty) server_assemble(S, k);
   S.step_no++;
@@ -529,7 +627,7 @@ void server_step(cg_server& S, int32_t* n_active) {
 void server_perturb(cg_server& S, int32_t id, float rho, float sigma, uint64_t seed) {
   Instance& I = instance(S, id);
   if (I.state != INST_IDLE) fail(CG_ERR_STATE, "cg_server_perturb_rhs: the instance has a request running or an unread result");
-  k_perturb_rhs<<<I.leaf.n, 64, 0, S.stream>>>(S.view.ctx(), I.L(I.leaf), I.leaf.n, S.d_ch[CG_CH_B], rho, sigma, seed);
+  k_perturb_rhs<<<I.leaf.n, 64, 0, S.stream>>>(S.view.ctx(), I.L(I.leaf), I.leaf.n, S.d_b, rho, sigma, seed);
   HIPC(hipGetLastError());
 }
 

@@ -80,6 +80,7 @@ typedef struct {
   float prolong_coeff;      /* prolongation scale (1) */
   int32_t use_graph;        /* capture one CG iteration in a hipGraph and replay it (1) */
   int32_t storage;          /* CG_STORE_F32 (0, the reference) or CG_STORE_BF16 */
+  int32_t verify_f64;       /* recompute |b - A x| / |b| in FP64 at the end (cg_result.rel_res_f64) (0) */
 } cg_params;
 
 typedef struct {
@@ -89,6 +90,9 @@ typedef struct {
   double bb;                /* |b|^2 over LEAF tiles */
   double rr;                /* final |r|^2 */
   double seconds;           /* wall time of the solve, host clock, device synchronised */
+  double rel_res_f64;       /* sqrt(|b - A x|^2 / |b|^2) of the FP32 x recomputed in FP64 over LEAF tiles
+                               (re-centred for pure Neumann) when verify_f64 is set, else -1; no reference
+                               counterpart */
 } cg_result;
 
 void cg_params_default(cg_params* p);

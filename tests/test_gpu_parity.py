@@ -268,3 +268,23 @@ def test_solve_vs_fp64_oracle(name):
     ax = o64.get(po.CH_AP)
     res = np.linalg.norm((b - ax)[m]) / np.linalg.norm(b[m])
     assert res < 5e-6, f"true relative residual {res:.3e}"
+
+
+@pytest.mark.parametrize("name,pure", [("band", False), ("vortex2root", False), ("plane", False)])
+def test_verify_f64_equals_oracle_fp64_residual(name, pure):
+    """verify_f64: the FP64 residual of the FP32 pressure, recomputed on the
+    device, equals the FP64 oracle's operator and dot on the same x, bit for bit."""
+    o, P, typ, nbr, b = pair(name)
+    both_set(o, P, po.CH_B, b)
+    S = ca.GMGSolver()
+    S.solve(P, False, 200, 1e-6, 2, 10, 1, pure, verify_f64=True)
+    x32 = P.get(ca.CH_X)
+    o64 = po.Oracle(params=o.params, fp64=True)
+    leaf = (typ == po.LEAF)[:, None]
+    o64.set(po.CH_X, np.where(leaf, x32.astype(np.float64), 0.0))
+    o64.full_negative_laplacian(po.CH_X, po.CH_AP)
+    r = np.where(leaf, b.astype(np.float32).astype(np.float64) - o64.get(po.CH_AP), 0.0)
+    o64.set(po.CH_TMP, r)
+    rr = o64.dot(po.CH_TMP, po.CH_TMP, po.LEAF)
+    assert S.last["rel_res_f64"] == np.sqrt(rr / S.last["bb"])
+    assert S.last["rel_res_f64"] < 2e-6

@@ -170,7 +170,7 @@ def test_warm_start_reaches_the_tolerance():
 def test_admission_by_budget_and_release():
     scs = scenes()
     big = scs[1].tree
-    per_tile = 7 * 2048 + 512 + 256 + 16 + 5 * 176
+    per_tile = 8 * 2048 + 2 * 512 + 256 + 16 + 5 * 176
     S = ca.Server(int(2.5 * big.n * per_tile), 16, **KW)
     a = S.admit(big)
     b = S.admit(big)
@@ -184,3 +184,48 @@ def test_admission_by_budget_and_release():
     assert res[7][2]["iters"] == ref["iters"]
     assert S.info()["instances"] == 2
     del b
+
+
+def test_perturbation_updates_the_kept_rhs():
+    """The instance keeps its right-hand side across requests: rho = 1, sigma = 0
+    leaves b as it was, so the warm request solves the same b (same |b|^2) from
+    its converged pressure and stops within a few iterations."""
+    sc = scenes()[1]
+    S = ca.Server(BUDGET, 4, **KW)
+    i = S.admit(sc.tree)
+    S.submit(i, sc.rhs, tag=0)
+    r0 = run_until_done(S, 1)[0][2]
+    S.perturb_rhs(i, 1.0, 0.0, 5)
+    S.submit(i, None, warm=True, tag=1)
+    r1 = run_until_done(S, 1)[1][2]
+    assert r1["bb"] == r0["bb"] and r1["converged"] == 1 and r1["iters"] <= 3
+
+
+def test_verified_requests_equal_verified_solve():
+    """verify_f64: every reported request carries its FP64 residual
+    |b - A x| / |b|, recomputed on the device from the kept b; with BF16 V-cycle
+    storage a cold request still equals the verified solve alone."""
+    scs = scenes()
+    S = ca.Server(BUDGET, 16, storage=ca.STORE_BF16, verify_f64=1, **KW)
+    ids = [S.admit(sc.tree) for sc in scs]
+    for q, (i, sc) in enumerate(zip(ids, scs)):
+        S.submit(i, sc.rhs, tag=q)
+    res = run_until_done(S, len(scs))
+    for q, sc in enumerate(scs):
+        P = ca.Problem(sc.tree)
+        P.set(ca.CH_B, sc.rhs)
+        G = ca.GMGSolver()
+        G.solve(P, False, 1000, 1e-6, KW["level_iters"], KW["coarsest_iters"], 1, False, storage=ca.STORE_BF16,
+                verify_f64=True)
+        r = res[q][2]
+        assert 0 <= r["rel_res_f64"] < 1e-6 and r["converged"] == 1, r
+        if G.last["rel_res_f64"] < 1e-6:  # no restart needed: the same run
+            assert (r["iters"], r["rr"], r["rel_res_f64"]) == (G.last["iters"], G.last["rr"], G.last["rel_res_f64"])
+    # warm requests in the mix are verified too
+    for q, (i, sc) in enumerate(zip(ids, scs)):
+        S.perturb_rhs(i, 0.95, 0.3 * float(np.sqrt(np.mean(sc.rhs[sc.rhs != 0] ** 2))), 100 + q)
+        S.submit(i, None, warm=True, tag=10 + q)
+    res = run_until_done(S, len(scs))
+    for q in range(len(scs)):
+        r = res[10 + q][2]
+        assert 0 <= r["rel_res_f64"] < 1e-6 and r["converged"] == 1, r
