@@ -79,7 +79,8 @@ def batch_scene_params(q, seed=0):
     return kw, target
 
 
-def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=64, nscenes=8, budget=24 << 30, rho=0.95, steps=50):
+def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=64, nscenes=8, budget=24 << 30, rho=0.95, steps=50,
+               storage=0, verify=False):
     """Serving form: requests arrive as a Poisson process of rate `lam` for
     `duration` seconds; request j goes to resident instance j mod conc, whose
     hierarchy and pressure stay on the device.  Each instance runs sequences of
@@ -87,14 +88,17 @@ def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=64, nscenes=8, budg
     b = rho b + sqrt(1 - rho^2) rms(b0) u (u seeded unit-variance noise) and the
     solve is warm-started from its previous pressure; every steps-th request
     starts a new sequence (cold).  Requests that find their instance busy wait;
-    latency is arrival -> result.  Instances are built like the batched form."""
+    latency is arrival -> result.  Instances are built like the batched form.
+    storage: V-cycle storage (0 FP32, 1 BF16); verify: every result's
+    |b - A x| / |b| recomputed in FP64 before it is reported (a failed check
+    restarts the request warm)."""
     import collections
     trees_b0 = []
     for q in range(nscenes):
         sc = ca.Scene(**batch_scene_params(q, seed=1)[0])
         trees_b0.append((sc.tree, np.ascontiguousarray(sc.rhs, np.float32)))
     S = ca.Server(budget, conc, level_iters=solver_kw["level_iters"], coarsest_iters=solver_kw["coarsest_iters"],
-                  smoother=solver_kw["smoother"])
+                  smoother=solver_kw["smoother"], storage=int(storage), verify_f64=int(bool(verify)))
     insts, sig = [], []
     for w in range(conc):
         tree, b0 = trees_b0[w % nscenes]
@@ -118,7 +122,7 @@ def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=64, nscenes=8, budg
     fifo = [collections.deque() for _ in range(W)]
     busy = [False] * W
     count = [1] * W  # solves done per instance (the warm-up one included)
-    lat, iters, act = [], [], []
+    lat, iters, act, f64 = [], [], [], []
     nxt = 0
     t0 = time.perf_counter()
     deadline = duration + 20.0
@@ -150,6 +154,7 @@ def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=64, nscenes=8, budg
             count[w] += 1
             lat.append(t_done - arrivals[j])
             iters.append(res["iters"])
+            f64.append(res["rel_res_f64"])
         if nxt >= len(arrivals) and not any(busy) and not any(fifo):
             break
         if now > deadline:
@@ -157,21 +162,29 @@ def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=64, nscenes=8, budg
     t_end = time.perf_counter() - t0
     lat_ms = np.array(lat) * 1e3
     leaf = [int((trees_b0[w % nscenes][0].type == 1).sum()) for w in range(W)]
+    prec = ("BF16 V-cycle storage" if storage else "FP32") + (", every result checked in FP64" if verify else "")
     return {
         "workload": f"Poisson requests at {lam:g}/s for {duration:g} s over {W} resident instances "
                     f"(lognormal sizes, median {int(np.median(leaf))} leaf tiles), warm-started correlated "
-                    f"sequences of {steps}, FP32, RB-GS 2+2, tol 1e-6 of |b|",
+                    f"sequences of {steps}, {prec}, RB-GS 2+2, tol 1e-6 of |b|",
         "instances": W, "requests": int(len(arrivals)), "completed": len(lat),
         "solves_per_s": len(lat) / t_end, "offered_per_s": lam,
         "p50_ms": float(np.percentile(lat_ms, 50)) if len(lat) else None,
         "p99_ms": float(np.percentile(lat_ms, 99)) if len(lat) else None,
         "iterations_median": float(np.median(iters)) if iters else None,
         "active_mean": float(np.mean(act)) if act else 0.0, "wall_s": t_end,
+        "fp64_rel_res_max": float(np.max(f64)) if (verify and f64) else None,
+        "fp64_checked": len(f64) if verify else 0,
+        "fp64_over_tol": int(np.sum(np.array(f64) >= 1e-6)) if verify else None,
     }
 
 
-def run_batched(ca, nprob, reps, solver_kw, use_graph=True):
-    """Solve nprob independent problems together; returns the form's record."""
+def run_batched(ca, nprob, reps, solver_kw, use_graph=True, storage=0, refine=False):
+    """Solve nprob independent problems together; returns the form's record.
+    storage: V-cycle storage (0 FP32, 1 BF16).  refine: FP64 iterative
+    refinement to 1e-6 of the FP64 residual (inner MGPCG to 5e-7); otherwise
+    the MGPCG to 1e-6 of its recursive residual, each pressure then checked in
+    FP64."""
     t0 = time.perf_counter()
     scenes = [ca.Scene(**batch_scene_params(q)[0]) for q in range(nprob)]
     t_build = time.perf_counter() - t0
@@ -182,8 +195,12 @@ def run_batched(ca, nprob, reps, solver_kw, use_graph=True):
     def once():
         B.set(ca.CH_B, b_all)
         t = time.perf_counter()
-        res = S.solve_batch(B, 1000, 1e-6, solver_kw["level_iters"], solver_kw["coarsest_iters"],
-                            smoother=solver_kw["smoother"], use_graph=use_graph)
+        if refine:
+            res = S.solve_refined(B, 1e-6, 5e-7, 20, 1000, solver_kw["level_iters"], solver_kw["coarsest_iters"],
+                                  smoother=solver_kw["smoother"], use_graph=use_graph, storage=storage)
+        else:
+            res = S.solve_batch(B, 1000, 1e-6, solver_kw["level_iters"], solver_kw["coarsest_iters"],
+                                smoother=solver_kw["smoother"], use_graph=use_graph, storage=storage, verify_f64=True)
         return time.perf_counter() - t, res
 
     once()  # warm-up (graph capture)
@@ -194,15 +211,83 @@ def run_batched(ca, nprob, reps, solver_kw, use_graph=True):
         assert all(r["converged"] == 1 for r in res)
     leaf = [int((sc.tree.type == 1).sum()) for sc in scenes]
     iters = [r["iters"] for r in res]
+    f64 = [r["rel_res"] if refine else r["rel_res_f64"] for r in res]
     total = float(np.sum(times))
+    # the V-cycle's dominant kernel on the pool's most populated level
+    lv = int(np.bincount(np.concatenate([sc.tree.level[sc.tree.type != 4] for sc in scenes])).argmax())
+    kid = ca.KERNEL_GS_SWEEP_BF16 if storage else ca.KERNEL_GS_SWEEP
+    gus, gtiles = B.benchmark_kernel(kid, lv, reps=20)
+    gbytes = gs_alg_bytes(gtiles, 2 if storage else 4)
+    prec = "BF16 V-cycle storage" if storage else "FP32"
+    how = "FP64 iterative refinement to 1e-6 of the FP64 residual" if refine else \
+        "tol 1e-6 each (recursive residual), every pressure checked in FP64"
     return {
         "workload": f"{nprob} independent adaptive problems (sphere band, lognormal leaf-tile counts), "
-                    "FP32, RB-GS 2+2, tol 1e-6 each, solved together",
+                    f"{prec}, RB-GS 2+2, {how}, solved together",
         "problems": nprob, "leaf_tiles_total": int(np.sum(leaf)), "leaf_tiles_median": int(np.median(leaf)),
         "leaf_tiles_min": int(np.min(leaf)), "leaf_tiles_max": int(np.max(leaf)),
         "solves_per_s": nprob * reps / total, "ms_per_batch": total / reps * 1e3,
         "iterations_min": int(np.min(iters)), "iterations_median": float(np.median(iters)),
         "iterations_max": int(np.max(iters)), "build_s": t_build, "reps": reps,
+        "fp64_rel_res_max": float(np.max(f64)),
+        "fp64_over_tol": int(np.sum(np.array(f64) >= 1e-6)),
+        "outer_max": int(max(r["outer"] for r in res)) if refine else None,
+        "gs_sweep": {"level": lv, "tiles": gtiles, "us": gus, "alg_bytes": gbytes,
+                     "GBps": gbytes / (gus * 1e-6) / 1e9, "frac_hbm_peak": gbytes / (gus * 1e-6) / 1e9 / PEAK_HBM_GBS},
+    }
+
+
+VORTEX_KW = dict(kind=3, nroot=[1, 1, 2], vortex_level=[6, 4], vortex_thresh=0.5, seed=0)
+VORTEX_SMALL_KW = dict(kind=3, nroot=[1, 1, 2], vortex_level=[5, 3], vortex_thresh=0.5, seed=0)
+
+
+def run_vortex(ca, solver_kw, reps=3, kw=VORTEX_KW):
+    """Mixed-precision form on the 512x512x1024-effective vortex-tube octree
+    (domain [0,1]x[0,1]x[0,2], two root tiles, leaf level 6 where the vorticity
+    of 8 seeded Gaussian tubes exceeds the threshold, level 4 elsewhere): BF16
+    V-cycle storage, FP32 smoothing and stencils, FP64 residual and x,
+    iterative refinement to 1e-8 of the FP64 residual.  The plain FP32 solve to
+    1e-6 of the same problem is timed beside it."""
+    t0 = time.perf_counter()
+    sc = ca.Scene(**kw)
+    t_build = time.perf_counter() - t0
+    P = ca.Problem(sc.tree)
+    b = np.ascontiguousarray(sc.rhs, np.float32)
+    S = ca.GMGSolver(1.0, 1.0)
+    nu, nc, sm = solver_kw["level_iters"], solver_kw["coarsest_iters"], solver_kw["smoother"]
+
+    def refined():
+        P.set(ca.CH_B, b)
+        return S.solve_refined(P, 1e-8, 1e-4, 20, 1000, nu, nc, smoother=sm, storage=ca.STORE_BF16)[0]
+
+    def plain():
+        P.set(ca.CH_B, b)
+        S.solve(P, False, 1000, 1e-6, nu, nc, 1, False, smoother=sm, verify_f64=True)
+        return S.last
+
+    refined()
+    plain()
+    rr = [refined() for _ in range(reps)]
+    pp = [plain() for _ in range(reps)]
+    r = rr[-1]
+    assert r["converged"] == 1
+    L = P.max_level
+    k32, nt = P.benchmark_kernel(ca.KERNEL_GS_SWEEP, L, reps=50)
+    k16, _ = P.benchmark_kernel(ca.KERNEL_GS_SWEEP_BF16, L, reps=50)
+    x64 = P.get_f64()
+    return {
+        "workload": "vortex-tube octree 512x512x1024 effective (8 seeded Gaussian tubes, leaf level 6 where "
+                    "|w| > 0.5, level 4 elsewhere), BF16 V-cycle storage, FP32 smoothing/stencils, FP64 residual "
+                    "and x, iterative refinement to 1e-8",
+        "tiles": P.n, "leaf_tiles": P.n_leaf, "leaf_cells": P.n_leaf * 512, "build_s": t_build,
+        "solves_per_s": reps / sum(q["seconds"] for q in rr), "ms_per_solve": float(np.median([q["seconds"] for q in rr])) * 1e3,
+        "refinement_steps": r["outer"], "iterations": r["iters"], "fp64_rel_res": r["rel_res"],
+        "plain_f32_1e-6": {"ms_per_solve": float(np.median([q["seconds"] for q in pp])) * 1e3,
+                           "iterations": pp[-1]["iters"], "fp64_rel_res": pp[-1]["rel_res_f64"]},
+        "gs_sweep_level": L, "gs_sweep_tiles": nt,
+        "gs_sweep_f32": {"us": k32, "GBps": gs_alg_bytes(nt, 4) / (k32 * 1e-6) / 1e9},
+        "gs_sweep_bf16": {"us": k16, "GBps": gs_alg_bytes(nt, 2) / (k16 * 1e-6) / 1e9},
+        "_x64_leaf": x64[sc.tree.type == 1],
     }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gs_color_adaptive256_hbm_traffic.txt")
 KERNEL_REPS = 20
@@ -240,10 +325,43 @@ def cpu_baseline(scene_kw, solver_kw, budget_s=20.0, max_solves=8):
     }
 
 
-def gs_alg_bytes(tiles):
+def fp64_oracle_check(ca, solver_kw, kw=VORTEX_SMALL_KW):
+    """Error of the mixed-precision pressure against the FP64 oracle, on a
+    vortex-tube octree one level coarser than the measured one (leaf level 5,
+    level 3 elsewhere) so that the FP64 CPU solve stays bounded: the same
+    refinement (BF16 storage, FP64 residual, 1e-8) on the GPU, the FP64 oracle
+    solved to 1e-10, relative L2 difference over LEAF INTERIOR cells."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    o = po.Oracle(**kw)
+    lev, crd, typ, _ = o.topology()
+    ct = o.celltypes()
+    P = ca.Problem(ca.Tree(o.params.h0, lev, crd, typ, ct))
+    b = o.get(po.CH_B).astype(np.float32)
+    P.set(ca.CH_B, b)
+    r = ca.GMGSolver().solve_refined(P, 1e-8, 1e-4, 20, 1000, solver_kw["level_iters"], solver_kw["coarsest_iters"],
+                                     smoother=solver_kw["smoother"], storage=ca.STORE_BF16)[0]
+    x = P.get_f64()
+    o64 = po.Oracle(params=o.params, fp64=True)
+    o64.set(po.CH_B, b.astype(np.float64))
+    t = time.perf_counter()
+    r64 = o64.solve(max_iters=500, tol=1e-10, level_iters=solver_kw["level_iters"],
+                    coarsest_iters=solver_kw["coarsest_iters"], smoother=solver_kw["smoother"])
+    t64 = time.perf_counter() - t
+    xr = o64.get(po.CH_X)
+    m = (typ == po.LEAF)[:, None] & (ct == po.INTERIOR)
+    return {"scene": "vortex-tube octree 256x256x512 effective (leaf level 5, level 3 elsewhere), "
+                     f"{int((typ == po.LEAF).sum())} leaf tiles",
+            "gpu_fp64_rel_res": r["rel_res"], "refinement_steps": r["outer"], "iterations": r["iters"],
+            "oracle_fp64_rel_res": r64["rel_res"], "oracle_seconds": t64, "oracle_threads": po.get_threads(),
+            "rel_l2_error_vs_fp64_oracle": float(np.linalg.norm((x - xr)[m]) / np.linalg.norm(xr[m]))}
+
+
+def gs_alg_bytes(tiles, s=4):
     """Algorithmic bytes of one red-black colour sweep: x and rhs read, x written
-    for half the cells, the cell type read (2s + t per cell, s = 4, t = 1)."""
-    return tiles * 512 * (2 * 4 + 1)
+    for half the cells, the cell type read (2s + t per cell; s = 4 for FP32,
+    2 for BF16 storage; t = 1)."""
+    return tiles * 512 * (2 * s + 1)
 
 
 def kernels_only(P, ca):
@@ -392,6 +510,8 @@ def main():
                     help="length of the Poisson request window of the served form (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-instances", type=int, default=64)
+    ap.add_argument("--no-vortex", action="store_true",
+                    help="skip the mixed-precision refinement form on the vortex-tube octree")
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only the dominant kernel and the copy reference (for counter runs)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
@@ -489,16 +609,31 @@ def main():
                      "avg_launch_us": us, "alg_bytes_per_launch": alg_bytes},
         "kernels_us": per_kernel,
     }
+    forms = {}
     if args.batch > 0:
-        bf = run_batched(ca, args.batch, args.batch_reps, solver_kw, use_graph=not args.no_graph)
-        bf["vs_plain_solves_per_s"] = bf["solves_per_s"] / out["value"]
-        out["forms"] = {"batched": bf}
+        for name, storage, refine in [("batched", 0, False), ("batched_mixed", 1, True)]:
+            f = run_batched(ca, args.batch, args.batch_reps, solver_kw, use_graph=not args.no_graph, storage=storage,
+                            refine=refine)
+            f["vs_plain_solves_per_s"] = f["solves_per_s"] / out["value"]
+            forms[name] = f
     if args.serve_seconds > 0:
-        sf = run_served(ca, solver_kw, duration=args.serve_seconds, lam=args.serve_rate, conc=args.serve_instances)
-        sf["vs_plain_solves_per_s"] = sf["solves_per_s"] / out["value"]
-        out.setdefault("forms", {})["served"] = sf
+        for name, storage, verify in [("served", 0, False), ("served_mixed", 1, True)]:
+            f = run_served(ca, solver_kw, duration=args.serve_seconds, lam=args.serve_rate, conc=args.serve_instances,
+                           storage=storage, verify=verify)
+            f["vs_plain_solves_per_s"] = f["solves_per_s"] / out["value"]
+            forms[name] = f
+    if not args.no_vortex:
+        f = run_vortex(ca, solver_kw)
+        f.pop("_x64_leaf")
+        forms["refined_vortex512"] = f
+    if forms:
+        out["forms"] = forms
+        out["mixed_vs_plain"] = {k: forms[k]["vs_plain_solves_per_s"] for k in ("batched_mixed", "served_mixed")
+                                 if k in forms}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene_kw, solver_kw)
+        if not args.no_vortex:
+            forms["refined_vortex512"]["fp64_oracle_check"] = fp64_oracle_check(ca, solver_kw)
     print(json.dumps(out), flush=True)
 
 

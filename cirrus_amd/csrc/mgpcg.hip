@@ -1217,15 +1217,19 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
   return guard([&] {
     if (!p || !avg_us || reps <= 0) fail(CG_ERR_INVALID, "cg_benchmark_kernel: bad argument");
     cg_problem& P = *p;
-    if ((kernel == CG_KERNEL_GS_SWEEP || kernel == CG_KERNEL_JACOBI) && (level < 0 || level > P.max_level))
+    if ((kernel == CG_KERNEL_GS_SWEEP || kernel == CG_KERNEL_JACOBI || kernel == CG_KERNEL_GS_SWEEP_BF16) &&
+        (level < 0 || level > P.max_level))
       fail(CG_ERR_INVALID, "level out of range");
-    if ((kernel == CG_KERNEL_RESTRICT || kernel == CG_KERNEL_PROLONG || kernel == CG_KERNEL_GS_PROLONG) &&
+    if ((kernel == CG_KERNEL_RESTRICT || kernel == CG_KERNEL_PROLONG || kernel == CG_KERNEL_GS_PROLONG ||
+         kernel == CG_KERNEL_GS_PROLONG_BF16) &&
         (level < 1 || level > P.max_level))
       fail(CG_ERR_INVALID, "level out of range");
+    if (kernel == CG_KERNEL_GS_SWEEP_BF16 || kernel == CG_KERNEL_GS_PROLONG_BF16) ensure_bf16(P);
     if ((kernel == CG_KERNEL_COARSE || kernel == CG_KERNEL_EMPTY) && level < 0) fail(CG_ERR_INVALID, "negative count");
     set_prob_state(P, 0.0, 1 << 30, 0);
     const Ctx c = P.ctx();
     const int base = F_VCYCLE | kSwizzle | F_CHECK_DONE;
+    const bf16* nob = nullptr;
     int64_t nt = 0;
     auto launch = [&]() {
       switch (kernel) {
@@ -1233,6 +1237,19 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
           nt = P.act[level].n;
           launch_gs(P, P.act[level], c, level, c.z, c.r, 0, base | F_GHOST_ZERO);
           break;
+        case CG_KERNEL_GS_SWEEP_BF16: {
+          const DList& a = P.act[level];
+          nt = a.n;
+          if (a.n) k_gs_color<bf16><<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, level, c.zh, nob, 0,
+                                                                              base | F_GHOST_ZERO);
+          break;
+        }
+        case CG_KERNEL_GS_PROLONG_BF16: {
+          const DList& al = P.lvl[level];
+          nt = al.n;
+          k_gs_prolong<bf16><<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, c.zbh, c.zh, 1.f, 1, base);
+          break;
+        }
         case CG_KERNEL_JACOBI: {
           const DList& a = P.act[level];
           nt = a.n;
