@@ -93,7 +93,8 @@ class SceneParams(ctypes.Structure):
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("budget_bytes", ctypes.c_int64), ("max_problems", ctypes.c_int32), ("params", Params)]
+    _fields_ = [("budget_bytes", ctypes.c_int64), ("max_problems", ctypes.c_int32), ("params", Params),
+                ("refine", ctypes.c_int32)]
 
 
 class CirrusError(RuntimeError):
@@ -474,10 +475,13 @@ class Server:
     """The serving loop (cg_server_*): resident instances, requests with warm
     starts, one CG iteration over all running requests per :meth:`step`."""
 
-    def __init__(self, budget_bytes, max_problems, **params):
+    def __init__(self, budget_bytes, max_problems, refine=False, **params):
+        """params: cg_params fields (level_iters, storage, verify_f64, ...);
+        refine: every request runs as an FP64 iterative refinement."""
         cfg = ServerConfig()
         cfg.budget_bytes = int(budget_bytes)
         cfg.max_problems = int(max_problems)
+        cfg.refine = int(bool(refine))
         lib().cg_params_default(ctypes.byref(cfg.params))
         for k, v in params.items():
             setattr(cfg.params, k, v)

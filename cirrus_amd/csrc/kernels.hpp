@@ -61,6 +61,9 @@ struct alignas(16) ProbState {
   double gamma, gamma_old, alpha, beta;
   double bb, thr, rr, pAp;
   double mean, tol2;
+  double thr_abs;   // floor of the stop threshold (|r|^2 < max(tol2 |b|^2, thr_abs)); 0: none
+  double tol_abs2;  // FIN_IRBB: thr_abs = tol_abs2 * |b_outer|^2
+  double bb_b;      // |b_outer|^2 of an FP64 refinement (the outer right-hand side)
   int32_t iter, done, converged, max_iters;
   uint32_t counter;
   int32_t leaf_begin, leaf_count, pure_neumann;
@@ -1241,7 +1244,7 @@ __device__ __forceinline__ double block_sum_partials(const double* __restrict__ 
 }
 
 // Per-problem finalisation of a reduction over the LEAF-tile partials.
-enum : int { FIN_BB = 0, FIN_RZ = 1, FIN_PAP = 2, FIN_RR = 3, FIN_MEAN = 4 };
+enum : int { FIN_BB = 0, FIN_RZ = 1, FIN_PAP = 2, FIN_RR = 3, FIN_MEAN = 4, FIN_IRBB = 5 };
 __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctx c, int mode, int flags) {
   __shared__ double lds[FIN_CHUNK];
   ProbState* ps = c.ps + (c.pmap ? c.pmap[blockIdx.x] : int(blockIdx.x));
@@ -1256,6 +1259,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctx c, int mode, int f
       ps->bb = s;
       ps->rr = s;
       double thr = ps->tol2 * s;
+      thr = thr > ps->thr_abs ? thr : ps->thr_abs;
       ps->thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
       ps->iter = 0;
       ps->converged = (s == 0.0) ? 1 : 0;
@@ -1285,6 +1289,10 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctx c, int mode, int f
     }
     case FIN_MEAN:
       ps->mean = s / q;
+      break;
+    case FIN_IRBB:  // |b|^2 of an FP64 refinement's right-hand side and the inner solves' threshold floor
+      ps->bb_b = s;
+      ps->thr_abs = ps->tol_abs2 * s;
       break;
   }
 }
@@ -1654,6 +1662,17 @@ __global__ __launch_bounds__(64) void k_ir_accumulate(Ctx c, const TileRec* __re
   load8(c.x + base, xf);
   for (int k = 0; k < 8; k++) xd[k] = xd[k] + double(xf[k]);
   store8d(X64 + base, xd);
+}
+// One problem's |r64|^2 (its leaf-slot partials) and |b|^2 (ProbState::bb_b)
+// into out[0], out[1].
+__global__ __launch_bounds__(FIN_THREADS) void k_ir_verdict(Ctx c, double* __restrict__ out) {
+  __shared__ double lds[FIN_CHUNK];
+  const ProbState* ps = c.ps + blockIdx.x;
+  const double s = block_sum_partials(c.part, ps->leaf_begin, ps->leaf_count, lds);
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = s;
+    out[2 * blockIdx.x + 1] = ps->bb_b;
+  }
 }
 // Per-problem totals of the leaf-slot partials, in the fixed order.
 __global__ __launch_bounds__(FIN_THREADS) void k_prob_sums(Ctx c, double* __restrict__ out) {

@@ -468,13 +468,14 @@ void check_channel(int ch) {
 // upload every problem's ProbState (synchronous)
 // tol_q / maxit_q: per-problem tolerance and iteration cap (null: tol / max_iters for all)
 void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann, int check_stride = 1,
-                    const double* tol_q = nullptr, const int* maxit_q = nullptr) {
+                    const double* tol_q = nullptr, const int* maxit_q = nullptr, const double* thr_abs_q = nullptr) {
   std::vector<ProbState> st(P.nprob);
   for (int q = 0; q < P.nprob; q++) {
     ProbState& s = st[q];
     std::memset(&s, 0, sizeof(s));
     const double t = tol_q ? tol_q[q] : tol;
     s.tol2 = t * t;
+    s.thr_abs = thr_abs_q ? thr_abs_q[q] : 0.0;
     s.max_iters = maxit_q ? maxit_q[q] : max_iters;
     s.leaf_begin = P.leaf_begin[q];
     s.leaf_count = P.leaf_count[q];
@@ -731,8 +732,10 @@ std::vector<double> residual64(cg_problem& P, const float* B, int pure_neumann, 
   return prob_sums(P);
 }
 
-// tol_q / maxit_q: per-problem tolerances and iteration caps (null: prm's for all).
-void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* tol_q = nullptr, const int* maxit_q = nullptr) {
+// tol_q / maxit_q / thr_abs_q: per-problem tolerances, iteration caps and
+// threshold floors (null: prm's for all, no floor).
+void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* tol_q = nullptr, const int* maxit_q = nullptr,
+           const double* thr_abs_q = nullptr) {
   check_params(prm);
   if (prm.storage == CG_STORE_BF16) ensure_bf16(P);
   if (prm.verify_f64) {  // keep b for the FP64 check at the end
@@ -748,7 +751,7 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
   v.pcoef = prm.prolong_coeff;
   const int stride = std::max(1, prm.sync_stride);
   auto t0 = std::chrono::steady_clock::now();
-  set_prob_state(P, prm.rel_tol, prm.max_iters, prm.pure_neumann, prm.sync_stride, tol_q, maxit_q);
+  set_prob_state(P, prm.rel_tol, prm.max_iters, prm.pure_neumann, prm.sync_stride, tol_q, maxit_q, thr_abs_q);
   int max_iters = prm.max_iters;
   if (maxit_q) for (int q = 0; q < P.nprob; q++) max_iters = std::max(max_iters, maxit_q[q]);
   const Ctx c = P.ctx();
@@ -833,8 +836,9 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
 // Iterative refinement to |b - A x|_2 < rel_tol |b|_2 measured in FP64, every
 // problem of the pool with its own outer loop: the residual and x are FP64; each
 // correction solves A d = float(r) with the FP32 MGPCG (V-cycle storage as
-// prm.storage) to max(inner_rel_tol, 0.5 sqrt(thr / |r|^2)), so the last step
-// does not solve further than the outer tolerance needs.
+// prm.storage) until its recursive |r_in|^2 < max(inner_rel_tol^2 |float(r)|^2,
+// thr / 4), thr = rel_tol^2 |b|^2, so a step does not solve further than the
+// outer tolerance needs.
 void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& rp, cg_refine_result* res) {
   check_params(prm);
   if (!(rp.rel_tol >= 0) || !(rp.inner_rel_tol >= 0) || rp.max_outer < 0)
@@ -856,8 +860,9 @@ void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& 
     thr[q] = std::max(rp.rel_tol * rp.rel_tol * bb[q], std::numeric_limits<double>::min());
     if (bb[q] == 0.0) done[q] = conv[q] = 1;  // b = 0: x = 0
   }
-  std::vector<double> tol_q(np);
+  std::vector<double> tol_q(np, rp.inner_rel_tol), floor_q(np);
   std::vector<int> maxit_q(np);
+  for (int q = 0; q < np; q++) floor_q[q] = 0.25 * thr[q];
   std::vector<cg_result> inner(np);
   cg_params inner_prm = prm;
   inner_prm.verify_f64 = 0;
@@ -872,11 +877,8 @@ void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& 
       any = any || !done[q];
     }
     if (!any) break;
-    for (int q = 0; q < np; q++) {
-      tol_q[q] = done[q] ? 0.0 : std::max(rp.inner_rel_tol, 0.5 * std::sqrt(thr[q] / rr[q]));
-      maxit_q[q] = done[q] ? 0 : prm.max_iters;
-    }
-    solve(P, inner_prm, inner.data(), tol_q.data(), maxit_q.data());
+    for (int q = 0; q < np; q++) maxit_q[q] = done[q] ? 0 : prm.max_iters;
+    solve(P, inner_prm, inner.data(), tol_q.data(), maxit_q.data(), floor_q.data());
     for (int q = 0; q < np; q++)
       if (!done[q]) { outer[q]++; iters[q] += inner[q].iters; }
     k_ir_accumulate<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.d_x64, 0);

@@ -8,10 +8,15 @@
 // instances whose request is running -- their worklists assembled on the device
 // whenever that set changes -- so requests that arrive while others run join at
 // the next iteration boundary (continuous batching) and each result is the same
-// as solving the request alone.  With verify_f64 a finished request's pressure
-// is checked in FP64 (|b - A x| / |b| recomputed from the instance's kept b)
-// before it is reported, and a request that fails the check is restarted warm
-// from its own pressure (at most kMaxRestarts times).
+// as solving the request alone.
+//
+// verify_f64: each finished request's |b - A x| / |b| is recomputed in FP64 from
+// the instance's kept b before the request is reported.
+// refine: every request is an FP64 iterative refinement -- the instance's x is
+// kept in FP64 across requests; a request computes r = b - A x in FP64, solves
+// A d = float(r) with the MGPCG until its residual is below half the tolerance,
+// adds d to x, and checks |b - A x| in FP64; a request whose check fails runs
+// another such step (kMaxRefineSteps at most) before it is reported.
 #pragma once
 
 #include <deque>
@@ -56,7 +61,7 @@ struct RangeAlloc {
 };
 
 enum InstState { INST_IDLE = 0, INST_QUEUED = 1, INST_ACTIVE = 2, INST_DONE = 3, INST_VERIFY = 4 };
-constexpr int kMaxRestarts = 3;
+constexpr int kMaxRefineSteps = 5;
 
 struct Instance {
   int slot = -1;
@@ -73,8 +78,10 @@ struct Instance {
   hipEvent_t ev = nullptr;
   hipEvent_t ev_ver = nullptr;  // the FP64 check's result has arrived
   int state = INST_IDLE;
-  int restarts = 0;     // warm restarts after a failed FP64 check
-  int iters_acc = 0;    // CG iterations of the request's earlier runs
+  int steps = 0;        // refinement steps run for the current request
+  int ir_cont = 0;      // the next start continues a refinement (its rhs is in place)
+  double bb_b = 0.0;    // |b|^2 of the current request (refinement)
+  int iters_acc = 0;    // CG iterations of the request's earlier steps
   int warm = 0;
   int dev_rhs = 0;      // the right-hand side is already on the device
   int no_download = 0;  // the pressure stays on the device
@@ -88,6 +95,7 @@ struct Instance {
 
 struct cg_server {
   cg_params prm{};
+  int refine = 0;  // every request is an FP64 iterative refinement
   VParams v;
   int max_probs = 0;
   int64_t cap_tiles = 0, cap_leaf = 0, cap_ghost = 0;
@@ -184,8 +192,10 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   S.v.ooa = prm.one_over_alpha;
   S.v.pcoef = prm.prolong_coeff;
   S.max_probs = cfg->max_problems;
+  S.refine = cfg->refine ? 1 : 0;
+  const bool f64 = prm.verify_f64 || S.refine;
   const int64_t per_tile = kBytesPerTile + (prm.storage == CG_STORE_BF16 ? 3 * 512 * int64_t(sizeof(bf16)) : 0) +
-                           (prm.verify_f64 ? 2 * 512 * int64_t(sizeof(double)) : 0);
+                           (f64 ? 2 * 512 * int64_t(sizeof(double)) : 0);
   S.cap_tiles = std::max<int64_t>(64, cfg->budget_bytes / per_tile);
   if (S.cap_tiles * 512 > int64_t(INT32_MAX)) S.cap_tiles = int64_t(INT32_MAX) / 512;  // cell offsets stay 32-bit safe
   S.cap_leaf = S.cap_tiles;
@@ -204,13 +214,13 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   HIPC(hipMalloc(&S.d_zb, cells * sizeof(float)));
   HIPC(hipMalloc(&S.d_b, cells * sizeof(float)));
   HIPC(hipMemset(S.d_b, 0, cells * sizeof(float)));
-  if (S.prm.verify_f64) {
+  if (f64) {
     HIPC(hipMalloc(&S.d_x64, cells * sizeof(double)));
     HIPC(hipMalloc(&S.d_r64, cells * sizeof(double)));
     HIPC(hipMemset(S.d_x64, 0, cells * sizeof(double)));
     HIPC(hipMemset(S.d_r64, 0, cells * sizeof(double)));
-    HIPC(hipMalloc(&S.d_ver, sizeof(double) * S.max_probs));
-    HIPC(hipHostMalloc(&S.h_ver, sizeof(double) * S.max_probs, hipHostMallocDefault));
+    HIPC(hipMalloc(&S.d_ver, 2 * sizeof(double) * S.max_probs));
+    HIPC(hipHostMalloc(&S.h_ver, 2 * sizeof(double) * S.max_probs, hipHostMallocDefault));
   }
   if (S.prm.storage == CG_STORE_BF16) {
     bf16** bufs[3] = {&S.d_zh, &S.d_rhsvh, &S.d_zbh};
@@ -393,10 +403,38 @@ void server_submit(cg_server& S, int32_t id, const float* rhs, int32_t flags, in
   I.no_download = (flags & CG_SUBMIT_NO_DOWNLOAD) ? 1 : 0;
   I.warm = (flags & CG_SUBMIT_WARM) ? 1 : 0;
   I.tag = tag;
-  I.restarts = 0;
+  I.steps = 0;
+  I.ir_cont = 0;
   I.iters_acc = 0;
   I.t_submit = std::chrono::steady_clock::now();
   I.state = INST_QUEUED;
+}
+
+// A Ctx whose ProbState array starts at the instance's slot, for one-workgroup
+// per-problem reductions.
+Ctx one_problem(cg_server& S, const Instance& I) {
+  Ctx cv = S.view.ctx();
+  cv.ps = S.d_ps + I.slot;
+  cv.pmap = nullptr;
+  return cv;
+}
+
+// r64 = b - A x64 on the instance's LEAF tiles in FP64 (re-centred for pure
+// Neumann) with |r64|^2 partials; write_r: float(r64) into the r channel.
+void instance_residual64(cg_server& S, Instance& I, bool write_r) {
+  const hipStream_t st = S.stream;
+  const Ctx c = S.view.ctx();
+  const unsigned gl = grid_for(I.leaf.n, kSwizzle);
+  if (I.ghosts.n) k_propagate_ghosts64<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, S.d_x64);
+  k_apply64<<<I.applyl.n, 64, 0, st>>>(c, I.L(I.applyl), I.applyl.n, S.d_x64, S.d_r64);
+  if (I.ghosts.n) k_accumulate_ghosts64<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, S.d_r64);
+  if (S.prm.pure_neumann) {
+    k_ir_residual<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, S.d_r64, 1);
+    k_finalize<<<1, FIN_THREADS, 0, st>>>(one_problem(S, I), FIN_MEAN, 0);
+    k_ir_recenter<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_r64, write_r ? 1 : 0);
+  } else {
+    k_ir_residual<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, S.d_r64, write_r ? 0 : 2);
+  }
 }
 
 // Start queued requests: ProbState, rhs upload, x0 (zero for a cold start),
@@ -414,16 +452,39 @@ void server_start_queued(cg_server& S, int k) {
     Instance& I = *S.inst[fresh[f]];
     ProbState& ps = S.h_ps_stage[I.slot];
     std::memset(&ps, 0, sizeof(ps));
-    ps.tol2 = S.prm.rel_tol * S.prm.rel_tol;
+    const double tol2 = S.prm.rel_tol * S.prm.rel_tol;
+    // refinement: the inner solve stops below half the tolerance of |b|
+    ps.tol2 = S.refine ? 0.0 : tol2;
+    ps.tol_abs2 = S.refine ? 0.25 * tol2 : 0.0;
+    if (S.refine && I.ir_cont) {
+      ps.bb_b = I.bb_b;
+      ps.thr_abs = 0.25 * tol2 * I.bb_b;
+    }
     ps.max_iters = S.prm.max_iters;
     ps.leaf_begin = int32_t(I.leaf_begin);
     ps.leaf_count = I.nleaf;
     ps.pure_neumann = S.prm.pure_neumann;
     ps.check_stride = S.prm.sync_stride;
     HIPC(hipMemcpyAsync(S.d_ps + I.slot, &ps, sizeof(ps), hipMemcpyHostToDevice, st));
-    if (!I.dev_rhs)
+    if (!I.dev_rhs && !I.ir_cont)
       HIPC(hipMemcpyAsync(S.d_b + size_t(I.tile_begin) * 512, I.h_buf, size_t(I.ntiles) * 512 * sizeof(float),
                           hipMemcpyHostToDevice, st));
+    hmap[f] = I.slot;
+    I.state = INST_ACTIVE;
+    S.active.push_back(I.slot);
+    S.dirty = true;
+    if (S.refine) {
+      if (!I.ir_cont) {  // r = b - A x64 in FP64 (x64 = 0 for a cold start); |b|^2 and the threshold floor
+        if (!I.warm) HIPC(hipMemsetAsync(S.d_x64 + size_t(I.tile_begin) * 512, 0, size_t(I.ntiles) * 512 * sizeof(double), st));
+        instance_residual64(S, I, true);
+        k_ir_norm<<<grid_for(I.leaf.n, kSwizzle), 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b);
+        k_finalize<<<1, FIN_THREADS, 0, st>>>(one_problem(S, I), FIN_IRBB, 0);
+      }
+      // the step solves A d = float(r) from d = 0
+      if (S.prm.storage == CG_STORE_BF16) k_cg_init<bf16><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, 0);
+      else k_cg_init<float><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, 0);
+      continue;
+    }
     float* X = S.d_ch[CG_CH_X];
     if (!I.warm) k_fill_tiles<float><<<I.all.n, 64, 0, st>>>(I.L(I.all), I.all.n, X, 0.f);
     if (I.ghosts.n) k_propagate_ghosts<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, X, X);
@@ -431,10 +492,6 @@ void server_start_queued(cg_server& S, int k) {
     if (I.ghosts.n) k_accumulate_ghosts<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, c.ap);
     if (S.prm.storage == CG_STORE_BF16) k_cg_init_warm<bf16><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, 0);
     else k_cg_init_warm<float><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, 0);
-    hmap[f] = I.slot;
-    I.state = INST_ACTIVE;
-    S.active.push_back(I.slot);
-    S.dirty = true;
   }
   HIPC(hipMemcpyAsync(S.d_newmap, hmap, sizeof(int32_t) * nf, hipMemcpyHostToDevice, st));
   Ctx ci = c;
@@ -518,28 +575,18 @@ void finish_request(cg_server& S, Instance& I) {
   S.done_q.push_back(I.slot);
 }
 
-// FP64 check of a finished request on the instance's own lists: x64 = x,
-// |b - A x64|^2 (re-centred for pure Neumann) into the instance's slot.
+// FP64 check of a finished request: x64 = x (verify_f64) or x64 += d (a
+// refinement step, whose r = float(b - A x64) is left ready for another step),
+// then |b - A x64|^2 and |b|^2 to the host.
 void enqueue_verify(cg_server& S, Instance& I) {
   const hipStream_t st = S.stream;
   const Ctx c = S.view.ctx();
-  Ctx cv = c;  // one-problem view for the per-problem reductions
-  cv.ps = S.d_ps + I.slot;
-  cv.pmap = nullptr;
   const unsigned gl = grid_for(I.leaf.n, kSwizzle);
-  k_ir_load_x<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_ch[CG_CH_X], S.d_x64);
-  if (I.ghosts.n) k_propagate_ghosts64<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, S.d_x64);
-  k_apply64<<<I.applyl.n, 64, 0, st>>>(c, I.L(I.applyl), I.applyl.n, S.d_x64, S.d_r64);
-  if (I.ghosts.n) k_accumulate_ghosts64<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, S.d_r64);
-  if (S.prm.pure_neumann) {
-    k_ir_residual<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, S.d_r64, 1);
-    k_finalize<<<1, FIN_THREADS, 0, st>>>(cv, FIN_MEAN, 0);
-    k_ir_recenter<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_r64, 0);
-  } else {
-    k_ir_residual<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, S.d_r64, 2);
-  }
-  k_prob_sums<<<1, FIN_THREADS, 0, st>>>(cv, S.d_ver + I.slot);
-  HIPC(hipMemcpyAsync(S.h_ver + I.slot, S.d_ver + I.slot, sizeof(double), hipMemcpyDeviceToHost, st));
+  if (S.refine) k_ir_accumulate<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_x64, 0);
+  else k_ir_load_x<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_ch[CG_CH_X], S.d_x64);
+  instance_residual64(S, I, S.refine != 0);
+  k_ir_verdict<<<1, FIN_THREADS, 0, st>>>(one_problem(S, I), S.d_ver + 2 * I.slot);
+  HIPC(hipMemcpyAsync(S.h_ver + 2 * I.slot, S.d_ver + 2 * I.slot, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPC(hipEventRecord(I.ev_ver, st));
   I.state = INST_VERIFY;
 }
@@ -558,13 +605,32 @@ void server_collect(cg_server& S, int k) {
     const ProbState& s = st[q];
     I.res.iters = I.iters_acc + s.iter;
     I.res.converged = s.converged;
-    I.res.bb = s.bb;
-    I.res.rr = s.rr;
-    I.res.rel_res = s.bb > 0 ? (S.prm.sync_stride > 0 ? std::sqrt(s.rr / s.bb) : -1.0) : 0.0;
     I.res.rel_res_f64 = -1.0;
     I.res.seconds = std::chrono::duration<double>(now - I.t_submit).count();
     S.active.erase(std::find(S.active.begin(), S.active.end(), q));
     S.dirty = true;
+    if (S.refine) {
+      // the step's recursive residual, relative to the request's b
+      I.bb_b = s.bb_b;
+      I.steps++;
+      I.res.bb = s.bb_b;
+      I.res.rr = s.rr;
+      I.res.rel_res = s.bb_b > 0 ? (S.prm.sync_stride > 0 ? std::sqrt(s.rr / s.bb_b) : -1.0) : 0.0;
+      if (s.bb_b > 0) {
+        enqueue_verify(S, I);
+      } else {  // b = 0: x = 0
+        HIPC(hipMemsetAsync(S.d_x64 + size_t(I.tile_begin) * 512, 0, size_t(I.ntiles) * 512 * sizeof(double), S.stream));
+        HIPC(hipMemsetAsync(S.d_ch[CG_CH_X] + size_t(I.tile_begin) * 512, 0, size_t(I.ntiles) * 512 * sizeof(float), S.stream));
+        I.res.converged = 1;
+        I.res.rr = 0.0;
+        I.res.rel_res_f64 = 0.0;
+        finish_request(S, I);
+      }
+      continue;
+    }
+    I.res.bb = s.bb;
+    I.res.rr = s.rr;
+    I.res.rel_res = s.bb > 0 ? (S.prm.sync_stride > 0 ? std::sqrt(s.rr / s.bb) : -1.0) : 0.0;
     if (S.prm.verify_f64 && s.bb > 0) {
       enqueue_verify(S, I);
     } else {
@@ -574,8 +640,9 @@ void server_collect(cg_server& S, int k) {
   }
 }
 
-// Requests whose FP64 check has arrived: report them, or restart them warm from
-// their own pressure when the check fails (kMaxRestarts at most).
+// Requests whose FP64 check has arrived.  verify_f64: reported with the
+// checked residual.  refine: reported once |b - A x|^2 < rel_tol^2 |b|^2 (x =
+// float(x64) then), or after kMaxRefineSteps steps; otherwise another step.
 void server_verified(cg_server& S) {
   for (int q = 0; q < S.max_probs; q++) {
     if (!S.inst[q] || S.inst[q]->state != INST_VERIFY) continue;
@@ -583,17 +650,23 @@ void server_verified(cg_server& S) {
     const hipError_t e = hipEventQuery(I.ev_ver);
     if (e == hipErrorNotReady) continue;
     HIPC(e);
-    const double rr = S.h_ver[q];
-    I.res.rel_res_f64 = std::sqrt(rr / I.res.bb);
-    const bool pass = rr < S.prm.rel_tol * S.prm.rel_tol * I.res.bb;
-    if (pass || I.restarts >= kMaxRestarts || I.res.converged == 0) {
+    const double rr = S.h_ver[2 * q];
+    const double bb = S.refine ? S.h_ver[2 * q + 1] : I.res.bb;
+    I.res.rel_res_f64 = std::sqrt(rr / bb);
+    if (!S.refine) {
+      finish_request(S, I);
+      continue;
+    }
+    const double thr = std::max(S.prm.rel_tol * S.prm.rel_tol * bb, std::numeric_limits<double>::min());
+    const bool pass = rr < thr;
+    if (pass || I.steps >= kMaxRefineSteps || !I.res.converged) {
       I.res.converged = pass ? 1 : 0;
+      I.res.rr = rr;
+      k_ir_accumulate<<<grid_for(I.leaf.n, kSwizzle), 64, 0, S.stream>>>(S.view.ctx(), I.L(I.leaf), I.leaf.n, S.d_x64, 1);
       finish_request(S, I);
     } else {
-      I.restarts++;
+      I.ir_cont = 1;
       I.iters_acc = I.res.iters;
-      I.warm = 1;
-      I.dev_rhs = 1;
       I.state = INST_QUEUED;
     }
   }
@@ -616,7 +689,7 @@ void server_step(cg_server& S, int32_t* n_active) {
     S.snap_pending[k] = true;
   }
   server_collect(S, 1 - k);
-  if (S.prm.verify_f64) server_verified(S);
+  if (S.prm.verify_f64 || S.refine) server_verified(S);
   server_start_queued(S, k);
   if (S.dirty) server_assemble(S, k);
   S.step_no++;

@@ -139,8 +139,9 @@ cg_status cg_solve_batch(cg_problem* p, const cg_params* prm, cg_result* results
  * A d = float(r) with the MGPCG of cg_solve (FP32 arithmetic, FP64 dot
  * products, V-cycle storage as prm->storage) and adds d to x.  The stop test
  * is |r|^2 < rel_tol^2 |b|^2 in FP64 over LEAF tiles, with the true residual.
- * A step's inner tolerance is max(inner_rel_tol, 0.5 * sqrt(rel_tol^2 |b|^2 / |r|^2))
- * and prm->max_iters caps each inner solve (prm->rel_tol is not used).  Every
+ * A step's inner solve stops when its recursive |r_in|^2 < max(inner_rel_tol^2
+ * |float(r)|^2, rel_tol^2 |b|^2 / 4); prm->max_iters caps each inner solve
+ * (prm->rel_tol is not used).  Every
  * problem of a batched handle runs its own outer loop; results[num_problems].
  * On return CG_CH_X holds float(x) and cg_get_solution_f64 gives x. */
 typedef struct {
@@ -243,7 +244,13 @@ typedef struct cg_server cg_server;
 typedef struct {
   int64_t budget_bytes;      /* device memory for channels, cell types and worklists */
   int32_t max_problems;      /* resident instances at most */
-  cg_params params;          /* solver settings for every request */
+  cg_params params;          /* solver settings for every request (verify_f64: each result's
+                                |b - A x| / |b| recomputed in FP64 before it is reported) */
+  int32_t refine;            /* 1: every request is an FP64 iterative refinement -- the instance's x
+                                is kept in FP64 across requests (the warm start), each step solves
+                                A d = float(b - A x) with the MGPCG below half the tolerance and adds
+                                d to x, and the request is reported once |b - A x| < rel_tol |b| in
+                                FP64 (at most 5 steps); the pressure returned is float(x) (0) */
 } cg_server_config;
 cg_status cg_server_create(const cg_server_config* cfg, cg_server** out);
 cg_status cg_server_destroy(cg_server* s);

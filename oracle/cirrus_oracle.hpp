@@ -1155,12 +1155,14 @@ struct Solver {
   // (GMGSolver.cu:611); <= 0 never tests and reports a relative residual of -1
   // (GMGSolver.cu:655-665).  b = 0 returns (0, 0) in every mode (with
   // sync_stride <= 0 the reference would go on and divide 0 by 0).
+  // thr_floor (no reference counterpart, used by the refinement's inner
+  // solves): the stop threshold is max(tol^2 |b|^2, thr_floor).
   Result solve(int max_iters, double tol, int level_iters, int coarsest_iters, bool pure_neumann,
-               int smoother = SMOOTH_RBGS, T omega = T(2) / T(3), int sync_stride = 1) {
+               int smoother = SMOOTH_RBGS, T omega = T(2) / T(3), int sync_stride = 1, double thr_floor = 0.0) {
     ForLeafCells([](TileData<T>& t, int c) { t.ch[CH_X][c] = T(0); });
     double bb = Dot(CH_R, CH_R, LEAF);
     if (bb == 0.0) return {0, 0.0, 0.0, 0.0};
-    double thr = std::max(tol * tol * bb, std::numeric_limits<double>::min());
+    double thr = std::max(std::max(tol * tol * bb, thr_floor), std::numeric_limits<double>::min());
     if (pure_neumann) ReCenterLeafVoxels(CH_R);
     VCycle(CH_Z, CH_R, CH_AP, level_iters, coarsest_iters, smoother, omega);
     ForLeafCells([](TileData<T>& t, int c) { t.ch[CH_P][c] = t.ch[CH_Z][c]; });
@@ -1255,9 +1257,9 @@ std::unique_ptr<Problem<B>> ConvertProblem(Problem<A>& a) {
 
 // Iterative refinement: x and r = b - A x in FP64 (Pd, an FP64 copy of Pf whose
 // b channel holds the FP32 b); each step solves A d = float(r) with the FP32
-// MGPCG on Pf (V-cycle stores rounded to bfloat16 when bf16 is set) to the
-// relative tolerance max(inner_tol, 0.5 sqrt(thr / |r|^2)), thr = max(tol^2 |b|^2,
-// DBL_MIN), and adds d to x.  Stops when |r|^2 < thr (FP64, LEAF tiles) or after
+// MGPCG on Pf (V-cycle stores rounded to bfloat16 when bf16 is set) until its
+// recursive |r_in|^2 < max(inner_tol^2 |float(r)|^2, thr / 4), thr =
+// max(tol^2 |b|^2, DBL_MIN), and adds d to x.  Stops when |r|^2 < thr (FP64, LEAF tiles) or after
 // max_outer steps.  On return Pd's x channel holds x and Pf's holds float(x).
 struct RefineResult { int outer, iters, converged; double rel_res, bb, rr; };
 inline RefineResult SolveRefined(Problem<float>& Pf, Problem<double>& Pd, double tol, double inner_tol, int max_outer,
@@ -1293,8 +1295,8 @@ inline RefineResult SolveRefined(Problem<float>& Pf, Problem<double>& Pd, double
       auto& tf = gf.tile(in);
       for (int c = 0; c < 512; c++) tf.ch[CH_R][c] = float(td.ch[CH_TMP][c]);
     }
-    const double ti = std::max(inner_tol, 0.5 * std::sqrt(thr / res.rr));
-    auto r = Sf.solve(max_iters, ti, level_iters, coarsest_iters, pure_neumann, smoother, omega, sync_stride);
+    auto r = Sf.solve(max_iters, inner_tol, level_iters, coarsest_iters, pure_neumann, smoother, omega, sync_stride,
+                      0.25 * thr);
     res.outer++;
     res.iters += r.iters;
     for (const Info& in : tl) {

@@ -204,7 +204,9 @@ def test_perturbation_updates_the_kept_rhs():
 def test_verified_requests_equal_verified_solve():
     """verify_f64: every reported request carries its FP64 residual
     |b - A x| / |b|, recomputed on the device from the kept b; with BF16 V-cycle
-    storage a cold request still equals the verified solve alone."""
+    storage a cold request still equals the verified solve alone.  (The FP32
+    pressure's FP64 residual can sit just above the tolerance its recursive
+    residual met; refine mode is what guarantees it.)"""
     scs = scenes()
     S = ca.Server(BUDGET, 16, storage=ca.STORE_BF16, verify_f64=1, **KW)
     ids = [S.admit(sc.tree) for sc in scs]
@@ -218,9 +220,8 @@ def test_verified_requests_equal_verified_solve():
         G.solve(P, False, 1000, 1e-6, KW["level_iters"], KW["coarsest_iters"], 1, False, storage=ca.STORE_BF16,
                 verify_f64=True)
         r = res[q][2]
-        assert 0 <= r["rel_res_f64"] < 1e-6 and r["converged"] == 1, r
-        if G.last["rel_res_f64"] < 1e-6:  # no restart needed: the same run
-            assert (r["iters"], r["rr"], r["rel_res_f64"]) == (G.last["iters"], G.last["rr"], G.last["rel_res_f64"])
+        assert 0 <= r["rel_res_f64"] < 5e-6 and r["converged"] == 1, r
+        assert (r["iters"], r["rr"], r["rel_res_f64"]) == (G.last["iters"], G.last["rr"], G.last["rel_res_f64"])
     # warm requests in the mix are verified too
     for q, (i, sc) in enumerate(zip(ids, scs)):
         S.perturb_rhs(i, 0.95, 0.3 * float(np.sqrt(np.mean(sc.rhs[sc.rhs != 0] ** 2))), 100 + q)
@@ -228,4 +229,47 @@ def test_verified_requests_equal_verified_solve():
     res = run_until_done(S, len(scs))
     for q in range(len(scs)):
         r = res[10 + q][2]
-        assert 0 <= r["rel_res_f64"] < 1e-6 and r["converged"] == 1, r
+        assert 0 <= r["rel_res_f64"] < 5e-6 and r["converged"] == 1, r
+
+
+def test_refined_requests_equal_pool_refinement():
+    """refine: a cold request is the pool's FP64 iterative refinement with no
+    inner relative tolerance (each step to half the outer tolerance), bit for bit;
+    its reported residual is the FP64 one."""
+    scs = scenes()
+    S = ca.Server(BUDGET, 16, refine=True, storage=ca.STORE_BF16, **KW)
+    ids = [S.admit(sc.tree) for sc in scs]
+    for q, (i, sc) in enumerate(zip(ids, scs)):
+        S.submit(i, sc.rhs, tag=q)
+    res = run_until_done(S, len(scs))
+    for q, sc in enumerate(scs):
+        P = ca.Problem(sc.tree)
+        P.set(ca.CH_B, sc.rhs)
+        ref = ca.GMGSolver().solve_refined(P, 1e-6, 0.0, 5, 1000, KW["level_iters"], KW["coarsest_iters"],
+                                           storage=ca.STORE_BF16)[0]
+        tag, inst, r, x = res[q]
+        assert r["converged"] == 1 and 0 <= r["rel_res_f64"] < 1e-6, r
+        assert (r["iters"], r["rr"], r["bb"]) == (ref["iters"], ref["rr"], ref["bb"]), q
+        assert r["rel_res_f64"] == ref["rel_res"]
+        m = sc.tree.type == po.LEAF
+        assert np.array_equal(x[m], P.get(ca.CH_X)[m]), q
+
+
+def test_refined_warm_sequences_same_alone_and_in_the_mix():
+    """refine: warm starts from the FP64 pressure kept on the device; each
+    result is verified in FP64 and equal alone and in the mix."""
+    scs = scenes()
+    kw = dict(refine=True, storage=ca.STORE_BF16, **KW)
+    S1 = ca.Server(BUDGET, 16, **kw)
+    a = S1.admit(scs[1].tree)
+    seq_alone = warm_sequence(S1, a, scs[1])
+    S2 = ca.Server(BUDGET, 16, **kw)
+    ids = [S2.admit(sc.tree) for sc in scs]
+    seq_mix = warm_sequence(S2, ids[1], scs[1], others=[(ids[0], scs[0]), (ids[2], scs[2]), (ids[3], scs[3])])
+    m = scs[1].tree.type == po.LEAF
+    for (t1, _, r1, x1), (t2, _, r2, x2) in zip(seq_alone, seq_mix):
+        assert r1["converged"] == 1 and r1["rel_res_f64"] < 1e-6
+        assert (r1["iters"], r1["rr"]) == (r2["iters"], r2["rr"])
+        assert np.array_equal(x1[m], x2[m])
+    # the warm requests need fewer iterations than the cold one
+    assert seq_alone[1][2]["iters"] < seq_alone[0][2]["iters"]
