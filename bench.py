@@ -366,12 +366,31 @@ def gs_alg_bytes(tiles, s=4):
 
 
 def kernels_only(P, ca):
-    """The dominant kernel and the copy reference, KERNEL_REPS launches each
-    (the workload for hardware-counter runs)."""
+    """The dominant kernel (FP32 and BF16 storage) and the copy reference,
+    KERNEL_REPS launches each (the workload for hardware-counter runs)."""
     L = P.max_level
     us, tiles = P.benchmark_kernel(ca.KERNEL_GS_SWEEP, L, reps=KERNEL_REPS)
+    bus, _ = P.benchmark_kernel(ca.KERNEL_GS_SWEEP_BF16, L, reps=KERNEL_REPS)
     cus, ctiles = P.benchmark_kernel(ca.KERNEL_COPY, 0, reps=KERNEL_REPS)
-    print(json.dumps({"gs_level": L, "gs_tiles": tiles, "gs_us": us, "copy_tiles": ctiles, "copy_us": cus}), flush=True)
+    print(json.dumps({"gs_level": L, "gs_tiles": tiles, "gs_us": us, "gs_bf16_us": bus, "copy_tiles": ctiles,
+                      "copy_us": cus}), flush=True)
+
+
+def _kname(raw):
+    """Kernel name without return type, namespace and arguments (template
+    arguments kept: k_gs_color<float>)."""
+    import re
+    m = re.match(r"_ZN2cg(\d+)", raw)
+    if m:  # a mangled name the profiler could not demangle (bfloat16 template arguments)
+        k = m.end()
+        name = raw[k:k + int(m.group(1))]
+        rest = raw[k + int(m.group(1)):]
+        targ = {"IDF16bE": "<bf16>", "IfE": "<float>"}
+        return name + next((v for t, v in targ.items() if rest.startswith(t)), "")
+    n = raw.split("(")[0].strip()
+    if n.startswith("void "):
+        n = n[5:]
+    return n.replace("cg::", "").replace("__bf16", "bf16")
 
 
 def _pmc_per_dispatch(path, counter):
@@ -383,7 +402,7 @@ def _pmc_per_dispatch(path, counter):
         for r in csv.DictReader(f):
             if r.get("Counter_Name") != counter:
                 continue
-            name = r["Kernel_Name"].split("(")[0]
+            name = _kname(r["Kernel_Name"])
             grid = int(r.get("Grid_Size", r.get("Grid_Size_X", 0)) or 0)
             acc[(name, grid)].append(float(r["Counter_Value"]))
     return {k: sum(v) / len(v) for k, v in acc.items()}
@@ -396,13 +415,14 @@ def summarize_pmc(fetch_csv, write_csv):
     write = _pmc_per_dispatch(write_csv, "WRITE_SIZE")
 
     def pick(d, name):
-        ks = [k for k in d if k[0].endswith(name)]
+        ks = [k for k in d if k[0] == name]
         if not ks:
             raise SystemExit(f"no {name} dispatches in the counter file")
         k = max(ks, key=lambda k: k[1])
         return k, d[k]
 
-    (gk, gf), (_, gw) = pick(fetch, "k_gs_color"), pick(write, "k_gs_color")
+    (gk, gf), (_, gw) = pick(fetch, "k_gs_color<float>"), pick(write, "k_gs_color<float>")
+    (bk, bf), (_, bw) = pick(fetch, "k_gs_color<bf16>"), pick(write, "k_gs_color<bf16>")
     (ck, cf), (_, cw) = pick(fetch, "k_copy_tiles"), pick(write, "k_copy_tiles")
     copy_tiles = ck[1] // 64
     copy_bytes = copy_tiles * 512 * 4
@@ -412,6 +432,9 @@ def summarize_pmc(fetch_csv, write_csv):
     rd = gf * 1024.0 * f_scale
     wr = gw * 1024.0 * w_scale
     alg = gs_alg_bytes(gs_tiles)
+    brd = bf * 1024.0 * f_scale
+    bwr = bw * 1024.0 * w_scale
+    balg = gs_alg_bytes(gs_tiles, 2)
     lines = [
         "HBM traffic of k_gs_color (red-black colour sweep, finest level) on the adaptive 256^3 workload",
         "",
@@ -423,11 +446,16 @@ def summarize_pmc(fetch_csv, write_csv):
         f"copy reference k_copy_tiles: {copy_tiles} tiles, {copy_bytes} B read and {copy_bytes} B written per dispatch",
         f"  FETCH_SIZE {cf:.1f} KiB/dispatch -> read scale {f_scale:.4f}",
         f"  WRITE_SIZE {cw:.1f} KiB/dispatch -> write scale {w_scale:.4f}",
-        f"k_gs_color: {gs_tiles} tiles (grid {gk[1]} threads)",
+        f"k_gs_color<float>: {gs_tiles} tiles (grid {gk[1]} threads)",
         f"  FETCH_SIZE {gf:.1f} KiB/dispatch -> {rd:.0f} B read",
         f"  WRITE_SIZE {gw:.1f} KiB/dispatch -> {wr:.0f} B written",
-        f"  algorithmic bytes per launch {alg} (2s+t per cell)",
+        f"  algorithmic bytes per launch {alg} (2s+t per cell, s = 4)",
         f"  traffic / algorithmic = {(rd + wr) / alg:.3f}",
+        f"k_gs_color<bf16> (BF16 V-cycle storage): {bk[1] // 64} tiles",
+        f"  FETCH_SIZE {bf:.1f} KiB/dispatch -> {brd:.0f} B read",
+        f"  WRITE_SIZE {bw:.1f} KiB/dispatch -> {bwr:.0f} B written",
+        f"  algorithmic bytes per launch {balg} (2s+t per cell, s = 2)",
+        f"  traffic / algorithmic = {(brd + bwr) / balg:.3f}",
         "",
         f"traffic_bytes_per_launch = {rd + wr:.0f}",
         f"traffic_tiles = {gs_tiles}",
@@ -451,7 +479,7 @@ def summarize_trace(trace_csv, stats_csv):
         "",
         "command:",
         "  rocprofv3 --kernel-trace --stats -d OUT -o run --output-format csv -- "
-        "python3 bench.py --steps 5 --warmup 1 --no-cpu-baseline",
+        "python3 bench.py --steps 5 --warmup 1 --no-cpu-baseline --batch 0 --serve-seconds 0 --no-vortex",
         "  python bench.py --summarize-trace OUT/run_kernel_trace.csv OUT/run_kernel_stats.csv",
         "",
         "rocprofv3 --stats (all launches of each kernel):",
@@ -459,13 +487,13 @@ def summarize_trace(trace_csv, stats_csv):
     ]
     with open(stats_csv) as f:
         for r in csv.DictReader(f):
-            name = r["Name"].split("(")[0].replace("cg::", "")
+            name = _kname(r["Name"])
             lines.append(f"  {name:24s} {int(r['Calls']):7d} {float(r['AverageNs']) / 1e3:9.3f} "
                          f"{float(r['TotalDurationNs']) / 1e6:9.3f} {float(r['Percentage']):6.2f}")
     acc = collections.defaultdict(list)
     with open(trace_csv) as f:
         for r in csv.DictReader(f):
-            name = r["Kernel_Name"].split("(")[0].replace("cg::", "")
+            name = _kname(r["Kernel_Name"])
             wg = int(r.get("Workgroup_Size_X", r.get("Workgroup_Size", 1)) or 1)
             grid = int(r.get("Grid_Size_X", r.get("Grid_Size", 0)) or 0)
             acc[(name, grid // max(1, wg))].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
