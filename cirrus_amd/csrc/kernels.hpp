@@ -180,6 +180,15 @@ __device__ __forceinline__ void store4(bf16* __restrict__ p, float4 a) {
   *reinterpret_cast<uint2*>(p) = make_uint2(bf_pack(a.x, a.y), bf_pack(a.z, a.w));
 }
 
+// two consecutive values
+__device__ __forceinline__ float2 load2(const float* __restrict__ p) { return *reinterpret_cast<const float2*>(p); }
+__device__ __forceinline__ float2 load2(const bf16* __restrict__ p) {
+  const uint32_t q = *reinterpret_cast<const uint32_t*>(p);
+  return make_float2(bf_lo(q), bf_hi(q));
+}
+__device__ __forceinline__ void store2(float* __restrict__ p, float2 a) { *reinterpret_cast<float2*>(p) = a; }
+__device__ __forceinline__ void store2(bf16* __restrict__ p, float2 a) { *reinterpret_cast<uint32_t*>(p) = bf_pack(a.x, a.y); }
+
 __device__ __forceinline__ uint64_t load_ct8(const uint8_t* __restrict__ p) {
   return *reinterpret_cast<const uint64_t*>(p);
 }
@@ -230,6 +239,76 @@ __device__ __forceinline__ bool prob_done(const Ctx& c, const TileRec& T, int fl
   return (flags & F_CHECK_DONE) && c.ps[T.prob].done;
 }
 
+__device__ __forceinline__ void unpack8(float4 a, float4 b, float v[8]) {
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+// Cells of one parity of a k-pencil held as two float4 (k 0..3, 4..7).
+__device__ __forceinline__ float4 even4(float4 a, float4 b) { return make_float4(a.x, a.z, b.x, b.z); }
+__device__ __forceinline__ float4 odd4(float4 a, float4 b) { return make_float4(a.y, a.w, b.y, b.w); }
+__device__ __forceinline__ float4 sel4(bool p, float4 if_false, float4 if_true) {
+  return make_float4(p ? if_true.x : if_false.x, p ? if_true.y : if_false.y, p ? if_true.z : if_false.z,
+                     p ? if_true.w : if_false.w);
+}
+__device__ __forceinline__ float comp4(float4 v, int m) { return m == 0 ? v.x : (m == 1 ? v.y : (m == 2 ? v.z : v.w)); }
+
+// ---------------------------------------------------------------------------
+// Colour-split layout of the V-cycle arrays (z on every level, its rhs and the
+// down-sweep scratch).  Within a tile the 256 cells of colour 0 ((i+j+k) even)
+// come first, then the 256 of colour 1; lane l = (i,j) keeps its four cells of
+// each colour at [colour*256 + l*4, +4) in k order.  A red-black colour pass
+// then reads the updated colour's rhs and writes its x as contiguous halves,
+// and a neighbour's other-colour quad lines up component by component with the
+// lane's own updated quad.  Channels seen by the caller and the CG vectors keep
+// the natural order.
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ int split_cell(int i, int j, int k) {
+  return (((i + j + k) & 1) << 8) | (((i << 3) | j) << 2) | (k >> 1);
+}
+__host__ __device__ __forceinline__ int split_index(int cell) { return split_cell(cell >> 6, (cell >> 3) & 7, cell & 7); }
+// colour of the even-k cells of pencil pl = (i,j)
+__device__ __forceinline__ int even_colour(int pl) { return ((pl >> 3) + pl) & 1; }
+// pencil pl of a tile (tile = the tile's first value) as natural float4 halves
+template <class V>
+__device__ __forceinline__ void load_pencil_split(const V* __restrict__ tile, int pl, float4& a, float4& b) {
+  const bool q0 = even_colour(pl);
+  const float4 h0 = load4(tile + pl * 4), h1 = load4(tile + 256 + pl * 4);
+  const float4 E = sel4(q0, h0, h1), O = sel4(q0, h1, h0);
+  a = make_float4(E.x, O.x, E.y, O.y);
+  b = make_float4(E.z, O.z, E.w, O.w);
+}
+template <class V>
+__device__ __forceinline__ void store_pencil_split(V* __restrict__ tile, int pl, float4 a, float4 b) {
+  const bool q0 = even_colour(pl);
+  const float4 E = even4(a, b), O = odd4(a, b);
+  store4(tile + pl * 4, sel4(q0, E, O));
+  store4(tile + 256 + pl * 4, sel4(q0, O, E));
+}
+template <class V>
+__device__ __forceinline__ void load8_split(const V* __restrict__ tile, int pl, float v[8]) {
+  float4 a, b;
+  load_pencil_split(tile, pl, a, b);
+  unpack8(a, b, v);
+}
+template <class V>
+__device__ __forceinline__ void store8_split(V* __restrict__ tile, int pl, const float v[8]) {
+  store_pencil_split(tile, pl, make_float4(v[0], v[1], v[2], v[3]), make_float4(v[4], v[5], v[6], v[7]));
+}
+// natural cells (i,j,K0..K0+3) of pencil pl, K0 in {0, 4}
+template <class V>
+__device__ __forceinline__ float4 load_quad_split(const V* __restrict__ tile, int pl, int K0) {
+  const bool q0 = even_colour(pl);
+  const float2 h0 = load2(tile + pl * 4 + (K0 >> 1)), h1 = load2(tile + 256 + pl * 4 + (K0 >> 1));
+  const float2 E = q0 ? h1 : h0, O = q0 ? h0 : h1;
+  return make_float4(E.x, O.x, E.y, O.y);
+}
+template <class V>
+__device__ __forceinline__ void store_quad_split(V* __restrict__ tile, int pl, int K0, float4 v) {
+  const bool q0 = even_colour(pl);
+  const float2 E = make_float2(v.x, v.z), O = make_float2(v.y, v.w);
+  store2(tile + pl * 4 + (K0 >> 1), q0 ? O : E);
+  store2(tile + 256 + pl * 4 + (K0 >> 1), q0 ? E : O);
+}
+
 // ---------------------------------------------------------------------------
 // Tile + halo staging
 // ---------------------------------------------------------------------------
@@ -253,8 +332,10 @@ struct Stencil {
 //   ghost_zero: treat X of GHOST tiles as 0 (downstroke of the V-cycle)
 //   bcv:        the 7 face coefficients of the view (faces x-..z+, inside)
 // X holds active tiles, XG ghost tiles (they differ only for the damped-Jacobi
-// ping-pong, where ghost values stay in the primary buffer).
-template <class V>
+// ping-pong, where ghost values stay in the primary buffer).  SPLIT: X and XG
+// are V-cycle arrays in the colour-split layout (the staged pencils are in
+// natural order either way).
+template <bool SPLIT, class V>
 __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const V* __restrict__ X,
                                            const V* __restrict__ XG, bool zero_all, bool ghost_zero,
                                            const float bcv[7], TileLDS& L, Stencil& S) {
@@ -262,6 +343,7 @@ __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const
   const size_t base = size_t(T.self) * 512 + lane * 8;
   const bool own_ghost = tr_type(T) == T_GHOST;
   if (zero_all || (ghost_zero && own_ghost)) { for (int k = 0; k < 8; k++) S.own[k] = 0.f; }
+  else if (SPLIT) load8_split((own_ghost ? XG : X) + size_t(T.self) * 512, lane, S.own);
   else load8((own_ghost ? XG : X) + base, S.own);
   S.m = load_ct8(c.fmask + base);
   const int pc = (i + 1) * 10 + (j + 1);
@@ -277,18 +359,22 @@ __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const
     const int dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
     const bool gh = tr_ntype(T, f) == T_GHOST;
     float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;  // x = 0 outside the grid (GMGSolver.cu:120-122)
-    if (n >= 0 && !(zero_all || (ghost_zero && gh))) load_pencil((gh ? XG : X) + size_t(n) * 512 + src * 8, v0, v1);
+    if (n >= 0 && !(zero_all || (ghost_zero && gh))) {
+      if (SPLIT) load_pencil_split((gh ? XG : X) + size_t(n) * 512, src, v0, v1);
+      else load_pencil((gh ? XG : X) + size_t(n) * 512 + src * 8, v0, v1);
+    }
     L.xs[0][dst] = v0;
     L.xs[1][dst] = v1;
   }
-  // k halos
+  // k halos: cell (i,j,7) of the z- neighbour, (i,j,0) of the z+ neighbour
   {
+    const int o7 = SPLIT ? split_cell(i, j, 7) : lane * 8 + 7, o0 = SPLIT ? split_cell(i, j, 0) : lane * 8;
     int n = T.nbr[2];
     bool gh = tr_ntype(T, 2) == T_GHOST;
-    S.xkm = (n < 0 || zero_all || (ghost_zero && gh)) ? 0.f : to_f((gh ? XG : X)[size_t(n) * 512 + lane * 8 + 7]);
+    S.xkm = (n < 0 || zero_all || (ghost_zero && gh)) ? 0.f : to_f((gh ? XG : X)[size_t(n) * 512 + o7]);
     n = T.nbr[5];
     gh = tr_ntype(T, 5) == T_GHOST;
-    S.xkp = (n < 0 || zero_all || (ghost_zero && gh)) ? 0.f : to_f((gh ? XG : X)[size_t(n) * 512 + lane * 8]);
+    S.xkp = (n < 0 || zero_all || (ghost_zero && gh)) ? 0.f : to_f((gh ? XG : X)[size_t(n) * 512 + o0]);
   }
   S.h = T.h;
   S.b_int = bcv[6];
@@ -310,9 +396,6 @@ __host__ __device__ __forceinline__ void view_coeffs(const TileRec& T, int sl, f
   }
 }
 
-__device__ __forceinline__ void unpack8(float4 a, float4 b, float v[8]) {
-  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-}
 
 // The 7-point row of one cell: faces visited in the reference's order x-, x+,
 // y-, y+, z-, z+ (GMGSolver.cu:62-73); returns A x and D (times h^3, 0 off
@@ -372,14 +455,6 @@ __device__ __forceinline__ void apply_pencil(const NbrPencils& N, const Stencil&
   }
 }
 
-// Cells of one parity of a k-pencil held as two float4 (k 0..3, 4..7).
-__device__ __forceinline__ float4 even4(float4 a, float4 b) { return make_float4(a.x, a.z, b.x, b.z); }
-__device__ __forceinline__ float4 odd4(float4 a, float4 b) { return make_float4(a.y, a.w, b.y, b.w); }
-__device__ __forceinline__ float4 sel4(bool p, float4 if_false, float4 if_true) {
-  return make_float4(p ? if_true.x : if_false.x, p ? if_true.y : if_false.y, p ? if_true.z : if_false.z,
-                     p ? if_true.w : if_false.w);
-}
-__device__ __forceinline__ float comp4(float4 v, int m) { return m == 0 ? v.x : (m == 1 ? v.y : (m == 2 ? v.z : v.w)); }
 
 // One red-black colour update on the lane's four cells of that colour: cell
 // (i,j,k) is updated when (i+j+k) % 2 == color, i.e. k = 2m + p with the lane's
@@ -431,26 +506,105 @@ __device__ __forceinline__ void gs_pencil(const NbrPencils& N, const Stencil& S,
 // ---------------------------------------------------------------------------
 // Per-tile bodies (one wavefront each)
 // ---------------------------------------------------------------------------
+// The V-cycle rhs of the tile (colour-split layout), natural order.
 template <class V>
 __device__ __forceinline__ void load_rhs(const Ctx& c, const TileRec& T, const V* F, int flags, float b[8]) {
   const V* src = (flags & F_VCYCLE) ? vc_rhs<V>(c) : F;
-  load8(src + size_t(T.self) * 512 + lane_id() * 8, b);
+  load8_split(src + size_t(T.self) * 512, lane_id(), b);
 }
 
-// Red-black Gauss-Seidel colour sweep of a LEAF|NONLEAF tile (GMGSolver.cu:216-269).
+// The four updated cells of a red-black colour pass, in split-layout quads:
+// cell k = 2m + p of lane (i,j) (p = (color+i+j)&1) with value X0.m, rhs B.m,
+// i/j neighbours XIM..XJP (component m: the neighbours' other-colour quads line
+// up with the lane's updated quad), the lane's other-colour quad XO and the
+// k-halo values xkm (cell (i,j,-1), used when p = 0) and xkp ((i,j,8), p = 1).
+__device__ __forceinline__ float4 gs_quad(float4 X0, float4 XO, float4 XIM, float4 XIP, float4 XJM, float4 XJP,
+                                          float xkm, float xkp, float4 B, uint64_t mask, const float bcv[7], float h,
+                                          int i, int j, bool p) {
+  // k-1 and k+1 neighbours of cell 2m+p
+  const float4 XKM = sel4(p, make_float4(xkm, XO.x, XO.y, XO.z), XO);
+  const float4 XKP = sel4(p, XO, make_float4(XO.y, XO.z, XO.w, xkp));
+  const float bi = bcv[6];
+  const float bim = (i == 0) ? bcv[0] : bi;
+  const float bip = (i == 7) ? bcv[3] : bi;
+  const float bjm = (j == 0) ? bcv[1] : bi;
+  const float bjp = (j == 7) ? bcv[4] : bi;
+  const int sh = p ? 8 : 0;
+  float nv[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const uint32_t mb = uint32_t(mask >> (16 * m + sh)) & 0xff;
+    const float bkm = (!p && m == 0) ? bcv[2] : bi;
+    const float bkp = (p && m == 3) ? bcv[5] : bi;
+    const float x0 = comp4(X0, m);
+    const float xn[6] = {comp4(XIM, m), comp4(XIP, m), comp4(XJM, m), comp4(XJP, m), comp4(XKM, m), comp4(XKP, m)};
+    const float bc[6] = {bim, bip, bjm, bjp, bkm, bkp};
+    float ax, dg;
+    cell_row_m(x0, mb, xn, bc, h, ax, dg);
+    float delta = 0.f;
+    if ((mb & M_INT) && dg != 0.f) delta = (comp4(B, m) - ax) / dg;
+    nv[m] = x0 + delta;
+  }
+  return make_float4(nv[0], nv[1], nv[2], nv[3]);
+}
+
+struct GsLDS {
+  float4 xo[100];  // other-colour quads of the tile's pencils and of the halo ring, index (i+1)*10 + (j+1)
+};
+
+// Red-black Gauss-Seidel colour sweep of a LEAF|NONLEAF tile (GMGSolver.cu:216-269)
+// on the colour-split arrays: reads the tile's two x halves, the updated
+// colour's rhs half and the neighbours' other-colour quads; writes the updated
+// colour's half (and, from an implicit zero guess, zeros to the other half).
 // An INTERIOR cell with D == 0 is left unchanged.
 template <class V>
 __device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int level, V* X, const V* F, int color,
-                                        int flags, TileLDS& L) {
-  Stencil S;
-  stage_tile(c, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
-  float b[8];
-  load_rhs(c, T, F, flags, b);
+                                        int flags, GsLDS& L) {
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const bool zero_all = flags & F_ZERO_X, ghost_zero = flags & F_GHOST_ZERO;
+  const int hc = color << 8, ho = (color ^ 1) << 8;
+  V* tile = X + size_t(T.self) * 512;
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 xc = z4, xo = z4;
+  if (!zero_all) {
+    xc = load4(tile + hc + lane * 4);
+    xo = load4(tile + ho + lane * 4);
+  }
+  const uint64_t mask = load_ct8(c.fmask + size_t(T.self) * 512 + lane * 8);
+  const V* src = (flags & F_VCYCLE) ? vc_rhs<V>(c) : F;
+  const float4 B = load4(src + size_t(T.self) * 512 + hc + lane * 4);
+  L.xo[(i + 1) * 10 + (j + 1)] = xo;
+  // halo ring: lanes 0..31 fetch the other-colour quad of one neighbour pencil
+  //   g = 0: x- (pencil (7,t) of nbr 0), 1: x+ ((0,t) of nbr 3), 2: y- ((t,7) of nbr 1), 3: y+ ((t,0) of nbr 4)
+  if (lane < 32) {
+    const int g = lane >> 3, t = lane & 7;
+    const int f = (g == 0) ? 0 : (g == 1) ? 3 : (g == 2) ? 1 : 4;
+    const int n = pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
+    const int src_pl = (g == 0) ? (56 + t) : (g == 1) ? t : (g == 2) ? (t * 8 + 7) : (t * 8);
+    const int dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
+    const bool gh = tr_ntype(T, f) == T_GHOST;
+    float4 v = z4;  // x = 0 outside the grid (GMGSolver.cu:120-122)
+    if (n >= 0 && !(zero_all || (ghost_zero && gh))) v = load4(X + size_t(n) * 512 + ho + src_pl * 4);
+    L.xo[dst] = v;
+  }
+  // k halos: cell (i,j,7) of the z- neighbour and (i,j,0) of the z+ neighbour
+  float xkm = 0.f, xkp = 0.f;
+  {
+    int n = T.nbr[2];
+    bool gh = tr_ntype(T, 2) == T_GHOST;
+    if (n >= 0 && !(zero_all || (ghost_zero && gh))) xkm = to_f(X[size_t(n) * 512 + split_cell(i, j, 7)]);
+    n = T.nbr[5];
+    gh = tr_ntype(T, 5) == T_GHOST;
+    if (n >= 0 && !(zero_all || (ghost_zero && gh))) xkp = to_f(X[size_t(n) * 512 + split_cell(i, j, 0)]);
+  }
   if (prob_done(c, T, flags)) return;
   wave_sync();
-  float4 out0, out1;
-  gs_pencil(lds_nbrs(L), S, b, color, out0, out1);
-  store_pencil(X + size_t(T.self) * 512 + lane_id() * 8, out0, out1);
+  const int pc = (i + 1) * 10 + (j + 1);
+  const bool p = ((color + i + j) & 1) != 0;
+  const float4 nv = gs_quad(xc, xo, L.xo[pc - 10], L.xo[pc + 10], L.xo[pc - 1], L.xo[pc + 1], xkm, xkp, B, mask, T.bc,
+                            T.h, i, j, p);
+  store4(tile + hc + lane * 4, nv);
+  if (zero_all) store4(tile + ho + lane * 4, z4);  // the zero guess of the other colour, for the next pass
 }
 
 // Damped-Jacobi sweep (the update of GMGSolver.cu:508-525 with the level's
@@ -459,7 +613,7 @@ template <class V>
 __device__ __forceinline__ void tile_jacobi(const Ctx& c, const TileRec& T, int level, const V* X, const V* XG,
                                             V* Y, const V* F, float omega, int flags, TileLDS& L) {
   Stencil S;
-  stage_tile(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
+  stage_tile<true>(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
   float b[8];
   load_rhs(c, T, F, flags, b);
   if (prob_done(c, T, flags)) return;
@@ -475,7 +629,7 @@ __device__ __forceinline__ void tile_jacobi(const Ctx& c, const TileRec& T, int 
       out[k] = (dg[k] == 0.f) ? 0.f : S.own[k] + (omega * res) / dg[k];
     }
   }
-  store8(Y + size_t(T.self) * 512 + lane_id() * 8, out);
+  store8_split(Y + size_t(T.self) * 512, lane_id(), out);
 }
 
 // 2x2x2 sums of a tile's values staged in LDS, in the reference's (ii, jj, kk)
@@ -494,6 +648,12 @@ __device__ __forceinline__ size_t parent_cell(const TileRec& T, int q) {
   const int oct = tr_octant(T), ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
   return size_t(T.parent) * 512 + ((ox * 4 + a) << 6) + ((oy * 4 + bq) << 3) + (oz * 4 + cq);
 }
+// the same parent cell in the colour-split layout
+__device__ __forceinline__ size_t parent_cell_split(const TileRec& T, int q) {
+  int a = q >> 4, bq = (q >> 2) & 3, cq = q & 3;
+  const int oct = tr_octant(T), ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
+  return size_t(T.parent) * 512 + split_cell(ox * 4 + a, oy * 4 + bq, oz * 4 + cq);
+}
 
 // Residual and restriction of one tile of `level` to level-1
 // (GMGSolver.cu:284-339).  Output channel CO (rhsv in V-cycle mode): the
@@ -503,7 +663,7 @@ template <class V>
 __device__ __forceinline__ void tile_restrict(const Ctx& c, const TileRec& T, int level, const V* X, const V* F,
                                               V* CO, float one_over_alpha, int flags, TileLDS& L, float* res) {
   Stencil S;
-  stage_tile(c, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
+  stage_tile<true>(c, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
   const int lane = lane_id();
   float b[8];
   if ((flags & F_VCYCLE) && tr_type(T) == T_GHOST) { for (int k = 0; k < 8; k++) b[k] = 0.f; }
@@ -517,7 +677,7 @@ __device__ __forceinline__ void tile_restrict(const Ctx& c, const TileRec& T, in
   float s = octet_sum(res, lane);
   const bool ghost = tr_type(T) == T_GHOST;
   V* out = (flags & F_VCYCLE) ? vc_rhs<V>(c) : CO;
-  size_t pcell = parent_cell(T, lane);
+  size_t pcell = parent_cell_split(T, lane);
   if (ghost) out[pcell] = from_f<V>(to_f(out[pcell]) + s);
   else out[pcell] = from_f<V>(s * one_over_alpha);
 }
@@ -532,21 +692,21 @@ __device__ __forceinline__ void tile_prolong(const Ctx& c, const TileRec& T, int
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t base = size_t(T.self) * 512 + lane * 8;
   const int oct = tr_octant(T), ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
-  const float4 pv = load4(XC + size_t(T.parent) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) + oz * 4);
+  const float4 pv = load_quad_split(XC + size_t(T.parent) * 512, (ox * 4 + (i >> 1)) * 8 + (oy * 4 + (j >> 1)), oz * 4);
   const float cz[4] = {pv.x, pv.y, pv.z, pv.w};
   const uint64_t ct = load_ct8(c.ctype + base);
   const bool ghost = tr_type(T) == T_GHOST;
   float fx[8];
   if (ghost || (flags & F_ZERO_X)) { for (int k = 0; k < 8; k++) fx[k] = 0.f; }
-  else load8(XF + base, fx);
+  else load8_split(XF + size_t(T.self) * 512, lane, fx);
   if (prob_done(c, T, flags)) return;
   float out[8];
   if (ghost) {
     for (int k = 0; k < 8; k++) out[k] = (ct_at(ct, k) & C_INT) ? cz[k >> 1] : 0.f;
-    store8(XG + base, out);
+    store8_split(XG + size_t(T.self) * 512, lane, out);
   } else {
     for (int k = 0; k < 8; k++) out[k] = (ct_at(ct, k) & C_INT) ? __builtin_fmaf(pcoef, cz[k >> 1], fx[k]) : 0.f;
-    store8(XO + base, out);
+    store8_split(XO + size_t(T.self) * 512, lane, out);
   }
 }
 
@@ -557,10 +717,11 @@ __device__ __forceinline__ void tile_prolong(const Ctx& c, const TileRec& T, int
 // INTERIOR cells of GHOST tiles, 0 elsewhere), so no separate launch is needed.
 // XD holds the down-sweep values, XO receives the level's result; GHOST tiles
 // in the list only store their prolongated values.
+// parent cells of pencil (i,j) of a child in octant oct (colour-split array)
 template <class V>
 __device__ __forceinline__ float4 parent_quad(const V* XC, int parent, int oct, int i, int j) {
   const int ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
-  return load4(XC + size_t(parent) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) + oz * 4);
+  return load_quad_split(XC + size_t(parent) * 512, (ox * 4 + (i >> 1)) * 8 + (oy * 4 + (j >> 1)), oz * 4);
 }
 // prolongated values of one pencil: active (fma with the down-sweep values) or
 // ghost; m: the pencil's face masks (their INTERIOR bits).  Values are rounded
@@ -587,14 +748,12 @@ __device__ __forceinline__ void khalo_prolong(const Ctx& c, const TileRec& T, co
   const int n = T.nbr[F];
   if (n < 0) { xv = 0.f; return; }
   const int i = lane >> 3, j = lane & 7;
-  const size_t o = size_t(n) * 512 + lane * 8 + KN;
-  const uint8_t mv = c.fmask[o];
+  const uint8_t mv = c.fmask[size_t(n) * 512 + lane * 8 + KN];
   const bool gh = tr_ntype(T, F) == T_GHOST;
   const int noct = oct ^ 4;
   const int ox = noct & 1, oy = (noct >> 1) & 1, oz = (noct >> 2) & 1;
-  const float pz = to_f(XO[size_t(T.nbr_parent[F]) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) +
-                           oz * 4 + (KN >> 1)]);
-  const float xo = gh ? 0.f : to_f(XD[o]);
+  const float pz = to_f(XO[size_t(T.nbr_parent[F]) * 512 + split_cell(ox * 4 + (i >> 1), oy * 4 + (j >> 1), oz * 4 + (KN >> 1))]);
+  const float xo = gh ? 0.f : to_f(XD[size_t(n) * 512 + split_cell(i, j, KN)]);
   xv = (mv & M_INT) ? vround<V>(gh ? pz : __builtin_fmaf(pcoef, pz, xo)) : 0.f;
 }
 
@@ -608,12 +767,12 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
   const uint64_t mown = load_ct8(c.fmask + base);
   const float4 pq = parent_quad(XO, T.parent, oct, i, j);
   float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), xb = xa;
-  if (!own_ghost) load_pencil(XD + base, xa, xb);
+  if (!own_ghost) load_pencil_split(XD + size_t(T.self) * 512, lane, xa, xb);
   float4 oa, ob;
   prolong_pencil<V>(xa, xb, pq, mown, own_ghost, pcoef, oa, ob);
   if (own_ghost) {  // GHOST tile: its prolongated values only
     if (prob_done(c, T, flags)) return;
-    store_pencil(XO + base, oa, ob);
+    store_pencil_split(XO + size_t(T.self) * 512, lane, oa, ob);
     return;
   }
   Stencil S;
@@ -636,11 +795,10 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
     const int noct = oct ^ (1 << (f % 3));
     float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;  // x = 0 outside the grid (GMGSolver.cu:120-122)
     if (n >= 0) {
-      const size_t nb = size_t(n) * 512 + (ni * 8 + nj) * 8;
       const bool gh = tr_ntype(T, f) == T_GHOST;
-      const uint64_t w = load_ct8(c.fmask + nb);
+      const uint64_t w = load_ct8(c.fmask + size_t(n) * 512 + (ni * 8 + nj) * 8);
       float4 ya = make_float4(0.f, 0.f, 0.f, 0.f), yb = ya;
-      if (!gh) load_pencil(XD + nb, ya, yb);
+      if (!gh) load_pencil_split(XD + size_t(n) * 512, ni * 8 + nj, ya, yb);
       prolong_pencil<V>(ya, yb, parent_quad(XO, pn, noct, ni, nj), w, gh, pcoef, va, vb);
     }
     L.xs[0][dst] = va;
@@ -653,12 +811,12 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
   S.b_int = T.bc[6];
   for (int f = 0; f < 6; f++) S.b_f[f] = T.bc[f];
   float b[8];
-  load8(vc_rhs<V>(c) + base, b);
+  load8_split(vc_rhs<V>(c) + size_t(T.self) * 512, lane, b);
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float4 out0, out1;
   gs_pencil(lds_nbrs(L), S, b, color, out0, out1);
-  store_pencil(XO + base, out0, out1);
+  store_pencil_split(XO + size_t(T.self) * 512, lane, out0, out1);
 }
 
 // ---------------------------------------------------------------------------
@@ -673,7 +831,7 @@ template <class V>
 __global__ __launch_bounds__(64) void k_gs_color(Ctx c, const TileRec* __restrict__ list, int n, int level, V* X,
                                                  const V* F, int color, int flags) {
   TILE_KERNEL_PROLOGUE
-  __shared__ TileLDS L;
+  __shared__ GsLDS L;
   tile_gs(c, T, level, X, F, color, flags, L);
 }
 template <class V>
@@ -824,7 +982,7 @@ __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, const CoarsePl
   for (int q = tid; q < cp.nslots * 128; q += CL_THREADS) {
     const int s = q >> 7, o = (q & 127) * 4;
     const size_t g = size_t(M.S[s].tile) * 512 + o;
-    *reinterpret_cast<float4*>(&M.b[s][o]) = load4(vc_rhs<V>(c) + g);
+    *reinterpret_cast<float4*>(&M.b[s][o]) = load_quad_split(vc_rhs<V>(c) + size_t(M.S[s].tile) * 512, o >> 3, o & 7);
     *reinterpret_cast<float4*>(&M.z[s * 512 + o]) = make_float4(0.f, 0.f, 0.f, 0.f);
     *reinterpret_cast<uint32_t*>(&M.ct[s][o]) = *reinterpret_cast<const uint32_t*>(c.ctype + g);
   }
@@ -938,7 +1096,7 @@ __global__ __launch_bounds__(CL_THREADS) void k_coarse_lds(Ctx c, const CoarsePl
   }
   for (int q = tid; q < cp.nslots * 128; q += CL_THREADS) {
     const int s = q >> 7, o = (q & 127) * 4;
-    store4(Z + size_t(M.S[s].tile) * 512 + o, *reinterpret_cast<const float4*>(&M.z[s * 512 + o]));
+    store_quad_split(Z + size_t(M.S[s].tile) * 512, o >> 3, o & 7, *reinterpret_cast<const float4*>(&M.z[s * 512 + o]));
   }
 }
 
@@ -963,7 +1121,7 @@ __global__ __launch_bounds__(64) void k_cg_init(Ctx c, const TileRec* __restrict
   store8(c.p + base, zero);
   float rv[8];
   load8(c.r + base, rv);
-  store8(vc_rhs<V>(c) + base, rv);
+  store8_split(vc_rhs<V>(c) + size_t(T.self) * 512, lane, rv);
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(rv[k]) * double(rv[k]);
   a = wave_sum(a);
@@ -986,7 +1144,7 @@ __global__ __launch_bounds__(64) void k_cg_init_warm(Ctx c, const TileRec* __res
   float zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   store8(c.p + base, zero);
   store8(c.r + base, rv);
-  store8(vc_rhs<V>(c) + base, rv);
+  store8_split(vc_rhs<V>(c) + size_t(T.self) * 512, lane, rv);
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(bv[k]) * double(bv[k]);
   a = wave_sum(a);
@@ -1084,7 +1242,7 @@ __global__ __launch_bounds__(64) void k_cg_dot_rz(Ctx c, const TileRec* __restri
   const size_t base = size_t(T.self) * 512 + lane * 8;
   float rv[8], zv[8];
   load8(c.r + base, rv);
-  load8(vc_z<V>(c) + base, zv);
+  load8_split(vc_z<V>(c) + size_t(T.self) * 512, lane, zv);
   if (prob_done(c, T, flags)) return;
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(rv[k]) * double(zv[k]);
@@ -1102,7 +1260,7 @@ __global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const TileRec* __rest
   const size_t base = size_t(T.self) * 512 + lane * 8;
   float pv[8], zv[8];
   load8(c.p + base, pv);
-  load8(vc_z<V>(c) + base, zv);
+  load8_split(vc_z<V>(c) + size_t(T.self) * 512, lane, zv);
   const uint64_t ct = load_ct8(c.ctype + base);
   if (prob_done(c, T, flags)) return;
   const float beta = float(c.ps[T.prob].beta);
@@ -1138,7 +1296,7 @@ __global__ __launch_bounds__(64) void k_apply(Ctx c, const TileRec* __restrict__
   Stencil S;
   float bcv[7];
   view_coeffs(T, sl, bcv);
-  stage_tile(c, T, X, X, false, false, bcv, L, S);
+  stage_tile<false>(c, T, X, X, false, false, bcv, L, S);
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float ax[8], dg[8];
@@ -1214,7 +1372,7 @@ __global__ __launch_bounds__(64) void k_cg_update_xr(Ctx c, const TileRec* __res
   }
   store8(c.x + base, xv);
   store8(c.r + base, rv);
-  store8(vc_rhs<V>(c) + base, rv);
+  store8_split(vc_rhs<V>(c) + size_t(T.self) * 512, lane, rv);
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(rv[k]) * double(rv[k]);
   a = wave_sum(a);
@@ -1330,7 +1488,7 @@ __global__ __launch_bounds__(64) void k_subtract_mean(Ctx c, const TileRec* __re
   const uint64_t ct = load_ct8(c.ctype + base);
   for (int k = 0; k < 8; k++) if (ct_at(ct, k) & C_INT) v[k] = float(double(v[k]) - mean);
   store8(X + base, v);
-  if (write_rhsv) store8(vc_rhs<V>(c) + base, v);
+  if (write_rhsv) store8_split(vc_rhs<V>(c) + size_t(T.self) * 512, lane_id(), v);
 }
 
 // ---------------------------------------------------------------------------
@@ -1401,6 +1559,24 @@ __global__ __launch_bounds__(64) void k_copy_tiles(const TileRec* __restrict__ l
   float v[8];
   load8(src + base, v);
   store8(dst + base, v);
+}
+// Natural <-> colour-split order of whole tiles (channels handed to the
+// operator API are natural; the V-cycle kernels work on split arrays).
+__global__ __launch_bounds__(64) void k_to_split(const TileRec* __restrict__ list, int n, const float* __restrict__ src,
+                                                 float* __restrict__ dst) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  float v[8];
+  load8(src + size_t(T.self) * 512 + lane_id() * 8, v);
+  store8_split(dst + size_t(T.self) * 512, lane_id(), v);
+}
+__global__ __launch_bounds__(64) void k_from_split(const TileRec* __restrict__ list, int n, const float* __restrict__ src,
+                                                   float* __restrict__ dst) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  float v[8];
+  load8_split(src + size_t(T.self) * 512, lane_id(), v);
+  store8(dst + size_t(T.self) * 512 + lane_id() * 8, v);
 }
 // Fill whole tiles of a list with a value.
 template <class V = float>

@@ -96,6 +96,7 @@ struct cg_problem {
   double* d_r64 = nullptr;
   float* d_bsave = nullptr;
   double* d_psum = nullptr;
+  float* d_api[3] = {nullptr, nullptr, nullptr};  // colour-split copies of channels for the operator API
   float* d_gsum = nullptr;
   ProbState* d_ps = nullptr;
   int32_t* d_dflag = nullptr;
@@ -150,6 +151,7 @@ cg_problem::~cg_problem() {
                   d_plans, d_part, d_part2, d_scalar, d_lists, d_tmp_list, d_coarse};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
+  for (auto q : d_api) if (q) (void)hipFree(q);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -508,6 +510,16 @@ template <> float* pool_z<float>(cg_problem& P) { return P.d_ch[CG_CH_Z]; }
 template <> bf16* pool_z<bf16>(cg_problem& P) { return P.d_zh; }
 template <> float* pool_zb<float>(cg_problem& P) { return P.d_zb; }
 template <> bf16* pool_zb<bf16>(cg_problem& P) { return P.d_zbh; }
+// Scratch k (0..2) of the operator API: a channel in the colour-split order of
+// the V-cycle kernels; src (natural, may be null) is converted into it.
+float* api_split(cg_problem& P, int k, const float* src) {
+  if (!P.d_api[k]) HIPC(hipMalloc(&P.d_api[k], size_t(P.n) * 512 * sizeof(float)));
+  if (src) k_to_split<<<P.n, 64, 0, P.stream>>>(P.L(P.all), P.n, src, P.d_api[k]);
+  return P.d_api[k];
+}
+void api_unsplit(cg_problem& P, const DList& l, const float* split, float* dst) {
+  if (l.n) k_from_split<<<l.n, 64, 0, P.stream>>>(P.L(l), l.n, split, dst);
+}
 // bfloat16 V-cycle vectors, allocated once per pool when first needed
 void ensure_bf16(cg_problem& P) {
   if (P.d_zh) return;
@@ -1091,9 +1103,10 @@ cg_status cg_vcycle(cg_problem* p, int32_t x_channel, int32_t f_channel, int32_t
     v.omega = omega;
     v.check_done = 0;
     // the V-cycle rhs of LEAF tiles starts as f (GMGSolver.cu:393-400)
-    k_copy_tiles<<<p->leaf.n, 64, 0, p->stream>>>(p->L(p->leaf), p->leaf.n, p->d_ch[f_channel], p->d_ch[CG_CH_TMP]);
-    float* X = p->d_ch[x_channel];
-    enqueue_vcycle(*p, smoother == CG_SMOOTH_RBGS ? p->d_zb : X, X, p->d_ch[f_channel], v);
+    k_to_split<<<p->leaf.n, 64, 0, p->stream>>>(p->L(p->leaf), p->leaf.n, p->d_ch[f_channel], p->d_ch[CG_CH_TMP]);
+    float* Z = api_split(*p, 0, nullptr);
+    enqueue_vcycle<float>(*p, smoother == CG_SMOOTH_RBGS ? p->d_zb : Z, Z, nullptr, v);
+    api_unsplit(*p, p->all, Z, p->d_ch[x_channel]);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
   });
@@ -1138,7 +1151,10 @@ cg_status cg_gauss_seidel_color(cg_problem* p, int32_t level, int32_t x_channel,
     check_channel(x_channel);
     check_channel(rhs_channel);
     if (level < 0 || level > p->max_level) fail(CG_ERR_INVALID, "level out of range");
-    launch_gs(*p, p->act[level], p->ctx(), level, p->d_ch[x_channel], p->d_ch[rhs_channel], color & 1, 0);
+    float* X = api_split(*p, 0, p->d_ch[x_channel]);
+    const float* F = api_split(*p, 1, p->d_ch[rhs_channel]);
+    launch_gs(*p, p->act[level], p->ctx(), level, X, F, color & 1, 0);
+    api_unsplit(*p, p->act[level], X, p->d_ch[x_channel]);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
   });
@@ -1152,9 +1168,11 @@ cg_status cg_jacobi_sweep(cg_problem* p, int32_t level, int32_t x_channel, int32
     if (level < 0 || level > p->max_level) fail(CG_ERR_INVALID, "level out of range");
     const DList& a = p->act[level];
     if (!a.n) return;
-    float* X = p->d_ch[x_channel];
-    k_jacobi<float><<<a.n, 64, 0, p->stream>>>(p->ctx(), p->L(a), a.n, level, X, X, p->d_zb, p->d_ch[rhs_channel], omega, 0);
-    k_copy_tiles<<<a.n, 64, 0, p->stream>>>(p->L(a), a.n, p->d_zb, X);
+    const float* X = api_split(*p, 0, p->d_ch[x_channel]);
+    const float* F = api_split(*p, 1, p->d_ch[rhs_channel]);
+    float* Y = api_split(*p, 2, nullptr);
+    k_jacobi<float><<<a.n, 64, 0, p->stream>>>(p->ctx(), p->L(a), a.n, level, X, X, Y, F, omega, 0);
+    api_unsplit(*p, a, Y, p->d_ch[x_channel]);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
   });
@@ -1170,8 +1188,11 @@ cg_status cg_residual_restrict(cg_problem* p, int32_t level, int32_t x_channel, 
     if (level < 1 || level > p->max_level) fail(CG_ERR_INVALID, "level out of range");
     const DList& al = p->lvl[level];
     if (!al.n) return;
-    k_restrict<float><<<al.n, 64, 0, p->stream>>>(p->ctx(), p->L(al), al.n, level, p->d_ch[x_channel], p->d_ch[rhs_channel],
-                                           p->d_ch[coarse_channel], one_over_alpha, 0);
+    const float* X = api_split(*p, 0, p->d_ch[x_channel]);
+    const float* F = api_split(*p, 1, p->d_ch[rhs_channel]);
+    float* CO = api_split(*p, 2, p->d_ch[coarse_channel]);
+    k_restrict<float><<<al.n, 64, 0, p->stream>>>(p->ctx(), p->L(al), al.n, level, X, F, CO, one_over_alpha, 0);
+    api_unsplit(*p, p->all, CO, p->d_ch[coarse_channel]);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
   });
@@ -1185,8 +1206,10 @@ cg_status cg_prolongate(cg_problem* p, int32_t level, int32_t coarse_channel, in
     if (level < 1 || level > p->max_level) fail(CG_ERR_INVALID, "level out of range");
     const DList& al = p->lvl[level];
     if (!al.n) return;
-    float* F = p->d_ch[fine_channel];
-    k_prolong<float><<<al.n, 64, 0, p->stream>>>(p->ctx(), p->L(al), al.n, level, p->d_ch[coarse_channel], F, F, F, prolong_coeff, 0);
+    const float* XC = api_split(*p, 0, p->d_ch[coarse_channel]);
+    float* F = api_split(*p, 1, p->d_ch[fine_channel]);
+    k_prolong<float><<<al.n, 64, 0, p->stream>>>(p->ctx(), p->L(al), al.n, level, XC, F, F, F, prolong_coeff, 0);
+    api_unsplit(*p, al, F, p->d_ch[fine_channel]);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
   });
