@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(_HERE, "libcirrus_mgpcg.so")
 LEAF, NONLEAF, GHOST = 1, 2, 4
 DIRICHLET, INTERIOR, NEUMANN = 1, 2, 4
 CH_X, CH_B, CH_R, CH_P, CH_AP, CH_Z, CH_TMP = 0, 1, 1, 2, 3, 4, 5
+CH_U, CH_V, CH_W = 6, 7, 8
 NUM_CHANNELS = 6
 SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
 STORE_F32, STORE_BF16 = 0, 1
@@ -42,6 +43,7 @@ EXPORTED_SYMBOLS = (
     "cg_server_create", "cg_server_destroy", "cg_server_admit", "cg_server_release", "cg_server_submit", "cg_server_step",
     "cg_server_next_result", "cg_server_info", "cg_server_perturb_rhs",
     "cg_refine_params_default", "cg_solve_refined", "cg_get_solution_f64",
+    "cg_velocity_divergence", "cg_add_gradient", "cg_project",
 )
 
 
@@ -153,6 +155,9 @@ def lib():
             "cg_server_next_result": (i32, [vp, i32, P(i32), P(ctypes.c_int64), P(i32), P(Result), vp]),
             "cg_server_info": (i32, [vp, P(ctypes.c_int64), P(ctypes.c_int64), P(i32)]),
             "cg_refine_params_default": (None, [P(RefineParams)]),
+            "cg_velocity_divergence": (i32, [vp, i32, i32]),
+            "cg_add_gradient": (i32, [vp, i32]),
+            "cg_project": (i32, [vp, P(Params), P(Result)]),
             "cg_solve_refined": (i32, [vp, P(Params), P(RefineParams), P(RefineResult)]),
             "cg_get_solution_f64": (i32, [vp, vp]),
             "cg_scene_create": (i32, [P(SceneParams), P(vp)]),
@@ -364,6 +369,17 @@ def FullNegativeLaplacian(prob, x_channel, Ax_channel, calc_diag=False):
     _check(lib().cg_full_negative_laplacian(prob.handle, x_channel, Ax_channel, int(calc_diag)), "FullNegativeLaplacian")
 
 
+def VelocityVolumeDivergenceOnLeafs(prob, div_channel=CH_B, propagate_ghosts=True):
+    """Divergence of the face velocities (channels 6-8) into div_channel
+    (src/PoissonSolver.cu:66-142), ghost velocities first taken from the parents."""
+    _check(lib().cg_velocity_divergence(prob.handle, div_channel, int(propagate_ghosts)), "VelocityVolumeDivergenceOnLeafs")
+
+
+def AddGradientToFaceCenters(prob, p_channel=CH_X):
+    """u += grad p on the face velocities (src/PoissonSolver.cu:144-209)."""
+    _check(lib().cg_add_gradient(prob.handle, p_channel), "AddGradientToFaceCenters")
+
+
 def NegativeLaplacianSameLevel(prob, level, subtree_level, tile_types, x_channel, Ax_channel, calc_diag=False):
     _check(lib().cg_negative_laplacian_same_level(prob.handle, level, subtree_level, tile_types, x_channel,
                                                   Ax_channel, int(calc_diag)), "NegativeLaplacianSameLevel")
@@ -462,6 +478,18 @@ class GMGSolver:
         _check(lib().cg_solve_refined(prob.handle, ctypes.byref(p), ctypes.byref(rp), res), "GMGSolver.solve_refined")
         self.last_refined = [r.as_dict() for r in res]
         return self.last_refined
+
+    def project(self, prob, max_iters=1000, relative_tolerance=1e-6, level_iters=1, coarsest_iters=10,
+                is_pure_neumann=False, smoother=SMOOTH_RBGS, storage=STORE_F32):
+        """FluidEuler::project without the velocity BCs (cg_project): divergence of
+        the face velocities, the MGPCG solve, the gradient added back; one result
+        dict per problem."""
+        p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
+                           coarsest_iters=coarsest_iters, smoother=smoother, pure_neumann=int(is_pure_neumann),
+                           one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff, storage=int(storage))
+        res = (Result * prob.num_problems)()
+        _check(lib().cg_project(prob.handle, ctypes.byref(p), res), "GMGSolver.project")
+        return [r.as_dict() for r in res]
 
     def dampedJacobiSolve(self, prob, verbose=False, max_iters=100, relative_tolerance=1e-6, omega=2.0 / 3.0):
         r = Result()

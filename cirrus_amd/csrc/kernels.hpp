@@ -1858,4 +1858,223 @@ __global__ __launch_bounds__(FIN_THREADS) void k_prob_sums(Ctx c, double* __rest
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+// ---------------------------------------------------------------------------
+// The projection around the solve (FluidEuler::project, FluidEuler.h:581-612):
+// face velocities u, v, w (one value per cell: its minus face, natural
+// layout), their divergence as the right-hand side, and the pressure gradient
+// added back.  One wavefront per tile, lane = pencil; neighbour pencils are
+// read straight from memory (these run once per projection).
+// ---------------------------------------------------------------------------
+struct Vel {
+  float* u[3];
+};
+__device__ __forceinline__ void load8c(const uint8_t* __restrict__ p, uint8_t v[8]) {
+  const uint64_t w = load_ct8(p);
+#pragma unroll
+  for (int k = 0; k < 8; k++) v[k] = ct_at(w, k);
+}
+
+// Ghost velocities from the parents (PropagateToChildren with cell types
+// INTERIOR|DIRICHLET|NEUMANN, PoissonGrid.cu:635-661): a typed ghost cell takes
+// its parent cell's value (0 if the parent cell is untyped).
+__global__ __launch_bounds__(64) void k_propagate_vel(Ctx c, const TileRec* __restrict__ list, int n, Vel U) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
+  const int oct = tr_octant(T), ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
+  const size_t pb = size_t(T.parent) * 512 + ((ox * 4 + (i >> 1)) << 6) + ((oy * 4 + (j >> 1)) << 3) + oz * 4;
+  const uint32_t pc = *reinterpret_cast<const uint32_t*>(c.ctype + pb);
+  const uint64_t ct = load_ct8(c.ctype + base);
+  const uint8_t mask = C_INT | C_DIR | C_NEU;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float4 pv = load4(U.u[a] + pb);
+    const float cv[4] = {pv.x, pv.y, pv.z, pv.w};
+    float v[8];
+    load8(U.u[a] + base, v);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (!(ct_at(ct, k) & mask)) continue;
+      const bool pin = uint8_t(pc >> (8 * (k >> 1))) & mask;
+      v[k] = pin ? cv[k >> 1] : 0.f;
+    }
+    store8(U.u[a] + base, v);
+  }
+}
+
+// Does a face between tiles of types t0 (the cell's) and t1 count?  LEAF-LEAF,
+// LEAF-GHOST, GHOST-LEAF (PoissonSolver.cu:113-119).
+__device__ __forceinline__ bool flux_face(uint8_t t0, uint8_t t1) {
+  return (t0 == T_LEAF && (t1 == T_LEAF || t1 == T_GHOST)) || (t0 == T_GHOST && t1 == T_LEAF);
+}
+
+// VelocityVolumeDivergenceOnLeafs (PoissonSolver.cu:66-137) on LEAF|GHOST
+// tiles: INTERIOR cells sum u h^2 over their counted faces in the order x-,
+// x+, y-, y+, z-, z+ (-u of the own minus face, +u of the plus neighbour's,
+// 0 outside the grid, where the neighbour counts as the tile's own type); the
+// other cells get 0.
+__global__ __launch_bounds__(64) void k_divergence(Ctx c, const TileRec* __restrict__ list, int n, Vel U,
+                                                   float* __restrict__ DIV) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t self = size_t(T.self) * 512;
+  const size_t base = self + lane * 8;
+  const uint8_t t0 = tr_type(T);
+  const float h = T.h;
+  float own[3][8];
+#pragma unroll
+  for (int a = 0; a < 3; a++) load8(U.u[a] + base, own[a]);
+  // plus neighbours: u of pencil (i+1,j), v of pencil (i,j+1), w of cell (i,j,k+1)
+  float up[8], vp[8], wkp = 0.f;
+  uint8_t tx = t0, ty = t0, tz = t0;  // tile types across the plus faces on the tile boundary
+  {
+    const int nx = T.nbr[3], ny = T.nbr[4], nz = T.nbr[5];
+    if (i < 7) load8(U.u[0] + base + 64, up);
+    else if (nx >= 0) load8(U.u[0] + size_t(nx) * 512 + j * 8, up);
+    else for (int k = 0; k < 8; k++) up[k] = 0.f;
+    if (j < 7) load8(U.u[1] + base + 8, vp);
+    else if (ny >= 0) load8(U.u[1] + size_t(ny) * 512 + i * 64, vp);
+    else for (int k = 0; k < 8; k++) vp[k] = 0.f;
+    if (nz >= 0) wkp = U.u[2][size_t(nz) * 512 + lane * 8];
+    if (i == 7 && nx >= 0) tx = tr_ntype(T, 3);
+    if (j == 7 && ny >= 0) ty = tr_ntype(T, 4);
+    if (nz >= 0) tz = tr_ntype(T, 5);
+  }
+  // tile types across the minus faces on the tile boundary
+  const uint8_t sx = (i == 0 && T.nbr[0] >= 0) ? tr_ntype(T, 0) : t0;
+  const uint8_t sy = (j == 0 && T.nbr[1] >= 0) ? tr_ntype(T, 1) : t0;
+  const uint8_t sz = (T.nbr[2] >= 0) ? tr_ntype(T, 2) : t0;
+  const uint64_t ct = load_ct8(c.ctype + base);
+  float out[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    float sum = 0.f;
+    if (ct_at(ct, k) & C_INT) {
+      const float fu[6] = {-own[0][k], up[k], -own[1][k], vp[k], -own[2][k], k < 7 ? own[2][k + 1] : wkp};
+      const uint8_t ft[6] = {sx, tx, sy, ty, k == 0 ? sz : t0, k == 7 ? tz : t0};
+#pragma unroll
+      for (int e = 0; e < 6; e++)
+        if (flux_face(t0, ft[e])) sum = __builtin_fmaf(fu[e] * h, h, sum);
+    }
+    out[k] = sum;
+  }
+  store8(DIV + base, out);
+}
+
+// AccumulateToParents(ch, ch, -1, LEAF, SUBTREE, mask, coeff, additive)
+// (PoissonGrid.cu:721-755) on LEAF tiles: each cell whose type is in mask adds
+// coeff times the sum, in child order (x fastest), of its 8 children (cells of
+// its ghost child tile) whose type is in mask.
+__global__ __launch_bounds__(64) void k_accumulate_children(Ctx c, const TileRec* __restrict__ list, int n,
+                                                            float* __restrict__ X, uint8_t mask, float coeff) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
+  const uint64_t ct = load_ct8(c.ctype + base);
+  float x[8];
+  load8(X + base, x);
+  // ghost children of the pencil's two octants (k 0..3, k 4..7)
+  const int o0 = (i >> 2) | ((j >> 2) << 1);
+  const int g[2] = {pick4(o0, uni(T.gch[0]), uni(T.gch[1]), uni(T.gch[2]), uni(T.gch[3])),
+                    pick4(o0, uni(T.gch[4]), uni(T.gch[5]), uni(T.gch[6]), uni(T.gch[7]))};
+  float fv[2][2][2][8];  // [oz][ii][jj][fine k]
+  uint8_t fc[2][2][2][8];
+#pragma unroll
+  for (int oz = 0; oz < 2; oz++)
+#pragma unroll
+    for (int ii = 0; ii < 2; ii++)
+#pragma unroll
+      for (int jj = 0; jj < 2; jj++) {
+        const int fi = 2 * (i & 3) + ii, fj = 2 * (j & 3) + jj;
+        if (g[oz] >= 0) {
+          const size_t fb = size_t(g[oz]) * 512 + (fi * 8 + fj) * 8;
+          load8(X + fb, fv[oz][ii][jj]);
+          load8c(c.ctype + fb, fc[oz][ii][jj]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 8; k++) { fv[oz][ii][jj][k] = 0.f; fc[oz][ii][jj][k] = 0; }
+        }
+      }
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if (!(ct_at(ct, k) & mask)) continue;
+    const int oz = k >> 2, kl = 2 * (k & 3);
+    float sum = 0.f;
+#pragma unroll
+    for (int cid = 0; cid < 8; cid++) {
+      const int ii = cid & 1, jj = (cid >> 1) & 1, kk = (cid >> 2) & 1;
+      if (fc[oz][ii][jj][kl + kk] & mask) sum += fv[oz][ii][jj][kl + kk];
+    }
+    x[k] = x[k] + sum * coeff;
+  }
+  store8(X + base, x);
+}
+
+// AddGradientToFaceCenters (PoissonSolver.cu:144-204) on LEAF|GHOST tiles
+// (ghost p already taken from the parents): ghost velocities start at 0; each
+// cell's minus face of each axis gets u += (p1 - p0) / dh with p1, p0 the
+// INTERIOR values (0 elsewhere) of the cell and its minus neighbour (0 outside
+// the grid, where the neighbour counts as the tile's own type and the cell's
+// own cell type), dh = h across LEAF-LEAF faces and 1.5 h across LEAF-GHOST /
+// GHOST-LEAF faces, no update across other faces or where either cell is NEUMANN.
+__global__ __launch_bounds__(64) void k_gradient(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ P,
+                                                 Vel U) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t self = size_t(T.self) * 512;
+  const size_t base = self + lane * 8;
+  const uint8_t t0 = tr_type(T);
+  const float h = T.h;
+  const float dh_lg = float(1.5 * double(h));
+  uint8_t c0[8];
+  float p1[8];
+  load8c(c.ctype + base, c0);
+  load8(P + base, p1);
+  // minus neighbours: pencil (i-1,j), pencil (i,j-1), cell (i,j,-1)
+  float px[8], py[8], pz = 0.f;
+  uint8_t cx[8], cy[8], cz = 0;
+  bool ex = false, ey = false, ez = false;  // outside the grid
+  {
+    const int nx = T.nbr[0], ny = T.nbr[1], nz = T.nbr[2];
+    if (i > 0) { load8(P + base - 64, px); load8c(c.ctype + base - 64, cx); }
+    else if (nx >= 0) { load8(P + size_t(nx) * 512 + (56 + j) * 8, px); load8c(c.ctype + size_t(nx) * 512 + (56 + j) * 8, cx); }
+    else ex = true;
+    if (j > 0) { load8(P + base - 8, py); load8c(c.ctype + base - 8, cy); }
+    else if (ny >= 0) { load8(P + size_t(ny) * 512 + (i * 8 + 7) * 8, py); load8c(c.ctype + size_t(ny) * 512 + (i * 8 + 7) * 8, cy); }
+    else ey = true;
+    if (nz >= 0) { pz = P[size_t(nz) * 512 + lane * 8 + 7]; cz = c.ctype[size_t(nz) * 512 + lane * 8 + 7]; }
+    else ez = true;
+  }
+  const uint8_t sx = (i == 0 && T.nbr[0] >= 0) ? tr_ntype(T, 0) : t0;
+  const uint8_t sy = (j == 0 && T.nbr[1] >= 0) ? tr_ntype(T, 1) : t0;
+  const uint8_t sz = (T.nbr[2] >= 0) ? tr_ntype(T, 2) : t0;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    float u[8];
+    if (t0 == T_GHOST) { for (int k = 0; k < 8; k++) u[k] = 0.f; }
+    else load8(U.u[a] + base, u);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      bool outside;
+      uint8_t nt, nc;
+      float pn;
+      if (a == 0) { outside = (i == 0) && ex; nt = sx; nc = outside ? c0[k] : cx[k]; pn = outside ? 0.f : px[k]; }
+      else if (a == 1) { outside = (j == 0) && ey; nt = sy; nc = outside ? c0[k] : cy[k]; pn = outside ? 0.f : py[k]; }
+      else if (k > 0) { outside = false; nt = t0; nc = c0[k - 1]; pn = p1[k - 1]; }
+      else { outside = ez; nt = sz; nc = outside ? c0[0] : cz; pn = outside ? 0.f : pz; }
+      const float pa = (c0[k] & C_INT) ? p1[k] : 0.f;
+      const float pb = (nc & C_INT) ? pn : 0.f;
+      const bool ll = (t0 & T_LEAF) && (nt & T_LEAF);
+      const bool lg = ((t0 & T_LEAF) && (nt & T_GHOST)) || ((t0 & T_GHOST) && (nt & T_LEAF));
+      if (!(ll || lg) || (c0[k] & C_NEU) || (nc & C_NEU)) continue;
+      u[k] = u[k] + (pa - pb) / (ll ? h : dh_lg);
+    }
+    store8(U.u[a] + base, u);
+  }
+}
+
 }  // namespace cg

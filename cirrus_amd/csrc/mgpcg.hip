@@ -97,6 +97,7 @@ struct cg_problem {
   float* d_bsave = nullptr;
   double* d_psum = nullptr;
   float* d_api[3] = {nullptr, nullptr, nullptr};  // colour-split copies of channels for the operator API
+  float* d_vel[3] = {nullptr, nullptr, nullptr};  // face velocities u, v, w (channels 6-8, allocated on first use)
   float* d_gsum = nullptr;
   ProbState* d_ps = nullptr;
   int32_t* d_dflag = nullptr;
@@ -152,6 +153,7 @@ cg_problem::~cg_problem() {
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
   for (auto q : d_api) if (q) (void)hipFree(q);
+  for (auto q : d_vel) if (q) (void)hipFree(q);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -465,6 +467,19 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
 
 void check_channel(int ch) {
   if (ch < 0 || ch >= CG_NUM_CHANNELS) fail(CG_ERR_INVALID, "channel index out of range");
+}
+// Any channel the caller may move: 0-5, or the face velocities 6-8 (allocated
+// and zeroed on first use).
+float* any_channel(cg_problem& P, int ch) {
+  if (ch >= 0 && ch < CG_NUM_CHANNELS) return P.d_ch[ch];
+  if (ch < CG_CH_U || ch > CG_CH_W) fail(CG_ERR_INVALID, "channel index out of range");
+  if (!P.owner) fail(CG_ERR_INVALID, "velocity channels need an owning problem handle");
+  float*& v = P.d_vel[ch - CG_CH_U];
+  if (!v) {
+    HIPC(hipMalloc(&v, size_t(P.n) * 512 * sizeof(float)));
+    HIPC(hipMemsetAsync(v, 0, size_t(P.n) * 512 * sizeof(float), P.stream));
+  }
+  return v;
 }
 
 // upload every problem's ProbState (synchronous)
@@ -910,6 +925,34 @@ void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& 
   }
 }
 
+// ---------------------------------------------------------------------------
+// The projection around the solve (FluidEuler::project, FluidEuler.h:581-612)
+// ---------------------------------------------------------------------------
+Vel velocities(cg_problem& P) {
+  Vel U;
+  for (int a = 0; a < 3; a++) U.u[a] = any_channel(P, CG_CH_U + a);
+  return U;
+}
+// [ghost velocities from the parents,] divergence into DIV, ghost sums into
+// their LEAF parents (VelocityVolumeDivergenceOnLeafs, PoissonSolver.cu:66-142)
+void velocity_divergence(cg_problem& P, float* DIV, bool propagate) {
+  const Ctx c = P.ctx();
+  const Vel U = velocities(P);
+  if (propagate && P.ghosts.n) k_propagate_vel<<<P.ghosts.n, 64, 0, P.stream>>>(c, P.L(P.ghosts), P.ghosts.n, U);
+  k_divergence<<<P.applyl.n, 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, U, DIV);
+  k_accumulate_children<<<P.leaf.n, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, DIV, C_INT, 1.0f);
+}
+// ghost p from the parents, u += grad p, ghost velocities into their LEAF
+// parents with weight 1/4 (AddGradientToFaceCenters, PoissonSolver.cu:144-209)
+void add_gradient(cg_problem& P, float* Pch) {
+  const Ctx c = P.ctx();
+  const Vel U = velocities(P);
+  if (P.ghosts.n) k_propagate_ghosts<<<P.ghosts.n, 64, 0, P.stream>>>(c, P.L(P.ghosts), P.ghosts.n, Pch, Pch);
+  k_gradient<<<P.applyl.n, 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, Pch, U);
+  for (int a = 0; a < 3; a++)
+    k_accumulate_children<<<P.leaf.n, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, U.u[a], C_INT | C_DIR, 1.0f / 4.0f);
+}
+
 }  // namespace
 
 #include "serve.hpp"
@@ -1018,10 +1061,10 @@ cg_status cg_problem_topology(const cg_problem* p, int32_t* nbr, int32_t* parent
 cg_status cg_set_channel(cg_problem* p, int32_t channel, const float* host) {
   return guard([&] {
     if (!p || !host) fail(CG_ERR_INVALID, "cg_set_channel: null argument");
-    check_channel(channel);
+    float* dst = any_channel(*p, channel);
     std::vector<float> buf(size_t(p->n) * 512);
     for (int i = 0; i < p->n; i++) std::memcpy(&buf[size_t(i) * 512], host + size_t(p->ext_of[i]) * 512, 512 * sizeof(float));
-    HIPC(hipMemcpyAsync(p->d_ch[channel], buf.data(), buf.size() * sizeof(float), hipMemcpyHostToDevice, p->stream));
+    HIPC(hipMemcpyAsync(dst, buf.data(), buf.size() * sizeof(float), hipMemcpyHostToDevice, p->stream));
     HIPC(hipStreamSynchronize(p->stream));
   });
 }
@@ -1029,11 +1072,42 @@ cg_status cg_set_channel(cg_problem* p, int32_t channel, const float* host) {
 cg_status cg_get_channel(cg_problem* p, int32_t channel, float* host) {
   return guard([&] {
     if (!p || !host) fail(CG_ERR_INVALID, "cg_get_channel: null argument");
-    check_channel(channel);
+    const float* src = any_channel(*p, channel);
     std::vector<float> buf(size_t(p->n) * 512);
-    HIPC(hipMemcpyAsync(buf.data(), p->d_ch[channel], buf.size() * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipMemcpyAsync(buf.data(), src, buf.size() * sizeof(float), hipMemcpyDeviceToHost, p->stream));
     HIPC(hipStreamSynchronize(p->stream));
     for (int i = 0; i < p->n; i++) std::memcpy(host + size_t(p->ext_of[i]) * 512, &buf[size_t(i) * 512], 512 * sizeof(float));
+  });
+}
+
+cg_status cg_velocity_divergence(cg_problem* p, int32_t div_channel, int32_t propagate_ghosts) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    check_channel(div_channel);
+    velocity_divergence(*p, p->d_ch[div_channel], propagate_ghosts != 0);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+cg_status cg_add_gradient(cg_problem* p, int32_t p_channel) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    check_channel(p_channel);
+    add_gradient(*p, p->d_ch[p_channel]);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+cg_status cg_project(cg_problem* p, const cg_params* prm, cg_result* results) {
+  return guard([&] {
+    if (!p || !prm || !results) fail(CG_ERR_INVALID, "cg_project: null argument");
+    velocity_divergence(*p, p->d_ch[CG_CH_B], true);
+    solve(*p, *prm, results);
+    add_gradient(*p, p->d_ch[CG_CH_X]);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(p->stream));
   });
 }
 

@@ -31,6 +31,9 @@ extern "C" {
 enum { CG_LEAF = 1, CG_NONLEAF = 2, CG_GHOST = 4 };
 enum { CG_DIRICHLET = 1, CG_INTERIOR = 2, CG_NEUMANN = 4 };
 enum { CG_CH_X = 0, CG_CH_B = 1, CG_CH_R = 1, CG_CH_P = 2, CG_CH_AP = 3, CG_CH_Z = 4, CG_CH_TMP = 5, CG_NUM_CHANNELS = 6 };
+/* Face velocities at the cells' minus faces (src/PoissonTile.h:12-24, channels u, v, w = 6, 7, 8);
+ * moved with cg_set_channel / cg_get_channel, allocated (zero) on first use. */
+enum { CG_CH_U = 6, CG_CH_V = 7, CG_CH_W = 8 };
 enum { CG_SMOOTH_RBGS = 0, CG_SMOOTH_JACOBI = 1 };
 /* Storage of the V-cycle vectors (its x on every level, its rhs and scratch);
  * arithmetic is FP32 either way.  BF16 rounds each stored value to nearest
@@ -122,7 +125,8 @@ cg_status cg_problem_info(const cg_problem* p, int32_t* num_tiles, int32_t* num_
  * reference's tile.mNeighbors (src/PoissonGrid.cu:121-134); parent[t] likewise. */
 cg_status cg_problem_topology(const cg_problem* p, int32_t* nbr, int32_t* parent);
 
-/* Channel transfer (host <-> device), caller's tile order, 512 floats per tile. */
+/* Channel transfer (host <-> device), caller's tile order, 512 floats per tile;
+ * channels 0-5 and the face velocities 6-8. */
 cg_status cg_set_channel(cg_problem* p, int32_t channel, const float* host);
 cg_status cg_get_channel(cg_problem* p, int32_t channel, float* host);
 
@@ -163,6 +167,21 @@ cg_status cg_solve_refined(cg_problem* p, const cg_params* prm, const cg_refine_
 /* The FP64 x of the last cg_solve_refined (caller's tile order, 512 per tile;
  * LEAF tiles carry the solution). */
 cg_status cg_get_solution_f64(cg_problem* p, double* host);
+
+/* ---- The projection around the solve (src/FluidEuler.h:581-612) ---------- */
+/* VelocityVolumeDivergenceOnLeafs (src/PoissonSolver.cu:66-142) into div_channel: with
+ * propagate_ghosts, the ghost velocities are first taken from the parents (PropagateToChildren,
+ * src/PoissonGrid.cu:635-661, as FluidEuler::project does); INTERIOR cells of LEAF|GHOST tiles
+ * sum u h^2 over LEAF-LEAF and LEAF-GHOST faces, ghost sums are added to their LEAF parents. */
+cg_status cg_velocity_divergence(cg_problem* p, int32_t div_channel, int32_t propagate_ghosts);
+/* AddGradientToFaceCenters (src/PoissonSolver.cu:144-209): ghost p from the parents, u += grad p
+ * on the minus faces of LEAF|GHOST tiles (1/h, 1/(1.5 h) across coarse/fine faces, none across
+ * NEUMANN cells), ghost velocities added to their LEAF parents with weight 1/4. */
+cg_status cg_add_gradient(cg_problem* p, int32_t p_channel);
+/* FluidEuler::project without the velocity boundary conditions: divergence of the face
+ * velocities into CG_CH_B, MGPCG solve (every problem of a batched handle) into CG_CH_X, the
+ * pressure gradient added to the face velocities; results[num_problems]. */
+cg_status cg_project(cg_problem* p, const cg_params* prm, cg_result* results);
 
 /* GMGSolver::dampedJacobiSolve (src/GMGSolver.cu:449-529). */
 cg_status cg_damped_jacobi_solve(cg_problem* p, int32_t max_iters, double rel_tol, float omega, cg_result* res);

@@ -716,7 +716,14 @@ void SceneRHS(Problem<T>& P) {
       }
     }
   }
-  // PoissonGrid.cu:635-661  PropagateToChildren(u+a, GHOST, INTERIOR|DIRICHLET|NEUMANN), coarse first
+  PropagateVelocityToGhosts(G, CH_U);
+  VelocityVolumeDivergenceOnLeafs(G, CH_U, CH_B);
+}
+
+// FluidEuler.h:582-585: PropagateToChildren(u+a, u+a, -1, GHOST, SUBTREE,
+// INTERIOR|DIRICHLET|NEUMANN) (PoissonGrid.cu:635-661), coarse levels first.
+template <class T>
+void PropagateVelocityToGhosts(Grid<T>& G, int u_ch) {
   const uint8_t ctypes = INTERIOR | DIRICHLET | NEUMANN;
   for (int axis = 0; axis < 3; axis++)
     for (int level = 0; level < G.numLayers; level++) {
@@ -734,42 +741,21 @@ void SceneRHS(Problem<T>& P) {
           T val = T(0);
           if (!ci.empty()) {
             auto& ct = G.tile(ci);
-            val = ct.ch[CH_U + axis][localOffset(cl)];
+            val = ct.ch[u_ch + axis][localOffset(cl)];
             if (!(ct.ctype[localOffset(cl)] & ctypes)) val = T(0);
           }
-          t.ch[CH_U + axis][vi] = val;
+          t.ch[u_ch + axis][vi] = val;
         }
       }
     }
-  // VelocityVolumeDivergenceOnLeafs (PoissonSolver.cu:72-137), LEAF|GHOST tiles
-#pragma omp parallel for schedule(static)
-  for (long ti = 0; ti < long(all.size()); ti++) {
-    const Info info = all[ti];
-    if (!(info.type & (LEAF | GHOST))) continue;
-    auto& t = G.tile(info);
-    T h = G.voxelSize(info.level);
-    uint8_t ttype = info.subtreeType(-1);
-    for (int vi = 0; vi < 512; vi++) {
-      if (!(t.ctype[vi] & INTERIOR)) { t.ch[CH_B][vi] = T(0); continue; }
-      Coord l = localCoord(vi), gc = composeGlobal(info.coord, l);
-      T sum = T(0);
-      for (int axis = 0; axis < 3; axis++)
-        for (int sgn : {-1, 1}) {
-          Coord n = gc; n[axis] += sgn;
-          Info ni; Coord nl;
-          G.findVoxel(info.level, n, ni, nl);
-          uint8_t nt;
-          T u0 = t.ch[CH_U + axis][vi], u1;
-          if (ni.empty()) { nt = ttype; u1 = T(0); }
-          else { nt = ni.subtreeType(-1); u1 = G.tile(ni).ch[CH_U + axis][localOffset(nl)]; }
-          if (!((ttype == LEAF && nt == LEAF) || (ttype == LEAF && nt == GHOST) || (ttype == GHOST && nt == LEAF))) continue;
-          T u = (sgn == -1) ? -u0 : u1;
-          sum = std::fma(u * h, h, sum);
-        }
-      t.ch[CH_B][vi] = sum;
-    }
-  }
-  // AccumulateToParents(b, b, -1, LEAF, SUBTREE, INTERIOR, 1, additive) (PoissonGrid.cu:721-755)
+}
+
+// AccumulateToParents(fine, coarse, -1, LEAF, SUBTREE, cell_types, coeff, additive)
+// (PoissonGrid.cu:721-755): on LEAF tiles, each cell whose type is in
+// cell_types gets coeff times the sum of its 8 children (x-fastest child
+// order) whose type is in cell_types; finest levels first.
+template <class T>
+void AccumulateLeafChildren(Grid<T>& G, int ch, uint8_t cell_types, T coeff) {
   for (int level = G.numLayers - 1; level >= 0; level--) {
     const auto& arr = G.arrays[level];
 #pragma omp parallel for schedule(static)
@@ -778,7 +764,7 @@ void SceneRHS(Problem<T>& P) {
       if (!(info.type & LEAF)) continue;
       auto& t = G.tile(info);
       for (int vi = 0; vi < 512; vi++) {
-        if (!(t.ctype[vi] & INTERIOR)) continue;
+        if (!(t.ctype[vi] & cell_types)) continue;
         Coord gc = composeGlobal(info.coord, localCoord(vi));
         T sum = T(0);
         for (int c = 0; c < 8; c++) {
@@ -786,13 +772,130 @@ void SceneRHS(Problem<T>& P) {
           G.findVoxel(level + 1, childCoord(gc, childIndexToOffset(c)), fi, fl);
           if (!fi.empty()) {
             auto& ft = G.tile(fi);
-            if (ft.ctype[localOffset(fl)] & INTERIOR) sum += ft.ch[CH_B][localOffset(fl)];
+            if (ft.ctype[localOffset(fl)] & cell_types) sum += ft.ch[ch][localOffset(fl)];
           }
         }
-        t.ch[CH_B][vi] += sum * T(1);
+        t.ch[ch][vi] += sum * coeff;
       }
     }
   }
+}
+
+// VelocityVolumeDivergenceOnLeafs (PoissonSolver.cu:66-142): on LEAF|GHOST
+// tiles, INTERIOR cells sum their face fluxes u h^2 over LEAF-LEAF and
+// LEAF-GHOST faces (faces x-, x+, y-, y+, z-, z+; the flux is -u of the cell's
+// own minus face, +u of the plus neighbour's), other cells 0; then the ghost
+// cells' sums are added to their LEAF parents (INTERIOR cells, weight 1).
+template <class T>
+void VelocityVolumeDivergenceOnLeafs(Grid<T>& G, int u_ch, int div_ch) {
+  const auto& all = G.all;
+#pragma omp parallel for schedule(static)
+  for (long ti = 0; ti < long(all.size()); ti++) {
+    const Info info = all[ti];
+    if (!(info.type & (LEAF | GHOST))) continue;
+    auto& t = G.tile(info);
+    T h = G.voxelSize(info.level);
+    uint8_t ttype = info.subtreeType(-1);
+    for (int vi = 0; vi < 512; vi++) {
+      if (!(t.ctype[vi] & INTERIOR)) { t.ch[div_ch][vi] = T(0); continue; }
+      Coord l = localCoord(vi), gc = composeGlobal(info.coord, l);
+      T sum = T(0);
+      for (int axis = 0; axis < 3; axis++)
+        for (int sgn : {-1, 1}) {
+          Coord n = gc; n[axis] += sgn;
+          Info ni; Coord nl;
+          G.findVoxel(info.level, n, ni, nl);
+          uint8_t nt;
+          T u0 = t.ch[u_ch + axis][vi], u1;
+          if (ni.empty()) { nt = ttype; u1 = T(0); }
+          else { nt = ni.subtreeType(-1); u1 = G.tile(ni).ch[u_ch + axis][localOffset(nl)]; }
+          if (!((ttype == LEAF && nt == LEAF) || (ttype == LEAF && nt == GHOST) || (ttype == GHOST && nt == LEAF))) continue;
+          T u = (sgn == -1) ? -u0 : u1;
+          sum = std::fma(u * h, h, sum);
+        }
+      t.ch[div_ch][vi] = sum;
+    }
+  }
+  AccumulateLeafChildren(G, div_ch, INTERIOR, T(1));
+}
+
+// AddGradientToFaceCenters (PoissonSolver.cu:144-209): ghost p from the
+// parents' INTERIOR values (PropagateValues, PoissonGrid.cu:614-634); on
+// LEAF|GHOST tiles (ghost velocities first set to 0) each cell's minus face
+// of each axis gets u += (p1 - p0) / dh, p1 / p0 the INTERIOR values (0
+// elsewhere) of the cell and its minus neighbour (0 outside the grid), dh = h
+// across LEAF-LEAF faces and 1.5 h across LEAF-GHOST faces, nothing across
+// other faces or faces touching a NEUMANN cell; then the ghost velocities are
+// added to their LEAF parents with weight 1/4 (INTERIOR|DIRICHLET cells).
+template <class T>
+void AddGradientToFaceCenters(Grid<T>& G, int p_ch, int u_ch) {
+  const auto& all = G.all;
+  // PropagateValues(p, p, -1, GHOST, SUBTREE): coarse levels first
+  for (int level = 0; level < G.numLayers; level++) {
+    const auto& arr = G.arrays[level];
+#pragma omp parallel for schedule(static)
+    for (long ti = 0; ti < long(arr.size()); ti++) {
+      const Info info = arr[ti];
+      if (!(info.type & GHOST)) continue;
+      auto& t = G.tile(info);
+      for (int vi = 0; vi < 512; vi++) {
+        if (!(t.ctype[vi] & INTERIOR)) { t.ch[p_ch][vi] = T(0); continue; }
+        Coord gc = composeGlobal(info.coord, localCoord(vi));
+        Info ci; Coord cl;
+        G.findVoxel(level - 1, parentCoord(gc), ci, cl);
+        T v = T(0);
+        if (!ci.empty()) {
+          auto& ct = G.tile(ci);
+          v = (ct.ctype[localOffset(cl)] & INTERIOR) ? ct.ch[p_ch][localOffset(cl)] : T(0);
+        }
+        t.ch[p_ch][vi] = v;
+      }
+    }
+  }
+  std::vector<std::array<T, 3 * 512>> out(all.size());
+#pragma omp parallel for schedule(static)
+  for (long ti = 0; ti < long(all.size()); ti++) {
+    const Info info = all[ti];
+    if (!(info.type & (LEAF | GHOST))) continue;
+    auto& t = G.tile(info);
+    const T h = G.voxelSize(info.level);
+    const uint8_t ttype = info.subtreeType(-1);
+    for (int vi = 0; vi < 512; vi++) {
+      Coord l = localCoord(vi), gc = composeGlobal(info.coord, l);
+      const uint8_t ctype = t.ctype[vi];
+      const T p1 = (ctype & INTERIOR) ? t.ch[p_ch][vi] : T(0);
+      for (int axis = 0; axis < 3; axis++) {
+        T u = (ttype == GHOST) ? T(0) : t.ch[u_ch + axis][vi];
+        Coord n = gc; n[axis] -= 1;
+        Info ni; Coord nl;
+        G.findVoxel(info.level, n, ni, nl);
+        uint8_t nt, nc;
+        T p0;
+        if (ni.empty()) { nt = ttype; nc = ctype; p0 = T(0); }
+        else {
+          auto& nt_ = G.tile(ni);
+          nt = ni.subtreeType(-1);
+          nc = nt_.ctype[localOffset(nl)];
+          p0 = (nc & INTERIOR) ? nt_.ch[p_ch][localOffset(nl)] : T(0);
+        }
+        T dh;
+        bool ok = true;
+        if ((ttype & LEAF) && (nt & LEAF)) dh = h;
+        else if (((ttype & LEAF) && (nt & GHOST)) || ((ttype & GHOST) && (nt & LEAF))) dh = T(1.5 * double(h));
+        else ok = false;
+        if (ok && !((ctype & NEUMANN) || (nc & NEUMANN))) u += (p1 - p0) / dh;
+        out[ti][axis * 512 + vi] = u;
+      }
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (long ti = 0; ti < long(all.size()); ti++) {
+    const Info info = all[ti];
+    if (!(info.type & (LEAF | GHOST))) continue;
+    auto& t = G.tile(info);
+    for (int axis = 0; axis < 3; axis++) std::memcpy(t.ch[u_ch + axis], &out[ti][axis * 512], 512 * sizeof(T));
+  }
+  for (int axis = 0; axis < 3; axis++) AccumulateLeafChildren(G, u_ch + axis, INTERIOR | DIRICHLET, T(1.0 / 4.0));
 }
 
 // ---------------------------------------------------------------------------

@@ -214,6 +214,20 @@ int orc_solve(void* h, int max_iters, double tol, int level_iters, int coarsest_
     return r.iters;
   });
 }
+// The projection's neighbours of the solve (FluidEuler.h:581-612): ghost
+// velocities from the parents, the volume divergence into div_ch, and the
+// pressure gradient added to the face velocities (channels 6-8).
+void orc_velocity_divergence(void* h, int div_ch, int propagate) {
+  with(h, [&](auto& P) {
+    if (propagate) PropagateVelocityToGhosts(P.g(), CH_U);
+    VelocityVolumeDivergenceOnLeafs(P.g(), CH_U, div_ch);
+    return 0;
+  });
+}
+void orc_add_gradient(void* h, int p_ch) {
+  with(h, [&](auto& P) { AddGradientToFaceCenters(P.g(), p_ch, CH_U); return 0; });
+}
+
 // Mixed-precision iterative refinement on an FP32 handle (SolveRefined).
 // out[0]=relative FP64 residual, [1]=|b|^2, [2]=|r|^2, [3]=CG iterations,
 // [4]=converged, [5]=wall seconds.  Returns the number of refinement steps;

@@ -125,6 +125,8 @@ def lib():
         L.orc_get_refined_x.restype = i
         L.orc_get_refined_x.argtypes = [vp, vp]
         L.orc_bf16_round.argtypes = [vp, vp, ctypes.c_long]
+        L.orc_velocity_divergence.argtypes = [vp, i, i]
+        L.orc_add_gradient.argtypes = [vp, i]
         L.orc_damped_jacobi_solve.restype = i
         L.orc_damped_jacobi_solve.argtypes = [vp, i, d, d, vp]
         L.orc_set_threads.argtypes = [i]
@@ -236,6 +238,15 @@ class Oracle:
                                     int(pure_neumann), smoother, omega, int(sync_stride), int(storage), _ptr(out))
         return {"outer": k, "iters": int(out[3]), "converged": int(out[4]), "rel_res": out[0], "bb": out[1],
                 "rr": out[2], "seconds": out[5]}
+
+    def velocity_divergence(self, div_ch=CH_B, propagate=True):
+        """[Ghost velocities from the parents,] then the volume divergence of
+        channels 6-8 into div_ch (VelocityVolumeDivergenceOnLeafs)."""
+        lib().orc_velocity_divergence(self.h, div_ch, int(propagate))
+
+    def add_gradient(self, p_ch=CH_X):
+        """u += grad p on the face velocities (AddGradientToFaceCenters)."""
+        lib().orc_add_gradient(self.h, p_ch)
 
     def refined_x(self):
         a = np.zeros((self.n, 512), np.float64)
