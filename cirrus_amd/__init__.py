@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = (
     "cg_server_create", "cg_server_destroy", "cg_server_admit", "cg_server_release", "cg_server_submit", "cg_server_step",
     "cg_server_next_result", "cg_server_info", "cg_server_perturb_rhs",
     "cg_refine_params_default", "cg_solve_refined", "cg_get_solution_f64",
-    "cg_velocity_divergence", "cg_add_gradient", "cg_project",
+    "cg_velocity_divergence", "cg_add_gradient", "cg_project", "cg_set_schedule", "cg_autotune",
 )
 
 
@@ -78,6 +78,14 @@ class RefineResult(ctypes.Structure):
     _fields_ = [("outer", ctypes.c_int32), ("iters", ctypes.c_int32), ("converged", ctypes.c_int32),
                 ("rel_res", ctypes.c_double), ("bb", ctypes.c_double), ("rr", ctypes.c_double),
                 ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_int32), ("base_seconds", ctypes.c_double), ("best_seconds", ctypes.c_double),
+                ("coarse_shift", ctypes.c_int32), ("swizzle", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -158,6 +166,8 @@ def lib():
             "cg_velocity_divergence": (i32, [vp, i32, i32]),
             "cg_add_gradient": (i32, [vp, i32]),
             "cg_project": (i32, [vp, P(Params), P(Result)]),
+            "cg_set_schedule": (i32, [vp, i32, i32]),
+            "cg_autotune": (i32, [vp, P(Params), i32, P(Params), P(TuneReport)]),
             "cg_solve_refined": (i32, [vp, P(Params), P(RefineParams), P(RefineResult)]),
             "cg_get_solution_f64": (i32, [vp, vp]),
             "cg_scene_create": (i32, [P(SceneParams), P(vp)]),
@@ -343,6 +353,11 @@ class Problem:
         _check(lib().cg_get_channel(self._h, ch, _ptr(a)), "cg_get_channel")
         return a
 
+    def set_schedule(self, coarse_shift=0, swizzle=True):
+        """Kernel coverage only, never a result: LDS-resident coarse levels given
+        back to grid-wide launches, XCD-aware tile order (cg_set_schedule)."""
+        _check(lib().cg_set_schedule(self._h, int(coarse_shift), int(bool(swizzle))), "cg_set_schedule")
+
     def get_f64(self):
         """The FP64 x of the last refined solve, [tiles, 512] float64."""
         a = np.zeros((self.n, 512), np.float64)
@@ -490,6 +505,21 @@ class GMGSolver:
         res = (Result * prob.num_problems)()
         _check(lib().cg_project(prob.handle, ctypes.byref(p), res), "GMGSolver.project")
         return [r.as_dict() for r in res]
+
+    def autotune(self, prob, max_iters=1000, relative_tolerance=1e-6, level_iters=1, coarsest_iters=10,
+                 smoother=SMOOTH_RBGS, storage=STORE_F32, allow_bf16=False):
+        """cg_autotune on the problem's current right-hand sides: returns (the
+        chosen solver keyword arguments, the report); the handle keeps the
+        chosen schedule."""
+        p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
+                           coarsest_iters=coarsest_iters, smoother=smoother, storage=int(storage),
+                           one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff)
+        best = Params()
+        rep = TuneReport()
+        _check(lib().cg_autotune(prob.handle, ctypes.byref(p), int(bool(allow_bf16)), ctypes.byref(best),
+                                 ctypes.byref(rep)), "GMGSolver.autotune")
+        kw = dict(level_iters=best.level_iters, smoother=best.smoother, storage=best.storage)
+        return kw, rep.as_dict()
 
     def dampedJacobiSolve(self, prob, verbose=False, max_iters=100, relative_tolerance=1e-6, omega=2.0 / 3.0):
         r = Result()

@@ -83,6 +83,12 @@ struct cg_problem {
   std::vector<DList> actg, lvlg; // per level, tiles the V-cycle smooths with grid-wide launches
   std::vector<CoarsePlan> plans; // LDS-resident coarse levels, one per problem with C >= 0
   std::vector<CoarseSlot> coarse_slots;
+  // schedule (no effect on results): XCD-aware tile order, LDS-resident coarse
+  // levels given up to grid-wide launches
+  int swz = F_SWIZZLE;
+  int coarse_shift = 0;
+  std::vector<int> lev, tbase, coarse_auto;
+  std::vector<uint8_t> ty;
 
   // device
   uint8_t* d_ctype = nullptr;
@@ -340,45 +346,17 @@ void coarse_plan(const std::vector<TileRec>& tiles, int base, int C, int prob, C
   }
 }
 
-void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
-  if (!descs || nprob <= 0) fail(CG_ERR_INVALID, "cg_problem_create: no tree description");
-  P.nprob = nprob;
-  std::vector<int> tbase(nprob + 1, 0);
-  for (int q = 0; q < nprob; q++) {
-    if (descs[q].num_tiles <= 0) fail(CG_ERR_INVALID, "cg_problem_create: empty or incomplete tree description");
-    tbase[q + 1] = tbase[q] + descs[q].num_tiles;
-  }
-  const int n = tbase[nprob];
-  P.n = n;
-  P.ext_of.assign(n, 0);
-  P.tiles.assign(n, TileRec());
-  std::vector<int> lev(n);
-  std::vector<uint8_t> ty(n);
-  P.leaf_begin.assign(nprob, 0);
-  P.leaf_count.assign(nprob, 0);
-  P.coarse_C.assign(nprob, -1);
-  int leaf_total = 0, ghost_total = 0;
-  for (int q = 0; q < nprob; q++) {
-    TreeBuild B;
-    build_tree(&descs[q], q, tbase[q], leaf_total, ghost_total, B);
-    for (int i = 0; i < descs[q].num_tiles; i++) {
-      P.tiles[tbase[q] + i] = B.tiles[i];
-      P.ext_of[tbase[q] + i] = tbase[q] + B.ext[i];
-      lev[tbase[q] + i] = B.lev[i];
-      ty[tbase[q] + i] = B.ty[i];
-    }
-    P.leaf_begin[q] = leaf_total;
-    P.leaf_count[q] = B.nleaf;
-    P.coarse_C[q] = coarse_depth(B.lev, B.ty);
-    leaf_total += B.nleaf;
-    ghost_total += B.nghost;
-  }
-  P.n_leaf = leaf_total;
-  P.n_ghost = ghost_total;
-  P.max_level = 0;
-  for (int i = 0; i < n; i++) P.max_level = std::max(P.max_level, lev[i]);
+// Worklists and coarse plans of a pool for its schedule: each problem keeps
+// its deepest LDS-resident coarse level coarse_auto - coarse_shift (-1: none),
+// the levels above it go to grid-wide launches.  (Re)uploads the lists and
+// plans; the captured iteration is dropped.
+void build_schedule(cg_problem& P) {
+  const int n = P.n, nprob = P.nprob;
+  const std::vector<int>& lev = P.lev;
+  const std::vector<uint8_t>& ty = P.ty;
+  const std::vector<int>& tbase = P.tbase;
+  for (int q = 0; q < nprob; q++) P.coarse_C[q] = std::max(-1, P.coarse_auto[q] - P.coarse_shift);
   const int L = P.max_level;
-
   // worklists (canonical order, problems one after another)
   auto& W = P.lists;
   W.clear();
@@ -419,6 +397,73 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
     P.plans.push_back(cp);
   }
 
+  HIPC(hipStreamSynchronize(P.stream));  // no kernel may still read the old lists
+  if (P.d_lists) HIPC(hipFree(P.d_lists));
+  if (P.d_coarse) HIPC(hipFree(P.d_coarse));
+  if (P.d_plans) HIPC(hipFree(P.d_plans));
+  P.d_lists = nullptr;
+  P.d_coarse = nullptr;
+  P.d_plans = nullptr;
+  HIPC(hipMalloc(&P.d_lists, sizeof(TileRec) * W.size()));
+  HIPC(hipMemcpyAsync(P.d_lists, W.data(), sizeof(TileRec) * W.size(), hipMemcpyHostToDevice, P.stream));
+  if (!P.plans.empty()) {
+    HIPC(hipMalloc(&P.d_coarse, sizeof(CoarseSlot) * P.coarse_slots.size()));
+    HIPC(hipMemcpyAsync(P.d_coarse, P.coarse_slots.data(), sizeof(CoarseSlot) * P.coarse_slots.size(),
+                        hipMemcpyHostToDevice, P.stream));
+    for (auto& cp : P.plans) cp.slots = P.d_coarse + size_t(cp.slots);
+    HIPC(hipMalloc(&P.d_plans, sizeof(CoarsePlan) * P.plans.size()));
+    HIPC(hipMemcpyAsync(P.d_plans, P.plans.data(), sizeof(CoarsePlan) * P.plans.size(), hipMemcpyHostToDevice, P.stream));
+  }
+  HIPC(hipStreamSynchronize(P.stream));
+  if (P.graph) HIPC(hipGraphExecDestroy(P.graph));
+  P.graph = nullptr;
+  P.graph_key.clear();
+}
+
+void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
+  if (!descs || nprob <= 0) fail(CG_ERR_INVALID, "cg_problem_create: no tree description");
+  P.nprob = nprob;
+  std::vector<int> tbase(nprob + 1, 0);
+  for (int q = 0; q < nprob; q++) {
+    if (descs[q].num_tiles <= 0) fail(CG_ERR_INVALID, "cg_problem_create: empty or incomplete tree description");
+    tbase[q + 1] = tbase[q] + descs[q].num_tiles;
+  }
+  const int n = tbase[nprob];
+  P.n = n;
+  P.ext_of.assign(n, 0);
+  P.tiles.assign(n, TileRec());
+  std::vector<int> lev(n);
+  std::vector<uint8_t> ty(n);
+  P.leaf_begin.assign(nprob, 0);
+  P.leaf_count.assign(nprob, 0);
+  P.coarse_C.assign(nprob, -1);
+  int leaf_total = 0, ghost_total = 0;
+  for (int q = 0; q < nprob; q++) {
+    TreeBuild B;
+    build_tree(&descs[q], q, tbase[q], leaf_total, ghost_total, B);
+    for (int i = 0; i < descs[q].num_tiles; i++) {
+      P.tiles[tbase[q] + i] = B.tiles[i];
+      P.ext_of[tbase[q] + i] = tbase[q] + B.ext[i];
+      lev[tbase[q] + i] = B.lev[i];
+      ty[tbase[q] + i] = B.ty[i];
+    }
+    P.leaf_begin[q] = leaf_total;
+    P.leaf_count[q] = B.nleaf;
+    P.coarse_C[q] = coarse_depth(B.lev, B.ty);
+    leaf_total += B.nleaf;
+    ghost_total += B.nghost;
+  }
+  P.n_leaf = leaf_total;
+  P.n_ghost = ghost_total;
+  P.max_level = 0;
+  for (int i = 0; i < n; i++) P.max_level = std::max(P.max_level, lev[i]);
+  const int L = P.max_level;
+
+  P.lev = lev;
+  P.ty = ty;
+  P.tbase = tbase;
+  P.coarse_auto = P.coarse_C;
+
   // device buffers
   HIPC(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
   const size_t cells = size_t(n) * 512;
@@ -441,16 +486,7 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   HIPC(hipMalloc(&P.d_part, np * sizeof(double)));
   HIPC(hipMalloc(&P.d_part2, np * sizeof(double)));
   HIPC(hipMalloc(&P.d_scalar, 8 * sizeof(double)));
-  HIPC(hipMalloc(&P.d_lists, sizeof(TileRec) * W.size()));
-  HIPC(hipMemcpyAsync(P.d_lists, W.data(), sizeof(TileRec) * W.size(), hipMemcpyHostToDevice, P.stream));
-  if (!P.plans.empty()) {
-    HIPC(hipMalloc(&P.d_coarse, sizeof(CoarseSlot) * P.coarse_slots.size()));
-    HIPC(hipMemcpyAsync(P.d_coarse, P.coarse_slots.data(), sizeof(CoarseSlot) * P.coarse_slots.size(),
-                        hipMemcpyHostToDevice, P.stream));
-    for (auto& cp : P.plans) cp.slots = P.d_coarse + size_t(cp.slots);
-    HIPC(hipMalloc(&P.d_plans, sizeof(CoarsePlan) * P.plans.size()));
-    HIPC(hipMemcpyAsync(P.d_plans, P.plans.data(), sizeof(CoarsePlan) * P.plans.size(), hipMemcpyHostToDevice, P.stream));
-  }
+  build_schedule(P);
   std::vector<uint8_t> ct(cells);
   for (int i = 0; i < n; i++) {
     const int q = P.tiles[i].prob;
@@ -561,7 +597,7 @@ void ensure_bf16(cg_problem& P) {
 template <class V>
 void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v) {
   const Ctx c = P.ctx();
-  const int base = F_VCYCLE | kSwizzle | (v.check_done ? F_CHECK_DONE : 0);
+  const int base = F_VCYCLE | P.swz | (v.check_done ? F_CHECK_DONE : 0);
   const int L = P.max_level;  // a problem's levels above its coarse_C use the grid-wide lists actg / lvlg
   V* ZB = pool_zb<V>(P);
   for (int l = L; l >= 1; l--) {
@@ -637,7 +673,7 @@ void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v) {
 template <class V>
 void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann) {
   const Ctx c = P.ctx();
-  const int fl = kSwizzle | F_CHECK_DONE;
+  const int fl = P.swz | F_CHECK_DONE;
   const unsigned np = unsigned(P.nprob);  // one finalize workgroup per problem
   if (pure_neumann) {
     k_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, fl);
@@ -783,15 +819,15 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
   if (maxit_q) for (int q = 0; q < P.nprob; q++) max_iters = std::max(max_iters, maxit_q[q]);
   const Ctx c = P.ctx();
   if (prm.storage == CG_STORE_BF16)
-    k_cg_init<bf16><<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, kSwizzle);
+    k_cg_init<bf16><<<grid_for(P.leaf.n, P.swz), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.swz);
   else
-    k_cg_init<float><<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, kSwizzle);
+    k_cg_init<float><<<grid_for(P.leaf.n, P.swz), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.swz);
   k_finalize<<<unsigned(P.nprob), FIN_THREADS, 0, P.stream>>>(c, FIN_BB, 0);
   HIPC(hipGetLastError());
 
   char keybuf[160];
-  std::snprintf(keybuf, sizeof(keybuf), "%d/%d/%d/%a/%a/%a/%d/%d", v.nu, v.coarsest, v.smoother, double(v.omega), double(v.ooa),
-                double(v.pcoef), prm.pure_neumann, prm.storage);
+  std::snprintf(keybuf, sizeof(keybuf), "%d/%d/%d/%a/%a/%a/%d/%d/%d", v.nu, v.coarsest, v.smoother, double(v.omega), double(v.ooa),
+                double(v.pcoef), prm.pure_neumann, prm.storage, P.swz);
   const bool use_graph = prm.use_graph != 0;
   if (use_graph && P.graph_key != keybuf) {
     if (P.graph) HIPC(hipGraphExecDestroy(P.graph));
@@ -922,6 +958,92 @@ void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& 
     r.rr = bb[q] > 0 ? rr[q] : 0.0;
     r.rel_res = bb[q] > 0 ? std::sqrt(rr[q] / bb[q]) : 0.0;
     r.seconds = std::chrono::duration<double>(t1 - t0).count();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: schedules timed on the pool's own right-hand sides
+// ---------------------------------------------------------------------------
+void set_schedule(cg_problem& P, int coarse_shift, int swizzle) {
+  if (coarse_shift < 0 || coarse_shift > CL_MAX_LEVELS) fail(CG_ERR_INVALID, "cg_set_schedule: coarse_shift out of range");
+  if (!P.owner) fail(CG_ERR_INVALID, "cg_set_schedule: not an owning handle");
+  P.swz = swizzle ? F_SWIZZLE : 0;
+  if (coarse_shift != P.coarse_shift) {
+    P.coarse_shift = coarse_shift;
+    build_schedule(P);
+  }
+  if (P.graph) HIPC(hipGraphExecDestroy(P.graph));
+  P.graph = nullptr;
+  P.graph_key.clear();
+}
+
+// Time candidate schedules on the pool's current right-hand sides (restored
+// before every solve): first the ones that leave every result unchanged (LDS
+// coarse depth, tile order) with the caller's smoother, then the smoother,
+// sweeps per level and (if allowed) bfloat16 storage with the best of those.
+// A candidate counts only if every problem converges; each is timed as the
+// best of two solves after a warm-up (graph capture).  The pool keeps the
+// chosen schedule; *best receives the chosen solver parameters.
+void autotune(cg_problem& P, const cg_params& base, int allow_bf16, cg_params* best, cg_tune_report* rep) {
+  check_params(base);
+  const size_t bytes = size_t(P.n) * 512 * sizeof(float);
+  float* bkeep = nullptr;
+  HIPC(hipMalloc(&bkeep, bytes));
+  HIPC(hipMemcpyAsync(bkeep, P.d_ch[CG_CH_R], bytes, hipMemcpyDeviceToDevice, P.stream));
+  std::vector<cg_result> res(P.nprob);
+  int candidates = 0;
+  auto run = [&](const cg_params& prm) {
+    double t = std::numeric_limits<double>::infinity();
+    for (int rep_i = 0; rep_i < 3; rep_i++) {
+      HIPC(hipMemcpyAsync(P.d_ch[CG_CH_R], bkeep, bytes, hipMemcpyDeviceToDevice, P.stream));
+      solve(P, prm, res.data());
+      for (const cg_result& r : res)
+        if (!r.converged) return std::numeric_limits<double>::infinity();
+      if (rep_i > 0) t = std::min(t, res[0].seconds);
+    }
+    candidates++;
+    return t;
+  };
+  const int shift0 = P.coarse_shift, swz0 = P.swz ? 1 : 0;
+  set_schedule(P, 0, 1);
+  const double t_base = run(base);
+  double t_best = t_base;
+  int best_shift = 0, best_swz = 1;
+  int max_auto = -1;
+  for (int q = 0; q < P.nprob; q++) max_auto = std::max(max_auto, P.coarse_auto[q]);
+  for (int shift = 0; shift <= std::min(2, max_auto + 1); shift++)
+    for (int swz = 1; swz >= 0; swz--) {
+      if (shift == 0 && swz == 1) continue;
+      set_schedule(P, shift, swz);
+      const double t = run(base);
+      if (t < t_best) { t_best = t; best_shift = shift; best_swz = swz; }
+    }
+  set_schedule(P, best_shift, best_swz);
+  cg_params chosen = base;
+  struct Cand { int smoother, nu; };
+  const Cand cands[] = {{CG_SMOOTH_RBGS, 1}, {CG_SMOOTH_RBGS, 2}, {CG_SMOOTH_RBGS, 3}, {CG_SMOOTH_JACOBI, 2}, {CG_SMOOTH_JACOBI, 3}};
+  for (int storage = CG_STORE_F32; storage <= (allow_bf16 ? CG_STORE_BF16 : CG_STORE_F32); storage++)
+    for (const Cand& cd : cands) {
+      cg_params prm = base;
+      prm.smoother = cd.smoother;
+      prm.level_iters = cd.nu;
+      prm.storage = storage;
+      if (prm.smoother == base.smoother && prm.level_iters == base.level_iters && prm.storage == base.storage) continue;
+      const double t = run(prm);
+      if (t < t_best) { t_best = t; chosen = prm; }
+    }
+  HIPC(hipMemcpyAsync(P.d_ch[CG_CH_R], bkeep, bytes, hipMemcpyDeviceToDevice, P.stream));
+  HIPC(hipStreamSynchronize(P.stream));
+  HIPC(hipFree(bkeep));
+  (void)shift0;
+  (void)swz0;
+  *best = chosen;
+  if (rep) {
+    rep->candidates = candidates;
+    rep->base_seconds = t_base;
+    rep->best_seconds = t_best;
+    rep->coarse_shift = best_shift;
+    rep->swizzle = best_swz;
   }
 }
 
@@ -1077,6 +1199,20 @@ cg_status cg_get_channel(cg_problem* p, int32_t channel, float* host) {
     HIPC(hipMemcpyAsync(buf.data(), src, buf.size() * sizeof(float), hipMemcpyDeviceToHost, p->stream));
     HIPC(hipStreamSynchronize(p->stream));
     for (int i = 0; i < p->n; i++) std::memcpy(host + size_t(p->ext_of[i]) * 512, &buf[size_t(i) * 512], 512 * sizeof(float));
+  });
+}
+
+cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    set_schedule(*p, coarse_shift, swizzle);
+  });
+}
+
+cg_status cg_autotune(cg_problem* p, const cg_params* base, int32_t allow_bf16, cg_params* best, cg_tune_report* report) {
+  return guard([&] {
+    if (!p || !base || !best) fail(CG_ERR_INVALID, "cg_autotune: null argument");
+    autotune(*p, *base, allow_bf16, best, report);
   });
 }
 

@@ -168,6 +168,28 @@ cg_status cg_solve_refined(cg_problem* p, const cg_params* prm, const cg_refine_
  * LEAF tiles carry the solution). */
 cg_status cg_get_solution_f64(cg_problem* p, double* host);
 
+/* ---- Schedules and the autotuner (no reference counterpart) ---------------
+ * A schedule changes how the kernels cover the tiles, never a result:
+ * coarse_shift = how many of each problem's LDS-resident coarse levels (as many
+ * as fit by default) go back to grid-wide launches; swizzle = XCD-aware tile
+ * order (1, default) or the plain order (0). */
+cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle);
+typedef struct {
+  int32_t candidates;        /* parameter sets timed */
+  double base_seconds;       /* one solve with the caller's parameters, default schedule */
+  double best_seconds;       /* one solve with the chosen schedule and parameters */
+  int32_t coarse_shift;      /* the chosen schedule (kept by the handle) */
+  int32_t swizzle;
+} cg_tune_report;
+/* Times solves of the handle's current right-hand sides (restored afterwards)
+ * under candidate schedules (coarse_shift 0-2, swizzle on/off) with the caller's
+ * smoother, then under candidate smoother settings (RB-GS with 1-3 sweeps per
+ * level, damped Jacobi with 2-3; with allow_bf16 also bfloat16 V-cycle storage),
+ * keeping the fastest whose every problem converges.  The handle keeps the
+ * chosen schedule; *best receives the chosen parameters (base with smoother,
+ * level_iters and storage replaced). */
+cg_status cg_autotune(cg_problem* p, const cg_params* base, int32_t allow_bf16, cg_params* best, cg_tune_report* report);
+
 /* ---- The projection around the solve (src/FluidEuler.h:581-612) ---------- */
 /* VelocityVolumeDivergenceOnLeafs (src/PoissonSolver.cu:66-142) into div_channel: with
  * propagate_ghosts, the ghost velocities are first taken from the parents (PropagateToChildren,
