@@ -180,23 +180,33 @@ def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=64, nscenes=8, budg
     }
 
 
-def run_batched(ca, nprob, reps, solver_kw, use_graph=True, storage=0, refine=False):
+def run_batched(ca, nprob, reps, solver_kw, use_graph=True, storage=0, refine=False, tune=False):
     """Solve nprob independent problems together; returns the form's record.
     storage: V-cycle storage (0 FP32, 1 BF16).  refine: FP64 iterative
     refinement to 1e-6 of the FP64 residual (inner MGPCG to 5e-7); otherwise
     the MGPCG to 1e-6 of its recursive residual, each pressure then checked in
-    FP64."""
+    FP64.  tune: the autotuner picks the schedule and the smoother settings
+    (bfloat16 storage allowed) on the batch's own right-hand sides first."""
     t0 = time.perf_counter()
     scenes = [ca.Scene(**batch_scene_params(q)[0]) for q in range(nprob)]
     t_build = time.perf_counter() - t0
     B = ca.Problem.batch([sc.tree for sc in scenes])
     b_all = np.concatenate([np.ascontiguousarray(sc.rhs, np.float32) for sc in scenes])
     S = ca.GMGSolver(1.0, 1.0)
+    kw = dict(level_iters=solver_kw["level_iters"], smoother=solver_kw["smoother"], storage=storage)
+    tune_rep = None
+    if tune:
+        B.set(ca.CH_B, b_all)
+        kw, tune_rep = S.autotune(B, 1000, 1e-6, solver_kw["level_iters"], solver_kw["coarsest_iters"],
+                                  smoother=solver_kw["smoother"], storage=storage, allow_bf16=True)
 
     def once():
         B.set(ca.CH_B, b_all)
         t = time.perf_counter()
-        if refine:
+        if tune:
+            res = S.solve_batch(B, 1000, 1e-6, coarsest_iters=solver_kw["coarsest_iters"], use_graph=use_graph,
+                                verify_f64=True, **kw)
+        elif refine:
             res = S.solve_refined(B, 1e-6, 5e-7, 20, 1000, solver_kw["level_iters"], solver_kw["coarsest_iters"],
                                   smoother=solver_kw["smoother"], use_graph=use_graph, storage=storage)
         else:
@@ -216,12 +226,16 @@ def run_batched(ca, nprob, reps, solver_kw, use_graph=True, storage=0, refine=Fa
     total = float(np.sum(times))
     # the V-cycle's dominant kernel on the pool's most populated level
     lv = int(np.bincount(np.concatenate([sc.tree.level[sc.tree.type != 4] for sc in scenes])).argmax())
-    kid = ca.KERNEL_GS_SWEEP_BF16 if storage else ca.KERNEL_GS_SWEEP
+    kid = ca.KERNEL_GS_SWEEP_BF16 if kw["storage"] else ca.KERNEL_GS_SWEEP
     gus, gtiles = B.benchmark_kernel(kid, lv, reps=20)
-    gbytes = gs_alg_bytes(gtiles, 2 if storage else 4)
-    prec = "BF16 V-cycle storage" if storage else "FP32"
+    gbytes = gs_alg_bytes(gtiles, 2 if kw["storage"] else 4)
+    prec = "BF16 V-cycle storage" if kw["storage"] else "FP32"
     how = "FP64 iterative refinement to 1e-6 of the FP64 residual" if refine else \
         "tol 1e-6 each (recursive residual), every pressure checked in FP64"
+    if tune:
+        sm = "RB-GS" if kw["smoother"] == 0 else "damped Jacobi"
+        how = (f"schedule and smoother chosen by the autotuner ({sm} {kw['level_iters']}+{kw['level_iters']}, "
+               f"coarse shift {tune_rep['coarse_shift']}, XCD order {tune_rep['swizzle']}), " + how)
     return {
         "workload": f"{nprob} independent adaptive problems (sphere band, lognormal leaf-tile counts), "
                     f"{prec}, RB-GS 2+2, {how}, solved together",
@@ -233,6 +247,7 @@ def run_batched(ca, nprob, reps, solver_kw, use_graph=True, storage=0, refine=Fa
         "fp64_rel_res_max": float(np.max(f64)),
         "fp64_over_tol": int(np.sum(np.array(f64) >= 1e-6)),
         "outer_max": int(max(r["outer"] for r in res)) if refine else None,
+        "autotune": tune_rep,
         "gs_sweep": {"level": lv, "tiles": gtiles, "us": gus, "alg_bytes": gbytes,
                      "GBps": gbytes / (gus * 1e-6) / 1e9, "frac_hbm_peak": gbytes / (gus * 1e-6) / 1e9 / PEAK_HBM_GBS},
     }
@@ -658,9 +673,10 @@ def main():
         "ms_per_projection": float(np.median(pt)) * 1e3, "projections_per_s": len(pt) / float(np.sum(pt)),
         "solve_ms": out["ms_per_step"], "iterations": pr["iters"]}
     if args.batch > 0:
-        for name, storage, refine in [("batched", 0, False), ("batched_mixed", 1, True)]:
+        for name, storage, refine, tune in [("batched", 0, False, False), ("batched_mixed", 1, True, False),
+                                            ("batched_autotuned", 0, False, True)]:
             f = run_batched(ca, args.batch, args.batch_reps, solver_kw, use_graph=not args.no_graph, storage=storage,
-                            refine=refine)
+                            refine=refine, tune=tune)
             f["vs_plain_solves_per_s"] = f["solves_per_s"] / out["value"]
             forms[name] = f
     if args.serve_seconds > 0:
@@ -675,8 +691,8 @@ def main():
         forms["refined_vortex512"] = f
     if forms:
         out["forms"] = forms
-        out["mixed_vs_plain"] = {k: forms[k]["vs_plain_solves_per_s"] for k in ("batched_mixed", "served_mixed")
-                                 if k in forms}
+        out["mixed_vs_plain"] = {k: forms[k]["vs_plain_solves_per_s"]
+                                 for k in ("batched_mixed", "served_mixed", "batched_autotuned") if k in forms}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene_kw, solver_kw)
         if not args.no_vortex:
