@@ -79,6 +79,39 @@ def batch_scene_params(q, seed=0):
     return kw, target
 
 
+def build_scenes(ca, kws, workers=16):
+    """Build the synthetic scenes of the given keyword sets on the host, in
+    parallel (the library's builder is re-entrant and releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max(1, min(workers, len(kws)))) as ex:
+        return list(ex.map(lambda kw: ca.Scene(**kw), kws))
+
+
+def plain_drop_in(ca, items, solver_kw, reps=1):
+    """The plain FP32 drop-in on the given problems: one handle per problem,
+    solved one after another from x = 0 (as GMGSolver::solve starts,
+    src/GMGSolver.cu:537) to 1e-6 of the recursive residual; one warm-up pass
+    captures each handle's graph.  Returns solves/s and the latency spread."""
+    S = ca.GMGSolver(1.0, 1.0)
+    probs = [ca.Problem(t) for t, _ in items]
+    lat = []
+    for rep_i in range(reps + 1):
+        for P, (_, b) in zip(probs, items):
+            P.set(ca.CH_B, b)
+            t = time.perf_counter()
+            S.solve(P, False, 1000, 1e-6, solver_kw["level_iters"], solver_kw["coarsest_iters"], 1, False,
+                    smoother=solver_kw["smoother"])
+            dt = time.perf_counter() - t
+            assert S.last["converged"] == 1
+            if rep_i > 0:
+                lat.append(dt)
+    for P in probs:
+        P.close()
+    lat_ms = np.array(lat) * 1e3
+    return {"problems": len(items), "solves_per_s": len(lat) / float(np.sum(lat)),
+            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99))}
+
+
 def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=64, nscenes=8, budget=24 << 30, rho=0.95, steps=50,
                storage=0, refine=False):
     """Serving form: requests arrive as a Poisson process of rate `lam` for
@@ -93,10 +126,8 @@ def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=64, nscenes=8, budg
     iterative refinement warm-started from the instance's FP64 pressure and
     reported once |b - A x| / |b| < 1e-6 in FP64."""
     import collections
-    trees_b0 = []
-    for q in range(nscenes):
-        sc = ca.Scene(**batch_scene_params(q, seed=1)[0])
-        trees_b0.append((sc.tree, np.ascontiguousarray(sc.rhs, np.float32)))
+    scenes = build_scenes(ca, [batch_scene_params(q, seed=1)[0] for q in range(nscenes)])
+    trees_b0 = [(sc.tree, np.ascontiguousarray(sc.rhs, np.float32)) for sc in scenes]
     S = ca.Server(budget, conc, level_iters=solver_kw["level_iters"], coarsest_iters=solver_kw["coarsest_iters"],
                   smoother=solver_kw["smoother"], storage=int(storage), refine=bool(refine))
     insts, sig = [], []
@@ -177,20 +208,19 @@ def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=64, nscenes=8, budg
         "fp64_rel_res_max": float(np.max(f64)) if (refine and f64) else None,
         "fp64_checked": len(f64) if refine else 0,
         "fp64_over_tol": int(np.sum(np.array(f64) >= 1e-6)) if refine else None,
+        "_items": trees_b0,
     }
 
 
-def run_batched(ca, nprob, reps, solver_kw, use_graph=True, storage=0, refine=False, tune=False):
+def run_batched(ca, scenes, B, reps, solver_kw, use_graph=True, storage=0, refine=False, tune=False, t_build=0.0):
     """Solve nprob independent problems together; returns the form's record.
     storage: V-cycle storage (0 FP32, 1 BF16).  refine: FP64 iterative
     refinement to 1e-6 of the FP64 residual (inner MGPCG to 5e-7); otherwise
     the MGPCG to 1e-6 of its recursive residual, each pressure then checked in
     FP64.  tune: the autotuner picks the schedule and the smoother settings
-    (bfloat16 storage allowed) on the batch's own right-hand sides first."""
-    t0 = time.perf_counter()
-    scenes = [ca.Scene(**batch_scene_params(q)[0]) for q in range(nprob)]
-    t_build = time.perf_counter() - t0
-    B = ca.Problem.batch([sc.tree for sc in scenes])
+    (bfloat16 storage allowed) on the batch's own right-hand sides first.
+    scenes: the built scenes; B: their batched handle."""
+    nprob = len(scenes)
     b_all = np.concatenate([np.ascontiguousarray(sc.rhs, np.float32) for sc in scenes])
     S = ca.GMGSolver(1.0, 1.0)
     kw = dict(level_iters=solver_kw["level_iters"], smoother=solver_kw["smoother"], storage=storage)
@@ -204,8 +234,8 @@ def run_batched(ca, nprob, reps, solver_kw, use_graph=True, storage=0, refine=Fa
         B.set(ca.CH_B, b_all)
         t = time.perf_counter()
         if tune:
-            res = S.solve_batch(B, 1000, 1e-6, coarsest_iters=solver_kw["coarsest_iters"], use_graph=use_graph,
-                                verify_f64=True, **kw)
+            res = S.solve_refined(B, 1e-6, 5e-7, 20, 1000, kw["level_iters"], solver_kw["coarsest_iters"],
+                                  smoother=kw["smoother"], use_graph=use_graph, storage=kw["storage"])
         elif refine:
             res = S.solve_refined(B, 1e-6, 5e-7, 20, 1000, solver_kw["level_iters"], solver_kw["coarsest_iters"],
                                   smoother=solver_kw["smoother"], use_graph=use_graph, storage=storage)
@@ -222,7 +252,7 @@ def run_batched(ca, nprob, reps, solver_kw, use_graph=True, storage=0, refine=Fa
         assert all(r["converged"] == 1 for r in res)
     leaf = [int((sc.tree.type == 1).sum()) for sc in scenes]
     iters = [r["iters"] for r in res]
-    f64 = [r["rel_res"] if refine else r["rel_res_f64"] for r in res]
+    f64 = [r["rel_res"] if (refine or tune) else r["rel_res_f64"] for r in res]
     total = float(np.sum(times))
     # the V-cycle's dominant kernel on the pool's most populated level
     lv = int(np.bincount(np.concatenate([sc.tree.level[sc.tree.type != 4] for sc in scenes])).argmax())
@@ -230,15 +260,16 @@ def run_batched(ca, nprob, reps, solver_kw, use_graph=True, storage=0, refine=Fa
     gus, gtiles = B.benchmark_kernel(kid, lv, reps=20)
     gbytes = gs_alg_bytes(gtiles, 2 if kw["storage"] else 4)
     prec = "BF16 V-cycle storage" if kw["storage"] else "FP32"
-    how = "FP64 iterative refinement to 1e-6 of the FP64 residual" if refine else \
+    how = "FP64 iterative refinement to 1e-6 of the FP64 residual" if (refine or tune) else \
         "tol 1e-6 each (recursive residual), every pressure checked in FP64"
     if tune:
         sm = "RB-GS" if kw["smoother"] == 0 else "damped Jacobi"
         how = (f"schedule and smoother chosen by the autotuner ({sm} {kw['level_iters']}+{kw['level_iters']}, "
-               f"coarse shift {tune_rep['coarse_shift']}, XCD order {tune_rep['swizzle']}), " + how)
+               f"coarse shift {tune_rep['coarse_shift']}, XCD order {tune_rep['swizzle']}, "
+               f"tile block {tune_rep['tile_block']}), " + how)
     return {
         "workload": f"{nprob} independent adaptive problems (sphere band, lognormal leaf-tile counts), "
-                    f"{prec}, RB-GS 2+2, {how}, solved together",
+                    f"{prec}, {'' if tune else 'RB-GS 2+2, '}{how}, solved together",
         "problems": nprob, "leaf_tiles_total": int(np.sum(leaf)), "leaf_tiles_median": int(np.median(leaf)),
         "leaf_tiles_min": int(np.min(leaf)), "leaf_tiles_max": int(np.max(leaf)),
         "solves_per_s": nprob * reps / total, "ms_per_batch": total / reps * 1e3,
@@ -246,7 +277,7 @@ def run_batched(ca, nprob, reps, solver_kw, use_graph=True, storage=0, refine=Fa
         "iterations_max": int(np.max(iters)), "build_s": t_build, "reps": reps,
         "fp64_rel_res_max": float(np.max(f64)),
         "fp64_over_tol": int(np.sum(np.array(f64) >= 1e-6)),
-        "outer_max": int(max(r["outer"] for r in res)) if refine else None,
+        "outer_max": int(max(r["outer"] for r in res)) if (refine or tune) else None,
         "autotune": tune_rep,
         "gs_sweep": {"level": lv, "tiles": gtiles, "us": gus, "alg_bytes": gbytes,
                      "GBps": gbytes / (gus * 1e-6) / 1e9, "frac_hbm_peak": gbytes / (gus * 1e-6) / 1e9 / PEAK_HBM_GBS},
@@ -380,15 +411,23 @@ def gs_alg_bytes(tiles, s=4):
     return tiles * 512 * (2 * s + 1)
 
 
-def kernels_only(P, ca):
-    """The dominant kernel (FP32 and BF16 storage) and the copy reference,
-    KERNEL_REPS launches each (the workload for hardware-counter runs)."""
-    L = P.max_level
-    us, tiles = P.benchmark_kernel(ca.KERNEL_GS_SWEEP, L, reps=KERNEL_REPS)
-    bus, _ = P.benchmark_kernel(ca.KERNEL_GS_SWEEP_BF16, L, reps=KERNEL_REPS)
+def kernels_only(P, ca, level=None, blocks=(1,)):
+    """The dominant kernel (FP32 and BF16 storage; for each tile block given)
+    and the copy reference, KERNEL_REPS launches each (the workload for
+    hardware-counter runs)."""
+    L = P.max_level if level is None else level
+    out = {"gs_level": L}
+    for blk in blocks:
+        P.set_schedule(0, True, blk)
+        us, tiles = P.benchmark_kernel(ca.KERNEL_GS_SWEEP, L, reps=KERNEL_REPS)
+        bus, _ = P.benchmark_kernel(ca.KERNEL_GS_SWEEP_BF16, L, reps=KERNEL_REPS)
+        out.update({"gs_tiles": tiles, f"gs_us_block{blk}": us, f"gs_bf16_us_block{blk}": bus,
+                    f"gs_GBps_block{blk}": gs_alg_bytes(tiles) / (us * 1e-6) / 1e9,
+                    f"gs_bf16_GBps_block{blk}": gs_alg_bytes(tiles, 2) / (bus * 1e-6) / 1e9})
+    P.set_schedule(0, True, 1)
     cus, ctiles = P.benchmark_kernel(ca.KERNEL_COPY, 0, reps=KERNEL_REPS)
-    print(json.dumps({"gs_level": L, "gs_tiles": tiles, "gs_us": us, "gs_bf16_us": bus, "copy_tiles": ctiles,
-                      "copy_us": cus}), flush=True)
+    out.update({"copy_tiles": ctiles, "copy_us": cus})
+    print(json.dumps(out), flush=True)
 
 
 def _kname(raw):
@@ -423,61 +462,69 @@ def _pmc_per_dispatch(path, counter):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
-def summarize_pmc(fetch_csv, write_csv):
-    """Scale FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the dominant kernel by
-    the copy kernel's reading of its known traffic; write the profile summary."""
+def summarize_pmc(fetch_csv, write_csv, title=None, out=None):
+    """Scale FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the colour sweeps by
+    the copy kernel's reading of its known traffic; write the profile summary.
+    Default: the adaptive 256^3 workload's summary, whose traffic line feeds
+    the roofline record."""
     fetch = _pmc_per_dispatch(fetch_csv, "FETCH_SIZE")
     write = _pmc_per_dispatch(write_csv, "WRITE_SIZE")
 
-    def pick(d, name):
+    def pick(d, name, required=True):
         ks = [k for k in d if k[0] == name]
         if not ks:
-            raise SystemExit(f"no {name} dispatches in the counter file")
+            if required:
+                raise SystemExit(f"no {name} dispatches in the counter file")
+            return None
         k = max(ks, key=lambda k: k[1])
         return k, d[k]
 
-    (gk, gf), (_, gw) = pick(fetch, "k_gs_color<float>"), pick(write, "k_gs_color<float>")
-    (bk, bf), (_, bw) = pick(fetch, "k_gs_color<bf16>"), pick(write, "k_gs_color<bf16>")
     (ck, cf), (_, cw) = pick(fetch, "k_copy_tiles"), pick(write, "k_copy_tiles")
     copy_tiles = ck[1] // 64
     copy_bytes = copy_tiles * 512 * 4
     f_scale = copy_bytes / (cf * 1024.0)
     w_scale = copy_bytes / (cw * 1024.0)
-    gs_tiles = gk[1] // 64
-    rd = gf * 1024.0 * f_scale
-    wr = gw * 1024.0 * w_scale
-    alg = gs_alg_bytes(gs_tiles)
-    brd = bf * 1024.0 * f_scale
-    bwr = bw * 1024.0 * w_scale
-    balg = gs_alg_bytes(gs_tiles, 2)
+    default = title is None
+    title = title or "the adaptive 256^3 workload"
+    extra = "" if default else " --kernels-batch N"
     lines = [
-        "HBM traffic of k_gs_color (red-black colour sweep, finest level) on the adaptive 256^3 workload",
+        f"HBM traffic of the red-black colour sweeps (most populated level) on {title}",
         "",
         "commands:",
-        "  rocprofv3 --pmc FETCH_SIZE -d OUT/fetch -o pmc --output-format csv -- python3 bench.py --kernels-only",
-        "  rocprofv3 --pmc WRITE_SIZE -d OUT/write -o pmc --output-format csv -- python3 bench.py --kernels-only",
-        "  python bench.py --summarize-pmc OUT/fetch/pmc_counter_collection.csv OUT/write/pmc_counter_collection.csv",
+        f"  rocprofv3 --pmc FETCH_SIZE -d OUT/fetch -o pmc --output-format csv -- python3 bench.py --kernels-only{extra}",
+        f"  rocprofv3 --pmc WRITE_SIZE -d OUT/write -o pmc --output-format csv -- python3 bench.py --kernels-only{extra}",
+        "  python bench.py --summarize-pmc OUT/fetch/pmc_counter_collection.csv OUT/write/pmc_counter_collection.csv"
+        + ("" if default else " --pmc-title '...' --pmc-out FILE"),
         "",
         f"copy reference k_copy_tiles: {copy_tiles} tiles, {copy_bytes} B read and {copy_bytes} B written per dispatch",
         f"  FETCH_SIZE {cf:.1f} KiB/dispatch -> read scale {f_scale:.4f}",
         f"  WRITE_SIZE {cw:.1f} KiB/dispatch -> write scale {w_scale:.4f}",
-        f"k_gs_color<float>: {gs_tiles} tiles (grid {gk[1]} threads)",
-        f"  FETCH_SIZE {gf:.1f} KiB/dispatch -> {rd:.0f} B read",
-        f"  WRITE_SIZE {gw:.1f} KiB/dispatch -> {wr:.0f} B written",
-        f"  algorithmic bytes per launch {alg} (2s+t per cell, s = 4)",
-        f"  traffic / algorithmic = {(rd + wr) / alg:.3f}",
-        f"k_gs_color<bf16> (BF16 V-cycle storage): {bk[1] // 64} tiles",
-        f"  FETCH_SIZE {bf:.1f} KiB/dispatch -> {brd:.0f} B read",
-        f"  WRITE_SIZE {bw:.1f} KiB/dispatch -> {bwr:.0f} B written",
-        f"  algorithmic bytes per launch {balg} (2s+t per cell, s = 2)",
-        f"  traffic / algorithmic = {(brd + bwr) / balg:.3f}",
-        "",
-        f"traffic_bytes_per_launch = {rd + wr:.0f}",
-        f"traffic_tiles = {gs_tiles}",
     ]
-    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
-    with open(TRAFFIC_FILE, "w") as f:
-        f.write("\n".join(lines) + "\n")
+    main_traffic = main_tiles = None
+    for name, s_bytes, per_wave in [("k_gs_color<float>", 4, 1), ("k_gs_color<bf16>", 2, 1),
+                                    ("k_gs_color2<float>", 4, 2), ("k_gs_color2<bf16>", 2, 2)]:
+        f = pick(fetch, name, required=per_wave == 1)
+        w = pick(write, name, required=per_wave == 1)
+        if f is None or w is None:
+            continue
+        (k, fv), (_, wv) = f, w
+        # the counter run launches each sweep over the same list; its tile count
+        # comes from the one-tile-per-wavefront grid
+        tiles = pick(fetch, "k_gs_color<float>")[0][1] // 64
+        rd, wr = fv * 1024.0 * f_scale, wv * 1024.0 * w_scale
+        alg = gs_alg_bytes(tiles, s_bytes)
+        lines += [f"{name}: {tiles} tiles, {per_wave} per wavefront (grid {k[1]} threads)",
+                  f"  FETCH_SIZE {fv:.1f} KiB/dispatch -> {rd:.0f} B read",
+                  f"  WRITE_SIZE {wv:.1f} KiB/dispatch -> {wr:.0f} B written",
+                  f"  algorithmic bytes per launch {alg} (2s+t per cell, s = {s_bytes})",
+                  f"  traffic / algorithmic = {(rd + wr) / alg:.3f}"]
+        if name == "k_gs_color<float>":
+            main_traffic, main_tiles = rd + wr, tiles
+    lines += ["", f"traffic_bytes_per_launch = {main_traffic:.0f}", f"traffic_tiles = {main_tiles}"]
+    path = out or TRAFFIC_FILE
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
 
@@ -547,22 +594,28 @@ def main():
     ap.add_argument("--workload", default="adaptive256", choices=list(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--batch", type=int, default=32,
+    ap.add_argument("--batch", type=int, default=256,
                     help="problems in the batched form measured beside the plain solve (0: skip)")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="length of the Poisson request window of the served form (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-rate-light", type=float, default=600.0,
+                    help="offered rate of the served_mixed_light form, below capacity (0: skip)")
     ap.add_argument("--serve-instances", type=int, default=64)
     ap.add_argument("--no-vortex", action="store_true",
                     help="skip the mixed-precision refinement form on the vortex-tube octree")
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only the dominant kernel and the copy reference (for counter runs)")
+    ap.add_argument("--kernels-batch", type=int, default=0,
+                    help="with --kernels-only: run the kernels on the pool of this many batched-form problems")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
+    ap.add_argument("--pmc-title", default=None, help="with --summarize-pmc: the workload the counters cover")
+    ap.add_argument("--pmc-out", default=None, help="with --summarize-pmc: the summary file to write")
     ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_CSV", "STATS_CSV"))
     args = ap.parse_args()
     if args.summarize_pmc:
-        summarize_pmc(*args.summarize_pmc)
+        summarize_pmc(*args.summarize_pmc, title=args.pmc_title, out=args.pmc_out)
         return
     if args.summarize_trace:
         summarize_trace(*args.summarize_trace)
@@ -583,7 +636,16 @@ def main():
     t_build = time.perf_counter() - t0
     P = ca.Problem(scene.tree)
     if args.kernels_only:
-        kernels_only(P, ca)
+        if args.kernels_batch > 0:
+            # the same kernels at batch scale, past the last-level cache: the
+            # pool of the batched form's problems, on its most populated level
+            scenes = build_scenes(ca, [batch_scene_params(q)[0] for q in range(args.kernels_batch)])
+            B = ca.Problem.batch([sc.tree for sc in scenes])
+            lv = int(np.bincount(np.concatenate([sc.tree.level[sc.tree.type != 4] for sc in scenes])).argmax())
+            del scenes
+            kernels_only(B, ca, level=lv, blocks=(1, 2))
+        else:
+            kernels_only(P, ca, blocks=(1, 2))
         return
     S = ca.GMGSolver(1.0, 1.0)
 
@@ -673,17 +735,43 @@ def main():
         "ms_per_projection": float(np.median(pt)) * 1e3, "projections_per_s": len(pt) / float(np.sum(pt)),
         "solve_ms": out["ms_per_step"], "iterations": pr["iters"]}
     if args.batch > 0:
+        t0 = time.perf_counter()
+        scenes = build_scenes(ca, [batch_scene_params(q)[0] for q in range(args.batch)])
+        t_build_b = time.perf_counter() - t0
+        items = [(sc.tree, np.ascontiguousarray(sc.rhs, np.float32)) for sc in scenes]
+        # the same problems through the plain drop-in, one at a time: the
+        # denominator of the batched forms' speed-up
+        base = plain_drop_in(ca, items, solver_kw)
+        forms["plain_drop_in_batch_problems"] = dict(base, workload="the batched forms' problems, FP32, one handle "
+                                                     "each, solved one after another from x = 0 (graph replay)")
+        B = ca.Problem.batch([sc.tree for sc in scenes])
         for name, storage, refine, tune in [("batched", 0, False, False), ("batched_mixed", 1, True, False),
                                             ("batched_autotuned", 0, False, True)]:
-            f = run_batched(ca, args.batch, args.batch_reps, solver_kw, use_graph=not args.no_graph, storage=storage,
-                            refine=refine, tune=tune)
-            f["vs_plain_solves_per_s"] = f["solves_per_s"] / out["value"]
+            f = run_batched(ca, scenes, B, args.batch_reps, solver_kw, use_graph=not args.no_graph, storage=storage,
+                            refine=refine, tune=tune, t_build=t_build_b)
+            f["vs_plain_solves_per_s"] = f["solves_per_s"] / base["solves_per_s"]
+            f["vs_plain_adaptive256"] = f["solves_per_s"] / out["value"]
             forms[name] = f
+        B.close()
+        del items, scenes
     if args.serve_seconds > 0:
-        for name, storage, refine in [("served", 0, False), ("served_mixed", 1, True)]:
-            f = run_served(ca, solver_kw, duration=args.serve_seconds, lam=args.serve_rate, conc=args.serve_instances,
+        base = None
+        for name, storage, refine, lam in [("served", 0, False, args.serve_rate),
+                                           ("served_mixed", 1, True, args.serve_rate),
+                                           ("served_mixed_light", 1, True, args.serve_rate_light)]:
+            if lam <= 0:
+                continue
+            f = run_served(ca, solver_kw, duration=args.serve_seconds, lam=lam, conc=args.serve_instances,
                            storage=storage, refine=refine)
-            f["vs_plain_solves_per_s"] = f["solves_per_s"] / out["value"]
+            items = f.pop("_items")
+            if base is None:
+                # the instances' problems through the plain drop-in, each solved
+                # cold as the reference's projection does
+                base = plain_drop_in(ca, items, solver_kw, reps=3)
+                forms["plain_drop_in_served_problems"] = dict(base, workload="the served instances' problems, FP32, "
+                                                              "one handle each, solved one after another from x = 0")
+            f["vs_plain_solves_per_s"] = f["solves_per_s"] / base["solves_per_s"]
+            f["vs_plain_adaptive256"] = f["solves_per_s"] / out["value"]
             forms[name] = f
     if not args.no_vortex:
         f = run_vortex(ca, solver_kw)

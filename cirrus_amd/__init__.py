@@ -85,7 +85,7 @@ class RefineResult(ctypes.Structure):
 
 class TuneReport(ctypes.Structure):
     _fields_ = [("candidates", ctypes.c_int32), ("base_seconds", ctypes.c_double), ("best_seconds", ctypes.c_double),
-                ("coarse_shift", ctypes.c_int32), ("swizzle", ctypes.c_int32)]
+                ("coarse_shift", ctypes.c_int32), ("swizzle", ctypes.c_int32), ("tile_block", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -353,10 +353,12 @@ class Problem:
         _check(lib().cg_get_channel(self._h, ch, _ptr(a)), "cg_get_channel")
         return a
 
-    def set_schedule(self, coarse_shift=0, swizzle=True):
+    def set_schedule(self, coarse_shift=0, swizzle=True, tile_block=1):
         """Kernel coverage only, never a result: LDS-resident coarse levels given
-        back to grid-wide launches, XCD-aware tile order (cg_set_schedule)."""
-        _check(lib().cg_set_schedule(self._h, int(coarse_shift), int(bool(swizzle))), "cg_set_schedule")
+        back to grid-wide launches, XCD-aware tile order, tiles per wavefront of
+        the colour sweeps (cg_set_schedule)."""
+        _check(lib().cg_set_schedule(self._h, int(coarse_shift), int(bool(swizzle)), int(tile_block)),
+               "cg_set_schedule")
 
     def get_f64(self):
         """The FP64 x of the last refined solve, [tiles, 512] float64."""

@@ -557,54 +557,87 @@ struct GsLDS {
 // colour's rhs half and the neighbours' other-colour quads; writes the updated
 // colour's half (and, from an implicit zero guess, zeros to the other half).
 // An INTERIOR cell with D == 0 is left unchanged.
+// The sweep is split into its loads (gs_load: every global load of the tile,
+// into registers) and the rest (gs_stage: the other-colour quads into LDS;
+// gs_finish: the update and its stores), so that a wavefront working several
+// tiles issues the loads of all of them before it waits on any.
+struct GsIn {
+  float4 xc, xo, B, halo;
+  uint64_t mask;
+  float xkm, xkp;
+};
 template <class V>
-__device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int level, V* X, const V* F, int color,
-                                        int flags, GsLDS& L) {
+__device__ __forceinline__ GsIn gs_load(const Ctx& c, const TileRec& T, const V* X, const V* F, int color, int flags) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const bool zero_all = flags & F_ZERO_X, ghost_zero = flags & F_GHOST_ZERO;
   const int hc = color << 8, ho = (color ^ 1) << 8;
-  V* tile = X + size_t(T.self) * 512;
+  const V* tile = X + size_t(T.self) * 512;
   const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  float4 xc = z4, xo = z4;
+  GsIn g;
+  g.xc = z4;
+  g.xo = z4;
   if (!zero_all) {
-    xc = load4(tile + hc + lane * 4);
-    xo = load4(tile + ho + lane * 4);
+    g.xc = load4(tile + hc + lane * 4);
+    g.xo = load4(tile + ho + lane * 4);
   }
-  const uint64_t mask = load_ct8(c.fmask + size_t(T.self) * 512 + lane * 8);
+  g.mask = load_ct8(c.fmask + size_t(T.self) * 512 + lane * 8);
   const V* src = (flags & F_VCYCLE) ? vc_rhs<V>(c) : F;
-  const float4 B = load4(src + size_t(T.self) * 512 + hc + lane * 4);
-  L.xo[(i + 1) * 10 + (j + 1)] = xo;
+  g.B = load4(src + size_t(T.self) * 512 + hc + lane * 4);
   // halo ring: lanes 0..31 fetch the other-colour quad of one neighbour pencil
-  //   g = 0: x- (pencil (7,t) of nbr 0), 1: x+ ((0,t) of nbr 3), 2: y- ((t,7) of nbr 1), 3: y+ ((t,0) of nbr 4)
+  //   q = 0: x- (pencil (7,t) of nbr 0), 1: x+ ((0,t) of nbr 3), 2: y- ((t,7) of nbr 1), 3: y+ ((t,0) of nbr 4)
+  g.halo = z4;  // x = 0 outside the grid (GMGSolver.cu:120-122)
   if (lane < 32) {
-    const int g = lane >> 3, t = lane & 7;
-    const int f = (g == 0) ? 0 : (g == 1) ? 3 : (g == 2) ? 1 : 4;
-    const int n = pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
-    const int src_pl = (g == 0) ? (56 + t) : (g == 1) ? t : (g == 2) ? (t * 8 + 7) : (t * 8);
-    const int dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
+    const int q = lane >> 3, t = lane & 7;
+    const int f = (q == 0) ? 0 : (q == 1) ? 3 : (q == 2) ? 1 : 4;
+    const int n = pick4(q, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
+    const int src_pl = (q == 0) ? (56 + t) : (q == 1) ? t : (q == 2) ? (t * 8 + 7) : (t * 8);
     const bool gh = tr_ntype(T, f) == T_GHOST;
-    float4 v = z4;  // x = 0 outside the grid (GMGSolver.cu:120-122)
-    if (n >= 0 && !(zero_all || (ghost_zero && gh))) v = load4(X + size_t(n) * 512 + ho + src_pl * 4);
-    L.xo[dst] = v;
+    if (n >= 0 && !(zero_all || (ghost_zero && gh))) g.halo = load4(X + size_t(n) * 512 + ho + src_pl * 4);
   }
   // k halos: cell (i,j,7) of the z- neighbour and (i,j,0) of the z+ neighbour
-  float xkm = 0.f, xkp = 0.f;
+  g.xkm = 0.f;
+  g.xkp = 0.f;
   {
     int n = T.nbr[2];
     bool gh = tr_ntype(T, 2) == T_GHOST;
-    if (n >= 0 && !(zero_all || (ghost_zero && gh))) xkm = to_f(X[size_t(n) * 512 + split_cell(i, j, 7)]);
+    if (n >= 0 && !(zero_all || (ghost_zero && gh))) g.xkm = to_f(X[size_t(n) * 512 + split_cell(i, j, 7)]);
     n = T.nbr[5];
     gh = tr_ntype(T, 5) == T_GHOST;
-    if (n >= 0 && !(zero_all || (ghost_zero && gh))) xkp = to_f(X[size_t(n) * 512 + split_cell(i, j, 0)]);
+    if (n >= 0 && !(zero_all || (ghost_zero && gh))) g.xkp = to_f(X[size_t(n) * 512 + split_cell(i, j, 0)]);
   }
+  return g;
+}
+__device__ __forceinline__ void gs_stage(const GsIn& g, GsLDS& L) {
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  L.xo[(i + 1) * 10 + (j + 1)] = g.xo;
+  if (lane < 32) {
+    const int q = lane >> 3, t = lane & 7;
+    const int dst = (q == 0) ? (t + 1) : (q == 1) ? (91 + t) : (q == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
+    L.xo[dst] = g.halo;
+  }
+}
+// after gs_stage and a wave_sync
+template <class V>
+__device__ __forceinline__ void gs_finish(const Ctx& c, const TileRec& T, V* X, int color, int flags, const GsIn& g,
+                                          const GsLDS& L) {
   if (prob_done(c, T, flags)) return;
-  wave_sync();
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const int hc = color << 8, ho = (color ^ 1) << 8;
+  V* tile = X + size_t(T.self) * 512;
   const int pc = (i + 1) * 10 + (j + 1);
   const bool p = ((color + i + j) & 1) != 0;
-  const float4 nv = gs_quad(xc, xo, L.xo[pc - 10], L.xo[pc + 10], L.xo[pc - 1], L.xo[pc + 1], xkm, xkp, B, mask, T.bc,
-                            T.h, i, j, p);
+  const float4 nv = gs_quad(g.xc, g.xo, L.xo[pc - 10], L.xo[pc + 10], L.xo[pc - 1], L.xo[pc + 1], g.xkm, g.xkp, g.B,
+                            g.mask, T.bc, T.h, i, j, p);
   store4(tile + hc + lane * 4, nv);
-  if (zero_all) store4(tile + ho + lane * 4, z4);  // the zero guess of the other colour, for the next pass
+  if (flags & F_ZERO_X) store4(tile + ho + lane * 4, make_float4(0.f, 0.f, 0.f, 0.f));  // the zero guess of the other colour
+}
+template <class V>
+__device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int level, V* X, const V* F, int color,
+                                        int flags, GsLDS& L) {
+  const GsIn g = gs_load(c, T, X, F, color, flags);
+  gs_stage(g, L);
+  wave_sync();
+  gs_finish(c, T, X, color, flags, g, L);
 }
 
 // Damped-Jacobi sweep (the update of GMGSolver.cu:508-525 with the level's
@@ -833,6 +866,28 @@ __global__ __launch_bounds__(64) void k_gs_color(Ctx c, const TileRec* __restric
   TILE_KERNEL_PROLOGUE
   __shared__ GsLDS L;
   tile_gs(c, T, level, X, F, color, flags, L);
+}
+// The same colour sweep with a tile block of two: wavefront w works tiles 2w
+// and 2w+1 of the list, loads of both in flight together.  Every tile's
+// update is the one above (tiles of a colour pass are independent).
+template <class V>
+__global__ __launch_bounds__(64) void k_gs_color2(Ctx c, const TileRec* __restrict__ list, int n, int level, V* X,
+                                                  const V* F, int color, int flags) {
+  const int w = work_index(flags & F_SWIZZLE);
+  const int t0 = 2 * w;
+  if (t0 >= n) return;
+  const bool two = t0 + 1 < n;
+  const TileRec T0 = list[t0];
+  const TileRec T1 = list[two ? t0 + 1 : t0];
+  __shared__ GsLDS L[2];
+  const GsIn g0 = gs_load(c, T0, X, F, color, flags);
+  GsIn g1 = g0;
+  if (two) g1 = gs_load(c, T1, X, F, color, flags);
+  gs_stage(g0, L[0]);
+  if (two) gs_stage(g1, L[1]);
+  wave_sync();
+  gs_finish(c, T0, X, color, flags, g0, L[0]);
+  if (two) gs_finish(c, T1, X, color, flags, g1, L[1]);
 }
 template <class V>
 __global__ __launch_bounds__(64) void k_gs_prolong(Ctx c, const TileRec* __restrict__ list, int n, const V* XD,

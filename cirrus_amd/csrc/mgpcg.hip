@@ -84,9 +84,10 @@ struct cg_problem {
   std::vector<CoarsePlan> plans; // LDS-resident coarse levels, one per problem with C >= 0
   std::vector<CoarseSlot> coarse_slots;
   // schedule (no effect on results): XCD-aware tile order, LDS-resident coarse
-  // levels given up to grid-wide launches
+  // levels given up to grid-wide launches, tiles per wavefront of the colour sweeps
   int swz = F_SWIZZLE;
   int coarse_shift = 0;
+  int gs_block = 1;
   std::vector<int> lev, tbase, coarse_auto;
   std::vector<uint8_t> ty;
 
@@ -552,7 +553,10 @@ struct VParams {
 template <class V>
 void launch_gs(cg_problem& P, const DList& a, const Ctx& c, int l, V* Z, const V* R, int color, int flags) {
   if (a.n == 0) return;
-  k_gs_color<V><<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
+  if (P.gs_block == 2 && a.n > 1)
+    k_gs_color2<V><<<grid_for((a.n + 1) / 2, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
+  else
+    k_gs_color<V><<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
 }
 // The pool's V-cycle vectors of storage type V.
 template <class V> V* pool_z(cg_problem& P);
@@ -964,10 +968,12 @@ void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& 
 // ---------------------------------------------------------------------------
 // Autotuner: schedules timed on the pool's own right-hand sides
 // ---------------------------------------------------------------------------
-void set_schedule(cg_problem& P, int coarse_shift, int swizzle) {
+void set_schedule(cg_problem& P, int coarse_shift, int swizzle, int tile_block) {
   if (coarse_shift < 0 || coarse_shift > CL_MAX_LEVELS) fail(CG_ERR_INVALID, "cg_set_schedule: coarse_shift out of range");
+  if (tile_block != 1 && tile_block != 2) fail(CG_ERR_INVALID, "cg_set_schedule: tile_block must be 1 or 2");
   if (!P.owner) fail(CG_ERR_INVALID, "cg_set_schedule: not an owning handle");
   P.swz = swizzle ? F_SWIZZLE : 0;
+  P.gs_block = tile_block;
   if (coarse_shift != P.coarse_shift) {
     P.coarse_shift = coarse_shift;
     build_schedule(P);
@@ -978,8 +984,9 @@ void set_schedule(cg_problem& P, int coarse_shift, int swizzle) {
 }
 
 // Time candidate schedules on the pool's current right-hand sides (restored
-// before every solve): first the ones that leave every result unchanged (LDS
-// coarse depth, tile order) with the caller's smoother, then the smoother,
+// before every solve): first the ones that leave every result unchanged (tile
+// order and tile block, then the LDS coarse depth with the best of those) with
+// the caller's smoother, then the smoother,
 // sweeps per level and (if allowed) bfloat16 storage with the best of those.
 // A candidate counts only if every problem converges; each is timed as the
 // best of two solves after a warm-up (graph capture).  The pool keeps the
@@ -1004,21 +1011,25 @@ void autotune(cg_problem& P, const cg_params& base, int allow_bf16, cg_params* b
     candidates++;
     return t;
   };
-  const int shift0 = P.coarse_shift, swz0 = P.swz ? 1 : 0;
-  set_schedule(P, 0, 1);
+  set_schedule(P, 0, 1, 1);
   const double t_base = run(base);
   double t_best = t_base;
-  int best_shift = 0, best_swz = 1;
+  int best_shift = 0, best_swz = 1, best_block = 1;
   int max_auto = -1;
   for (int q = 0; q < P.nprob; q++) max_auto = std::max(max_auto, P.coarse_auto[q]);
-  for (int shift = 0; shift <= std::min(2, max_auto + 1); shift++)
-    for (int swz = 1; swz >= 0; swz--) {
-      if (shift == 0 && swz == 1) continue;
-      set_schedule(P, shift, swz);
+  for (int swz = 1; swz >= 0; swz--)
+    for (int block = 1; block <= 2; block++) {
+      if (swz == 1 && block == 1) continue;
+      set_schedule(P, 0, swz, block);
       const double t = run(base);
-      if (t < t_best) { t_best = t; best_shift = shift; best_swz = swz; }
+      if (t < t_best) { t_best = t; best_swz = swz; best_block = block; }
     }
-  set_schedule(P, best_shift, best_swz);
+  for (int shift = 1; shift <= std::min(2, max_auto + 1); shift++) {
+    set_schedule(P, shift, best_swz, best_block);
+    const double t = run(base);
+    if (t < t_best) { t_best = t; best_shift = shift; }
+  }
+  set_schedule(P, best_shift, best_swz, best_block);
   cg_params chosen = base;
   struct Cand { int smoother, nu; };
   const Cand cands[] = {{CG_SMOOTH_RBGS, 1}, {CG_SMOOTH_RBGS, 2}, {CG_SMOOTH_RBGS, 3}, {CG_SMOOTH_JACOBI, 2}, {CG_SMOOTH_JACOBI, 3}};
@@ -1035,8 +1046,6 @@ void autotune(cg_problem& P, const cg_params& base, int allow_bf16, cg_params* b
   HIPC(hipMemcpyAsync(P.d_ch[CG_CH_R], bkeep, bytes, hipMemcpyDeviceToDevice, P.stream));
   HIPC(hipStreamSynchronize(P.stream));
   HIPC(hipFree(bkeep));
-  (void)shift0;
-  (void)swz0;
   *best = chosen;
   if (rep) {
     rep->candidates = candidates;
@@ -1044,6 +1053,7 @@ void autotune(cg_problem& P, const cg_params& base, int allow_bf16, cg_params* b
     rep->best_seconds = t_best;
     rep->coarse_shift = best_shift;
     rep->swizzle = best_swz;
+    rep->tile_block = best_block;
   }
 }
 
@@ -1202,10 +1212,10 @@ cg_status cg_get_channel(cg_problem* p, int32_t channel, float* host) {
   });
 }
 
-cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle) {
+cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle, int32_t tile_block) {
   return guard([&] {
     if (!p) fail(CG_ERR_INVALID, "null problem");
-    set_schedule(*p, coarse_shift, swizzle);
+    set_schedule(*p, coarse_shift, swizzle, tile_block);
   });
 }
 
@@ -1475,8 +1485,7 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
         case CG_KERNEL_GS_SWEEP_BF16: {
           const DList& a = P.act[level];
           nt = a.n;
-          if (a.n) k_gs_color<bf16><<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, level, c.zh, nob, 0,
-                                                                              base | F_GHOST_ZERO);
+          launch_gs(P, a, c, level, c.zh, nob, 0, base | F_GHOST_ZERO);
           break;
         }
         case CG_KERNEL_GS_PROLONG_BF16: {

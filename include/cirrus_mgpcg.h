@@ -172,17 +172,21 @@ cg_status cg_get_solution_f64(cg_problem* p, double* host);
  * A schedule changes how the kernels cover the tiles, never a result:
  * coarse_shift = how many of each problem's LDS-resident coarse levels (as many
  * as fit by default) go back to grid-wide launches; swizzle = XCD-aware tile
- * order (1, default) or the plain order (0). */
-cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle);
+ * order (1, default) or the plain order (0); tile_block = tiles per wavefront
+ * of the grid-wide colour sweeps (1, default, or 2: the loads of two tiles in
+ * flight together). */
+cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle, int32_t tile_block);
 typedef struct {
   int32_t candidates;        /* parameter sets timed */
   double base_seconds;       /* one solve with the caller's parameters, default schedule */
   double best_seconds;       /* one solve with the chosen schedule and parameters */
   int32_t coarse_shift;      /* the chosen schedule (kept by the handle) */
   int32_t swizzle;
+  int32_t tile_block;
 } cg_tune_report;
 /* Times solves of the handle's current right-hand sides (restored afterwards)
- * under candidate schedules (coarse_shift 0-2, swizzle on/off) with the caller's
+ * under candidate schedules (swizzle on/off x tile_block 1/2, then coarse_shift
+ * 1-2 with the best of those) with the caller's
  * smoother, then under candidate smoother settings (RB-GS with 1-3 sweeps per
  * level, damped Jacobi with 2-3; with allow_bf16 also bfloat16 V-cycle storage),
  * keeping the fastest whose every problem converges.  The handle keeps the

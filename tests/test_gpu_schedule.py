@@ -1,7 +1,8 @@
 """Schedules change how the kernels cover the tiles, never a result: every
-combination of LDS-resident coarse depth and tile order gives the oracle's
-solve bit for bit (FP32 and bfloat16 storage, where the LDS kernel rounds
-exactly where the grid-wide kernels store).  The autotuner picks among such
+combination of LDS-resident coarse depth, tile order and tile block (tiles per
+wavefront of the colour sweeps; lists of odd length leave one wavefront a
+single tile) gives the oracle's solve bit for bit (FP32 and bfloat16 storage,
+where the LDS kernel rounds exactly where the grid-wide kernels store).  The autotuner picks among such
 schedules and among smoother settings by timing solves of the handle's own
 right-hand sides."""
 import numpy as np
@@ -30,12 +31,33 @@ def test_every_schedule_gives_the_oracle_solve(name, storage):
     xo = o.get(po.CH_X).astype(np.float32)[typ == po.LEAF]
     for shift in (0, 1, 2, 3):
         for swz in (True, False):
-            P.set_schedule(shift, swz)
-            P.set(ca.CH_B, b.astype(np.float32))
-            S = ca.GMGSolver()
-            S.solve(P, False, 200, 1e-6, 2, 10, 1, False, storage=storage)
-            assert (S.last["iters"], S.last["rr"]) == (ro["iters"], ro["rr"]), (shift, swz)
-            assert np.array_equal(P.get(ca.CH_X)[typ == po.LEAF], xo), (shift, swz)
+            for block in (1, 2):
+                P.set_schedule(shift, swz, block)
+                P.set(ca.CH_B, b.astype(np.float32))
+                S = ca.GMGSolver()
+                S.solve(P, False, 200, 1e-6, 2, 10, 1, False, storage=storage)
+                assert (S.last["iters"], S.last["rr"]) == (ro["iters"], ro["rr"]), (shift, swz, block)
+                assert np.array_equal(P.get(ca.CH_X)[typ == po.LEAF], xo), (shift, swz, block)
+
+
+def test_tile_block_on_a_batch():
+    """A batched pool with two tiles per wavefront: each problem's pressure
+    equals its solve with one tile per wavefront, bit for bit."""
+    trees, bs = [], []
+    for kw in SCENES.values():
+        sc = ca.Scene(**kw)
+        trees.append(sc.tree)
+        bs.append(sc.rhs.astype(np.float32))
+    b = np.concatenate(bs)
+    out = []
+    for block in (1, 2):
+        B = ca.Problem.batch(trees)
+        B.set_schedule(0, True, block)
+        B.set(ca.CH_B, b)
+        res = ca.GMGSolver().solve_batch(B, 1000, 1e-6, 2, 10, storage=ca.STORE_BF16)
+        out.append(([(r["iters"], r["rr"]) for r in res], B.get(ca.CH_X)))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
 
 
 def test_autotune_on_a_batch():
@@ -50,6 +72,7 @@ def test_autotune_on_a_batch():
     S = ca.GMGSolver()
     kw, rep = S.autotune(B, 1000, 1e-6, 2, 10, allow_bf16=True)
     assert rep["candidates"] >= 8 and rep["best_seconds"] <= rep["base_seconds"]
+    assert rep["tile_block"] in (1, 2)
     assert np.array_equal(B.get(ca.CH_B), b)  # the right-hand sides are restored
     res = S.solve_batch(B, 1000, 1e-6, coarsest_iters=10, verify_f64=True, **kw)
     assert all(r["converged"] == 1 and r["rel_res_f64"] < 5e-6 for r in res)
