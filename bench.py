@@ -112,7 +112,7 @@ def plain_drop_in(ca, items, solver_kw, reps=1):
             "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99))}
 
 
-def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=64, nscenes=8, budget=24 << 30, rho=0.95, steps=50,
+def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=256, nscenes=8, budget=200 << 30, rho=0.95, steps=50,
                storage=0, refine=False):
     """Serving form: requests arrive as a Poisson process of rate `lam` for
     `duration` seconds; request j goes to resident instance j mod conc, whose
@@ -337,6 +337,13 @@ def run_vortex(ca, solver_kw, reps=3, kw=VORTEX_KW):
         "_x64_leaf": x64[sc.tree.type == 1],
     }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gs_color_adaptive256_hbm_traffic.txt")
+FORMS = ["projection", "plain_drop_in", "batched", "batched_mixed", "batched_autotuned", "served", "served_mixed",
+         "served_mixed_light", "vortex"]
+
+
+def note(msg):
+    """Progress line on stderr (the JSON result goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 KERNEL_REPS = 20
 
 
@@ -531,18 +538,19 @@ def summarize_pmc(fetch_csv, write_csv, title=None, out=None):
 STATS_FILE = os.path.join(ROOT, "profiles", "mgpcg_adaptive256_kernel_stats.txt")
 
 
-def summarize_trace(trace_csv, stats_csv):
+def summarize_trace(trace_csv, stats_csv, title=None, out=None, bench_args=None):
     """Per-kernel and per-(kernel, grid) durations of a --kernel-trace --stats run;
-    writes the profile summary."""
+    writes the profile summary (default: the adaptive 256^3 workload's)."""
     import csv
     import collections
+    bench_args = bench_args or "--steps 5 --warmup 1 --no-cpu-baseline --batch 0 --serve-seconds 0 --no-vortex"
     lines = [
-        "Kernel durations, adaptive 256^3 MGPCG workload (one MI355X)",
+        f"Kernel durations, {title or 'adaptive 256^3 MGPCG workload'} (one MI355X)",
         "",
         "command:",
-        "  rocprofv3 --kernel-trace --stats -d OUT -o run --output-format csv -- "
-        "python3 bench.py --steps 5 --warmup 1 --no-cpu-baseline --batch 0 --serve-seconds 0 --no-vortex",
-        "  python bench.py --summarize-trace OUT/run_kernel_trace.csv OUT/run_kernel_stats.csv",
+        f"  rocprofv3 --kernel-trace --stats -d OUT -o run --output-format csv -- python3 bench.py {bench_args}",
+        "  python bench.py --summarize-trace OUT/run_kernel_trace.csv OUT/run_kernel_stats.csv"
+        + ("" if title is None else " --trace-title '...' --trace-args '...' --trace-out FILE"),
         "",
         "rocprofv3 --stats (all launches of each kernel):",
         f"  {'kernel':24s} {'calls':>7s} {'avg us':>9s} {'total ms':>9s} {'%':>6s}",
@@ -564,8 +572,9 @@ def summarize_trace(trace_csv, stats_csv):
     for (name, wgs), v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
         v = sorted(v)
         lines.append(f"  {name:24s} {wgs:6d} {len(v):6d} {sum(v) / len(v):8.3f} {v[len(v) // 2]:9.3f}")
-    os.makedirs(os.path.dirname(STATS_FILE), exist_ok=True)
-    with open(STATS_FILE, "w") as f:
+    path = out or STATS_FILE
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
@@ -602,7 +611,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-rate-light", type=float, default=600.0,
                     help="offered rate of the served_mixed_light form, below capacity (0: skip)")
-    ap.add_argument("--serve-instances", type=int, default=64)
+    ap.add_argument("--serve-instances", type=int, default=256)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0, help="HBM admission budget of the served forms")
+    ap.add_argument("--forms", default="all", help=f"comma list of the forms to run beside the plain solve ({FORMS})")
     ap.add_argument("--no-vortex", action="store_true",
                     help="skip the mixed-precision refinement form on the vortex-tube octree")
     ap.add_argument("--kernels-only", action="store_true",
@@ -611,6 +622,9 @@ def main():
                     help="with --kernels-only: run the kernels on the pool of this many batched-form problems")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--pmc-title", default=None, help="with --summarize-pmc: the workload the counters cover")
+    ap.add_argument("--trace-title", default=None, help="with --summarize-trace: the workload the trace covers")
+    ap.add_argument("--trace-args", default=None, help="with --summarize-trace: the bench.py arguments of the traced run")
+    ap.add_argument("--trace-out", default=None, help="with --summarize-trace: the summary file to write")
     ap.add_argument("--pmc-out", default=None, help="with --summarize-pmc: the summary file to write")
     ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_CSV", "STATS_CSV"))
     args = ap.parse_args()
@@ -618,7 +632,7 @@ def main():
         summarize_pmc(*args.summarize_pmc, title=args.pmc_title, out=args.pmc_out)
         return
     if args.summarize_trace:
-        summarize_trace(*args.summarize_trace)
+        summarize_trace(*args.summarize_trace, title=args.trace_title, out=args.trace_out, bench_args=args.trace_args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: every solve here runs on a single MI355X; "
@@ -715,43 +729,59 @@ def main():
                      "avg_launch_us": us, "alg_bytes_per_launch": alg_bytes},
         "kernels_us": per_kernel,
     }
+    note(f"plain solve: {out['value']:.1f} solves/s")
     forms = {}
+    wanted = set(FORMS if args.forms == "all" else args.forms.split(","))
+    unknown = wanted - set(FORMS)
+    if unknown:
+        raise SystemExit(f"unknown forms {sorted(unknown)}; known: {FORMS}")
     # the whole projection around the solve (FluidEuler::project without the
     # velocity BCs): divergence of the scene's face velocities, the solve, the
     # gradient added back; velocities re-uploaded before each, outside the timing
-    uvw = [np.ascontiguousarray(scene.uvw[:, a, :]) for a in range(3)]
-    pt = []
-    for rep_i in range(12):
-        for a in range(3):
-            P.set(ca.CH_U + a, uvw[a])
-        t = time.perf_counter()
-        pr = S.project(P, 1000, 1e-6, solver_kw["level_iters"], solver_kw["coarsest_iters"],
-                       smoother=solver_kw["smoother"])[0]
-        if rep_i >= 2:
-            pt.append(time.perf_counter() - t)
-        assert pr["converged"] == 1
-    forms["projection"] = {
-        "workload": desc + "; divergence of the face velocities, MGPCG, pressure gradient added to the velocities",
-        "ms_per_projection": float(np.median(pt)) * 1e3, "projections_per_s": len(pt) / float(np.sum(pt)),
-        "solve_ms": out["ms_per_step"], "iterations": pr["iters"]}
-    if args.batch > 0:
+    if "projection" in wanted:
+        uvw = [np.ascontiguousarray(scene.uvw[:, a, :]) for a in range(3)]
+        pt = []
+        for rep_i in range(12):
+            for a in range(3):
+                P.set(ca.CH_U + a, uvw[a])
+            t = time.perf_counter()
+            pr = S.project(P, 1000, 1e-6, solver_kw["level_iters"], solver_kw["coarsest_iters"],
+                           smoother=solver_kw["smoother"])[0]
+            if rep_i >= 2:
+                pt.append(time.perf_counter() - t)
+            assert pr["converged"] == 1
+        forms["projection"] = {
+            "workload": desc + "; divergence of the face velocities, MGPCG, pressure gradient added to the velocities",
+            "ms_per_projection": float(np.median(pt)) * 1e3, "projections_per_s": len(pt) / float(np.sum(pt)),
+            "solve_ms": out["ms_per_step"], "iterations": pr["iters"]}
+        note("projection done")
+    batched_forms = [f for f in ("batched", "batched_mixed", "batched_autotuned") if f in wanted]
+    if args.batch > 0 and batched_forms:
         t0 = time.perf_counter()
         scenes = build_scenes(ca, [batch_scene_params(q)[0] for q in range(args.batch)])
         t_build_b = time.perf_counter() - t0
         items = [(sc.tree, np.ascontiguousarray(sc.rhs, np.float32)) for sc in scenes]
         # the same problems through the plain drop-in, one at a time: the
         # denominator of the batched forms' speed-up
-        base = plain_drop_in(ca, items, solver_kw)
-        forms["plain_drop_in_batch_problems"] = dict(base, workload="the batched forms' problems, FP32, one handle "
-                                                     "each, solved one after another from x = 0 (graph replay)")
+        note(f"{args.batch} batch problems built in {t_build_b:.1f} s")
+        base = None
+        if "plain_drop_in" in wanted:
+            base = plain_drop_in(ca, items, solver_kw)
+            forms["plain_drop_in_batch_problems"] = dict(base, workload="the batched forms' problems, FP32, one handle "
+                                                         "each, solved one after another from x = 0 (graph replay)")
+            note(f"plain drop-in on the batch problems: {base['solves_per_s']:.1f} solves/s")
         B = ca.Problem.batch([sc.tree for sc in scenes])
         for name, storage, refine, tune in [("batched", 0, False, False), ("batched_mixed", 1, True, False),
                                             ("batched_autotuned", 0, False, True)]:
+            if name not in wanted:
+                continue
             f = run_batched(ca, scenes, B, args.batch_reps, solver_kw, use_graph=not args.no_graph, storage=storage,
                             refine=refine, tune=tune, t_build=t_build_b)
-            f["vs_plain_solves_per_s"] = f["solves_per_s"] / base["solves_per_s"]
+            if base:
+                f["vs_plain_solves_per_s"] = f["solves_per_s"] / base["solves_per_s"]
             f["vs_plain_adaptive256"] = f["solves_per_s"] / out["value"]
             forms[name] = f
+            note(f"{name}: {f['solves_per_s']:.1f} solves/s")
         B.close()
         del items, scenes
     if args.serve_seconds > 0:
@@ -759,31 +789,35 @@ def main():
         for name, storage, refine, lam in [("served", 0, False, args.serve_rate),
                                            ("served_mixed", 1, True, args.serve_rate),
                                            ("served_mixed_light", 1, True, args.serve_rate_light)]:
-            if lam <= 0:
+            if lam <= 0 or name not in wanted:
                 continue
             f = run_served(ca, solver_kw, duration=args.serve_seconds, lam=lam, conc=args.serve_instances,
-                           storage=storage, refine=refine)
+                           budget=int(args.serve_budget_gb * (1 << 30)), storage=storage, refine=refine)
             items = f.pop("_items")
-            if base is None:
+            if base is None and "plain_drop_in" in wanted:
                 # the instances' problems through the plain drop-in, each solved
                 # cold as the reference's projection does
                 base = plain_drop_in(ca, items, solver_kw, reps=3)
                 forms["plain_drop_in_served_problems"] = dict(base, workload="the served instances' problems, FP32, "
                                                               "one handle each, solved one after another from x = 0")
-            f["vs_plain_solves_per_s"] = f["solves_per_s"] / base["solves_per_s"]
+            if base:
+                f["vs_plain_solves_per_s"] = f["solves_per_s"] / base["solves_per_s"]
             f["vs_plain_adaptive256"] = f["solves_per_s"] / out["value"]
             forms[name] = f
-    if not args.no_vortex:
+            note(f"{name}: {f['solves_per_s']:.1f} solves/s")
+    if not args.no_vortex and "vortex" in wanted:
         f = run_vortex(ca, solver_kw)
         f.pop("_x64_leaf")
         forms["refined_vortex512"] = f
+        note("vortex done")
     if forms:
         out["forms"] = forms
         out["mixed_vs_plain"] = {k: forms[k]["vs_plain_solves_per_s"]
-                                 for k in ("batched_mixed", "served_mixed", "batched_autotuned") if k in forms}
+                                 for k in ("batched_mixed", "served_mixed", "batched_autotuned")
+                                 if k in forms and "vs_plain_solves_per_s" in forms[k]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene_kw, solver_kw)
-        if not args.no_vortex:
+        if "refined_vortex512" in forms:
             forms["refined_vortex512"]["fp64_oracle_check"] = fp64_oracle_check(ca, solver_kw)
     print(json.dumps(out), flush=True)
 

@@ -44,6 +44,8 @@ EXPORTED_SYMBOLS = (
     "cg_server_next_result", "cg_server_info", "cg_server_perturb_rhs",
     "cg_refine_params_default", "cg_solve_refined", "cg_get_solution_f64",
     "cg_velocity_divergence", "cg_add_gradient", "cg_project", "cg_set_schedule", "cg_autotune",
+    "cg_tiles_save", "cg_tiles_load", "cg_tiles_import_holder", "cg_tileset_tree", "cg_tileset_channels",
+    "cg_tileset_destroy",
 )
 
 
@@ -166,7 +168,7 @@ def lib():
             "cg_velocity_divergence": (i32, [vp, i32, i32]),
             "cg_add_gradient": (i32, [vp, i32]),
             "cg_project": (i32, [vp, P(Params), P(Result)]),
-            "cg_set_schedule": (i32, [vp, i32, i32]),
+            "cg_set_schedule": (i32, [vp, i32, i32, i32]),
             "cg_autotune": (i32, [vp, P(Params), i32, P(Params), P(TuneReport)]),
             "cg_solve_refined": (i32, [vp, P(Params), P(RefineParams), P(RefineResult)]),
             "cg_get_solution_f64": (i32, [vp, vp]),
@@ -174,6 +176,12 @@ def lib():
             "cg_scene_destroy": (i32, [vp]),
             "cg_scene_tree": (i32, [vp, P(TreeDesc), P(vp)]),
             "cg_scene_velocity": (i32, [vp, P(vp)]),
+            "cg_tiles_save": (i32, [ctypes.c_char_p, P(TreeDesc), i32, vp, vp]),
+            "cg_tiles_load": (i32, [ctypes.c_char_p, P(vp)]),
+            "cg_tiles_import_holder": (i32, [ctypes.c_char_p, i32, vp, P(vp)]),
+            "cg_tileset_tree": (i32, [vp, P(TreeDesc)]),
+            "cg_tileset_channels": (i32, [vp, P(i32), P(vp), P(vp)]),
+            "cg_tileset_destroy": (i32, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -292,6 +300,57 @@ class Scene:
         self.uvw = arr(uvw, ctypes.c_float, 3 * 512 * n).reshape(n, 3, 512)
         lib().cg_scene_destroy(h)
         self._h = None
+
+
+def save_tiles(path, tree, channels=None):
+    """Write a tree and channels ({channel number: [tiles, 512] float32}) to a
+    portable tile file (cg_tiles_save)."""
+    channels = channels or {}
+    ids = np.array(sorted(channels), np.int32)
+    arrs = [np.ascontiguousarray(channels[int(c)], np.float32).reshape(tree.n, 512) for c in ids]
+    ptrs = (ctypes.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
+    d = tree.desc()
+    _check(lib().cg_tiles_save(os.fsencode(path), ctypes.byref(d), len(ids), _ptr(ids), ptrs), "cg_tiles_save")
+
+
+def _tileset(h):
+    try:
+        d = TreeDesc()
+        _check(lib().cg_tileset_tree(h, ctypes.byref(d)), "cg_tileset_tree")
+        n = d.num_tiles
+
+        def arr(ptr, ctype, count):
+            if count == 0:
+                return np.zeros(0, np.dtype(ctype))
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), shape=(count,)).copy()
+
+        tree = Tree(d.h0, arr(d.level, ctypes.c_int32, n), arr(d.coord, ctypes.c_int32, 3 * n),
+                    arr(d.type, ctypes.c_uint8, n), arr(d.cell_type, ctypes.c_uint8, 512 * n))
+        m = ctypes.c_int32()
+        ids, data = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().cg_tileset_channels(h, ctypes.byref(m), ctypes.byref(ids), ctypes.byref(data)), "cg_tileset_channels")
+        idv = arr(ids, ctypes.c_int32, m.value)
+        dv = arr(data, ctypes.c_float, m.value * 512 * n).reshape(m.value, n, 512)
+        return tree, {int(c): dv[q] for q, c in enumerate(idv)}
+    finally:
+        lib().cg_tileset_destroy(h)
+
+
+def load_tiles(path):
+    """Read a portable tile file: (Tree, {channel number: [tiles, 512] float32})."""
+    h = ctypes.c_void_p()
+    _check(lib().cg_tiles_load(os.fsencode(path), ctypes.byref(h)), "cg_tiles_load")
+    return _tileset(h)
+
+
+def import_holder(path, channels=(0, 1, 6, 7, 8)):
+    """Read the reference's HAHostTileHolder dump (src/PoissonIOFunc.cu:38-124):
+    (Tree, {channel number: [tiles, 512] float32}) for the given channels."""
+    ids = np.array(channels, np.int32)
+    h = ctypes.c_void_p()
+    _check(lib().cg_tiles_import_holder(os.fsencode(path), len(ids), _ptr(ids), ctypes.byref(h)),
+           "cg_tiles_import_holder")
+    return _tileset(h)
 
 
 class Problem:

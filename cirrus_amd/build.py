@@ -32,11 +32,13 @@ def build(verbose=True):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     tree_o = os.path.join(objdir, "tree.o")
+    file_o = os.path.join(objdir, "tilefile.o")
     mg_o = os.path.join(objdir, "mgpcg.o")
     _run([HIPCC, "-x", "c++"] + COMMON + ["-c", os.path.join(CSRC, "tree.cpp"), "-o", tree_o], verbose)
+    _run([HIPCC, "-x", "c++"] + COMMON + ["-c", os.path.join(CSRC, "tilefile.cpp"), "-o", file_o], verbose)
     _run([HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, "mgpcg.hip"), "-o", mg_o], verbose)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, mg_o, tree_o], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, mg_o, tree_o, file_o], verbose)
     os.replace(tmp, LIB)
     return LIB
 

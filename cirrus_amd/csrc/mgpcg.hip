@@ -18,6 +18,7 @@
 
 #include "cirrus_mgpcg.h"
 #include "kernels.hpp"
+#include "tilefile.hpp"
 #include "tree.hpp"
 
 namespace {
@@ -1096,6 +1097,9 @@ struct cg_scene {
   Scene s;
   std::vector<int32_t> coord_flat;
 };
+struct cg_tileset {
+  TileSet s;
+};
 
 extern "C" {
 
@@ -1685,6 +1689,75 @@ cg_status cg_scene_tree(const cg_scene* s, cg_tree_desc* desc, const float** rhs
 cg_status cg_scene_velocity(const cg_scene* s, const float** uvw) {
   if (!s || !uvw) { g_err = "null argument"; return CG_ERR_INVALID; }
   *uvw = s->s.uvw.data();
+  return CG_OK;
+}
+
+cg_status cg_tiles_save(const char* path, const cg_tree_desc* desc, int32_t num_channels, const int32_t* channel_ids,
+                        const float* const* channels) {
+  return guard([&] {
+    if (!path || !desc || desc->num_tiles < 0 || num_channels < 0 || (num_channels && (!channel_ids || !channels)))
+      fail(CG_ERR_INVALID, "cg_tiles_save: bad argument");
+    const size_t n = size_t(desc->num_tiles);
+    if (n && (!desc->level || !desc->coord || !desc->type || !desc->cell_type))
+      fail(CG_ERR_INVALID, "cg_tiles_save: null tree array");
+    TileSet t;
+    t.h0 = desc->h0;
+    t.level.assign(desc->level, desc->level + n);
+    t.coord.assign(desc->coord, desc->coord + 3 * n);
+    t.type.assign(desc->type, desc->type + n);
+    t.ctype.assign(desc->cell_type, desc->cell_type + 512 * n);
+    t.channel_id.assign(channel_ids, channel_ids + num_channels);
+    t.channel.resize(size_t(num_channels) * 512 * n);
+    for (int32_t c = 0; c < num_channels; c++) {
+      if (!channels[c]) fail(CG_ERR_INVALID, "cg_tiles_save: null channel");
+      std::memcpy(&t.channel[size_t(c) * 512 * n], channels[c], 512 * n * sizeof(float));
+    }
+    write_tile_file(path, t);
+  });
+}
+
+cg_status cg_tiles_load(const char* path, cg_tileset** out) {
+  if (!path || !out) { g_err = "null argument"; return CG_ERR_INVALID; }
+  *out = nullptr;
+  auto S = std::make_unique<cg_tileset>();
+  cg_status st = guard([&] { S->s = read_tile_file(path); });
+  if (st == CG_OK) *out = S.release();
+  return st;
+}
+
+cg_status cg_tiles_import_holder(const char* path, int32_t num_channels, const int32_t* channel_ids, cg_tileset** out) {
+  if (!path || !out || num_channels < 0 || (num_channels && !channel_ids)) { g_err = "bad argument"; return CG_ERR_INVALID; }
+  *out = nullptr;
+  auto S = std::make_unique<cg_tileset>();
+  cg_status st = guard([&] {
+    S->s = read_holder_dump(path, std::vector<int32_t>(channel_ids, channel_ids + num_channels));
+  });
+  if (st == CG_OK) *out = S.release();
+  return st;
+}
+
+cg_status cg_tileset_tree(const cg_tileset* s, cg_tree_desc* desc) {
+  if (!s || !desc) { g_err = "null argument"; return CG_ERR_INVALID; }
+  desc->h0 = s->s.h0;
+  desc->num_tiles = int32_t(s->s.size());
+  desc->level = s->s.level.data();
+  desc->coord = s->s.coord.data();
+  desc->type = s->s.type.data();
+  desc->cell_type = s->s.ctype.data();
+  return CG_OK;
+}
+
+cg_status cg_tileset_channels(const cg_tileset* s, int32_t* num_channels, const int32_t** channel_ids,
+                              const float** data) {
+  if (!s || !num_channels || !channel_ids || !data) { g_err = "null argument"; return CG_ERR_INVALID; }
+  *num_channels = int32_t(s->s.channel_id.size());
+  *channel_ids = s->s.channel_id.data();
+  *data = s->s.channel.data();
+  return CG_OK;
+}
+
+cg_status cg_tileset_destroy(cg_tileset* s) {
+  delete s;
   return CG_OK;
 }
 

@@ -358,6 +358,34 @@ cg_status cg_scene_tree(const cg_scene* s, cg_tree_desc* desc, const float** rhs
  * propagation) from which the RHS was formed. */
 cg_status cg_scene_velocity(const cg_scene* s, const float** uvw);
 
+/* ---- Tile files ------------------------------------------------------------
+ * A portable on-disk form of a tile octree (no reference counterpart: the
+ * reference's own dump, WriteHAHostTileHolderToFile in
+ * src/PoissonIOFunc.cu:38-75, writes raw structs with pointers and is tied to
+ * its compiler's layout).  Format, version 1, little-endian: a 64-byte header
+ * (magic "CIRTILES", uint32 version, uint32 channel count m, int64 tile count
+ * n, float64 h0, uint64 FNV-1a 64 checksum of the rest, zero padding), then
+ * int32 level[n], int32 coord[3n], uint8 type[n], uint8 cell_type[512n] and per
+ * channel an int32 channel number (0-10, the reference's channels,
+ * src/PoissonTile.h:12-23), an int32 element type (1 = float32) and
+ * float32 data[512n]; cells i-major.  Readers check the magic, the version,
+ * the size, the checksum and every type value. */
+typedef struct cg_tileset cg_tileset;
+cg_status cg_tiles_save(const char* path, const cg_tree_desc* desc, int32_t num_channels, const int32_t* channel_ids,
+                        const float* const* channels);
+cg_status cg_tiles_load(const char* path, cg_tileset** out);
+/* ReadHAHostTileHolderFromFile (src/PoissonIOFunc.cu:79-124) for a dump of
+ * HAHostTileHolder<PoissonTile<float>> written on x86-64: every tile a level
+ * list points to, with its cell types and the requested channels (e.g. x = 0,
+ * b = 1, u/v/w = 6/7/8). */
+cg_status cg_tiles_import_holder(const char* path, int32_t num_channels, const int32_t* channel_ids, cg_tileset** out);
+/* Borrowed views valid until cg_tileset_destroy: the tree (tiles in file
+ * order), the channel numbers and the channel data ([num_channels][512 n]). */
+cg_status cg_tileset_tree(const cg_tileset* s, cg_tree_desc* desc);
+cg_status cg_tileset_channels(const cg_tileset* s, int32_t* num_channels, const int32_t** channel_ids,
+                              const float** data);
+cg_status cg_tileset_destroy(cg_tileset* s);
+
 #ifdef __cplusplus
 }
 #endif
