@@ -256,10 +256,10 @@ def run_batched(ca, scenes, B, reps, solver_kw, use_graph=True, storage=0, refin
     total = float(np.sum(times))
     # the V-cycle's dominant kernel on the pool's most populated level
     lv = int(np.bincount(np.concatenate([sc.tree.level[sc.tree.type != 4] for sc in scenes])).argmax())
-    kid = ca.KERNEL_GS_SWEEP_BF16 if kw["storage"] else ca.KERNEL_GS_SWEEP
+    kid = {0: ca.KERNEL_GS_SWEEP, 1: ca.KERNEL_GS_SWEEP_BF16, 2: ca.KERNEL_GS_SWEEP_F16}[kw["storage"]]
     gus, gtiles = B.benchmark_kernel(kid, lv, reps=20)
     gbytes = gs_alg_bytes(gtiles, 2 if kw["storage"] else 4)
-    prec = "BF16 V-cycle storage" if kw["storage"] else "FP32"
+    prec = {0: "FP32", 1: "BF16 V-cycle storage", 2: "F16 V-cycle storage (power-of-two rhs scale)"}[kw["storage"]]
     how = "FP64 iterative refinement to 1e-6 of the FP64 residual" if (refine or tune) else \
         "tol 1e-6 each (recursive residual), every pressure checked in FP64"
     if tune:
@@ -337,8 +337,8 @@ def run_vortex(ca, solver_kw, reps=3, kw=VORTEX_KW):
         "_x64_leaf": x64[sc.tree.type == 1],
     }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gs_color_adaptive256_hbm_traffic.txt")
-FORMS = ["projection", "plain_drop_in", "batched", "batched_mixed", "batched_autotuned", "served", "served_mixed",
-         "served_mixed_light", "vortex"]
+FORMS = ["projection", "plain_drop_in", "batched", "batched_mixed", "batched_mixed_f16", "batched_autotuned", "served",
+         "served_mixed", "served_mixed_light", "vortex"]
 
 
 def note(msg):
@@ -755,7 +755,7 @@ def main():
             "ms_per_projection": float(np.median(pt)) * 1e3, "projections_per_s": len(pt) / float(np.sum(pt)),
             "solve_ms": out["ms_per_step"], "iterations": pr["iters"]}
         note("projection done")
-    batched_forms = [f for f in ("batched", "batched_mixed", "batched_autotuned") if f in wanted]
+    batched_forms = [f for f in ("batched", "batched_mixed", "batched_mixed_f16", "batched_autotuned") if f in wanted]
     if args.batch > 0 and batched_forms:
         t0 = time.perf_counter()
         scenes = build_scenes(ca, [batch_scene_params(q)[0] for q in range(args.batch)])
@@ -772,7 +772,7 @@ def main():
             note(f"plain drop-in on the batch problems: {base['solves_per_s']:.1f} solves/s")
         B = ca.Problem.batch([sc.tree for sc in scenes])
         for name, storage, refine, tune in [("batched", 0, False, False), ("batched_mixed", 1, True, False),
-                                            ("batched_autotuned", 0, False, True)]:
+                                            ("batched_mixed_f16", 2, True, False), ("batched_autotuned", 0, False, True)]:
             if name not in wanted:
                 continue
             f = run_batched(ca, scenes, B, args.batch_reps, solver_kw, use_graph=not args.no_graph, storage=storage,
@@ -813,7 +813,7 @@ def main():
     if forms:
         out["forms"] = forms
         out["mixed_vs_plain"] = {k: forms[k]["vs_plain_solves_per_s"]
-                                 for k in ("batched_mixed", "served_mixed", "batched_autotuned")
+                                 for k in ("batched_mixed", "batched_mixed_f16", "served_mixed", "batched_autotuned")
                                  if k in forms and "vs_plain_solves_per_s" in forms[k]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene_kw, solver_kw)

@@ -31,6 +31,7 @@ namespace cg {
 enum : uint8_t { T_LEAF = 1, T_NONLEAF = 2, T_GHOST = 4 };
 enum : uint8_t { C_DIR = 1, C_INT = 2, C_NEU = 4 };
 typedef __bf16 bf16;
+typedef _Float16 f16;
 
 struct alignas(16) TileRec {
   int32_t nbr[6];     // same-level face neighbours x-,y-,z-,x+,y+,z+ (-1: none)
@@ -68,6 +69,7 @@ struct alignas(16) ProbState {
   uint32_t counter;
   int32_t leaf_begin, leaf_count, pure_neumann;
   int32_t check_stride;  // test convergence after iterations k*check_stride; <= 0: never
+  int32_t scale_exp;     // float16 storage: the right-hand side is solved scaled by 2^scale_exp
 };
 
 // Channel pointers passed to every kernel by value.
@@ -85,6 +87,9 @@ struct Ctx {
   bf16* zh;      // z, rhsv and zb stored as bfloat16 (mixed-precision form; null until used)
   bf16* rhsvh;
   bf16* zbh;
+  f16* zf;       // the same buffers holding float16 values (one storage type per solve)
+  f16* rhsvf;
+  f16* zbf;
   float* gsum;   // 64 per ghost: 2x2x2 sums of the ghost rows of A p
   ProbState* ps;
   int32_t* dflag; // per-problem copy of ProbState::done, polled by the host
@@ -100,6 +105,8 @@ template <> __device__ __forceinline__ float* vc_z<float>(const Ctx& c) { return
 template <> __device__ __forceinline__ bf16* vc_z<bf16>(const Ctx& c) { return c.zh; }
 template <> __device__ __forceinline__ float* vc_rhs<float>(const Ctx& c) { return c.rhsv; }
 template <> __device__ __forceinline__ bf16* vc_rhs<bf16>(const Ctx& c) { return c.rhsvh; }
+template <> __device__ __forceinline__ f16* vc_z<f16>(const Ctx& c) { return c.zf; }
+template <> __device__ __forceinline__ f16* vc_rhs<f16>(const Ctx& c) { return c.rhsvf; }
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -188,6 +195,50 @@ __device__ __forceinline__ float2 load2(const bf16* __restrict__ p) {
 }
 __device__ __forceinline__ void store2(float* __restrict__ p, float2 a) { *reinterpret_cast<float2*>(p) = a; }
 __device__ __forceinline__ void store2(bf16* __restrict__ p, float2 a) { *reinterpret_cast<uint32_t*>(p) = bf_pack(a.x, a.y); }
+
+// Or as float16 (11-bit significand, largest finite value 65504): a store
+// clamps to +-65504 and rounds to nearest even (v_cvt_f16_f32); a solve in this
+// storage scales its right-hand side by a power of two (k_finalize FIN_SCALE)
+// so that the values sit well inside that range.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float to_f(f16 v) { return float(v); }
+template <> __device__ __forceinline__ f16 from_f<f16>(float v) {
+  v = v > 65504.f ? 65504.f : (v < -65504.f ? -65504.f : v);
+  return f16(v);
+}
+__device__ __forceinline__ float h_lo(uint32_t w) { return float(__builtin_bit_cast(f16, uint16_t(w & 0xffffu))); }
+__device__ __forceinline__ float h_hi(uint32_t w) { return float(__builtin_bit_cast(f16, uint16_t(w >> 16))); }
+__device__ __forceinline__ uint32_t h_pack(float a, float b) {
+  return __builtin_bit_cast(uint32_t, f16x2{from_f<f16>(a), from_f<f16>(b)});
+}
+__device__ __forceinline__ void load8(const f16* __restrict__ p, float v[8]) {
+  const uint4 q = *reinterpret_cast<const uint4*>(p);
+  v[0] = h_lo(q.x); v[1] = h_hi(q.x); v[2] = h_lo(q.y); v[3] = h_hi(q.y);
+  v[4] = h_lo(q.z); v[5] = h_hi(q.z); v[6] = h_lo(q.w); v[7] = h_hi(q.w);
+}
+__device__ __forceinline__ void store8(f16* __restrict__ p, const float v[8]) {
+  *reinterpret_cast<uint4*>(p) = make_uint4(h_pack(v[0], v[1]), h_pack(v[2], v[3]), h_pack(v[4], v[5]), h_pack(v[6], v[7]));
+}
+__device__ __forceinline__ void load_pencil(const f16* __restrict__ p, float4& a, float4& b) {
+  const uint4 q = *reinterpret_cast<const uint4*>(p);
+  a = make_float4(h_lo(q.x), h_hi(q.x), h_lo(q.y), h_hi(q.y));
+  b = make_float4(h_lo(q.z), h_hi(q.z), h_lo(q.w), h_hi(q.w));
+}
+__device__ __forceinline__ void store_pencil(f16* __restrict__ p, float4 a, float4 b) {
+  *reinterpret_cast<uint4*>(p) = make_uint4(h_pack(a.x, a.y), h_pack(a.z, a.w), h_pack(b.x, b.y), h_pack(b.z, b.w));
+}
+__device__ __forceinline__ float4 load4(const f16* __restrict__ p) {
+  const uint2 q = *reinterpret_cast<const uint2*>(p);
+  return make_float4(h_lo(q.x), h_hi(q.x), h_lo(q.y), h_hi(q.y));
+}
+__device__ __forceinline__ void store4(f16* __restrict__ p, float4 a) {
+  *reinterpret_cast<uint2*>(p) = make_uint2(h_pack(a.x, a.y), h_pack(a.z, a.w));
+}
+__device__ __forceinline__ float2 load2(const f16* __restrict__ p) {
+  const uint32_t q = *reinterpret_cast<const uint32_t*>(p);
+  return make_float2(h_lo(q), h_hi(q));
+}
+__device__ __forceinline__ void store2(f16* __restrict__ p, float2 a) { *reinterpret_cast<uint32_t*>(p) = h_pack(a.x, a.y); }
 
 __device__ __forceinline__ uint64_t load_ct8(const uint8_t* __restrict__ p) {
   return *reinterpret_cast<const uint64_t*>(p);
@@ -1457,7 +1508,7 @@ __device__ __forceinline__ double block_sum_partials(const double* __restrict__ 
 }
 
 // Per-problem finalisation of a reduction over the LEAF-tile partials.
-enum : int { FIN_BB = 0, FIN_RZ = 1, FIN_PAP = 2, FIN_RR = 3, FIN_MEAN = 4, FIN_IRBB = 5 };
+enum : int { FIN_BB = 0, FIN_RZ = 1, FIN_PAP = 2, FIN_RR = 3, FIN_MEAN = 4, FIN_IRBB = 5, FIN_SCALE = 6 };
 __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctx c, int mode, int flags) {
   __shared__ double lds[FIN_CHUNK];
   ProbState* ps = c.ps + (c.pmap ? c.pmap[blockIdx.x] : int(blockIdx.x));
@@ -1507,6 +1558,18 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctx c, int mode, int f
       ps->bb_b = s;
       ps->thr_abs = ps->tol_abs2 * s;
       break;
+    case FIN_SCALE: {  // float16 storage: the right-hand side's scale 2^k, from |b|^2 / cells = 2^e f,
+                       // f in [1, 2): k = -floor(e/2) - 4 (root mean square near 2^-4)
+      int k = 0;
+      if (s > 0.0) {
+        const int e = ilogb(s / (512.0 * double(ps->leaf_count)));
+        const int half = (e >= 0) ? e / 2 : -((-e + 1) / 2);
+        k = min(120, max(-120, -half - 4));
+      }
+      ps->scale_exp = k;
+      ps->thr_abs = ldexp(ps->thr_abs, 2 * k);
+      break;
+    }
   }
 }
 
@@ -1807,6 +1870,18 @@ __global__ __launch_bounds__(64) void k_ir_norm(Ctx c, const TileRec* __restrict
   for (int k = 0; k < 8; k++) a += double(b[k]) * double(b[k]);
   a = wave_sum(a);
   if (lane_id() == 0) c.part[T.leaf_slot] = a;
+}
+// X *= 2^(sign * scale_exp) on every cell of the LEAF tiles (exact: a power of two).
+__global__ __launch_bounds__(64) void k_scale_leaf(Ctx c, const TileRec* __restrict__ list, int n, float* __restrict__ X,
+                                                   int sign) {
+  const int flags = F_SWIZZLE;
+  TILE_KERNEL_PROLOGUE
+  const float f = ldexpf(1.f, sign * c.ps[T.prob].scale_exp);
+  const size_t base = size_t(T.self) * 512 + lane_id() * 8;
+  float v[8];
+  load8(X + base, v);
+  for (int k = 0; k < 8; k++) v[k] = v[k] * f;
+  store8(X + base, v);
 }
 // r64 = b - A x64 on every cell of the LEAF tiles (R holds A x64 on entry).
 //   mode 0: the next inner solve's rhs r = float(r64), partials of |r64|^2;

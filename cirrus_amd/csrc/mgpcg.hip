@@ -97,7 +97,7 @@ struct cg_problem {
   uint8_t* d_fmask = nullptr;
   float* d_ch[CG_NUM_CHANNELS] = {};
   float* d_zb = nullptr;
-  bf16* d_zh = nullptr;     // bfloat16 z, V-cycle rhs and scratch (allocated at the first such solve)
+  bf16* d_zh = nullptr;     // 16-bit z, V-cycle rhs and scratch (bfloat16 or float16 values; allocated at the first such solve)
   bf16* d_rhsvh = nullptr;
   bf16* d_zbh = nullptr;
   double* d_x64 = nullptr;  // iterative refinement: FP64 x and residual, the caller's b, per-problem sums
@@ -139,6 +139,9 @@ struct cg_problem {
     c.zh = d_zh;
     c.rhsvh = d_rhsvh;
     c.zbh = d_zbh;
+    c.zf = reinterpret_cast<f16*>(d_zh);
+    c.rhsvf = reinterpret_cast<f16*>(d_rhsvh);
+    c.zbf = reinterpret_cast<f16*>(d_zbh);
     c.gsum = d_gsum;
     c.ps = d_ps;
     c.dflag = d_dflag;
@@ -566,6 +569,8 @@ template <> float* pool_z<float>(cg_problem& P) { return P.d_ch[CG_CH_Z]; }
 template <> bf16* pool_z<bf16>(cg_problem& P) { return P.d_zh; }
 template <> float* pool_zb<float>(cg_problem& P) { return P.d_zb; }
 template <> bf16* pool_zb<bf16>(cg_problem& P) { return P.d_zbh; }
+template <> f16* pool_z<f16>(cg_problem& P) { return reinterpret_cast<f16*>(P.d_zh); }
+template <> f16* pool_zb<f16>(cg_problem& P) { return reinterpret_cast<f16*>(P.d_zbh); }
 // Scratch k (0..2) of the operator API: a channel in the colour-split order of
 // the V-cycle kernels; src (natural, may be null) is converted into it.
 float* api_split(cg_problem& P, int k, const float* src) {
@@ -699,6 +704,7 @@ void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann) {
 }
 void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann, int storage) {
   if (storage == CG_STORE_BF16) enqueue_cg_iteration<bf16>(P, v, pure_neumann);
+  else if (storage == CG_STORE_F16) enqueue_cg_iteration<f16>(P, v, pure_neumann);
   else enqueue_cg_iteration<float>(P, v, pure_neumann);
 }
 
@@ -753,7 +759,8 @@ void check_params(const cg_params& prm) {
   if (prm.max_iters < 0 || !(prm.rel_tol >= 0) || prm.level_iters < 0 || prm.coarsest_iters < 0)
     fail(CG_ERR_INVALID, "cg_solve: invalid parameters");
   if (prm.smoother != CG_SMOOTH_RBGS && prm.smoother != CG_SMOOTH_JACOBI) fail(CG_ERR_INVALID, "cg_solve: unknown smoother");
-  if (prm.storage != CG_STORE_F32 && prm.storage != CG_STORE_BF16) fail(CG_ERR_INVALID, "cg_solve: unknown storage type");
+  if (prm.storage != CG_STORE_F32 && prm.storage != CG_STORE_BF16 && prm.storage != CG_STORE_F16)
+    fail(CG_ERR_INVALID, "cg_solve: unknown storage type");
 }
 
 // ---------------------------------------------------------------------------
@@ -805,7 +812,7 @@ std::vector<double> residual64(cg_problem& P, const float* B, int pure_neumann, 
 void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* tol_q = nullptr, const int* maxit_q = nullptr,
            const double* thr_abs_q = nullptr) {
   check_params(prm);
-  if (prm.storage == CG_STORE_BF16) ensure_bf16(P);
+  if (prm.storage != CG_STORE_F32) ensure_bf16(P);
   if (prm.verify_f64) {  // keep b for the FP64 check at the end
     ensure_f64(P);
     HIPC(hipMemcpyAsync(P.d_bsave, P.d_ch[CG_CH_R], size_t(P.n) * 512 * sizeof(float), hipMemcpyDeviceToDevice, P.stream));
@@ -823,8 +830,16 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
   int max_iters = prm.max_iters;
   if (maxit_q) for (int q = 0; q < P.nprob; q++) max_iters = std::max(max_iters, maxit_q[q]);
   const Ctx c = P.ctx();
+  const bool f16s = prm.storage == CG_STORE_F16;
+  if (f16s) {  // float16 storage: solve for 2^k b, k from |b|^2 per problem (FIN_SCALE)
+    k_ir_norm<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r);
+    k_finalize<<<unsigned(P.nprob), FIN_THREADS, 0, P.stream>>>(c, FIN_SCALE, 0);
+    k_scale_leaf<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, 1);
+  }
   if (prm.storage == CG_STORE_BF16)
     k_cg_init<bf16><<<grid_for(P.leaf.n, P.swz), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.swz);
+  else if (f16s)
+    k_cg_init<f16><<<grid_for(P.leaf.n, P.swz), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.swz);
   else
     k_cg_init<float><<<grid_for(P.leaf.n, P.swz), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.swz);
   k_finalize<<<unsigned(P.nprob), FIN_THREADS, 0, P.stream>>>(c, FIN_BB, 0);
@@ -875,9 +890,17 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
     have_prev = true;
     slot = 1 - slot;
   }
+  if (f16s) {  // back to the caller's scale: x and the residual r
+    k_scale_leaf<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.x, -1);
+    k_scale_leaf<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, -1);
+  }
   HIPC(hipStreamSynchronize(P.stream));
   auto t1 = std::chrono::steady_clock::now();
-  const std::vector<ProbState> st = read_states(P);
+  std::vector<ProbState> st = read_states(P);
+  for (ProbState& s : st) {  // |b|^2 and |r|^2 at the caller's scale (exact: powers of two)
+    s.bb = std::ldexp(s.bb, -2 * s.scale_exp);
+    s.rr = std::ldexp(s.rr, -2 * s.scale_exp);
+  }
   std::vector<double> rr64;
   if (prm.verify_f64) {
     k_ir_load_x<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.x, P.d_x64);
@@ -912,7 +935,7 @@ void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& 
   if (!(rp.rel_tol >= 0) || !(rp.inner_rel_tol >= 0) || rp.max_outer < 0)
     fail(CG_ERR_INVALID, "cg_solve_refined: invalid refinement parameters");
   ensure_f64(P);
-  if (prm.storage == CG_STORE_BF16) ensure_bf16(P);
+  if (prm.storage != CG_STORE_F32) ensure_bf16(P);
   auto t0 = std::chrono::steady_clock::now();
   const int np = P.nprob;
   const size_t cells = size_t(P.n) * 512;
@@ -1034,7 +1057,7 @@ void autotune(cg_problem& P, const cg_params& base, int allow_bf16, cg_params* b
   cg_params chosen = base;
   struct Cand { int smoother, nu; };
   const Cand cands[] = {{CG_SMOOTH_RBGS, 1}, {CG_SMOOTH_RBGS, 2}, {CG_SMOOTH_RBGS, 3}, {CG_SMOOTH_JACOBI, 2}, {CG_SMOOTH_JACOBI, 3}};
-  for (int storage = CG_STORE_F32; storage <= (allow_bf16 ? CG_STORE_BF16 : CG_STORE_F32); storage++)
+  for (int storage = CG_STORE_F32; storage <= (allow_bf16 ? CG_STORE_F16 : CG_STORE_F32); storage++)
     for (const Cand& cd : cands) {
       cg_params prm = base;
       prm.smoother = cd.smoother;
@@ -1466,14 +1489,16 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
   return guard([&] {
     if (!p || !avg_us || reps <= 0) fail(CG_ERR_INVALID, "cg_benchmark_kernel: bad argument");
     cg_problem& P = *p;
-    if ((kernel == CG_KERNEL_GS_SWEEP || kernel == CG_KERNEL_JACOBI || kernel == CG_KERNEL_GS_SWEEP_BF16) &&
+    if ((kernel == CG_KERNEL_GS_SWEEP || kernel == CG_KERNEL_JACOBI || kernel == CG_KERNEL_GS_SWEEP_BF16 ||
+         kernel == CG_KERNEL_GS_SWEEP_F16) &&
         (level < 0 || level > P.max_level))
       fail(CG_ERR_INVALID, "level out of range");
     if ((kernel == CG_KERNEL_RESTRICT || kernel == CG_KERNEL_PROLONG || kernel == CG_KERNEL_GS_PROLONG ||
          kernel == CG_KERNEL_GS_PROLONG_BF16) &&
         (level < 1 || level > P.max_level))
       fail(CG_ERR_INVALID, "level out of range");
-    if (kernel == CG_KERNEL_GS_SWEEP_BF16 || kernel == CG_KERNEL_GS_PROLONG_BF16) ensure_bf16(P);
+    if (kernel == CG_KERNEL_GS_SWEEP_BF16 || kernel == CG_KERNEL_GS_PROLONG_BF16 || kernel == CG_KERNEL_GS_SWEEP_F16)
+      ensure_bf16(P);
     if ((kernel == CG_KERNEL_COARSE || kernel == CG_KERNEL_EMPTY) && level < 0) fail(CG_ERR_INVALID, "negative count");
     set_prob_state(P, 0.0, 1 << 30, 0);
     const Ctx c = P.ctx();
@@ -1490,6 +1515,12 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
           const DList& a = P.act[level];
           nt = a.n;
           launch_gs(P, a, c, level, c.zh, nob, 0, base | F_GHOST_ZERO);
+          break;
+        }
+        case CG_KERNEL_GS_SWEEP_F16: {
+          const DList& a = P.act[level];
+          nt = a.n;
+          launch_gs(P, a, c, level, c.zf, static_cast<const f16*>(nullptr), 0, base | F_GHOST_ZERO);
           break;
         }
         case CG_KERNEL_GS_PROLONG_BF16: {

@@ -184,6 +184,7 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   if (!cfg || cfg->budget_bytes <= 0 || cfg->max_problems <= 0) fail(CG_ERR_INVALID, "cg_server_create: bad configuration");
   const cg_params& prm = cfg->params;
   check_params(prm);
+  if (prm.storage == CG_STORE_F16) fail(CG_ERR_INVALID, "cg_server_create: float16 storage is not available to the server");
   S.prm = prm;
   S.v.nu = prm.level_iters;
   S.v.coarsest = prm.coarsest_iters;

@@ -37,8 +37,14 @@ enum { CG_CH_U = 6, CG_CH_V = 7, CG_CH_W = 8 };
 enum { CG_SMOOTH_RBGS = 0, CG_SMOOTH_JACOBI = 1 };
 /* Storage of the V-cycle vectors (its x on every level, its rhs and scratch);
  * arithmetic is FP32 either way.  BF16 rounds each stored value to nearest
- * even and halves their HBM traffic; no reference counterpart. */
-enum { CG_STORE_F32 = 0, CG_STORE_BF16 = 1 };
+ * even and halves their HBM traffic.  F16 does the same in float16 (three more
+ * significand bits, range +-65504: stores clamp there); a solve in F16 scales
+ * the right-hand side by a power of two 2^k, k = -floor(e/2) - 4 where
+ * |b|^2 / cells = 2^e f (f in [1, 2)), and scales x and r back, so the result
+ * is the solve of 2^k b divided by 2^k exactly.  F16 is available to cg_solve,
+ * cg_solve_batch and cg_solve_refined, not to the server.  No reference
+ * counterpart. */
+enum { CG_STORE_F32 = 0, CG_STORE_BF16 = 1, CG_STORE_F16 = 2 };
 
 typedef enum {
   CG_OK = 0,
@@ -82,7 +88,7 @@ typedef struct {
   float one_over_alpha;     /* restriction scale (1) */
   float prolong_coeff;      /* prolongation scale (1) */
   int32_t use_graph;        /* capture one CG iteration in a hipGraph and replay it (1) */
-  int32_t storage;          /* CG_STORE_F32 (0, the reference) or CG_STORE_BF16 */
+  int32_t storage;          /* CG_STORE_F32 (0, the reference), CG_STORE_BF16 or CG_STORE_F16 */
   int32_t verify_f64;       /* recompute |b - A x| / |b| in FP64 at the end (cg_result.rel_res_f64) (0) */
 } cg_params;
 
@@ -188,11 +194,11 @@ typedef struct {
  * under candidate schedules (swizzle on/off x tile_block 1/2, then coarse_shift
  * 1-2 with the best of those) with the caller's
  * smoother, then under candidate smoother settings (RB-GS with 1-3 sweeps per
- * level, damped Jacobi with 2-3; with allow_bf16 also bfloat16 V-cycle storage),
+ * level, damped Jacobi with 2-3; with allow_16bit also bfloat16 and float16 V-cycle storage),
  * keeping the fastest whose every problem converges.  The handle keeps the
  * chosen schedule; *best receives the chosen parameters (base with smoother,
  * level_iters and storage replaced). */
-cg_status cg_autotune(cg_problem* p, const cg_params* base, int32_t allow_bf16, cg_params* best, cg_tune_report* report);
+cg_status cg_autotune(cg_problem* p, const cg_params* base, int32_t allow_16bit, cg_params* best, cg_tune_report* report);
 
 /* ---- The projection around the solve (src/FluidEuler.h:581-612) ---------- */
 /* VelocityVolumeDivergenceOnLeafs (src/PoissonSolver.cu:66-142) into div_channel: with
@@ -258,11 +264,13 @@ cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel);
  * over all tiles: 4096 bytes moved per tile, the reference point that scales
  * hardware byte counters), CG_KERNEL_GS_PROLONG (prolongation to `level` fused
  * with the first up-sweep colour pass), CG_KERNEL_GS_SWEEP_BF16 and
- * CG_KERNEL_GS_PROLONG_BF16 (the same with bfloat16 V-cycle storage).
+ * CG_KERNEL_GS_PROLONG_BF16 (the same with bfloat16 V-cycle storage),
+ * CG_KERNEL_GS_SWEEP_F16 (the colour sweep with float16 storage).
  * Channels are overwritten. */
 enum { CG_KERNEL_GS_SWEEP = 0, CG_KERNEL_APPLY = 1, CG_KERNEL_RESTRICT = 2, CG_KERNEL_PROLONG = 3,
        CG_KERNEL_JACOBI = 4, CG_KERNEL_UPDATE_XR = 5, CG_KERNEL_COARSE = 6, CG_KERNEL_EMPTY = 7,
-       CG_KERNEL_COPY = 8, CG_KERNEL_GS_PROLONG = 9, CG_KERNEL_GS_SWEEP_BF16 = 10, CG_KERNEL_GS_PROLONG_BF16 = 11 };
+       CG_KERNEL_COPY = 8, CG_KERNEL_GS_PROLONG = 9, CG_KERNEL_GS_SWEEP_BF16 = 10, CG_KERNEL_GS_PROLONG_BF16 = 11,
+       CG_KERNEL_GS_SWEEP_F16 = 12 };
 cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int32_t reps, double* avg_us,
                               int64_t* tiles);
 

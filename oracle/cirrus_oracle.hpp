@@ -91,6 +91,35 @@ inline float bf16_round(float v) {
   return r;
 }
 
+// The value a float16 store of v leaves: v clamped to the largest finite
+// float16 (+-65504, so no store overflows), then rounded to 11 significant
+// bits (10 below the 2^-14 subnormal threshold, spacing 2^-24), nearest,
+// ties to even.  NaN passes through.
+inline float fp16_round(float v) {
+  if (v > 65504.f) v = 65504.f;
+  if (v < -65504.f) v = -65504.f;
+  if (!(v == v) || v == 0.f) return v;
+  const double a = std::fabs(double(v));
+  int e;
+  std::frexp(a, &e);  // a = m 2^e, m in [0.5, 1)
+  const int q = std::max(e - 11, -24);  // exponent of the last kept bit
+  const double r = std::nearbyint(std::ldexp(double(v), -q));  // exact scaling, round to nearest even
+  return float(std::ldexp(r, q));
+}
+
+// Exponent k of the power-of-two scale 2^k that a float16-storage solve
+// applies to its right-hand side (and undoes on x and r): the mean square of
+// b over the problem's cells, bb / cells = 2^e f with f in [1, 2), gives
+// k = -floor(e / 2) - 4, so the scaled root mean square lies near 2^-4 and the
+// V-cycle's coarse sums stay far below the float16 maximum.  Scaling by a power
+// of two is exact in FP32 and FP64, so only the float16 stores see it.
+inline int fp16_scale_exp(double bb, double cells) {
+  if (!(bb > 0.0)) return 0;
+  const int e = std::ilogb(bb / cells);
+  const int half = (e >= 0) ? e / 2 : -((-e + 1) / 2);
+  return std::min(120, std::max(-120, -half - 4));
+}
+
 // HATileInfo.h:19 / PoissonTile.h:6
 enum : uint8_t { LEAF = 1, NONLEAF = 2, GHOST = 4 };
 enum : uint8_t { DIRICHLET = 1, INTERIOR = 2, NEUMANN = 4 };
@@ -973,12 +1002,13 @@ template <class T>
 struct Solver {
   Problem<T>& P;
   T one_over_alpha = T(1), prolong_coeff = T(1);  // GMGSolver.h:19-22
-  // Mixed-precision form (no reference counterpart): every value the V-cycle
-  // stores in its x and rhs channels is rounded to bfloat16 (T=float only).
-  bool vround = false;
+  // Mixed-precision forms (no reference counterpart): every value the V-cycle
+  // stores in its x and rhs channels is rounded to bfloat16 (vround 1) or to
+  // float16 (vround 2) (T=float only).
+  int vround = 0;
   explicit Solver(Problem<T>& p) : P(p) {}
   T st(T v) const {
-    if constexpr (std::is_same_v<T, float>) return vround ? bf16_round(v) : v;
+    if constexpr (std::is_same_v<T, float>) return vround == 1 ? bf16_round(v) : vround == 2 ? fp16_round(v) : v;
     else return v;
   }
   Grid<T>& G() { return P.g(); }
@@ -1260,8 +1290,28 @@ struct Solver {
   // sync_stride <= 0 the reference would go on and divide 0 by 0).
   // thr_floor (no reference counterpart, used by the refinement's inner
   // solves): the stop threshold is max(tol^2 |b|^2, thr_floor).
+  // With float16 V-cycle storage (vround 2) the right-hand side is first
+  // scaled by 2^k (fp16_scale_exp) and x and r are scaled back at the end;
+  // |b|^2 and |r|^2 are reported unscaled, the threshold floor scaled with b.
   Result solve(int max_iters, double tol, int level_iters, int coarsest_iters, bool pure_neumann,
                int smoother = SMOOTH_RBGS, T omega = T(2) / T(3), int sync_stride = 1, double thr_floor = 0.0) {
+    if (vround != 2 || !std::is_same_v<T, float>) return solve_body(max_iters, tol, level_iters, coarsest_iters, pure_neumann, smoother, omega, sync_stride, thr_floor);
+    const double bb0 = Dot(CH_R, CH_R, LEAF);
+    const int k = fp16_scale_exp(bb0, 512.0 * double(P.leafCanon.size()));
+    const T up = T(std::ldexp(1.0, k)), down = T(std::ldexp(1.0, -k));
+    ForLeafCells([&](TileData<T>& t, int c) { t.ch[CH_R][c] = t.ch[CH_R][c] * up; });
+    Result r = solve_body(max_iters, tol, level_iters, coarsest_iters, pure_neumann, smoother, omega, sync_stride,
+                          std::ldexp(thr_floor, 2 * k));
+    ForLeafCells([&](TileData<T>& t, int c) {
+      t.ch[CH_X][c] = t.ch[CH_X][c] * down;
+      t.ch[CH_R][c] = t.ch[CH_R][c] * down;
+    });
+    r.bb = std::ldexp(r.bb, -2 * k);
+    r.rr = std::ldexp(r.rr, -2 * k);
+    return r;
+  }
+  Result solve_body(int max_iters, double tol, int level_iters, int coarsest_iters, bool pure_neumann,
+                    int smoother, T omega, int sync_stride, double thr_floor) {
     ForLeafCells([](TileData<T>& t, int c) { t.ch[CH_X][c] = T(0); });
     double bb = Dot(CH_R, CH_R, LEAF);
     if (bb == 0.0) return {0, 0.0, 0.0, 0.0};
@@ -1367,10 +1417,10 @@ std::unique_ptr<Problem<B>> ConvertProblem(Problem<A>& a) {
 struct RefineResult { int outer, iters, converged; double rel_res, bb, rr; };
 inline RefineResult SolveRefined(Problem<float>& Pf, Problem<double>& Pd, double tol, double inner_tol, int max_outer,
                                  int max_iters, int level_iters, int coarsest_iters, bool pure_neumann, int smoother,
-                                 float omega, int sync_stride, bool bf16) {
+                                 float omega, int sync_stride, int storage) {
   Solver<double> Sd(Pd);
   Solver<float> Sf(Pf);
-  Sf.vround = bf16;
+  Sf.vround = storage;
   Grid<double>& gd = Pd.g();
   Grid<float>& gf = Pf.g();
   for (const Info& in : gd.all) { auto& t = gd.tile(in); for (int c = 0; c < 512; c++) t.ch[CH_X][c] = 0.0; }

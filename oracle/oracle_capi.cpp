@@ -187,7 +187,7 @@ void orc_vcycle(void* h, int xch, int fch, int rhsch, int level_iters, int coars
   with(h, [&](auto& P) {
     using T = std::remove_reference_t<decltype(P.g().h0)>;
     Solver<T> S(P);
-    S.vround = storage == 1;
+    S.vround = storage;
     S.VCycle(xch, fch, rhsch, level_iters, coarsest_iters, smoother, T(omega));
     return 0;
   });
@@ -200,13 +200,14 @@ void orc_recenter(void* h, int ch) {
 }
 // GMGSolver::solve.  out[0]=relative residual (recursive r), out[1]=|b|^2,
 // out[2]=|r|^2, out[3]=wall seconds of the solve.  Returns the iteration count.
-// storage 1: V-cycle stores rounded to bfloat16 (FP32 handles only).
+// storage 1: V-cycle stores rounded to bfloat16; 2: to float16, with the
+// right-hand side scaled by a power of two (FP32 handles only).
 int orc_solve(void* h, int max_iters, double tol, int level_iters, int coarsest_iters, int pure_neumann,
               int smoother, double omega, int sync_stride, int storage, double* out) {
   return with(h, [&](auto& P) {
     using T = std::remove_reference_t<decltype(P.g().h0)>;
     Solver<T> S(P);
-    S.vround = storage == 1;
+    S.vround = storage;
     auto t0 = std::chrono::steady_clock::now();
     auto r = S.solve(max_iters, tol, level_iters, coarsest_iters, pure_neumann != 0, smoother, T(omega), sync_stride);
     auto t1 = std::chrono::steady_clock::now();
@@ -240,7 +241,7 @@ int orc_solve_refined(void* h, double tol, double inner_tol, int max_outer, int 
   auto t0 = std::chrono::steady_clock::now();
   o->mirror = ConvertProblem<double>(*o->pf);
   auto r = SolveRefined(*o->pf, *o->mirror, tol, inner_tol, max_outer, max_iters, level_iters, coarsest_iters,
-                        pure_neumann != 0, smoother, float(omega), sync_stride, storage == 1);
+                        pure_neumann != 0, smoother, float(omega), sync_stride, storage);
   auto t1 = std::chrono::steady_clock::now();
   if (out) {
     out[0] = r.rel_res; out[1] = r.bb; out[2] = r.rr; out[3] = r.iters; out[4] = r.converged;
@@ -257,6 +258,10 @@ int orc_get_refined_x(void* h, double* out) {
   return 1;
 }
 // Round FP32 values to bfloat16 as the V-cycle stores do (for pinning).
+void orc_fp16_round(const float* in, float* out, long n) {
+  for (long i = 0; i < n; i++) out[i] = fp16_round(in[i]);
+}
+int orc_fp16_scale_exp(double bb, double cells) { return fp16_scale_exp(bb, cells); }
 void orc_bf16_round(const float* in, float* out, long n) {
   for (long i = 0; i < n; i++) out[i] = bf16_round(in[i]);
 }

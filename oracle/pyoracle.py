@@ -17,7 +17,21 @@ DIRICHLET, INTERIOR, NEUMANN = 1, 2, 4
 CH_X, CH_B, CH_R, CH_P, CH_AP, CH_Z, CH_TMP, CH_U, CH_V, CH_W = 0, 1, 1, 2, 3, 4, 5, 6, 7, 8
 SCENE_UNIFORM, SCENE_SPHERE_BAND, SCENE_PLANE, SCENE_VORTEX = 0, 1, 2, 3
 SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
-STORE_F32, STORE_BF16 = 0, 1
+STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
+
+
+def fp16_round(a):
+    """FP32 values as a float16 V-cycle store leaves them (clamped to +-65504,
+    rounded to nearest even; float32 array)."""
+    a = np.ascontiguousarray(a, np.float32)
+    out = np.empty_like(a)
+    lib().orc_fp16_round(_ptr(a), _ptr(out), a.size)
+    return out
+
+
+def fp16_scale_exp(bb, cells):
+    """Exponent of the power-of-two right-hand-side scale of a float16-storage solve."""
+    return int(lib().orc_fp16_scale_exp(float(bb), float(cells)))
 
 
 def bf16_round(a):
@@ -125,6 +139,9 @@ def lib():
         L.orc_get_refined_x.restype = i
         L.orc_get_refined_x.argtypes = [vp, vp]
         L.orc_bf16_round.argtypes = [vp, vp, ctypes.c_long]
+        L.orc_fp16_round.argtypes = [vp, vp, ctypes.c_long]
+        L.orc_fp16_scale_exp.restype = i
+        L.orc_fp16_scale_exp.argtypes = [d, d]
         L.orc_velocity_divergence.argtypes = [vp, i, i]
         L.orc_add_gradient.argtypes = [vp, i]
         L.orc_damped_jacobi_solve.restype = i

@@ -3,7 +3,9 @@
 BF16 storage: the MGPCG whose V-cycle vectors (x on every level, its rhs and
 scratch) are stored as bfloat16 -- FP32 arithmetic, every store rounded to
 nearest even -- equals the oracle's solve with the same rounding at the same
-stores, bit for bit (iterations, |r|^2, pressure).
+stores, bit for bit (iterations, |r|^2, pressure).  F16 storage (float16
+stores, clamped to +-65504, with the right-hand side scaled by a power of two
+for the solve) likewise.
 
 Iterative refinement: x and the residual in FP64 around that MGPCG.  The whole
 outer loop equals the oracle's SolveRefined bit for bit (steps, iterations, the
@@ -53,12 +55,13 @@ def leaf_eq(a, b, typ):
 
 @pytest.mark.parametrize("name", list(SCENES))
 @pytest.mark.parametrize("smoother,nu", [(ca.SMOOTH_RBGS, 2), (ca.SMOOTH_JACOBI, 2), (ca.SMOOTH_RBGS, 1)])
-def test_bf16_storage_solve_bitexact(name, smoother, nu):
+@pytest.mark.parametrize("storage", [ca.STORE_BF16, ca.STORE_F16])
+def test_16bit_storage_solve_bitexact(name, smoother, nu, storage):
     o, P, typ, b, _ = pair(name)
     set_b(o, P, b)
-    ro = o.solve(max_iters=200, tol=1e-6, level_iters=nu, coarsest_iters=10, smoother=smoother, storage=po.STORE_BF16)
+    ro = o.solve(max_iters=200, tol=1e-6, level_iters=nu, coarsest_iters=10, smoother=smoother, storage=storage)
     S = ca.GMGSolver()
-    S.solve(P, False, 200, 1e-6, nu, 10, 1, False, smoother=smoother, storage=ca.STORE_BF16)
+    S.solve(P, False, 200, 1e-6, nu, 10, 1, False, smoother=smoother, storage=storage)
     r = S.last
     assert r["converged"] == 1
     assert (r["iters"], r["rr"], r["bb"]) == (ro["iters"], ro["rr"], ro["bb"])
@@ -66,13 +69,14 @@ def test_bf16_storage_solve_bitexact(name, smoother, nu):
 
 
 @pytest.mark.parametrize("use_graph,stride", [(False, 1), (True, 3)])
-def test_bf16_storage_launch_modes_and_pure_neumann(use_graph, stride):
+@pytest.mark.parametrize("storage", [ca.STORE_BF16, ca.STORE_F16])
+def test_16bit_storage_launch_modes_and_pure_neumann(use_graph, stride, storage):
     o, P, typ, b, _ = pair("allneumann", ALL_NEUMANN)
     set_b(o, P, b)
     ro = o.solve(max_iters=200, tol=1e-6, level_iters=2, coarsest_iters=10, pure_neumann=True, sync_stride=stride,
-                 storage=po.STORE_BF16)
+                 storage=storage)
     S = ca.GMGSolver()
-    S.solve(P, False, 200, 1e-6, 2, 10, stride, True, use_graph=use_graph, storage=ca.STORE_BF16)
+    S.solve(P, False, 200, 1e-6, 2, 10, stride, True, use_graph=use_graph, storage=storage)
     assert (S.last["iters"], S.last["rr"]) == (ro["iters"], ro["rr"])
     assert leaf_eq(P.get(ca.CH_X), o.get(po.CH_X).astype(np.float32), typ)
 
@@ -81,7 +85,7 @@ def test_f32_solve_unchanged_after_bf16_solve():
     """Both storage types on one handle: the FP32 solve still equals the
     reference transcription."""
     o, P, typ, b, _ = pair("band")
-    for storage in (ca.STORE_BF16, ca.STORE_F32, ca.STORE_BF16):
+    for storage in (ca.STORE_BF16, ca.STORE_F32, ca.STORE_F16, ca.STORE_BF16, ca.STORE_F32):
         set_b(o, P, b)
         ro = o.solve(max_iters=200, tol=1e-6, level_iters=2, coarsest_iters=10, storage=storage)
         S = ca.GMGSolver()
@@ -100,19 +104,20 @@ def fp64_error(o, typ, b, x64):
 
 
 @pytest.mark.parametrize("name", list(SCENES))
-def test_refined_bitexact_and_fp64_error(name):
+@pytest.mark.parametrize("storage", [ca.STORE_BF16, ca.STORE_F16])
+def test_refined_bitexact_and_fp64_error(name, storage):
     o, P, typ, b, _ = pair(name)
     set_b(o, P, b)
-    ro = o.solve_refined(tol=1e-8, inner_tol=1e-4, level_iters=2, coarsest_iters=10, storage=po.STORE_BF16)
+    ro = o.solve_refined(tol=1e-8, inner_tol=1e-4, level_iters=2, coarsest_iters=10, storage=storage)
     S = ca.GMGSolver()
-    r = S.solve_refined(P, 1e-8, 1e-4, 20, 1000, 2, 10, storage=ca.STORE_BF16)[0]
+    r = S.solve_refined(P, 1e-8, 1e-4, 20, 1000, 2, 10, storage=storage)[0]
     assert r["converged"] == 1 and r["rel_res"] < 1e-8
     assert (r["outer"], r["iters"], r["rr"], r["bb"]) == (ro["outer"], ro["iters"], ro["rr"], ro["bb"])
     x64 = P.get_f64()
     assert leaf_eq(x64, o.refined_x(), typ), "FP64 pressure differs from the oracle"
     assert leaf_eq(P.get(ca.CH_X), x64.astype(np.float32), typ)
     err = fp64_error(o, typ, b, x64)
-    print(f"\n{name}: {r['outer']} refinement steps, {r['iters']} CG iterations, FP64 residual {r['rel_res']:.2e}, "
+    print(f"\n{name} storage {storage}: {r['outer']} refinement steps, {r['iters']} CG iterations, FP64 residual {r['rel_res']:.2e}, "
           f"error vs FP64 oracle {err:.2e}")
     assert err < REFINED_ERR_TOL, err
 
@@ -147,9 +152,11 @@ def test_refined_outer_cap_and_zero_rhs():
     assert not P.get_f64()[typ == po.LEAF].any()
 
 
-def test_refined_batch_equals_alone():
+@pytest.mark.parametrize("storage", [ca.STORE_BF16, ca.STORE_F16])
+def test_refined_batch_equals_alone(storage):
     """Refinement on a batched handle: every problem runs its own outer loop and
-    equals the oracle and its solve alone."""
+    equals the oracle and its solve alone (with float16 storage each problem
+    has its own right-hand-side scale)."""
     names = ["plane", "band", "vortex2root", "uniform32"]
     trees, bs = [], []
     for nm in names:
@@ -158,12 +165,12 @@ def test_refined_batch_equals_alone():
         bs.append(b.astype(np.float32))
     B = ca.Problem.batch(trees)
     B.set(ca.CH_B, np.concatenate(bs))
-    res = ca.GMGSolver().solve_refined(B, 1e-8, 1e-4, 20, 1000, 2, 10, storage=ca.STORE_BF16)
+    res = ca.GMGSolver().solve_refined(B, 1e-8, 1e-4, 20, 1000, 2, 10, storage=storage)
     xb = B.split(B.get_f64())
     for q, nm in enumerate(names):
         o, P, typ, b, tree = pair(nm)
         set_b(o, P, b)
-        ro = o.solve_refined(tol=1e-8, inner_tol=1e-4, level_iters=2, coarsest_iters=10, storage=po.STORE_BF16)
+        ro = o.solve_refined(tol=1e-8, inner_tol=1e-4, level_iters=2, coarsest_iters=10, storage=storage)
         assert (res[q]["outer"], res[q]["iters"], res[q]["rr"]) == (ro["outer"], ro["iters"], ro["rr"]), nm
         assert leaf_eq(xb[q], o.refined_x(), typ), nm
 
@@ -196,3 +203,27 @@ def test_server_bf16_cold_requests_equal_solve():
         assert (res["iters"], res["rr"]) == (G.last["iters"], G.last["rr"]), q
         leaf = sc.tree.type == po.LEAF
         assert np.array_equal(x[leaf], P.get(ca.CH_X)[leaf]), q
+
+
+def test_f16_batch_solve_equals_alone_and_server_refuses_f16():
+    """The float16 solve on a batched handle: every problem (its own scale)
+    equals the oracle; the serving loop does not offer float16 storage."""
+    names = ["plane", "band", "uniform32"]
+    trees, bs = [], []
+    for nm in names:
+        o, P, typ, b, tree = pair(nm)
+        trees.append(tree)
+        bs.append(b.astype(np.float32))
+    B = ca.Problem.batch(trees)
+    B.set(ca.CH_B, np.concatenate(bs))
+    res = ca.GMGSolver().solve_batch(B, 200, 1e-6, 2, 10, storage=ca.STORE_F16, verify_f64=True)
+    xb = B.split(B.get(ca.CH_X))
+    for q, nm in enumerate(names):
+        o, P, typ, b, tree = pair(nm)
+        set_b(o, P, b)
+        ro = o.solve(max_iters=200, tol=1e-6, level_iters=2, coarsest_iters=10, storage=po.STORE_F16)
+        assert (res[q]["iters"], res[q]["rr"], res[q]["bb"]) == (ro["iters"], ro["rr"], ro["bb"]), nm
+        assert leaf_eq(xb[q], o.get(po.CH_X).astype(np.float32), typ), nm
+        assert 0 < res[q]["rel_res_f64"] < 2e-6
+    with pytest.raises(ca.CirrusError, match="float16"):
+        ca.Server(1 << 28, 4, storage=ca.STORE_F16)

@@ -1,7 +1,7 @@
 """Schedules change how the kernels cover the tiles, never a result: every
 combination of LDS-resident coarse depth, tile order and tile block (tiles per
 wavefront of the colour sweeps; lists of odd length leave one wavefront a
-single tile) gives the oracle's solve bit for bit (FP32 and bfloat16 storage,
+single tile) gives the oracle's solve bit for bit (FP32, bfloat16 and float16 storage,
 where the LDS kernel rounds exactly where the grid-wide kernels store).  The autotuner picks among such
 schedules and among smoother settings by timing solves of the handle's own
 right-hand sides."""
@@ -21,7 +21,7 @@ SCENES = {
 
 
 @pytest.mark.parametrize("name", list(SCENES))
-@pytest.mark.parametrize("storage", [ca.STORE_F32, ca.STORE_BF16])
+@pytest.mark.parametrize("storage", [ca.STORE_F32, ca.STORE_BF16, ca.STORE_F16])
 def test_every_schedule_gives_the_oracle_solve(name, storage):
     o = po.Oracle(**SCENES[name])
     lev, crd, typ, nbr = o.topology()

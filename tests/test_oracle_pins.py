@@ -551,6 +551,98 @@ def test_bf16_vcycle_stores_only_bf16_values_and_stays_close(smoother, nu):
     assert 0 < rel < 2e-2, rel
 
 
+def test_fp16_round_every_value_and_tie():
+    """float16 stores: every float16 value is kept, the midpoints between
+    neighbouring float16 values and random FP32 patterns round as numpy's
+    float32 -> float16 conversion (nearest, ties to even, subnormals kept), and
+    magnitudes past the largest finite value clamp to +-65504."""
+    h = np.arange(0, 1 << 16, dtype=np.uint32).astype(np.uint16).view(np.float16)
+    f = h[np.isfinite(h)].astype(np.float32)
+    assert np.array_equal(po.fp16_round(f).view(np.uint32), f.view(np.uint32))
+    pos = np.unique(np.abs(f.astype(np.float64)))
+    mids = ((pos[:-1] + pos[1:]) / 2).astype(np.float32)  # exact in FP32
+    mids = np.concatenate([mids, -mids])
+    assert np.array_equal(po.fp16_round(mids), mids.astype(np.float16).astype(np.float32))
+    rng = np.random.default_rng(0)
+    r = rng.integers(0, 1 << 32, 1 << 20, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    r = r[np.isfinite(r) & (np.abs(r) <= 65504)]
+    assert np.array_equal(po.fp16_round(r), r.astype(np.float16).astype(np.float32))
+    big = np.array([65504.0, 65519.0, 65520.0, 1e6, 3e38, -7e4], np.float32)
+    assert np.array_equal(po.fp16_round(big), np.array([65504, 65504, 65504, 65504, 65504, -65504], np.float32))
+
+
+def test_fp16_scale_exponent():
+    """The float16 solve's right-hand-side scale 2^k puts the mean square of
+    the scaled b in [2^-8, 2^-6) (root mean square in [2^-4, 2^-3))."""
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        cells = float(512 * rng.integers(1, 20000))
+        bb = cells * 2.0 ** rng.uniform(-90, 60)
+        k = po.fp16_scale_exp(bb, cells)
+        ms = (bb / cells) * 4.0 ** k
+        assert 2.0 ** -8 <= ms < 2.0 ** -6, (bb, cells, k)
+    assert po.fp16_scale_exp(0.0, 512.0) == 0
+
+
+def _is_fp16(a):
+    a = np.ascontiguousarray(a, np.float32)
+    return np.array_equal(a.astype(np.float16).astype(np.float32), a)
+
+
+def test_fp16_vcycle_stores_only_fp16_values_and_stays_close():
+    """With float16 storage every value the V-cycle leaves in its x and rhs
+    channels is a float16 value, and on a right-hand side scaled as the solve
+    scales it, z stays within float16 accuracy of the FP32 V-cycle."""
+    o = po.Oracle(fp64=False, **SCENES["band"])
+    lev, crd, typ, nbr = o.topology()
+    b = o.get(po.CH_B)
+    k = po.fp16_scale_exp(o.dot(po.CH_B, po.CH_B), 512.0 * np.sum(typ == po.LEAF))
+    bs = (b.astype(np.float32) * np.float32(2.0 ** k)).astype(np.float64)
+    o.set(po.CH_B, bs)
+    o.vcycle(po.CH_Z, po.CH_R, po.CH_AP, 2, 10, po.SMOOTH_RBGS, 2.0 / 3.0, storage=po.STORE_F16)
+    z16, rhs16 = o.get(po.CH_Z), o.get(po.CH_AP)
+    assert _is_fp16(z16) and _is_fp16(rhs16)
+    o.set(po.CH_B, bs)
+    o.vcycle(po.CH_Z, po.CH_R, po.CH_AP, 2, 10, po.SMOOTH_RBGS, 2.0 / 3.0)
+    z32 = o.get(po.CH_Z)
+    assert not _is_fp16(z32)
+    rel = np.linalg.norm(z16 - z32) / np.linalg.norm(z32)
+    assert 0 < rel < 5e-3, rel
+
+
+@pytest.mark.parametrize("j", [-7, 5])
+def test_fp16_solve_is_exactly_scale_equivariant(j):
+    """The float16 solve of 2^j b is 2^j times the solve of b, bit for bit (the
+    scale exponent absorbs j), with the same iterations; it converges."""
+    o = po.Oracle(fp64=False, **SCENES["band"])
+    b = o.get(po.CH_B)
+    r1 = o.solve(max_iters=200, tol=1e-6, level_iters=2, coarsest_iters=10, storage=po.STORE_F16)
+    x1 = o.get(po.CH_X)
+    o.set(po.CH_B, b * 2.0 ** j)
+    r2 = o.solve(max_iters=200, tol=1e-6, level_iters=2, coarsest_iters=10, storage=po.STORE_F16)
+    x2 = o.get(po.CH_X)
+    assert r1["iters"] == r2["iters"] and r1["rel_res"] == r2["rel_res"] and r1["rel_res"] < 1e-6
+    assert r2["bb"] == r1["bb"] * 4.0 ** j
+    assert np.array_equal(x2, x1 * 2.0 ** j)
+
+
+@pytest.mark.parametrize("storage", [po.STORE_BF16, po.STORE_F16])
+def test_refined_solve_meets_the_independent_fp64_system_16bit(storage):
+    """Refinement with either 16-bit V-cycle storage reaches the FP64 system's
+    tolerance (the bfloat16 case is checked in detail below)."""
+    o64, lev, crd, typ, nbr, ct, lc = scene("band")
+    o = po.Oracle(fp64=False, **SCENES["band"])
+    b = o.get(po.CH_B)
+    r = o.solve_refined(tol=1e-9, inner_tol=1e-4, level_iters=2, coarsest_iters=10, storage=storage)
+    assert r["converged"] == 1 and r["rel_res"] < 1e-9
+    x = lc.from_tiles(o.refined_x())
+    m = lc.ct == po.INTERIOR
+    A = lc.matrix()[m][:, m]
+    bl = lc.from_tiles(b)[m]
+    res = np.linalg.norm(bl - A @ x[m]) / np.linalg.norm(bl)
+    assert res < 1e-9 and abs(res - r["rel_res"]) < 1e-3 * r["rel_res"], (res, r["rel_res"])
+
+
 def test_refined_solve_meets_the_independent_fp64_system():
     """Iterative refinement (FP64 x and residual, BF16-storage MGPCG inside):
     the FP64 x satisfies the independently assembled system to the outer
