@@ -131,10 +131,13 @@ def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=256, nscenes=8, bud
     S = ca.Server(budget, conc, level_iters=solver_kw["level_iters"], coarsest_iters=solver_kw["coarsest_iters"],
                   smoother=solver_kw["smoother"], storage=int(storage), refine=bool(refine))
     insts, sig = [], []
+    admit_s = []
     for w in range(conc):
         tree, b0 = trees_b0[w % nscenes]
         try:
+            t_a = time.perf_counter()
             insts.append(S.admit(tree))
+            admit_s.append(time.perf_counter() - t_a)
         except ca.CirrusBusy:
             break
         sig.append(float(np.sqrt(1 - rho * rho) * np.sqrt(np.mean(b0[b0 != 0] ** 2))))
@@ -208,6 +211,8 @@ def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=256, nscenes=8, bud
         "fp64_rel_res_max": float(np.max(f64)) if (refine and f64) else None,
         "fp64_checked": len(f64) if refine else 0,
         "fp64_over_tol": int(np.sum(np.array(f64) >= 1e-6)) if refine else None,
+        "admit_ms_median": float(np.median(admit_s)) * 1e3 if admit_s else None,
+        "admit_ms_max": float(np.max(admit_s)) * 1e3 if admit_s else None,
         "_items": trees_b0,
     }
 
@@ -648,7 +653,9 @@ def main():
     t0 = time.perf_counter()
     scene = ca.Scene(**scene_kw)
     t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
     P = ca.Problem(scene.tree)
+    t_create = time.perf_counter() - t0
     if args.kernels_only:
         if args.kernels_batch > 0:
             # the same kernels at batch scale, past the last-level cache: the
@@ -724,6 +731,7 @@ def main():
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "build_s": t_build,
+        "create_s": t_create,
         "roofline": {"bound": "hbm", "kernel": f"k_gs_color level {L} ({tiles} tiles)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "avg_launch_us": us, "alg_bytes_per_launch": alg_bytes},
@@ -770,7 +778,10 @@ def main():
             forms["plain_drop_in_batch_problems"] = dict(base, workload="the batched forms' problems, FP32, one handle "
                                                          "each, solved one after another from x = 0 (graph replay)")
             note(f"plain drop-in on the batch problems: {base['solves_per_s']:.1f} solves/s")
+        t0 = time.perf_counter()
         B = ca.Problem.batch([sc.tree for sc in scenes])
+        t_pool = time.perf_counter() - t0
+        note(f"pool of {args.batch} problems ({B.n} tiles) created in {t_pool:.2f} s")
         for name, storage, refine, tune in [("batched", 0, False, False), ("batched_mixed", 1, True, False),
                                             ("batched_mixed_f16", 2, True, False), ("batched_autotuned", 0, False, True)]:
             if name not in wanted:
@@ -780,6 +791,7 @@ def main():
             if base:
                 f["vs_plain_solves_per_s"] = f["solves_per_s"] / base["solves_per_s"]
             f["vs_plain_adaptive256"] = f["solves_per_s"] / out["value"]
+            f["pool_create_s"] = t_pool
             forms[name] = f
             note(f"{name}: {f['solves_per_s']:.1f} solves/s")
         B.close()

@@ -1011,15 +1011,24 @@ void set_schedule(cg_problem& P, int coarse_shift, int swizzle, int tile_block) 
 // before every solve): first the ones that leave every result unchanged (tile
 // order and tile block, then the LDS coarse depth with the best of those) with
 // the caller's smoother, then the smoother,
-// sweeps per level and (if allowed) bfloat16 storage with the best of those.
+// sweeps per level and (if allowed) 16-bit storage with the best of those.
 // A candidate counts only if every problem converges; each is timed as the
 // best of two solves after a warm-up (graph capture).  The pool keeps the
 // chosen schedule; *best receives the chosen solver parameters.
-void autotune(cg_problem& P, const cg_params& base, int allow_bf16, cg_params* best, cg_tune_report* rep) {
+// A device buffer freed when it goes out of scope (also when a call throws).
+struct DeviceBuffer {
+  void* p = nullptr;
+  explicit DeviceBuffer(size_t bytes) { HIPC(hipMalloc(&p, bytes)); }
+  ~DeviceBuffer() { if (p) (void)hipFree(p); }
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+};
+
+void autotune(cg_problem& P, const cg_params& base, int allow_16bit, cg_params* best, cg_tune_report* rep) {
   check_params(base);
   const size_t bytes = size_t(P.n) * 512 * sizeof(float);
-  float* bkeep = nullptr;
-  HIPC(hipMalloc(&bkeep, bytes));
+  DeviceBuffer keep(bytes);
+  float* bkeep = static_cast<float*>(keep.p);
   HIPC(hipMemcpyAsync(bkeep, P.d_ch[CG_CH_R], bytes, hipMemcpyDeviceToDevice, P.stream));
   std::vector<cg_result> res(P.nprob);
   int candidates = 0;
@@ -1057,7 +1066,7 @@ void autotune(cg_problem& P, const cg_params& base, int allow_bf16, cg_params* b
   cg_params chosen = base;
   struct Cand { int smoother, nu; };
   const Cand cands[] = {{CG_SMOOTH_RBGS, 1}, {CG_SMOOTH_RBGS, 2}, {CG_SMOOTH_RBGS, 3}, {CG_SMOOTH_JACOBI, 2}, {CG_SMOOTH_JACOBI, 3}};
-  for (int storage = CG_STORE_F32; storage <= (allow_bf16 ? CG_STORE_F16 : CG_STORE_F32); storage++)
+  for (int storage = CG_STORE_F32; storage <= (allow_16bit ? CG_STORE_F16 : CG_STORE_F32); storage++)
     for (const Cand& cd : cands) {
       cg_params prm = base;
       prm.smoother = cd.smoother;
@@ -1069,7 +1078,6 @@ void autotune(cg_problem& P, const cg_params& base, int allow_bf16, cg_params* b
     }
   HIPC(hipMemcpyAsync(P.d_ch[CG_CH_R], bkeep, bytes, hipMemcpyDeviceToDevice, P.stream));
   HIPC(hipStreamSynchronize(P.stream));
-  HIPC(hipFree(bkeep));
   *best = chosen;
   if (rep) {
     rep->candidates = candidates;
@@ -1246,10 +1254,10 @@ cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle, 
   });
 }
 
-cg_status cg_autotune(cg_problem* p, const cg_params* base, int32_t allow_bf16, cg_params* best, cg_tune_report* report) {
+cg_status cg_autotune(cg_problem* p, const cg_params* base, int32_t allow_16bit, cg_params* best, cg_tune_report* report) {
   return guard([&] {
     if (!p || !base || !best) fail(CG_ERR_INVALID, "cg_autotune: null argument");
-    autotune(*p, *base, allow_bf16, best, report);
+    autotune(*p, *base, allow_16bit, best, report);
   });
 }
 

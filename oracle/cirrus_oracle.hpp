@@ -51,6 +51,11 @@
 //   dampedJacobiSolve              closed-form first iterate, residual decreases
 //   bf16_round                     every bfloat16 value and tie case, against
 //                                  an exact rational rounding in Python
+//   fp16_round, fp16_scale_exp     every float16 value and midpoint and random
+//                                  patterns against numpy's float16
+//                                  conversion; the scaled mean square's range;
+//                                  the float16 solve is exactly equivariant
+//                                  under power-of-two scalings of b
 //   SolveRefined (no reference counterpart: the mixed-precision form)
 //                                  its FP64 residual against the independent
 //                                  assembly; the refined x against a direct
