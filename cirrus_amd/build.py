@@ -16,7 +16,7 @@ ARCH = os.environ.get("CIRRUS_OFFLOAD_ARCH", "gfx950")
 # -ffp-contract=off: the kernels spell out every fused multiply-add so results
 # match the CPU transcription bit for bit.  Packed FP32 instructions are
 # disabled for the device code.
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
           "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
@@ -38,7 +38,7 @@ def build(verbose=True):
     _run([HIPCC, "-x", "c++"] + COMMON + ["-c", os.path.join(CSRC, "tilefile.cpp"), "-o", file_o], verbose)
     _run([HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, "mgpcg.hip"), "-o", mg_o], verbose)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, mg_o, tree_o, file_o], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, mg_o, tree_o, file_o], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -12,7 +13,9 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -362,32 +365,44 @@ void build_schedule(cg_problem& P) {
   const std::vector<int>& tbase = P.tbase;
   for (int q = 0; q < nprob; q++) P.coarse_C[q] = std::max(-1, P.coarse_auto[q] - P.coarse_shift);
   const int L = P.max_level;
-  // worklists (canonical order, problems one after another)
-  auto& W = P.lists;
-  W.clear();
-  auto add_list = [&](auto pred) {
-    DList dl;
-    dl.off = int(W.size());
-    for (int i = 0; i < n; i++) if (pred(i)) W.push_back(P.tiles[i]);
-    dl.n = int(W.size()) - dl.off;
-    return dl;
+  // worklists (canonical order, problems one after another): the tile indices
+  // of each list first, then one sized array of records
+  std::vector<std::vector<int>> bucket(L + 1);  // tiles of each level, in order
+  for (int i = 0; i < n; i++) bucket[lev[i]].push_back(i);
+  std::vector<std::pair<DList*, std::vector<int>>> sel;
+  auto add_list = [&](DList* dl, const std::vector<int>& from, auto pred) {
+    std::vector<int> ids;
+    for (int i : from) if (pred(i)) ids.push_back(i);
+    sel.emplace_back(dl, std::move(ids));
   };
+  std::vector<int> every(n);
+  for (int i = 0; i < n; i++) every[i] = i;
   auto grid_level = [&](int i) { return lev[i] > P.coarse_C[P.tiles[i].prob]; };
-  P.leaf = add_list([&](int i) { return ty[i] == LEAF; });
-  P.gather = add_list([&](int i) { return ty[i] == LEAF && tr_has_gchild(P.tiles[i]); });
-  P.applyl = add_list([&](int i) { return ty[i] == LEAF || ty[i] == GHOST; });
-  P.ghosts = add_list([&](int i) { return ty[i] == GHOST; });
-  P.all = add_list([&](int) { return true; });
+  add_list(&P.leaf, every, [&](int i) { return ty[i] == LEAF; });
+  add_list(&P.gather, every, [&](int i) { return ty[i] == LEAF && tr_has_gchild(P.tiles[i]); });
+  add_list(&P.applyl, every, [&](int i) { return ty[i] == LEAF || ty[i] == GHOST; });
+  add_list(&P.ghosts, every, [&](int i) { return ty[i] == GHOST; });
+  add_list(&P.all, every, [&](int) { return true; });
   P.act.assign(L + 1, DList());
   P.lvl.assign(L + 1, DList());
   P.actg.assign(L + 1, DList());
   P.lvlg.assign(L + 1, DList());
   for (int l = 0; l <= L; l++) {
-    P.act[l] = add_list([&](int i) { return lev[i] == l && ty[i] != GHOST; });
-    P.lvl[l] = add_list([&](int i) { return lev[i] == l; });
-    P.actg[l] = add_list([&](int i) { return lev[i] == l && ty[i] != GHOST && grid_level(i); });
-    P.lvlg[l] = add_list([&](int i) { return lev[i] == l && grid_level(i); });
+    add_list(&P.act[l], bucket[l], [&](int i) { return ty[i] != GHOST; });
+    add_list(&P.lvl[l], bucket[l], [&](int) { return true; });
+    add_list(&P.actg[l], bucket[l], [&](int i) { return ty[i] != GHOST && grid_level(i); });
+    add_list(&P.lvlg[l], bucket[l], [&](int i) { return grid_level(i); });
   }
+  auto& W = P.lists;
+  size_t total = 0;
+  for (auto& s : sel) {
+    s.first->off = int(total);
+    s.first->n = int(s.second.size());
+    total += s.second.size();
+  }
+  W.resize(total);
+  for (auto& s : sel)
+    for (size_t k = 0; k < s.second.size(); k++) W[size_t(s.first->off) + k] = P.tiles[s.second[k]];
 
   // coarse plans and slots
   P.plans.clear();
@@ -442,27 +457,61 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   P.leaf_begin.assign(nprob, 0);
   P.leaf_count.assign(nprob, 0);
   P.coarse_C.assign(nprob, -1);
-  int leaf_total = 0, ghost_total = 0;
+  // leaf and ghost slot bases from the type counts, then the trees are built
+  // independently on up to 16 host threads (each writes its own tile range)
+  std::vector<int> leaf_base(nprob + 1, 0), ghost_base(nprob + 1, 0);
   for (int q = 0; q < nprob; q++) {
+    int nl = 0, ng = 0;
+    if (descs[q].type)
+      for (int t = 0; t < descs[q].num_tiles; t++) { nl += descs[q].type[t] == LEAF; ng += descs[q].type[t] == GHOST; }
+    leaf_base[q + 1] = leaf_base[q] + nl;
+    ghost_base[q + 1] = ghost_base[q] + ng;
+  }
+  auto build_one = [&](int q) {
     TreeBuild B;
-    build_tree(&descs[q], q, tbase[q], leaf_total, ghost_total, B);
+    build_tree(&descs[q], q, tbase[q], leaf_base[q], ghost_base[q], B);
     for (int i = 0; i < descs[q].num_tiles; i++) {
       P.tiles[tbase[q] + i] = B.tiles[i];
       P.ext_of[tbase[q] + i] = tbase[q] + B.ext[i];
       lev[tbase[q] + i] = B.lev[i];
       ty[tbase[q] + i] = B.ty[i];
     }
-    P.leaf_begin[q] = leaf_total;
+    P.leaf_begin[q] = leaf_base[q];
     P.leaf_count[q] = B.nleaf;
     P.coarse_C[q] = coarse_depth(B.lev, B.ty);
-    leaf_total += B.nleaf;
-    ghost_total += B.nghost;
+  };
+  const int nthreads = std::min(nprob, std::max(1, std::min(16, int(std::thread::hardware_concurrency()))));
+  if (nthreads <= 1) {
+    for (int q = 0; q < nprob; q++) build_one(q);
+  } else {
+    std::atomic<int> next{0};
+    std::mutex mu;
+    cg_status first_st = CG_OK;
+    std::string first_msg;
+    int first_q = nprob;
+    auto worker = [&]() {
+      for (int q = next++; q < nprob; q = next++) {
+        try {
+          build_one(q);
+        } catch (const Fail& e) {
+          std::lock_guard<std::mutex> lk(mu);
+          if (q < first_q) { first_q = q; first_st = e.st; first_msg = g_err; }
+        } catch (const std::exception& e) {
+          std::lock_guard<std::mutex> lk(mu);
+          if (q < first_q) { first_q = q; first_st = CG_ERR_INVALID; first_msg = e.what(); }
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; t++) pool.emplace_back(worker);
+    for (auto& th : pool) th.join();
+    if (first_st != CG_OK) fail(first_st, first_msg);  // the failure of the first bad tree, as a serial build reports it
   }
+  const int leaf_total = leaf_base[nprob], ghost_total = ghost_base[nprob];
   P.n_leaf = leaf_total;
   P.n_ghost = ghost_total;
   P.max_level = 0;
   for (int i = 0; i < n; i++) P.max_level = std::max(P.max_level, lev[i]);
-  const int L = P.max_level;
 
   P.lev = lev;
   P.ty = ty;

@@ -152,6 +152,25 @@ def test_problem_create_rejects_bad_types_and_empty_trees():
     assert st == CG_ERR_INVALID and "h0" in msg
 
 
+def test_batch_create_reports_the_first_bad_tree():
+    """Trees of a batch are built on several host threads; a malformed tree
+    fails the whole batch before any device work, and with two malformed trees
+    the message is the first one's, as a one-by-one build would report."""
+    import ctypes
+    h0, lev, crd, typ, ct = _small_tree()
+    good = ca.Tree(h0, lev, crd, typ, ct)
+    dup = ca.Tree(h0, np.append(lev, lev[-1]), np.vstack([crd, crd[-1]]), np.append(typ, typ[-1]),
+                  np.vstack([ct, ct[-1]]))
+    keep = lev != 0
+    orphan = ca.Tree(h0, lev[keep], crd[keep], typ[keep], ct[keep])
+    for trees, word in (([good] * 5 + [dup] + [good] * 5 + [orphan], "duplicate"),
+                        ([good] * 7 + [orphan] + [dup], "parent")):
+        descs = (ca.TreeDesc * len(trees))(*[t.desc() for t in trees])
+        h = ctypes.c_void_p()
+        st = ca.lib().cg_problem_create_batch(descs, len(trees), ctypes.byref(h))
+        assert st == CG_ERR_INVALID and word in ca.lib().cg_last_error().decode()
+
+
 def test_scene_create_rejects_invalid_parameters():
     with pytest.raises(ca.CirrusError):
         ca.Scene(nroot=[0, 1, 1])
