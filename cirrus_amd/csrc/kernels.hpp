@@ -1257,6 +1257,28 @@ __global__ __launch_bounds__(64) void k_cg_init_warm(Ctx c, const TileRec* __res
   if (lane == 0) c.part[T.leaf_slot] = a;
 }
 
+// Pool build on the device: out[k] = tiles[ids[k]] (the worklists, records
+// copied as 16-byte words), and the cell types permuted from the caller's tile
+// order (src, tile t at t*512) to the canonical order (tile i from src tile
+// ext[i]).
+__global__ __launch_bounds__(256) void k_gather_tiles(const TileRec* __restrict__ tiles, const int32_t* __restrict__ ids,
+                                                      int64_t n, TileRec* __restrict__ out) {
+  constexpr int W = sizeof(TileRec) / 16;
+  const int64_t words = n * W;
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < words; k += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = k / W, w = k - r * W;
+    reinterpret_cast<int4*>(out + r)[w] = reinterpret_cast<const int4*>(tiles + ids[r])[w];
+  }
+}
+__global__ __launch_bounds__(64) void k_permute_ctype(const uint8_t* __restrict__ src, const int32_t* __restrict__ ext,
+                                                      int n, uint8_t* __restrict__ dst) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const uint64_t* s8 = reinterpret_cast<const uint64_t*>(src + size_t(ext[i]) * 512);
+  uint64_t* d8 = reinterpret_cast<uint64_t*>(dst + size_t(i) * 512);
+  d8[threadIdx.x] = s8[threadIdx.x];
+}
+
 // Face masks of every cell of the listed tiles (Ctx::fmask): bit e of cell c is
 // set when face e (x-, x+, y-, y+, z-, z+) is not cut, i.e. neither c nor its
 // neighbour across e is NEUMANN (GMGSolver.cu:26-39; a neighbour outside the

@@ -81,7 +81,6 @@ struct cg_problem {
   std::vector<int> leaf_count;   // per problem: LEAF tiles
   std::vector<int> coarse_C;     // per problem: deepest LDS-resident level (-1: none)
   std::vector<TileRec> tiles;    // one record per tile (internal order)
-  std::vector<TileRec> lists;    // all worklists, concatenated
   DList leaf, gather, applyl, ghosts, all;
   std::vector<DList> act, lvl;   // per level, all problems (operator API)
   std::vector<DList> actg, lvlg; // per level, tiles the V-cycle smooths with grid-wide launches
@@ -118,6 +117,7 @@ struct cg_problem {
   double* d_part2 = nullptr;
   double* d_scalar = nullptr;
   TileRec* d_lists = nullptr;
+  TileRec* d_tiles = nullptr;    // every tile's record in canonical order (the worklists are gathered from it)
   TileRec* d_tmp_list = nullptr;
   int tmp_list_cap = 0;
   CoarseSlot* d_coarse = nullptr;
@@ -163,7 +163,7 @@ cg_problem::~cg_problem() {
   for (auto e : ev) if (e) (void)hipEventDestroy(e);
   if (h_pin) (void)hipHostFree(h_pin);
   void* ptrs[] = {d_ctype, d_fmask, d_zb, d_zh, d_rhsvh, d_zbh, d_x64, d_r64, d_bsave, d_psum, d_gsum, d_ps, d_dflag,
-                  d_plans, d_part, d_part2, d_scalar, d_lists, d_tmp_list, d_coarse};
+                  d_plans, d_part, d_part2, d_scalar, d_lists, d_tiles, d_tmp_list, d_coarse};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
   for (auto q : d_api) if (q) (void)hipFree(q);
@@ -174,6 +174,15 @@ cg_problem::~cg_problem() {
 namespace {
 
 constexpr int kSwizzle = F_SWIZZLE;
+
+// A device buffer freed when it goes out of scope (also when a call throws).
+struct DeviceBuffer {
+  void* p = nullptr;
+  explicit DeviceBuffer(size_t bytes) { HIPC(hipMalloc(&p, bytes)); }
+  ~DeviceBuffer() { if (p) (void)hipFree(p); }
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+};
 
 inline unsigned grid_for(int n, int flags) { return (flags & F_SWIZZLE) ? unsigned((n + 7) & ~7) : unsigned(n); }
 
@@ -393,16 +402,16 @@ void build_schedule(cg_problem& P) {
     add_list(&P.actg[l], bucket[l], [&](int i) { return ty[i] != GHOST && grid_level(i); });
     add_list(&P.lvlg[l], bucket[l], [&](int i) { return grid_level(i); });
   }
-  auto& W = P.lists;
+  // the index lists go to the device, which gathers the records from d_tiles
+  std::vector<int32_t> ids;
   size_t total = 0;
+  for (auto& s : sel) total += s.second.size();
+  ids.reserve(total);
   for (auto& s : sel) {
-    s.first->off = int(total);
+    s.first->off = int(ids.size());
     s.first->n = int(s.second.size());
-    total += s.second.size();
+    ids.insert(ids.end(), s.second.begin(), s.second.end());
   }
-  W.resize(total);
-  for (auto& s : sel)
-    for (size_t k = 0; k < s.second.size(); k++) W[size_t(s.first->off) + k] = P.tiles[s.second[k]];
 
   // coarse plans and slots
   P.plans.clear();
@@ -424,8 +433,16 @@ void build_schedule(cg_problem& P) {
   P.d_lists = nullptr;
   P.d_coarse = nullptr;
   P.d_plans = nullptr;
-  HIPC(hipMalloc(&P.d_lists, sizeof(TileRec) * W.size()));
-  HIPC(hipMemcpyAsync(P.d_lists, W.data(), sizeof(TileRec) * W.size(), hipMemcpyHostToDevice, P.stream));
+  HIPC(hipMalloc(&P.d_lists, sizeof(TileRec) * std::max<size_t>(1, total)));
+  if (total) {
+    DeviceBuffer d_ids(sizeof(int32_t) * total);
+    HIPC(hipMemcpyAsync(d_ids.p, ids.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice, P.stream));
+    const int64_t words = int64_t(total) * int64_t(sizeof(TileRec) / 16);
+    const unsigned blocks = unsigned(std::min<int64_t>((words + 255) / 256, 65536));
+    k_gather_tiles<<<blocks, 256, 0, P.stream>>>(P.d_tiles, static_cast<const int32_t*>(d_ids.p), int64_t(total), P.d_lists);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(P.stream));  // d_ids is freed at the end of this scope
+  }
   if (!P.plans.empty()) {
     HIPC(hipMalloc(&P.d_coarse, sizeof(CoarseSlot) * P.coarse_slots.size()));
     HIPC(hipMemcpyAsync(P.d_coarse, P.coarse_slots.data(), sizeof(CoarseSlot) * P.coarse_slots.size(),
@@ -540,13 +557,21 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   HIPC(hipMalloc(&P.d_part, np * sizeof(double)));
   HIPC(hipMalloc(&P.d_part2, np * sizeof(double)));
   HIPC(hipMalloc(&P.d_scalar, 8 * sizeof(double)));
+  HIPC(hipMalloc(&P.d_tiles, sizeof(TileRec) * size_t(n)));
+  HIPC(hipMemcpyAsync(P.d_tiles, P.tiles.data(), sizeof(TileRec) * size_t(n), hipMemcpyHostToDevice, P.stream));
   build_schedule(P);
-  std::vector<uint8_t> ct(cells);
-  for (int i = 0; i < n; i++) {
-    const int q = P.tiles[i].prob;
-    std::memcpy(&ct[size_t(i) * 512], descs[q].cell_type + size_t(P.ext_of[i] - tbase[q]) * 512, 512);
+  // cell types: each tree's array goes up as given (into d_fmask, free until the
+  // face masks are formed) and is permuted to the canonical order on the device
+  for (int q = 0; q < nprob; q++)
+    HIPC(hipMemcpyAsync(P.d_fmask + size_t(tbase[q]) * 512, descs[q].cell_type, size_t(descs[q].num_tiles) * 512,
+                        hipMemcpyHostToDevice, P.stream));
+  {
+    DeviceBuffer d_ext(sizeof(int32_t) * size_t(n));
+    HIPC(hipMemcpyAsync(d_ext.p, P.ext_of.data(), sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice, P.stream));
+    k_permute_ctype<<<unsigned(n), 64, 0, P.stream>>>(P.d_fmask, static_cast<const int32_t*>(d_ext.p), n, P.d_ctype);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(P.stream));
   }
-  HIPC(hipMemcpyAsync(P.d_ctype, ct.data(), cells, hipMemcpyHostToDevice, P.stream));
   k_face_mask<<<P.all.n, 64, 0, P.stream>>>(P.d_ctype, P.d_fmask, P.L(P.all), P.all.n);
   HIPC(hipGetLastError());
   HIPC(hipHostMalloc(&P.h_pin, 2 * sizeof(int) * nprob, hipHostMallocDefault));
@@ -1064,15 +1089,6 @@ void set_schedule(cg_problem& P, int coarse_shift, int swizzle, int tile_block) 
 // A candidate counts only if every problem converges; each is timed as the
 // best of two solves after a warm-up (graph capture).  The pool keeps the
 // chosen schedule; *best receives the chosen solver parameters.
-// A device buffer freed when it goes out of scope (also when a call throws).
-struct DeviceBuffer {
-  void* p = nullptr;
-  explicit DeviceBuffer(size_t bytes) { HIPC(hipMalloc(&p, bytes)); }
-  ~DeviceBuffer() { if (p) (void)hipFree(p); }
-  DeviceBuffer(const DeviceBuffer&) = delete;
-  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-};
-
 void autotune(cg_problem& P, const cg_params& base, int allow_16bit, cg_params* best, cg_tune_report* rep) {
   check_params(base);
   const size_t bytes = size_t(P.n) * 512 * sizeof(float);

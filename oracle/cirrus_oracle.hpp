@@ -641,6 +641,9 @@ struct Problem {
   std::vector<int> canonOfStore;    // store index -> canonical index
   std::vector<Info> leafCanon;      // LEAF tiles in canonical order
   std::vector<Tube> tubes;
+  // AMGSolver's stored coefficients per tile storage index: [0..2] the cell's
+  // minus-face off-diagonal along x, y, z, [3] the diagonal (AMGSolver.cu:38-39)
+  std::vector<std::array<T, 512>> amg[4];
 
   Grid<T>& g() { return *grid; }
 
@@ -1286,6 +1289,259 @@ struct Solver {
     }
   }
 
+  // ===== AMGSolver (AMGSolver.cu), the stored-coefficient preconditioner =====
+  // AMGSolver.cu:17-36: LEAF-LEAF faces 1/h, LEAF-GHOST faces 1/(1.5 h) (not
+  // the GMG operator's (2/3)/h), both over h; 0 across NEUMANN.
+  static T AMGCoeff(T h, uint8_t t0, uint8_t t1, uint8_t c0, uint8_t c1) {
+    const int both = (t0 & LEAF) && (t1 & LEAF);
+    const int lg = ((t0 & LEAF) && (t1 & GHOST)) || ((t0 & GHOST) && (t1 & LEAF));
+    const T odh = (T(1) / h) * T(both) + (T(1) / (T(1.5) * h)) * T(lg);
+    const int hn = (c0 & NEUMANN) || (c1 & NEUMANN);
+    return hn ? T(0) : odh / h;
+  }
+  std::array<T, 512>& AC(int a, const Info& in) { return P.amg[a][size_t(in.tile)]; }
+  // CalculateAMGCoefficients (AMGSolver.cu:40-107) on tiles of `types`
+  void CalculateAMGCoefficients(uint8_t types) {
+    Grid<T>& g = G();
+    const auto& all = g.all;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long ti = 0; ti < long(all.size()); ti++) {
+      const Info info = all[ti];
+      if (!(info.type & types)) continue;
+      auto& t = g.tile(info);
+      const T h = g.voxelSize(info.level);
+      for (int vi = 0; vi < 512; vi++) {
+        const Coord gc = composeGlobal(info.coord, localCoord(vi));
+        const uint8_t c0 = t.ctype[vi];
+        T diag = T(0), off[3] = {T(0), T(0), T(0)};
+        for (int axis = 0; axis < 3; axis++)
+          for (int sgn : {-1, 1}) {
+            Coord ng = gc;
+            ng[axis] += sgn;
+            Info ni; Coord nl;
+            g.findVoxel(info.level, ng, ni, nl);
+            const uint8_t t1 = ni.empty() ? info.type : ni.type;
+            const uint8_t c1 = ni.empty() ? uint8_t(DIRICHLET) : g.tile(ni).ctype[localOffset(nl)];
+            const T coeff = ((AMGCoeff(h, info.type, t1, c0, c1) * h) * h) * h;
+            if (sgn == -1 && c1 == INTERIOR) off[axis] = -coeff;
+            diag = diag + coeff;
+          }
+        for (int a = 0; a < 3; a++) AC(a, info)[vi] = (c0 == INTERIOR) ? off[a] : T(0);
+        AC(3, info)[vi] = (c0 == INTERIOR) ? diag : T(0);
+      }
+    }
+  }
+  // AMGSolver::prepareTypesAndCoeffs = CoarsenTypesAndAMGCoeffs (AMGSolver.cu:110-209):
+  // ghost cell types from the parents, LEAF/GHOST coefficients, then finest
+  // level first the Galerkin sums of the 8 children into NONLEAF cells (whose
+  // cell type becomes INTERIOR if any child is INTERIOR, else DIRICHLET) and
+  // into LEAF cells with GHOST children.  Cell types are changed in place.
+  void AMGPrepare() {
+    Grid<T>& g = G();
+    for (int a = 0; a < 4; a++) P.amg[a].assign(g.store.size(), std::array<T, 512>{});
+    const auto& all = g.all;
+#pragma omp parallel for schedule(static)
+    for (long ti = 0; ti < long(all.size()); ti++) {
+      const Info info = all[ti];
+      if (!(info.type & GHOST)) continue;
+      auto& t = g.tile(info);
+      for (int vi = 0; vi < 512; vi++) {
+        Info pi; Coord pl;
+        g.findVoxel(info.level - 1, parentCoord(composeGlobal(info.coord, localCoord(vi))), pi, pl);
+        t.ctype[vi] = pi.empty() ? uint8_t(DIRICHLET) : g.tile(pi).ctype[localOffset(pl)];
+      }
+    }
+    CalculateAMGCoefficients(LEAF | GHOST);
+    for (int level = g.numLayers - 1; level >= 0; level--) {
+      const auto& arr = g.arrays[level];
+#pragma omp parallel for schedule(static)
+      for (long ti = 0; ti < long(arr.size()); ti++) {
+        const Info info = arr[ti];
+        if (!(info.type & (NONLEAF | LEAF))) continue;
+        auto& t = g.tile(info);
+        for (int vi = 0; vi < 512; vi++) {
+          const Coord gc = composeGlobal(info.coord, localCoord(vi));
+          int interior = 0;
+          bool ghost_child = false;
+          T cod[3][2][2][2];
+          T dsum = T(0);
+          for (int ci = 0; ci < 2; ci++)
+            for (int cj = 0; cj < 2; cj++)
+              for (int ck = 0; ck < 2; ck++) {
+                Info cinfo; Coord cl;
+                g.findVoxel(level + 1, Coord(gc[0] * 2 + ci, gc[1] * 2 + cj, gc[2] * 2 + ck), cinfo, cl);
+                if (!cinfo.empty()) {
+                  if (cinfo.type == GHOST) ghost_child = true;
+                  const int co = localOffset(cl);
+                  dsum = dsum + AC(3, cinfo)[co];
+                  interior += g.tile(cinfo).ctype[co] == INTERIOR;
+                  for (int a = 0; a < 3; a++) cod[a][ci][cj][ck] = AC(a, cinfo)[co];
+                } else {
+                  for (int a = 0; a < 3; a++) cod[a][ci][cj][ck] = T(0);
+                }
+              }
+          T offs[3] = {T(0), T(0), T(0)};
+          for (int ci = 0; ci < 2; ci++)
+            for (int cj = 0; cj < 2; cj++)
+              for (int ck = 0; ck < 2; ck++) {
+                if (ci == 0) offs[0] = offs[0] + cod[0][ci][cj][ck];
+                if (ci == 1) dsum = dsum + T(2) * cod[0][ci][cj][ck];
+                if (cj == 0) offs[1] = offs[1] + cod[1][ci][cj][ck];
+                if (cj == 1) dsum = dsum + T(2) * cod[1][ci][cj][ck];
+                if (ck == 0) offs[2] = offs[2] + cod[2][ci][cj][ck];
+                if (ck == 1) dsum = dsum + T(2) * cod[2][ci][cj][ck];
+              }
+          if (info.type == NONLEAF) {
+            t.ctype[vi] = interior > 0 ? uint8_t(INTERIOR) : uint8_t(DIRICHLET);
+            for (int a = 0; a < 3; a++) AC(a, info)[vi] = offs[a] * one_over_alpha;
+            AC(3, info)[vi] = dsum * one_over_alpha;
+          }
+          if (info.type == LEAF && ghost_child) {
+            for (int a = 0; a < 3; a++) AC(a, info)[vi] = AC(a, info)[vi] + offs[a];
+            AC(3, info)[vi] = AC(3, info)[vi] + dsum;
+          }
+        }
+      }
+    }
+  }
+  // A x with the stored coefficients on one tile (AMGLaplacianTileData::
+  // negativeLapSameLevel, AMGSolver.cu:232-250, tile + halo as
+  // LoadAMGLaplacianTileDataAndTileType, AMGSolver.cu:396-466): the x halo from
+  // the face neighbours (0 where there is none), the plus faces' coefficients
+  // from the plus neighbours' minus-face entries (0 where there is none).
+  void AMGTileRows(const Info& info, int xch, T out[512]) {
+    Grid<T>& g = G();
+    auto& t = g.tile(info);
+    auto X = [&](int i, int j, int k) -> T {
+      if (i >= 0 && i < 8 && j >= 0 && j < 8 && k >= 0 && k < 8) return t.ch[xch][localOffset(i, j, k)];
+      int f = i < 0 ? 0 : i > 7 ? 3 : j < 0 ? 1 : j > 7 ? 4 : k < 0 ? 2 : 5;
+      const Info& ni = t.nbr[f];
+      if (ni.empty()) return T(0);
+      return g.tile(ni).ch[xch][localOffset((i + 8) & 7, (j + 8) & 7, (k + 8) & 7)];
+    };
+    auto OFF = [&](int a, int i, int j, int k) -> T {
+      if (i < 8 && j < 8 && k < 8) return AC(a, info)[localOffset(i, j, k)];
+      const Info& ni = t.nbr[a + 3];
+      if (ni.empty()) return T(0);
+      return AC(a, ni)[localOffset(i & 7, j & 7, k & 7)];
+    };
+    for (int vi = 0; vi < 512; vi++) {
+      const Coord l = localCoord(vi);
+      T sum = X(l[0], l[1], l[2]) * AC(3, info)[vi];
+      for (int a = 0; a < 3; a++)
+        for (int sgn : {-1, 1}) {
+          int n[3] = {l[0], l[1], l[2]}, u[3] = {l[0], l[1], l[2]};
+          n[a] += sgn;
+          if (sgn == 1) u[a] += 1;
+          sum = std::fma(OFF(a, u[0], u[1], u[2]), X(n[0], n[1], n[2]), sum);
+        }
+      out[vi] = (t.ctype[vi] == INTERIOR) ? sum : T(0);
+    }
+  }
+  // NegativeLaplacianAMG128 (AMGSolver.cu:468-533) on the tiles of `level` whose type is in `types`
+  void NegLapAMG(int level, uint8_t types, int xch, int axch) {
+    Grid<T>& g = G();
+    const auto& arr = g.arrays[level];
+    std::vector<std::array<T, 512>> out(arr.size());
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long ti = 0; ti < long(arr.size()); ti++)
+      if (arr[ti].type & types) AMGTileRows(arr[ti], xch, out[ti].data());
+    for (long ti = 0; ti < long(arr.size()); ti++)
+      if (arr[ti].type & types) std::memcpy(g.tile(arr[ti]).ch[axch], out[ti].data(), sizeof(T) * 512);
+  }
+  // NegativeLaplacianAndGaussSeidelSameLevel128Kernel (AMGSolver.cu:540-578):
+  // one colour on the LEAF|NONLEAF tiles of `level` (D = 0: no update, as in the GMG path)
+  void GSColorAMG(int level, int xch, int rhsch, int color) {
+    Grid<T>& g = G();
+    const auto& arr = g.arrays[level];
+    std::vector<std::array<T, 512>> out(arr.size());
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long ti = 0; ti < long(arr.size()); ti++) {
+      const Info info = arr[ti];
+      if (!(info.type & (LEAF | NONLEAF))) continue;
+      auto& t = g.tile(info);
+      T ax[512];
+      AMGTileRows(info, xch, ax);
+      for (int vi = 0; vi < 512; vi++) {
+        const Coord l = localCoord(vi);
+        const T x0 = t.ch[xch][vi];
+        out[ti][vi] = x0;
+        if ((l[0] + l[1] + l[2]) % 2 != color) continue;
+        const T D = AC(3, info)[vi];
+        // ax holds 0 off INTERIOR cells, where no update happens
+        const T delta = (t.ctype[vi] & INTERIOR) ? ((D != T(0)) ? (t.ch[rhsch][vi] - ax[vi]) / D : T(0)) : T(0);
+        out[ti][vi] = x0 + delta;
+      }
+    }
+    for (long ti = 0; ti < long(arr.size()); ti++)
+      if (arr[ti].type & (LEAF | NONLEAF)) std::memcpy(g.tile(arr[ti]).ch[xch], out[ti].data(), sizeof(T) * 512);
+  }
+  // GaussSeidelAMG (AMGSolver.cu:598-603)
+  void GaussSeidelAMG(int iters, int order, int level, int xch, int rhsch) {
+    for (int i = 0; i < iters; i++) {
+      GSColorAMG(level, xch, rhsch, order);
+      GSColorAMG(level, xch, rhsch, 1 - order);
+    }
+  }
+  // VCycleAMG (AMGSolver.cu:610-715)
+  void VCycleAMG(int xch, int fch, int tmpch, int rhsch, int level_iters, int coarsest_iters) {
+    Grid<T>& g = G();
+    for (const Info& info : g.all) {
+      auto& t = g.tile(info);
+      for (int vi = 0; vi < 512; vi++) {
+        t.ch[xch][vi] = T(0);
+        t.ch[rhsch][vi] = (info.type & GHOST) ? T(0) : t.ch[fch][vi];
+      }
+    }
+    for (int i = g.maxLevel; i > 0; i--) {
+      GaussSeidelAMG(level_iters, 0, i, xch, rhsch);
+      NegLapAMG(i, LEAF | NONLEAF | GHOST, xch, tmpch);
+      for (const Info& info : g.arrays[i]) {  // BinaryTransform rhs - tmp on INTERIOR cells
+        auto& t = g.tile(info);
+        for (int vi = 0; vi < 512; vi++)
+          if (t.ctype[vi] & INTERIOR) t.ch[tmpch][vi] = t.ch[rhsch][vi] - t.ch[tmpch][vi];
+      }
+      for (const Info& info : g.arrays[i - 1]) {
+        auto& t = g.tile(info);
+        if (info.type == NONLEAF) {  // Restrict (PoissonSolver.cu:292-316)
+          for (int vi = 0; vi < 512; vi++) {
+            const Coord gc = composeGlobal(info.coord, localCoord(vi));
+            T sum = T(0);
+            for (int sidx = 0; sidx < 8; sidx++) {
+              Info ci; Coord cl;
+              g.findVoxel(i, childCoord(gc, childIndexToOffset(sidx)), ci, cl);
+              if (!ci.empty()) {
+                const auto& ct = g.tile(ci);
+                const int co = localOffset(cl);
+                sum = sum + ((ct.ctype[co] & INTERIOR) ? ct.ch[tmpch][co] : T(0));
+              }
+            }
+            t.ch[rhsch][vi] = sum * one_over_alpha;
+          }
+        } else if (info.type == LEAF) {  // Copy f, then AccumulateToParents of the ghost children (PoissonGrid.cu:721-755)
+          for (int vi = 0; vi < 512; vi++) {
+            const Coord gc = composeGlobal(info.coord, localCoord(vi));
+            T sum = T(0);
+            for (int sidx = 0; sidx < 8; sidx++) {
+              Info ci; Coord cl;
+              g.findVoxel(i, childCoord(gc, childIndexToOffset(sidx)), ci, cl);
+              if (!ci.empty()) sum = sum + g.tile(ci).ch[tmpch][localOffset(cl)];
+            }
+            t.ch[rhsch][vi] = std::fma(sum, T(1), t.ch[fch][vi]);
+          }
+        } else {  // GHOST: Fill 0
+          for (int vi = 0; vi < 512; vi++) t.ch[rhsch][vi] = T(0);
+        }
+      }
+    }
+    GaussSeidelAMG(coarsest_iters / 2, 0, 0, xch, rhsch);
+    GaussSeidelAMG(coarsest_iters / 2, 1, 0, xch, rhsch);
+    for (int i = 1; i <= g.maxLevel; i++) {
+      ProlongateAndUpdate(i, xch, xch);
+      GaussSeidelAMG(level_iters, 1, i, xch, rhsch);
+    }
+  }
+
   struct Result { int iters; double rel_res; double bb; double rr; };
 
   // GMGSolver::solve (GMGSolver.cu:531-666)
@@ -1315,6 +1571,13 @@ struct Solver {
     r.rr = std::ldexp(r.rr, -2 * k);
     return r;
   }
+  // amg: the preconditioner is VCycleAMG (AMGSolver::solve, AMGSolver.cu:722-855,
+  // the same PCG loop as GMGSolver::solve); AMGPrepare must have run.
+  bool amg = false;
+  void Precondition(int level_iters, int coarsest_iters, int smoother, T omega) {
+    if (amg) VCycleAMG(CH_Z, CH_R, CH_AP, CH_TMP, level_iters, coarsest_iters);
+    else VCycle(CH_Z, CH_R, CH_AP, level_iters, coarsest_iters, smoother, omega);
+  }
   Result solve_body(int max_iters, double tol, int level_iters, int coarsest_iters, bool pure_neumann,
                     int smoother, T omega, int sync_stride, double thr_floor) {
     ForLeafCells([](TileData<T>& t, int c) { t.ch[CH_X][c] = T(0); });
@@ -1322,7 +1585,7 @@ struct Solver {
     if (bb == 0.0) return {0, 0.0, 0.0, 0.0};
     double thr = std::max(std::max(tol * tol * bb, thr_floor), std::numeric_limits<double>::min());
     if (pure_neumann) ReCenterLeafVoxels(CH_R);
-    VCycle(CH_Z, CH_R, CH_AP, level_iters, coarsest_iters, smoother, omega);
+    Precondition(level_iters, coarsest_iters, smoother, omega);
     ForLeafCells([](TileData<T>& t, int c) { t.ch[CH_P][c] = t.ch[CH_Z][c]; });
     double gamma = Dot(CH_Z, CH_R, LEAF);
     double rr = bb;
@@ -1341,7 +1604,7 @@ struct Solver {
         if (rr < thr) return {i + 1, std::sqrt(rr / bb), bb, rr};
       }
       if (pure_neumann) ReCenterLeafVoxels(CH_R);
-      VCycle(CH_Z, CH_R, CH_AP, level_iters, coarsest_iters, smoother, omega);
+      Precondition(level_iters, coarsest_iters, smoother, omega);
       double gamma_old = gamma;
       gamma = Dot(CH_R, CH_Z, LEAF);
       T beta = T(gamma / gamma_old);

@@ -215,6 +215,47 @@ int orc_solve(void* h, int max_iters, double tol, int level_iters, int coarsest_
     return r.iters;
   });
 }
+// AMGSolver (AMGSolver.cu): prepareTypesAndCoeffs, then solve / one V-cycle
+// (z from r) / the stored-coefficient operator on one level; the coefficients
+// out in canonical tile order (a = 0..2 minus-face off-diagonals, 3 diagonal).
+void orc_amg_prepare(void* h) {
+  with(h, [&](auto& P) { Solver<std::remove_reference_t<decltype(P.g().h0)>> S(P); S.AMGPrepare(); return 0; });
+}
+int orc_amg_solve(void* h, int max_iters, double tol, int level_iters, int coarsest_iters, int pure_neumann,
+                  int sync_stride, double* out) {
+  return with(h, [&](auto& P) {
+    using T = std::remove_reference_t<decltype(P.g().h0)>;
+    if (P.amg[0].empty()) throw std::runtime_error("orc_amg_solve: orc_amg_prepare first");
+    Solver<T> S(P);
+    S.amg = true;
+    auto r = S.solve(max_iters, tol, level_iters, coarsest_iters, pure_neumann != 0, SMOOTH_RBGS, T(2) / T(3), sync_stride);
+    if (out) { out[0] = r.rel_res; out[1] = r.bb; out[2] = r.rr; }
+    return r.iters;
+  });
+}
+void orc_amg_vcycle(void* h, int level_iters, int coarsest_iters) {
+  with(h, [&](auto& P) {
+    Solver<std::remove_reference_t<decltype(P.g().h0)>> S(P);
+    S.VCycleAMG(CH_Z, CH_R, CH_AP, CH_TMP, level_iters, coarsest_iters);
+    return 0;
+  });
+}
+void orc_amg_apply(void* h, int level, int xch, int axch) {
+  with(h, [&](auto& P) {
+    Solver<std::remove_reference_t<decltype(P.g().h0)>> S(P);
+    S.NegLapAMG(level, LEAF | NONLEAF | GHOST, xch, axch);
+    return 0;
+  });
+}
+void orc_amg_coeffs(void* h, int a, double* out) {
+  with(h, [&](auto& P) {
+    for (size_t i = 0; i < P.canon.size(); i++) {
+      const auto& c = P.amg[a][size_t(P.canon[i].tile)];
+      for (int k = 0; k < 512; k++) out[512 * i + k] = double(c[k]);
+    }
+    return 0;
+  });
+}
 // The projection's neighbours of the solve (FluidEuler.h:581-612): ghost
 // velocities from the parents, the volume divergence into div_ch, and the
 // pressure gradient added to the face velocities (channels 6-8).

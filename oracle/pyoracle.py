@@ -140,6 +140,12 @@ def lib():
         L.orc_get_refined_x.argtypes = [vp, vp]
         L.orc_bf16_round.argtypes = [vp, vp, ctypes.c_long]
         L.orc_fp16_round.argtypes = [vp, vp, ctypes.c_long]
+        L.orc_amg_prepare.argtypes = [vp]
+        L.orc_amg_solve.restype = i
+        L.orc_amg_solve.argtypes = [vp, i, d, i, i, i, i, vp]
+        L.orc_amg_vcycle.argtypes = [vp, i, i]
+        L.orc_amg_apply.argtypes = [vp, i, i, i]
+        L.orc_amg_coeffs.argtypes = [vp, i, vp]
         L.orc_fp16_scale_exp.restype = i
         L.orc_fp16_scale_exp.argtypes = [d, d]
         L.orc_velocity_divergence.argtypes = [vp, i, i]
@@ -245,6 +251,31 @@ class Oracle:
         it = lib().orc_solve(self.h, max_iters, tol, level_iters, coarsest_iters, int(pure_neumann),
                              smoother, omega, int(sync_stride), int(storage), _ptr(out))
         return {"iters": it, "rel_res": out[0], "bb": out[1], "rr": out[2], "seconds": out[3]}
+
+    def amg_prepare(self):
+        """AMGSolver::prepareTypesAndCoeffs: stored coefficients, coarse cell types."""
+        lib().orc_amg_prepare(self.h)
+
+    def amg_solve(self, max_iters=1000, tol=1e-6, level_iters=1, coarsest_iters=10, pure_neumann=False, sync_stride=1):
+        out = np.zeros(3, np.float64)
+        it = lib().orc_amg_solve(self.h, max_iters, tol, level_iters, coarsest_iters, int(pure_neumann),
+                                 int(sync_stride), _ptr(out))
+        return {"iters": it, "rel_res": out[0], "bb": out[1], "rr": out[2]}
+
+    def amg_vcycle(self, level_iters=1, coarsest_iters=10):
+        """z = M_amg^-1 r (channels CH_Z from CH_R; CH_AP and CH_TMP are scratch)."""
+        lib().orc_amg_vcycle(self.h, level_iters, coarsest_iters)
+
+    def amg_apply(self, level, xch, axch):
+        lib().orc_amg_apply(self.h, level, xch, axch)
+
+    def amg_coeffs(self):
+        """[4, tiles, 512]: minus-face off-diagonals along x, y, z and the diagonal."""
+        n = len(self.topology()[0])
+        out = np.zeros((4, n, 512), np.float64)
+        for a in range(4):
+            lib().orc_amg_coeffs(self.h, a, _ptr(out[a]))
+        return out
 
     def solve_refined(self, tol=1e-8, inner_tol=1e-4, max_outer=20, max_iters=1000, level_iters=1, coarsest_iters=10,
                       pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, sync_stride=1, storage=STORE_BF16):

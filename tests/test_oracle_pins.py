@@ -717,3 +717,110 @@ def test_divergence_of_gradient_is_minus_the_operator(name):
     x = lc.from_tiles(pv)
     ref = (lc.matrix() @ x)[lc.ct == po.INTERIOR]
     assert np.abs(lc.from_tiles(d)[lc.ct == po.INTERIOR] + ref).max() < 1e-12 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------------------
+# AMGSolver: the stored-coefficient preconditioner
+# ---------------------------------------------------------------------------
+def _amg_scene(name, fp64=True):
+    o = po.Oracle(fp64=fp64, **SCENES[name])
+    b = o.get(po.CH_B)
+    o.amg_prepare()
+    return o, b
+
+
+def test_amg_leaf_operator_equals_the_pinned_operator_on_a_uniform_level():
+    """On a uniform LEAF level (no ghosts) the stored coefficients give the same
+    rows as the GMG operator, which the independent finite-volume assembly pins
+    (x = 0 off INTERIOR cells, as in every solve: a DIRICHLET neighbour enters
+    the diagonal only)."""
+    o, _ = _amg_scene("uniform32")
+    lev, crd, typ, nbr = o.topology()
+    L = lev.max()
+    x = np.random.default_rng(3).standard_normal((len(lev), 512)) * (o.celltypes() == po.INTERIOR)
+    o.set(po.CH_P, x)
+    o.amg_apply(L, po.CH_P, po.CH_AP)
+    a_amg = o.get(po.CH_AP)[lev == L]
+    o.set(po.CH_P, x)
+    o.full_negative_laplacian(po.CH_P, po.CH_AP)
+    a_gmg = o.get(po.CH_AP)[lev == L]
+    assert np.abs(a_amg - a_gmg).max() <= 1e-12 * np.abs(a_gmg).max()
+
+
+@pytest.mark.parametrize("name", ["uniform32", "band"])
+def test_amg_coarse_coefficients_are_the_galerkin_product(name):
+    """A NONLEAF cell's stored row is R A P of its children's rows (R sums the
+    8 children, P injects): diagonal = children's diagonals + twice their
+    internal couplings, minus-face entry = the sum over the 4 children on that
+    face.  Checked by assembling the child level's stored operator as a matrix
+    and multiplying, cell by cell, in FP64."""
+    o, _ = _amg_scene(name)
+    lev, crd, typ, nbr = o.topology()
+    C = o.amg_coeffs()
+    ct = o.celltypes()
+    n = len(lev)
+    # global cell index -> (tile, cell) per level
+    where = {}
+    for t in range(n):
+        for c in range(512):
+            i, j, k = c >> 6, (c >> 3) & 7, c & 7
+            where[(int(lev[t]), int(crd[t, 0]) * 8 + i, int(crd[t, 1]) * 8 + j, int(crd[t, 2]) * 8 + k)] = (t, c)
+
+    def row(l, g):
+        """Stored row of cell g at level l as {neighbour cell key: value}."""
+        t, c = where[(l,) + g]
+        r = {(l,) + g: C[3, t, c]}
+        for a in range(3):
+            m = list(g); m[a] -= 1
+            if C[a, t, c] != 0:
+                r[(l,) + tuple(m)] = r.get((l,) + tuple(m), 0.0) + C[a, t, c]
+            p = list(g); p[a] += 1
+            key = (l,) + tuple(p)
+            if key in where:
+                tp, cp = where[key]
+                if C[a, tp, cp] != 0:
+                    r[key] = r.get(key, 0.0) + C[a, tp, cp]
+        return r
+
+    checked = 0
+    rng = np.random.default_rng(4)
+    for t in rng.permutation(np.where(typ == po.NONLEAF)[0])[:6]:
+        l = int(lev[t])
+        for c in rng.choice(512, 24, replace=False):
+            i, j, k = c >> 6, (c >> 3) & 7, c & 7
+            G = (int(crd[t, 0]) * 8 + i, int(crd[t, 1]) * 8 + j, int(crd[t, 2]) * 8 + k)
+            kids = [(l + 1, 2 * G[0] + a, 2 * G[1] + b, 2 * G[2] + d) for d in (0, 1) for b in (0, 1) for a in (0, 1)]
+            # (R A P)[G, H] = sum over children f of G and children g of H of A[f, g]
+            rap = {}
+            for f in kids:
+                for key, v in row(f[0], f[1:]).items():
+                    parent = (l, key[1] >> 1, key[2] >> 1, key[3] >> 1)
+                    rap[parent] = rap.get(parent, 0.0) + v
+            if ct[t, c] != po.INTERIOR:
+                assert C[:, t, c].tolist() == [0.0, 0.0, 0.0, C[3, t, c]] or ct[t, c] == po.DIRICHLET
+                continue
+            scale = max(abs(v) for v in rap.values()) or 1.0
+            assert abs(rap.get((l,) + G, 0.0) - C[3, t, c]) <= 1e-12 * scale
+            for a in range(3):
+                m = list(G); m[a] -= 1
+                assert abs(rap.get((l,) + tuple(m), 0.0) - C[a, t, c]) <= 1e-12 * scale, (a, rap, C[:, t, c])
+            checked += 1
+    assert checked > 0
+
+
+@pytest.mark.parametrize("name", ["band", "vortex", "plane"])
+def test_amg_solve_reaches_the_same_pressure(name):
+    """The AMG-preconditioned CG solves the same system as the GMG one (the
+    CG operator is FullNegativeLaplacian in both, src/AMGSolver.cu:752), so in
+    FP64 both reach the same pressure; the AMG V-cycle only changes the path."""
+    o, b = _amg_scene(name)
+    ra = o.amg_solve(max_iters=500, tol=1e-12, level_iters=2, coarsest_iters=10)
+    xa = o.get(po.CH_X)
+    g = po.Oracle(fp64=True, **SCENES[name])
+    rg = g.solve(max_iters=500, tol=1e-12, level_iters=2, coarsest_iters=10)
+    xg = g.get(po.CH_X)
+    lev, crd, typ, nbr = g.topology()
+    m = (typ == po.LEAF)[:, None] & (g.celltypes() == po.INTERIOR)
+    assert ra["rel_res"] < 1e-12 and rg["rel_res"] < 1e-12
+    assert np.linalg.norm((xa - xg)[m]) / np.linalg.norm(xg[m]) < 1e-9
+    assert ra["iters"] < 200
