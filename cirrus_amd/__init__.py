@@ -46,7 +46,7 @@ EXPORTED_SYMBOLS = (
     "cg_refine_params_default", "cg_solve_refined", "cg_get_solution_f64",
     "cg_velocity_divergence", "cg_add_gradient", "cg_project", "cg_set_schedule", "cg_autotune",
     "cg_tiles_save", "cg_tiles_load", "cg_tiles_import_holder", "cg_tileset_tree", "cg_tileset_channels",
-    "cg_tileset_destroy",
+    "cg_tileset_destroy", "cg_amg_prepare", "cg_amg_solve", "cg_amg_coefficients",
 )
 
 
@@ -183,6 +183,9 @@ def lib():
             "cg_tileset_tree": (i32, [vp, P(TreeDesc)]),
             "cg_tileset_channels": (i32, [vp, P(i32), P(vp), P(vp)]),
             "cg_tileset_destroy": (i32, [vp]),
+            "cg_amg_prepare": (i32, [vp, f32]),
+            "cg_amg_solve": (i32, [vp, P(Params), P(Result)]),
+            "cg_amg_coefficients": (i32, [vp, i32, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -489,6 +492,42 @@ def default_params(**kw):
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+class AMGSolver:
+    """AMGSolver (src/AMGSolver.h:17-53) over a :class:`Problem`: the MGPCG with
+    the stored-coefficient V-cycle.  ``prepareTypesAndCoeffs`` first (cg_amg_prepare),
+    then ``solve`` (cg_amg_solve; one result per problem kept in ``self.last_batch``)."""
+
+    def __init__(self, one_over_alpha=1.0, prolong_coeff=1.0):
+        self.one_over_alpha = float(one_over_alpha)
+        self.prolong_coeff = float(prolong_coeff)
+
+    def prepareTypesAndCoeffs(self, prob):
+        _check(lib().cg_amg_prepare(prob.handle, self.one_over_alpha), "AMGSolver.prepareTypesAndCoeffs")
+
+    def solve(self, prob, verbose=False, max_iters=1000, relative_tolerance=1e-6, level_iters=1, coarsest_iters=10,
+              sync_stride=1, is_pure_neumann=False, use_graph=True):
+        """Returns (iters, relative residual) of the first problem like the reference."""
+        p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
+                           coarsest_iters=coarsest_iters, sync_stride=int(sync_stride),
+                           pure_neumann=int(is_pure_neumann), one_over_alpha=self.one_over_alpha,
+                           prolong_coeff=self.prolong_coeff, use_graph=int(use_graph))
+        res = (Result * prob.num_problems)()
+        _check(lib().cg_amg_solve(prob.handle, ctypes.byref(p), res), "AMGSolver.solve")
+        self.last_batch = [r.as_dict() for r in res]
+        if verbose:
+            print(f"AMG-PCG {res[0].iters} iters, rel residual {res[0].rel_res:.3e}")
+        return res[0].iters, res[0].rel_res
+
+    @staticmethod
+    def coefficients(prob):
+        """[5, tiles, 512] float32: minus-face entries along x, y, z, the
+        diagonal, the AMG cell types (cg_amg_coefficients)."""
+        out = np.zeros((5, prob.n, 512), np.float32)
+        for w in range(5):
+            _check(lib().cg_amg_coefficients(prob.handle, w, _ptr(out[w])), "cg_amg_coefficients")
+        return out
 
 
 class GMGSolver:

@@ -1718,6 +1718,326 @@ __global__ __launch_bounds__(64) void k_from_split(const TileRec* __restrict__ l
   load8_split(src + size_t(T.self) * 512, lane_id(), v);
   store8(dst + size_t(T.self) * 512 + lane_id() * 8, v);
 }
+// ---------------------------------------------------------------------------
+// AMGSolver (AMGSolver.cu): stored coefficients and their V-cycle, natural
+// cell order, FP32.  Coefficient arrays: off[a] = the cell's minus-face entry
+// along axis a, diag; cell types of the AMG hierarchy in amgct (NONLEAF cells
+// retyped by the coarsening, the others as given).
+// ---------------------------------------------------------------------------
+struct AmgC {
+  float* off[3];
+  float* diag;
+  uint8_t* ct;
+};
+// AMGSolver.cu:17-36
+__device__ __forceinline__ float amg_coeff(float h, uint8_t t0, uint8_t t1, uint8_t c0, uint8_t c1) {
+  const int both = (t0 & T_LEAF) && (t1 & T_LEAF);
+  const int lg = ((t0 & T_LEAF) && (t1 & T_GHOST)) || ((t0 & T_GHOST) && (t1 & T_LEAF));
+  const float odh = (1.f / h) * float(both) + (1.f / (1.5f * h)) * float(lg);
+  const int hn = (c0 & C_NEU) || (c1 & C_NEU);
+  return hn ? 0.f : odh / h;
+}
+// face neighbour tile f (x-,y-,z-,x+,y+,z+) of T: index (-1: none) and type
+__device__ __forceinline__ int amg_nbr(const TileRec& T, int f) { return uni(T.nbr[f]); }
+
+// Step 1 of CoarsenTypesAndAMGCoeffs (AMGSolver.cu:118-131): a GHOST cell takes
+// its parent cell's type (a GHOST tile's parent always exists).
+__global__ __launch_bounds__(64) void k_amg_ghost_types(const TileRec* __restrict__ list, int n,
+                                                        const uint8_t* __restrict__ ctype, uint8_t* __restrict__ amgct) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const int o = tr_octant(T);
+  const size_t pb = size_t(T.parent) * 512 + (((o & 1) * 4 + (i >> 1)) * 8 + (((o >> 1) & 1) * 4 + (j >> 1))) * 8 +
+                    ((o >> 2) & 1) * 4;
+  for (int k = 0; k < 8; k++) amgct[size_t(T.self) * 512 + lane * 8 + k] = ctype[pb + (k >> 1)];
+}
+
+// CalculateAMGCoefficients (AMGSolver.cu:40-107) on LEAF|GHOST tiles: lane
+// (i,j) owns cells k = 0..7; neighbour cell types from the own tile or the
+// face neighbour (none: DIRICHLET cell, the tile's own type).
+__global__ __launch_bounds__(64) void k_amg_coeffs(const TileRec* __restrict__ list, int n, AmgC A) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t tb = size_t(T.self) * 512;
+  const uint8_t t0 = tr_type(T);
+  const float h = T.h;
+  const uint8_t* ctype = A.ct;
+  const uint64_t own = load_ct8(ctype + tb + lane * 8);
+  // the four in-plane neighbour pencils' cell types and tile types
+  uint64_t nct[4];
+  uint8_t ntt[4];
+  const int pi[4] = {i - 1, i + 1, i, i};
+  const int pj[4] = {j, j, j - 1, j + 1};
+  const int pf[4] = {0, 3, 1, 4};
+  for (int q = 0; q < 4; q++) {
+    const bool inside = pi[q] >= 0 && pi[q] < 8 && pj[q] >= 0 && pj[q] < 8;
+    if (inside) { nct[q] = load_ct8(ctype + tb + (pi[q] * 8 + pj[q]) * 8); ntt[q] = t0; continue; }
+    const int nb = amg_nbr(T, pf[q]);
+    if (nb < 0) { nct[q] = 0x0101010101010101ull; ntt[q] = t0; continue; }  // DIRICHLET, own type
+    nct[q] = load_ct8(ctype + size_t(nb) * 512 + (((pi[q] + 8) & 7) * 8 + ((pj[q] + 8) & 7)) * 8);
+    ntt[q] = tr_ntype(T, pf[q]);
+  }
+  uint8_t kmc = C_DIR, kpc = C_DIR, kmt = t0, kpt = t0;
+  {
+    const int nb = amg_nbr(T, 2);
+    if (nb >= 0) { kmc = ctype[size_t(nb) * 512 + lane * 8 + 7]; kmt = tr_ntype(T, 2); }
+    const int np = amg_nbr(T, 5);
+    if (np >= 0) { kpc = ctype[size_t(np) * 512 + lane * 8]; kpt = tr_ntype(T, 5); }
+  }
+  float out[4][8];
+  for (int k = 0; k < 8; k++) {
+    const uint8_t c0 = ct_at(own, k);
+    const uint8_t c1s[6] = {ct_at(nct[0], k), ct_at(nct[1], k), ct_at(nct[2], k), ct_at(nct[3], k),
+                            k > 0 ? ct_at(own, k - 1) : kmc, k < 7 ? ct_at(own, k + 1) : kpc};
+    const uint8_t t1s[6] = {ntt[0], ntt[1], ntt[2], ntt[3], k > 0 ? t0 : kmt, k < 7 ? t0 : kpt};
+    float diag = 0.f, off[3] = {0.f, 0.f, 0.f};
+    for (int e = 0; e < 6; e++) {  // x-, x+, y-, y+, z-, z+
+      const float coeff = ((amg_coeff(h, t0, t1s[e], c0, c1s[e]) * h) * h) * h;
+      if ((e & 1) == 0 && c1s[e] == C_INT) off[e >> 1] = -coeff;
+      diag = diag + coeff;
+    }
+    const bool in = c0 == C_INT;
+    for (int a = 0; a < 3; a++) out[a][k] = in ? off[a] : 0.f;
+    out[3][k] = in ? diag : 0.f;
+  }
+  for (int a = 0; a < 3; a++) store8(A.off[a] + tb + lane * 8, out[a]);
+  store8(A.diag + tb + lane * 8, out[3]);
+}
+
+// Step 3 of CoarsenTypesAndAMGCoeffs (AMGSolver.cu:137-207) from the child
+// side: the tiles of one level (children) each fill the 4x4x4 cells of their
+// parent's octant, lane = parent cell.  LEAF|NONLEAF children: the NONLEAF
+// parent's cell type and the Galerkin sums times one_over_alpha; GHOST
+// children: the sums added to the LEAF parent's own coefficients.  Sums in the
+// reference's order (children ci, cj, ck lexicographic).
+__global__ __launch_bounds__(64) void k_amg_coarsen(const TileRec* __restrict__ list, int n, AmgC A, float alpha) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id();
+  const int qi = lane >> 4, qj = (lane >> 2) & 3, qk = lane & 3;
+  const size_t tb = size_t(T.self) * 512;
+  float cod[3][8];
+  float dsum = 0.f;
+  int interior = 0;
+  for (int ci = 0; ci < 2; ci++)
+    for (int cj = 0; cj < 2; cj++)
+      for (int ck = 0; ck < 2; ck++) {
+        const int q = (ci * 2 + cj) * 2 + ck;
+        const size_t cc = tb + ((2 * qi + ci) * 8 + (2 * qj + cj)) * 8 + (2 * qk + ck);
+        dsum = dsum + A.diag[cc];
+        interior += A.ct[cc] == C_INT;
+        for (int a = 0; a < 3; a++) cod[a][q] = A.off[a][cc];
+      }
+  float offs[3] = {0.f, 0.f, 0.f};
+  for (int ci = 0; ci < 2; ci++)
+    for (int cj = 0; cj < 2; cj++)
+      for (int ck = 0; ck < 2; ck++) {
+        const int q = (ci * 2 + cj) * 2 + ck;
+        if (ci == 0) offs[0] = offs[0] + cod[0][q];
+        if (ci == 1) dsum = dsum + 2.f * cod[0][q];
+        if (cj == 0) offs[1] = offs[1] + cod[1][q];
+        if (cj == 1) dsum = dsum + 2.f * cod[1][q];
+        if (ck == 0) offs[2] = offs[2] + cod[2][q];
+        if (ck == 1) dsum = dsum + 2.f * cod[2][q];
+      }
+  const int o = tr_octant(T);
+  const size_t pc = size_t(T.parent) * 512 + (((o & 1) * 4 + qi) * 8 + (((o >> 1) & 1) * 4 + qj)) * 8 + (((o >> 2) & 1) * 4 + qk);
+  if (tr_type(T) == T_GHOST) {
+    for (int a = 0; a < 3; a++) A.off[a][pc] = A.off[a][pc] + offs[a];
+    A.diag[pc] = A.diag[pc] + dsum;
+  } else {
+    A.ct[pc] = interior > 0 ? C_INT : C_DIR;
+    for (int a = 0; a < 3; a++) A.off[a][pc] = offs[a] * alpha;
+    A.diag[pc] = dsum * alpha;
+  }
+}
+
+// VCycleAMG's first step (AMGSolver.cu:616-628): x = 0, rhs = f (0 on GHOST tiles).
+__global__ __launch_bounds__(64) void k_amg_init(const TileRec* __restrict__ list, int n, float* __restrict__ X,
+                                                 const float* __restrict__ F, float* __restrict__ RHS, int flags) {
+  TILE_KERNEL_PROLOGUE
+  const size_t b = size_t(T.self) * 512 + lane_id() * 8;
+  float v[8], z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (tr_type(T) == T_GHOST) store8(RHS + b, z);
+  else { load8(F + b, v); store8(RHS + b, v); }
+  store8(X + b, z);
+}
+
+// A x with the stored coefficients for the lane's 8 cells (negativeLapSameLevel,
+// AMGSolver.cu:232-250): x0 d + sum over x-, x+, y-, y+, z-, z+ of the face's
+// entry times the neighbour's x, the plus faces' entries read from the plus
+// neighbour's minus-face entry (0 where there is no neighbour tile).
+__device__ __forceinline__ void amg_rows(const TileRec& T, const float* __restrict__ X, const AmgC& A, float x0[8],
+                                         float ax[8]) {
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t tb = size_t(T.self) * 512;
+  load8(X + tb + lane * 8, x0);
+  float xs[4][8], os[2][8];  // x of the x-, x+, y-, y+ pencils; off_x of x+, off_y of y+
+  const int pi[4] = {i - 1, i + 1, i, i};
+  const int pj[4] = {j, j, j - 1, j + 1};
+  const int pf[4] = {0, 3, 1, 4};
+  for (int q = 0; q < 4; q++) {
+    const bool inside = pi[q] >= 0 && pi[q] < 8 && pj[q] >= 0 && pj[q] < 8;
+    const int nb = inside ? int(T.self) : amg_nbr(T, pf[q]);
+    const size_t pb = size_t(nb) * 512 + (((pi[q] + 8) & 7) * 8 + ((pj[q] + 8) & 7)) * 8;
+    if (nb < 0) {
+      for (int k = 0; k < 8; k++) xs[q][k] = 0.f;
+      if (q == 1) for (int k = 0; k < 8; k++) os[0][k] = 0.f;
+      if (q == 3) for (int k = 0; k < 8; k++) os[1][k] = 0.f;
+      continue;
+    }
+    load8(X + pb, xs[q]);
+    if (q == 1) load8(A.off[0] + pb, os[0]);
+    if (q == 3) load8(A.off[1] + pb, os[1]);
+  }
+  float ox[8], oy[8], oz[8], dg[8];
+  load8(A.off[0] + tb + lane * 8, ox);
+  load8(A.off[1] + tb + lane * 8, oy);
+  load8(A.off[2] + tb + lane * 8, oz);
+  load8(A.diag + tb + lane * 8, dg);
+  float xkm = 0.f, xkp = 0.f, ozp = 0.f;
+  {
+    const int nb = amg_nbr(T, 2);
+    if (nb >= 0) xkm = X[size_t(nb) * 512 + lane * 8 + 7];
+    const int np = amg_nbr(T, 5);
+    if (np >= 0) { xkp = X[size_t(np) * 512 + lane * 8]; ozp = A.off[2][size_t(np) * 512 + lane * 8]; }
+  }
+  for (int k = 0; k < 8; k++) {
+    float sum = x0[k] * dg[k];
+    sum = __builtin_fmaf(ox[k], xs[0][k], sum);
+    sum = __builtin_fmaf(os[0][k], xs[1][k], sum);
+    sum = __builtin_fmaf(oy[k], xs[2][k], sum);
+    sum = __builtin_fmaf(os[1][k], xs[3][k], sum);
+    sum = __builtin_fmaf(oz[k], k > 0 ? x0[k - 1] : xkm, sum);
+    sum = __builtin_fmaf(k < 7 ? oz[k + 1] : ozp, k < 7 ? x0[k + 1] : xkp, sum);
+    ax[k] = sum;
+  }
+}
+
+// One colour of GaussSeidelAMG (AMGSolver.cu:540-578) on LEAF|NONLEAF tiles
+// (D = 0 leaves the cell unchanged, as in the GMG smoother).
+__global__ __launch_bounds__(64) void k_amg_gs(Ctx c, const TileRec* __restrict__ list, int n, float* __restrict__ X,
+                                               const float* __restrict__ RHS, AmgC A, int color, int flags) {
+  TILE_KERNEL_PROLOGUE
+  float x0[8], ax[8];
+  amg_rows(T, X, A, x0, ax);
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t b = size_t(T.self) * 512 + lane * 8;
+  float rhs[8], dg[8];
+  load8(RHS + b, rhs);
+  load8(A.diag + b, dg);
+  const uint64_t ct = load_ct8(A.ct + b);
+  if (prob_done(c, T, flags)) return;
+  wave_sync();  // every lane has read its neighbours before any lane writes
+  float out[8];
+  for (int k = 0; k < 8; k++) {
+    out[k] = x0[k];
+    if (((i + j + k) & 1) != color) continue;
+    const float delta = (ct_at(ct, k) & C_INT) ? ((dg[k] != 0.f) ? (rhs[k] - ax[k]) / dg[k] : 0.f) : 0.f;
+    out[k] = x0[k] + delta;
+  }
+  store8(X + b, out);
+}
+
+// The residual of VCycleAMG (AMGSolver.cu:638-642): tmp = rhs - A x on
+// INTERIOR cells, 0 elsewhere, on every tile of a level.
+__global__ __launch_bounds__(64) void k_amg_residual(Ctx c, const TileRec* __restrict__ list, int n,
+                                                     const float* __restrict__ X, const float* __restrict__ RHS,
+                                                     float* __restrict__ TMP, AmgC A, int flags) {
+  TILE_KERNEL_PROLOGUE
+  float x0[8], ax[8], rhs[8];
+  amg_rows(T, X, A, x0, ax);
+  const size_t b = size_t(T.self) * 512 + lane_id() * 8;
+  load8(RHS + b, rhs);
+  const uint64_t ct = load_ct8(A.ct + b);
+  if (prob_done(c, T, flags)) return;
+  float out[8];
+  for (int k = 0; k < 8; k++) out[k] = (ct_at(ct, k) == C_INT) ? rhs[k] - ax[k] : 0.f;
+  store8(TMP + b, out);
+}
+
+// Restrict (PoissonSolver.cu:292-316) from the child side: the LEAF|NONLEAF
+// tiles of a level fill their NONLEAF parent's octant, rhs = sum over the 8
+// children (x fastest) of their INTERIOR residuals, times one_over_alpha.
+__global__ __launch_bounds__(64) void k_amg_restrict(Ctx c, const TileRec* __restrict__ list, int n,
+                                                     const float* __restrict__ TMP, float* __restrict__ RHS, AmgC A,
+                                                     float alpha, int flags) {
+  TILE_KERNEL_PROLOGUE
+  if (prob_done(c, T, flags)) return;
+  const int lane = lane_id();
+  const int qi = lane >> 4, qj = (lane >> 2) & 3, qk = lane & 3;
+  const size_t tb = size_t(T.self) * 512;
+  float sum = 0.f;
+  for (int s = 0; s < 8; s++) {
+    const size_t cc = tb + ((2 * qi + (s & 1)) * 8 + (2 * qj + ((s >> 1) & 1))) * 8 + (2 * qk + (s >> 2));
+    sum = sum + ((A.ct[cc] & C_INT) ? TMP[cc] : 0.f);
+  }
+  const int o = tr_octant(T);
+  RHS[size_t(T.parent) * 512 + (((o & 1) * 4 + qi) * 8 + (((o >> 1) & 1) * 4 + qj)) * 8 + (((o >> 2) & 1) * 4 + qk)] =
+      sum * alpha;
+}
+
+// VCycleAMG's rhs on the LEAF and GHOST tiles of the coarser level
+// (AMGSolver.cu:652-656): LEAF rhs = f plus the 8-cell sums of the GHOST
+// children's residuals (AccumulateToParents, PoissonGrid.cu:721-755), GHOST rhs = 0.
+__global__ __launch_bounds__(64) void k_amg_leaf_rhs(Ctx c, const TileRec* __restrict__ list, int n,
+                                                     const float* __restrict__ TMP, const float* __restrict__ F,
+                                                     float* __restrict__ RHS, int flags) {
+  TILE_KERNEL_PROLOGUE
+  const uint8_t ty = tr_type(T);
+  if (ty == T_NONLEAF || prob_done(c, T, flags)) return;
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t b = size_t(T.self) * 512 + lane * 8;
+  float out[8];
+  if (ty == T_GHOST) {
+    for (int k = 0; k < 8; k++) out[k] = 0.f;
+  } else {
+    float f[8];
+    load8(F + b, f);
+    for (int k = 0; k < 8; k++) {
+      const int o = (i >> 2) | ((j >> 2) << 1) | ((k >> 2) << 2);
+      // the ghost child of octant o, by register selects (no indexed private array)
+      const int g = (o & 4) ? ((o & 2) ? ((o & 1) ? T.gch[7] : T.gch[6]) : ((o & 1) ? T.gch[5] : T.gch[4]))
+                            : ((o & 2) ? ((o & 1) ? T.gch[3] : T.gch[2]) : ((o & 1) ? T.gch[1] : T.gch[0]));
+      float sum = 0.f;
+      if (g >= 0)
+        for (int cid = 0; cid < 8; cid++)
+          sum = sum + TMP[size_t(g) * 512 + ((2 * (i & 3) + (cid & 1)) * 8 + (2 * (j & 3) + ((cid >> 1) & 1))) * 8 +
+                          (2 * (k & 3) + (cid >> 2))];
+      out[k] = __builtin_fmaf(sum, 1.f, f[k]);
+    }
+  }
+  store8(RHS + b, out);
+}
+
+// ProlongateAndUpdate128 (GMGSolver.cu:345-380) on the AMG hierarchy, natural
+// order: GHOST x = parent x on INTERIOR cells, others x += pcoef parent x on
+// INTERIOR cells, 0 elsewhere.
+__global__ __launch_bounds__(64) void k_amg_prolong(Ctx c, const TileRec* __restrict__ list, int n, float* __restrict__ X,
+                                                    AmgC A, float pcoef, int flags) {
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t b = size_t(T.self) * 512 + lane * 8;
+  float x[8];
+  load8(X + b, x);
+  const uint64_t ct = load_ct8(A.ct + b);
+  const int o = tr_octant(T);
+  const size_t pb = size_t(T.parent) * 512 + (((o & 1) * 4 + (i >> 1)) * 8 + (((o >> 1) & 1) * 4 + (j >> 1))) * 8 +
+                    ((o >> 2) & 1) * 4;
+  float px[4];
+  for (int q = 0; q < 4; q++) px[q] = X[pb + q];
+  if (prob_done(c, T, flags)) return;
+  const bool ghost = tr_type(T) == T_GHOST;
+  for (int k = 0; k < 8; k++) {
+    const bool in = ct_at(ct, k) & C_INT;
+    const float cx = px[k >> 1];
+    x[k] = in ? (ghost ? cx : __builtin_fmaf(pcoef, cx, x[k])) : 0.f;
+  }
+  store8(X + b, x);
+}
+
 // Fill whole tiles of a list with a value.
 template <class V = float>
 __global__ __launch_bounds__(64) void k_fill_tiles(const TileRec* __restrict__ list, int n, V* __restrict__ dst, float val) {

@@ -97,6 +97,9 @@ struct cg_problem {
   // device
   uint8_t* d_ctype = nullptr;
   uint8_t* d_fmask = nullptr;
+  float* d_amgc = nullptr;    // AMG stored coefficients: off x, off y, off z, diag (4 x cells; cg_amg_prepare)
+  uint8_t* d_amgct = nullptr; // AMG cell types
+  bool amg_ready = false;
   float* d_ch[CG_NUM_CHANNELS] = {};
   float* d_zb = nullptr;
   bf16* d_zh = nullptr;     // 16-bit z, V-cycle rhs and scratch (bfloat16 or float16 values; allocated at the first such solve)
@@ -163,7 +166,7 @@ cg_problem::~cg_problem() {
   for (auto e : ev) if (e) (void)hipEventDestroy(e);
   if (h_pin) (void)hipHostFree(h_pin);
   void* ptrs[] = {d_ctype, d_fmask, d_zb, d_zh, d_rhsvh, d_zbh, d_x64, d_r64, d_bsave, d_psum, d_gsum, d_ps, d_dflag,
-                  d_plans, d_part, d_part2, d_scalar, d_lists, d_tiles, d_tmp_list, d_coarse};
+                  d_plans, d_part, d_part2, d_scalar, d_lists, d_tiles, d_tmp_list, d_coarse, d_amgc, d_amgct};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
   for (auto q : d_api) if (q) (void)hipFree(q);
@@ -626,6 +629,7 @@ struct VParams {
   int nu = 1, coarsest = 10, smoother = CG_SMOOTH_RBGS;
   float omega = 2.f / 3.f, ooa = 1.f, pcoef = 1.f;
   int check_done = 1;
+  int amg = 0;  // the preconditioner is VCycleAMG (cg_amg_solve)
 };
 
 template <class V>
@@ -752,6 +756,72 @@ void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v) {
 }
 
 // One CG iteration of the rotated loop (GMGSolver.cu:573-653 shifted so that it
+// ---------------------------------------------------------------------------
+// AMGSolver (AMGSolver.cu): stored coefficients and VCycleAMG
+// ---------------------------------------------------------------------------
+AmgC amg_arrays(const cg_problem& P) {
+  const size_t cells = size_t(P.n) * 512;
+  AmgC A;
+  for (int a = 0; a < 3; a++) A.off[a] = P.d_amgc + size_t(a) * cells;
+  A.diag = P.d_amgc + 3 * cells;
+  A.ct = P.d_amgct;
+  return A;
+}
+// AMGSolver::prepareTypesAndCoeffs (CoarsenTypesAndAMGCoeffs, AMGSolver.cu:110-209).
+// The AMG cell types are kept beside the handle's own, which the GMG path keeps using.
+void amg_prepare(cg_problem& P, float one_over_alpha) {
+  if (!P.owner) fail(CG_ERR_INVALID, "cg_amg_prepare: not an owning handle");
+  const size_t cells = size_t(P.n) * 512;
+  if (!P.d_amgc) {
+    HIPC(hipMalloc(&P.d_amgc, 4 * cells * sizeof(float)));
+    HIPC(hipMalloc(&P.d_amgct, cells));
+  }
+  HIPC(hipMemsetAsync(P.d_amgc, 0, 4 * cells * sizeof(float), P.stream));
+  HIPC(hipMemcpyAsync(P.d_amgct, P.d_ctype, cells, hipMemcpyDeviceToDevice, P.stream));
+  const AmgC A = amg_arrays(P);
+  if (P.ghosts.n) k_amg_ghost_types<<<P.ghosts.n, 64, 0, P.stream>>>(P.L(P.ghosts), P.ghosts.n, P.d_ctype, P.d_amgct);
+  if (P.applyl.n) k_amg_coeffs<<<P.applyl.n, 64, 0, P.stream>>>(P.L(P.applyl), P.applyl.n, A);
+  for (int l = P.max_level; l >= 1; l--)  // finest level first: children fill their parents
+    if (P.lvl[l].n) k_amg_coarsen<<<P.lvl[l].n, 64, 0, P.stream>>>(P.L(P.lvl[l]), P.lvl[l].n, A, one_over_alpha);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(P.stream));
+  P.amg_ready = true;
+}
+// VCycleAMG (AMGSolver.cu:610-715): x in the FP32 scratch array, rhs in the
+// TMP channel, the residual in the AP channel (as the reference uses its
+// channels), all in natural cell order; z for the CG kernels is then put in
+// the split order on LEAF tiles.
+void enqueue_amg_vcycle(cg_problem& P, const VParams& v) {
+  const Ctx c = P.ctx();
+  const AmgC A = amg_arrays(P);
+  const int fl = P.swz | (v.check_done ? F_CHECK_DONE : 0);
+  float* X = P.d_zb;
+  float* RHS = P.d_ch[CG_CH_TMP];
+  float* TMP = P.d_ch[CG_CH_AP];
+  const float* F = c.r;
+  k_amg_init<<<grid_for(P.all.n, fl), 64, 0, P.stream>>>(P.L(P.all), P.all.n, X, F, RHS, fl);
+  auto gs = [&](int l, int color) {
+    if (P.act[l].n) k_amg_gs<<<grid_for(P.act[l].n, fl), 64, 0, P.stream>>>(c, P.L(P.act[l]), P.act[l].n, X, RHS, A, color, fl);
+  };
+  const int L = P.max_level;
+  for (int l = L; l >= 1; l--) {
+    for (int it = 0; it < v.nu; it++) { gs(l, 0); gs(l, 1); }
+    if (P.lvl[l].n)
+      k_amg_residual<<<grid_for(P.lvl[l].n, fl), 64, 0, P.stream>>>(c, P.L(P.lvl[l]), P.lvl[l].n, X, RHS, TMP, A, fl);
+    if (P.act[l].n)
+      k_amg_restrict<<<grid_for(P.act[l].n, fl), 64, 0, P.stream>>>(c, P.L(P.act[l]), P.act[l].n, TMP, RHS, A, v.ooa, fl);
+    if (P.lvl[l - 1].n)
+      k_amg_leaf_rhs<<<grid_for(P.lvl[l - 1].n, fl), 64, 0, P.stream>>>(c, P.L(P.lvl[l - 1]), P.lvl[l - 1].n, TMP, F, RHS, fl);
+  }
+  for (int it = 0; it < v.coarsest / 2; it++) { gs(0, 0); gs(0, 1); }
+  for (int it = 0; it < v.coarsest / 2; it++) { gs(0, 1); gs(0, 0); }
+  for (int l = 1; l <= L; l++) {
+    if (P.lvl[l].n) k_amg_prolong<<<grid_for(P.lvl[l].n, fl), 64, 0, P.stream>>>(c, P.L(P.lvl[l]), P.lvl[l].n, X, A, v.pcoef, fl);
+    for (int it = 0; it < v.nu; it++) { gs(l, 1); gs(l, 0); }
+  }
+  k_to_split<<<unsigned(P.leaf.n), 64, 0, P.stream>>>(P.L(P.leaf), P.leaf.n, X, c.z);
+}
+
 // starts with the V-cycle): [re-centre r], z = M^-1 r, gamma/beta, p, A p and
 // alpha, x/r and the convergence test.
 template <class V>
@@ -766,7 +836,8 @@ void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann) {
   }
   // RB-GS: down-sweep in the scratch buffer, z assembled in z by the up-sweep
   V* Z = pool_z<V>(P);
-  enqueue_vcycle<V>(P, v.smoother == CG_SMOOTH_RBGS ? pool_zb<V>(P) : Z, Z, nullptr, v);
+  if (v.amg) enqueue_amg_vcycle(P, v);
+  else enqueue_vcycle<V>(P, v.smoother == CG_SMOOTH_RBGS ? pool_zb<V>(P) : Z, Z, nullptr, v);
   k_cg_dot_rz<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
   k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_RZ, fl);
   k_cg_update_p<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
@@ -884,8 +955,10 @@ std::vector<double> residual64(cg_problem& P, const float* B, int pure_neumann, 
 // tol_q / maxit_q / thr_abs_q: per-problem tolerances, iteration caps and
 // threshold floors (null: prm's for all, no floor).
 void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* tol_q = nullptr, const int* maxit_q = nullptr,
-           const double* thr_abs_q = nullptr) {
+           const double* thr_abs_q = nullptr, bool amg = false) {
   check_params(prm);
+  if (amg && !P.amg_ready) fail(CG_ERR_STATE, "cg_amg_solve: call cg_amg_prepare first");
+  if (amg && prm.storage != CG_STORE_F32) fail(CG_ERR_INVALID, "cg_amg_solve: FP32 storage only");
   if (prm.storage != CG_STORE_F32) ensure_bf16(P);
   if (prm.verify_f64) {  // keep b for the FP64 check at the end
     ensure_f64(P);
@@ -920,8 +993,9 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
   HIPC(hipGetLastError());
 
   char keybuf[160];
-  std::snprintf(keybuf, sizeof(keybuf), "%d/%d/%d/%a/%a/%a/%d/%d/%d", v.nu, v.coarsest, v.smoother, double(v.omega), double(v.ooa),
-                double(v.pcoef), prm.pure_neumann, prm.storage, P.swz);
+  v.amg = amg ? 1 : 0;
+  std::snprintf(keybuf, sizeof(keybuf), "%d/%d/%d/%a/%a/%a/%d/%d/%d/%d", v.nu, v.coarsest, v.smoother, double(v.omega),
+                double(v.ooa), double(v.pcoef), prm.pure_neumann, prm.storage, P.swz, v.amg);
   const bool use_graph = prm.use_graph != 0;
   if (use_graph && P.graph_key != keybuf) {
     if (P.graph) HIPC(hipGraphExecDestroy(P.graph));
@@ -1308,6 +1382,40 @@ cg_status cg_get_channel(cg_problem* p, int32_t channel, float* host) {
     std::vector<float> buf(size_t(p->n) * 512);
     HIPC(hipMemcpyAsync(buf.data(), src, buf.size() * sizeof(float), hipMemcpyDeviceToHost, p->stream));
     HIPC(hipStreamSynchronize(p->stream));
+    for (int i = 0; i < p->n; i++) std::memcpy(host + size_t(p->ext_of[i]) * 512, &buf[size_t(i) * 512], 512 * sizeof(float));
+  });
+}
+
+cg_status cg_amg_prepare(cg_problem* p, float one_over_alpha) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    amg_prepare(*p, one_over_alpha);
+  });
+}
+
+cg_status cg_amg_solve(cg_problem* p, const cg_params* prm, cg_result* results) {
+  return guard([&] {
+    if (!p || !prm || !results) fail(CG_ERR_INVALID, "cg_amg_solve: null argument");
+    solve(*p, *prm, results, nullptr, nullptr, nullptr, true);
+  });
+}
+
+cg_status cg_amg_coefficients(cg_problem* p, int32_t which, float* host) {
+  return guard([&] {
+    if (!p || !host) fail(CG_ERR_INVALID, "cg_amg_coefficients: null argument");
+    if (!p->amg_ready) fail(CG_ERR_STATE, "cg_amg_coefficients: call cg_amg_prepare first");
+    if (which < 0 || which > 4) fail(CG_ERR_INVALID, "cg_amg_coefficients: which must be 0..4");
+    const size_t cells = size_t(p->n) * 512;
+    std::vector<float> buf(cells);
+    if (which < 4) {
+      HIPC(hipMemcpyAsync(buf.data(), p->d_amgc + size_t(which) * cells, cells * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+      HIPC(hipStreamSynchronize(p->stream));
+    } else {
+      std::vector<uint8_t> ct(cells);
+      HIPC(hipMemcpyAsync(ct.data(), p->d_amgct, cells, hipMemcpyDeviceToHost, p->stream));
+      HIPC(hipStreamSynchronize(p->stream));
+      for (size_t k = 0; k < cells; k++) buf[k] = float(ct[k]);
+    }
     for (int i = 0; i < p->n; i++) std::memcpy(host + size_t(p->ext_of[i]) * 512, &buf[size_t(i) * 512], 512 * sizeof(float));
   });
 }

@@ -215,6 +215,26 @@ cg_status cg_add_gradient(cg_problem* p, int32_t p_channel);
  * pressure gradient added to the face velocities; results[num_problems]. */
 cg_status cg_project(cg_problem* p, const cg_params* prm, cg_result* results);
 
+/* ---- AMGSolver (src/AMGSolver.h:17-53, src/AMGSolver.cu) ------------------
+ * The stored-coefficient preconditioner.  cg_amg_prepare is
+ * AMGSolver::prepareTypesAndCoeffs (CoarsenTypesAndAMGCoeffs,
+ * src/AMGSolver.cu:110-209): ghost cell types from the parents, the LEAF/GHOST
+ * coefficients (CalculateAMGCoefficients, :40-107; a LEAF-GHOST face weighs
+ * 1/(1.5h) here), then, finest level first, the Galerkin sums of the children
+ * into NONLEAF cells (retyped INTERIOR if a child is, else DIRICHLET) and into
+ * LEAF cells with GHOST children.  The AMG cell types are kept beside the
+ * handle's own (the GMG path keeps using those).  cg_amg_solve is
+ * AMGSolver::solve (:722-855): the MGPCG loop of cg_solve with VCycleAMG
+ * (:610-715) as the preconditioner (prm: level_iters, coarsest_iters,
+ * one_over_alpha, prolong_coeff, sync_stride, pure_neumann, max_iters,
+ * rel_tol, use_graph; FP32 storage only; smoother ignored).  One result per
+ * problem of the handle.  cg_amg_coefficients: which = 0..2 the cells'
+ * minus-face entries along x, y, z, 3 the diagonal, 4 the AMG cell types (as
+ * floats), 512 per tile in the caller's tile order. */
+cg_status cg_amg_prepare(cg_problem* p, float one_over_alpha);
+cg_status cg_amg_solve(cg_problem* p, const cg_params* prm, cg_result* results);
+cg_status cg_amg_coefficients(cg_problem* p, int32_t which, float* host);
+
 /* GMGSolver::dampedJacobiSolve (src/GMGSolver.cu:449-529). */
 cg_status cg_damped_jacobi_solve(cg_problem* p, int32_t max_iters, double rel_tol, float omega, cg_result* res);
 
