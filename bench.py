@@ -289,6 +289,36 @@ def run_batched(ca, scenes, B, reps, solver_kw, use_graph=True, storage=0, refin
     }
 
 
+def run_amg(ca, P, b_host, solver_kw, reps=10):
+    """AMGSolver on the adaptive 256^3 workload: prepareTypesAndCoeffs once,
+    then AMG-preconditioned solves to 1e-6 (same settings as the plain GMG
+    solve); the stored-coefficient colour sweep on the finest level timed
+    alone (algorithmic bytes 29 per cell: x, rhs, the four coefficients and the
+    cell type read, x written)."""
+    S = ca.AMGSolver(1.0, 1.0)
+    t = time.perf_counter()
+    S.prepareTypesAndCoeffs(P)
+    t_prep = time.perf_counter() - t
+    nu, nc = solver_kw["level_iters"], solver_kw["coarsest_iters"]
+    lat = []
+    for rep_i in range(reps + 2):
+        P.set(ca.CH_B, b_host)
+        t = time.perf_counter()
+        S.solve(P, False, 1000, 1e-6, nu, nc, 1, False)
+        if rep_i >= 2:
+            lat.append(time.perf_counter() - t)
+        assert S.last_batch[0]["converged"] == 1
+    L = P.max_level
+    us, tiles = P.benchmark_kernel(ca.KERNEL_AMG_GS, L, reps=100)
+    alg = tiles * 512 * 29
+    return {"workload": "adaptive octree 256^3 effective, AMG stored-coefficient V-cycle (RB-GS 2+2 on the stored "
+                        "rows), FP32, tol 1e-6",
+            "prepare_ms": t_prep * 1e3, "solves_per_s": len(lat) / float(np.sum(lat)),
+            "ms_per_solve": float(np.median(lat)) * 1e3, "iterations": S.last_batch[0]["iters"],
+            "amg_gs_sweep": {"level": L, "tiles": tiles, "us": us, "alg_bytes": alg,
+                             "GBps": alg / (us * 1e-6) / 1e9, "frac_hbm_peak": alg / (us * 1e-6) / 1e9 / PEAK_HBM_GBS}}
+
+
 VORTEX_KW = dict(kind=3, nroot=[1, 1, 2], vortex_level=[6, 4], vortex_thresh=0.5, seed=0)
 VORTEX_SMALL_KW = dict(kind=3, nroot=[1, 1, 2], vortex_level=[5, 3], vortex_thresh=0.5, seed=0)
 
@@ -342,7 +372,7 @@ def run_vortex(ca, solver_kw, reps=3, kw=VORTEX_KW):
         "_x64_leaf": x64[sc.tree.type == 1],
     }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gs_color_adaptive256_hbm_traffic.txt")
-FORMS = ["projection", "plain_drop_in", "batched", "batched_mixed", "batched_mixed_f16", "batched_autotuned", "served",
+FORMS = ["projection", "amg", "plain_drop_in", "batched", "batched_mixed", "batched_mixed_f16", "batched_autotuned", "served",
          "served_mixed", "served_mixed_light", "vortex"]
 
 
@@ -763,6 +793,10 @@ def main():
             "ms_per_projection": float(np.median(pt)) * 1e3, "projections_per_s": len(pt) / float(np.sum(pt)),
             "solve_ms": out["ms_per_step"], "iterations": pr["iters"]}
         note("projection done")
+    if "amg" in wanted:
+        forms["amg"] = run_amg(ca, P, b_host, solver_kw)
+        forms["amg"]["vs_gmg_solve"] = forms["amg"]["solves_per_s"] / out["value"]
+        note(f"amg: {forms['amg']['solves_per_s']:.1f} solves/s")
     batched_forms = [f for f in ("batched", "batched_mixed", "batched_mixed_f16", "batched_autotuned") if f in wanted]
     if args.batch > 0 and batched_forms:
         t0 = time.perf_counter()

@@ -32,7 +32,7 @@ STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 SCENE_UNIFORM, SCENE_SPHERE_BAND, SCENE_PLANE, SCENE_VORTEX = 0, 1, 2, 3
 (KERNEL_GS_SWEEP, KERNEL_APPLY, KERNEL_RESTRICT, KERNEL_PROLONG, KERNEL_JACOBI, KERNEL_UPDATE_XR, KERNEL_COARSE,
  KERNEL_EMPTY, KERNEL_COPY, KERNEL_GS_PROLONG, KERNEL_GS_SWEEP_BF16, KERNEL_GS_PROLONG_BF16,
- KERNEL_GS_SWEEP_F16) = range(13)
+ KERNEL_GS_SWEEP_F16, KERNEL_AMG_GS) = range(14)
 
 EXPORTED_SYMBOLS = (
     "cg_params_default", "cg_last_error", "cg_device_count", "cg_problem_create", "cg_problem_create_batch",

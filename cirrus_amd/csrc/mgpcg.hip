@@ -1670,8 +1670,9 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
   return guard([&] {
     if (!p || !avg_us || reps <= 0) fail(CG_ERR_INVALID, "cg_benchmark_kernel: bad argument");
     cg_problem& P = *p;
+    if (kernel == CG_KERNEL_AMG_GS && !P.amg_ready) fail(CG_ERR_STATE, "cg_benchmark_kernel: call cg_amg_prepare first");
     if ((kernel == CG_KERNEL_GS_SWEEP || kernel == CG_KERNEL_JACOBI || kernel == CG_KERNEL_GS_SWEEP_BF16 ||
-         kernel == CG_KERNEL_GS_SWEEP_F16) &&
+         kernel == CG_KERNEL_GS_SWEEP_F16 || kernel == CG_KERNEL_AMG_GS) &&
         (level < 0 || level > P.max_level))
       fail(CG_ERR_INVALID, "level out of range");
     if ((kernel == CG_KERNEL_RESTRICT || kernel == CG_KERNEL_PROLONG || kernel == CG_KERNEL_GS_PROLONG ||
@@ -1696,6 +1697,14 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
           const DList& a = P.act[level];
           nt = a.n;
           launch_gs(P, a, c, level, c.zh, nob, 0, base | F_GHOST_ZERO);
+          break;
+        }
+        case CG_KERNEL_AMG_GS: {
+          const DList& a = P.act[level];
+          nt = a.n;
+          if (a.n)
+            k_amg_gs<<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, P.d_zb, P.d_ch[CG_CH_TMP], amg_arrays(P), 0,
+                                                               base);
           break;
         }
         case CG_KERNEL_GS_SWEEP_F16: {

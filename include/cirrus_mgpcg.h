@@ -285,12 +285,14 @@ cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel);
  * hardware byte counters), CG_KERNEL_GS_PROLONG (prolongation to `level` fused
  * with the first up-sweep colour pass), CG_KERNEL_GS_SWEEP_BF16 and
  * CG_KERNEL_GS_PROLONG_BF16 (the same with bfloat16 V-cycle storage),
- * CG_KERNEL_GS_SWEEP_F16 (the colour sweep with float16 storage).
+ * CG_KERNEL_GS_SWEEP_F16 (the colour sweep with float16 storage),
+ * CG_KERNEL_AMG_GS (one colour of the AMG stored-coefficient sweep on `level`;
+ * after cg_amg_prepare).
  * Channels are overwritten. */
 enum { CG_KERNEL_GS_SWEEP = 0, CG_KERNEL_APPLY = 1, CG_KERNEL_RESTRICT = 2, CG_KERNEL_PROLONG = 3,
        CG_KERNEL_JACOBI = 4, CG_KERNEL_UPDATE_XR = 5, CG_KERNEL_COARSE = 6, CG_KERNEL_EMPTY = 7,
        CG_KERNEL_COPY = 8, CG_KERNEL_GS_PROLONG = 9, CG_KERNEL_GS_SWEEP_BF16 = 10, CG_KERNEL_GS_PROLONG_BF16 = 11,
-       CG_KERNEL_GS_SWEEP_F16 = 12 };
+       CG_KERNEL_GS_SWEEP_F16 = 12, CG_KERNEL_AMG_GS = 13 };
 cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int32_t reps, double* avg_us,
                               int64_t* tiles);
 
