@@ -694,9 +694,9 @@ def main():
             B = ca.Problem.batch([sc.tree for sc in scenes])
             lv = int(np.bincount(np.concatenate([sc.tree.level[sc.tree.type != 4] for sc in scenes])).argmax())
             del scenes
-            kernels_only(B, ca, level=lv, blocks=(1, 2))
+            kernels_only(B, ca, level=lv, blocks=(1, 2, 4))
         else:
-            kernels_only(P, ca, blocks=(1, 2))
+            kernels_only(P, ca, blocks=(1, 2, 4))
         return
     S = ca.GMGSolver(1.0, 1.0)
 

@@ -918,6 +918,19 @@ __global__ __launch_bounds__(64) void k_gs_color(Ctx c, const TileRec* __restric
   __shared__ GsLDS L;
   tile_gs(c, T, level, X, F, color, flags, L);
 }
+// The same colour sweep with four wavefronts per workgroup, one tile each
+// (tiles 4b .. 4b+3 of the list in workgroup b): a quarter of the workgroups
+// to dispatch for the same tiles.
+template <class V>
+__global__ __launch_bounds__(256) void k_gs_color4(Ctx c, const TileRec* __restrict__ list, int n, int level, V* X,
+                                                   const V* F, int color, int flags) {
+  const int wave = threadIdx.x >> 6;
+  const int w = work_index(flags & F_SWIZZLE) * 4 + wave;
+  __shared__ GsLDS L[4];
+  if (w >= n) return;  // whole wavefronts leave; the others synchronise within their wavefront only
+  const TileRec T = list[w];
+  tile_gs(c, T, level, X, F, color, flags, L[wave]);
+}
 // The same colour sweep with a tile block of two: wavefront w works tiles 2w
 // and 2w+1 of the list, loads of both in flight together.  Every tile's
 // update is the one above (tiles of a colour pass are independent).

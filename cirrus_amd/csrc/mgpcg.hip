@@ -637,6 +637,8 @@ void launch_gs(cg_problem& P, const DList& a, const Ctx& c, int l, V* Z, const V
   if (a.n == 0) return;
   if (P.gs_block == 2 && a.n > 1)
     k_gs_color2<V><<<grid_for((a.n + 1) / 2, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
+  else if (P.gs_block == 4 && a.n > 1)
+    k_gs_color4<V><<<grid_for((a.n + 3) / 4, flags), 256, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
   else
     k_gs_color<V><<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
 }
@@ -1142,7 +1144,8 @@ void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& 
 // ---------------------------------------------------------------------------
 void set_schedule(cg_problem& P, int coarse_shift, int swizzle, int tile_block) {
   if (coarse_shift < 0 || coarse_shift > CL_MAX_LEVELS) fail(CG_ERR_INVALID, "cg_set_schedule: coarse_shift out of range");
-  if (tile_block != 1 && tile_block != 2) fail(CG_ERR_INVALID, "cg_set_schedule: tile_block must be 1 or 2");
+  if (tile_block != 1 && tile_block != 2 && tile_block != 4)
+    fail(CG_ERR_INVALID, "cg_set_schedule: tile_block must be 1, 2 or 4");
   if (!P.owner) fail(CG_ERR_INVALID, "cg_set_schedule: not an owning handle");
   P.swz = swizzle ? F_SWIZZLE : 0;
   P.gs_block = tile_block;
@@ -1190,7 +1193,7 @@ void autotune(cg_problem& P, const cg_params& base, int allow_16bit, cg_params* 
   int max_auto = -1;
   for (int q = 0; q < P.nprob; q++) max_auto = std::max(max_auto, P.coarse_auto[q]);
   for (int swz = 1; swz >= 0; swz--)
-    for (int block = 1; block <= 2; block++) {
+    for (int block : {1, 2, 4}) {
       if (swz == 1 && block == 1) continue;
       set_schedule(P, 0, swz, block);
       const double t = run(base);

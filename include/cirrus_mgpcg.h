@@ -178,9 +178,10 @@ cg_status cg_get_solution_f64(cg_problem* p, double* host);
  * A schedule changes how the kernels cover the tiles, never a result:
  * coarse_shift = how many of each problem's LDS-resident coarse levels (as many
  * as fit by default) go back to grid-wide launches; swizzle = XCD-aware tile
- * order (1, default) or the plain order (0); tile_block = tiles per wavefront
- * of the grid-wide colour sweeps (1, default, or 2: the loads of two tiles in
- * flight together). */
+ * order (1, default) or the plain order (0); tile_block = how the grid-wide
+ * colour sweeps take their tiles: 1 (default) one tile per 64-thread
+ * workgroup, 2 two tiles per wavefront (the loads of both in flight together),
+ * 4 four wavefronts of one tile each per 256-thread workgroup. */
 cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle, int32_t tile_block);
 typedef struct {
   int32_t candidates;        /* parameter sets timed */
