@@ -1522,20 +1522,40 @@ __global__ __launch_bounds__(64) void k_cg_update_xr(Ctx c, const TileRec* __res
 
 // Sum of a problem's per-tile partials in the fixed order (lane q adds
 // partials q, q+64, ... in sequence, then the butterfly), staged through LDS by
-// the whole workgroup so the loads are issued at once.
+// the whole workgroup.  Each thread issues all its loads of a chunk before the
+// first wait (addresses clamped into the range, so every load is in bounds),
+// and the summing wave reads its LDS values in groups of FIN_GROUP before
+// adding them in sequence: a chunk costs one memory round trip, not one per
+// element.
 constexpr int FIN_THREADS = 256;
 constexpr int FIN_CHUNK = 4096;
+constexpr int FIN_PER_THREAD = FIN_CHUNK / FIN_THREADS;
+constexpr int FIN_GROUP = 16;
 __device__ __forceinline__ double block_sum_partials(const double* __restrict__ part, int begin, int count, double* lds) {
   const int tid = threadIdx.x;
   double a = 0.0;
   for (int c0 = 0; c0 < count; c0 += FIN_CHUNK) {
-    const int m = min(FIN_CHUNK, count - c0);
-    __syncthreads();
-    for (int q = tid; q < m; q += FIN_THREADS) lds[q] = part[begin + c0 + q];
+    const int m = min(FIN_CHUNK, count - c0);  // >= 1
+    const double* src = part + begin + c0;
+    double v[FIN_PER_THREAD];
+#pragma unroll
+    for (int u = 0; u < FIN_PER_THREAD; u++) v[u] = src[min(tid + u * FIN_THREADS, m - 1)];
+    __syncthreads();  // the previous chunk's sums are done with lds
+#pragma unroll
+    for (int u = 0; u < FIN_PER_THREAD; u++)
+      if (tid + u * FIN_THREADS < m) lds[tid + u * FIN_THREADS] = v[u];
     __syncthreads();
     if (tid < 64) {
       // lane tid continues its sequence tid, tid+64, ... across chunks (FIN_CHUNK is a multiple of 64)
-      for (int q = tid; q < m; q += 64) a += lds[q];
+      const int len = (m - tid + 63) >> 6;  // values of this lane in the chunk
+      for (int g0 = 0; g0 < len; g0 += FIN_GROUP) {
+        double w[FIN_GROUP];
+#pragma unroll
+        for (int u = 0; u < FIN_GROUP; u++) w[u] = lds[min(tid + (g0 + u) * 64, FIN_CHUNK - 1)];
+#pragma unroll
+        for (int u = 0; u < FIN_GROUP; u++)
+          if (g0 + u < len) a += w[u];
+      }
     }
   }
   if (tid < 64) a = wave_sum(a);
