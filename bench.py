@@ -481,12 +481,14 @@ def _kname(raw):
         k = m.end()
         name = raw[k:k + int(m.group(1))]
         rest = raw[k + int(m.group(1)):]
-        targ = {"IDF16bE": "<bf16>", "IfE": "<float>"}
+        targ = {"IDF16bE": "<bf16>", "IfE": "<float>", "IDF16bLb0EE": "<bf16>", "IfLb0EE": "<float>",
+                "IDF16bLb1EE": "<bf16, true>", "IfLb1EE": "<float, true>"}
         return name + next((v for t, v in targ.items() if rest.startswith(t)), "")
     n = raw.split("(")[0].strip()
     if n.startswith("void "):
         n = n[5:]
-    return n.replace("cg::", "").replace("__bf16", "bf16")
+    # the colour sweep's <V, false> instance is the plain pass; <V, true> also sums r.z
+    return n.replace("cg::", "").replace("__bf16", "bf16").replace(", false>", ">")
 
 
 def _pmc_per_dispatch(path, counter):

@@ -284,6 +284,7 @@ enum : int {
   F_VCYCLE = 4,      // V-cycle mode: rhs from rhsv; restriction into rhsv (ghosts +=)
   F_SWIZZLE = 16,    // XCD-aware tile order
   F_CHECK_DONE = 32, // skip tiles of converged problems
+  F_DOT_RZ = 64,     // last colour pass of a level's up-sweep: also the r.z partial of each LEAF tile
 };
 
 __device__ __forceinline__ bool prob_done(const Ctx& c, const TileRec& T, int flags) {
@@ -614,10 +615,13 @@ struct GsLDS {
 // tiles issues the loads of all of them before it waits on any.
 struct GsIn {
   float4 xc, xo, B, halo;
+  float4 r0, r1;  // with F_DOT_RZ on a LEAF tile: r of the lane's pencil, cells 0..3 and 4..7
   uint64_t mask;
   float xkm, xkp;
 };
-template <class V>
+// DOT: the pass carries F_DOT_RZ (a separate instance, so that the other
+// passes keep their register count).
+template <class V, bool DOT = false>
 __device__ __forceinline__ GsIn gs_load(const Ctx& c, const TileRec& T, const V* X, const V* F, int color, int flags) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const bool zero_all = flags & F_ZERO_X, ghost_zero = flags & F_GHOST_ZERO;
@@ -632,6 +636,11 @@ __device__ __forceinline__ GsIn gs_load(const Ctx& c, const TileRec& T, const V*
     g.xo = load4(tile + ho + lane * 4);
   }
   g.mask = load_ct8(c.fmask + size_t(T.self) * 512 + lane * 8);
+  if (DOT && tr_type(T) == T_LEAF) {
+    const float4* rp = reinterpret_cast<const float4*>(c.r + size_t(T.self) * 512 + lane * 8);
+    g.r0 = rp[0];
+    g.r1 = rp[1];
+  }
   const V* src = (flags & F_VCYCLE) ? vc_rhs<V>(c) : F;
   g.B = load4(src + size_t(T.self) * 512 + hc + lane * 4);
   // halo ring: lanes 0..31 fetch the other-colour quad of one neighbour pencil
@@ -668,7 +677,7 @@ __device__ __forceinline__ void gs_stage(const GsIn& g, GsLDS& L) {
   }
 }
 // after gs_stage and a wave_sync
-template <class V>
+template <class V, bool DOT = false>
 __device__ __forceinline__ void gs_finish(const Ctx& c, const TileRec& T, V* X, int color, int flags, const GsIn& g,
                                           const GsLDS& L) {
   if (prob_done(c, T, flags)) return;
@@ -681,14 +690,30 @@ __device__ __forceinline__ void gs_finish(const Ctx& c, const TileRec& T, V* X, 
                             g.mask, T.bc, T.h, i, j, p);
   store4(tile + hc + lane * 4, nv);
   if (flags & F_ZERO_X) store4(tile + ho + lane * 4, make_float4(0.f, 0.f, 0.f, 0.f));  // the zero guess of the other colour
+  if (DOT && tr_type(T) == T_LEAF) {
+    // this pass leaves the tile's z final: the updated colour as just stored,
+    // the other colour as loaded; r.z over the lane's pencil in cell order,
+    // as k_cg_dot_rz sums it
+    const float zc[4] = {vround<V>(nv.x), vround<V>(nv.y), vround<V>(nv.z), vround<V>(nv.w)};
+    const float zo[4] = {g.xo.x, g.xo.y, g.xo.z, g.xo.w};
+    const float rv[8] = {g.r0.x, g.r0.y, g.r0.z, g.r0.w, g.r1.x, g.r1.y, g.r1.z, g.r1.w};
+    double a = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const float z = ((k & 1) == int(p)) ? zc[k >> 1] : zo[k >> 1];
+      a += double(rv[k]) * double(z);
+    }
+    a = wave_sum(a);
+    if (lane == 0) c.part[T.leaf_slot] = a;
+  }
 }
-template <class V>
+template <class V, bool DOT>
 __device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int level, V* X, const V* F, int color,
                                         int flags, GsLDS& L) {
-  const GsIn g = gs_load(c, T, X, F, color, flags);
+  const GsIn g = gs_load<V, DOT>(c, T, X, F, color, flags);
   gs_stage(g, L);
   wave_sync();
-  gs_finish(c, T, X, color, flags, g, L);
+  gs_finish<V, DOT>(c, T, X, color, flags, g, L);
 }
 
 // Damped-Jacobi sweep (the update of GMGSolver.cu:508-525 with the level's
@@ -911,17 +936,17 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
   if (w >= n) return;                            \
   const TileRec T = list[w];
 
-template <class V>
+template <class V, bool DOT = false>
 __global__ __launch_bounds__(64) void k_gs_color(Ctx c, const TileRec* __restrict__ list, int n, int level, V* X,
                                                  const V* F, int color, int flags) {
   TILE_KERNEL_PROLOGUE
   __shared__ GsLDS L;
-  tile_gs(c, T, level, X, F, color, flags, L);
+  tile_gs<V, DOT>(c, T, level, X, F, color, flags, L);
 }
 // The same colour sweep with four wavefronts per workgroup, one tile each
 // (tiles 4b .. 4b+3 of the list in workgroup b): a quarter of the workgroups
 // to dispatch for the same tiles.
-template <class V>
+template <class V, bool DOT = false>
 __global__ __launch_bounds__(256) void k_gs_color4(Ctx c, const TileRec* __restrict__ list, int n, int level, V* X,
                                                    const V* F, int color, int flags) {
   const int wave = threadIdx.x >> 6;
@@ -929,12 +954,12 @@ __global__ __launch_bounds__(256) void k_gs_color4(Ctx c, const TileRec* __restr
   __shared__ GsLDS L[4];
   if (w >= n) return;  // whole wavefronts leave; the others synchronise within their wavefront only
   const TileRec T = list[w];
-  tile_gs(c, T, level, X, F, color, flags, L[wave]);
+  tile_gs<V, DOT>(c, T, level, X, F, color, flags, L[wave]);
 }
 // The same colour sweep with a tile block of two: wavefront w works tiles 2w
 // and 2w+1 of the list, loads of both in flight together.  Every tile's
 // update is the one above (tiles of a colour pass are independent).
-template <class V>
+template <class V, bool DOT = false>
 __global__ __launch_bounds__(64) void k_gs_color2(Ctx c, const TileRec* __restrict__ list, int n, int level, V* X,
                                                   const V* F, int color, int flags) {
   const int w = work_index(flags & F_SWIZZLE);
@@ -944,14 +969,14 @@ __global__ __launch_bounds__(64) void k_gs_color2(Ctx c, const TileRec* __restri
   const TileRec T0 = list[t0];
   const TileRec T1 = list[two ? t0 + 1 : t0];
   __shared__ GsLDS L[2];
-  const GsIn g0 = gs_load(c, T0, X, F, color, flags);
+  const GsIn g0 = gs_load<V, DOT>(c, T0, X, F, color, flags);
   GsIn g1 = g0;
-  if (two) g1 = gs_load(c, T1, X, F, color, flags);
+  if (two) g1 = gs_load<V, DOT>(c, T1, X, F, color, flags);
   gs_stage(g0, L[0]);
   if (two) gs_stage(g1, L[1]);
   wave_sync();
-  gs_finish(c, T0, X, color, flags, g0, L[0]);
-  if (two) gs_finish(c, T1, X, color, flags, g1, L[1]);
+  gs_finish<V, DOT>(c, T0, X, color, flags, g0, L[0]);
+  if (two) gs_finish<V, DOT>(c, T1, X, color, flags, g1, L[1]);
 }
 template <class V>
 __global__ __launch_bounds__(64) void k_gs_prolong(Ctx c, const TileRec* __restrict__ list, int n, const V* XD,

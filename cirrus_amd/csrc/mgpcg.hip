@@ -90,6 +90,7 @@ struct cg_problem {
   // levels given up to grid-wide launches, tiles per wavefront of the colour sweeps
   int swz = F_SWIZZLE;
   int coarse_shift = 0;
+  bool rz_in_vcycle = false;     // every LEAF tile is on a grid-wide level >= 1 (its up-sweep can sum r.z)
   int gs_block = 1;
   std::vector<int> lev, tbase, coarse_auto;
   std::vector<uint8_t> ty;
@@ -391,6 +392,9 @@ void build_schedule(cg_problem& P) {
   for (int i = 0; i < n; i++) every[i] = i;
   auto grid_level = [&](int i) { return lev[i] > P.coarse_C[P.tiles[i].prob]; };
   add_list(&P.leaf, every, [&](int i) { return ty[i] == LEAF; });
+  P.rz_in_vcycle = true;
+  for (int i = 0; i < n; i++)
+    if (ty[i] == LEAF && !(lev[i] >= 1 && grid_level(i))) P.rz_in_vcycle = false;
   add_list(&P.gather, every, [&](int i) { return ty[i] == LEAF && tr_has_gchild(P.tiles[i]); });
   add_list(&P.applyl, every, [&](int i) { return ty[i] == LEAF || ty[i] == GHOST; });
   add_list(&P.ghosts, every, [&](int i) { return ty[i] == GHOST; });
@@ -632,15 +636,23 @@ struct VParams {
   int amg = 0;  // the preconditioner is VCycleAMG (cg_amg_solve)
 };
 
+template <class V, bool DOT>
+void launch_gs_as(cg_problem& P, const DList& a, const Ctx& c, int l, V* Z, const V* R, int color, int flags) {
+  if (P.gs_block == 2 && a.n > 1)
+    k_gs_color2<V, DOT><<<grid_for((a.n + 1) / 2, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
+  else if (P.gs_block == 4 && a.n > 1)
+    k_gs_color4<V, DOT><<<grid_for((a.n + 3) / 4, flags), 256, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
+  else
+    k_gs_color<V, DOT><<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
+}
 template <class V>
 void launch_gs(cg_problem& P, const DList& a, const Ctx& c, int l, V* Z, const V* R, int color, int flags) {
   if (a.n == 0) return;
-  if (P.gs_block == 2 && a.n > 1)
-    k_gs_color2<V><<<grid_for((a.n + 1) / 2, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
-  else if (P.gs_block == 4 && a.n > 1)
-    k_gs_color4<V><<<grid_for((a.n + 3) / 4, flags), 256, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
-  else
-    k_gs_color<V><<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
+  if (flags & F_DOT_RZ) {
+    launch_gs_as<V, true>(P, a, c, l, Z, R, color, flags);
+    return;
+  }
+  launch_gs_as<V, false>(P, a, c, l, Z, R, color, flags);
 }
 // The pool's V-cycle vectors of storage type V.
 template <class V> V* pool_z(cg_problem& P);
@@ -684,8 +696,10 @@ void ensure_bf16(cg_problem& P) {
 // damped-Jacobi smoother, which ping-pongs with the scratch buffer instead).
 // V is the storage type of Z, ZO, the scratch buffer and the V-cycle rhs (R is
 // only a placeholder: in V-cycle mode the rhs comes from the V-cycle rhs array).
+// last_flags is added to the last colour pass of each grid-wide level's
+// up-sweep (RB-GS with nu >= 1), the pass after which that level's z is final.
 template <class V>
-void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v) {
+void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v, int last_flags = 0) {
   const Ctx c = P.ctx();
   const int base = F_VCYCLE | P.swz | (v.check_done ? F_CHECK_DONE : 0);
   const int L = P.max_level;  // a problem's levels above its coarse_C use the grid-wide lists actg / lvlg
@@ -735,10 +749,10 @@ void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v) {
     if (!jac && v.nu > 0) {
       // prolongation fused into the first (black) pass; the level's result goes to ZO
       if (al.n) k_gs_prolong<V><<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, Z, ZO, v.pcoef, 1, base);
-      launch_gs(P, a, c, l, ZO, R, 0, base);
+      launch_gs(P, a, c, l, ZO, R, 0, base | (v.nu == 1 ? last_flags : 0));
       for (int s = 1; s < v.nu; s++) {
         launch_gs(P, a, c, l, ZO, R, 1, base);
-        launch_gs(P, a, c, l, ZO, R, 0, base);
+        launch_gs(P, a, c, l, ZO, R, 0, base | (s == v.nu - 1 ? last_flags : 0));
       }
       continue;
     }
@@ -838,9 +852,12 @@ void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann) {
   }
   // RB-GS: down-sweep in the scratch buffer, z assembled in z by the up-sweep
   V* Z = pool_z<V>(P);
+  // r.z: summed by the last up-sweep pass of each level where every LEAF tile
+  // gets its final z (same partials, same order), else by its own pass
+  const bool rz_fused = !v.amg && v.smoother == CG_SMOOTH_RBGS && v.nu > 0 && P.rz_in_vcycle;
   if (v.amg) enqueue_amg_vcycle(P, v);
-  else enqueue_vcycle<V>(P, v.smoother == CG_SMOOTH_RBGS ? pool_zb<V>(P) : Z, Z, nullptr, v);
-  k_cg_dot_rz<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
+  else enqueue_vcycle<V>(P, v.smoother == CG_SMOOTH_RBGS ? pool_zb<V>(P) : Z, Z, nullptr, v, rz_fused ? F_DOT_RZ : 0);
+  if (!rz_fused) k_cg_dot_rz<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
   k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_RZ, fl);
   k_cg_update_p<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
   k_apply<<<grid_for(P.applyl.n, fl), 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, c.p, c.ap, -1, 0, 1, fl);

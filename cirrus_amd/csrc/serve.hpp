@@ -68,6 +68,7 @@ struct Instance {
   int64_t tile_begin = 0, leaf_begin = 0, ghost_begin = 0;
   int ntiles = 0, nleaf = 0, nghost = 0;
   int max_level = 0, C = -1;
+  bool rz_ok = false;    // every LEAF tile is on a grid-wide level >= 1
   std::vector<int> ext;  // internal tile i -> caller index
   TileRec* d_recs = nullptr;
   DList leaf, gather, applyl, ghosts, all;
@@ -329,6 +330,9 @@ void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
   };
   auto grid_level = [&](int i) { return B.lev[i] > I->C; };
   I->leaf = add_list([&](int i) { return B.ty[i] == LEAF; });
+  I->rz_ok = true;
+  for (int i = 0; i < n; i++)
+    if (B.ty[i] == LEAF && !(B.lev[i] >= 1 && grid_level(i))) I->rz_ok = false;
   I->gather = add_list([&](int i) { return B.ty[i] == LEAF && tr_has_gchild(B.tiles[i]); });
   I->applyl = add_list([&](int i) { return B.ty[i] == LEAF || B.ty[i] == GHOST; });
   I->ghosts = add_list([&](int i) { return B.ty[i] == GHOST; });
@@ -540,6 +544,9 @@ void server_assemble(cg_server& S, int k) {
     return out;
   };
   P.leaf = kind([](const Instance& I) { return &I.leaf; });
+  P.rz_in_vcycle = true;
+  for (int q : S.active)
+    if (!S.inst[q]->rz_ok) P.rz_in_vcycle = false;
   P.gather = kind([](const Instance& I) { return &I.gather; });
   P.applyl = kind([](const Instance& I) { return &I.applyl; });
   P.actg.assign(L + 1, DList());
