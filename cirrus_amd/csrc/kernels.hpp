@@ -454,12 +454,25 @@ __host__ __device__ __forceinline__ void view_coeffs(const TileRec& T, int sl, f
 // INTERIOR cells).
 // The NEUMANN cuts and the INTERIOR test come from the cell's face mask m: a
 // NEUMANN cell on either side of a face cuts it (coefficient 0).
+// BITS (the colour sweeps' four cells): bit e of m as all ones or zero (one
+// bitfield extract), so that the coefficient or +0 comes from one AND rather
+// than a compare and a select; the empty asm keeps the compiler from turning
+// the AND back into a select.  The same value either way; the eight-cell
+// operator rows keep the select, which needs fewer registers there.
+template <bool BITS = false>
 __device__ __forceinline__ void cell_row_m(float x0, uint32_t m, const float xn[6], const float bc[6], float h, float& ax,
                                            float& dg) {
   float s = 0.f, d = 0.f;
 #pragma unroll
   for (int e = 0; e < 6; e++) {
-    const float cf = (m & (1u << e)) ? bc[e] : 0.f;
+    float cf;
+    if (BITS) {
+      uint32_t open = uint32_t(int32_t(m << (31 - e)) >> 31);
+      __asm__("" : "+v"(open));
+      cf = __uint_as_float(__float_as_uint(bc[e]) & open);
+    } else {
+      cf = (m & (1u << e)) ? bc[e] : 0.f;
+    }
     s = __builtin_fmaf(cf, x0 - xn[e], s);
     d = d + cf;
   }
@@ -538,14 +551,14 @@ __device__ __forceinline__ void gs_pencil(const NbrPencils& N, const Stencil& S,
 #pragma unroll
   for (int m = 0; m < 4; m++) {
     const int k0 = 2 * m;  // the cell is k0 + p
-    const uint32_t mb = uint32_t(S.m >> (8 * k0 + sh)) & 0xff;
+    const uint32_t mb = (m < 2 ? uint32_t(S.m) : uint32_t(S.m >> 32)) >> (16 * (m & 1) + sh);  // bits 0..6 used
     const float bkm = (!p && m == 0) ? S.b_f[2] : S.b_int;
     const float bkp = (p && m == 3) ? S.b_f[5] : S.b_int;
     const float x0 = comp4(X0, m);
     const float xn[6] = {comp4(XIM, m), comp4(XIP, m), comp4(XJM, m), comp4(XJP, m), comp4(XKM, m), comp4(XKP, m)};
     const float bc[6] = {bim, bip, bjm, bjp, bkm, bkp};
     float ax, dg;
-    cell_row_m(x0, mb, xn, bc, S.h, ax, dg);
+    cell_row_m<true>(x0, mb, xn, bc, S.h, ax, dg);
     float delta = 0.f;
     if ((mb & M_INT) && dg != 0.f) delta = (comp4(B, m) - ax) / dg;
     nv[m] = x0 + delta;
@@ -585,14 +598,14 @@ __device__ __forceinline__ float4 gs_quad(float4 X0, float4 XO, float4 XIM, floa
   float nv[4];
 #pragma unroll
   for (int m = 0; m < 4; m++) {
-    const uint32_t mb = uint32_t(mask >> (16 * m + sh)) & 0xff;
+    const uint32_t mb = (m < 2 ? uint32_t(mask) : uint32_t(mask >> 32)) >> (16 * (m & 1) + sh);  // bits 0..6 used
     const float bkm = (!p && m == 0) ? bcv[2] : bi;
     const float bkp = (p && m == 3) ? bcv[5] : bi;
     const float x0 = comp4(X0, m);
     const float xn[6] = {comp4(XIM, m), comp4(XIP, m), comp4(XJM, m), comp4(XJP, m), comp4(XKM, m), comp4(XKP, m)};
     const float bc[6] = {bim, bip, bjm, bjp, bkm, bkp};
     float ax, dg;
-    cell_row_m(x0, mb, xn, bc, h, ax, dg);
+    cell_row_m<true>(x0, mb, xn, bc, h, ax, dg);
     float delta = 0.f;
     if ((mb & M_INT) && dg != 0.f) delta = (comp4(B, m) - ax) / dg;
     nv[m] = x0 + delta;
@@ -609,31 +622,69 @@ struct GsLDS {
 // colour's rhs half and the neighbours' other-colour quads; writes the updated
 // colour's half (and, from an implicit zero guess, zeros to the other half).
 // An INTERIOR cell with D == 0 is left unchanged.
-// The sweep is split into its loads (gs_load: every global load of the tile,
-// into registers) and the rest (gs_stage: the other-colour quads into LDS;
-// gs_finish: the update and its stores), so that a wavefront working several
-// tiles issues the loads of all of them before it waits on any.
+// The sweep is split into its loads (gs_issue: every global load of the tile,
+// as raw bits into registers; gs_convert: to FP32), and the rest (gs_stage:
+// the other-colour quads into LDS; gs_finish: the update and its stores), so
+// that a wavefront issues every load of its tile (or tiles) before it waits
+// on any.  A 16-bit value widens with one shift or conversion; done inside the
+// branch that loads it, that would make the branch wait for its load, so the
+// raw bits are held (an empty asm the compiler cannot see through) until all
+// loads are out.
+template <class V> struct RawV;
+template <> struct RawV<float> { typedef float4 Q; typedef float S; };
+template <> struct RawV<bf16> { typedef uint2 Q; typedef uint32_t S; };
+template <> struct RawV<f16> { typedef uint2 Q; typedef uint32_t S; };
+__device__ __forceinline__ float4 load_q_raw(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ uint2 load_q_raw(const bf16* p) { return *reinterpret_cast<const uint2*>(p); }
+__device__ __forceinline__ uint2 load_q_raw(const f16* p) { return *reinterpret_cast<const uint2*>(p); }
+__device__ __forceinline__ float load_s_raw(const float* p) { return *p; }
+__device__ __forceinline__ uint32_t load_s_raw(const bf16* p) { return *reinterpret_cast<const uint16_t*>(p); }
+__device__ __forceinline__ uint32_t load_s_raw(const f16* p) { return *reinterpret_cast<const uint16_t*>(p); }
+template <class V> __device__ __forceinline__ float4 quad_f(typename RawV<V>::Q q);
+template <> __device__ __forceinline__ float4 quad_f<float>(float4 q) { return q; }
+template <> __device__ __forceinline__ float4 quad_f<bf16>(uint2 q) { return make_float4(bf_lo(q.x), bf_hi(q.x), bf_lo(q.y), bf_hi(q.y)); }
+template <> __device__ __forceinline__ float4 quad_f<f16>(uint2 q) { return make_float4(h_lo(q.x), h_hi(q.x), h_lo(q.y), h_hi(q.y)); }
+template <class V> __device__ __forceinline__ float single_f(typename RawV<V>::S v);
+template <> __device__ __forceinline__ float single_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ float single_f<bf16>(uint32_t v) { return bf_lo(v); }
+template <> __device__ __forceinline__ float single_f<f16>(uint32_t v) { return h_lo(v); }
+__device__ __forceinline__ void hold(float4&) {}
+__device__ __forceinline__ void hold(float&) {}
+__device__ __forceinline__ void hold(uint2& q) { __asm__("" : "+v"(q.x), "+v"(q.y)); }
+__device__ __forceinline__ void hold(uint32_t& v) { __asm__("" : "+v"(v)); }
+
 struct GsIn {
   float4 xc, xo, B, halo;
   float4 r0, r1;  // with F_DOT_RZ on a LEAF tile: r of the lane's pencil, cells 0..3 and 4..7
   uint64_t mask;
   float xkm, xkp;
+  int32_t done;   // the problem's done flag, loaded with the tile's data (used with F_CHECK_DONE only)
+};
+template <class V>
+struct GsRaw {
+  typename RawV<V>::Q xc, xo, B, halo;
+  typename RawV<V>::S xkm, xkp;
+  float4 r0, r1;
+  uint64_t mask;
+  int32_t done;
 };
 // DOT: the pass carries F_DOT_RZ (a separate instance, so that the other
 // passes keep their register count).
 template <class V, bool DOT = false>
-__device__ __forceinline__ GsIn gs_load(const Ctx& c, const TileRec& T, const V* X, const V* F, int color, int flags) {
+__device__ __forceinline__ GsRaw<V> gs_issue(const Ctx& c, const TileRec& T, const V* X, const V* F, int color,
+                                             int flags) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const bool zero_all = flags & F_ZERO_X, ghost_zero = flags & F_GHOST_ZERO;
   const int hc = color << 8, ho = (color ^ 1) << 8;
   const V* tile = X + size_t(T.self) * 512;
-  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  GsIn g;
-  g.xc = z4;
-  g.xo = z4;
+  GsRaw<V> g{};  // zero bits: +0 in every storage type
+  // one unconditional scalar load issued with the others, tested after the LDS
+  // stage; without F_CHECK_DONE the face masks stand in as a readable address
+  const int32_t* dp = (flags & F_CHECK_DONE) ? &c.ps[T.prob].done : reinterpret_cast<const int32_t*>(c.fmask);
+  g.done = *dp;
   if (!zero_all) {
-    g.xc = load4(tile + hc + lane * 4);
-    g.xo = load4(tile + ho + lane * 4);
+    g.xc = load_q_raw(tile + hc + lane * 4);
+    g.xo = load_q_raw(tile + ho + lane * 4);
   }
   g.mask = load_ct8(c.fmask + size_t(T.self) * 512 + lane * 8);
   if (DOT && tr_type(T) == T_LEAF) {
@@ -642,30 +693,54 @@ __device__ __forceinline__ GsIn gs_load(const Ctx& c, const TileRec& T, const V*
     g.r1 = rp[1];
   }
   const V* src = (flags & F_VCYCLE) ? vc_rhs<V>(c) : F;
-  g.B = load4(src + size_t(T.self) * 512 + hc + lane * 4);
+  g.B = load_q_raw(src + size_t(T.self) * 512 + hc + lane * 4);
   // halo ring: lanes 0..31 fetch the other-colour quad of one neighbour pencil
   //   q = 0: x- (pencil (7,t) of nbr 0), 1: x+ ((0,t) of nbr 3), 2: y- ((t,7) of nbr 1), 3: y+ ((t,0) of nbr 4)
-  g.halo = z4;  // x = 0 outside the grid (GMGSolver.cu:120-122)
+  // x = 0 outside the grid (GMGSolver.cu:120-122)
   if (lane < 32) {
     const int q = lane >> 3, t = lane & 7;
     const int f = (q == 0) ? 0 : (q == 1) ? 3 : (q == 2) ? 1 : 4;
     const int n = pick4(q, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
     const int src_pl = (q == 0) ? (56 + t) : (q == 1) ? t : (q == 2) ? (t * 8 + 7) : (t * 8);
     const bool gh = tr_ntype(T, f) == T_GHOST;
-    if (n >= 0 && !(zero_all || (ghost_zero && gh))) g.halo = load4(X + size_t(n) * 512 + ho + src_pl * 4);
+    if (n >= 0 && !(zero_all || (ghost_zero && gh))) g.halo = load_q_raw(X + size_t(n) * 512 + ho + src_pl * 4);
   }
   // k halos: cell (i,j,7) of the z- neighbour and (i,j,0) of the z+ neighbour
-  g.xkm = 0.f;
-  g.xkp = 0.f;
   {
     int n = T.nbr[2];
     bool gh = tr_ntype(T, 2) == T_GHOST;
-    if (n >= 0 && !(zero_all || (ghost_zero && gh))) g.xkm = to_f(X[size_t(n) * 512 + split_cell(i, j, 7)]);
+    if (n >= 0 && !(zero_all || (ghost_zero && gh))) g.xkm = load_s_raw(X + size_t(n) * 512 + split_cell(i, j, 7));
     n = T.nbr[5];
     gh = tr_ntype(T, 5) == T_GHOST;
-    if (n >= 0 && !(zero_all || (ghost_zero && gh))) g.xkp = to_f(X[size_t(n) * 512 + split_cell(i, j, 0)]);
+    if (n >= 0 && !(zero_all || (ghost_zero && gh))) g.xkp = load_s_raw(X + size_t(n) * 512 + split_cell(i, j, 0));
   }
   return g;
+}
+template <class V>
+__device__ __forceinline__ GsIn gs_convert(GsRaw<V>& r) {
+  hold(r.xc);
+  hold(r.xo);
+  hold(r.B);
+  hold(r.halo);
+  hold(r.xkm);
+  hold(r.xkp);
+  GsIn g;
+  g.xc = quad_f<V>(r.xc);
+  g.xo = quad_f<V>(r.xo);
+  g.B = quad_f<V>(r.B);
+  g.halo = quad_f<V>(r.halo);
+  g.xkm = single_f<V>(r.xkm);
+  g.xkp = single_f<V>(r.xkp);
+  g.r0 = r.r0;
+  g.r1 = r.r1;
+  g.mask = r.mask;
+  g.done = r.done;
+  return g;
+}
+template <class V, bool DOT = false>
+__device__ __forceinline__ GsIn gs_load(const Ctx& c, const TileRec& T, const V* X, const V* F, int color, int flags) {
+  GsRaw<V> r = gs_issue<V, DOT>(c, T, X, F, color, flags);
+  return gs_convert<V>(r);
 }
 __device__ __forceinline__ void gs_stage(const GsIn& g, GsLDS& L) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
@@ -680,7 +755,7 @@ __device__ __forceinline__ void gs_stage(const GsIn& g, GsLDS& L) {
 template <class V, bool DOT = false>
 __device__ __forceinline__ void gs_finish(const Ctx& c, const TileRec& T, V* X, int color, int flags, const GsIn& g,
                                           const GsLDS& L) {
-  if (prob_done(c, T, flags)) return;
+  if ((flags & F_CHECK_DONE) && g.done) return;
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const int hc = color << 8, ho = (color ^ 1) << 8;
   V* tile = X + size_t(T.self) * 512;
@@ -969,9 +1044,11 @@ __global__ __launch_bounds__(64) void k_gs_color2(Ctx c, const TileRec* __restri
   const TileRec T0 = list[t0];
   const TileRec T1 = list[two ? t0 + 1 : t0];
   __shared__ GsLDS L[2];
-  const GsIn g0 = gs_load<V, DOT>(c, T0, X, F, color, flags);
-  GsIn g1 = g0;
-  if (two) g1 = gs_load<V, DOT>(c, T1, X, F, color, flags);
+  GsRaw<V> r0 = gs_issue<V, DOT>(c, T0, X, F, color, flags);
+  GsRaw<V> r1{};  // unused without a second tile (no copy of r0, which would wait for its loads)
+  if (two) r1 = gs_issue<V, DOT>(c, T1, X, F, color, flags);
+  const GsIn g0 = gs_convert<V>(r0);
+  const GsIn g1 = gs_convert<V>(r1);
   gs_stage(g0, L[0]);
   if (two) gs_stage(g1, L[1]);
   wave_sync();
