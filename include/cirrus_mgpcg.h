@@ -192,11 +192,12 @@ typedef struct {
   int32_t tile_block;
 } cg_tune_report;
 /* Times solves of the handle's current right-hand sides (restored afterwards)
- * under candidate schedules (swizzle on/off x tile_block 1/2, then coarse_shift
+ * under candidate schedules (swizzle on/off x tile_block 1/2/4, then coarse_shift
  * 1-2 with the best of those) with the caller's
  * smoother, then under candidate smoother settings (RB-GS with 1-3 sweeps per
  * level, damped Jacobi with 2-3; with allow_16bit also bfloat16 and float16 V-cycle storage),
- * keeping the fastest whose every problem converges.  The handle keeps the
+ * keeping the fastest whose every problem converges (a later candidate must be
+ * more than 2% faster to replace an earlier one).  The handle keeps the
  * chosen schedule; *best receives the chosen parameters (base with smoother,
  * level_iters and storage replaced). */
 cg_status cg_autotune(cg_problem* p, const cg_params* base, int32_t allow_16bit, cg_params* best, cg_tune_report* report);
