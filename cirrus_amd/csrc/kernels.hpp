@@ -96,6 +96,7 @@ struct Ctx {
   const int32_t* pmap;  // finalize workgroup -> problem (null: identity)
   double* part;  // per-leaf-slot dot partials
   double* part2; // second partial stream (mean counts)
+  const float* diag;  // per cell, colour-split order: the V-cycle colour sweep's diagonal D (k_diag)
 };
 
 // The V-cycle vectors of the storage type V.
@@ -550,7 +551,6 @@ __device__ __forceinline__ void gs_pencil(const NbrPencils& N, const Stencil& S,
   float nv[4];
 #pragma unroll
   for (int m = 0; m < 4; m++) {
-    const int k0 = 2 * m;  // the cell is k0 + p
     const uint32_t mb = (m < 2 ? uint32_t(S.m) : uint32_t(S.m >> 32)) >> (16 * (m & 1) + sh);  // bits 0..6 used
     const float bkm = (!p && m == 0) ? S.b_f[2] : S.b_int;
     const float bkp = (p && m == 3) ? S.b_f[5] : S.b_int;
@@ -650,6 +650,8 @@ template <> __device__ __forceinline__ float single_f<bf16>(uint32_t v) { return
 template <> __device__ __forceinline__ float single_f<f16>(uint32_t v) { return h_lo(v); }
 __device__ __forceinline__ void hold(float4&) {}
 __device__ __forceinline__ void hold(float&) {}
+// the same for an FP32 value whose later select must not pull its wait forward
+__device__ __forceinline__ void hold_f(float& v) { __asm__("" : "+v"(v)); }
 __device__ __forceinline__ void hold(uint2& q) { __asm__("" : "+v"(q.x), "+v"(q.y)); }
 __device__ __forceinline__ void hold(uint32_t& v) { __asm__("" : "+v"(v)); }
 
@@ -1055,6 +1057,134 @@ __global__ __launch_bounds__(64) void k_gs_color2(Ctx c, const TileRec* __restri
   gs_finish<V, DOT>(c, T0, X, color, flags, g0, L[0]);
   if (two) gs_finish<V, DOT>(c, T1, X, color, flags, g1, L[1]);
 }
+// The colour sweep's diagonal of every cell of a tile (the D of gs_quad: the
+// same coefficient choice and the same operations, in the colour-split order),
+// for the fused first iteration below.  Fixed by the face masks and the tile's
+// coefficients, so formed once when the problem is built.
+__global__ __launch_bounds__(64) void k_diag(Ctx c, const TileRec* __restrict__ list, int n, float* __restrict__ D) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const uint64_t mask = load_ct8(c.fmask + size_t(T.self) * 512 + lane * 8);
+  const float bi = T.bc[6];
+  const float bim = (i == 0) ? T.bc[0] : bi;
+  const float bip = (i == 7) ? T.bc[3] : bi;
+  const float bjm = (j == 0) ? T.bc[1] : bi;
+  const float bjp = (j == 7) ? T.bc[4] : bi;
+  const float xn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int col = 0; col < 2; col++) {
+    const bool p = ((col + i + j) & 1) != 0;
+    const int sh = p ? 8 : 0;
+    float d[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const uint32_t mb = (m < 2 ? uint32_t(mask) : uint32_t(mask >> 32)) >> (16 * (m & 1) + sh);
+      const float bkm = (!p && m == 0) ? T.bc[2] : bi;
+      const float bkp = (p && m == 3) ? T.bc[5] : bi;
+      const float bc[6] = {bim, bip, bjm, bjp, bkm, bkp};
+      float ax;
+      cell_row_m<true>(0.f, mb, xn, bc, T.h, ax, d[m]);
+    }
+    *reinterpret_cast<float4*>(D + size_t(T.self) * 512 + (col << 8) + lane * 4) = make_float4(d[0], d[1], d[2], d[3]);
+  }
+}
+
+// The value a colour-0 cell holds after the first pass from the zero guess:
+// 0 + (b - A 0) / D on INTERIOR cells with D != 0 (A 0 = +0), 0 elsewhere
+// (D is 0 off INTERIOR cells), as stored in V.
+template <class V>
+__device__ __forceinline__ float first_red(float b, float d) {
+  const float delta = (d != 0.f) ? (b - 0.f) / d : 0.f;
+  return vround<V>(0.f + delta);
+}
+
+// The first red-black iteration of a down-sweep level from the zero guess in
+// one launch: both colour passes of GMGSolver.cu:216-269 (colour 0 from x = 0,
+// the other colour written 0; then colour 1).  After the first pass a colour-0
+// cell depends on its own rhs and diagonal only, so the colour-0 values of the
+// halo are formed from the neighbours' rhs and stored diagonals and the second
+// pass needs no grid-wide wait.  Every value is the one the two passes give.
+template <class V>
+__global__ __launch_bounds__(64) void k_gs_first(Ctx c, const TileRec* __restrict__ list, int n, V* X, int flags) {
+  TILE_KERNEL_PROLOGUE
+  __shared__ GsLDS L;
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const bool ghost_zero = flags & F_GHOST_ZERO;
+  const V* R = vc_rhs<V>(c);
+  // ---- loads, all issued before the first wait
+  const int32_t* dp = (flags & F_CHECK_DONE) ? &c.ps[T.prob].done : reinterpret_cast<const int32_t*>(c.fmask);
+  const int32_t done = *dp;
+  const uint64_t mask = load_ct8(c.fmask + size_t(T.self) * 512 + lane * 8);
+  typename RawV<V>::Q rb0 = load_q_raw(R + size_t(T.self) * 512 + lane * 4);        // colour-0 rhs quad
+  typename RawV<V>::Q rb1 = load_q_raw(R + size_t(T.self) * 512 + 256 + lane * 4);  // colour-1 rhs quad
+  // halo ring of the colour-1 pass: lanes 0..31, the colour-0 quad of one neighbour pencil
+  // (a missing or ghost neighbour is read at the tile itself and its value
+  // taken as 0 afterwards, so that the loads need no inner branch)
+  typename RawV<V>::Q hb{};
+  float4 hd = make_float4(0.f, 0.f, 0.f, 0.f);
+  int dst = 0;
+  bool hok = false;
+  if (lane < 32) {
+    const int q = lane >> 3, t = lane & 7;
+    const int f = (q == 0) ? 0 : (q == 1) ? 3 : (q == 2) ? 1 : 4;
+    const int nb = pick4(q, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
+    const int src_pl = (q == 0) ? (56 + t) : (q == 1) ? t : (q == 2) ? (t * 8 + 7) : (t * 8);
+    dst = (q == 0) ? (t + 1) : (q == 1) ? (91 + t) : (q == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
+    const bool gh = tr_ntype(T, f) == T_GHOST;
+    hok = nb >= 0 && !(ghost_zero && gh);  // x = 0 outside the grid and on ghosts in the down-sweep
+    const int ln = hok ? nb : T.self;
+    hb = load_q_raw(R + size_t(ln) * 512 + src_pl * 4);
+    hd = *reinterpret_cast<const float4*>(c.diag + size_t(ln) * 512 + src_pl * 4);
+  }
+  // k halos: cell (i,j,7) of the z- neighbour and (i,j,0) of the z+ neighbour;
+  // their value after the first pass is 0 unless they are colour-0 cells
+  const int nzm = T.nbr[2], nzp = T.nbr[5];
+  const bool kokm = nzm >= 0 && !(ghost_zero && tr_ntype(T, 2) == T_GHOST) && ((i + j + 7) & 1) == 0;
+  const bool kokp = nzp >= 0 && !(ghost_zero && tr_ntype(T, 5) == T_GHOST) && ((i + j) & 1) == 0;
+  const int lzm = nzm >= 0 ? nzm : T.self, lzp = nzp >= 0 ? nzp : T.self;
+  typename RawV<V>::S kbm = load_s_raw(R + size_t(lzm) * 512 + split_cell(i, j, 7));
+  typename RawV<V>::S kbp = load_s_raw(R + size_t(lzp) * 512 + split_cell(i, j, 0));
+  float kdm = c.diag[size_t(lzm) * 512 + split_cell(i, j, 7)];
+  float kdp = c.diag[size_t(lzp) * 512 + split_cell(i, j, 0)];
+  hold(rb0);
+  hold(rb1);
+  hold(hb);
+  hold_f(hd.x);
+  hold_f(hd.y);
+  hold_f(hd.z);
+  hold_f(hd.w);
+  hold(kbm);
+  hold(kbp);
+  hold_f(kdm);
+  hold_f(kdp);
+  if (!hok) hd = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (!kokm) kdm = 0.f;
+  if (!kokp) kdp = 0.f;
+  // ---- pass 1: the tile's colour-0 cells from x = 0 (gs_quad as the colour-0 pass runs it)
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  const bool p0 = ((i + j) & 1) != 0, p1 = !p0;
+  const float4 n0 = gs_quad(z4, z4, z4, z4, z4, z4, 0.f, 0.f, quad_f<V>(rb0), mask, T.bc, T.h, i, j, p0);
+  const float4 xr = make_float4(vround<V>(n0.x), vround<V>(n0.y), vround<V>(n0.z), vround<V>(n0.w));
+  // the halo's colour-0 values as that pass leaves them
+  const float4 hbf = quad_f<V>(hb);
+  const float4 hv = make_float4(first_red<V>(hbf.x, hd.x), first_red<V>(hbf.y, hd.y), first_red<V>(hbf.z, hd.z),
+                                first_red<V>(hbf.w, hd.w));
+  const float xkm = first_red<V>(single_f<V>(kbm), kdm);
+  const float xkp = first_red<V>(single_f<V>(kbp), kdp);
+  L.xo[(i + 1) * 10 + (j + 1)] = xr;
+  if (lane < 32) L.xo[dst] = hv;
+  wave_sync();
+  if ((flags & F_CHECK_DONE) && done) return;
+  // ---- pass 2: colour 1, its own values 0 before the update
+  const int pc = (i + 1) * 10 + (j + 1);
+  const float4 n1 = gs_quad(z4, xr, L.xo[pc - 10], L.xo[pc + 10], L.xo[pc - 1], L.xo[pc + 1], xkm, xkp,
+                            quad_f<V>(rb1), mask, T.bc, T.h, i, j, p1);
+  V* tile = X + size_t(T.self) * 512;
+  store4(tile + lane * 4, xr);
+  store4(tile + 256 + lane * 4, n1);
+}
+
 template <class V>
 __global__ __launch_bounds__(64) void k_gs_prolong(Ctx c, const TileRec* __restrict__ list, int n, const V* XD,
                                                    V* XO, float pcoef, int color, int flags) {

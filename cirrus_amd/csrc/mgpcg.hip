@@ -17,6 +17,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <type_traits>
 #include <vector>
 
 #include "cirrus_mgpcg.h"
@@ -98,6 +99,7 @@ struct cg_problem {
   // device
   uint8_t* d_ctype = nullptr;
   uint8_t* d_fmask = nullptr;
+  float* d_diag = nullptr;  // the colour sweep's diagonal per cell (k_diag), for k_gs_first
   float* d_amgc = nullptr;    // AMG stored coefficients: off x, off y, off z, diag (4 x cells; cg_amg_prepare)
   uint8_t* d_amgct = nullptr; // AMG cell types
   bool amg_ready = false;
@@ -136,6 +138,7 @@ struct cg_problem {
     Ctx c;
     c.ctype = d_ctype;
     c.fmask = d_fmask;
+    c.diag = d_diag;
     c.x = d_ch[CG_CH_X];
     c.r = r;
     c.p = d_ch[CG_CH_P];
@@ -166,7 +169,7 @@ cg_problem::~cg_problem() {
   if (graph) (void)hipGraphExecDestroy(graph);
   for (auto e : ev) if (e) (void)hipEventDestroy(e);
   if (h_pin) (void)hipHostFree(h_pin);
-  void* ptrs[] = {d_ctype, d_fmask, d_zb, d_zh, d_rhsvh, d_zbh, d_x64, d_r64, d_bsave, d_psum, d_gsum, d_ps, d_dflag,
+  void* ptrs[] = {d_ctype, d_fmask, d_diag, d_zb, d_zh, d_rhsvh, d_zbh, d_x64, d_r64, d_bsave, d_psum, d_gsum, d_ps, d_dflag,
                   d_plans, d_part, d_part2, d_scalar, d_lists, d_tiles, d_tmp_list, d_coarse, d_amgc, d_amgct};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
@@ -547,6 +550,7 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   const size_t cells = size_t(n) * 512;
   HIPC(hipMalloc(&P.d_ctype, cells));
   HIPC(hipMalloc(&P.d_fmask, cells));
+  HIPC(hipMalloc(&P.d_diag, cells * sizeof(float)));
   for (int c = 0; c < CG_NUM_CHANNELS; c++) {
     HIPC(hipMalloc(&P.d_ch[c], cells * sizeof(float)));
     HIPC(hipMemsetAsync(P.d_ch[c], 0, cells * sizeof(float), P.stream));
@@ -580,6 +584,7 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
     HIPC(hipStreamSynchronize(P.stream));
   }
   k_face_mask<<<P.all.n, 64, 0, P.stream>>>(P.d_ctype, P.d_fmask, P.L(P.all), P.all.n);
+  k_diag<<<P.all.n, 64, 0, P.stream>>>(P.ctx(), P.L(P.all), P.all.n, P.d_diag);
   HIPC(hipGetLastError());
   HIPC(hipHostMalloc(&P.h_pin, 2 * sizeof(int) * nprob, hipHostMallocDefault));
   HIPC(hipEventCreateWithFlags(&P.ev[0], hipEventDisableTiming));
@@ -696,6 +701,13 @@ void ensure_bf16(cg_problem& P) {
 // damped-Jacobi smoother, which ping-pongs with the scratch buffer instead).
 // V is the storage type of Z, ZO, the scratch buffer and the V-cycle rhs (R is
 // only a placeholder: in V-cycle mode the rhs comes from the V-cycle rhs array).
+// The fused first iteration (k_gs_first) saves a launch and the first pass's
+// traffic, but reads the halo's diagonals, which no other kernel brings into
+// the cache; with 16-bit storage on a level of many tiles that costs more than
+// the second launch saves (measured on the 256-problem pool), so large 16-bit
+// levels keep the two passes.  Results are the same either way.
+constexpr int kFirstFused16Max = 16384;
+
 // last_flags is added to the last colour pass of each grid-wide level's
 // up-sweep (RB-GS with nu >= 1), the pass after which that level's z is final.
 template <class V>
@@ -708,6 +720,11 @@ void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v, in
     const DList& a = P.actg[l];
     if (v.smoother == CG_SMOOTH_RBGS) {
       for (int s = 0; s < v.nu; s++) {
+        if (s == 0 && P.d_diag && (std::is_same<V, float>::value || a.n <= kFirstFused16Max)) {
+          // both colours of the first iteration from the zero guess in one launch
+          if (a.n) k_gs_first<V><<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, Z, base | F_GHOST_ZERO);
+          continue;
+        }
         launch_gs(P, a, c, l, Z, R, 0, base | F_GHOST_ZERO | (s == 0 ? F_ZERO_X : 0));
         launch_gs(P, a, c, l, Z, R, 1, base | F_GHOST_ZERO);
       }

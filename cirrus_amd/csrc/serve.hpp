@@ -114,6 +114,7 @@ struct cg_server {
   cg_problem view;  // the kernels' view of the assembled active set (owner = false)
   uint8_t* d_ctype = nullptr;
   uint8_t* d_fmask = nullptr;
+  float* d_diag = nullptr;
   float* d_ch[CG_NUM_CHANNELS] = {};
   float* d_zb = nullptr;
   float* d_b = nullptr;  // each instance's right-hand side, kept across its requests
@@ -160,7 +161,7 @@ cg_server::~cg_server() {
     if (I->ev_ver) (void)hipEventDestroy(I->ev_ver);
   }
   if (h_ver) (void)hipHostFree(h_ver);
-  void* ptrs[] = {d_ctype, d_fmask, d_zb, d_b, d_x64, d_r64, d_ver, d_zh, d_rhsvh, d_zbh, d_gsum, d_ps, d_dflag, d_dflag_init, d_pmap, d_newmap, d_part, d_part2, d_asm,
+  void* ptrs[] = {d_ctype, d_fmask, d_diag, d_zb, d_b, d_x64, d_r64, d_ver, d_zh, d_rhsvh, d_zbh, d_gsum, d_ps, d_dflag, d_dflag_init, d_pmap, d_newmap, d_part, d_part2, d_asm,
                   d_plans, (void*)d_srcs, d_offs, d_counts};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
@@ -173,13 +174,13 @@ cg_server::~cg_server() {
 
 namespace {
 
-// Device bytes one tile costs: 6 channels, the V-cycle scratch and the kept
-// rhs (FP32), cell types and face masks, ghost row sums, dot partials and room
+// Device bytes one tile costs: 6 channels, the V-cycle scratch, the kept rhs
+// and the colour sweep's diagonal (FP32), cell types and face masks, ghost row sums, dot partials and room
 // for its records in the assembled worklists; BF16 storage adds three bfloat16
 // vectors, verify_f64 two FP64 vectors.
 constexpr int kRecKindsPerTile = 5;  // leaf, gather, applyl, actg, lvlg: at most 5 records per tile
 constexpr int64_t kBytesPerTile =
-    (CG_NUM_CHANNELS + 2) * 512 * 4 + 2 * 512 + 64 * 4 + 2 * 8 + kRecKindsPerTile * int64_t(sizeof(TileRec));
+    (CG_NUM_CHANNELS + 3) * 512 * 4 + 2 * 512 + 64 * 4 + 2 * 8 + kRecKindsPerTile * int64_t(sizeof(TileRec));
 
 void server_create(const cg_server_config* cfg, cg_server& S) {
   if (!cfg || cfg->budget_bytes <= 0 || cfg->max_problems <= 0) fail(CG_ERR_INVALID, "cg_server_create: bad configuration");
@@ -212,6 +213,7 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   const size_t cells = size_t(S.cap_tiles) * 512;
   HIPC(hipMalloc(&S.d_ctype, cells));
   HIPC(hipMalloc(&S.d_fmask, cells));
+  HIPC(hipMalloc(&S.d_diag, cells * sizeof(float)));
   for (int c = 0; c < CG_NUM_CHANNELS; c++) HIPC(hipMalloc(&S.d_ch[c], cells * sizeof(float)));
   HIPC(hipMalloc(&S.d_zb, cells * sizeof(float)));
   HIPC(hipMalloc(&S.d_b, cells * sizeof(float)));
@@ -262,6 +264,7 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   P.stream = S.stream;
   P.d_ctype = S.d_ctype;
   P.d_fmask = S.d_fmask;
+  P.d_diag = S.d_diag;
   for (int c = 0; c < CG_NUM_CHANNELS; c++) P.d_ch[c] = S.d_ch[c];
   P.d_zb = S.d_zb;
   P.d_zh = S.d_zh;
@@ -358,6 +361,7 @@ void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
   HIPC(hipMemcpy(S.d_ctype + size_t(tb) * 512, ct.data(), ct.size(), hipMemcpyHostToDevice));
   // face masks on the server's stream, ahead of the instance's first request
   k_face_mask<<<I->all.n, 64, 0, S.stream>>>(S.d_ctype, S.d_fmask, I->L(I->all), I->all.n);
+  k_diag<<<I->all.n, 64, 0, S.stream>>>(S.view.ctx(), I->L(I->all), I->all.n, S.d_diag);
   HIPC(hipGetLastError());
   HIPC(hipMemset(S.d_ch[CG_CH_X] + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float)));
   HIPC(hipMemset(S.d_b + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float)));
