@@ -24,6 +24,7 @@
 // per problem over the tile partials the same way).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 
 namespace cg {
@@ -362,6 +363,71 @@ __device__ __forceinline__ void store_quad_split(V* __restrict__ tile, int pl, i
   store2(tile + 256 + pl * 4 + (K0 >> 1), q0 ? E : O);
 }
 
+// Raw loads: a quad or a single value of V-typed storage as loaded (FP32
+// values, or the bits of 16-bit ones), widened to FP32 only after every load of
+// a tile has been issued; a widening inside the branch that loads a value would
+// make that branch wait for its load.  hold() puts an empty asm the compiler
+// cannot see through on the raw bits, so that it cannot move the widening up.
+template <class V> struct RawV;
+template <> struct RawV<float> { typedef float4 Q; typedef float S; };
+template <> struct RawV<bf16> { typedef uint2 Q; typedef uint32_t S; };
+template <> struct RawV<f16> { typedef uint2 Q; typedef uint32_t S; };
+__device__ __forceinline__ float4 load_q_raw(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ uint2 load_q_raw(const bf16* p) { return *reinterpret_cast<const uint2*>(p); }
+__device__ __forceinline__ uint2 load_q_raw(const f16* p) { return *reinterpret_cast<const uint2*>(p); }
+__device__ __forceinline__ float load_s_raw(const float* p) { return *p; }
+__device__ __forceinline__ uint32_t load_s_raw(const bf16* p) { return *reinterpret_cast<const uint16_t*>(p); }
+__device__ __forceinline__ uint32_t load_s_raw(const f16* p) { return *reinterpret_cast<const uint16_t*>(p); }
+template <class V> __device__ __forceinline__ float4 quad_f(typename RawV<V>::Q q);
+template <> __device__ __forceinline__ float4 quad_f<float>(float4 q) { return q; }
+template <> __device__ __forceinline__ float4 quad_f<bf16>(uint2 q) { return make_float4(bf_lo(q.x), bf_hi(q.x), bf_lo(q.y), bf_hi(q.y)); }
+template <> __device__ __forceinline__ float4 quad_f<f16>(uint2 q) { return make_float4(h_lo(q.x), h_hi(q.x), h_lo(q.y), h_hi(q.y)); }
+template <class V> __device__ __forceinline__ float single_f(typename RawV<V>::S v);
+template <> __device__ __forceinline__ float single_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ float single_f<bf16>(uint32_t v) { return bf_lo(v); }
+template <> __device__ __forceinline__ float single_f<f16>(uint32_t v) { return h_lo(v); }
+__device__ __forceinline__ void hold(float4&) {}
+__device__ __forceinline__ void hold(float&) {}
+// the same for an FP32 value whose later select must not pull its wait forward
+__device__ __forceinline__ void hold_f(float& v) { __asm__("" : "+v"(v)); }
+__device__ __forceinline__ void hold(uint2& q) { __asm__("" : "+v"(q.x), "+v"(q.y)); }
+__device__ __forceinline__ void hold(uint32_t& v) { __asm__("" : "+v"(v)); }
+
+// Raw loads of a colour-split pencil (both colour halves' quads), widened and
+// put in natural order only after every load of the tile is out.
+template <class V> struct RawPencil { typename RawV<V>::Q h0, h1; };
+template <class V>
+__device__ __forceinline__ RawPencil<V> load_pencil_split_raw(const V* __restrict__ tile, int pl) {
+  RawPencil<V> r;
+  r.h0 = load_q_raw(tile + pl * 4);
+  r.h1 = load_q_raw(tile + 256 + pl * 4);
+  return r;
+}
+// load_pencil_split's arrangement of the raw halves
+template <class V>
+__device__ __forceinline__ void pencil_split_f(const RawPencil<V>& r, int pl, float4& a, float4& b) {
+  const bool q0 = even_colour(pl);
+  const float4 h0 = quad_f<V>(r.h0), h1 = quad_f<V>(r.h1);
+  const float4 E = sel4(q0, h0, h1), O = sel4(q0, h1, h0);
+  a = make_float4(E.x, O.x, E.y, O.y);
+  b = make_float4(E.z, O.z, E.w, O.w);
+}
+// hold for every storage type (FP32 values too: their rearrangement must not
+// pull the wait forward either)
+__device__ __forceinline__ void hold_all(float4& q) {
+  hold_f(q.x);
+  hold_f(q.y);
+  hold_f(q.z);
+  hold_f(q.w);
+}
+__device__ __forceinline__ void hold_all(uint2& q) { hold(q); }
+__device__ __forceinline__ void hold_all(float& v) { hold_f(v); }
+__device__ __forceinline__ void hold_all(uint32_t& v) { hold(v); }
+template <class V> __device__ __forceinline__ void hold_all(RawPencil<V>& r) {
+  hold_all(r.h0);
+  hold_all(r.h1);
+}
+
 // ---------------------------------------------------------------------------
 // Tile + halo staging
 // ---------------------------------------------------------------------------
@@ -388,10 +454,88 @@ struct Stencil {
 // ping-pong, where ghost values stay in the primary buffer).  SPLIT: X and XG
 // are V-cycle arrays in the colour-split layout (the staged pencils are in
 // natural order either way).
+template <class V>
+struct StageRaw {
+  RawPencil<V> own, halo;
+  typename RawV<V>::S km, kp;
+  uint64_t m;
+  int dst, src;
+  bool own_zero, hok, kokm, kokp;
+};
+// The loads of stage_tile on the colour-split arrays, as raw bits: the own
+// pencil, the face masks, the halo pencil of lanes 0..31 and the two k halos.
+// A missing neighbour (or one whose values count as 0) is read at the tile
+// itself and the value dropped afterwards, so that no load sits in an inner
+// branch.
+template <class V>
+__device__ __forceinline__ StageRaw<V> stage_issue(const Ctx& c, const TileRec& T, const V* __restrict__ X,
+                                                   const V* __restrict__ XG, bool zero_all, bool ghost_zero) {
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const bool own_ghost = tr_type(T) == T_GHOST;
+  StageRaw<V> r{};
+  r.own_zero = zero_all || (ghost_zero && own_ghost);
+  if (!r.own_zero) r.own = load_pencil_split_raw((own_ghost ? XG : X) + size_t(T.self) * 512, lane);
+  r.m = load_ct8(c.fmask + size_t(T.self) * 512 + lane * 8);
+  // halo ring: lanes 0..31 fetch one neighbour pencil each
+  //   g = 0: x- (pencil (7,t) of nbr 0), 1: x+ ((0,t) of nbr 3), 2: y- ((t,7) of nbr 1), 3: y+ ((t,0) of nbr 4)
+  if (lane < 32) {
+    const int g = lane >> 3, t = lane & 7;
+    const int f = (g == 0) ? 0 : (g == 1) ? 3 : (g == 2) ? 1 : 4;
+    const int n = pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
+    r.src = (g == 0) ? (56 + t) : (g == 1) ? t : (g == 2) ? (t * 8 + 7) : (t * 8);
+    r.dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
+    const bool gh = tr_ntype(T, f) == T_GHOST;
+    r.hok = n >= 0 && !(zero_all || (ghost_zero && gh));  // else x = 0 (outside the grid: GMGSolver.cu:120-122)
+    r.halo = load_pencil_split_raw((r.hok && gh ? XG : X) + size_t(r.hok ? n : T.self) * 512, r.src);
+  }
+  // k halos: cell (i,j,7) of the z- neighbour, (i,j,0) of the z+ neighbour
+  const int nm = T.nbr[2], np = T.nbr[5];
+  const bool ghm = tr_ntype(T, 2) == T_GHOST, ghp = tr_ntype(T, 5) == T_GHOST;
+  r.kokm = nm >= 0 && !(zero_all || (ghost_zero && ghm));
+  r.kokp = np >= 0 && !(zero_all || (ghost_zero && ghp));
+  r.km = load_s_raw((r.kokm && ghm ? XG : X) + size_t(r.kokm ? nm : T.self) * 512 + split_cell(i, j, 7));
+  r.kp = load_s_raw((r.kokp && ghp ? XG : X) + size_t(r.kokp ? np : T.self) * 512 + split_cell(i, j, 0));
+  return r;
+}
+// The rest of stage_tile: one wait, then the values into LDS and the stencil.
+template <class V>
+__device__ __forceinline__ void stage_commit(StageRaw<V>& r, const TileRec& T, const float bcv[7], TileLDS& L,
+                                             Stencil& S) {
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  hold_all(r.own);
+  hold_all(r.halo);
+  hold_all(r.km);
+  hold_all(r.kp);
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+  if (!r.own_zero) pencil_split_f(r.own, lane, a, b);
+  S.own[0] = a.x; S.own[1] = a.y; S.own[2] = a.z; S.own[3] = a.w;
+  S.own[4] = b.x; S.own[5] = b.y; S.own[6] = b.z; S.own[7] = b.w;
+  S.m = r.m;
+  const int pc = (i + 1) * 10 + (j + 1);
+  L.xs[0][pc] = a;
+  L.xs[1][pc] = b;
+  if (lane < 32) {
+    float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+    if (r.hok) pencil_split_f(r.halo, r.src, v0, v1);
+    L.xs[0][r.dst] = v0;
+    L.xs[1][r.dst] = v1;
+  }
+  S.xkm = r.kokm ? single_f<V>(r.km) : 0.f;
+  S.xkp = r.kokp ? single_f<V>(r.kp) : 0.f;
+  S.h = T.h;
+  S.b_int = bcv[6];
+  for (int f = 0; f < 6; f++) S.b_f[f] = bcv[f];
+}
+
 template <bool SPLIT, class V>
 __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const V* __restrict__ X,
                                            const V* __restrict__ XG, bool zero_all, bool ghost_zero,
                                            const float bcv[7], TileLDS& L, Stencil& S) {
+  if (SPLIT && !std::is_same<V, float>::value) {  // 16-bit storage: every load before the first wait
+    StageRaw<V> r = stage_issue(c, T, X, XG, zero_all, ghost_zero);
+    stage_commit(r, T, bcv, L, S);
+    return;
+  }
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t base = size_t(T.self) * 512 + lane * 8;
   const bool own_ghost = tr_type(T) == T_GHOST;
@@ -571,12 +715,6 @@ __device__ __forceinline__ void gs_pencil(const NbrPencils& N, const Stencil& S,
 // ---------------------------------------------------------------------------
 // Per-tile bodies (one wavefront each)
 // ---------------------------------------------------------------------------
-// The V-cycle rhs of the tile (colour-split layout), natural order.
-template <class V>
-__device__ __forceinline__ void load_rhs(const Ctx& c, const TileRec& T, const V* F, int flags, float b[8]) {
-  const V* src = (flags & F_VCYCLE) ? vc_rhs<V>(c) : F;
-  load8_split(src + size_t(T.self) * 512, lane_id(), b);
-}
 
 // The four updated cells of a red-black colour pass, in split-layout quads:
 // cell k = 2m + p of lane (i,j) (p = (color+i+j)&1) with value X0.m, rhs B.m,
@@ -630,31 +768,6 @@ struct GsLDS {
 // branch that loads it, that would make the branch wait for its load, so the
 // raw bits are held (an empty asm the compiler cannot see through) until all
 // loads are out.
-template <class V> struct RawV;
-template <> struct RawV<float> { typedef float4 Q; typedef float S; };
-template <> struct RawV<bf16> { typedef uint2 Q; typedef uint32_t S; };
-template <> struct RawV<f16> { typedef uint2 Q; typedef uint32_t S; };
-__device__ __forceinline__ float4 load_q_raw(const float* p) { return *reinterpret_cast<const float4*>(p); }
-__device__ __forceinline__ uint2 load_q_raw(const bf16* p) { return *reinterpret_cast<const uint2*>(p); }
-__device__ __forceinline__ uint2 load_q_raw(const f16* p) { return *reinterpret_cast<const uint2*>(p); }
-__device__ __forceinline__ float load_s_raw(const float* p) { return *p; }
-__device__ __forceinline__ uint32_t load_s_raw(const bf16* p) { return *reinterpret_cast<const uint16_t*>(p); }
-__device__ __forceinline__ uint32_t load_s_raw(const f16* p) { return *reinterpret_cast<const uint16_t*>(p); }
-template <class V> __device__ __forceinline__ float4 quad_f(typename RawV<V>::Q q);
-template <> __device__ __forceinline__ float4 quad_f<float>(float4 q) { return q; }
-template <> __device__ __forceinline__ float4 quad_f<bf16>(uint2 q) { return make_float4(bf_lo(q.x), bf_hi(q.x), bf_lo(q.y), bf_hi(q.y)); }
-template <> __device__ __forceinline__ float4 quad_f<f16>(uint2 q) { return make_float4(h_lo(q.x), h_hi(q.x), h_lo(q.y), h_hi(q.y)); }
-template <class V> __device__ __forceinline__ float single_f(typename RawV<V>::S v);
-template <> __device__ __forceinline__ float single_f<float>(float v) { return v; }
-template <> __device__ __forceinline__ float single_f<bf16>(uint32_t v) { return bf_lo(v); }
-template <> __device__ __forceinline__ float single_f<f16>(uint32_t v) { return h_lo(v); }
-__device__ __forceinline__ void hold(float4&) {}
-__device__ __forceinline__ void hold(float&) {}
-// the same for an FP32 value whose later select must not pull its wait forward
-__device__ __forceinline__ void hold_f(float& v) { __asm__("" : "+v"(v)); }
-__device__ __forceinline__ void hold(uint2& q) { __asm__("" : "+v"(q.x), "+v"(q.y)); }
-__device__ __forceinline__ void hold(uint32_t& v) { __asm__("" : "+v"(v)); }
-
 struct GsIn {
   float4 xc, xo, B, halo;
   float4 r0, r1;  // with F_DOT_RZ on a LEAF tile: r of the lane's pencil, cells 0..3 and 4..7
@@ -793,15 +906,39 @@ __device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int leve
   gs_finish<V, DOT>(c, T, X, color, flags, g, L);
 }
 
+// stage_tile on the V-cycle arrays plus the tile's rhs b (natural order; 0 with
+// zero_b).  With 16-bit storage the rhs is loaded with the stage's loads, ahead
+// of the one wait; FP32 keeps the rhs load after the staging, which measured
+// faster on the 256-problem pool.
+template <class V>
+__device__ __forceinline__ void stage_with_rhs(const Ctx& c, const TileRec& T, const V* X, const V* XG, const V* F,
+                                               int flags, bool zero_b, TileLDS& L, Stencil& S, float b[8]) {
+  const V* src = ((flags & F_VCYCLE) ? vc_rhs<V>(c) : F) + size_t(T.self) * 512;
+  if (std::is_same<V, float>::value) {
+    stage_tile<true>(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
+    if (zero_b) { for (int k = 0; k < 8; k++) b[k] = 0.f; }
+    else load8_split(src, lane_id(), b);
+    return;
+  }
+  StageRaw<V> r = stage_issue(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO);
+  RawPencil<V> rb{};
+  if (!zero_b) rb = load_pencil_split_raw(src, lane_id());
+  hold_all(rb);
+  stage_commit(r, T, T.bc, L, S);
+  float4 ba = make_float4(0.f, 0.f, 0.f, 0.f), bb = ba;
+  if (!zero_b) pencil_split_f(rb, lane_id(), ba, bb);
+  b[0] = ba.x; b[1] = ba.y; b[2] = ba.z; b[3] = ba.w;
+  b[4] = bb.x; b[5] = bb.y; b[6] = bb.z; b[7] = bb.w;
+}
+
 // Damped-Jacobi sweep (the update of GMGSolver.cu:508-525 with the level's
 // operator): Y = X + omega (b - A X) / D on INTERIOR cells, 0 where D == 0.
 template <class V>
 __device__ __forceinline__ void tile_jacobi(const Ctx& c, const TileRec& T, int level, const V* X, const V* XG,
                                             V* Y, const V* F, float omega, int flags, TileLDS& L) {
   Stencil S;
-  stage_tile<true>(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
   float b[8];
-  load_rhs(c, T, F, flags, b);
+  stage_with_rhs(c, T, X, XG, F, flags, false, L, S, b);
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float ax[8], dg[8];
@@ -849,11 +986,9 @@ template <class V>
 __device__ __forceinline__ void tile_restrict(const Ctx& c, const TileRec& T, int level, const V* X, const V* F,
                                               V* CO, float one_over_alpha, int flags, TileLDS& L, float* res) {
   Stencil S;
-  stage_tile<true>(c, T, X, X, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
   const int lane = lane_id();
   float b[8];
-  if ((flags & F_VCYCLE) && tr_type(T) == T_GHOST) { for (int k = 0; k < 8; k++) b[k] = 0.f; }
-  else load_rhs(c, T, F, flags, b);
+  stage_with_rhs(c, T, X, X, F, flags, (flags & F_VCYCLE) && tr_type(T) == T_GHOST, L, S, b);
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float ax[8], dg[8];
