@@ -1931,14 +1931,27 @@ __device__ __forceinline__ double block_sum_partials(const double* __restrict__ 
 
 // Per-problem finalisation of a reduction over the LEAF-tile partials.
 enum : int { FIN_BB = 0, FIN_RZ = 1, FIN_PAP = 2, FIN_RR = 3, FIN_MEAN = 4, FIN_IRBB = 5, FIN_SCALE = 6 };
-__global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctx c, int mode, int flags) {
+// range_count >= 0: the launch's one problem has its partials at range_begin ..
+// range_begin + range_count - 1 (its leaf range), given by the host so that
+// their loads go out with the problem state's rather than after it.
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctx c, int mode, int flags, int range_begin = 0,
+                                                          int range_count = -1) {
   __shared__ double lds[FIN_CHUNK];
   ProbState* ps = c.ps + (c.pmap ? c.pmap[blockIdx.x] : int(blockIdx.x));
-  if (mode != FIN_BB && (flags & F_CHECK_DONE) && ps->done) return;
-  if (mode == FIN_MEAN && !ps->pure_neumann) return;
-  const double s = block_sum_partials(c.part, ps->leaf_begin, ps->leaf_count, lds);
+  const bool given = range_count >= 0;
+  if (!given) {
+    if (mode != FIN_BB && (flags & F_CHECK_DONE) && ps->done) return;
+    if (mode == FIN_MEAN && !ps->pure_neumann) return;
+  }
+  const int begin = given ? range_begin : ps->leaf_begin;
+  const int count = given ? range_count : ps->leaf_count;
+  const double s = block_sum_partials(c.part, begin, count, lds);
+  if (given) {  // the same tests, after the sum (whose value is then dropped)
+    if (mode != FIN_BB && (flags & F_CHECK_DONE) && ps->done) return;
+    if (mode == FIN_MEAN && !ps->pure_neumann) return;
+  }
   double q = 0.0;
-  if (mode == FIN_MEAN) q = block_sum_partials(c.part2, ps->leaf_begin, ps->leaf_count, lds);
+  if (mode == FIN_MEAN) q = block_sum_partials(c.part2, begin, count, lds);
   if (threadIdx.x != 0) return;
   switch (mode) {
     case FIN_BB: {  // GMGSolver.cu:542-554

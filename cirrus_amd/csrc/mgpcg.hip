@@ -862,6 +862,9 @@ void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann) {
   const Ctx c = P.ctx();
   const int fl = P.swz | F_CHECK_DONE;
   const unsigned np = unsigned(P.nprob);  // one finalize workgroup per problem
+  // one problem and no problem map: its partials are slots 0 .. leaf tiles - 1
+  const bool one = P.nprob == 1 && !c.pmap && P.leaf_count.size() == 1 && P.leaf_begin.size() == 1;
+  const int rb = one ? P.leaf_begin[0] : 0, rc = one ? P.leaf_count[0] : -1;
   if (pure_neumann) {
     k_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, fl);
     k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_MEAN, fl);
@@ -875,13 +878,13 @@ void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann) {
   if (v.amg) enqueue_amg_vcycle(P, v);
   else enqueue_vcycle<V>(P, v.smoother == CG_SMOOTH_RBGS ? pool_zb<V>(P) : Z, Z, nullptr, v, rz_fused ? F_DOT_RZ : 0);
   if (!rz_fused) k_cg_dot_rz<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
-  k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_RZ, fl);
+  k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_RZ, fl, rb, rc);
   k_cg_update_p<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
   k_apply<<<grid_for(P.applyl.n, fl), 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, c.p, c.ap, -1, 0, 1, fl);
   if (P.gather.n) k_ghost_gather<<<grid_for(P.gather.n, fl), 64, 0, P.stream>>>(c, P.L(P.gather), P.gather.n, fl);
-  k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_PAP, fl);
+  k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_PAP, fl, rb, rc);
   k_cg_update_xr<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
-  k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_RR, fl);
+  k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_RR, fl, rb, rc);
 }
 void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann, int storage) {
   if (storage == CG_STORE_BF16) enqueue_cg_iteration<bf16>(P, v, pure_neumann);
