@@ -232,8 +232,9 @@ def run_batched(ca, scenes, B, reps, solver_kw, use_graph=True, storage=0, refin
     tune_rep = None
     if tune:
         B.set(ca.CH_B, b_all)
-        kw, tune_rep = S.autotune(B, 1000, 1e-6, solver_kw["level_iters"], solver_kw["coarsest_iters"],
-                                  smoother=solver_kw["smoother"], storage=storage, allow_bf16=True)
+        # candidates timed as the refined solve this form runs
+        kw, tune_rep = S.autotune_refined(B, 1e-6, 5e-7, 20, 1000, solver_kw["level_iters"], solver_kw["coarsest_iters"],
+                                          smoother=solver_kw["smoother"], storage=storage, allow_16bit=True)
 
     def once():
         B.set(ca.CH_B, b_all)

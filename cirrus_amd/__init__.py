@@ -44,7 +44,7 @@ EXPORTED_SYMBOLS = (
     "cg_server_create", "cg_server_destroy", "cg_server_admit", "cg_server_release", "cg_server_submit", "cg_server_step",
     "cg_server_next_result", "cg_server_info", "cg_server_perturb_rhs",
     "cg_refine_params_default", "cg_solve_refined", "cg_get_solution_f64",
-    "cg_velocity_divergence", "cg_add_gradient", "cg_project", "cg_set_schedule", "cg_autotune",
+    "cg_velocity_divergence", "cg_add_gradient", "cg_project", "cg_set_schedule", "cg_autotune", "cg_autotune_refined",
     "cg_tiles_save", "cg_tiles_load", "cg_tiles_import_holder", "cg_tileset_tree", "cg_tileset_channels",
     "cg_tileset_destroy", "cg_amg_prepare", "cg_amg_solve", "cg_amg_coefficients",
 )
@@ -171,6 +171,7 @@ def lib():
             "cg_project": (i32, [vp, P(Params), P(Result)]),
             "cg_set_schedule": (i32, [vp, i32, i32, i32]),
             "cg_autotune": (i32, [vp, P(Params), i32, P(Params), P(TuneReport)]),
+            "cg_autotune_refined": (i32, [vp, P(Params), P(RefineParams), i32, P(Params), P(TuneReport)]),
             "cg_solve_refined": (i32, [vp, P(Params), P(RefineParams), P(RefineResult)]),
             "cg_get_solution_f64": (i32, [vp, vp]),
             "cg_scene_create": (i32, [P(SceneParams), P(vp)]),
@@ -619,6 +620,24 @@ class GMGSolver:
         rep = TuneReport()
         _check(lib().cg_autotune(prob.handle, ctypes.byref(p), int(bool(allow_bf16)), ctypes.byref(best),
                                  ctypes.byref(rep)), "GMGSolver.autotune")
+        kw = dict(level_iters=best.level_iters, smoother=best.smoother, storage=best.storage)
+        return kw, rep.as_dict()
+
+    def autotune_refined(self, prob, relative_tolerance=1e-6, inner_tolerance=5e-7, max_outer=20, max_iters=1000,
+                         level_iters=1, coarsest_iters=10, smoother=SMOOTH_RBGS, storage=STORE_BF16, allow_16bit=True):
+        """cg_autotune_refined: the same search, every candidate timed as the
+        FP64-refined solve (:meth:`solve_refined` with these tolerances);
+        returns (the chosen solver keyword arguments, the report)."""
+        p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
+                           coarsest_iters=coarsest_iters, smoother=smoother, storage=int(storage),
+                           one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff)
+        rp = RefineParams()
+        lib().cg_refine_params_default(ctypes.byref(rp))
+        rp.rel_tol, rp.inner_rel_tol, rp.max_outer = float(relative_tolerance), float(inner_tolerance), int(max_outer)
+        best = Params()
+        rep = TuneReport()
+        _check(lib().cg_autotune_refined(prob.handle, ctypes.byref(p), ctypes.byref(rp), int(bool(allow_16bit)),
+                                         ctypes.byref(best), ctypes.byref(rep)), "GMGSolver.autotune_refined")
         kw = dict(level_iters=best.level_iters, smoother=best.smoother, storage=best.storage)
         return kw, rep.as_dict()
 

@@ -1203,22 +1203,36 @@ void set_schedule(cg_problem& P, int coarse_shift, int swizzle, int tile_block) 
 // A candidate counts only if every problem converges; each is timed as the
 // best of two solves after a warm-up (graph capture).  The pool keeps the
 // chosen schedule; *best receives the chosen solver parameters.
-void autotune(cg_problem& P, const cg_params& base, int allow_16bit, cg_params* best, cg_tune_report* rep) {
+void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& rp, cg_refine_result* res);
+// rp non-null: every candidate is timed as the FP64-refined solve it would run
+// (cg_solve_refined), not as a single solve to base.rel_tol.
+void autotune(cg_problem& P, const cg_params& base, int allow_16bit, cg_params* best, cg_tune_report* rep,
+              const cg_refine_params* rp = nullptr) {
   check_params(base);
   const size_t bytes = size_t(P.n) * 512 * sizeof(float);
   DeviceBuffer keep(bytes);
   float* bkeep = static_cast<float*>(keep.p);
   HIPC(hipMemcpyAsync(bkeep, P.d_ch[CG_CH_R], bytes, hipMemcpyDeviceToDevice, P.stream));
   std::vector<cg_result> res(P.nprob);
+  std::vector<cg_refine_result> rres(P.nprob);
   int candidates = 0;
   auto run = [&](const cg_params& prm) {
     double t = std::numeric_limits<double>::infinity();
     for (int rep_i = 0; rep_i < 3; rep_i++) {
       HIPC(hipMemcpyAsync(P.d_ch[CG_CH_R], bkeep, bytes, hipMemcpyDeviceToDevice, P.stream));
-      solve(P, prm, res.data());
-      for (const cg_result& r : res)
-        if (!r.converged) return std::numeric_limits<double>::infinity();
-      if (rep_i > 0) t = std::min(t, res[0].seconds);
+      double secs = 0.0;
+      if (rp) {
+        solve_refined(P, prm, *rp, rres.data());
+        for (const cg_refine_result& r : rres)
+          if (!r.converged) return std::numeric_limits<double>::infinity();
+        secs = rres[0].seconds;
+      } else {
+        solve(P, prm, res.data());
+        for (const cg_result& r : res)
+          if (!r.converged) return std::numeric_limits<double>::infinity();
+        secs = res[0].seconds;
+      }
+      if (rep_i > 0) t = std::min(t, secs);
     }
     candidates++;
     return t;
@@ -1475,6 +1489,16 @@ cg_status cg_autotune(cg_problem* p, const cg_params* base, int32_t allow_16bit,
   return guard([&] {
     if (!p || !base || !best) fail(CG_ERR_INVALID, "cg_autotune: null argument");
     autotune(*p, *base, allow_16bit, best, report);
+  });
+}
+
+cg_status cg_autotune_refined(cg_problem* p, const cg_params* base, const cg_refine_params* rp, int32_t allow_16bit,
+                              cg_params* best, cg_tune_report* report) {
+  return guard([&] {
+    if (!p || !base || !rp || !best) fail(CG_ERR_INVALID, "cg_autotune_refined: null argument");
+    if (!(rp->rel_tol >= 0) || !(rp->inner_rel_tol >= 0) || rp->max_outer < 0)
+      fail(CG_ERR_INVALID, "cg_autotune_refined: invalid refinement parameters");
+    autotune(*p, *base, allow_16bit, best, report, rp);
   });
 }
 

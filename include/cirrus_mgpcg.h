@@ -201,6 +201,12 @@ typedef struct {
  * chosen schedule; *best receives the chosen parameters (base with smoother,
  * level_iters and storage replaced). */
 cg_status cg_autotune(cg_problem* p, const cg_params* base, int32_t allow_16bit, cg_params* best, cg_tune_report* report);
+/* The same search with every candidate timed as the FP64-refined solve
+ * (cg_solve_refined with rp) that the caller will run, whose best settings can
+ * differ from a single solve's (fewer sweeps per level win a single solve to
+ * rel_tol, more win the refined one).  No counterpart in the reference. */
+cg_status cg_autotune_refined(cg_problem* p, const cg_params* base, const cg_refine_params* rp, int32_t allow_16bit,
+                              cg_params* best, cg_tune_report* report);
 
 /* ---- The projection around the solve (src/FluidEuler.h:581-612) ---------- */
 /* VelocityVolumeDivergenceOnLeafs (src/PoissonSolver.cu:66-142) into div_channel: with

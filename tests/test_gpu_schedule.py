@@ -77,3 +77,23 @@ def test_autotune_on_a_batch():
     assert np.array_equal(B.get(ca.CH_B), b)  # the right-hand sides are restored
     res = S.solve_batch(B, 1000, 1e-6, coarsest_iters=10, verify_f64=True, **kw)
     assert all(r["converged"] == 1 and r["rel_res_f64"] < 5e-6 for r in res)
+
+
+def test_autotune_refined_on_a_batch():
+    """cg_autotune_refined times the FP64-refined solve; its choice then
+    refines every problem to the tolerance and the right-hand sides are back."""
+    trees, bs = [], []
+    for kw in SCENES.values():
+        sc = ca.Scene(**kw)
+        trees.append(sc.tree)
+        bs.append(sc.rhs.astype(np.float32))
+    B = ca.Problem.batch(trees)
+    b = np.concatenate(bs)
+    B.set(ca.CH_B, b)
+    S = ca.GMGSolver()
+    kw, rep = S.autotune_refined(B, 1e-6, 5e-7, 20, 1000, 2, 10, storage=ca.STORE_BF16, allow_16bit=True)
+    assert rep["candidates"] >= 8 and rep["best_seconds"] <= rep["base_seconds"]
+    assert np.array_equal(B.get(ca.CH_B), b)
+    res = S.solve_refined(B, 1e-6, 5e-7, 20, 1000, kw["level_iters"], 10, smoother=kw["smoother"],
+                          storage=kw["storage"])
+    assert all(r["converged"] == 1 and r["rel_res"] < 1e-6 for r in res)
