@@ -369,9 +369,17 @@ __device__ __forceinline__ void store_quad_split(V* __restrict__ tile, int pl, i
 // make that branch wait for its load.  hold() puts an empty asm the compiler
 // cannot see through on the raw bits, so that it cannot move the widening up.
 template <class V> struct RawV;
-template <> struct RawV<float> { typedef float4 Q; typedef float S; };
-template <> struct RawV<bf16> { typedef uint2 Q; typedef uint32_t S; };
-template <> struct RawV<f16> { typedef uint2 Q; typedef uint32_t S; };
+template <> struct RawV<float> { typedef float4 Q; typedef float S; typedef float2 S2; };
+template <> struct RawV<bf16> { typedef uint2 Q; typedef uint32_t S; typedef uint32_t S2; };
+template <> struct RawV<f16> { typedef uint2 Q; typedef uint32_t S; typedef uint32_t S2; };
+// a pair of consecutive values
+__device__ __forceinline__ float2 load2_raw(const float* p) { return *reinterpret_cast<const float2*>(p); }
+__device__ __forceinline__ uint32_t load2_raw(const bf16* p) { return *reinterpret_cast<const uint32_t*>(p); }
+__device__ __forceinline__ uint32_t load2_raw(const f16* p) { return *reinterpret_cast<const uint32_t*>(p); }
+template <class V> __device__ __forceinline__ float2 pair_f(typename RawV<V>::S2 v);
+template <> __device__ __forceinline__ float2 pair_f<float>(float2 v) { return v; }
+template <> __device__ __forceinline__ float2 pair_f<bf16>(uint32_t v) { return make_float2(bf_lo(v), bf_hi(v)); }
+template <> __device__ __forceinline__ float2 pair_f<f16>(uint32_t v) { return make_float2(h_lo(v), h_hi(v)); }
 __device__ __forceinline__ float4 load_q_raw(const float* p) { return *reinterpret_cast<const float4*>(p); }
 __device__ __forceinline__ uint2 load_q_raw(const bf16* p) { return *reinterpret_cast<const uint2*>(p); }
 __device__ __forceinline__ uint2 load_q_raw(const f16* p) { return *reinterpret_cast<const uint2*>(p); }
@@ -423,6 +431,10 @@ __device__ __forceinline__ void hold_all(float4& q) {
 __device__ __forceinline__ void hold_all(uint2& q) { hold(q); }
 __device__ __forceinline__ void hold_all(float& v) { hold_f(v); }
 __device__ __forceinline__ void hold_all(uint32_t& v) { hold(v); }
+__device__ __forceinline__ void hold_all(float2& v) {
+  hold_f(v.x);
+  hold_f(v.y);
+}
 template <class V> __device__ __forceinline__ void hold_all(RawPencil<V>& r) {
   hold_all(r.h0);
   hold_all(r.h1);
@@ -1320,6 +1332,161 @@ __global__ __launch_bounds__(64) void k_gs_first(Ctx c, const TileRec* __restric
   store4(tile + 256 + lane * 4, n1);
 }
 
+// The same pass with every global load issued first, as raw bits (own pencil,
+// parent quad, face masks, halo pencils and their parents' quads, k halos, rhs,
+// done flag) and one wait before the values are formed.  Missing neighbours
+// and lanes 32..63 read the tile itself and its parent (values dropped), so no
+// load sits in a branch.  Faster on small levels, where each launch waits on
+// memory latency; slower on big ones (more loads and registers), so the host
+// picks it by level size.  Every value is the one tile_gs_prolong forms.
+template <class V>
+struct KHaloRaw {
+  int n;
+  bool gh;
+  uint32_t mv;
+  typename RawV<V>::S pz, xo;
+};
+template <int F, int KN, class V>
+__device__ __forceinline__ KHaloRaw<V> khalo_prolong_issue(const Ctx& c, const TileRec& T, const V* XD, const V* XO,
+                                                           int oct, int lane) {
+  KHaloRaw<V> k;
+  k.n = T.nbr[F];
+  k.gh = tr_ntype(T, F) == T_GHOST;
+  const int n = k.n >= 0 ? k.n : T.self;
+  const int pn = k.n >= 0 ? T.nbr_parent[F] : T.parent;
+  const int i = lane >> 3, j = lane & 7;
+  k.mv = c.fmask[size_t(n) * 512 + lane * 8 + KN];
+  const int noct = oct ^ 4;
+  const int ox = noct & 1, oy = (noct >> 1) & 1, oz = (noct >> 2) & 1;
+  k.pz = load_s_raw(XO + size_t(pn) * 512 + split_cell(ox * 4 + (i >> 1), oy * 4 + (j >> 1), oz * 4 + (KN >> 1)));
+  k.xo = load_s_raw(XD + size_t(n) * 512 + split_cell(i, j, KN));
+  return k;
+}
+// khalo_prolong's value from the raw loads
+template <class V>
+__device__ __forceinline__ float khalo_prolong_value(KHaloRaw<V> k, float pcoef) {
+  if (k.n < 0) return 0.f;
+  const float pz = single_f<V>(k.pz);
+  const float xo = k.gh ? 0.f : single_f<V>(k.xo);
+  return (k.mv & M_INT) ? vround<V>(k.gh ? pz : __builtin_fmaf(pcoef, pz, xo)) : 0.f;
+}
+template <class V> struct RawQuadS { typename RawV<V>::S2 h0, h1; };
+template <class V>
+__device__ __forceinline__ RawQuadS<V> load_quad_split_raw(const V* __restrict__ tile, int pl, int K0) {
+  RawQuadS<V> r;
+  r.h0 = load2_raw(tile + pl * 4 + (K0 >> 1));
+  r.h1 = load2_raw(tile + 256 + pl * 4 + (K0 >> 1));
+  return r;
+}
+// load_quad_split's arrangement of the raw pairs
+template <class V>
+__device__ __forceinline__ float4 quad_split_f(const RawQuadS<V>& r, int pl) {
+  const bool q0 = even_colour(pl);
+  const float2 h0 = pair_f<V>(r.h0), h1 = pair_f<V>(r.h1);
+  const float2 E = q0 ? h1 : h0, O = q0 ? h0 : h1;
+  return make_float4(E.x, O.x, E.y, O.y);
+}
+template <class V>
+__device__ __forceinline__ void tile_gs_prolong_eager(const Ctx& c, const TileRec& T, const V* XD, V* XO, float pcoef,
+                                                      int color, int flags, TileLDS& L) {
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const size_t base = size_t(T.self) * 512 + lane * 8;
+  const int oct = tr_octant(T);
+  const bool own_ghost = tr_type(T) == T_GHOST;
+  // ---- loads
+  const int32_t* dp = (flags & F_CHECK_DONE) ? &c.ps[T.prob].done : reinterpret_cast<const int32_t*>(c.fmask);
+  const int32_t done = *dp;
+  const uint64_t mown = load_ct8(c.fmask + base);
+  const int ppl = ((oct & 1) * 4 + (i >> 1)) * 8 + (((oct >> 1) & 1) * 4 + (j >> 1));  // parent pencil
+  RawQuadS<V> rpq = load_quad_split_raw(XO + size_t(T.parent) * 512, ppl, ((oct >> 2) & 1) * 4);
+  RawPencil<V> rown{}, rrhs{}, rh{};
+  RawQuadS<V> rhpq{};
+  KHaloRaw<V> km{}, kp{};
+  uint64_t hw = 0;
+  int hn = -1, hpl = 0, hppl = 0, dst = 0;
+  bool hgh = false;
+  if (!own_ghost) {
+    rown = load_pencil_split_raw(XD + size_t(T.self) * 512, lane);
+    rrhs = load_pencil_split_raw(vc_rhs<V>(c) + size_t(T.self) * 512, lane);
+    // halo ring of lanes 0..31 as in tile_gs_prolong; every lane loads
+    const int g = (lane >> 3) & 3, t = lane & 7;
+    const int f = (g == 0) ? 0 : (g == 1) ? 3 : (g == 2) ? 1 : 4;
+    hn = lane < 32 ? pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4])) : -1;
+    const int pn = pick4(g, uni(T.nbr_parent[0]), uni(T.nbr_parent[3]), uni(T.nbr_parent[1]), uni(T.nbr_parent[4]));
+    const int ni = (g == 0) ? 7 : (g == 1) ? 0 : t, nj = (g == 2) ? 7 : (g == 3) ? 0 : t;  // pencil in the neighbour
+    dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
+    const int noct = oct ^ (1 << (f % 3));
+    hpl = ni * 8 + nj;
+    hppl = ((noct & 1) * 4 + (ni >> 1)) * 8 + (((noct >> 1) & 1) * 4 + (nj >> 1));
+    hgh = tr_ntype(T, f) == T_GHOST;
+    const int ln = hn >= 0 ? hn : T.self, lpn = hn >= 0 ? pn : T.parent;
+    hw = load_ct8(c.fmask + size_t(ln) * 512 + hpl * 8);
+    rh = load_pencil_split_raw(XD + size_t(ln) * 512, hpl);
+    rhpq = load_quad_split_raw(XO + size_t(lpn) * 512, hppl, ((noct >> 2) & 1) * 4);
+    km = khalo_prolong_issue<2, 7>(c, T, XD, XO, oct, lane);
+    kp = khalo_prolong_issue<5, 0>(c, T, XD, XO, oct, lane);
+  }
+  // ---- one wait, then the values
+  hold_all(rpq.h0);
+  hold_all(rpq.h1);
+  hold_all(rown);
+  hold_all(rrhs);
+  hold_all(rh);
+  hold_all(rhpq.h0);
+  hold_all(rhpq.h1);
+  hold_all(km.mv);
+  hold_all(km.pz);
+  hold_all(km.xo);
+  hold_all(kp.mv);
+  hold_all(kp.pz);
+  hold_all(kp.xo);
+  float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), xb = xa;
+  if (!own_ghost) pencil_split_f(rown, lane, xa, xb);
+  float4 oa, ob;
+  prolong_pencil<V>(xa, xb, quad_split_f(rpq, ppl), mown, own_ghost, pcoef, oa, ob);
+  if (own_ghost) {  // GHOST tile: its prolongated values only
+    if ((flags & F_CHECK_DONE) && done) return;
+    store_pencil_split(XO + size_t(T.self) * 512, lane, oa, ob);
+    return;
+  }
+  Stencil S;
+  S.own[0] = oa.x; S.own[1] = oa.y; S.own[2] = oa.z; S.own[3] = oa.w;
+  S.own[4] = ob.x; S.own[5] = ob.y; S.own[6] = ob.z; S.own[7] = ob.w;
+  S.m = mown;
+  const int pc = (i + 1) * 10 + (j + 1);
+  L.xs[0][pc] = oa;
+  L.xs[1][pc] = ob;
+  if (lane < 32) {
+    float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;  // x = 0 outside the grid (GMGSolver.cu:120-122)
+    if (hn >= 0) {
+      float4 ya = make_float4(0.f, 0.f, 0.f, 0.f), yb = ya;
+      if (!hgh) pencil_split_f(rh, hpl, ya, yb);
+      prolong_pencil<V>(ya, yb, quad_split_f(rhpq, hppl), hw, hgh, pcoef, va, vb);
+    }
+    L.xs[0][dst] = va;
+    L.xs[1][dst] = vb;
+  }
+  S.xkm = khalo_prolong_value(km, pcoef);
+  S.xkp = khalo_prolong_value(kp, pcoef);
+  S.h = T.h;
+  S.b_int = T.bc[6];
+  for (int f = 0; f < 6; f++) S.b_f[f] = T.bc[f];
+  float4 ba, bb;
+  pencil_split_f(rrhs, lane, ba, bb);
+  const float b[8] = {ba.x, ba.y, ba.z, ba.w, bb.x, bb.y, bb.z, bb.w};
+  if ((flags & F_CHECK_DONE) && done) return;
+  wave_sync();
+  float4 out0, out1;
+  gs_pencil(lds_nbrs(L), S, b, color, out0, out1);
+  store_pencil_split(XO + size_t(T.self) * 512, lane, out0, out1);
+}
+template <class V>
+__global__ __launch_bounds__(64) void k_gs_prolong_eager(Ctx c, const TileRec* __restrict__ list, int n, const V* XD,
+                                                         V* XO, float pcoef, int color, int flags) {
+  TILE_KERNEL_PROLOGUE
+  __shared__ TileLDS L;
+  tile_gs_prolong_eager(c, T, XD, XO, pcoef, color, flags, L);
+}
 template <class V>
 __global__ __launch_bounds__(64) void k_gs_prolong(Ctx c, const TileRec* __restrict__ list, int n, const V* XD,
                                                    V* XO, float pcoef, int color, int flags) {

@@ -707,6 +707,9 @@ void ensure_bf16(cg_problem& P) {
 // the second launch saves (measured on the 256-problem pool), so large 16-bit
 // levels keep the two passes.  Results are the same either way.
 constexpr int kFirstFused16Max = 16384;
+// Levels of at most this many tiles use k_gs_prolong_eager (all loads before
+// one wait: faster where a launch waits on latency, slower on big levels).
+constexpr int kProlongEagerMax = 16384;
 
 // last_flags is added to the last colour pass of each grid-wide level's
 // up-sweep (RB-GS with nu >= 1), the pass after which that level's z is final.
@@ -765,7 +768,10 @@ void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v, in
     const bool jac = v.smoother != CG_SMOOTH_RBGS;
     if (!jac && v.nu > 0) {
       // prolongation fused into the first (black) pass; the level's result goes to ZO
-      if (al.n) k_gs_prolong<V><<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, Z, ZO, v.pcoef, 1, base);
+      if (al.n > kProlongEagerMax)
+        k_gs_prolong<V><<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, Z, ZO, v.pcoef, 1, base);
+      else if (al.n)
+        k_gs_prolong_eager<V><<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, Z, ZO, v.pcoef, 1, base);
       launch_gs(P, a, c, l, ZO, R, 0, base | (v.nu == 1 ? last_flags : 0));
       for (int s = 1; s < v.nu; s++) {
         launch_gs(P, a, c, l, ZO, R, 1, base);
