@@ -922,11 +922,12 @@ __device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int leve
 // zero_b).  With 16-bit storage the rhs is loaded with the stage's loads, ahead
 // of the one wait; FP32 keeps the rhs load after the staging, which measured
 // faster on the 256-problem pool.
-template <class V>
+// EAGER: the raw form for FP32 as well (the host picks it for small levels).
+template <class V, bool EAGER = false>
 __device__ __forceinline__ void stage_with_rhs(const Ctx& c, const TileRec& T, const V* X, const V* XG, const V* F,
                                                int flags, bool zero_b, TileLDS& L, Stencil& S, float b[8]) {
   const V* src = ((flags & F_VCYCLE) ? vc_rhs<V>(c) : F) + size_t(T.self) * 512;
-  if (std::is_same<V, float>::value) {
+  if (std::is_same<V, float>::value && !EAGER) {
     stage_tile<true>(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
     if (zero_b) { for (int k = 0; k < 8; k++) b[k] = 0.f; }
     else load8_split(src, lane_id(), b);
@@ -994,13 +995,13 @@ __device__ __forceinline__ size_t parent_cell_split(const TileRec& T, int q) {
 // (GMGSolver.cu:284-339).  Output channel CO (rhsv in V-cycle mode): the
 // parent cell = sum * one_over_alpha, or += sum for a GHOST fine tile (whose
 // LEAF parent's rhs already holds r).  In V-cycle mode a ghost's rhs is 0.
-template <class V>
+template <class V, bool EAGER = false>
 __device__ __forceinline__ void tile_restrict(const Ctx& c, const TileRec& T, int level, const V* X, const V* F,
                                               V* CO, float one_over_alpha, int flags, TileLDS& L, float* res) {
   Stencil S;
   const int lane = lane_id();
   float b[8];
-  stage_with_rhs(c, T, X, X, F, flags, (flags & F_VCYCLE) && tr_type(T) == T_GHOST, L, S, b);
+  stage_with_rhs<V, EAGER>(c, T, X, X, F, flags, (flags & F_VCYCLE) && tr_type(T) == T_GHOST, L, S, b);
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float ax[8], dg[8];
@@ -1501,13 +1502,13 @@ __global__ __launch_bounds__(64) void k_jacobi(Ctx c, const TileRec* __restrict_
   __shared__ TileLDS L;
   tile_jacobi(c, T, level, X, XG, Y, F, omega, flags, L);
 }
-template <class V>
+template <class V, bool EAGER = false>
 __global__ __launch_bounds__(64) void k_restrict(Ctx c, const TileRec* __restrict__ list, int n, int level, const V* X,
                                                  const V* F, V* CO, float one_over_alpha, int flags) {
   TILE_KERNEL_PROLOGUE
   __shared__ TileLDS L;
   __shared__ float res[512];
-  tile_restrict(c, T, level, X, F, CO, one_over_alpha, flags, L, res);
+  tile_restrict<V, EAGER>(c, T, level, X, F, CO, one_over_alpha, flags, L, res);
 }
 template <class V>
 __global__ __launch_bounds__(64) void k_prolong(Ctx c, const TileRec* __restrict__ list, int n, int level,

@@ -707,8 +707,9 @@ void ensure_bf16(cg_problem& P) {
 // the second launch saves (measured on the 256-problem pool), so large 16-bit
 // levels keep the two passes.  Results are the same either way.
 constexpr int kFirstFused16Max = 16384;
-// Levels of at most this many tiles use k_gs_prolong_eager (all loads before
-// one wait: faster where a launch waits on latency, slower on big levels).
+// Levels of at most this many tiles use k_gs_prolong_eager and the eager
+// k_restrict (all loads before one wait: faster where a launch waits on
+// latency, slower on big levels).
 constexpr int kProlongEagerMax = 16384;
 
 // last_flags is added to the last colour pass of each grid-wide level's
@@ -742,7 +743,10 @@ void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v, in
     }
     const DList& al = P.lvlg[l];
     int fl = base | F_GHOST_ZERO | (v.nu == 0 ? F_ZERO_X : 0);
-    if (al.n) k_restrict<V><<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, Z, R, nullptr, v.ooa, fl);
+    if (al.n > kProlongEagerMax)
+      k_restrict<V><<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, Z, R, nullptr, v.ooa, fl);
+    else if (al.n)
+      k_restrict<V, true><<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, Z, R, nullptr, v.ooa, fl);
   }
   if (!P.plans.empty()) {
     VCycleArgs va{v.nu, v.coarsest, v.smoother, v.omega, v.ooa, v.pcoef};
