@@ -36,55 +36,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from cirrus_amd.workloads import ADAPTIVE256, PLAIN64, VORTEX256, VORTEX512, batch_scene_params, build_scenes  # noqa: E402
+
 WORKLOADS = {
     # name: (scene kwargs, solver kwargs, description)
-    "adaptive256": (dict(kind=1, solid_r=0.15, band_r=0.15, band_w=[0.05, 0.15], band_level=[5, 4, 3]),
-                    dict(level_iters=2, coarsest_iters=10, smoother=0),
+    "adaptive256": (ADAPTIVE256, dict(level_iters=2, coarsest_iters=10, smoother=0),
                     "adaptive octree 256^3 effective (levels 3-5, sphere band r=0.15), FP32, RB-GS 2+2, tol 1e-6"),
-    "plain64": (dict(kind=0, uniform_level=3, solid_r=0.2),
-                dict(level_iters=2, coarsest_iters=10, smoother=1),
+    "plain64": (PLAIN64, dict(level_iters=2, coarsest_iters=10, smoother=1),
                 "uniform 64^3, solid sphere r=0.2, FP32, damped Jacobi 2+2 (w=2/3), tol 1e-6"),
 }
 PEAK_HBM_GBS = 8000.0
-
-# Batched form: independent adaptive problems built like the plain ones, each
-# around a solid sphere (centre U[0.3,0.7]^3, radius U[0.1,0.25]) with a
-# refinement band; leaf-tile counts drawn from LogNormal(ln 2048, 1) clipped to
-# [64, 16384] and met approximately by the band levels and width (the realised
-# counts are reported); every domain face Neumann or Dirichlet at random, at
-# least one Dirichlet.  FP32, RB-GS 2+2, 10 bottom iterations, tol 1e-6.
-# Band configurations and their leaf-tile counts at radius 0.175 (counts scale
-# roughly with the sphere's area):
-_BAND_TABLE = [(2, 0.0, 64), (3, 0.0, 512), (3, 0.1, 288), (4, 0.005, 680), (4, 0.1, 1128), (5, 0.005, 1968),
-               (5, 0.02, 2528), (5, 0.05, 3760), (5, 0.1, 7400), (6, 0.005, 11096), (6, 0.02, 14736)]
-
-
-def batch_scene_params(q, seed=0):
-    """Scene keyword arguments of problem q of the batched workload."""
-    rng = np.random.default_rng([seed, q])
-    target = float(np.clip(rng.lognormal(np.log(2048.0), 1.0), 64, 16384))
-    r = float(rng.uniform(0.1, 0.25))
-    centre = list(rng.uniform(0.3, 0.7, 3))
-    bc = [4 if rng.random() < 0.5 else 1 for _ in range(6)]  # NEUMANN = 4, DIRICHLET = 1
-    if 1 not in bc:
-        bc[int(rng.integers(6))] = 1
-    scale = (r / 0.175) ** 2
-    best = min(_BAND_TABLE, key=lambda e: abs(np.log(e[2] * (scale if e[1] > 0 else 1.0)) - np.log(target)))
-    lf, w, _ = best
-    kw = dict(solid_c=centre, solid_r=r, face_bc=bc, seed=1000 + q)
-    if w == 0.0:
-        kw.update(kind=0, uniform_level=lf)
-    else:
-        kw.update(kind=1, band_c=centre, band_r=r, band_w=[w, 3 * w], band_level=[lf, lf - 1, max(lf - 2, 2)])
-    return kw, target
-
-
-def build_scenes(ca, kws, workers=16):
-    """Build the synthetic scenes of the given keyword sets on the host, in
-    parallel (the library's builder is re-entrant and releases the GIL)."""
-    from concurrent.futures import ThreadPoolExecutor
-    with ThreadPoolExecutor(max_workers=max(1, min(workers, len(kws)))) as ex:
-        return list(ex.map(lambda kw: ca.Scene(**kw), kws))
 
 
 def plain_drop_in(ca, items, solver_kw, reps=1):
@@ -320,8 +281,8 @@ def run_amg(ca, P, b_host, solver_kw, reps=10):
                              "GBps": alg / (us * 1e-6) / 1e9, "frac_hbm_peak": alg / (us * 1e-6) / 1e9 / PEAK_HBM_GBS}}
 
 
-VORTEX_KW = dict(kind=3, nroot=[1, 1, 2], vortex_level=[6, 4], vortex_thresh=0.5, seed=0)
-VORTEX_SMALL_KW = dict(kind=3, nroot=[1, 1, 2], vortex_level=[5, 3], vortex_thresh=0.5, seed=0)
+VORTEX_KW = VORTEX512
+VORTEX_SMALL_KW = VORTEX256
 
 
 def run_vortex(ca, solver_kw, reps=3, kw=VORTEX_KW):

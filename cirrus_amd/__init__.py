@@ -37,14 +37,15 @@ SCENE_UNIFORM, SCENE_SPHERE_BAND, SCENE_PLANE, SCENE_VORTEX = 0, 1, 2, 3
 EXPORTED_SYMBOLS = (
     "cg_params_default", "cg_last_error", "cg_device_count", "cg_problem_create", "cg_problem_create_batch",
     "cg_problem_num_problems", "cg_solve_batch", "cg_problem_destroy", "cg_problem_info",
-    "cg_problem_topology", "cg_set_channel", "cg_get_channel", "cg_solve", "cg_damped_jacobi_solve",
+    "cg_problem_topology", "cg_set_channel", "cg_get_channel", "cg_set_channel_device", "cg_get_channel_device",
+    "cg_solve", "cg_damped_jacobi_solve",
     "cg_vcycle", "cg_full_negative_laplacian", "cg_negative_laplacian_same_level", "cg_gauss_seidel_color",
     "cg_jacobi_sweep", "cg_residual_restrict", "cg_prolongate", "cg_dot", "cg_recenter_leaf_voxels",
     "cg_calc_cell_types_from_leafs", "cg_benchmark_kernel", "cg_scene_create", "cg_scene_destroy", "cg_scene_tree", "cg_scene_velocity",
     "cg_server_create", "cg_server_destroy", "cg_server_admit", "cg_server_release", "cg_server_submit", "cg_server_step",
     "cg_server_next_result", "cg_server_info", "cg_server_perturb_rhs",
     "cg_refine_params_default", "cg_solve_refined", "cg_get_solution_f64",
-    "cg_velocity_divergence", "cg_add_gradient", "cg_project", "cg_set_schedule", "cg_autotune", "cg_autotune_refined",
+    "cg_velocity_divergence", "cg_add_gradient", "cg_project", "cg_set_schedule", "cg_set_level_forms", "cg_autotune", "cg_autotune_refined",
     "cg_tiles_save", "cg_tiles_load", "cg_tiles_import_holder", "cg_tileset_tree", "cg_tileset_channels",
     "cg_tileset_destroy", "cg_amg_prepare", "cg_amg_solve", "cg_amg_coefficients",
 )
@@ -143,6 +144,8 @@ def lib():
             "cg_problem_topology": (i32, [vp, vp, vp]),
             "cg_set_channel": (i32, [vp, i32, vp]),
             "cg_get_channel": (i32, [vp, i32, vp]),
+            "cg_set_channel_device": (i32, [vp, i32, vp, vp]),
+            "cg_get_channel_device": (i32, [vp, i32, vp, vp]),
             "cg_solve": (i32, [vp, P(Params), P(Result)]),
             "cg_damped_jacobi_solve": (i32, [vp, i32, f64, f32, P(Result)]),
             "cg_vcycle": (i32, [vp, i32, i32, i32, i32, i32, f32]),
@@ -170,6 +173,7 @@ def lib():
             "cg_add_gradient": (i32, [vp, i32]),
             "cg_project": (i32, [vp, P(Params), P(Result)]),
             "cg_set_schedule": (i32, [vp, i32, i32, i32]),
+            "cg_set_level_forms": (i32, [vp, i32, i32]),
             "cg_autotune": (i32, [vp, P(Params), i32, P(Params), P(TuneReport)]),
             "cg_autotune_refined": (i32, [vp, P(Params), P(RefineParams), i32, P(Params), P(TuneReport)]),
             "cg_solve_refined": (i32, [vp, P(Params), P(RefineParams), P(RefineResult)]),
@@ -417,12 +421,30 @@ class Problem:
         _check(lib().cg_get_channel(self._h, ch, _ptr(a)), "cg_get_channel")
         return a
 
+    def set_device(self, ch, device_ptr, stream=None):
+        """Channel from a device buffer (512 float32 per tile, this handle's
+        tile order; e.g. a torch CUDA tensor's data_ptr()), ordered on the
+        caller's stream (a hipStream_t as an int, None: the null stream)."""
+        _check(lib().cg_set_channel_device(self._h, ch, ctypes.c_void_p(int(device_ptr)),
+                                           ctypes.c_void_p(int(stream) if stream else None)), "cg_set_channel_device")
+
+    def get_device(self, ch, device_ptr, stream=None):
+        """Channel into a device buffer (see :meth:`set_device`)."""
+        _check(lib().cg_get_channel_device(self._h, ch, ctypes.c_void_p(int(device_ptr)),
+                                           ctypes.c_void_p(int(stream) if stream else None)), "cg_get_channel_device")
+
     def set_schedule(self, coarse_shift=0, swizzle=True, tile_block=1):
         """Kernel coverage only, never a result: LDS-resident coarse levels given
         back to grid-wide launches, XCD-aware tile order, tiles per wavefront of
         the colour sweeps (cg_set_schedule)."""
         _check(lib().cg_set_schedule(self._h, int(coarse_shift), int(bool(swizzle)), int(tile_block)),
                "cg_set_schedule")
+
+    def set_level_forms(self, eager_max_tiles=-1, fused16_max_tiles=-1):
+        """Level-size limits of the eager prolongation/restriction forms and of
+        the fused first 16-bit sweep (cg_set_level_forms; results unchanged;
+        0 never, negative the default)."""
+        _check(lib().cg_set_level_forms(self._h, int(eager_max_tiles), int(fused16_max_tiles)), "cg_set_level_forms")
 
     def get_f64(self):
         """The FP64 x of the last refined solve, [tiles, 512] float64."""

@@ -1826,6 +1826,20 @@ __global__ __launch_bounds__(64) void k_permute_ctype(const uint8_t* __restrict_
   uint64_t* d8 = reinterpret_cast<uint64_t*>(dst + size_t(i) * 512);
   d8[threadIdx.x] = s8[threadIdx.x];
 }
+// A float channel between the caller's tile order and the canonical order:
+// to_canonical: dst tile i = src tile ext[i]; otherwise dst tile ext[i] = src
+// tile i.  One wavefront per tile, 32 bytes per lane.
+__global__ __launch_bounds__(64) void k_permute_channel(const float* __restrict__ src, const int32_t* __restrict__ ext,
+                                                        int n, float* __restrict__ dst, int to_canonical) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const size_t e = size_t(ext[i]) * 512, c = size_t(i) * 512;
+  const float4* s4 = reinterpret_cast<const float4*>(src + (to_canonical ? e : c)) + 2 * threadIdx.x;
+  float4* d4 = reinterpret_cast<float4*>(dst + (to_canonical ? c : e)) + 2 * threadIdx.x;
+  const float4 a = s4[0], b = s4[1];
+  d4[0] = a;
+  d4[1] = b;
+}
 
 // Face masks of every cell of the listed tiles (Ctx::fmask): bit e of cell c is
 // set when face e (x-, x+, y-, y+, z-, z+) is not cut, i.e. neither c nor its

@@ -93,6 +93,8 @@ struct cg_problem {
   int coarse_shift = 0;
   bool rz_in_vcycle = false;     // every LEAF tile is on a grid-wide level >= 1 (its up-sweep can sum r.z)
   int gs_block = 1;
+  int eager_max = 16384;    // levels of at most this many tiles: every load before one wait (prolongation pass, restriction)
+  int fused16_max = 16384;  // 16-bit levels of at most this many tiles: first down-sweep iteration in one launch
   std::vector<int> lev, tbase, coarse_auto;
   std::vector<uint8_t> ty;
 
@@ -126,10 +128,12 @@ struct cg_problem {
   TileRec* d_tiles = nullptr;    // every tile's record in canonical order (the worklists are gathered from it)
   TileRec* d_tmp_list = nullptr;
   int tmp_list_cap = 0;
+  int32_t* d_ext = nullptr;      // internal tile -> caller index (channel moves from and to device buffers)
   CoarseSlot* d_coarse = nullptr;
   hipStream_t stream = nullptr;
   int* h_pin = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t ev_io[2] = {nullptr, nullptr};  // orders device channel moves against the caller's stream
   hipGraphExec_t graph = nullptr;
   std::string graph_key;
 
@@ -168,9 +172,10 @@ cg_problem::~cg_problem() {
   if (!owner) return;
   if (graph) (void)hipGraphExecDestroy(graph);
   for (auto e : ev) if (e) (void)hipEventDestroy(e);
+  for (auto e : ev_io) if (e) (void)hipEventDestroy(e);
   if (h_pin) (void)hipHostFree(h_pin);
   void* ptrs[] = {d_ctype, d_fmask, d_diag, d_zb, d_zh, d_rhsvh, d_zbh, d_x64, d_r64, d_bsave, d_psum, d_gsum, d_ps, d_dflag,
-                  d_plans, d_part, d_part2, d_scalar, d_lists, d_tiles, d_tmp_list, d_coarse, d_amgc, d_amgct};
+                  d_plans, d_part, d_part2, d_scalar, d_lists, d_tiles, d_tmp_list, d_coarse, d_amgc, d_amgct, d_ext};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
   for (auto q : d_api) if (q) (void)hipFree(q);
@@ -370,11 +375,12 @@ void coarse_plan(const std::vector<TileRec>& tiles, int base, int C, int prob, C
   }
 }
 
-// Worklists and coarse plans of a pool for its schedule: each problem keeps
-// its deepest LDS-resident coarse level coarse_auto - coarse_shift (-1: none),
-// the levels above it go to grid-wide launches.  (Re)uploads the lists and
-// plans; the captured iteration is dropped.
-void build_schedule(cg_problem& P) {
+// Worklists and coarse plans of a pool for its schedule (host side only):
+// each problem keeps its deepest LDS-resident coarse level coarse_auto -
+// coarse_shift (-1: none), the levels above it go to grid-wide launches.  The
+// lists' tile indices come back in `ids` (every list's DList indexes it); the
+// coarse plans hold slot offsets until upload_schedule patches them.
+void plan_schedule(cg_problem& P, std::vector<int32_t>& ids) {
   const int n = P.n, nprob = P.nprob;
   const std::vector<int>& lev = P.lev;
   const std::vector<uint8_t>& ty = P.ty;
@@ -413,7 +419,7 @@ void build_schedule(cg_problem& P) {
     add_list(&P.lvlg[l], bucket[l], [&](int i) { return grid_level(i); });
   }
   // the index lists go to the device, which gathers the records from d_tiles
-  std::vector<int32_t> ids;
+  ids.clear();
   size_t total = 0;
   for (auto& s : sel) total += s.second.size();
   ids.reserve(total);
@@ -435,7 +441,25 @@ void build_schedule(cg_problem& P) {
     cp.slots = reinterpret_cast<const CoarseSlot*>(size_t(first));  // offset, patched after upload
     P.plans.push_back(cp);
   }
+  for (const CoarsePlan& cp : P.plans) {  // every slot link stays inside its own plan (the kernel's LDS)
+    const size_t first = size_t(cp.slots);
+    if (cp.nslots > CL_MAX_TILES || first + size_t(cp.nslots) > P.coarse_slots.size())
+      fail(CG_ERR_INVALID, "coarse plan: too many slots");
+    for (int k = 0; k < cp.nslots; k++) {
+      const CoarseSlot& cs = P.coarse_slots[first + k];
+      for (int f = 0; f < 6; f++)
+        if (cs.nbr[f] >= cp.nslots) fail(CG_ERR_INVALID, "coarse plan: neighbour slot out of range");
+      if (cs.parent >= cp.nslots || cs.tile < 0 || cs.tile >= P.n) fail(CG_ERR_INVALID, "coarse plan: slot out of range");
+    }
+  }
+  for (int32_t t : ids)
+    if (t < 0 || t >= P.n) fail(CG_ERR_INVALID, "worklist: tile index out of range");
+}
 
+// The planned lists and coarse plans to the device (records gathered there
+// from d_tiles); the captured iteration is dropped.
+void upload_schedule(cg_problem& P, const std::vector<int32_t>& ids) {
+  const size_t total = ids.size();
   HIPC(hipStreamSynchronize(P.stream));  // no kernel may still read the old lists
   if (P.d_lists) HIPC(hipFree(P.d_lists));
   if (P.d_coarse) HIPC(hipFree(P.d_coarse));
@@ -465,6 +489,12 @@ void build_schedule(cg_problem& P) {
   if (P.graph) HIPC(hipGraphExecDestroy(P.graph));
   P.graph = nullptr;
   P.graph_key.clear();
+}
+
+void build_schedule(cg_problem& P) {
+  std::vector<int32_t> ids;
+  plan_schedule(P, ids);
+  upload_schedule(P, ids);
 }
 
 void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
@@ -544,6 +574,9 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   P.ty = ty;
   P.tbase = tbase;
   P.coarse_auto = P.coarse_C;
+  // every host-side plan is made before the first device call
+  std::vector<int32_t> sched_ids;
+  plan_schedule(P, sched_ids);
 
   // device buffers
   HIPC(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
@@ -570,25 +603,25 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   HIPC(hipMalloc(&P.d_scalar, 8 * sizeof(double)));
   HIPC(hipMalloc(&P.d_tiles, sizeof(TileRec) * size_t(n)));
   HIPC(hipMemcpyAsync(P.d_tiles, P.tiles.data(), sizeof(TileRec) * size_t(n), hipMemcpyHostToDevice, P.stream));
-  build_schedule(P);
+  upload_schedule(P, sched_ids);
   // cell types: each tree's array goes up as given (into d_fmask, free until the
   // face masks are formed) and is permuted to the canonical order on the device
   for (int q = 0; q < nprob; q++)
     HIPC(hipMemcpyAsync(P.d_fmask + size_t(tbase[q]) * 512, descs[q].cell_type, size_t(descs[q].num_tiles) * 512,
                         hipMemcpyHostToDevice, P.stream));
-  {
-    DeviceBuffer d_ext(sizeof(int32_t) * size_t(n));
-    HIPC(hipMemcpyAsync(d_ext.p, P.ext_of.data(), sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice, P.stream));
-    k_permute_ctype<<<unsigned(n), 64, 0, P.stream>>>(P.d_fmask, static_cast<const int32_t*>(d_ext.p), n, P.d_ctype);
-    HIPC(hipGetLastError());
-    HIPC(hipStreamSynchronize(P.stream));
-  }
+  HIPC(hipMalloc(&P.d_ext, sizeof(int32_t) * size_t(n)));
+  HIPC(hipMemcpyAsync(P.d_ext, P.ext_of.data(), sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice, P.stream));
+  k_permute_ctype<<<unsigned(n), 64, 0, P.stream>>>(P.d_fmask, P.d_ext, n, P.d_ctype);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(P.stream));
   k_face_mask<<<P.all.n, 64, 0, P.stream>>>(P.d_ctype, P.d_fmask, P.L(P.all), P.all.n);
   k_diag<<<P.all.n, 64, 0, P.stream>>>(P.ctx(), P.L(P.all), P.all.n, P.d_diag);
   HIPC(hipGetLastError());
   HIPC(hipHostMalloc(&P.h_pin, 2 * sizeof(int) * nprob, hipHostMallocDefault));
   HIPC(hipEventCreateWithFlags(&P.ev[0], hipEventDisableTiming));
   HIPC(hipEventCreateWithFlags(&P.ev[1], hipEventDisableTiming));
+  HIPC(hipEventCreateWithFlags(&P.ev_io[0], hipEventDisableTiming));
+  HIPC(hipEventCreateWithFlags(&P.ev_io[1], hipEventDisableTiming));
   HIPC(hipStreamSynchronize(P.stream));
 }
 
@@ -609,6 +642,45 @@ float* any_channel(cg_problem& P, int ch) {
   return v;
 }
 
+// The caller's device buffer of one channel (512 floats per tile, caller's
+// tile order) must be device-accessible memory holding the whole channel.
+void check_device_buffer(const void* ptr, size_t bytes, const char* what) {
+  if (!ptr) fail(CG_ERR_INVALID, std::string(what) + ": null device pointer");
+  hipPointerAttribute_t a;
+  const hipError_t e = hipPointerGetAttributes(&a, ptr);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // the query's error is the answer, not a pending failure
+    fail(CG_ERR_INVALID, std::string(what) + ": the pointer is not memory known to HIP");
+  }
+  if (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged && a.type != hipMemoryTypeHost)
+    fail(CG_ERR_INVALID, std::string(what) + ": the pointer is not device-accessible memory");
+  void* base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, const_cast<void*>(ptr)) == hipSuccess && base) {
+    const size_t off = size_t(static_cast<const char*>(ptr) - static_cast<const char*>(base));
+    if (off > size || size - off < bytes)
+      fail(CG_ERR_INVALID, std::string(what) + ": the buffer is smaller than 512 floats per tile");
+  } else {
+    (void)hipGetLastError();
+  }
+}
+
+// A channel moved between the caller's device buffer and the handle, on the
+// handle's stream, ordered after the work already enqueued on the caller's
+// stream and before the work the caller enqueues there next; no host wait.
+void channel_device(cg_problem& P, int ch, const float* src, float* dst, hipStream_t cs) {
+  const bool set = src != nullptr;
+  float* d = any_channel(P, ch);
+  check_device_buffer(set ? static_cast<const void*>(src) : static_cast<const void*>(dst),
+                      size_t(P.n) * 512 * sizeof(float), set ? "cg_set_channel_device" : "cg_get_channel_device");
+  HIPC(hipEventRecord(P.ev_io[0], cs));
+  HIPC(hipStreamWaitEvent(P.stream, P.ev_io[0], 0));
+  k_permute_channel<<<unsigned(P.n), 64, 0, P.stream>>>(set ? src : d, P.d_ext, P.n, set ? d : dst, set ? 1 : 0);
+  HIPC(hipGetLastError());
+  HIPC(hipEventRecord(P.ev_io[1], P.stream));
+  HIPC(hipStreamWaitEvent(cs, P.ev_io[1], 0));
+}
+
 // upload every problem's ProbState (synchronous)
 // tol_q / maxit_q: per-problem tolerance and iteration cap (null: tol / max_iters for all)
 void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann, int check_stride = 1,
@@ -626,9 +698,9 @@ void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann, 
     s.pure_neumann = pure_neumann;
     s.check_stride = check_stride;
   }
-  HIPC(hipStreamSynchronize(P.stream));
-  HIPC(hipMemcpy(P.d_ps, st.data(), sizeof(ProbState) * P.nprob, hipMemcpyHostToDevice));
-  HIPC(hipMemset(P.d_dflag, 0, sizeof(int32_t) * P.nprob));
+  HIPC(hipMemcpyAsync(P.d_ps, st.data(), sizeof(ProbState) * P.nprob, hipMemcpyHostToDevice, P.stream));
+  HIPC(hipMemsetAsync(P.d_dflag, 0, sizeof(int32_t) * P.nprob, P.stream));
+  HIPC(hipStreamSynchronize(P.stream));  // st is a local buffer
 }
 
 // ---------------------------------------------------------------------------
@@ -711,6 +783,7 @@ constexpr int kFirstFused16Max = 16384;
 // k_restrict (all loads before one wait: faster where a launch waits on
 // latency, slower on big levels).
 constexpr int kProlongEagerMax = 16384;
+// Both limits are per handle (cg_set_level_forms), defaulting to these.
 
 // last_flags is added to the last colour pass of each grid-wide level's
 // up-sweep (RB-GS with nu >= 1), the pass after which that level's z is final.
@@ -724,7 +797,7 @@ void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v, in
     const DList& a = P.actg[l];
     if (v.smoother == CG_SMOOTH_RBGS) {
       for (int s = 0; s < v.nu; s++) {
-        if (s == 0 && P.d_diag && (std::is_same<V, float>::value || a.n <= kFirstFused16Max)) {
+        if (s == 0 && P.d_diag && (std::is_same<V, float>::value || a.n <= P.fused16_max)) {
           // both colours of the first iteration from the zero guess in one launch
           if (a.n) k_gs_first<V><<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, Z, base | F_GHOST_ZERO);
           continue;
@@ -743,7 +816,7 @@ void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v, in
     }
     const DList& al = P.lvlg[l];
     int fl = base | F_GHOST_ZERO | (v.nu == 0 ? F_ZERO_X : 0);
-    if (al.n > kProlongEagerMax)
+    if (al.n > P.eager_max)
       k_restrict<V><<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, Z, R, nullptr, v.ooa, fl);
     else if (al.n)
       k_restrict<V, true><<<grid_for(al.n, fl), 64, 0, P.stream>>>(c, P.L(al), al.n, l, Z, R, nullptr, v.ooa, fl);
@@ -772,7 +845,7 @@ void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v, in
     const bool jac = v.smoother != CG_SMOOTH_RBGS;
     if (!jac && v.nu > 0) {
       // prolongation fused into the first (black) pass; the level's result goes to ZO
-      if (al.n > kProlongEagerMax)
+      if (al.n > P.eager_max)
         k_gs_prolong<V><<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, Z, ZO, v.pcoef, 1, base);
       else if (al.n)
         k_gs_prolong_eager<V><<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, Z, ZO, v.pcoef, 1, base);
@@ -1454,6 +1527,22 @@ cg_status cg_get_channel(cg_problem* p, int32_t channel, float* host) {
   });
 }
 
+cg_status cg_set_channel_device(cg_problem* p, int32_t channel, const float* device_src, void* stream) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "cg_set_channel_device: null problem");
+    if (!device_src) fail(CG_ERR_INVALID, "cg_set_channel_device: null device pointer");
+    channel_device(*p, channel, device_src, nullptr, static_cast<hipStream_t>(stream));
+  });
+}
+
+cg_status cg_get_channel_device(cg_problem* p, int32_t channel, float* device_dst, void* stream) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "cg_get_channel_device: null problem");
+    if (!device_dst) fail(CG_ERR_INVALID, "cg_get_channel_device: null device pointer");
+    channel_device(*p, channel, nullptr, device_dst, static_cast<hipStream_t>(stream));
+  });
+}
+
 cg_status cg_amg_prepare(cg_problem* p, float one_over_alpha) {
   return guard([&] {
     if (!p) fail(CG_ERR_INVALID, "null problem");
@@ -1492,6 +1581,19 @@ cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle, 
   return guard([&] {
     if (!p) fail(CG_ERR_INVALID, "null problem");
     set_schedule(*p, coarse_shift, swizzle, tile_block);
+  });
+}
+
+cg_status cg_set_level_forms(cg_problem* p, int32_t eager_max_tiles, int32_t fused16_max_tiles) {
+  return guard([&] {
+    if (!p) fail(CG_ERR_INVALID, "null problem");
+    if (!p->owner) fail(CG_ERR_INVALID, "cg_set_level_forms: not an owning handle");
+    p->eager_max = eager_max_tiles < 0 ? kProlongEagerMax : eager_max_tiles;
+    p->fused16_max = fused16_max_tiles < 0 ? kFirstFused16Max : fused16_max_tiles;
+    HIPC(hipStreamSynchronize(p->stream));
+    if (p->graph) HIPC(hipGraphExecDestroy(p->graph));
+    p->graph = nullptr;
+    p->graph_key.clear();
   });
 }
 

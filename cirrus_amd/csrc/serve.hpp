@@ -22,43 +22,9 @@
 #include <deque>
 #include <map>
 
-namespace {
+#include "range_alloc.hpp"
 
-// First-fit allocator of index ranges [0, cap).
-struct RangeAlloc {
-  std::map<int64_t, int64_t> free_;  // start -> length
-  void init(int64_t cap) {
-    free_.clear();
-    if (cap > 0) free_[0] = cap;
-  }
-  int64_t alloc(int64_t n) {
-    if (n <= 0) return 0;
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second < n) continue;
-      const int64_t at = it->first, len = it->second;
-      free_.erase(it);
-      if (len > n) free_[at + n] = len - n;
-      return at;
-    }
-    return -1;
-  }
-  void release(int64_t at, int64_t n) {
-    if (n <= 0) return;
-    auto it = free_.emplace(at, n).first;
-    auto nx = std::next(it);
-    if (nx != free_.end() && it->first + it->second == nx->first) {
-      it->second += nx->second;
-      free_.erase(nx);
-    }
-    if (it != free_.begin()) {
-      auto pv = std::prev(it);
-      if (pv->first + pv->second == it->first) {
-        pv->second += it->second;
-        free_.erase(it);
-      }
-    }
-  }
-};
+namespace {
 
 enum InstState { INST_IDLE = 0, INST_QUEUED = 1, INST_ACTIVE = 2, INST_DONE = 3, INST_VERIFY = 4 };
 constexpr int kMaxRefineSteps = 5;
@@ -217,12 +183,12 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   for (int c = 0; c < CG_NUM_CHANNELS; c++) HIPC(hipMalloc(&S.d_ch[c], cells * sizeof(float)));
   HIPC(hipMalloc(&S.d_zb, cells * sizeof(float)));
   HIPC(hipMalloc(&S.d_b, cells * sizeof(float)));
-  HIPC(hipMemset(S.d_b, 0, cells * sizeof(float)));
+  HIPC(hipMemsetAsync(S.d_b, 0, cells * sizeof(float), S.stream));
   if (f64) {
     HIPC(hipMalloc(&S.d_x64, cells * sizeof(double)));
     HIPC(hipMalloc(&S.d_r64, cells * sizeof(double)));
-    HIPC(hipMemset(S.d_x64, 0, cells * sizeof(double)));
-    HIPC(hipMemset(S.d_r64, 0, cells * sizeof(double)));
+    HIPC(hipMemsetAsync(S.d_x64, 0, cells * sizeof(double), S.stream));
+    HIPC(hipMemsetAsync(S.d_r64, 0, cells * sizeof(double), S.stream));
     HIPC(hipMalloc(&S.d_ver, 2 * sizeof(double) * S.max_probs));
     HIPC(hipHostMalloc(&S.h_ver, 2 * sizeof(double) * S.max_probs, hipHostMallocDefault));
   }
@@ -230,12 +196,12 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
     bf16** bufs[3] = {&S.d_zh, &S.d_rhsvh, &S.d_zbh};
     for (bf16** b : bufs) {
       HIPC(hipMalloc(b, cells * sizeof(bf16)));
-      HIPC(hipMemset(*b, 0, cells * sizeof(bf16)));
+      HIPC(hipMemsetAsync(*b, 0, cells * sizeof(bf16), S.stream));
     }
   }
   HIPC(hipMalloc(&S.d_gsum, size_t(S.cap_ghost) * 64 * sizeof(float)));
   HIPC(hipMalloc(&S.d_ps, sizeof(ProbState) * S.max_probs));
-  HIPC(hipMemset(S.d_ps, 0, sizeof(ProbState) * S.max_probs));
+  HIPC(hipMemsetAsync(S.d_ps, 0, sizeof(ProbState) * S.max_probs, S.stream));
   HIPC(hipMalloc(&S.d_dflag, sizeof(int32_t) * S.max_probs));
   HIPC(hipMalloc(&S.d_dflag_init, sizeof(int32_t) * S.max_probs));
   HIPC(hipMalloc(&S.d_pmap, sizeof(int32_t) * S.max_probs));
@@ -279,6 +245,7 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   P.d_lists = S.d_asm;
   P.d_plans = S.d_plans;
   P.nprob = 0;
+  HIPC(hipStreamSynchronize(S.stream));  // the zero fills above are done before any request
 }
 
 void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
@@ -363,9 +330,9 @@ void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
   k_face_mask<<<I->all.n, 64, 0, S.stream>>>(S.d_ctype, S.d_fmask, I->L(I->all), I->all.n);
   k_diag<<<I->all.n, 64, 0, S.stream>>>(S.view.ctx(), I->L(I->all), I->all.n, S.d_diag);
   HIPC(hipGetLastError());
-  HIPC(hipMemset(S.d_ch[CG_CH_X] + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float)));
-  HIPC(hipMemset(S.d_b + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float)));
-  if (S.d_x64) HIPC(hipMemset(S.d_x64 + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(double)));
+  HIPC(hipMemsetAsync(S.d_ch[CG_CH_X] + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float), S.stream));
+  HIPC(hipMemsetAsync(S.d_b + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float), S.stream));
+  if (S.d_x64) HIPC(hipMemsetAsync(S.d_x64 + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(double), S.stream));
   HIPC(hipHostMalloc(&I->h_buf, size_t(n) * 512 * sizeof(float), hipHostMallocDefault));
   HIPC(hipEventCreateWithFlags(&I->ev, hipEventDisableTiming));
   HIPC(hipEventCreateWithFlags(&I->ev_ver, hipEventDisableTiming));
@@ -386,9 +353,9 @@ void server_release(cg_server& S, int32_t id) {
   if (I.state == INST_DONE)
     for (auto it = S.done_q.begin(); it != S.done_q.end(); ++it)
       if (*it == id) { S.done_q.erase(it); break; }
-  S.tiles_alloc.release(I.tile_begin, I.ntiles);
-  S.leaf_alloc.release(I.leaf_begin, I.nleaf);
-  S.ghost_alloc.release(I.ghost_begin, I.nghost);
+  if (!S.tiles_alloc.release(I.tile_begin, I.ntiles) || !S.leaf_alloc.release(I.leaf_begin, I.nleaf) ||
+      !S.ghost_alloc.release(I.ghost_begin, I.nghost))
+    fail(CG_ERR_STATE, "cg_server_release: the instance's ranges were already free");
   HIPC(hipFree(I.d_recs));
   I.d_recs = nullptr;
   if (I.d_slots) HIPC(hipFree(I.d_slots));

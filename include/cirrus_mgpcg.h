@@ -135,6 +135,17 @@ cg_status cg_problem_topology(const cg_problem* p, int32_t* nbr, int32_t* parent
  * channels 0-5 and the face velocities 6-8. */
 cg_status cg_set_channel(cg_problem* p, int32_t channel, const float* host);
 cg_status cg_get_channel(cg_problem* p, int32_t channel, float* host);
+/* The same moves between device buffers and the handle, replacing the
+ * reference's in-place channel access on its device tiles (GMGSolver::solve
+ * reads b and writes x on HADeviceGrid, src/GMGSolver.cu:531-666, which
+ * FluidEuler::project fills and reads on the device, src/FluidEuler.h:581-612).
+ * The buffer is device-accessible memory of 512 floats per tile in the
+ * caller's tile order (checked: memory known to HIP, large enough).  `stream`
+ * is the caller's hipStream_t (NULL: the null stream): the move waits for the
+ * work already enqueued there, and work the caller enqueues there afterwards
+ * sees its result; the call itself does not wait on the host. */
+cg_status cg_set_channel_device(cg_problem* p, int32_t channel, const float* device_src, void* stream);
+cg_status cg_get_channel_device(cg_problem* p, int32_t channel, float* device_dst, void* stream);
 
 /* GMGSolver::solve (src/GMGSolver.cu:531-666): solves A x = b with b in
  * CG_CH_B, pressure returned in CG_CH_X on LEAF tiles.  Channels 1-5 are
@@ -183,6 +194,14 @@ cg_status cg_get_solution_f64(cg_problem* p, double* host);
  * workgroup, 2 two tiles per wavefront (the loads of both in flight together),
  * 4 four wavefronts of one tile each per 256-thread workgroup. */
 cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle, int32_t tile_block);
+/* Two more kernel forms chosen by level size, also without effect on results:
+ * levels (all problems of the handle together) of at most eager_max_tiles
+ * tiles run the prolongation-fused colour pass and the restriction with every
+ * load issued before one wait (shorter latency chain; more registers); with
+ * 16-bit storage, levels of at most fused16_max_tiles tiles run the first
+ * down-sweep iteration as one launch (it reads the halo's diagonals).  0: never,
+ * a negative value: the default 16384.  No reference counterpart. */
+cg_status cg_set_level_forms(cg_problem* p, int32_t eager_max_tiles, int32_t fused16_max_tiles);
 typedef struct {
   int32_t candidates;        /* parameter sets timed */
   double base_seconds;       /* one solve with the caller's parameters, default schedule */
