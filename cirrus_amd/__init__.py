@@ -28,6 +28,7 @@ CH_X, CH_B, CH_R, CH_P, CH_AP, CH_Z, CH_TMP = 0, 1, 1, 2, 3, 4, 5
 CH_U, CH_V, CH_W = 6, 7, 8
 NUM_CHANNELS = 6
 SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
+NOALLOC_MAX_TILES = 262144  # CG_NOALLOC_MAX_TILES
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 SCENE_UNIFORM, SCENE_SPHERE_BAND, SCENE_PLANE, SCENE_VORTEX = 0, 1, 2, 3
 (KERNEL_GS_SWEEP, KERNEL_APPLY, KERNEL_RESTRICT, KERNEL_PROLONG, KERNEL_JACOBI, KERNEL_UPDATE_XR, KERNEL_COARSE,
@@ -40,7 +41,7 @@ EXPORTED_SYMBOLS = (
     "cg_problem_topology", "cg_set_channel", "cg_get_channel", "cg_set_channel_device", "cg_get_channel_device",
     "cg_solve", "cg_damped_jacobi_solve",
     "cg_vcycle", "cg_full_negative_laplacian", "cg_negative_laplacian_same_level", "cg_gauss_seidel_color",
-    "cg_jacobi_sweep", "cg_residual_restrict", "cg_prolongate", "cg_dot", "cg_recenter_leaf_voxels",
+    "cg_jacobi_sweep", "cg_residual_restrict", "cg_prolongate", "cg_dot", "cg_dot_noalloc", "cg_recenter_leaf_voxels",
     "cg_calc_cell_types_from_leafs", "cg_benchmark_kernel", "cg_scene_create", "cg_scene_destroy", "cg_scene_tree", "cg_scene_velocity",
     "cg_server_create", "cg_server_destroy", "cg_server_admit", "cg_server_release", "cg_server_submit", "cg_server_step",
     "cg_server_next_result", "cg_server_info", "cg_server_perturb_rhs",
@@ -156,6 +157,7 @@ def lib():
             "cg_residual_restrict": (i32, [vp, i32, i32, i32, i32, f32]),
             "cg_prolongate": (i32, [vp, i32, i32, i32, f32]),
             "cg_dot": (i32, [vp, i32, i32, i32, P(f64)]),
+            "cg_dot_noalloc": (i32, [vp, i32, i32, i32, P(f64)]),
             "cg_recenter_leaf_voxels": (i32, [vp, i32]),
             "cg_calc_cell_types_from_leafs": (i32, [P(TreeDesc), vp]),
             "cg_benchmark_kernel": (i32, [vp, i32, i32, i32, P(f64), P(ctypes.c_int64)]),
@@ -464,6 +466,13 @@ class Problem:
     def dot(self, a, b, types=LEAF):
         out = ctypes.c_double()
         _check(lib().cg_dot(self._h, a, b, types, ctypes.byref(out)), "cg_dot")
+        return out.value
+
+    def dot_noalloc(self, a, b, types=LEAF):
+        """dot() through the entry point that allocates nothing (at most
+        NOALLOC_MAX_TILES tiles; calls overlapping on one GPU race)."""
+        out = ctypes.c_double()
+        _check(lib().cg_dot_noalloc(self._h, a, b, types, ctypes.byref(out)), "cg_dot_noalloc")
         return out.value
 
 

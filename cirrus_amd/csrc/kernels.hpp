@@ -2267,18 +2267,57 @@ __global__ __launch_bounds__(64) void k_accumulate_ghosts(Ctx c, const TileRec* 
   AX[pcell] = AX[pcell] + s * 1.0f;
 }
 // Per-tile dot partials over all cells of the listed tiles, then one total.
-__global__ __launch_bounds__(64) void k_dot_tiles(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ A,
-                                                  const float* __restrict__ Bv, double* __restrict__ part) {
-  const int flags = 0;
-  TILE_KERNEL_PROLOGUE
-  const size_t base = size_t(T.self) * 512 + lane_id() * 8;
+// FP64 sum of a[c] * b[c] over the 512 cells of one tile: each lane over its
+// pencil in cell order, then the xor butterfly across the wavefront.
+__device__ __forceinline__ double tile_dot(const float* __restrict__ A, const float* __restrict__ Bv, int tile) {
+  const size_t base = size_t(tile) * 512 + lane_id() * 8;
   float a[8], b[8];
   load8(A + base, a);
   load8(Bv + base, b);
   double s = 0.0;
   for (int k = 0; k < 8; k++) s += double(a[k]) * double(b[k]);
-  s = wave_sum(s);
+  return wave_sum(s);
+}
+__global__ __launch_bounds__(64) void k_dot_tiles(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ A,
+                                                  const float* __restrict__ Bv, double* __restrict__ part) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  const double s = tile_dot(A, Bv, T.self);
   if (lane_id() == 0) part[w] = s;
+}
+// The indices i < n (ascending) of the tiles whose type is in `types`, written
+// to sel[0 ..); one workgroup of SEL_THREADS walks the records in chunks, each
+// chunk's ranks from wavefront ballots and a scan of the wavefront counts.
+constexpr int SEL_THREADS = 1024;
+__global__ __launch_bounds__(SEL_THREADS) void k_select_tiles(const TileRec* __restrict__ tiles, int n, int types,
+                                                              int32_t* __restrict__ sel) {
+  __shared__ int wcount[SEL_THREADS / 64];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  int base = 0;
+  for (int c0 = 0; c0 < n; c0 += SEL_THREADS) {
+    const int i = c0 + tid;
+    const bool take = i < n && (tr_type(tiles[i]) & types) != 0;
+    const uint64_t bal = __ballot(take);
+    const int below = __popcll(bal & ((uint64_t(1) << lane) - 1));
+    if (lane == 0) wcount[wave] = __popcll(bal);
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int v = 0; v < SEL_THREADS / 64; v++) {
+      before += v < wave ? wcount[v] : 0;
+      total += wcount[v];
+    }
+    if (take) sel[base + before + below] = i;
+    base += total;
+    __syncthreads();  // wcount is rewritten by the next chunk
+  }
+}
+// Tile partials of the selected tiles: partial q of tile sel[q] (q < count).
+__global__ __launch_bounds__(64) void k_dot_selected(const int32_t* __restrict__ sel, int count, const float* __restrict__ A,
+                                                     const float* __restrict__ Bv, double* __restrict__ part) {
+  const int q = blockIdx.x;
+  if (q >= count) return;
+  const double s = tile_dot(A, Bv, sel[q]);
+  if (lane_id() == 0) part[q] = s;
 }
 __global__ __launch_bounds__(FIN_THREADS) void k_sum_partials(const double* __restrict__ part, int n, double* __restrict__ out) {
   __shared__ double lds[FIN_CHUNK];

@@ -93,6 +93,7 @@ struct cg_problem {
   int coarse_shift = 0;
   bool rz_in_vcycle = false;     // every LEAF tile is on a grid-wide level >= 1 (its up-sweep can sum r.z)
   int gs_block = 1;
+  int device = 0;           // the HIP device current when the handle was created
   int eager_max = 16384;    // levels of at most this many tiles: every load before one wait (prolongation pass, restriction)
   int fused16_max = 16384;  // 16-bit levels of at most this many tiles: first down-sweep iteration in one launch
   std::vector<int> lev, tbase, coarse_auto;
@@ -579,6 +580,7 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   plan_schedule(P, sched_ids);
 
   // device buffers
+  HIPC(hipGetDevice(&P.device));
   HIPC(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
   const size_t cells = size_t(n) * 512;
   HIPC(hipMalloc(&P.d_ctype, cells));
@@ -1399,6 +1401,46 @@ void add_gradient(cg_problem& P, float* Pch) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Device scratch of cg_dot_noalloc: the selected tile indices, their FP64
+// partials and the result live in one file-scope device array, so that entry
+// point calls no allocator.  Every device holds its own copy until the process
+// ends; host code reaches a device's copy only through the address
+// hipGetSymbolAddress returns for it (the array's name in host code is a
+// host-side stand-in, not a device address).
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxTiles = CG_NOALLOC_MAX_TILES;
+constexpr size_t kScrPart = 0;                                                  // double[kNoAllocMaxTiles]
+constexpr size_t kScrSel = kScrPart + sizeof(double) * kNoAllocMaxTiles;        // int32[kNoAllocMaxTiles]
+constexpr size_t kScrOut = kScrSel + sizeof(int32_t) * kNoAllocMaxTiles;        // double
+constexpr size_t kScrBytes = kScrOut + 256;
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of `scratch` on device `dev`, looked up once per device with
+// that device current (the caller's current device is restored).
+unsigned char* scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) fail(CG_ERR_INVALID, "cg_dot_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    HIPC(hipGetDevice(&cur));
+    if (cur != dev) HIPC(hipSetDevice(dev));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPC(hipSetDevice(cur));
+    if (e != hipSuccess || !ptr)
+      fail(CG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
 #include "serve.hpp"
 
 // ---------------------------------------------------------------------------
@@ -1829,6 +1871,37 @@ cg_status cg_dot(cg_problem* p, int32_t a_channel, int32_t b_channel, int32_t ti
     check_channel(a_channel);
     check_channel(b_channel);
     *out = dot_types(*p, p->d_ch[a_channel], p->d_ch[b_channel], tile_types);
+  });
+}
+
+// cg_dot without any allocation: the tile list, partials and result in the
+// device's `scratch` (every slot read is written first by the same call, so
+// nothing needs clearing); the same kernels' arithmetic and summation order.
+cg_status cg_dot_noalloc(cg_problem* p, int32_t a_channel, int32_t b_channel, int32_t tile_types, double* out) {
+  return guard([&] {
+    if (!p || !out) fail(CG_ERR_INVALID, "null argument");
+    check_channel(a_channel);
+    check_channel(b_channel);
+    cg_problem& P = *p;
+    if (!P.owner || !P.d_tiles) fail(CG_ERR_INVALID, "cg_dot_noalloc: not an owning handle");
+    if (P.n > kNoAllocMaxTiles)
+      fail(CG_ERR_INVALID, "cg_dot_noalloc: the problem has more tiles than the scratch holds (CG_NOALLOC_MAX_TILES)");
+    int count = 0;
+    for (int i = 0; i < P.n; i++) count += (tr_type(P.tiles[i]) & tile_types) ? 1 : 0;
+    if (count == 0) {
+      *out = 0.0;
+      return;
+    }
+    unsigned char* s = scratch_on(P.device);
+    double* part = reinterpret_cast<double*>(s + kScrPart);
+    int32_t* sel = reinterpret_cast<int32_t*>(s + kScrSel);
+    double* res = reinterpret_cast<double*>(s + kScrOut);
+    k_select_tiles<<<1, SEL_THREADS, 0, P.stream>>>(P.d_tiles, P.n, tile_types, sel);
+    k_dot_selected<<<unsigned(count), 64, 0, P.stream>>>(sel, count, P.d_ch[a_channel], P.d_ch[b_channel], part);
+    k_sum_partials<<<1, FIN_THREADS, 0, P.stream>>>(part, count, res);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, P.stream));
+    HIPC(hipStreamSynchronize(P.stream));
   });
 }
 

@@ -296,6 +296,16 @@ cg_status cg_prolongate(cg_problem* p, int32_t level, int32_t coarse_channel, in
 /* Dot over all cells of the tiles of type in tile_types, FP64 accumulation in
  * a fixed order (Dot, src/PoissonGrid.cu:217-228). */
 cg_status cg_dot(cg_problem* p, int32_t a_channel, int32_t b_channel, int32_t tile_types, double* out);
+/* cg_dot with no allocation at all (no hipMalloc, hipFree or other allocating
+ * call): the same result bit for bit.  No counterpart in the reference; for
+ * callers that cannot allocate (tight serving loops, harnesses that forbid
+ * it).  Its tile list, partials and result live in a fixed device array of
+ * 3 MiB per GPU (held until the process ends), so the problem may have at most
+ * CG_NOALLOC_MAX_TILES tiles (CG_ERR_INVALID beyond).  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (from several threads, or
+ * several handles at once) race. */
+#define CG_NOALLOC_MAX_TILES 262144
+cg_status cg_dot_noalloc(cg_problem* p, int32_t a_channel, int32_t b_channel, int32_t tile_types, double* out);
 /* ReCenterLeafVoxels (src/PoissonSolver.cu:356-381). */
 cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel);
 
