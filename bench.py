@@ -73,119 +73,137 @@ def plain_drop_in(ca, items, solver_kw, reps=1):
             "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99))}
 
 
-def run_served(ca, solver_kw, duration=3.0, lam=2000.0, conc=256, nscenes=8, budget=200 << 30, rho=0.95, steps=50,
-               storage=0, refine=False):
-    """Serving form: requests arrive as a Poisson process of rate `lam` for
-    `duration` seconds; request j goes to resident instance j mod conc, whose
-    hierarchy and pressure stay on the device.  Each instance runs sequences of
-    `steps` correlated solves: its right-hand side is updated on the device as
-    b = rho b + sqrt(1 - rho^2) rms(b0) u (u seeded unit-variance noise) and the
-    solve is warm-started from its previous pressure; every steps-th request
-    starts a new sequence (cold).  Requests that find their instance busy wait;
-    latency is arrival -> result.  Instances are built like the batched form.
-    storage: V-cycle storage (0 FP32, 1 BF16); refine: every request is an FP64
-    iterative refinement warm-started from the instance's FP64 pressure and
-    reported once |b - A x| / |b| < 1e-6 in FP64."""
+def run_served(ca, solver_kw, scenes, duration=60.0, lam=2000.0, max_resident=256, budget=200 << 30, rho=0.95,
+               steps=50, storage=1, refine=True, seed=7, drain_s=20.0):
+    """Serving form: projection requests arrive as a Poisson process of rate
+    `lam` for `duration` seconds.  They come from simulated instances, each a
+    sequence of `steps` correlated solves (a simulation issues step k+1 only
+    after step k's pressure is back): an arriving request is taken by an idle
+    instance that has steps left, or else starts a new instance, whose octree is
+    drawn at random from `scenes` (lognormal leaf-tile counts) and admitted into
+    the server under the HBM budget (at most `max_resident` instances); an
+    instance that has run its `steps` solves is released, freeing its memory.
+    Requests that find neither wait in arrival order.  An instance's first solve
+    is cold (its scene's right-hand side from the host); each later one updates
+    the right-hand side on the device as b = rho b + sqrt(1 - rho^2) rms(b0) u
+    (u seeded unit-variance noise) and starts from the instance's previous
+    pressure.  Latency is arrival -> result.  storage: V-cycle storage (0 FP32,
+    1 BF16); refine: every request is an FP64 iterative refinement reported
+    once |b - A x| / |b| < 1e-6 in FP64; otherwise the FP32 solve to 1e-6 of
+    its recursive residual, each result's FP64 residual reported beside it.
+    Requests not finished `drain_s` seconds after the window closes are
+    counted as unfinished."""
     import collections
-    scenes = build_scenes(ca, [batch_scene_params(q, seed=1)[0] for q in range(nscenes)])
-    trees_b0 = [(sc.tree, np.ascontiguousarray(sc.rhs, np.float32)) for sc in scenes]
-    S = ca.Server(budget, conc, level_iters=solver_kw["level_iters"], coarsest_iters=solver_kw["coarsest_iters"],
-                  smoother=solver_kw["smoother"], storage=int(storage), refine=bool(refine))
-    insts, sig = [], []
-    admit_s = []
-    for w in range(conc):
-        tree, b0 = trees_b0[w % nscenes]
-        try:
-            t_a = time.perf_counter()
-            insts.append(S.admit(tree))
-            admit_s.append(time.perf_counter() - t_a)
-        except ca.CirrusBusy:
-            break
-        sig.append(float(np.sqrt(1 - rho * rho) * np.sqrt(np.mean(b0[b0 != 0] ** 2))))
-    W = len(insts)
-    # bring every instance to a warm state (first, cold solve from its host rhs)
-    for w, i in enumerate(insts):
-        S.submit(i, trees_b0[w % nscenes][1], warm=False, tag=-1 - w, download=False)
-    left = W
-    while left:
-        S.step()
-        while S.next_result(with_x=False) is not None:
-            left -= 1
-    rng = np.random.default_rng(7)
+    rng = np.random.default_rng(seed)
+    srv = ca.Server(budget, max_resident, level_iters=solver_kw["level_iters"], coarsest_iters=solver_kw["coarsest_iters"],
+                    smoother=solver_kw["smoother"], storage=int(storage), refine=bool(refine),
+                    verify_f64=0 if refine else 1)
+    pool = [(sc.tree, np.ascontiguousarray(sc.rhs, np.float32)) for sc in scenes]
+    sig0 = [float(np.sqrt(1 - rho * rho) * np.sqrt(np.mean(b0[b0 != 0] ** 2))) for _, b0 in pool]
     arrivals = np.cumsum(rng.exponential(1.0 / lam, int(lam * duration * 1.2) + 16))
     arrivals = arrivals[arrivals < duration]
-    fifo = [collections.deque() for _ in range(W)]
-    busy = [False] * W
-    count = [1] * W  # solves done per instance (the warm-up one included)
-    lat, iters, act, f64 = [], [], [], []
+    waiting = collections.deque()
+    idle = collections.deque()       # instances with steps left and no request running
+    inst_scene, inst_done = {}, {}   # instance -> scene index, solves finished
+    running = {}                     # request tag -> instance
+    lat, iters, act, f64, admit_s, sizes = [], [], [], [], [], []
+    n_admit = n_release = n_busy = 0
     nxt = 0
     t0 = time.perf_counter()
-    deadline = duration + 20.0
+    deadline = duration + drain_s
+    t_last = 0.0
     while True:
         now = time.perf_counter() - t0
         while nxt < len(arrivals) and arrivals[nxt] <= now:
-            fifo[nxt % W].append(nxt)
+            waiting.append(nxt)
             nxt += 1
-        for w in range(W):
-            if not busy[w] and fifo[w]:
-                j = fifo[w][0]
-                warm = count[w] % steps != 0
-                if warm:
-                    S.perturb_rhs(insts[w], rho, sig[w], 1000003 * w + count[w])
-                    S.submit(insts[w], None, warm=True, tag=j, download=False)
-                else:
-                    S.submit(insts[w], trees_b0[w % nscenes][1], warm=False, tag=j, download=False)
-                busy[w] = True
-        act.append(S.step())
+        while waiting:
+            j = waiting[0]
+            if idle:
+                inst = idle.popleft()
+                q = inst_scene[inst]
+                srv.perturb_rhs(inst, rho, sig0[q], 1000003 * inst + 7919 * inst_done[inst] + j)
+                srv.submit(inst, None, warm=True, tag=j, download=False)
+            elif len(inst_scene) < max_resident:
+                q = int(rng.integers(len(pool)))
+                t_a = time.perf_counter()
+                try:
+                    inst = srv.admit(pool[q][0])
+                except ca.CirrusBusy:
+                    n_busy += 1
+                    break
+                admit_s.append(time.perf_counter() - t_a)
+                n_admit += 1
+                sizes.append(int((pool[q][0].type == 1).sum()))
+                inst_scene[inst], inst_done[inst] = q, 0
+                srv.submit(inst, pool[q][1], warm=False, tag=j, download=False)
+            else:
+                break
+            waiting.popleft()
+            running[j] = inst
+        act.append(srv.step())
         t_done = time.perf_counter() - t0
         while True:
-            r = S.next_result(with_x=False)
+            r = srv.next_result(with_x=False)
             if r is None:
                 break
             j, inst, res, _ = r
-            w = j % W
-            fifo[w].popleft()
-            busy[w] = False
-            count[w] += 1
+            running.pop(j, None)
             lat.append(t_done - arrivals[j])
             iters.append(res["iters"])
             f64.append(res["rel_res_f64"])
-        if nxt >= len(arrivals) and not any(busy) and not any(fifo):
+            t_last = t_done
+            inst_done[inst] += 1
+            if inst_done[inst] >= steps:
+                srv.release(inst)
+                n_release += 1
+                del inst_scene[inst], inst_done[inst]
+            else:
+                idle.append(inst)
+        if nxt >= len(arrivals) and not running and not waiting:
             break
         if now > deadline:
             break
-    t_end = time.perf_counter() - t0
+    info = srv.info()
+    srv.close()
     lat_ms = np.array(lat) * 1e3
-    leaf = [int((trees_b0[w % nscenes][0].type == 1).sum()) for w in range(W)]
     prec = ("BF16 V-cycle storage" if storage else "FP32") + \
-        (", FP64 iterative refinement (FP64 pressure kept per instance, every result checked in FP64)" if refine else "")
+        (", FP64 iterative refinement (FP64 pressure kept per instance, every result checked in FP64)" if refine
+         else ", every result's FP64 residual checked")
+    done_s = max(t_last, 1e-9)
     return {
-        "workload": f"Poisson requests at {lam:g}/s for {duration:g} s over {W} resident instances "
-                    f"(lognormal sizes, median {int(np.median(leaf))} leaf tiles), warm-started correlated "
-                    f"sequences of {steps}, {prec}, RB-GS 2+2, tol 1e-6 of |b|",
-        "instances": W, "requests": int(len(arrivals)), "completed": len(lat),
-        "solves_per_s": len(lat) / t_end, "offered_per_s": lam,
+        "workload": f"Poisson requests at {lam:g}/s for {duration:g} s from simulated instances of {steps} "
+                    f"warm-started correlated steps; instance octrees drawn per instance from {len(pool)} lognormal "
+                    f"scenes, admitted and released during the stream under a {budget / 2**30:g} GB HBM budget "
+                    f"(at most {max_resident} resident); {prec}, RB-GS 2+2, tol 1e-6 of |b|",
+        "offered_per_s": lam, "window_s": duration, "requests": int(len(arrivals)),
+        "completed": len(lat), "unfinished": int(len(arrivals) - len(lat)),
+        "solves_per_s": len(lat) / done_s, "wall_s": done_s,
         "p50_ms": float(np.percentile(lat_ms, 50)) if len(lat) else None,
         "p99_ms": float(np.percentile(lat_ms, 99)) if len(lat) else None,
         "iterations_median": float(np.median(iters)) if iters else None,
-        "active_mean": float(np.mean(act)) if act else 0.0, "wall_s": t_end,
-        "fp64_rel_res_max": float(np.max(f64)) if (refine and f64) else None,
-        "fp64_checked": len(f64) if refine else 0,
-        "fp64_over_tol": int(np.sum(np.array(f64) >= 1e-6)) if refine else None,
+        "active_mean": float(np.mean(act)) if act else 0.0,
+        "instances_admitted": n_admit, "instances_released": n_release, "admissions_refused_busy": n_busy,
+        "instance_leaf_tiles_median": int(np.median(sizes)) if sizes else None,
+        "instance_leaf_tiles_max": int(np.max(sizes)) if sizes else None,
+        "distinct_scenes": len(pool), "resident_at_end": info["instances"],
+        "fp64_checked": len(f64), "fp64_rel_res_max": float(np.max(f64)) if f64 else None,
+        "fp64_over_tol": int(np.sum(np.array(f64) >= 1e-6)) if f64 else None,
         "admit_ms_median": float(np.median(admit_s)) * 1e3 if admit_s else None,
         "admit_ms_max": float(np.max(admit_s)) * 1e3 if admit_s else None,
-        "_items": trees_b0,
     }
 
 
-def run_batched(ca, scenes, B, reps, solver_kw, use_graph=True, storage=0, refine=False, tune=False, t_build=0.0):
+def run_batched(ca, scenes, B, reps, solver_kw, use_graph=True, storage=0, refine=False, tune=False, t_build=0.0,
+                oracle_sample=False):
     """Solve nprob independent problems together; returns the form's record.
     storage: V-cycle storage (0 FP32, 1 BF16).  refine: FP64 iterative
     refinement to 1e-6 of the FP64 residual (inner MGPCG to 5e-7); otherwise
     the MGPCG to 1e-6 of its recursive residual, each pressure then checked in
     FP64.  tune: the autotuner picks the schedule and the smoother settings
     (bfloat16 storage allowed) on the batch's own right-hand sides first.
-    scenes: the built scenes; B: their batched handle."""
+    scenes: the built scenes; B: their batched handle.  oracle_sample: keep
+    the FP64 pressures of six problems for the FP64-oracle comparison."""
     nprob = len(scenes)
     b_all = np.concatenate([np.ascontiguousarray(sc.rhs, np.float32) for sc in scenes])
     S = ca.GMGSolver(1.0, 1.0)
@@ -219,6 +237,16 @@ def run_batched(ca, scenes, B, reps, solver_kw, use_graph=True, storage=0, refin
         assert all(r["converged"] == 1 for r in res)
     leaf = [int((sc.tree.type == 1).sum()) for sc in scenes]
     iters = [r["iters"] for r in res]
+    check = None
+    if oracle_sample and (refine or tune):
+        # FP64 pressures of a sample of the problems (largest, smallest and
+        # quantiles between) for the comparison with the FP64 oracle
+        order = np.argsort(leaf, kind="stable")
+        picks = sorted({int(order[int(round(f * (nprob - 1)))]) for f in (1.0, 0.9, 0.75, 0.5, 0.25, 0.0)})
+        x64 = B.split(B.get_f64())
+        bq = B.split(b_all)
+        check = [(q, batch_scene_params(q)[0], scenes[q].tree.level, scenes[q].tree.coord, scenes[q].tree.type,
+                  bq[q].copy(), x64[q].copy()) for q in picks]
     f64 = [r["rel_res"] if (refine or tune) else r["rel_res_f64"] for r in res]
     total = float(np.sum(times))
     # the V-cycle's dominant kernel on the pool's most populated level
@@ -248,7 +276,34 @@ def run_batched(ca, scenes, B, reps, solver_kw, use_graph=True, storage=0, refin
         "autotune": tune_rep,
         "gs_sweep": {"level": lv, "tiles": gtiles, "us": gus, "alg_bytes": gbytes,
                      "GBps": gbytes / (gus * 1e-6) / 1e9, "frac_hbm_peak": gbytes / (gus * 1e-6) / 1e9 / PEAK_HBM_GBS},
+        "_check": check,
     }
+
+
+def run_plain64(ca, steps=50, warmup=5):
+    """The uniform 64^3 configuration through the plain drop-in (solid sphere
+    r=0.2, damped Jacobi 2+2 with w=2/3, 10 bottom iterations, FP32, tol
+    1e-6, from x = 0), timed like the headline solve."""
+    scene_kw, skw, desc = WORKLOADS["plain64"]
+    sc = ca.Scene(**scene_kw)
+    P = ca.Problem(sc.tree)
+    b = np.ascontiguousarray(sc.rhs, np.float32)
+    S = ca.GMGSolver(1.0, 1.0)
+    lat = []
+    for i in range(warmup + steps):
+        P.set(ca.CH_B, b)
+        t = time.perf_counter()
+        S.solve(P, False, 1000, 1e-6, skw["level_iters"], skw["coarsest_iters"], 1, False, smoother=skw["smoother"],
+                verify_f64=i == warmup + steps - 1)
+        if i >= warmup:
+            lat.append(time.perf_counter() - t)
+        assert S.last["converged"] == 1
+    lat_ms = np.array(lat) * 1e3
+    out = {"workload": desc, "tiles": P.n, "leaf_tiles": P.n_leaf, "solves_per_s": steps / float(np.sum(lat)),
+           "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+           "iterations": S.last["iters"], "fp64_rel_res": S.last["rel_res_f64"]}
+    P.close()
+    return out
 
 
 def run_amg(ca, P, b_host, solver_kw, reps=10):
@@ -282,7 +337,6 @@ def run_amg(ca, P, b_host, solver_kw, reps=10):
 
 
 VORTEX_KW = VORTEX512
-VORTEX_SMALL_KW = VORTEX256
 
 
 def run_vortex(ca, solver_kw, reps=3, kw=VORTEX_KW):
@@ -312,13 +366,13 @@ def run_vortex(ca, solver_kw, reps=3, kw=VORTEX_KW):
     refined()
     plain()
     rr = [refined() for _ in range(reps)]
+    x64 = P.get_f64()  # the refined pressure (a verified plain solve overwrites the FP64 array)
     pp = [plain() for _ in range(reps)]
     r = rr[-1]
     assert r["converged"] == 1
     L = P.max_level
     k32, nt = P.benchmark_kernel(ca.KERNEL_GS_SWEEP, L, reps=50)
     k16, _ = P.benchmark_kernel(ca.KERNEL_GS_SWEEP_BF16, L, reps=50)
-    x64 = P.get_f64()
     return {
         "workload": "vortex-tube octree 512x512x1024 effective (8 seeded Gaussian tubes, leaf level 6 where "
                     "|w| > 0.5, level 4 elsewhere), BF16 V-cycle storage, FP32 smoothing/stencils, FP64 residual "
@@ -331,10 +385,10 @@ def run_vortex(ca, solver_kw, reps=3, kw=VORTEX_KW):
         "gs_sweep_level": L, "gs_sweep_tiles": nt,
         "gs_sweep_f32": {"us": k32, "GBps": gs_alg_bytes(nt, 4) / (k32 * 1e-6) / 1e9},
         "gs_sweep_bf16": {"us": k16, "GBps": gs_alg_bytes(nt, 2) / (k16 * 1e-6) / 1e9},
-        "_x64_leaf": x64[sc.tree.type == 1],
+        "_check": (kw, sc.tree.level, sc.tree.coord, sc.tree.type, b, x64),
     }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gs_color_adaptive256_hbm_traffic.txt")
-FORMS = ["projection", "amg", "plain_drop_in", "batched", "batched_mixed", "batched_mixed_f16", "batched_autotuned", "served",
+FORMS = ["plain64", "projection", "amg", "plain_drop_in", "batched", "batched_mixed", "batched_mixed_f16", "batched_autotuned", "served",
          "served_mixed", "served_mixed_light", "vortex"]
 
 
@@ -369,43 +423,75 @@ def cpu_baseline(scene_kw, solver_kw, budget_s=20.0, max_solves=8):
         r = o.solve(**kw)
         times.append(r["seconds"])
         iters = r["iters"]
-    return {
+    return dict({
         "value": len(times) / sum(times), "unit": "solves/s", "cores": po.get_threads(), "kind": "port",
         "sample": f"{len(times)} full solves of the same problem ({iters} iterations each), "
                   f"median {np.median(times) * 1e3:.1f} ms",
-    }
+    }, **host_cpu())
 
 
-def fp64_oracle_check(ca, solver_kw, kw=VORTEX_SMALL_KW):
-    """Error of the mixed-precision pressure against the FP64 oracle, on a
-    vortex-tube octree one level coarser than the measured one (leaf level 5,
-    level 3 elsewhere) so that the FP64 CPU solve stays bounded: the same
-    refinement (BF16 storage, FP64 residual, 1e-8) on the GPU, the FP64 oracle
-    solved to 1e-10, relative L2 difference over LEAF INTERIOR cells."""
+def host_cpu():
+    """Model, sockets and logical CPUs of this host (from /proc/cpuinfo) and
+    the CPUs this process may run on."""
+    model, sockets, logical = None, set(), 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    sockets.add(v)
+                elif k == "processor":
+                    logical += 1
+    except OSError:
+        pass
+    return {"cpu_model": model, "sockets": len(sockets) or None, "host_logical_cpus": logical or None,
+            "cpus_available": len(os.sched_getaffinity(0))}
+
+
+def fp64_oracle_error(kw, level, coord, ttype, b32, x64, solver_kw, tol=1e-11):
+    """Relative L2 error, over LEAF INTERIOR cells, of a GPU pressure x64
+    (FP64; the library scene's tile order given by level/coord/type; solved
+    for the FP32 right-hand side b32) against the FP64 oracle's solution of the
+    same system to |b - A x| < tol |b|.  The oracle starts from the GPU's x64:
+    it forms r = b - A x64 with its FP64 operator, solves A d = r by its FP64
+    MGPCG until |r - A d| < tol |b|, and x64 + d is the reference pressure, so
+    the error is |d| / |x64 + d| (the oracle's own error is then about tol
+    times the condition number, far below the one measured)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
-    o = po.Oracle(**kw)
-    lev, crd, typ, _ = o.topology()
-    ct = o.celltypes()
-    P = ca.Problem(ca.Tree(o.params.h0, lev, crd, typ, ct))
-    b = o.get(po.CH_B).astype(np.float32)
-    P.set(ca.CH_B, b)
-    r = ca.GMGSolver().solve_refined(P, 1e-8, 1e-4, 20, 1000, solver_kw["level_iters"], solver_kw["coarsest_iters"],
-                                     smoother=solver_kw["smoother"], storage=ca.STORE_BF16)[0]
-    x = P.get_f64()
-    o64 = po.Oracle(params=o.params, fp64=True)
-    o64.set(po.CH_B, b.astype(np.float64))
+    o64 = po.Oracle(fp64=True, **kw)
+    lev, crd, typ, _ = o64.topology()
+    ko = {(int(l), *map(int, c)): i for i, (l, c) in enumerate(zip(lev, crd))}
+    to_o = np.array([ko[(int(l), *map(int, c))] for l, c in zip(level, coord)], np.int64)
+    assert np.array_equal(typ[to_o], ttype)
+    n = len(lev)
+    leaf = typ == po.LEAF
+    m = leaf[:, None] & (o64.celltypes() == po.INTERIOR)
+    b = np.zeros((n, 512))
+    b[to_o] = np.asarray(b32, np.float64)
+    b[~m] = 0.0
+    x0 = np.zeros((n, 512))
+    x0[to_o] = x64
+    x0[~leaf] = 0.0
     t = time.perf_counter()
-    r64 = o64.solve(max_iters=500, tol=1e-10, level_iters=solver_kw["level_iters"],
-                    coarsest_iters=solver_kw["coarsest_iters"], smoother=solver_kw["smoother"])
+    o64.set(po.CH_X, x0)
+    o64.full_negative_laplacian(po.CH_X, po.CH_AP)
+    r = np.where(m, b - o64.get(po.CH_AP), 0.0)
+    bn, rn = float(np.linalg.norm(b[m])), float(np.linalg.norm(r[m]))
+    o64.set(po.CH_B, r)
+    r64 = o64.solve(max_iters=1000, tol=min(1.0, tol * bn / rn) if rn > 0 else 1.0,
+                    level_iters=solver_kw["level_iters"], coarsest_iters=solver_kw["coarsest_iters"],
+                    smoother=solver_kw["smoother"])
     t64 = time.perf_counter() - t
-    xr = o64.get(po.CH_X)
-    m = (typ == po.LEAF)[:, None] & (ct == po.INTERIOR)
-    return {"scene": "vortex-tube octree 256x256x512 effective (leaf level 5, level 3 elsewhere), "
-                     f"{int((typ == po.LEAF).sum())} leaf tiles",
-            "gpu_fp64_rel_res": r["rel_res"], "refinement_steps": r["outer"], "iterations": r["iters"],
-            "oracle_fp64_rel_res": r64["rel_res"], "oracle_seconds": t64, "oracle_threads": po.get_threads(),
-            "rel_l2_error_vs_fp64_oracle": float(np.linalg.norm((x - xr)[m]) / np.linalg.norm(xr[m]))}
+    d = o64.get(po.CH_X)
+    xr = x0 + d
+    err = float(np.linalg.norm(d[m]) / np.linalg.norm(xr[m]))
+    o64.close()
+    return err, {"gpu_fp64_rel_res_by_oracle": rn / bn, "oracle_fp64_rel_res": r64["rel_res"] * rn / bn,
+                 "oracle_iterations": r64["iters"], "oracle_seconds": t64, "oracle_threads": po.get_threads()}
 
 
 def gs_alg_bytes(tiles, s=4):
@@ -619,16 +705,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="adaptive256", choices=list(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle-check", action="store_true",
+                    help="skip the FP64-oracle errors of the mixed-precision results (host solves after the timing)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--batch", type=int, default=256,
                     help="problems in the batched form measured beside the plain solve (0: skip)")
     ap.add_argument("--batch-reps", type=int, default=3)
-    ap.add_argument("--serve-seconds", type=float, default=3.0,
-                    help="length of the Poisson request window of the served form (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="length of the Poisson request window of the served_mixed form (0: skip the served forms)")
+    ap.add_argument("--serve-seconds-fp32", type=float, default=10.0,
+                    help="length of the request window of the FP32 served form")
+    ap.add_argument("--serve-seconds-light", type=float, default=20.0,
+                    help="length of the request window of the served_mixed_light form")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
-    ap.add_argument("--serve-rate-light", type=float, default=600.0,
-                    help="offered rate of the served_mixed_light form, below capacity (0: skip)")
-    ap.add_argument("--serve-instances", type=int, default=256)
+    ap.add_argument("--serve-rate-light", type=float, default=0.0,
+                    help="offered rate of the served_mixed_light form (0: 0.7 of what served_mixed completed per s)")
+    ap.add_argument("--serve-instances", type=int, default=256, help="most instances resident at once")
+    ap.add_argument("--serve-scenes", type=int, default=256,
+                    help="lognormal scenes the served instances draw their octrees from")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0, help="HBM admission budget of the served forms")
     ap.add_argument("--forms", default="all", help=f"comma list of the forms to run beside the plain solve ({FORMS})")
     ap.add_argument("--no-vortex", action="store_true",
@@ -751,10 +845,14 @@ def main():
     }
     note(f"plain solve: {out['value']:.1f} solves/s")
     forms = {}
+    pending_checks = []  # (form, data) compared with the FP64 oracle at the end
     wanted = set(FORMS if args.forms == "all" else args.forms.split(","))
     unknown = wanted - set(FORMS)
     if unknown:
         raise SystemExit(f"unknown forms {sorted(unknown)}; known: {FORMS}")
+    if "plain64" in wanted and args.workload != "plain64":
+        forms["plain64"] = run_plain64(ca)
+        note(f"plain64: {forms['plain64']['solves_per_s']:.1f} solves/s")
     # the whole projection around the solve (FluidEuler::project without the
     # velocity BCs): divergence of the scene's face velocities, the solve, the
     # gradient added back; velocities re-uploaded before each, outside the timing
@@ -798,12 +896,17 @@ def main():
         B = ca.Problem.batch([sc.tree for sc in scenes])
         t_pool = time.perf_counter() - t0
         note(f"pool of {args.batch} problems ({B.n} tiles) created in {t_pool:.2f} s")
-        for name, storage, refine, tune in [("batched", 0, False, False), ("batched_mixed", 1, True, False),
+        # FP32 storage is also refined in FP64, so that every pressure meets
+        # 1e-6 of its FP64 residual like the mixed forms'
+        for name, storage, refine, tune in [("batched", 0, True, False), ("batched_mixed", 1, True, False),
                                             ("batched_mixed_f16", 2, True, False), ("batched_autotuned", 0, False, True)]:
             if name not in wanted:
                 continue
             f = run_batched(ca, scenes, B, args.batch_reps, solver_kw, use_graph=not args.no_graph, storage=storage,
-                            refine=refine, tune=tune, t_build=t_build_b)
+                            refine=refine, tune=tune, t_build=t_build_b, oracle_sample=name == "batched_mixed")
+            chk = f.pop("_check")
+            if chk:
+                pending_checks.append((name, chk))
             if base:
                 f["vs_plain_solves_per_s"] = f["solves_per_s"] / base["solves_per_s"]
             f["vs_plain_adaptive256"] = f["solves_per_s"] / out["value"]
@@ -812,30 +915,52 @@ def main():
             note(f"{name}: {f['solves_per_s']:.1f} solves/s")
         B.close()
         del items, scenes
-    if args.serve_seconds > 0:
+    served_forms = [f for f in ("served", "served_mixed", "served_mixed_light") if f in wanted]
+    if args.serve_seconds > 0 and served_forms:
+        t0 = time.perf_counter()
+        sscenes = build_scenes(ca, [batch_scene_params(q, seed=1)[0] for q in range(args.serve_scenes)])
+        note(f"{len(sscenes)} serving scenes built in {time.perf_counter() - t0:.1f} s")
         base = None
-        for name, storage, refine, lam in [("served", 0, False, args.serve_rate),
-                                           ("served_mixed", 1, True, args.serve_rate),
-                                           ("served_mixed_light", 1, True, args.serve_rate_light)]:
-            if lam <= 0 or name not in wanted:
+        if "plain_drop_in" in wanted:
+            # the scenes the instances draw from, through the plain drop-in, each
+            # solved cold as the reference's projection does
+            items = [(sc.tree, np.ascontiguousarray(sc.rhs, np.float32)) for sc in sscenes]
+            base = plain_drop_in(ca, items, solver_kw, reps=1)
+            forms["plain_drop_in_served_problems"] = dict(base, workload="the serving scenes (the instances draw from "
+                                                          "them uniformly), FP32, one handle each, solved one after "
+                                                          "another from x = 0")
+            del items
+        capacity = None
+        budget = int(args.serve_budget_gb * (1 << 30))
+        for name in served_forms:
+            if name == "served":  # FP32 storage, every result FP64-verified (refinement)
+                storage, lam, dur = 0, args.serve_rate, args.serve_seconds_fp32
+            elif name == "served_mixed":  # the serving configuration as stated
+                storage, lam, dur = 1, args.serve_rate, args.serve_seconds
+            else:  # below capacity: 0.7 of what served_mixed completed per second
+                storage, dur = 1, args.serve_seconds_light
+                lam = args.serve_rate_light if args.serve_rate_light > 0 else \
+                    (0.7 * capacity if capacity else 0.0)
+            if lam <= 0 or dur <= 0:
                 continue
-            f = run_served(ca, solver_kw, duration=args.serve_seconds, lam=lam, conc=args.serve_instances,
-                           budget=int(args.serve_budget_gb * (1 << 30)), storage=storage, refine=refine)
-            items = f.pop("_items")
-            if base is None and "plain_drop_in" in wanted:
-                # the instances' problems through the plain drop-in, each solved
-                # cold as the reference's projection does
-                base = plain_drop_in(ca, items, solver_kw, reps=3)
-                forms["plain_drop_in_served_problems"] = dict(base, workload="the served instances' problems, FP32, "
-                                                              "one handle each, solved one after another from x = 0")
+            try:
+                f = run_served(ca, solver_kw, sscenes, duration=dur, lam=lam, max_resident=args.serve_instances,
+                               budget=budget, storage=storage, refine=True)
+            except ca.CirrusError as e:  # recorded in the line; the other forms still run
+                forms[name] = {"error": str(e)}
+                note(f"{name}: {e}")
+                continue
+            if name == "served_mixed":
+                capacity = f["solves_per_s"]
             if base:
                 f["vs_plain_solves_per_s"] = f["solves_per_s"] / base["solves_per_s"]
             f["vs_plain_adaptive256"] = f["solves_per_s"] / out["value"]
             forms[name] = f
-            note(f"{name}: {f['solves_per_s']:.1f} solves/s")
+            note(f"{name}: {f['solves_per_s']:.1f} solves/s at {lam:.0f}/s offered, p99 {f['p99_ms']} ms")
+        del sscenes
     if not args.no_vortex and "vortex" in wanted:
         f = run_vortex(ca, solver_kw)
-        f.pop("_x64_leaf")
+        pending_checks.append(("refined_vortex512", f.pop("_check")))
         forms["refined_vortex512"] = f
         note("vortex done")
     if forms:
@@ -845,8 +970,26 @@ def main():
                                  if k in forms and "vs_plain_solves_per_s" in forms[k]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene_kw, solver_kw)
-        if "refined_vortex512" in forms:
-            forms["refined_vortex512"]["fp64_oracle_check"] = fp64_oracle_check(ca, solver_kw)
+        if "plain64" in forms:
+            p64_kw, p64_skw, _ = WORKLOADS["plain64"]
+            forms["plain64"]["cpu_baseline"] = cpu_baseline(p64_kw, p64_skw, budget_s=5.0)
+            forms["plain64"]["vs_cpu_baseline"] = forms["plain64"]["solves_per_s"] / forms["plain64"]["cpu_baseline"]["value"]
+    # errors of the mixed-precision results against the FP64 oracle (after
+    # every timed part; the oracle solves the same scenes on the host)
+    if not args.no_oracle_check:
+        for name, chk in pending_checks:
+            note(f"FP64 oracle check: {name}")
+            if name == "refined_vortex512":
+                kw, level, coord, ttype, b32, x64 = chk
+                err, info = fp64_oracle_error(kw, level, coord, ttype, b32, x64, solver_kw)
+                forms[name]["fp64_oracle_check"] = dict(info, rel_l2_error_vs_fp64_oracle=err)
+            else:
+                rows = []
+                for q, kw, level, coord, ttype, b32, x64 in chk:
+                    err, info = fp64_oracle_error(kw, level, coord, ttype, b32, x64, solver_kw)
+                    rows.append(dict(info, problem=q, leaf_tiles=int((ttype == 1).sum()), rel_l2_error_vs_fp64_oracle=err))
+                forms[name]["fp64_oracle_check"] = {
+                    "problems": rows, "rel_l2_error_max": max(r["rel_l2_error_vs_fp64_oracle"] for r in rows)}
     print(json.dumps(out), flush=True)
 
 
