@@ -274,8 +274,8 @@ def run_batched(ca, scenes, B, reps, solver_kw, use_graph=True, storage=0, refin
         "fp64_over_tol": int(np.sum(np.array(f64) >= 1e-6)),
         "outer_max": int(max(r["outer"] for r in res)) if (refine or tune) else None,
         "autotune": tune_rep,
-        "gs_sweep": {"level": lv, "tiles": gtiles, "us": gus, "alg_bytes": gbytes,
-                     "GBps": gbytes / (gus * 1e-6) / 1e9, "frac_hbm_peak": gbytes / (gus * 1e-6) / 1e9 / PEAK_HBM_GBS},
+        "gs_sweep": gs_record(lv, gtiles, gus, gbytes, "k_gs_color<float>" if kw["storage"] == 0 else
+                              ("k_gs_color<bf16>" if kw["storage"] == 1 else "k_gs_color<f16>")),
         "_check": check,
     }
 
@@ -304,6 +304,16 @@ def run_plain64(ca, steps=50, warmup=5):
            "iterations": S.last["iters"], "fp64_rel_res": S.last["rel_res_f64"]}
     P.close()
     return out
+
+
+def gs_record(level, tiles, us, alg_bytes, kname):
+    """Roofline record of a colour sweep at batch scale: the hipEvent timing of
+    this run and, where the committed kernel-trace summary covers the same
+    launch, the profiler's per-dispatch mean."""
+    rp = read_trace_mean(BATCH_TRACE_FILE, kname, tiles)
+    return {"level": level, "tiles": tiles, "us": us, "alg_bytes": alg_bytes, "GBps": alg_bytes / (us * 1e-6) / 1e9,
+            "frac_hbm_peak": alg_bytes / (us * 1e-6) / 1e9 / PEAK_HBM_GBS, "rocprof_us": rp,
+            "frac_hbm_peak_rocprof": alg_bytes / (rp * 1e-6) / 1e9 / PEAK_HBM_GBS if rp else None}
 
 
 def run_amg(ca, P, b_host, solver_kw, reps=10):
@@ -388,6 +398,8 @@ def run_vortex(ca, solver_kw, reps=3, kw=VORTEX_KW):
         "_check": (kw, sc.tree.level, sc.tree.coord, sc.tree.type, b, x64),
     }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gs_color_adaptive256_hbm_traffic.txt")
+SINGLE_TRACE_FILE = os.path.join(ROOT, "profiles", "k_gs_color_adaptive256_kernel_stats.txt")
+BATCH_TRACE_FILE = os.path.join(ROOT, "profiles", "k_gs_color_batch256_kernel_stats.txt")
 FORMS = ["plain64", "projection", "amg", "plain_drop_in", "batched", "batched_mixed", "batched_mixed_f16", "batched_autotuned", "served",
          "served_mixed", "served_mixed_light", "vortex"]
 
@@ -682,6 +694,20 @@ def summarize_trace(trace_csv, stats_csv, title=None, out=None, bench_args=None)
     print("\n".join(lines))
 
 
+def read_trace_mean(path, kernel, tiles):
+    """Mean per-dispatch duration (us) of `kernel` at `tiles` workgroups from a
+    committed kernel-trace summary (its per-launch-size table), if present."""
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 5 and parts[0] == kernel and parts[1] == str(tiles):
+                    return float(parts[3])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def read_traffic(tiles):
     """Traffic per launch from the committed counter summary, if it covers this launch."""
     try:
@@ -705,6 +731,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="adaptive256", choices=list(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pair-sweeps", type=int, default=-1,
+                    help="one-launch red-black iterations on the measured handles: 1 on, 0 off, -1 the default")
     ap.add_argument("--no-oracle-check", action="store_true",
                     help="skip the FP64-oracle errors of the mixed-precision results (host solves after the timing)")
     ap.add_argument("--no-graph", action="store_true")
@@ -724,7 +752,8 @@ def main():
     ap.add_argument("--serve-scenes", type=int, default=256,
                     help="lognormal scenes the served instances draw their octrees from")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0, help="HBM admission budget of the served forms")
-    ap.add_argument("--forms", default="all", help=f"comma list of the forms to run beside the plain solve ({FORMS})")
+    ap.add_argument("--forms", default="all",
+                    help=f"comma list of the forms to run beside the plain solve ({FORMS}), or all, or none")
     ap.add_argument("--no-vortex", action="store_true",
                     help="skip the mixed-precision refinement form on the vortex-tube octree")
     ap.add_argument("--kernels-only", action="store_true",
@@ -762,6 +791,8 @@ def main():
     t0 = time.perf_counter()
     P = ca.Problem(scene.tree)
     t_create = time.perf_counter() - t0
+    if args.pair_sweeps >= 0:
+        P.set_level_forms(-1, -1, args.pair_sweeps)
     if args.kernels_only:
         if args.kernels_batch > 0:
             # the same kernels at batch scale, past the last-level cache: the
@@ -809,6 +840,7 @@ def main():
     us, tiles = P.benchmark_kernel(ca.KERNEL_GS_SWEEP, L, reps=200)
     alg_bytes = gs_alg_bytes(tiles)
     traffic = read_traffic(tiles)
+    rp_us = read_trace_mean(SINGLE_TRACE_FILE, "k_gs_color<float>", tiles)
     achieved = alg_bytes / (us * 1e-6) / 1e9
     per_kernel = {}
     for name, kid, lvl in [("apply", ca.KERNEL_APPLY, 0), ("restrict_L", ca.KERNEL_RESTRICT, L),
@@ -840,13 +872,15 @@ def main():
         "create_s": t_create,
         "roofline": {"bound": "hbm", "kernel": f"k_gs_color level {L} ({tiles} tiles)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "avg_launch_us": us, "alg_bytes_per_launch": alg_bytes},
+                     "avg_launch_us": us, "alg_bytes_per_launch": alg_bytes,
+                     "rocprof_avg_launch_us": rp_us, "frac_rocprof": (alg_bytes / (rp_us * 1e-6) / 1e9 / PEAK_HBM_GBS)
+                     if rp_us else None, "rocprof_summary": os.path.relpath(SINGLE_TRACE_FILE, ROOT)},
         "kernels_us": per_kernel,
     }
     note(f"plain solve: {out['value']:.1f} solves/s")
     forms = {}
     pending_checks = []  # (form, data) compared with the FP64 oracle at the end
-    wanted = set(FORMS if args.forms == "all" else args.forms.split(","))
+    wanted = set(FORMS if args.forms == "all" else ([] if args.forms == "none" else args.forms.split(",")))
     unknown = wanted - set(FORMS)
     if unknown:
         raise SystemExit(f"unknown forms {sorted(unknown)}; known: {FORMS}")
@@ -895,6 +929,8 @@ def main():
         t0 = time.perf_counter()
         B = ca.Problem.batch([sc.tree for sc in scenes])
         t_pool = time.perf_counter() - t0
+        if args.pair_sweeps >= 0:
+            B.set_level_forms(-1, -1, args.pair_sweeps)
         note(f"pool of {args.batch} problems ({B.n} tiles) created in {t_pool:.2f} s")
         # FP32 storage is also refined in FP64, so that every pressure meets
         # 1e-6 of its FP64 residual like the mixed forms'

@@ -33,7 +33,7 @@ STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 SCENE_UNIFORM, SCENE_SPHERE_BAND, SCENE_PLANE, SCENE_VORTEX = 0, 1, 2, 3
 (KERNEL_GS_SWEEP, KERNEL_APPLY, KERNEL_RESTRICT, KERNEL_PROLONG, KERNEL_JACOBI, KERNEL_UPDATE_XR, KERNEL_COARSE,
  KERNEL_EMPTY, KERNEL_COPY, KERNEL_GS_PROLONG, KERNEL_GS_SWEEP_BF16, KERNEL_GS_PROLONG_BF16,
- KERNEL_GS_SWEEP_F16, KERNEL_AMG_GS) = range(14)
+ KERNEL_GS_SWEEP_F16, KERNEL_AMG_GS, KERNEL_GS_PAIR, KERNEL_GS_PAIR_BF16) = range(16)
 
 EXPORTED_SYMBOLS = (
     "cg_params_default", "cg_last_error", "cg_device_count", "cg_problem_create", "cg_problem_create_batch",
@@ -175,7 +175,7 @@ def lib():
             "cg_add_gradient": (i32, [vp, i32]),
             "cg_project": (i32, [vp, P(Params), P(Result)]),
             "cg_set_schedule": (i32, [vp, i32, i32, i32]),
-            "cg_set_level_forms": (i32, [vp, i32, i32]),
+            "cg_set_level_forms": (i32, [vp, i32, i32, i32]),
             "cg_autotune": (i32, [vp, P(Params), i32, P(Params), P(TuneReport)]),
             "cg_autotune_refined": (i32, [vp, P(Params), P(RefineParams), i32, P(Params), P(TuneReport)]),
             "cg_solve_refined": (i32, [vp, P(Params), P(RefineParams), P(RefineResult)]),
@@ -442,11 +442,13 @@ class Problem:
         _check(lib().cg_set_schedule(self._h, int(coarse_shift), int(bool(swizzle)), int(tile_block)),
                "cg_set_schedule")
 
-    def set_level_forms(self, eager_max_tiles=-1, fused16_max_tiles=-1):
+    def set_level_forms(self, eager_max_tiles=-1, fused16_max_tiles=-1, pair_sweeps=-1):
         """Level-size limits of the eager prolongation/restriction forms and of
-        the fused first 16-bit sweep (cg_set_level_forms; results unchanged;
-        0 never, negative the default)."""
-        _check(lib().cg_set_level_forms(self._h, int(eager_max_tiles), int(fused16_max_tiles)), "cg_set_level_forms")
+        the fused first 16-bit sweep, and whether red-black iterations after
+        the first run as one launch of both colours (cg_set_level_forms;
+        results unchanged; 0 never, negative the default)."""
+        _check(lib().cg_set_level_forms(self._h, int(eager_max_tiles), int(fused16_max_tiles), int(pair_sweeps)),
+               "cg_set_level_forms")
 
     def get_f64(self):
         """The FP64 x of the last refined solve, [tiles, 512] float64."""

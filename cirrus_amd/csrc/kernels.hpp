@@ -98,6 +98,7 @@ struct Ctx {
   double* part;  // per-leaf-slot dot partials
   double* part2; // second partial stream (mean counts)
   const float* diag;  // per cell, colour-split order: the V-cycle colour sweep's diagonal D (k_diag)
+  const TileRec* tiles;  // every tile's record by tile index (null where the handle has none)
 };
 
 // The V-cycle vectors of the storage type V.
@@ -1168,6 +1169,27 @@ __global__ __launch_bounds__(64) void k_gs_color(Ctx c, const TileRec* __restric
   __shared__ GsLDS L;
   tile_gs<V, DOT>(c, T, level, X, F, color, flags, L);
 }
+// A colour sweep that writes the tile's whole result (the updated colour and
+// the other colour as read) to Y and leaves X as it was, so that the next
+// launch may read Y while it writes X.
+template <class V>
+__global__ __launch_bounds__(64) void k_gs_color_to(Ctx c, const TileRec* __restrict__ list, int n, const V* X, V* Y,
+                                                    int color, int flags) {
+  TILE_KERNEL_PROLOGUE
+  __shared__ GsLDS L;
+  const GsIn g = gs_load<V, false>(c, T, X, static_cast<const V*>(nullptr), color, flags);
+  gs_stage(g, L);
+  wave_sync();
+  if ((flags & F_CHECK_DONE) && g.done) return;
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const int pc = (i + 1) * 10 + (j + 1);
+  const bool p = ((color + i + j) & 1) != 0;
+  const float4 nv = gs_quad(g.xc, g.xo, L.xo[pc - 10], L.xo[pc + 10], L.xo[pc - 1], L.xo[pc + 1], g.xkm, g.xkp, g.B,
+                            g.mask, T.bc, T.h, i, j, p);
+  V* tile = Y + size_t(T.self) * 512;
+  store4(tile + (color << 8) + lane * 4, nv);
+  store4(tile + ((color ^ 1) << 8) + lane * 4, g.xo);
+}
 // The same colour sweep with four wavefronts per workgroup, one tile each
 // (tiles 4b .. 4b+3 of the list in workgroup b): a quarter of the workgroups
 // to dispatch for the same tiles.
@@ -1332,6 +1354,8 @@ __global__ __launch_bounds__(64) void k_gs_first(Ctx c, const TileRec* __restric
   store4(tile + lane * 4, xr);
   store4(tile + 256 + lane * 4, n1);
 }
+
+#include "gs_pair.hpp"
 
 // The same pass with every global load issued first, as raw bits (own pencil,
 // parent quad, face masks, halo pencils and their parents' quads, k halos, rhs,

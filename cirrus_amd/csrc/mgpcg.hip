@@ -96,6 +96,7 @@ struct cg_problem {
   int device = 0;           // the HIP device current when the handle was created
   int eager_max = 16384;    // levels of at most this many tiles: every load before one wait (prolongation pass, restriction)
   int fused16_max = 16384;  // 16-bit levels of at most this many tiles: first down-sweep iteration in one launch
+  int gs_pair = 0;          // RB-GS iterations after the first as one launch (k_gs_pair; needs d_tiles; slower, off)
   std::vector<int> lev, tbase, coarse_auto;
   std::vector<uint8_t> ty;
 
@@ -163,6 +164,7 @@ struct cg_problem {
     c.pmap = d_pmap;
     c.part = d_part;
     c.part2 = d_part2;
+    c.tiles = gs_pair ? d_tiles : nullptr;
     return c;
   }
   Ctx ctx() const { return ctx(d_ch[CG_CH_Z], d_ch[CG_CH_R]); }
@@ -733,6 +735,15 @@ void launch_gs(cg_problem& P, const DList& a, const Ctx& c, int l, V* Z, const V
   }
   launch_gs_as<V, false>(P, a, c, l, Z, R, color, flags);
 }
+// One red-black iteration of level list a, colour c1 then 1 - c1, in one
+// k_gs_pair launch: the listed tiles' values are read from XI, ghosts' from XG,
+// the results written to XO (neither of the others).
+template <class V>
+void launch_gs_pair(cg_problem& P, const DList& a, const Ctx& c, const V* XI, const V* XG, V* XO, int c1, int flags) {
+  if (a.n == 0) return;
+  if (flags & F_DOT_RZ) k_gs_pair<V, true><<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, XI, XG, XO, c1, flags);
+  else k_gs_pair<V, false><<<grid_for(a.n, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, XI, XG, XO, c1, flags);
+}
 // The pool's V-cycle vectors of storage type V.
 template <class V> V* pool_z(cg_problem& P);
 template <class V> V* pool_zb(cg_problem& P);
@@ -798,14 +809,25 @@ void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v, in
   for (int l = L; l >= 1; l--) {
     const DList& a = P.actg[l];
     if (v.smoother == CG_SMOOTH_RBGS) {
+      // With one-launch iterations (k_gs_pair, which must not write the array
+      // it reads) the level's values alternate between ZO's level region,
+      // unused until the up-sweep, and Z, so that they end in Z.
+      const bool pairs = c.tiles != nullptr && v.nu >= 2;
+      V* cur = (pairs && v.nu % 2 == 0) ? ZO : Z;
       for (int s = 0; s < v.nu; s++) {
         if (s == 0 && P.d_diag && (std::is_same<V, float>::value || a.n <= P.fused16_max)) {
           // both colours of the first iteration from the zero guess in one launch
-          if (a.n) k_gs_first<V><<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, Z, base | F_GHOST_ZERO);
+          if (a.n) k_gs_first<V><<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, cur, base | F_GHOST_ZERO);
           continue;
         }
-        launch_gs(P, a, c, l, Z, R, 0, base | F_GHOST_ZERO | (s == 0 ? F_ZERO_X : 0));
-        launch_gs(P, a, c, l, Z, R, 1, base | F_GHOST_ZERO);
+        if (s == 0 || !pairs) {  // two colour passes in place (s == 0: from the zero guess)
+          launch_gs(P, a, c, l, cur, R, 0, base | F_GHOST_ZERO | (s == 0 ? F_ZERO_X : 0));
+          launch_gs(P, a, c, l, cur, R, 1, base | F_GHOST_ZERO);
+          continue;
+        }
+        V* nxt = cur == Z ? ZO : Z;
+        launch_gs_pair<V>(P, a, c, cur, cur, nxt, 0, base | F_GHOST_ZERO);  // ghosts read as 0
+        cur = nxt;
       }
     } else {
       const V* src = Z;
@@ -851,10 +873,29 @@ void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v, in
         k_gs_prolong<V><<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, Z, ZO, v.pcoef, 1, base);
       else if (al.n)
         k_gs_prolong_eager<V><<<grid_for(al.n, base), 64, 0, P.stream>>>(c, P.L(al), al.n, Z, ZO, v.pcoef, 1, base);
-      launch_gs(P, a, c, l, ZO, R, 0, base | (v.nu == 1 ? last_flags : 0));
+      if (c.tiles == nullptr || v.nu < 2) {
+        launch_gs(P, a, c, l, ZO, R, 0, base | (v.nu == 1 ? last_flags : 0));
+        for (int s = 1; s < v.nu; s++) {
+          launch_gs(P, a, c, l, ZO, R, 1, base);
+          launch_gs(P, a, c, l, ZO, R, 0, base | (s == v.nu - 1 ? last_flags : 0));
+        }
+        continue;
+      }
+      // one-launch iterations: the red pass and the iterations alternate the
+      // level's values between Z's level region (its down-sweep values are
+      // consumed by the prolongation pass) and ZO, ending in ZO; ghosts keep
+      // their prolongated values in ZO throughout
+      V* cur = ZO;
+      if ((v.nu - 1) % 2 == 1) {
+        if (a.n) k_gs_color_to<V><<<grid_for(a.n, base), 64, 0, P.stream>>>(c, P.L(a), a.n, ZO, Z, 0, base);
+        cur = Z;
+      } else {
+        launch_gs(P, a, c, l, ZO, R, 0, base);
+      }
       for (int s = 1; s < v.nu; s++) {
-        launch_gs(P, a, c, l, ZO, R, 1, base);
-        launch_gs(P, a, c, l, ZO, R, 0, base | (s == v.nu - 1 ? last_flags : 0));
+        V* nxt = cur == ZO ? Z : ZO;
+        launch_gs_pair<V>(P, a, c, cur, ZO, nxt, 1, base | (s == v.nu - 1 ? last_flags : 0));
+        cur = nxt;
       }
       continue;
     }
@@ -1229,8 +1270,15 @@ void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& 
   std::vector<cg_result> inner(np);
   cg_params inner_prm = prm;
   inner_prm.verify_f64 = 0;
+  bool first = true;
   while (true) {
-    const std::vector<double> r2 = residual64(P, P.d_bsave, prm.pure_neumann, true);
+    // from x64 = 0 the first residual is b itself (A 0 = 0, r64 = b exactly, its
+    // float is the b already in the r channel and |r64|^2 the |b|^2 above, summed
+    // the same way), so that step's FP64 operator is skipped; the re-centred
+    // pure-Neumann residual is formed as usual
+    const bool from_zero = first && !prm.pure_neumann && rp.max_outer > 0;
+    first = false;
+    const std::vector<double> r2 = from_zero ? bb : residual64(P, P.d_bsave, prm.pure_neumann, true);
     bool any = false;
     for (int q = 0; q < np; q++) {
       if (done[q]) continue;
@@ -1626,12 +1674,13 @@ cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle, 
   });
 }
 
-cg_status cg_set_level_forms(cg_problem* p, int32_t eager_max_tiles, int32_t fused16_max_tiles) {
+cg_status cg_set_level_forms(cg_problem* p, int32_t eager_max_tiles, int32_t fused16_max_tiles, int32_t pair_sweeps) {
   return guard([&] {
     if (!p) fail(CG_ERR_INVALID, "null problem");
     if (!p->owner) fail(CG_ERR_INVALID, "cg_set_level_forms: not an owning handle");
     p->eager_max = eager_max_tiles < 0 ? kProlongEagerMax : eager_max_tiles;
     p->fused16_max = fused16_max_tiles < 0 ? kFirstFused16Max : fused16_max_tiles;
+    p->gs_pair = pair_sweeps > 0 ? 1 : 0;
     HIPC(hipStreamSynchronize(p->stream));
     if (p->graph) HIPC(hipGraphExecDestroy(p->graph));
     p->graph = nullptr;
@@ -1921,18 +1970,22 @@ cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel) {
 
 cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int32_t reps, double* avg_us, int64_t* tiles) {
   return guard([&] {
-    if (!p || !avg_us || reps <= 0) fail(CG_ERR_INVALID, "cg_benchmark_kernel: bad argument");
+    if (!p || !avg_us || reps <= 0 || reps > 100000) fail(CG_ERR_INVALID, "cg_benchmark_kernel: bad argument");
     cg_problem& P = *p;
     if (kernel == CG_KERNEL_AMG_GS && !P.amg_ready) fail(CG_ERR_STATE, "cg_benchmark_kernel: call cg_amg_prepare first");
+    if ((kernel == CG_KERNEL_GS_PAIR || kernel == CG_KERNEL_GS_PAIR_BF16) && !P.d_tiles)
+      fail(CG_ERR_STATE, "cg_benchmark_kernel: the handle has no tile-indexed records");
     if ((kernel == CG_KERNEL_GS_SWEEP || kernel == CG_KERNEL_JACOBI || kernel == CG_KERNEL_GS_SWEEP_BF16 ||
-         kernel == CG_KERNEL_GS_SWEEP_F16 || kernel == CG_KERNEL_AMG_GS) &&
+         kernel == CG_KERNEL_GS_SWEEP_F16 || kernel == CG_KERNEL_AMG_GS || kernel == CG_KERNEL_GS_PAIR ||
+         kernel == CG_KERNEL_GS_PAIR_BF16) &&
         (level < 0 || level > P.max_level))
       fail(CG_ERR_INVALID, "level out of range");
     if ((kernel == CG_KERNEL_RESTRICT || kernel == CG_KERNEL_PROLONG || kernel == CG_KERNEL_GS_PROLONG ||
          kernel == CG_KERNEL_GS_PROLONG_BF16) &&
         (level < 1 || level > P.max_level))
       fail(CG_ERR_INVALID, "level out of range");
-    if (kernel == CG_KERNEL_GS_SWEEP_BF16 || kernel == CG_KERNEL_GS_PROLONG_BF16 || kernel == CG_KERNEL_GS_SWEEP_F16)
+    if (kernel == CG_KERNEL_GS_SWEEP_BF16 || kernel == CG_KERNEL_GS_PROLONG_BF16 || kernel == CG_KERNEL_GS_SWEEP_F16 ||
+        kernel == CG_KERNEL_GS_PAIR_BF16)
       ensure_bf16(P);
     if ((kernel == CG_KERNEL_COARSE || kernel == CG_KERNEL_EMPTY) && level < 0) fail(CG_ERR_INVALID, "negative count");
     set_prob_state(P, 0.0, 1 << 30, 0);
@@ -1950,6 +2003,18 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
           const DList& a = P.act[level];
           nt = a.n;
           launch_gs(P, a, c, level, c.zh, nob, 0, base | F_GHOST_ZERO);
+          break;
+        }
+        case CG_KERNEL_GS_PAIR: {  // one red-black iteration in one launch (z read, the scratch written)
+          const DList& a = P.act[level];
+          nt = a.n;
+          launch_gs_pair<float>(P, a, c, c.z, c.z, c.zb, 0, base | F_GHOST_ZERO);
+          break;
+        }
+        case CG_KERNEL_GS_PAIR_BF16: {
+          const DList& a = P.act[level];
+          nt = a.n;
+          launch_gs_pair<bf16>(P, a, c, c.zh, c.zh, c.zbh, 0, base | F_GHOST_ZERO);
           break;
         }
         case CG_KERNEL_AMG_GS: {
@@ -2025,18 +2090,21 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
       }
     };
     launch();  // warm-up
-    hipEvent_t e0, e1;
-    HIPC(hipEventCreate(&e0));
-    HIPC(hipEventCreate(&e1));
-    HIPC(hipEventRecord(e0, P.stream));
+    // back-to-back launches between one pair of events (an event pair around
+    // each launch would add its own few microseconds to every launch)
+    struct Events {
+      hipEvent_t e[2] = {nullptr, nullptr};
+      ~Events() { for (hipEvent_t x : e) if (x) (void)hipEventDestroy(x); }
+    } ev;
+    HIPC(hipEventCreate(&ev.e[0]));
+    HIPC(hipEventCreate(&ev.e[1]));
+    HIPC(hipEventRecord(ev.e[0], P.stream));
     for (int i = 0; i < reps; i++) launch();
-    HIPC(hipEventRecord(e1, P.stream));
-    HIPC(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPC(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPC(hipEventRecord(ev.e[1], P.stream));
+    HIPC(hipEventSynchronize(ev.e[1]));
     HIPC(hipGetLastError());
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
     *avg_us = double(ms) * 1e3 / reps;
     if (tiles) *tiles = nt;
   });

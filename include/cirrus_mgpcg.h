@@ -200,8 +200,12 @@ cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle, 
  * load issued before one wait (shorter latency chain; more registers); with
  * 16-bit storage, levels of at most fused16_max_tiles tiles run the first
  * down-sweep iteration as one launch (it reads the halo's diagonals).  0: never,
- * a negative value: the default 16384.  No reference counterpart. */
-cg_status cg_set_level_forms(cg_problem* p, int32_t eager_max_tiles, int32_t fused16_max_tiles);
+ * a negative value: the default 16384.  pair_sweeps: 1 runs every red-black
+ * iteration after the first as one launch of both colours (the halo's first
+ * colour formed again from the neighbours' data; bit-identical, but slower
+ * than the two passes on every level measured), 0 or a negative value as two
+ * colour passes (the default).  No reference counterpart. */
+cg_status cg_set_level_forms(cg_problem* p, int32_t eager_max_tiles, int32_t fused16_max_tiles, int32_t pair_sweeps);
 typedef struct {
   int32_t candidates;        /* parameter sets timed */
   double base_seconds;       /* one solve with the caller's parameters, default schedule */
@@ -310,8 +314,9 @@ cg_status cg_dot_noalloc(cg_problem* p, int32_t a_channel, int32_t b_channel, in
 cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel);
 
 /* Average device duration (microseconds, hipEvents on the problem's stream)
- * of `reps` back-to-back launches of one kernel of the solve on its real
- * worklist, for roofline measurement; no reference counterpart.
+ * of `reps` (at most 100,000) back-to-back launches of one kernel of the
+ * solve on its real worklist, for roofline measurement; no reference
+ * counterpart.
  * kernel: CG_KERNEL_GS_SWEEP (one colour sweep of `level`, V-cycle mode),
  * CG_KERNEL_APPLY (CG operator on LEAF+GHOST tiles), CG_KERNEL_RESTRICT and
  * CG_KERNEL_PROLONG (from/to `level`), CG_KERNEL_JACOBI (one sweep of `level`),
@@ -324,12 +329,13 @@ cg_status cg_recenter_leaf_voxels(cg_problem* p, int32_t channel);
  * CG_KERNEL_GS_PROLONG_BF16 (the same with bfloat16 V-cycle storage),
  * CG_KERNEL_GS_SWEEP_F16 (the colour sweep with float16 storage),
  * CG_KERNEL_AMG_GS (one colour of the AMG stored-coefficient sweep on `level`;
- * after cg_amg_prepare).
+ * after cg_amg_prepare), CG_KERNEL_GS_PAIR and CG_KERNEL_GS_PAIR_BF16 (one
+ * red-black iteration of `level` in one launch, FP32 / bfloat16 storage).
  * Channels are overwritten. */
 enum { CG_KERNEL_GS_SWEEP = 0, CG_KERNEL_APPLY = 1, CG_KERNEL_RESTRICT = 2, CG_KERNEL_PROLONG = 3,
        CG_KERNEL_JACOBI = 4, CG_KERNEL_UPDATE_XR = 5, CG_KERNEL_COARSE = 6, CG_KERNEL_EMPTY = 7,
        CG_KERNEL_COPY = 8, CG_KERNEL_GS_PROLONG = 9, CG_KERNEL_GS_SWEEP_BF16 = 10, CG_KERNEL_GS_PROLONG_BF16 = 11,
-       CG_KERNEL_GS_SWEEP_F16 = 12, CG_KERNEL_AMG_GS = 13 };
+       CG_KERNEL_GS_SWEEP_F16 = 12, CG_KERNEL_AMG_GS = 13, CG_KERNEL_GS_PAIR = 14, CG_KERNEL_GS_PAIR_BF16 = 15 };
 cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int32_t reps, double* avg_us,
                               int64_t* tiles);
 

@@ -12,10 +12,10 @@ scene bit for bit (tiles, types, cell types, right-hand side); each GPU solve
 must equal the oracle's solve bit for bit (iterations, |r|^2, pressure); the
 FP32 pressure of the adaptive scene must lie within REL_L2_TOL of the FP64
 oracle's converged pressure.  The pool is solved under each level-form setting
-(the eager prolongation/restriction and fused first 16-bit sweep never, at the
-default level-size limits, always), so both kernel forms meet the oracle on
-levels of every size, small and large (the pool's finest level has more than
-16,384 tiles).
+(the eager prolongation/restriction, the fused first 16-bit sweep and the
+one-launch red-black iterations never, at the defaults, always), so every
+kernel form meets the oracle on levels of every size, small and large (the
+pool's finest level has more than 16,384 tiles).
 """
 import numpy as np
 import pytest
@@ -32,7 +32,7 @@ REL_L2_TOL = 1e-4
 # problems of the batched pool (seed 0): the two largest, a large one, the
 # median, three mid-sized ones and the smallest (uniform level 2)
 POOL = [242, 218, 96, 188, 124, 109, 5, 17]
-LEVEL_FORMS = {"never": (0, 0), "default": (-1, -1), "always": (1 << 30, 1 << 30)}
+LEVEL_FORMS = {"never": (0, 0, 0), "default": (-1, -1, -1), "always": (1 << 30, 1 << 30, 1)}
 _cache = {}
 
 
@@ -147,7 +147,7 @@ def test_pool_batch_solve_bitexact(storage, forms):
         res = ca.GMGSolver().solve_batch(B, 1000, 1e-6, 2, 10, storage=storage)
         xb = B.split(B.get(ca.CH_X))
     finally:
-        B.set_level_forms(-1, -1)
+        B.set_level_forms(-1, -1, -1)
     for q, ((sc, _, _), (ro, xo)) in enumerate(zip(items, ref)):
         assert res[q]["converged"] == 1, POOL[q]
         assert (res[q]["iters"], res[q]["rr"], res[q]["bb"]) == (ro["iters"], ro["rr"], ro["bb"]), POOL[q]

@@ -181,6 +181,23 @@ def test_solve_bitexact(name, smoother, nu):
     assert_same(P.get(ca.CH_X), o.get(po.CH_X), "pressure", mask=leaf)
 
 
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("nu", [2, 3])
+def test_solve_bitexact_one_launch_iterations(name, nu):
+    """Red-black iterations after the first as one launch of both colours
+    (cg_set_level_forms pair_sweeps = 1; the values alternate between two
+    arrays): the same solve bit for bit."""
+    o, P, typ, nbr, b = pair(name)
+    P.set_level_forms(-1, -1, 1)
+    try:
+        o, P, typ, ro, rd = _solve_both(name, ca.SMOOTH_RBGS, nu)
+    finally:
+        P.set_level_forms(-1, -1, -1)
+    assert rd["converged"] == 1
+    assert (rd["iters"], rd["rr"], rd["bb"]) == (ro["iters"], ro["rr"], ro["bb"])
+    assert_same(P.get(ca.CH_X), o.get(po.CH_X), "pressure", mask=typ == po.LEAF)
+
+
 @pytest.mark.parametrize("use_graph,stride", [(False, 1), (True, 4), (False, 3)])
 def test_solve_launch_modes_and_sync_stride(use_graph, stride):
     """Graph replay or direct launches; the residual tested every `stride`
