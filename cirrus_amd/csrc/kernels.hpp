@@ -1227,6 +1227,48 @@ __global__ __launch_bounds__(64) void k_gs_color2(Ctx c, const TileRec* __restri
   gs_finish<V, DOT>(c, T0, X, color, flags, g0, L[0]);
   if (two) gs_finish<V, DOT>(c, T1, X, color, flags, g1, L[1]);
 }
+// The same colour sweep with each wavefront walking its share of the list:
+// while it updates one tile, the loads of its next tile are already out (a
+// straight-line loop; after its last tile a wavefront loads that tile once
+// more and drops it), so a wavefront's memory latency overlaps its own work
+// instead of needing more wavefronts.  With F_SWIZZLE, the workgroups of XCD x
+// (b % 8) walk the x-th eighth of the list (neighbouring tiles, one L2).  The
+// grid is sized to the wavefronts the device holds at once.  Every tile's
+// update is the one above.
+template <class V, bool DOT = false>
+__global__ __launch_bounds__(64) void k_gs_color_loop(Ctx c, const TileRec* __restrict__ list, int n, int level, V* X,
+                                                      const V* F, int color, int flags) {
+  __shared__ GsLDS L;
+  const int b = blockIdx.x, nb = gridDim.x;
+  int t, end, stride;
+  if (flags & F_SWIZZLE) {
+    const int per = (n + 7) >> 3, x = b & 7;
+    t = x * per + (b >> 3);
+    end = min(n, (x + 1) * per);
+    stride = nb >> 3;  // nb is a multiple of 8
+  } else {
+    t = b;
+    end = n;
+    stride = nb;
+  }
+  if (t >= end) return;
+  TileRec T = list[t];
+  GsRaw<V> r = gs_issue<V, DOT>(c, T, X, F, color, flags);
+  while (true) {
+    const int tn = t + stride < end ? t + stride : t;
+    const TileRec Tn = list[tn];
+    GsRaw<V> rn = gs_issue<V, DOT>(c, Tn, X, F, color, flags);  // the next tile's loads
+    const GsIn g = gs_convert<V>(r);
+    gs_stage(g, L);
+    wave_sync();
+    gs_finish<V, DOT>(c, T, X, color, flags, g, L);
+    if (tn == t) break;
+    wave_sync();  // the LDS is staged again for the next tile
+    t = tn;
+    T = Tn;
+    r = rn;
+  }
+}
 // The colour sweep's diagonal of every cell of a tile (the D of gs_quad: the
 // same coefficient choice and the same operations, in the colour-split order),
 // for the fused first iteration below.  Fixed by the face masks and the tile's
@@ -3097,10 +3139,8 @@ __global__ __launch_bounds__(64) void k_divergence(Ctx c, const TileRec* __restr
 // (PoissonGrid.cu:721-755) on LEAF tiles: each cell whose type is in mask adds
 // coeff times the sum, in child order (x fastest), of its 8 children (cells of
 // its ghost child tile) whose type is in mask.
-__global__ __launch_bounds__(64) void k_accumulate_children(Ctx c, const TileRec* __restrict__ list, int n,
-                                                            float* __restrict__ X, uint8_t mask, float coeff) {
-  const int flags = 0;
-  TILE_KERNEL_PROLOGUE
+__device__ __forceinline__ void tile_accumulate_children(const Ctx& c, const TileRec& T, float* __restrict__ X,
+                                                         uint8_t mask, float coeff) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t base = size_t(T.self) * 512 + lane * 8;
   const uint64_t ct = load_ct8(c.ctype + base);
@@ -3141,6 +3181,20 @@ __global__ __launch_bounds__(64) void k_accumulate_children(Ctx c, const TileRec
     x[k] = x[k] + sum * coeff;
   }
   store8(X + base, x);
+}
+__global__ __launch_bounds__(64) void k_accumulate_children(Ctx c, const TileRec* __restrict__ list, int n,
+                                                            float* __restrict__ X, uint8_t mask, float coeff) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  tile_accumulate_children(c, T, X, mask, coeff);
+}
+// The same for the three face-velocity channels in one launch (one after the
+// other inside each wavefront).
+__global__ __launch_bounds__(64) void k_accumulate_children_vel(Ctx c, const TileRec* __restrict__ list, int n, Vel U,
+                                                                uint8_t mask, float coeff) {
+  const int flags = 0;
+  TILE_KERNEL_PROLOGUE
+  for (int a = 0; a < 3; a++) tile_accumulate_children(c, T, U.u[a], mask, coeff);
 }
 
 // AddGradientToFaceCenters (PoissonSolver.cu:144-204) on LEAF|GHOST tiles

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <memory>
 #include <mutex>
@@ -134,6 +135,9 @@ struct cg_problem {
   CoarseSlot* d_coarse = nullptr;
   hipStream_t stream = nullptr;
   int* h_pin = nullptr;
+  ProbState* h_ps = nullptr;     // pinned staging of the ProbState uploads
+  hipEvent_t ev_ps = nullptr;    // recorded after the last upload from h_ps
+  bool ps_pending = false;
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipEvent_t ev_io[2] = {nullptr, nullptr};  // orders device channel moves against the caller's stream
   hipGraphExec_t graph = nullptr;
@@ -177,6 +181,8 @@ cg_problem::~cg_problem() {
   for (auto e : ev) if (e) (void)hipEventDestroy(e);
   for (auto e : ev_io) if (e) (void)hipEventDestroy(e);
   if (h_pin) (void)hipHostFree(h_pin);
+  if (h_ps) (void)hipHostFree(h_ps);
+  if (ev_ps) (void)hipEventDestroy(ev_ps);
   void* ptrs[] = {d_ctype, d_fmask, d_diag, d_zb, d_zh, d_rhsvh, d_zbh, d_x64, d_r64, d_bsave, d_psum, d_gsum, d_ps, d_dflag,
                   d_plans, d_part, d_part2, d_scalar, d_lists, d_tiles, d_tmp_list, d_coarse, d_amgc, d_amgct, d_ext};
   for (void* q : ptrs) if (q) (void)hipFree(q);
@@ -622,6 +628,8 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   k_diag<<<P.all.n, 64, 0, P.stream>>>(P.ctx(), P.L(P.all), P.all.n, P.d_diag);
   HIPC(hipGetLastError());
   HIPC(hipHostMalloc(&P.h_pin, 2 * sizeof(int) * nprob, hipHostMallocDefault));
+  HIPC(hipHostMalloc(&P.h_ps, sizeof(ProbState) * nprob, hipHostMallocDefault));
+  HIPC(hipEventCreateWithFlags(&P.ev_ps, hipEventDisableTiming));
   HIPC(hipEventCreateWithFlags(&P.ev[0], hipEventDisableTiming));
   HIPC(hipEventCreateWithFlags(&P.ev[1], hipEventDisableTiming));
   HIPC(hipEventCreateWithFlags(&P.ev_io[0], hipEventDisableTiming));
@@ -689,7 +697,16 @@ void channel_device(cg_problem& P, int ch, const float* src, float* dst, hipStre
 // tol_q / maxit_q: per-problem tolerance and iteration cap (null: tol / max_iters for all)
 void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann, int check_stride = 1,
                     const double* tol_q = nullptr, const int* maxit_q = nullptr, const double* thr_abs_q = nullptr) {
-  std::vector<ProbState> st(P.nprob);
+  // staged in pinned memory, so the upload is ordered on the stream with no
+  // host wait (only for the previous upload from the same buffer)
+  std::vector<ProbState> local;
+  ProbState* st = P.h_ps;
+  if (st) {
+    if (P.ps_pending) HIPC(hipEventSynchronize(P.ev_ps));
+  } else {
+    local.resize(P.nprob);
+    st = local.data();
+  }
   for (int q = 0; q < P.nprob; q++) {
     ProbState& s = st[q];
     std::memset(&s, 0, sizeof(s));
@@ -702,9 +719,14 @@ void set_prob_state(cg_problem& P, double tol, int max_iters, int pure_neumann, 
     s.pure_neumann = pure_neumann;
     s.check_stride = check_stride;
   }
-  HIPC(hipMemcpyAsync(P.d_ps, st.data(), sizeof(ProbState) * P.nprob, hipMemcpyHostToDevice, P.stream));
+  HIPC(hipMemcpyAsync(P.d_ps, st, sizeof(ProbState) * P.nprob, hipMemcpyHostToDevice, P.stream));
   HIPC(hipMemsetAsync(P.d_dflag, 0, sizeof(int32_t) * P.nprob, P.stream));
-  HIPC(hipStreamSynchronize(P.stream));  // st is a local buffer
+  if (P.h_ps) {
+    HIPC(hipEventRecord(P.ev_ps, P.stream));
+    P.ps_pending = true;
+  } else {
+    HIPC(hipStreamSynchronize(P.stream));  // a local buffer
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -717,9 +739,27 @@ struct VParams {
   int amg = 0;  // the preconditioner is VCycleAMG (cg_amg_solve)
 };
 
+// Workgroups of a looped colour sweep (tile block 8): as many as the device
+// holds at once (the kernel's occupancy per CU times the CUs), rounded down
+// to a multiple of the 8 XCDs; found once per kernel instance.
+template <class V, bool DOT>
+int loop_waves() {
+  static const int w = [] {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gs_color_loop<V, DOT>, 64, 0) != hipSuccess || per <= 0)
+      return 0;
+    return std::max(8, (cus * per) & ~7);
+  }();
+  return w;
+}
 template <class V, bool DOT>
 void launch_gs_as(cg_problem& P, const DList& a, const Ctx& c, int l, V* Z, const V* R, int color, int flags) {
-  if (P.gs_block == 2 && a.n > 1)
+  const int lw = P.gs_block == 8 ? loop_waves<V, DOT>() : 0;
+  if (lw > 0 && a.n > lw) {
+    k_gs_color_loop<V, DOT><<<unsigned(lw), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
+  } else if (P.gs_block == 2 && a.n > 1)
     k_gs_color2<V, DOT><<<grid_for((a.n + 1) / 2, flags), 64, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
   else if (P.gs_block == 4 && a.n > 1)
     k_gs_color4<V, DOT><<<grid_for((a.n + 3) / 4, flags), 256, 0, P.stream>>>(c, P.L(a), a.n, l, Z, R, color, flags);
@@ -1119,8 +1159,10 @@ std::vector<double> residual64(cg_problem& P, const float* B, int pure_neumann, 
 
 // tol_q / maxit_q / thr_abs_q: per-problem tolerances, iteration caps and
 // threshold floors (null: prm's for all, no floor).
+// tail: work enqueued after the last iteration, before the host waits for the
+// solve (cg_project's gradient), so the device does not idle in between.
 void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* tol_q = nullptr, const int* maxit_q = nullptr,
-           const double* thr_abs_q = nullptr, bool amg = false) {
+           const double* thr_abs_q = nullptr, bool amg = false, const std::function<void()>* tail = nullptr) {
   check_params(prm);
   if (amg && !P.amg_ready) fail(CG_ERR_STATE, "cg_amg_solve: call cg_amg_prepare first");
   if (amg && prm.storage != CG_STORE_F32) fail(CG_ERR_INVALID, "cg_amg_solve: FP32 storage only");
@@ -1207,6 +1249,7 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
     k_scale_leaf<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.x, -1);
     k_scale_leaf<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, -1);
   }
+  if (tail) (*tail)();
   HIPC(hipStreamSynchronize(P.stream));
   auto t1 = std::chrono::steady_clock::now();
   std::vector<ProbState> st = read_states(P);
@@ -1314,8 +1357,8 @@ void solve_refined(cg_problem& P, const cg_params& prm, const cg_refine_params& 
 // ---------------------------------------------------------------------------
 void set_schedule(cg_problem& P, int coarse_shift, int swizzle, int tile_block) {
   if (coarse_shift < 0 || coarse_shift > CL_MAX_LEVELS) fail(CG_ERR_INVALID, "cg_set_schedule: coarse_shift out of range");
-  if (tile_block != 1 && tile_block != 2 && tile_block != 4)
-    fail(CG_ERR_INVALID, "cg_set_schedule: tile_block must be 1, 2 or 4");
+  if (tile_block != 1 && tile_block != 2 && tile_block != 4 && tile_block != 8)
+    fail(CG_ERR_INVALID, "cg_set_schedule: tile_block must be 1, 2, 4 or 8");
   if (!P.owner) fail(CG_ERR_INVALID, "cg_set_schedule: not an owning handle");
   P.swz = swizzle ? F_SWIZZLE : 0;
   P.gs_block = tile_block;
@@ -1381,7 +1424,7 @@ void autotune(cg_problem& P, const cg_params& base, int allow_16bit, cg_params* 
   int max_auto = -1;
   for (int q = 0; q < P.nprob; q++) max_auto = std::max(max_auto, P.coarse_auto[q]);
   for (int swz = 1; swz >= 0; swz--)
-    for (int block : {1, 2, 4}) {
+    for (int block : {1, 2, 4, 8}) {
       if (swz == 1 && block == 1) continue;
       set_schedule(P, 0, swz, block);
       const double t = run(base);
@@ -1443,8 +1486,7 @@ void add_gradient(cg_problem& P, float* Pch) {
   const Vel U = velocities(P);
   if (P.ghosts.n) k_propagate_ghosts<<<P.ghosts.n, 64, 0, P.stream>>>(c, P.L(P.ghosts), P.ghosts.n, Pch, Pch);
   k_gradient<<<P.applyl.n, 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, Pch, U);
-  for (int a = 0; a < 3; a++)
-    k_accumulate_children<<<P.leaf.n, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, U.u[a], C_INT | C_DIR, 1.0f / 4.0f);
+  k_accumulate_children_vel<<<P.leaf.n, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, U, C_INT | C_DIR, 1.0f / 4.0f);
 }
 
 }  // namespace
@@ -1729,8 +1771,8 @@ cg_status cg_project(cg_problem* p, const cg_params* prm, cg_result* results) {
   return guard([&] {
     if (!p || !prm || !results) fail(CG_ERR_INVALID, "cg_project: null argument");
     velocity_divergence(*p, p->d_ch[CG_CH_B], true);
-    solve(*p, *prm, results);
-    add_gradient(*p, p->d_ch[CG_CH_X]);
+    const std::function<void()> gradient = [p] { add_gradient(*p, p->d_ch[CG_CH_X]); };
+    solve(*p, *prm, results, nullptr, nullptr, nullptr, false, &gradient);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(p->stream));
   });

@@ -192,7 +192,10 @@ cg_status cg_get_solution_f64(cg_problem* p, double* host);
  * order (1, default) or the plain order (0); tile_block = how the grid-wide
  * colour sweeps take their tiles: 1 (default) one tile per 64-thread
  * workgroup, 2 two tiles per wavefront (the loads of both in flight together),
- * 4 four wavefronts of one tile each per 256-thread workgroup. */
+ * 4 four wavefronts of one tile each per 256-thread workgroup, 8 as many
+ * wavefronts as the device holds at once, each walking its share of a level's
+ * tiles with the next tile's loads issued before the current one is updated
+ * (on levels with more tiles than that; others as 1). */
 cg_status cg_set_schedule(cg_problem* p, int32_t coarse_shift, int32_t swizzle, int32_t tile_block);
 /* Two more kernel forms chosen by level size, also without effect on results:
  * levels (all problems of the handle together) of at most eager_max_tiles

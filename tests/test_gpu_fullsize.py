@@ -153,3 +153,23 @@ def test_pool_batch_solve_bitexact(storage, forms):
         assert (res[q]["iters"], res[q]["rr"], res[q]["bb"]) == (ro["iters"], ro["rr"], ro["bb"]), POOL[q]
         leaf = sc.tree.type == po.LEAF
         assert np.array_equal(xb[q][leaf], xo[leaf]), f"pool problem {POOL[q]}: pressure differs from the oracle"
+
+
+@pytest.mark.parametrize("storage", [ca.STORE_F32, ca.STORE_BF16])
+def test_pool_looped_colour_sweeps_bitexact(storage):
+    """Colour sweeps with each wavefront walking its share of a level's tiles
+    and the next tile's loads issued ahead (tile block 8, used on levels with
+    more tiles than the device holds wavefronts): the same solves bit for bit."""
+    items, B, _ = pool()
+    ref = oracle_solves(storage)
+    B.set_schedule(0, True, 8)
+    try:
+        B.set(ca.CH_B, np.concatenate([np.ascontiguousarray(sc.rhs, np.float32) for sc, _, _ in items]))
+        res = ca.GMGSolver().solve_batch(B, 1000, 1e-6, 2, 10, storage=storage)
+        xb = B.split(B.get(ca.CH_X))
+    finally:
+        B.set_schedule(0, True, 1)
+    for q, ((sc, _, _), (ro, xo)) in enumerate(zip(items, ref)):
+        assert (res[q]["iters"], res[q]["rr"], res[q]["bb"]) == (ro["iters"], ro["rr"], ro["bb"]), POOL[q]
+        leaf = sc.tree.type == po.LEAF
+        assert np.array_equal(xb[q][leaf], xo[leaf]), f"pool problem {POOL[q]}: pressure differs from the oracle"

@@ -73,7 +73,7 @@ def test_autotune_on_a_batch():
     S = ca.GMGSolver()
     kw, rep = S.autotune(B, 1000, 1e-6, 2, 10, allow_bf16=True)
     assert rep["candidates"] >= 8 and rep["best_seconds"] <= rep["base_seconds"]
-    assert rep["tile_block"] in (1, 2, 4)
+    assert rep["tile_block"] in (1, 2, 4, 8)
     assert np.array_equal(B.get(ca.CH_B), b)  # the right-hand sides are restored
     res = S.solve_batch(B, 1000, 1e-6, coarsest_iters=10, verify_f64=True, **kw)
     assert all(r["converged"] == 1 and r["rel_res_f64"] < 5e-6 for r in res)
