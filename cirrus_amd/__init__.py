@@ -47,7 +47,7 @@ EXPORTED_SYMBOLS = (
     "cg_server_next_result", "cg_server_info", "cg_server_perturb_rhs",
     "cg_refine_params_default", "cg_solve_refined", "cg_get_solution_f64",
     "cg_velocity_divergence", "cg_add_gradient", "cg_project", "cg_set_schedule", "cg_set_level_forms", "cg_autotune", "cg_autotune_refined",
-    "cg_tiles_save", "cg_tiles_load", "cg_tiles_import_holder", "cg_tileset_tree", "cg_tileset_channels",
+    "cg_scene_tree_device", "cg_tiles_save", "cg_tiles_load", "cg_tiles_import_holder", "cg_tileset_tree", "cg_tileset_channels",
     "cg_tileset_destroy", "cg_amg_prepare", "cg_amg_solve", "cg_amg_coefficients",
 )
 
@@ -181,6 +181,7 @@ def lib():
             "cg_solve_refined": (i32, [vp, P(Params), P(RefineParams), P(RefineResult)]),
             "cg_get_solution_f64": (i32, [vp, vp]),
             "cg_scene_create": (i32, [P(SceneParams), P(vp)]),
+            "cg_scene_tree_device": (i32, [P(SceneParams), ctypes.c_int64, P(vp)]),
             "cg_scene_destroy": (i32, [vp]),
             "cg_scene_tree": (i32, [vp, P(TreeDesc), P(vp)]),
             "cg_scene_velocity": (i32, [vp, P(vp)]),
@@ -345,6 +346,16 @@ def _tileset(h):
         return tree, {int(c): dv[q] for q, c in enumerate(idv)}
     finally:
         lib().cg_tileset_destroy(h)
+
+
+def scene_tree_device(max_tiles=0, **kw):
+    """The scene's octree and cell types built on the device (uniform, sphere
+    band and plane scenes; cg_scene_tree_device): a Tree, tiles in the
+    device build's order."""
+    p = scene_params(**kw)
+    h = ctypes.c_void_p()
+    _check(lib().cg_scene_tree_device(ctypes.byref(p), int(max_tiles), ctypes.byref(h)), "cg_scene_tree_device")
+    return _tileset(h)[0]
 
 
 def load_tiles(path):

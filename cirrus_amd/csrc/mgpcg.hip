@@ -1489,6 +1489,101 @@ void add_gradient(cg_problem& P, float* Pch) {
   k_accumulate_children_vel<<<P.leaf.n, 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, U, C_INT | C_DIR, 1.0f / 4.0f);
 }
 
+#include "devbuild.hpp"
+
+// Device-side build of a synthetic scene's octree and cell types (devbuild.hpp);
+// the result goes to the host as a tile set (tree, cell types, no channels).
+void scene_tree_device(const cg_scene_params& p, int64_t max_tiles, TileSet& out) {
+  if (p.kind != CG_SCENE_UNIFORM && p.kind != CG_SCENE_SPHERE_BAND && p.kind != CG_SCENE_PLANE)
+    fail(CG_ERR_INVALID, "cg_scene_tree_device: uniform, sphere band and plane scenes only");
+  if (p.nroot[0] <= 0 || p.nroot[1] <= 0 || p.nroot[2] <= 0 || !(p.h0 > 0) || p.num_layers < 1 || p.num_layers > 19)
+    fail(CG_ERR_INVALID, "cg_scene_tree_device: bad scene parameters");
+  if (int64_t(p.nroot[0]) * p.nroot[1] * p.nroot[2] > (1 << 19))
+    fail(CG_ERR_INVALID, "cg_scene_tree_device: too many root tiles");
+  const int cap = int(max_tiles > 0 ? std::min<int64_t>(max_tiles, int64_t(1) << 26) : int64_t(1) << 22);
+  uint32_t hcap = 1;
+  while (hcap < 2u * uint32_t(cap)) hcap <<= 1;
+  DeviceBuffer b_count{sizeof(int)}, b_flags{2 * sizeof(int)}, b_level{sizeof(int32_t) * size_t(cap)},
+      b_coord{3 * sizeof(int32_t) * size_t(cap)}, b_type{size_t(cap)}, b_mark{size_t(cap)},
+      b_hkeys{sizeof(unsigned long long) * hcap}, b_hvals{sizeof(int32_t) * hcap};
+  DevTree B;
+  B.cap = cap;
+  B.hmask = hcap - 1;
+  B.count = static_cast<int*>(b_count.p);
+  B.flags = static_cast<int*>(b_flags.p);
+  B.level = static_cast<int32_t*>(b_level.p);
+  B.coord = static_cast<int32_t*>(b_coord.p);
+  B.type = static_cast<uint8_t*>(b_type.p);
+  B.mark = static_cast<uint8_t*>(b_mark.p);
+  B.hkeys = static_cast<unsigned long long*>(b_hkeys.p);
+  B.hvals = static_cast<int32_t*>(b_hvals.p);
+  struct Stream {
+    hipStream_t s = nullptr;
+    ~Stream() { if (s) (void)hipStreamDestroy(s); }
+  } st;
+  HIPC(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+  HIPC(hipMemsetAsync(B.hkeys, 0xff, sizeof(unsigned long long) * hcap, st.s));
+  HIPC(hipMemsetAsync(B.hvals, 0xff, sizeof(int32_t) * hcap, st.s));
+  HIPC(hipMemsetAsync(B.count, 0, sizeof(int), st.s));
+  HIPC(hipMemsetAsync(B.flags, 0, 2 * sizeof(int), st.s));
+  const int nroots = p.nroot[0] * p.nroot[1] * p.nroot[2];
+  k_db_roots<<<unsigned((nroots + 255) / 256), 256, 0, st.s>>>(B, p);
+  HIPC(hipGetLastError());
+  int h_count = 0, h_flags[2] = {0, 0};
+  auto read_state = [&] {
+    HIPC(hipMemcpyAsync(&h_count, B.count, sizeof(int), hipMemcpyDeviceToHost, st.s));
+    HIPC(hipMemcpyAsync(h_flags, B.flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st.s));
+    HIPC(hipStreamSynchronize(st.s));
+    if (h_flags[1]) fail(CG_ERR_NOMEM, "cg_scene_tree_device: more tiles than max_tiles");
+  };
+  read_state();
+  // refinement rounds until no tile is marked (each round adds at least one
+  // level somewhere, so the rounds are bounded by the layers times the ripple)
+  bool settled = false;
+  for (int round = 0; round < 64 * p.num_layers; round++) {
+    const int n = h_count;
+    HIPC(hipMemsetAsync(B.flags, 0, sizeof(int), st.s));
+    k_db_mark<<<unsigned((n + 255) / 256), 256, 0, st.s>>>(B, p, n);
+    HIPC(hipGetLastError());
+    read_state();
+    if (!h_flags[0]) {
+      settled = true;
+      break;
+    }
+    k_db_refine<<<unsigned((n + 255) / 256), 256, 0, st.s>>>(B, n);
+    HIPC(hipGetLastError());
+    read_state();
+  }
+  if (!settled) fail(CG_ERR_INVALID, "cg_scene_tree_device: refinement did not settle");
+  const int n0 = h_count;
+  k_db_ghosts<<<unsigned((6 * int64_t(n0) + 255) / 256), 256, 0, st.s>>>(B, n0);
+  HIPC(hipGetLastError());
+  read_state();
+  const int n = h_count;
+  out.h0 = p.h0;
+  out.level.resize(n);
+  out.coord.resize(3 * size_t(n));
+  out.type.resize(n);
+  HIPC(hipMemcpyAsync(out.level.data(), B.level, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost, st.s));
+  HIPC(hipStreamSynchronize(st.s));
+  int maxl = 0;
+  for (int v : out.level) maxl = std::max(maxl, v);
+  DeviceBuffer b_ctype{size_t(n) * 512};
+  B.ctype = static_cast<uint8_t*>(b_ctype.p);
+  HIPC(hipMemsetAsync(B.ctype, 0, size_t(n) * 512, st.s));
+  k_db_leaf_types<<<unsigned(n), 512, 0, st.s>>>(B, p, n);
+  for (int l = maxl; l >= 0; l--) k_db_nonleaf_types<<<unsigned(n), 512, 0, st.s>>>(B, n, l);
+  k_db_ghost_types<<<unsigned(n), 512, 0, st.s>>>(B, n);
+  HIPC(hipGetLastError());
+  out.ctype.resize(size_t(n) * 512);
+  HIPC(hipMemcpyAsync(out.coord.data(), B.coord, 3 * sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost, st.s));
+  HIPC(hipMemcpyAsync(out.type.data(), B.type, size_t(n), hipMemcpyDeviceToHost, st.s));
+  HIPC(hipMemcpyAsync(out.ctype.data(), B.ctype, size_t(n) * 512, hipMemcpyDeviceToHost, st.s));
+  HIPC(hipStreamSynchronize(st.s));
+  out.channel_id.clear();
+  out.channel.clear();
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2298,6 +2393,15 @@ cg_status cg_tiles_save(const char* path, const cg_tree_desc* desc, int32_t num_
     }
     write_tile_file(path, t);
   });
+}
+
+cg_status cg_scene_tree_device(const cg_scene_params* sp, int64_t max_tiles, cg_tileset** out) {
+  if (!sp || !out) { g_err = "null argument"; return CG_ERR_INVALID; }
+  *out = nullptr;
+  auto S = std::make_unique<cg_tileset>();
+  cg_status st = guard([&] { scene_tree_device(*sp, max_tiles, S->s); });
+  if (st == CG_OK) *out = S.release();
+  return st;
 }
 
 cg_status cg_tiles_load(const char* path, cg_tileset** out) {

@@ -423,6 +423,19 @@ typedef struct {
   int32_t num_layers;        /* maximum number of levels (10) */
 } cg_scene_params;
 
+/* The octree and cell types of a scene built on the device, every step a
+ * kernel: refinement rounds to the scene's target levels with 2:1 face
+ * balance (RefineLeafsOneStep / IterativeRefine, src/PoissonGrid.h:35-162,
+ * 317-326), ghost spawning (SpawnGhostTiles, src/PoissonGrid.cu:30-60), the
+ * walls and solid of the scene on LEAF cells, then CalcCellTypesFromLeafs
+ * (src/PoissonSolver.cu:7-61) -- the tree cg_scene_create builds on the host,
+ * tile for tile.  Scene kinds CG_SCENE_UNIFORM, CG_SCENE_SPHERE_BAND and
+ * CG_SCENE_PLANE.  max_tiles: the build's capacity (<= 0: 4,194,304;
+ * CG_ERR_NOMEM when the tree needs more).  The result is a tile set (tree and
+ * cell types, no channels; cg_tileset_tree, cg_tileset_destroy). */
+typedef struct cg_tileset cg_tileset;
+cg_status cg_scene_tree_device(const cg_scene_params* sp, int64_t max_tiles, cg_tileset** out);
+
 typedef struct cg_scene cg_scene;
 /* Build the tree, cell types and RHS on the host. */
 cg_status cg_scene_create(const cg_scene_params* sp, cg_scene** out);
@@ -445,8 +458,8 @@ cg_status cg_scene_velocity(const cg_scene* s, const float** uvw);
  * channel an int32 channel number (0-10, the reference's channels,
  * src/PoissonTile.h:12-23), an int32 element type (1 = float32) and
  * float32 data[512n]; cells i-major.  Readers check the magic, the version,
- * the size, the checksum and every type value. */
-typedef struct cg_tileset cg_tileset;
+ * the size, the checksum and every type value.  (cg_tileset is declared with
+ * cg_scene_tree_device above.) */
 cg_status cg_tiles_save(const char* path, const cg_tree_desc* desc, int32_t num_channels, const int32_t* channel_ids,
                         const float* const* channels);
 cg_status cg_tiles_load(const char* path, cg_tileset** out);
