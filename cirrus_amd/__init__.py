@@ -29,6 +29,7 @@ CH_U, CH_V, CH_W = 6, 7, 8
 NUM_CHANNELS = 6
 SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
 NOALLOC_MAX_TILES = 262144  # CG_NOALLOC_MAX_TILES
+ADAPT_REFINE, ADAPT_COARSEN = 1, 2  # CG_ADAPT_*
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 SCENE_UNIFORM, SCENE_SPHERE_BAND, SCENE_PLANE, SCENE_VORTEX = 0, 1, 2, 3
 (KERNEL_GS_SWEEP, KERNEL_APPLY, KERNEL_RESTRICT, KERNEL_PROLONG, KERNEL_JACOBI, KERNEL_UPDATE_XR, KERNEL_COARSE,
@@ -47,7 +48,7 @@ EXPORTED_SYMBOLS = (
     "cg_server_next_result", "cg_server_info", "cg_server_perturb_rhs",
     "cg_refine_params_default", "cg_solve_refined", "cg_get_solution_f64",
     "cg_velocity_divergence", "cg_add_gradient", "cg_project", "cg_set_schedule", "cg_set_level_forms", "cg_autotune", "cg_autotune_refined",
-    "cg_scene_tree_device", "cg_tiles_save", "cg_tiles_load", "cg_tiles_import_holder", "cg_tileset_tree", "cg_tileset_channels",
+    "cg_scene_tree_device", "cg_tree_adapt_device", "cg_tiles_save", "cg_tiles_load", "cg_tiles_import_holder", "cg_tileset_tree", "cg_tileset_channels",
     "cg_tileset_destroy", "cg_amg_prepare", "cg_amg_solve", "cg_amg_coefficients",
 )
 
@@ -182,6 +183,7 @@ def lib():
             "cg_get_solution_f64": (i32, [vp, vp]),
             "cg_scene_create": (i32, [P(SceneParams), P(vp)]),
             "cg_scene_tree_device": (i32, [P(SceneParams), ctypes.c_int64, P(vp)]),
+            "cg_tree_adapt_device": (i32, [P(TreeDesc), P(SceneParams), i32, ctypes.c_int64, P(vp)]),
             "cg_scene_destroy": (i32, [vp]),
             "cg_scene_tree": (i32, [vp, P(TreeDesc), P(vp)]),
             "cg_scene_velocity": (i32, [vp, P(vp)]),
@@ -355,6 +357,18 @@ def scene_tree_device(max_tiles=0, **kw):
     p = scene_params(**kw)
     h = ctypes.c_void_p()
     _check(lib().cg_scene_tree_device(ctypes.byref(p), int(max_tiles), ctypes.byref(h)), "cg_scene_tree_device")
+    return _tileset(h)[0]
+
+
+def tree_adapt_device(tree, ops=ADAPT_REFINE | ADAPT_COARSEN, max_tiles=0, **kw):
+    """A Tree adapted on the device to the target levels of the scene given by
+    the keywords (cg_tree_adapt_device): refinement rounds, then CoarsenStep
+    until none deletes, then ghosts and the scene's cell types."""
+    p = scene_params(**kw)
+    d = tree.desc()
+    h = ctypes.c_void_p()
+    _check(lib().cg_tree_adapt_device(ctypes.byref(d), ctypes.byref(p), int(ops), int(max_tiles), ctypes.byref(h)),
+           "cg_tree_adapt_device")
     return _tileset(h)[0]
 
 

@@ -20,6 +20,17 @@
 //     neighbour has no tile is on the wall; NEUMANN wins) and solid sphere,
 //     NONLEAF cells from their children (finest level first), GHOST cells from
 //     their parents.
+// Adaptation of an existing tree (the active tiles of a host tree loaded into
+// the table) runs the refinement rounds to the scene's target levels, then
+// CoarsenStep (src/PoissonGrid.h:165-314) until a step deletes nothing, as
+// the reference's RefineWithPointValue / CoarsenWithPointValue do
+// (src/FluidEuler.h:68-111), then spawns ghosts and types the cells.  The
+// coarsening runs without ghost tiles: in the reference's step a ghost's flags
+// reach no LEAF or NONLEAF tile (steps 1-2 read only children and NONLEAF
+// neighbours, step 3 reads a LEAF's NONLEAF parent), and its ghost-only parts
+// (step 3 on ghosts, step 4) only delete ghosts whose places the final
+// SpawnGhostTiles call decides again, so the ghosts the final spawning gives
+// are the reference's.
 // The scene's target levels (uniform, sphere band, plane) use only
 // comparisons, +, -, *, and a square root, which round the same way on the
 // host and the device, so the tree is the host builder's tree exactly.
@@ -245,4 +256,69 @@ __global__ __launch_bounds__(512) void k_db_ghost_types(DevTree B, int n) {
   const int gi = x * 8 + (c >> 6), gj = y * 8 + ((c >> 3) & 7), gk = z * 8 + (c & 7);
   const int pi = (gi >> 1) & 7, pj = (gj >> 1) & 7, pk = (gk >> 1) & 7;
   B.ctype[size_t(t) * 512 + c] = B.ctype[size_t(pa) * 512 + ((pi << 6) | (pj << 3) | pk)];
+}
+
+// ---- Adaptation ------------------------------------------------------------
+// CoarsenStep's flags, the reference's status bits (src/PoissonTile.h:7).
+constexpr uint8_t kDcDelete = 2, kDcCoarsen = 4;
+
+// The active tiles of a host tree (arrays already on the device).
+__global__ void k_db_load(DevTree B, const int32_t* __restrict__ level, const int32_t* __restrict__ coord,
+                          const uint8_t* __restrict__ type, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || !(type[t] & (T_LEAF | T_NONLEAF))) return;
+  db_insert(B, level[t], coord[3 * t], coord[3 * t + 1], coord[3 * t + 2], type[t]);
+}
+// CoarsenStep 1 on level l (src/PoissonGrid.h:178-201): a NONLEAF tile is
+// flagged COARSEN when its 8 children are LEAF tiles flagged DELETE.
+__global__ void k_dc_nonleaf(DevTree B, int n, int l) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || B.level[t] != l || B.type[t] != T_NONLEAF) return;
+  const int x = B.coord[3 * t], y = B.coord[3 * t + 1], z = B.coord[3 * t + 2];
+  bool co = true;
+  for (int o = 0; o < 8; o++) {
+    const int c = db_find(B, l + 1, 2 * x + (o & 1), 2 * y + ((o >> 1) & 1), 2 * z + ((o >> 2) & 1));
+    if (c < 0 || B.type[c] != T_LEAF || !(B.mark[c] & kDcDelete)) co = false;
+  }
+  B.mark[t] = co ? kDcCoarsen : 0;
+}
+// CoarsenStep 2 on level l (src/PoissonGrid.h:204-225): a LEAF tile above its
+// target level is flagged DELETE unless a same-level NONLEAF face neighbour
+// is not flagged COARSEN.
+__global__ void k_dc_leaf(DevTree B, cg_scene_params p, int n, int l) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || B.level[t] != l || B.type[t] != T_LEAF) return;
+  const int x = B.coord[3 * t], y = B.coord[3 * t + 1], z = B.coord[3 * t + 2];
+  bool del = db_target(p, l, x, y, z) < l;
+  for (int f = 0; f < 6; f++) {
+    int q[3] = {x, y, z};
+    q[f >> 1] += (f & 1) ? 1 : -1;
+    const int nb = db_find(B, l, q[0], q[1], q[2]);
+    if (nb >= 0 && B.type[nb] == T_NONLEAF && !(B.mark[nb] & kDcCoarsen)) del = false;
+  }
+  B.mark[t] = del ? kDcDelete : 0;
+}
+// CoarsenStep 3 for LEAF tiles (src/PoissonGrid.h:230-254): a LEAF tile goes
+// exactly when its parent is flagged COARSEN.  Parents are NONLEAF tiles,
+// which this launch does not write, so all levels go in one launch.
+__global__ void k_dc_down(DevTree B, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || B.type[t] != T_LEAF) return;
+  const int l = B.level[t];
+  const int pa = l > 0 ? db_find(B, l - 1, B.coord[3 * t] >> 1, B.coord[3 * t + 1] >> 1, B.coord[3 * t + 2] >> 1) : -1;
+  B.mark[t] = (pa >= 0 && (B.mark[pa] & kDcCoarsen)) ? uint8_t(kDcDelete | kDcCoarsen) : uint8_t(0);
+}
+// The end of CoarsenStep (src/PoissonGrid.h:278-313): tiles flagged DELETE
+// are dropped (counted in D.flags[0]), NONLEAF tiles flagged COARSEN become
+// LEAF tiles, the rest keep their type; survivors go into the empty table D.
+__global__ void k_dc_compact(DevTree B, DevTree D, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint8_t m = B.mark[t];
+  if (m & kDcDelete) {
+    atomicAdd(&D.flags[0], 1);
+    return;
+  }
+  const uint8_t ty = (B.type[t] == T_NONLEAF && (m & kDcCoarsen)) ? uint8_t(T_LEAF) : B.type[t];
+  db_insert(D, B.level[t], B.coord[3 * t], B.coord[3 * t + 1], B.coord[3 * t + 2], ty);
 }

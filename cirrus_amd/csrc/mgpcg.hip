@@ -1491,97 +1491,212 @@ void add_gradient(cg_problem& P, float* Pch) {
 
 #include "devbuild.hpp"
 
-// Device-side build of a synthetic scene's octree and cell types (devbuild.hpp);
-// the result goes to the host as a tile set (tree, cell types, no channels).
-void scene_tree_device(const cg_scene_params& p, int64_t max_tiles, TileSet& out) {
-  if (p.kind != CG_SCENE_UNIFORM && p.kind != CG_SCENE_SPHERE_BAND && p.kind != CG_SCENE_PLANE)
-    fail(CG_ERR_INVALID, "cg_scene_tree_device: uniform, sphere band and plane scenes only");
-  if (p.nroot[0] <= 0 || p.nroot[1] <= 0 || p.nroot[2] <= 0 || !(p.h0 > 0) || p.num_layers < 1 || p.num_layers > 19)
-    fail(CG_ERR_INVALID, "cg_scene_tree_device: bad scene parameters");
-  if (int64_t(p.nroot[0]) * p.nroot[1] * p.nroot[2] > (1 << 19))
-    fail(CG_ERR_INVALID, "cg_scene_tree_device: too many root tiles");
-  const int cap = int(max_tiles > 0 ? std::min<int64_t>(max_tiles, int64_t(1) << 26) : int64_t(1) << 22);
-  uint32_t hcap = 1;
-  while (hcap < 2u * uint32_t(cap)) hcap <<= 1;
-  DeviceBuffer b_count{sizeof(int)}, b_flags{2 * sizeof(int)}, b_level{sizeof(int32_t) * size_t(cap)},
-      b_coord{3 * sizeof(int32_t) * size_t(cap)}, b_type{size_t(cap)}, b_mark{size_t(cap)},
-      b_hkeys{sizeof(unsigned long long) * hcap}, b_hvals{sizeof(int32_t) * hcap};
+// The device buffers of one DevTree (devbuild.hpp) of `cap` tiles; its hash
+// table has at least twice as many slots.
+struct DevTreeStore {
+  DeviceBuffer count, flags, level, coord, type, mark, hkeys, hvals;
   DevTree B;
-  B.cap = cap;
-  B.hmask = hcap - 1;
-  B.count = static_cast<int*>(b_count.p);
-  B.flags = static_cast<int*>(b_flags.p);
-  B.level = static_cast<int32_t*>(b_level.p);
-  B.coord = static_cast<int32_t*>(b_coord.p);
-  B.type = static_cast<uint8_t*>(b_type.p);
-  B.mark = static_cast<uint8_t*>(b_mark.p);
-  B.hkeys = static_cast<unsigned long long*>(b_hkeys.p);
-  B.hvals = static_cast<int32_t*>(b_hvals.p);
-  struct Stream {
-    hipStream_t s = nullptr;
-    ~Stream() { if (s) (void)hipStreamDestroy(s); }
-  } st;
-  HIPC(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
-  HIPC(hipMemsetAsync(B.hkeys, 0xff, sizeof(unsigned long long) * hcap, st.s));
-  HIPC(hipMemsetAsync(B.hvals, 0xff, sizeof(int32_t) * hcap, st.s));
-  HIPC(hipMemsetAsync(B.count, 0, sizeof(int), st.s));
-  HIPC(hipMemsetAsync(B.flags, 0, 2 * sizeof(int), st.s));
-  const int nroots = p.nroot[0] * p.nroot[1] * p.nroot[2];
-  k_db_roots<<<unsigned((nroots + 255) / 256), 256, 0, st.s>>>(B, p);
-  HIPC(hipGetLastError());
-  int h_count = 0, h_flags[2] = {0, 0};
-  auto read_state = [&] {
-    HIPC(hipMemcpyAsync(&h_count, B.count, sizeof(int), hipMemcpyDeviceToHost, st.s));
-    HIPC(hipMemcpyAsync(h_flags, B.flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st.s));
-    HIPC(hipStreamSynchronize(st.s));
-    if (h_flags[1]) fail(CG_ERR_NOMEM, "cg_scene_tree_device: more tiles than max_tiles");
-  };
-  read_state();
-  // refinement rounds until no tile is marked (each round adds at least one
-  // level somewhere, so the rounds are bounded by the layers times the ripple)
-  bool settled = false;
-  for (int round = 0; round < 64 * p.num_layers; round++) {
-    const int n = h_count;
-    HIPC(hipMemsetAsync(B.flags, 0, sizeof(int), st.s));
-    k_db_mark<<<unsigned((n + 255) / 256), 256, 0, st.s>>>(B, p, n);
-    HIPC(hipGetLastError());
-    read_state();
-    if (!h_flags[0]) {
-      settled = true;
-      break;
-    }
-    k_db_refine<<<unsigned((n + 255) / 256), 256, 0, st.s>>>(B, n);
+  static uint32_t slots(int cap) {
+    uint32_t h = 1;
+    while (h < 2u * uint32_t(cap)) h <<= 1;
+    return h;
+  }
+  explicit DevTreeStore(int cap)
+      : count{sizeof(int)}, flags{2 * sizeof(int)}, level{sizeof(int32_t) * size_t(cap)},
+        coord{3 * sizeof(int32_t) * size_t(cap)}, type{size_t(cap)}, mark{size_t(cap)},
+        hkeys{sizeof(unsigned long long) * slots(cap)}, hvals{sizeof(int32_t) * slots(cap)} {
+    B.cap = cap;
+    B.hmask = slots(cap) - 1;
+    B.count = static_cast<int*>(count.p);
+    B.flags = static_cast<int*>(flags.p);
+    B.level = static_cast<int32_t*>(level.p);
+    B.coord = static_cast<int32_t*>(coord.p);
+    B.type = static_cast<uint8_t*>(type.p);
+    B.mark = static_cast<uint8_t*>(mark.p);
+    B.hkeys = static_cast<unsigned long long*>(hkeys.p);
+    B.hvals = static_cast<int32_t*>(hvals.p);
+  }
+  void clear(hipStream_t s) {
+    HIPC(hipMemsetAsync(B.hkeys, 0xff, sizeof(unsigned long long) * (size_t(B.hmask) + 1), s));
+    HIPC(hipMemsetAsync(B.hvals, 0xff, sizeof(int32_t) * (size_t(B.hmask) + 1), s));
+    HIPC(hipMemsetAsync(B.count, 0, sizeof(int), s));
+    HIPC(hipMemsetAsync(B.flags, 0, 2 * sizeof(int), s));
+  }
+};
+
+// The host driver of the device-side tree build and adaptation (devbuild.hpp):
+// the tree lives in one of two DevTreeStores (the second receives the
+// survivors of a coarsening step); every step is a kernel, the host reads the
+// tile count and the flags between steps.
+class DevBuilder {
+ public:
+  DevBuilder(const cg_scene_params& p, int64_t max_tiles, const char* who) : p_(p), who_(who) {
+    if (p.kind != CG_SCENE_UNIFORM && p.kind != CG_SCENE_SPHERE_BAND && p.kind != CG_SCENE_PLANE)
+      fail(CG_ERR_INVALID, std::string(who) + ": uniform, sphere band and plane scenes only");
+    if (p.nroot[0] <= 0 || p.nroot[1] <= 0 || p.nroot[2] <= 0 || !(p.h0 > 0) || p.num_layers < 1 || p.num_layers > 19)
+      fail(CG_ERR_INVALID, std::string(who) + ": bad scene parameters");
+    if (int64_t(p.nroot[0]) * p.nroot[1] * p.nroot[2] > (1 << 19))
+      fail(CG_ERR_INVALID, std::string(who) + ": too many root tiles");
+    cap_ = int(max_tiles > 0 ? std::min<int64_t>(max_tiles, int64_t(1) << 26) : int64_t(1) << 22);
+    HIPC(hipStreamCreateWithFlags(&st_.s, hipStreamNonBlocking));
+    cur_ = std::make_unique<DevTreeStore>(cap_);
+    cur_->clear(st_.s);
+  }
+  int tiles() const { return h_count_; }
+
+  // Level 0 of the scene (FluidEuler.h:415-447).
+  void roots() {
+    const int nroots = p_.nroot[0] * p_.nroot[1] * p_.nroot[2];
+    k_db_roots<<<unsigned((nroots + 255) / 256), 256, 0, st_.s>>>(cur_->B, p_);
     HIPC(hipGetLastError());
     read_state();
   }
-  if (!settled) fail(CG_ERR_INVALID, "cg_scene_tree_device: refinement did not settle");
-  const int n0 = h_count;
-  k_db_ghosts<<<unsigned((6 * int64_t(n0) + 255) / 256), 256, 0, st.s>>>(B, n0);
-  HIPC(hipGetLastError());
-  read_state();
-  const int n = h_count;
-  out.h0 = p.h0;
-  out.level.resize(n);
-  out.coord.resize(3 * size_t(n));
-  out.type.resize(n);
-  HIPC(hipMemcpyAsync(out.level.data(), B.level, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost, st.s));
-  HIPC(hipStreamSynchronize(st.s));
-  int maxl = 0;
-  for (int v : out.level) maxl = std::max(maxl, v);
-  DeviceBuffer b_ctype{size_t(n) * 512};
-  B.ctype = static_cast<uint8_t*>(b_ctype.p);
-  HIPC(hipMemsetAsync(B.ctype, 0, size_t(n) * 512, st.s));
-  k_db_leaf_types<<<unsigned(n), 512, 0, st.s>>>(B, p, n);
-  for (int l = maxl; l >= 0; l--) k_db_nonleaf_types<<<unsigned(n), 512, 0, st.s>>>(B, n, l);
-  k_db_ghost_types<<<unsigned(n), 512, 0, st.s>>>(B, n);
-  HIPC(hipGetLastError());
-  out.ctype.resize(size_t(n) * 512);
-  HIPC(hipMemcpyAsync(out.coord.data(), B.coord, 3 * sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost, st.s));
-  HIPC(hipMemcpyAsync(out.type.data(), B.type, size_t(n), hipMemcpyDeviceToHost, st.s));
-  HIPC(hipMemcpyAsync(out.ctype.data(), B.ctype, size_t(n) * 512, hipMemcpyDeviceToHost, st.s));
-  HIPC(hipStreamSynchronize(st.s));
-  out.channel_id.clear();
-  out.channel.clear();
+
+  // The active (LEAF and NONLEAF) tiles of a host tree; its ghosts are left
+  // out and spawned again at the end.
+  void load(const cg_tree_desc& d) {
+    if (d.num_tiles < 0 || (d.num_tiles && (!d.level || !d.coord || !d.type)))
+      fail(CG_ERR_INVALID, who_ + ": bad tree");
+    if (d.h0 != p_.h0) fail(CG_ERR_INVALID, who_ + ": the tree's h0 differs from the scene's");
+    if (d.num_tiles > cap_) fail(CG_ERR_NOMEM, who_ + ": more tiles than max_tiles");
+    const int n = d.num_tiles;
+    for (int i = 0; i < n; i++) {
+      const int l = d.level[i], x = d.coord[3 * i], y = d.coord[3 * i + 1], z = d.coord[3 * i + 2];
+      const uint8_t t = d.type[i];
+      if (t != T_LEAF && t != T_NONLEAF && t != T_GHOST) fail(CG_ERR_INVALID, who_ + ": bad tile type");
+      if (l < 0 || l >= p_.num_layers || !db_key_ok(x, y, z))
+        fail(CG_ERR_INVALID, who_ + ": tile level or coordinate outside the scene's layers");
+      if (t != T_GHOST && ((x >> l) >= p_.nroot[0] || (y >> l) >= p_.nroot[1] || (z >> l) >= p_.nroot[2]))
+        fail(CG_ERR_INVALID, who_ + ": active tile outside the scene's root tiles");
+    }
+    if (n == 0) return;
+    DeviceBuffer lv{sizeof(int32_t) * size_t(n)}, cd{3 * sizeof(int32_t) * size_t(n)}, ty{size_t(n)};
+    HIPC(hipMemcpyAsync(lv.p, d.level, sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice, st_.s));
+    HIPC(hipMemcpyAsync(cd.p, d.coord, 3 * sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice, st_.s));
+    HIPC(hipMemcpyAsync(ty.p, d.type, size_t(n), hipMemcpyHostToDevice, st_.s));
+    k_db_load<<<unsigned((n + 255) / 256), 256, 0, st_.s>>>(cur_->B, static_cast<const int32_t*>(lv.p),
+                                                           static_cast<const int32_t*>(cd.p),
+                                                           static_cast<const uint8_t*>(ty.p), n);
+    HIPC(hipGetLastError());
+    read_state();
+  }
+
+  // Refinement rounds until no tile is marked (each round adds at least one
+  // level somewhere, so the rounds are bounded by the layers times the ripple).
+  void refine() {
+    DevTree& B = cur_->B;
+    for (int round = 0; round < 64 * p_.num_layers; round++) {
+      const int n = h_count_;
+      HIPC(hipMemsetAsync(B.flags, 0, sizeof(int), st_.s));
+      k_db_mark<<<unsigned((n + 255) / 256), 256, 0, st_.s>>>(B, p_, n);
+      HIPC(hipGetLastError());
+      read_state();
+      if (!h_flags_[0]) return;
+      k_db_refine<<<unsigned((n + 255) / 256), 256, 0, st_.s>>>(B, n);
+      HIPC(hipGetLastError());
+      read_state();
+    }
+    fail(CG_ERR_INVALID, who_ + ": refinement did not settle");
+  }
+
+  // CoarsenStep until a step deletes nothing (CoarsenWithPointValue,
+  // FluidEuler.h:90-111); returns the number of steps that deleted tiles.
+  int coarsen() {
+    int steps = 0;
+    while (true) {
+      DevTree& B = cur_->B;
+      const int n = h_count_;
+      if (n == 0) return steps;
+      const unsigned g = unsigned((n + 255) / 256);
+      for (int l = p_.num_layers - 1; l >= 0; l--) {
+        k_dc_nonleaf<<<g, 256, 0, st_.s>>>(B, n, l);
+        k_dc_leaf<<<g, 256, 0, st_.s>>>(B, p_, n, l);
+      }
+      k_dc_down<<<g, 256, 0, st_.s>>>(B, n);
+      if (!next_) next_ = std::make_unique<DevTreeStore>(cap_);
+      next_->clear(st_.s);
+      k_dc_compact<<<g, 256, 0, st_.s>>>(B, next_->B, n);
+      HIPC(hipGetLastError());
+      std::swap(cur_, next_);
+      read_state();
+      const int deleted = h_flags_[0];
+      if (h_count_ + deleted != n) fail(CG_ERR_INVALID, who_ + ": coarsening lost tiles");
+      if (deleted == 0) return steps;
+      steps++;
+    }
+  }
+
+  // Ghosts, cell types, and the tree to the host.
+  void finish(TileSet& out) {
+    DevTree& B = cur_->B;
+    const int n0 = h_count_;
+    if (n0) k_db_ghosts<<<unsigned((6 * int64_t(n0) + 255) / 256), 256, 0, st_.s>>>(B, n0);
+    HIPC(hipGetLastError());
+    read_state();
+    const int n = h_count_;
+    out.h0 = p_.h0;
+    out.level.resize(n);
+    out.coord.resize(3 * size_t(n));
+    out.type.resize(n);
+    out.ctype.resize(size_t(n) * 512);
+    out.channel_id.clear();
+    out.channel.clear();
+    if (n == 0) return;
+    HIPC(hipMemcpyAsync(out.level.data(), B.level, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost, st_.s));
+    HIPC(hipStreamSynchronize(st_.s));
+    int maxl = 0;
+    for (int v : out.level) maxl = std::max(maxl, v);
+    DeviceBuffer b_ctype{size_t(n) * 512};
+    B.ctype = static_cast<uint8_t*>(b_ctype.p);
+    HIPC(hipMemsetAsync(B.ctype, 0, size_t(n) * 512, st_.s));
+    k_db_leaf_types<<<unsigned(n), 512, 0, st_.s>>>(B, p_, n);
+    for (int l = maxl; l >= 0; l--) k_db_nonleaf_types<<<unsigned(n), 512, 0, st_.s>>>(B, n, l);
+    k_db_ghost_types<<<unsigned(n), 512, 0, st_.s>>>(B, n);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(out.coord.data(), B.coord, 3 * sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost, st_.s));
+    HIPC(hipMemcpyAsync(out.type.data(), B.type, size_t(n), hipMemcpyDeviceToHost, st_.s));
+    HIPC(hipMemcpyAsync(out.ctype.data(), B.ctype, size_t(n) * 512, hipMemcpyDeviceToHost, st_.s));
+    HIPC(hipStreamSynchronize(st_.s));
+    B.ctype = nullptr;
+  }
+
+ private:
+  void read_state() {
+    HIPC(hipMemcpyAsync(&h_count_, cur_->B.count, sizeof(int), hipMemcpyDeviceToHost, st_.s));
+    HIPC(hipMemcpyAsync(h_flags_, cur_->B.flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st_.s));
+    HIPC(hipStreamSynchronize(st_.s));
+    if (h_flags_[1]) fail(CG_ERR_NOMEM, who_ + ": more tiles than max_tiles");
+  }
+  struct Stream {
+    hipStream_t s = nullptr;
+    ~Stream() { if (s) (void)hipStreamDestroy(s); }
+  };
+  const cg_scene_params p_;
+  const std::string who_;
+  int cap_ = 0;
+  Stream st_;  // destroyed after the stores (declared before them)
+  std::unique_ptr<DevTreeStore> cur_, next_;
+  int h_count_ = 0, h_flags_[2] = {0, 0};
+};
+
+// Device-side build of a synthetic scene's octree and cell types; the result
+// goes to the host as a tile set (tree, cell types, no channels).
+void scene_tree_device(const cg_scene_params& p, int64_t max_tiles, TileSet& out) {
+  DevBuilder D(p, max_tiles, "cg_scene_tree_device");
+  D.roots();
+  D.refine();
+  D.finish(out);
+}
+
+// Device-side adaptation of a tree to a scene's target levels: refinement
+// rounds, then coarsening steps, then ghosts and the scene's cell types.
+void tree_adapt_device(const cg_tree_desc& d, const cg_scene_params& p, int ops, int64_t max_tiles, TileSet& out) {
+  if (ops & ~(CG_ADAPT_REFINE | CG_ADAPT_COARSEN)) fail(CG_ERR_INVALID, "cg_tree_adapt_device: bad ops");
+  DevBuilder D(p, max_tiles, "cg_tree_adapt_device");
+  D.load(d);
+  if (ops & CG_ADAPT_REFINE) D.refine();
+  if (ops & CG_ADAPT_COARSEN) D.coarsen();
+  D.finish(out);
 }
 
 }  // namespace
@@ -2400,6 +2515,16 @@ cg_status cg_scene_tree_device(const cg_scene_params* sp, int64_t max_tiles, cg_
   *out = nullptr;
   auto S = std::make_unique<cg_tileset>();
   cg_status st = guard([&] { scene_tree_device(*sp, max_tiles, S->s); });
+  if (st == CG_OK) *out = S.release();
+  return st;
+}
+
+cg_status cg_tree_adapt_device(const cg_tree_desc* tree, const cg_scene_params* sp, int32_t ops, int64_t max_tiles,
+                               cg_tileset** out) {
+  if (!tree || !sp || !out) { g_err = "null argument"; return CG_ERR_INVALID; }
+  *out = nullptr;
+  auto S = std::make_unique<cg_tileset>();
+  cg_status st = guard([&] { tree_adapt_device(*tree, *sp, ops, max_tiles, S->s); });
   if (st == CG_OK) *out = S.release();
   return st;
 }

@@ -436,6 +436,22 @@ typedef struct {
 typedef struct cg_tileset cg_tileset;
 cg_status cg_scene_tree_device(const cg_scene_params* sp, int64_t max_tiles, cg_tileset** out);
 
+/* Device-side adaptation of a tree to a scene's target levels, as a
+ * simulation step adapts its grid (src/FluidEuler.h:820-835;
+ * RefineWithPointValue then CoarsenWithPointValue, src/FluidEuler.h:68-111):
+ * with CG_ADAPT_REFINE, refinement rounds with 2:1 face balance until none
+ * refines (RefineLeafsOneStep / SpawnGhostTiles, src/PoissonGrid.h:35-162,
+ * src/PoissonGrid.cu:30-60); with CG_ADAPT_COARSEN, then CoarsenStep
+ * (src/PoissonGrid.h:165-314) until a step deletes nothing; then ghost
+ * spawning and the scene's cell types (walls and solid on LEAF cells,
+ * CalcCellTypesFromLeafs).  The tree's LEAF and NONLEAF tiles are the input
+ * (its ghosts and cell types are not read); its h0 must equal the scene's and
+ * its tiles must lie in the scene's root tiles and layers.  Scene kinds as
+ * cg_scene_tree_device; max_tiles likewise.  The result is a tile set. */
+enum { CG_ADAPT_REFINE = 1, CG_ADAPT_COARSEN = 2 };
+cg_status cg_tree_adapt_device(const cg_tree_desc* tree, const cg_scene_params* sp, int32_t ops, int64_t max_tiles,
+                               cg_tileset** out);
+
 typedef struct cg_scene cg_scene;
 /* Build the tree, cell types and RHS on the host. */
 cg_status cg_scene_create(const cg_scene_params* sp, cg_scene** out);

@@ -30,6 +30,13 @@
 //     CalculateNeighborTiles)      2:1 face balance, ghost-placement rule and
 //                                  neighbour symmetry checked tile by tile;
 //                                  closed-form uniform counts
+//   CoarsenStep, AdaptScene        tests/test_oracle_adapt.py: uniform
+//                                  coarsening against its closed form; the
+//                                  adaptation of any tree to a target whose
+//                                  value on a tile is its children's maximum
+//                                  equals that scene's own build; 2:1 balance,
+//                                  the ghost rule and the coarsening's fixed
+//                                  point on other targets
 //   SceneCellType                  brute-force count of solid / wall cells
 //   CalcCellTypesFromLeafs         dense 2x2x2 block reduction per level;
 //                                  ghost cells equal their parent cell
@@ -128,7 +135,7 @@ inline int fp16_scale_exp(double bb, double cells) {
 // HATileInfo.h:19 / PoissonTile.h:6
 enum : uint8_t { LEAF = 1, NONLEAF = 2, GHOST = 4 };
 enum : uint8_t { DIRICHLET = 1, INTERIOR = 2, NEUMANN = 4 };
-enum : int { REFINE_FLAG = 1 };  // PoissonTile.h:7
+enum : int { REFINE_FLAG = 1, DELETE_FLAG = 2, COARSEN_FLAG = 4 };  // PoissonTile.h:7
 // PoissonTile.h:12-24 (node channels and vorticity/dye are not on the path)
 enum : int { CH_X = 0, CH_B = 1, CH_R = 1, CH_P = 2, CH_AP = 3, CH_Z = 4, CH_TMP = 5,
              CH_U = 6, CH_V = 7, CH_W = 8, NCH = 9 };
@@ -430,6 +437,28 @@ struct Grid {
     else *store[in.tile] = TileData<T>();
     in.coord = b; in.level = level; in.type = type;
   }
+  // HAGrid.h:409-431: empty the slot, then walk the probe run after it and
+  // move back every entry whose home slot does not lie cyclically in
+  // (hole, entry] (backward-shift deletion for linear probing), so that
+  // later lookups never stop early at the hole.
+  void removeTileHost(int level, const Coord& b) {
+    uint32_t i = tileIdx(hashT, uint32_t(level), b);
+    if (hashT[level][i].empty()) return;
+    const uint32_t mask = (1u << log2[level]) - 1u;
+    uint32_t j = i;
+    while (true) {
+      j = (j + 1) & mask;
+      if (hashT[level][j].empty()) break;
+      uint32_t home = hashCoord(hashT[level][j].coord, log2[level]);
+      if ((j > i && (home <= i || home > j)) || (j < i && (home <= i && home > j))) {
+        std::swap(hashT[level][i], hashT[level][j]);
+        i = j;
+      }
+    }
+    store[hashT[level][i].tile].reset();
+    hashT[level][i] = Info();
+    count[level]--;
+  }
   void grow(int level) {
     std::vector<Info> old = hashT[level];
     log2[level]++;
@@ -550,6 +579,87 @@ void IterativeRefine(Grid<T>& g, F target) {
   }
 }
 
+// PoissonGrid.h:165-314.  Flags live in the tiles' status bits; the "kernel"
+// launches of one level read flags that earlier launches wrote and write only
+// their own tiles', so the loops below give the launches' results.  Returns
+// the number of deleted tiles per level.
+template <class T, class F>
+std::vector<int> CoarsenStep(Grid<T>& g, F target) {
+  auto set = [&](const Info& in, int mask, bool v) {
+    int& s = g.tile(in).status;
+    s = v ? (s | mask) : (s & ~mask);
+  };
+  auto status = [&](const Info& in) { return g.tile(in).status; };
+  // upstroke, finest level first
+  for (int level = g.numLayers - 1; level >= 0; level--) {
+    // 1: a NONLEAF tile may coarsen when its 8 children are LEAF tiles flagged DELETE
+    for (const Info& info : g.arrays[level]) {
+      if (!(info.type & NONLEAF)) continue;
+      bool to_coarsen = true;
+      for (int c = 0; c < 8; c++) {
+        Info ci = g.tileInfo(level + 1, childCoord(info.coord, childIndexToOffset(c)));
+        if (ci.isLeaf()) {
+          if (!(status(ci) & DELETE_FLAG)) to_coarsen = false;
+        } else {
+          to_coarsen = false;
+        }
+      }
+      set(info, COARSEN_FLAG, to_coarsen);
+      set(info, DELETE_FLAG, false);
+    }
+    // 2: a LEAF tile above its target level may go unless a same-level NONLEAF
+    // neighbour stays refined
+    for (const Info& info : g.arrays[level]) {
+      if (!(info.type & LEAF)) continue;
+      bool to_delete = target(level, info.coord) < info.level;
+      for (int nb = 0; nb < 6; nb++) {
+        Info ni = g.tileInfo(level, info.coord + neighborOffset(nb));
+        if ((ni.type & NONLEAF) && !(status(ni) & COARSEN_FLAG)) to_delete = false;
+      }
+      set(info, COARSEN_FLAG, false);
+      set(info, DELETE_FLAG, to_delete);
+    }
+  }
+  // downstroke, coarsest level first
+  for (int level = 0; level < g.numLayers; level++) {
+    // 3: LEAF and GHOST tiles go with a coarsening parent (and pass it on)
+    for (const Info& info : g.arrays[level]) {
+      if (!(info.type & (LEAF | GHOST))) continue;
+      Info pi = g.tileInfo(level - 1, parentCoord(info.coord));
+      bool to_delete = !pi.empty() && (status(pi) & COARSEN_FLAG);
+      set(info, DELETE_FLAG, to_delete);
+      if (to_delete) set(info, COARSEN_FLAG, true);
+    }
+    // 4: a GHOST tile without a surviving same-level LEAF neighbour goes
+    for (const Info& info : g.arrays[level]) {
+      if (!(info.type & GHOST)) continue;
+      bool to_delete = true;
+      for (int nb = 0; nb < 6; nb++) {
+        Info ni = g.tileInfo(level, info.coord + neighborOffset(nb));
+        if ((ni.type & LEAF) && !(status(ni) & DELETE_FLAG)) to_delete = false;
+      }
+      if (to_delete) set(info, DELETE_FLAG, true);
+    }
+  }
+  // remove DELETE tiles; COARSEN tiles that stay become LEAF tiles
+  std::vector<int> deleted(g.numLayers, 0);
+  for (int level = 0; level < g.numLayers; level++) {
+    const std::vector<Info> arr = g.arrays[level];
+    for (const Info& info : arr) {
+      const int st = status(info);
+      if (st & DELETE_FLAG) {
+        g.removeTileHost(level, info.coord);
+        deleted[level]++;
+      } else if (st & COARSEN_FLAG) {
+        g.hashT[level][g.tileIdx(g.hashT, uint32_t(level), info.coord)].type = LEAF;
+      }
+    }
+  }
+  g.compressHost();
+  g.syncHostAndDevice();
+  return deleted;
+}
+
 // PoissonGrid.cu:121-134
 template <class T>
 void CalculateNeighborTiles(Grid<T>& g) {
@@ -662,11 +772,10 @@ struct Problem {
   }
 };
 
-// Build the scene's grid: FluidEuler.h:415-447 (roots, compress, ghosts),
-// IterativeRefine (PoissonGrid.h:317), cell typing, CalcCellTypesFromLeafs,
-// CalculateNeighborTiles.
+// The scene's octree: FluidEuler.h:415-447 (roots, compress, ghosts), then
+// IterativeRefine (PoissonGrid.h:317) to the scene's target levels.
 template <class T>
-std::unique_ptr<Problem<T>> BuildScene(const SceneParams& p) {
+std::unique_ptr<Problem<T>> BuildSceneTree(const SceneParams& p) {
   auto P = std::make_unique<Problem<T>>();
   P->params = p;
   P->tubes = make_tubes(p);
@@ -682,7 +791,15 @@ std::unique_ptr<Problem<T>> BuildScene(const SceneParams& p) {
   SpawnGhostTiles(G);
   const auto& tubes = P->tubes;
   IterativeRefine(G, [&](int level, const Coord& b) { return scene_target(p, tubes, level, b); });
-  // leaf cell types
+  return P;
+}
+
+// Cell types of the scene on the tree (LEAF cells, then
+// CalcCellTypesFromLeafs), neighbour tables and the canonical order.
+template <class T>
+void FinishScene(Problem<T>& P) {
+  Grid<T>& G = P.g();
+  const SceneParams& p = P.params;
   for (const Info& info : G.all) {
     if (!(info.type & LEAF)) continue;
     auto& t = G.tile(info);
@@ -690,7 +807,46 @@ std::unique_ptr<Problem<T>> BuildScene(const SceneParams& p) {
   }
   CalcCellTypesFromLeafs(G);
   CalculateNeighborTiles(G);
-  P->build_canon();
+  P.build_canon();
+}
+
+template <class T>
+std::unique_ptr<Problem<T>> BuildScene(const SceneParams& p) {
+  auto P = BuildSceneTree<T>(p);
+  FinishScene(*P);
+  return P;
+}
+
+// Adaptation of the tree of scene `from` to the target levels of scene `to`
+// (same domain, root tiles and layers), as a simulation step does it
+// (FluidEuler.h:820-835): refinement steps with ghost spawning until none
+// refines (RefineWithPointValue, FluidEuler.h:68-88), then CoarsenStep until
+// none deletes and a final SpawnGhostTiles (CoarsenWithPointValue,
+// FluidEuler.h:90-111).  ops: bit 0 refine, bit 1 coarsen.  The result carries
+// scene `to`'s cell types.
+enum : int { ADAPT_REFINE = 1, ADAPT_COARSEN = 2 };
+template <class T>
+std::unique_ptr<Problem<T>> AdaptScene(const SceneParams& from, const SceneParams& to, int ops) {
+  for (int d = 0; d < 3; d++)
+    if (from.nroot[d] != to.nroot[d]) throw std::runtime_error("adapt: root tiles differ");
+  if (from.h0 != to.h0 || from.num_layers != to.num_layers) throw std::runtime_error("adapt: domain differs");
+  auto P = BuildSceneTree<T>(from);
+  P->params = to;
+  P->tubes = make_tubes(to);
+  Grid<T>& G = P->g();
+  const auto& tubes = P->tubes;
+  auto target = [&](int level, const Coord& b) { return scene_target(to, tubes, level, b); };
+  if (ops & ADAPT_REFINE) IterativeRefine(G, target);
+  if (ops & ADAPT_COARSEN) {
+    while (true) {
+      auto cnt = CoarsenStep(G, target);
+      int total = 0;
+      for (int c : cnt) total += c;
+      if (total == 0) break;
+    }
+    SpawnGhostTiles(G);
+  }
+  FinishScene(*P);
   return P;
 }
 
@@ -1662,6 +1818,10 @@ std::unique_ptr<Problem<B>> ConvertProblem(Problem<A>& a) {
   gb.count = ga.count;
   gb.store.clear();
   for (const auto& t : ga.store) {
+    if (!t) {  // a removed tile's slot
+      gb.store.emplace_back();
+      continue;
+    }
     auto d = std::make_unique<TileData<B>>();
     for (int ch = 0; ch < NCH; ch++)
       for (int c = 0; c < 512; c++) d->ch[ch][c] = B(t->ch[ch][c]);

@@ -99,6 +99,22 @@ void* orc_create(const orc_scene_params* cp, int fp64) {
   }
   return o;
 }
+// The tree of scene `from` adapted to scene `to`'s target levels (AdaptScene;
+// ops bit 0 refine, bit 1 coarsen), typed and given its RHS by scene `to`.
+void* orc_create_adapted(const orc_scene_params* cfrom, const orc_scene_params* cto, int ops, int fp64) {
+  OrcHandle* o = new OrcHandle();
+  o->fp64 = fp64;
+  SceneParams from = to_params(cfrom), to = to_params(cto);
+  try {
+    if (fp64) { o->pd = AdaptScene<double>(from, to, ops); SceneRHS(*o->pd); }
+    else { o->pf = AdaptScene<float>(from, to, ops); SceneRHS(*o->pf); }
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "oracle: %s\n", e.what());
+    delete o;
+    return nullptr;
+  }
+  return o;
+}
 void orc_destroy(void* h) { delete static_cast<OrcHandle*>(h); }
 
 int orc_num_tiles(void* h) { return with(h, [](auto& P) { return int(P.canon.size()); }); }

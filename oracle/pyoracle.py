@@ -18,6 +18,7 @@ CH_X, CH_B, CH_R, CH_P, CH_AP, CH_Z, CH_TMP, CH_U, CH_V, CH_W = 0, 1, 1, 2, 3, 4
 SCENE_UNIFORM, SCENE_SPHERE_BAND, SCENE_PLANE, SCENE_VORTEX = 0, 1, 2, 3
 SMOOTH_RBGS, SMOOTH_JACOBI = 0, 1
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
+ADAPT_REFINE, ADAPT_COARSEN = 1, 2
 
 
 def fp16_round(a):
@@ -112,6 +113,8 @@ def lib():
         P = ctypes.POINTER
         L.orc_create.restype = vp
         L.orc_create.argtypes = [P(SceneParams), i]
+        L.orc_create_adapted.restype = vp
+        L.orc_create_adapted.argtypes = [P(SceneParams), P(SceneParams), i, i]
         L.orc_destroy.argtypes = [vp]
         for n in ("orc_num_tiles", "orc_num_leaf_tiles", "orc_max_level"):
             getattr(L, n).restype = i
@@ -165,15 +168,26 @@ def _ptr(a):
 class Oracle:
     """One scene (grid + RHS) held by the oracle, FP32 or FP64."""
 
-    def __init__(self, params=None, fp64=False, **kw):
+    def __init__(self, params=None, fp64=False, _handle=None, **kw):
         self.params = params if params is not None else scene_params(**kw)
         self.fp64 = bool(fp64)
-        self.h = lib().orc_create(ctypes.byref(self.params), int(self.fp64))
+        self.h = _handle if _handle is not None else lib().orc_create(ctypes.byref(self.params), int(self.fp64))
         if not self.h:
             raise RuntimeError("oracle scene build failed")
         self.n = lib().orc_num_tiles(self.h)
         self.n_leaf = lib().orc_num_leaf_tiles(self.h)
         self.max_level = lib().orc_max_level(self.h)
+
+    @classmethod
+    def adapted(cls, frm, to, ops=ADAPT_REFINE | ADAPT_COARSEN, fp64=False):
+        """The tree of scene `frm` (a dict of scene parameters) adapted to the
+        target levels of scene `to`: refinement steps, then coarsening steps
+        (AdaptScene), with scene `to`'s cell types and RHS."""
+        pf, pt = scene_params(**frm), scene_params(**to)
+        h = lib().orc_create_adapted(ctypes.byref(pf), ctypes.byref(pt), int(ops), int(bool(fp64)))
+        if not h:
+            raise RuntimeError("oracle adaptation failed")
+        return cls(params=pt, fp64=fp64, _handle=h)
 
     def close(self):
         if self.h:
