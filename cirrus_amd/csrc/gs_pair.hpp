@@ -212,8 +212,8 @@ __global__ __launch_bounds__(64) void k_gs_pair(Ctx c, const TileRec* __restrict
     // ---- pass 1: the tile's colour-c1 cells
     const int pc = (i + 1) * 10 + (j + 1);
     const bool p1 = ((c1 + i + j) & 1) != 0;
-    const float4 n1 = gs_quad(xa, xb, L.q[pc - 10], L.q[pc + 10], L.q[pc - 1], L.q[pc + 1], vm, vp, ba, mask, T.bc, h,
-                              i, j, p1);
+    const float4 n1 = gs_quad(xa, xb, L.q[pc - 10], L.q[pc + 10], L.q[pc - 1], L.q[pc + 1], vm, vp, ba, quad_mask(mask, p1), T.bc,
+                              h, i, j, p1);
     const float4 x1 = make_float4(vround<V>(n1.x), vround<V>(n1.y), vround<V>(n1.z), vround<V>(n1.w));
 
     // the ring pencil's colour-c1 cells as the neighbour's first pass leaves them
@@ -231,7 +231,8 @@ __global__ __launch_bounds__(64) void k_gs_pair(Ctx c, const TileRec* __restrict
       else if (g == 1) { XIM = ownq; XIP = rdeep; XJM = along_m; XJP = along_p; }
       else if (g == 2) { XIM = along_m; XIP = along_p; XJM = rdeep; XJP = ownq; }
       else { XIM = along_m; XIP = along_p; XJM = ownq; XJP = rdeep; }
-      const float4 nr = gs_quad(rc1, rc2, XIM, XIP, XJM, XJP, rp ? 0.f : kv, rp ? kv : 0.f, rb, rm, rnb.bc, h, ri, rj, rp);
+      const float4 nr = gs_quad(rc1, rc2, XIM, XIP, XJM, XJP, rp ? 0.f : kv, rp ? kv : 0.f, rb, quad_mask(rm, rp), rnb.bc, h, ri,
+                                rj, rp);
       r1 = make_float4(vround<V>(nr.x), vround<V>(nr.y), vround<V>(nr.z), vround<V>(nr.w));
     }
     // the plane cell next to the lane's colour-c2 end cell, as its first pass leaves it
@@ -265,7 +266,7 @@ __global__ __launch_bounds__(64) void k_gs_pair(Ctx c, const TileRec* __restrict
     // ---- pass 2: the tile's colour-c2 cells
     const bool p2 = !p1;
     const float4 n2 = gs_quad(xb, x1, L.q[pc - 10], L.q[pc + 10], L.q[pc - 1], L.q[pc + 1], upd_m ? kh : 0.f,
-                              upd_p ? kh : 0.f, bb, mask, T.bc, h, i, j, p2);
+                              upd_p ? kh : 0.f, bb, quad_mask(mask, p2), T.bc, h, i, j, p2);
     V* tile = XO + size_t(self) * 512;
     store4(tile + (c1 << 8) + lane * 4, x1);
     store4(tile + (c2 << 8) + lane * 4, n2);

@@ -78,6 +78,8 @@ struct Ctx {
   const uint8_t* ctype;
   const uint8_t* fmask;  // per cell: bit e (faces x-, x+, y-, y+, z-, z+) set if the face is not cut by a
                          // NEUMANN cell on either side, bit 6 set on INTERIOR cells (see k_face_mask)
+  const uint32_t* fsplit;  // the same masks by colour: per tile [colour][lane], byte m = cell k = 2m + p
+                           // of the lane's pencil, p = (colour + i + j) & 1 (k_face_mask)
   float* x;      // pressure / CG x
   float* r;      // rhs b, then residual r
   float* p;      // CG direction
@@ -250,7 +252,12 @@ __device__ __forceinline__ uint64_t load_ct8(const uint8_t* __restrict__ p) {
 // A wavefront-uniform value (keeps record fields in scalar registers).
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 // v[g] for a lane-dependent g in 0..3, as register selects
-__device__ __forceinline__ int pick4(int g, int a, int b, int c, int d) { return g == 0 ? a : (g == 1 ? b : (g == 2 ? c : d)); }
+// One of four values by g in 0..3 (two selects on g's bits: no branches).
+__device__ __forceinline__ int pick4(int g, int a, int b, int c, int d) {
+  const int lo = (g & 1) ? b : a;
+  const int hi = (g & 1) ? d : c;
+  return (g & 2) ? hi : lo;
+}
 
 // Wavefront sum of a double in the fixed xor-butterfly order (masks 32..1);
 // every lane ends with the same bits.
@@ -369,6 +376,13 @@ __device__ __forceinline__ void store_quad_split(V* __restrict__ tile, int pl, i
 // a tile has been issued; a widening inside the branch that loads a value would
 // make that branch wait for its load.  hold() puts an empty asm the compiler
 // cannot see through on the raw bits, so that it cannot move the widening up.
+// A device array of zeros as large as one tile of FP32 values: loads of values
+// that are 0 by definition read it (L2-resident) instead of being skipped.
+// Nothing writes it; it is not declared const so that it sits in global
+// memory like the arrays it stands in for (one kind of load for both).
+__device__ __align__(16) float kZeroPage[512] = {};
+template <class V> __device__ __forceinline__ const V* zero_page() { return reinterpret_cast<const V*>(kZeroPage); }
+
 template <class V> struct RawV;
 template <> struct RawV<float> { typedef float4 Q; typedef float S; typedef float2 S2; };
 template <> struct RawV<bf16> { typedef uint2 Q; typedef uint32_t S; typedef uint32_t S2; };
@@ -450,6 +464,16 @@ struct TileLDS {
 
 // Face-mask bits of a cell (Ctx::fmask).
 enum : int { M_INT = 64 };
+// A cell with all six faces uncut and INTERIOR.
+constexpr uint32_t M_OPEN_INT = 0x7fu;
+
+// The masks of a pencil's four colour cells k = p, 2+p, 4+p, 6+p (byte m = cell
+// 2m + p) from its eight (byte k = cell k).
+__device__ __forceinline__ uint32_t quad_mask(uint64_t m, bool p) {
+  const uint64_t s = p ? (m >> 8) : m;
+  return uint32_t(s & 0xffu) | uint32_t((s >> 8) & 0xff00u) | uint32_t((s >> 16) & 0xff0000u) |
+         uint32_t((s >> 24) & 0xff000000u);
+}
 
 struct Stencil {
   float own[8];
@@ -735,7 +759,7 @@ __device__ __forceinline__ void gs_pencil(const NbrPencils& N, const Stencil& S,
 // up with the lane's updated quad), the lane's other-colour quad XO and the
 // k-halo values xkm (cell (i,j,-1), used when p = 0) and xkp ((i,j,8), p = 1).
 __device__ __forceinline__ float4 gs_quad(float4 X0, float4 XO, float4 XIM, float4 XIP, float4 XJM, float4 XJP,
-                                          float xkm, float xkp, float4 B, uint64_t mask, const float bcv[7], float h,
+                                          float xkm, float xkp, float4 B, uint32_t mq, const float bcv[7], float h,
                                           int i, int j, bool p) {
   // k-1 and k+1 neighbours of cell 2m+p
   const float4 XKM = sel4(p, make_float4(xkm, XO.x, XO.y, XO.z), XO);
@@ -745,11 +769,43 @@ __device__ __forceinline__ float4 gs_quad(float4 X0, float4 XO, float4 XIM, floa
   const float bip = (i == 7) ? bcv[3] : bi;
   const float bjm = (j == 0) ? bcv[1] : bi;
   const float bjp = (j == 7) ? bcv[4] : bi;
-  const int sh = p ? 8 : 0;
   float nv[4];
+  if (__all(mq == M_OPEN_INT * 0x01010101u)) {
+    // Every updated cell of the wavefront is INTERIOR with six uncut faces:
+    // each face coefficient is its base coefficient, so the diagonal sum
+    // depends on the cell only through the k faces.  Its sums are formed once
+    // per lane in the same order as cell_row_m forms them (faces x-, x+, y-,
+    // y+, z-, z+; adding a zero is exact), so every value is the general path's.
+    const float d4 = (((0.f + bim) + bip) + bjm) + bjp;
+    const float dmid = (d4 + bi) + bi;
+    const float dlo = (d4 + bcv[2]) + bi;   // cell k = 0 (p = 0, m = 0)
+    const float dhi = (d4 + bi) + bcv[5];   // cell k = 7 (p = 1, m = 3)
+    const float gmid = ((dmid * h) * h) * h;
+    const float glo = ((dlo * h) * h) * h;
+    const float ghi = ((dhi * h) * h) * h;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const float bkm = (!p && m == 0) ? bcv[2] : bi;
+      const float bkp = (p && m == 3) ? bcv[5] : bi;
+      const float x0 = comp4(X0, m);
+      float s = 0.f;
+      s = __builtin_fmaf(bim, x0 - comp4(XIM, m), s);
+      s = __builtin_fmaf(bip, x0 - comp4(XIP, m), s);
+      s = __builtin_fmaf(bjm, x0 - comp4(XJM, m), s);
+      s = __builtin_fmaf(bjp, x0 - comp4(XJP, m), s);
+      s = __builtin_fmaf(bkm, x0 - comp4(XKM, m), s);
+      s = __builtin_fmaf(bkp, x0 - comp4(XKP, m), s);
+      const float ax = ((s * h) * h) * h;
+      const float dg = (!p && m == 0) ? glo : ((p && m == 3) ? ghi : gmid);
+      float delta = 0.f;
+      if (dg != 0.f) delta = (comp4(B, m) - ax) / dg;
+      nv[m] = x0 + delta;
+    }
+    return make_float4(nv[0], nv[1], nv[2], nv[3]);
+  }
 #pragma unroll
   for (int m = 0; m < 4; m++) {
-    const uint32_t mb = (m < 2 ? uint32_t(mask) : uint32_t(mask >> 32)) >> (16 * (m & 1) + sh);  // bits 0..6 used
+    const uint32_t mb = mq >> (8 * m);  // bits 0..6 used
     const float bkm = (!p && m == 0) ? bcv[2] : bi;
     const float bkp = (p && m == 3) ? bcv[5] : bi;
     const float x0 = comp4(X0, m);
@@ -784,7 +840,7 @@ struct GsLDS {
 struct GsIn {
   float4 xc, xo, B, halo;
   float4 r0, r1;  // with F_DOT_RZ on a LEAF tile: r of the lane's pencil, cells 0..3 and 4..7
-  uint64_t mask;
+  uint32_t mask;  // the face masks of the lane's four cells of the pass's colour
   float xkm, xkp;
   int32_t done;   // the problem's done flag, loaded with the tile's data (used with F_CHECK_DONE only)
 };
@@ -793,7 +849,7 @@ struct GsRaw {
   typename RawV<V>::Q xc, xo, B, halo;
   typename RawV<V>::S xkm, xkp;
   float4 r0, r1;
-  uint64_t mask;
+  uint32_t mask;
   int32_t done;
 };
 // DOT: the pass carries F_DOT_RZ (a separate instance, so that the other
@@ -804,43 +860,49 @@ __device__ __forceinline__ GsRaw<V> gs_issue(const Ctx& c, const TileRec& T, con
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const bool zero_all = flags & F_ZERO_X, ghost_zero = flags & F_GHOST_ZERO;
   const int hc = color << 8, ho = (color ^ 1) << 8;
-  const V* tile = X + size_t(T.self) * 512;
-  GsRaw<V> g{};  // zero bits: +0 in every storage type
+  // Every load below is unconditional: a value that is 0 by definition (x from
+  // a zero guess, outside the grid, a ghost in the down-sweep, the halo of
+  // lanes 32..63) is read from the zero page instead of skipped, so no load
+  // sits in a branch whose merge would make the wavefront wait for it.
+  const V* zp = zero_page<V>();
+  const V* tile = zero_all ? zp : X + size_t(T.self) * 512;
+  GsRaw<V> g;
   // one unconditional scalar load issued with the others, tested after the LDS
   // stage; without F_CHECK_DONE the face masks stand in as a readable address
   const int32_t* dp = (flags & F_CHECK_DONE) ? &c.ps[T.prob].done : reinterpret_cast<const int32_t*>(c.fmask);
   g.done = *dp;
-  if (!zero_all) {
-    g.xc = load_q_raw(tile + hc + lane * 4);
-    g.xo = load_q_raw(tile + ho + lane * 4);
-  }
-  g.mask = load_ct8(c.fmask + size_t(T.self) * 512 + lane * 8);
-  if (DOT && tr_type(T) == T_LEAF) {
-    const float4* rp = reinterpret_cast<const float4*>(c.r + size_t(T.self) * 512 + lane * 8);
-    g.r0 = rp[0];
-    g.r1 = rp[1];
+  g.xc = load_q_raw(tile + hc + lane * 4);
+  g.xo = load_q_raw(tile + ho + lane * 4);
+  g.mask = c.fsplit[size_t(T.self) * 128 + color * 64 + lane];
+  if (DOT) {
+    // r of the lane's pencil on LEAF tiles (zeros otherwise)
+    const float* rp = tr_type(T) == T_LEAF ? c.r + size_t(T.self) * 512 : zero_page<float>();
+    const float4* r4 = reinterpret_cast<const float4*>(rp + lane * 8);
+    g.r0 = r4[0];
+    g.r1 = r4[1];
   }
   const V* src = (flags & F_VCYCLE) ? vc_rhs<V>(c) : F;
   g.B = load_q_raw(src + size_t(T.self) * 512 + hc + lane * 4);
   // halo ring: lanes 0..31 fetch the other-colour quad of one neighbour pencil
   //   q = 0: x- (pencil (7,t) of nbr 0), 1: x+ ((0,t) of nbr 3), 2: y- ((t,7) of nbr 1), 3: y+ ((t,0) of nbr 4)
   // x = 0 outside the grid (GMGSolver.cu:120-122)
-  if (lane < 32) {
-    const int q = lane >> 3, t = lane & 7;
-    const int f = (q == 0) ? 0 : (q == 1) ? 3 : (q == 2) ? 1 : 4;
+  {
+    const int q = (lane >> 3) & 3, t = lane & 7;
+    const int f = pick4(q, 0, 3, 1, 4);
     const int n = pick4(q, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
-    const int src_pl = (q == 0) ? (56 + t) : (q == 1) ? t : (q == 2) ? (t * 8 + 7) : (t * 8);
+    const int src_pl = pick4(q, 56 + t, t, t * 8 + 7, t * 8);
     const bool gh = tr_ntype(T, f) == T_GHOST;
-    if (n >= 0 && !(zero_all || (ghost_zero && gh))) g.halo = load_q_raw(X + size_t(n) * 512 + ho + src_pl * 4);
+    const bool ok = lane < 32 && n >= 0 && !(zero_all || (ghost_zero && gh));
+    g.halo = load_q_raw(ok ? X + size_t(n) * 512 + ho + src_pl * 4 : zp);
   }
   // k halos: cell (i,j,7) of the z- neighbour and (i,j,0) of the z+ neighbour
   {
     int n = T.nbr[2];
-    bool gh = tr_ntype(T, 2) == T_GHOST;
-    if (n >= 0 && !(zero_all || (ghost_zero && gh))) g.xkm = load_s_raw(X + size_t(n) * 512 + split_cell(i, j, 7));
+    bool ok = n >= 0 && !(zero_all || (ghost_zero && tr_ntype(T, 2) == T_GHOST));
+    g.xkm = load_s_raw(ok ? X + size_t(n) * 512 + split_cell(i, j, 7) : zp);
     n = T.nbr[5];
-    gh = tr_ntype(T, 5) == T_GHOST;
-    if (n >= 0 && !(zero_all || (ghost_zero && gh))) g.xkp = load_s_raw(X + size_t(n) * 512 + split_cell(i, j, 0));
+    ok = n >= 0 && !(zero_all || (ghost_zero && tr_ntype(T, 5) == T_GHOST));
+    g.xkp = load_s_raw(ok ? X + size_t(n) * 512 + split_cell(i, j, 0) : zp);
   }
   return g;
 }
@@ -1376,7 +1438,7 @@ __global__ __launch_bounds__(64) void k_gs_first(Ctx c, const TileRec* __restric
   // ---- pass 1: the tile's colour-0 cells from x = 0 (gs_quad as the colour-0 pass runs it)
   const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
   const bool p0 = ((i + j) & 1) != 0, p1 = !p0;
-  const float4 n0 = gs_quad(z4, z4, z4, z4, z4, z4, 0.f, 0.f, quad_f<V>(rb0), mask, T.bc, T.h, i, j, p0);
+  const float4 n0 = gs_quad(z4, z4, z4, z4, z4, z4, 0.f, 0.f, quad_f<V>(rb0), quad_mask(mask, p0), T.bc, T.h, i, j, p0);
   const float4 xr = make_float4(vround<V>(n0.x), vround<V>(n0.y), vround<V>(n0.z), vround<V>(n0.w));
   // the halo's colour-0 values as that pass leaves them
   const float4 hbf = quad_f<V>(hb);
@@ -1391,7 +1453,7 @@ __global__ __launch_bounds__(64) void k_gs_first(Ctx c, const TileRec* __restric
   // ---- pass 2: colour 1, its own values 0 before the update
   const int pc = (i + 1) * 10 + (j + 1);
   const float4 n1 = gs_quad(z4, xr, L.xo[pc - 10], L.xo[pc + 10], L.xo[pc - 1], L.xo[pc + 1], xkm, xkp,
-                            quad_f<V>(rb1), mask, T.bc, T.h, i, j, p1);
+                            quad_f<V>(rb1), quad_mask(mask, p1), T.bc, T.h, i, j, p1);
   V* tile = X + size_t(T.self) * 512;
   store4(tile + lane * 4, xr);
   store4(tile + 256 + lane * 4, n1);
@@ -1913,7 +1975,7 @@ __global__ __launch_bounds__(64) void k_permute_channel(const float* __restrict_
 // grid is DIRICHLET, GMGSolver.cu:120-122); bit 6 when c is INTERIOR.  Cell
 // types do not change during a solve, so the masks are formed once.
 __global__ __launch_bounds__(64) void k_face_mask(const uint8_t* __restrict__ ctype, uint8_t* __restrict__ fmask,
-                                                  const TileRec* __restrict__ list, int n) {
+                                                  uint32_t* __restrict__ fsplit, const TileRec* __restrict__ list, int n) {
   const int flags = 0;
   TILE_KERNEL_PROLOGUE
   __shared__ uint64_t cs[100];
@@ -1945,6 +2007,9 @@ __global__ __launch_bounds__(64) void k_face_mask(const uint8_t* __restrict__ ct
     out |= uint64_t(m) << (8 * k);
   }
   *reinterpret_cast<uint64_t*>(fmask + base) = out;
+  const bool p0 = ((i + j) & 1) != 0;  // colour 0 holds the cells k = 2m + p0
+  fsplit[size_t(T.self) * 128 + lane] = quad_mask(out, p0);
+  fsplit[size_t(T.self) * 128 + 64 + lane] = quad_mask(out, !p0);
 }
 
 // Seeded update of a right-hand side, for a synthetic client of the serving

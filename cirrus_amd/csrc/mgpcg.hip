@@ -103,7 +103,8 @@ struct cg_problem {
 
   // device
   uint8_t* d_ctype = nullptr;
-  uint8_t* d_fmask = nullptr;
+  uint8_t* d_fmask = nullptr;   // face masks, then (same allocation) the colour-split masks d_fsplit
+  uint32_t* d_fsplit = nullptr;
   float* d_diag = nullptr;  // the colour sweep's diagonal per cell (k_diag), for k_gs_first
   float* d_amgc = nullptr;    // AMG stored coefficients: off x, off y, off z, diag (4 x cells; cg_amg_prepare)
   uint8_t* d_amgct = nullptr; // AMG cell types
@@ -148,6 +149,7 @@ struct cg_problem {
     Ctx c;
     c.ctype = d_ctype;
     c.fmask = d_fmask;
+    c.fsplit = d_fsplit;
     c.diag = d_diag;
     c.x = d_ch[CG_CH_X];
     c.r = r;
@@ -592,7 +594,8 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   HIPC(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
   const size_t cells = size_t(n) * 512;
   HIPC(hipMalloc(&P.d_ctype, cells));
-  HIPC(hipMalloc(&P.d_fmask, cells));
+  HIPC(hipMalloc(&P.d_fmask, 2 * cells));
+  P.d_fsplit = reinterpret_cast<uint32_t*>(P.d_fmask + cells);
   HIPC(hipMalloc(&P.d_diag, cells * sizeof(float)));
   for (int c = 0; c < CG_NUM_CHANNELS; c++) {
     HIPC(hipMalloc(&P.d_ch[c], cells * sizeof(float)));
@@ -624,7 +627,7 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   k_permute_ctype<<<unsigned(n), 64, 0, P.stream>>>(P.d_fmask, P.d_ext, n, P.d_ctype);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(P.stream));
-  k_face_mask<<<P.all.n, 64, 0, P.stream>>>(P.d_ctype, P.d_fmask, P.L(P.all), P.all.n);
+  k_face_mask<<<P.all.n, 64, 0, P.stream>>>(P.d_ctype, P.d_fmask, P.d_fsplit, P.L(P.all), P.all.n);
   k_diag<<<P.all.n, 64, 0, P.stream>>>(P.ctx(), P.L(P.all), P.all.n, P.d_diag);
   HIPC(hipGetLastError());
   HIPC(hipHostMalloc(&P.h_pin, 2 * sizeof(int) * nprob, hipHostMallocDefault));

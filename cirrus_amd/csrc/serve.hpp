@@ -79,7 +79,8 @@ struct cg_server {
   // device
   cg_problem view;  // the kernels' view of the assembled active set (owner = false)
   uint8_t* d_ctype = nullptr;
-  uint8_t* d_fmask = nullptr;
+  uint8_t* d_fmask = nullptr;   // face masks, then (same allocation) the colour-split masks d_fsplit
+  uint32_t* d_fsplit = nullptr;
   float* d_diag = nullptr;
   float* d_ch[CG_NUM_CHANNELS] = {};
   float* d_zb = nullptr;
@@ -178,7 +179,8 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   HIPC(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
   const size_t cells = size_t(S.cap_tiles) * 512;
   HIPC(hipMalloc(&S.d_ctype, cells));
-  HIPC(hipMalloc(&S.d_fmask, cells));
+  HIPC(hipMalloc(&S.d_fmask, 2 * cells));
+  S.d_fsplit = reinterpret_cast<uint32_t*>(S.d_fmask + cells);
   HIPC(hipMalloc(&S.d_diag, cells * sizeof(float)));
   for (int c = 0; c < CG_NUM_CHANNELS; c++) HIPC(hipMalloc(&S.d_ch[c], cells * sizeof(float)));
   HIPC(hipMalloc(&S.d_zb, cells * sizeof(float)));
@@ -230,6 +232,7 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   P.stream = S.stream;
   P.d_ctype = S.d_ctype;
   P.d_fmask = S.d_fmask;
+  P.d_fsplit = S.d_fsplit;
   P.d_diag = S.d_diag;
   for (int c = 0; c < CG_NUM_CHANNELS; c++) P.d_ch[c] = S.d_ch[c];
   P.d_zb = S.d_zb;
@@ -327,7 +330,7 @@ void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
   for (int i = 0; i < n; i++) std::memcpy(&ct[size_t(i) * 512], d->cell_type + size_t(B.ext[i]) * 512, 512);
   HIPC(hipMemcpy(S.d_ctype + size_t(tb) * 512, ct.data(), ct.size(), hipMemcpyHostToDevice));
   // face masks on the server's stream, ahead of the instance's first request
-  k_face_mask<<<I->all.n, 64, 0, S.stream>>>(S.d_ctype, S.d_fmask, I->L(I->all), I->all.n);
+  k_face_mask<<<I->all.n, 64, 0, S.stream>>>(S.d_ctype, S.d_fmask, S.d_fsplit, I->L(I->all), I->all.n);
   k_diag<<<I->all.n, 64, 0, S.stream>>>(S.view.ctx(), I->L(I->all), I->all.n, S.d_diag);
   HIPC(hipGetLastError());
   HIPC(hipMemsetAsync(S.d_ch[CG_CH_X] + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float), S.stream));
