@@ -36,10 +36,9 @@ struct PairLDS {
   PairNbr nb[6];
 };
 
-// ring group g (0: x-, 1: x+, 2: y-, 3: y+) -> face index, the ring pencil's
-// (i,j) in the neighbour, its deeper pencil, the tile's own adjacent pencil and
-// the LDS index of ring position t
-__device__ __forceinline__ int ring_face(int g) { return g == 0 ? 0 : (g == 1 ? 3 : (g == 2 ? 1 : 4)); }
+// ring group g (0: x-, 1: x+, 2: y-, 3: y+) -> (ring_face, kernels.hpp) the ring
+// pencil's (i,j) in the neighbour, its deeper pencil, the tile's own adjacent
+// pencil and the LDS index of ring position t
 __device__ __forceinline__ void ring_ij(int g, int t, int& ri, int& rj) {
   ri = g == 0 ? 7 : (g == 1 ? 0 : t);
   rj = g == 2 ? 7 : (g == 3 ? 0 : t);

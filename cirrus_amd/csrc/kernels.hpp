@@ -496,46 +496,69 @@ struct StageRaw {
   RawPencil<V> own, halo;
   typename RawV<V>::S km, kp;
   uint64_t m;
-  int dst, src;
-  bool own_zero, hok, kokm, kokp;
 };
-// The loads of stage_tile on the colour-split arrays, as raw bits: the own
-// pencil, the face masks, the halo pencil of lanes 0..31 and the two k halos.
-// A missing neighbour (or one whose values count as 0) is read at the tile
-// itself and the value dropped afterwards, so that no load sits in an inner
-// branch.
-template <class V>
+// A pencil's raw loads in either layout: colour-split (SPLIT, both halves'
+// quads) or natural (cells 0..3 and 4..7).
+template <bool SPLIT, class V>
+__device__ __forceinline__ RawPencil<V> load_pencil_any_raw(const V* __restrict__ tile, int pl) {
+  if (SPLIT) return load_pencil_split_raw(tile, pl);
+  RawPencil<V> r;
+  r.h0 = load_q_raw(tile + pl * 8);
+  r.h1 = load_q_raw(tile + pl * 8 + 4);
+  return r;
+}
+template <bool SPLIT, class V>
+__device__ __forceinline__ void pencil_any_f(const RawPencil<V>& r, int pl, float4& a, float4& b) {
+  if (SPLIT) {
+    pencil_split_f(r, pl, a, b);
+  } else {
+    a = quad_f<V>(r.h0);
+    b = quad_f<V>(r.h1);
+  }
+}
+// The halo ring's lanes: lane g*8 + t (g < 4) fetches one neighbour pencil
+//   g = 0: x- (pencil (7,t) of nbr 0), 1: x+ ((0,t) of nbr 3), 2: y- ((t,7) of nbr 1), 3: y+ ((t,0) of nbr 4)
+// into LDS slot ring_dst (pencil index (i+1)*10 + (j+1) of the 10x10 frame).
+__device__ __forceinline__ int ring_face(int g) { return pick4(g, 0, 3, 1, 4); }
+__device__ __forceinline__ int ring_src(int g, int t) { return pick4(g, 56 + t, t, t * 8 + 7, t * 8); }
+__device__ __forceinline__ int ring_dst(int g, int t) { return pick4(g, t + 1, 91 + t, (t + 1) * 10, (t + 1) * 10 + 9); }
+
+// The loads of stage_tile, as raw bits: the own pencil, the face masks, the
+// halo pencil of lanes 0..31 and the two k halos.  Every load is
+// unconditional: a value that counts as 0 (a zero guess, outside the grid, a
+// ghost in the down-sweep, lanes 32..63 of the ring) is read from the zero
+// page, so no load sits in a branch whose merge would make the wavefront wait.
+template <bool SPLIT, class V>
 __device__ __forceinline__ StageRaw<V> stage_issue(const Ctx& c, const TileRec& T, const V* __restrict__ X,
                                                    const V* __restrict__ XG, bool zero_all, bool ghost_zero) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const bool own_ghost = tr_type(T) == T_GHOST;
-  StageRaw<V> r{};
-  r.own_zero = zero_all || (ghost_zero && own_ghost);
-  if (!r.own_zero) r.own = load_pencil_split_raw((own_ghost ? XG : X) + size_t(T.self) * 512, lane);
+  const V* zp = zero_page<V>();
+  StageRaw<V> r;
+  const bool own_zero = zero_all || (ghost_zero && own_ghost);
+  r.own = load_pencil_any_raw<SPLIT>(own_zero ? zp : (own_ghost ? XG : X) + size_t(T.self) * 512, lane);
   r.m = load_ct8(c.fmask + size_t(T.self) * 512 + lane * 8);
-  // halo ring: lanes 0..31 fetch one neighbour pencil each
-  //   g = 0: x- (pencil (7,t) of nbr 0), 1: x+ ((0,t) of nbr 3), 2: y- ((t,7) of nbr 1), 3: y+ ((t,0) of nbr 4)
-  if (lane < 32) {
-    const int g = lane >> 3, t = lane & 7;
-    const int f = (g == 0) ? 0 : (g == 1) ? 3 : (g == 2) ? 1 : 4;
+  {
+    const int g = (lane >> 3) & 3, t = lane & 7;
+    const int f = ring_face(g);
     const int n = pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
-    r.src = (g == 0) ? (56 + t) : (g == 1) ? t : (g == 2) ? (t * 8 + 7) : (t * 8);
-    r.dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
     const bool gh = tr_ntype(T, f) == T_GHOST;
-    r.hok = n >= 0 && !(zero_all || (ghost_zero && gh));  // else x = 0 (outside the grid: GMGSolver.cu:120-122)
-    r.halo = load_pencil_split_raw((r.hok && gh ? XG : X) + size_t(r.hok ? n : T.self) * 512, r.src);
+    // else x = 0 (outside the grid: GMGSolver.cu:120-122)
+    const bool ok = lane < 32 && n >= 0 && !(zero_all || (ghost_zero && gh));
+    r.halo = load_pencil_any_raw<SPLIT>(ok ? (gh ? XG : X) + size_t(n) * 512 : zp, ok ? ring_src(g, t) : 0);
   }
   // k halos: cell (i,j,7) of the z- neighbour, (i,j,0) of the z+ neighbour
+  const int o7 = SPLIT ? split_cell(i, j, 7) : lane * 8 + 7, o0 = SPLIT ? split_cell(i, j, 0) : lane * 8;
   const int nm = T.nbr[2], np = T.nbr[5];
   const bool ghm = tr_ntype(T, 2) == T_GHOST, ghp = tr_ntype(T, 5) == T_GHOST;
-  r.kokm = nm >= 0 && !(zero_all || (ghost_zero && ghm));
-  r.kokp = np >= 0 && !(zero_all || (ghost_zero && ghp));
-  r.km = load_s_raw((r.kokm && ghm ? XG : X) + size_t(r.kokm ? nm : T.self) * 512 + split_cell(i, j, 7));
-  r.kp = load_s_raw((r.kokp && ghp ? XG : X) + size_t(r.kokp ? np : T.self) * 512 + split_cell(i, j, 0));
+  const bool okm = nm >= 0 && !(zero_all || (ghost_zero && ghm));
+  const bool okp = np >= 0 && !(zero_all || (ghost_zero && ghp));
+  r.km = load_s_raw(okm ? (ghm ? XG : X) + size_t(nm) * 512 + o7 : zp);
+  r.kp = load_s_raw(okp ? (ghp ? XG : X) + size_t(np) * 512 + o0 : zp);
   return r;
 }
 // The rest of stage_tile: one wait, then the values into LDS and the stencil.
-template <class V>
+template <bool SPLIT, class V>
 __device__ __forceinline__ void stage_commit(StageRaw<V>& r, const TileRec& T, const float bcv[7], TileLDS& L,
                                              Stencil& S) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
@@ -543,8 +566,8 @@ __device__ __forceinline__ void stage_commit(StageRaw<V>& r, const TileRec& T, c
   hold_all(r.halo);
   hold_all(r.km);
   hold_all(r.kp);
-  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
-  if (!r.own_zero) pencil_split_f(r.own, lane, a, b);
+  float4 a, b;
+  pencil_any_f<SPLIT>(r.own, lane, a, b);
   S.own[0] = a.x; S.own[1] = a.y; S.own[2] = a.z; S.own[3] = a.w;
   S.own[4] = b.x; S.own[5] = b.y; S.own[6] = b.z; S.own[7] = b.w;
   S.m = r.m;
@@ -552,67 +575,33 @@ __device__ __forceinline__ void stage_commit(StageRaw<V>& r, const TileRec& T, c
   L.xs[0][pc] = a;
   L.xs[1][pc] = b;
   if (lane < 32) {
-    float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
-    if (r.hok) pencil_split_f(r.halo, r.src, v0, v1);
-    L.xs[0][r.dst] = v0;
-    L.xs[1][r.dst] = v1;
+    const int g = lane >> 3, t = lane & 7;
+    float4 v0, v1;
+    pencil_any_f<SPLIT>(r.halo, ring_src(g, t), v0, v1);
+    L.xs[0][ring_dst(g, t)] = v0;
+    L.xs[1][ring_dst(g, t)] = v1;
   }
-  S.xkm = r.kokm ? single_f<V>(r.km) : 0.f;
-  S.xkp = r.kokp ? single_f<V>(r.kp) : 0.f;
+  S.xkm = single_f<V>(r.km);
+  S.xkp = single_f<V>(r.kp);
   S.h = T.h;
   S.b_int = bcv[6];
   for (int f = 0; f < 6; f++) S.b_f[f] = bcv[f];
 }
 
+// Stage tile T's X values (zeros where the flags say so) and face masks.
+//   zero_all:   treat X as 0 everywhere (first sweep from a zero guess)
+//   ghost_zero: treat X of GHOST tiles as 0 (downstroke of the V-cycle)
+//   bcv:        the 7 face coefficients of the view (faces x-..z+, inside)
+// X holds active tiles, XG ghost tiles (they differ only for the damped-Jacobi
+// ping-pong, where ghost values stay in the primary buffer).  SPLIT: X and XG
+// are V-cycle arrays in the colour-split layout (the staged pencils are in
+// natural order either way).
 template <bool SPLIT, class V>
 __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const V* __restrict__ X,
                                            const V* __restrict__ XG, bool zero_all, bool ghost_zero,
                                            const float bcv[7], TileLDS& L, Stencil& S) {
-  if (SPLIT && !std::is_same<V, float>::value) {  // 16-bit storage: every load before the first wait
-    StageRaw<V> r = stage_issue(c, T, X, XG, zero_all, ghost_zero);
-    stage_commit(r, T, bcv, L, S);
-    return;
-  }
-  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
-  const size_t base = size_t(T.self) * 512 + lane * 8;
-  const bool own_ghost = tr_type(T) == T_GHOST;
-  if (zero_all || (ghost_zero && own_ghost)) { for (int k = 0; k < 8; k++) S.own[k] = 0.f; }
-  else if (SPLIT) load8_split((own_ghost ? XG : X) + size_t(T.self) * 512, lane, S.own);
-  else load8((own_ghost ? XG : X) + base, S.own);
-  S.m = load_ct8(c.fmask + base);
-  const int pc = (i + 1) * 10 + (j + 1);
-  L.xs[0][pc] = make_float4(S.own[0], S.own[1], S.own[2], S.own[3]);
-  L.xs[1][pc] = make_float4(S.own[4], S.own[5], S.own[6], S.own[7]);
-  // halo ring: lanes 0..31 fetch one neighbour pencil each
-  //   g = 0: x- (pencil (7,t) of nbr 0), 1: x+ ((0,t) of nbr 3), 2: y- ((t,7) of nbr 1), 3: y+ ((t,0) of nbr 4)
-  if (lane < 32) {
-    const int g = lane >> 3, t = lane & 7;
-    const int f = (g == 0) ? 0 : (g == 1) ? 3 : (g == 2) ? 1 : 4;
-    const int n = pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
-    const int src = (g == 0) ? (56 + t) : (g == 1) ? t : (g == 2) ? (t * 8 + 7) : (t * 8);
-    const int dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
-    const bool gh = tr_ntype(T, f) == T_GHOST;
-    float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;  // x = 0 outside the grid (GMGSolver.cu:120-122)
-    if (n >= 0 && !(zero_all || (ghost_zero && gh))) {
-      if (SPLIT) load_pencil_split((gh ? XG : X) + size_t(n) * 512, src, v0, v1);
-      else load_pencil((gh ? XG : X) + size_t(n) * 512 + src * 8, v0, v1);
-    }
-    L.xs[0][dst] = v0;
-    L.xs[1][dst] = v1;
-  }
-  // k halos: cell (i,j,7) of the z- neighbour, (i,j,0) of the z+ neighbour
-  {
-    const int o7 = SPLIT ? split_cell(i, j, 7) : lane * 8 + 7, o0 = SPLIT ? split_cell(i, j, 0) : lane * 8;
-    int n = T.nbr[2];
-    bool gh = tr_ntype(T, 2) == T_GHOST;
-    S.xkm = (n < 0 || zero_all || (ghost_zero && gh)) ? 0.f : to_f((gh ? XG : X)[size_t(n) * 512 + o7]);
-    n = T.nbr[5];
-    gh = tr_ntype(T, 5) == T_GHOST;
-    S.xkp = (n < 0 || zero_all || (ghost_zero && gh)) ? 0.f : to_f((gh ? XG : X)[size_t(n) * 512 + o0]);
-  }
-  S.h = T.h;
-  S.b_int = bcv[6];
-  for (int f = 0; f < 6; f++) S.b_f[f] = bcv[f];
+  StageRaw<V> r = stage_issue<SPLIT>(c, T, X, XG, zero_all, ghost_zero);
+  stage_commit<SPLIT>(r, T, bcv, L, S);
 }
 
 // Face coefficients of tile T seen with the subtree view at level sl
@@ -982,27 +971,18 @@ __device__ __forceinline__ void tile_gs(const Ctx& c, const TileRec& T, int leve
 }
 
 // stage_tile on the V-cycle arrays plus the tile's rhs b (natural order; 0 with
-// zero_b).  With 16-bit storage the rhs is loaded with the stage's loads, ahead
-// of the one wait; FP32 keeps the rhs load after the staging, which measured
-// faster on the 256-problem pool.
-// EAGER: the raw form for FP32 as well (the host picks it for small levels).
+// zero_b), the rhs loaded with the stage's loads, ahead of the one wait.
+// EAGER (the host's choice for small levels) stages the same way.
 template <class V, bool EAGER = false>
 __device__ __forceinline__ void stage_with_rhs(const Ctx& c, const TileRec& T, const V* X, const V* XG, const V* F,
                                                int flags, bool zero_b, TileLDS& L, Stencil& S, float b[8]) {
-  const V* src = ((flags & F_VCYCLE) ? vc_rhs<V>(c) : F) + size_t(T.self) * 512;
-  if (std::is_same<V, float>::value && !EAGER) {
-    stage_tile<true>(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO, T.bc, L, S);
-    if (zero_b) { for (int k = 0; k < 8; k++) b[k] = 0.f; }
-    else load8_split(src, lane_id(), b);
-    return;
-  }
-  StageRaw<V> r = stage_issue(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO);
-  RawPencil<V> rb{};
-  if (!zero_b) rb = load_pencil_split_raw(src, lane_id());
+  const V* src = zero_b ? zero_page<V>() : ((flags & F_VCYCLE) ? vc_rhs<V>(c) : F) + size_t(T.self) * 512;
+  StageRaw<V> r = stage_issue<true>(c, T, X, XG, flags & F_ZERO_X, flags & F_GHOST_ZERO);
+  RawPencil<V> rb = load_pencil_split_raw(src, lane_id());
   hold_all(rb);
-  stage_commit(r, T, T.bc, L, S);
-  float4 ba = make_float4(0.f, 0.f, 0.f, 0.f), bb = ba;
-  if (!zero_b) pencil_split_f(rb, lane_id(), ba, bb);
+  stage_commit<true>(r, T, T.bc, L, S);
+  float4 ba, bb;
+  pencil_split_f(rb, lane_id(), ba, bb);
   b[0] = ba.x; b[1] = ba.y; b[2] = ba.z; b[3] = ba.w;
   b[4] = bb.x; b[5] = bb.y; b[6] = bb.z; b[7] = bb.w;
 }
@@ -1470,34 +1450,39 @@ __global__ __launch_bounds__(64) void k_gs_first(Ctx c, const TileRec* __restric
 // picks it by level size.  Every value is the one tile_gs_prolong forms.
 template <class V>
 struct KHaloRaw {
-  int n;
-  bool gh;
   uint32_t mv;
   typename RawV<V>::S pz, xo;
 };
+// The raw loads of one prolongated k-halo value: face mask, parent value and
+// down-sweep value of cell (i,j,KN) of face neighbour F, each from the zero
+// page where it does not exist (no neighbour, a ghost neighbour's down-sweep
+// value, a ghost tile's own stage).
 template <int F, int KN, class V>
 __device__ __forceinline__ KHaloRaw<V> khalo_prolong_issue(const Ctx& c, const TileRec& T, const V* XD, const V* XO,
-                                                           int oct, int lane) {
+                                                           int oct, int lane, bool skip) {
   KHaloRaw<V> k;
-  k.n = T.nbr[F];
-  k.gh = tr_ntype(T, F) == T_GHOST;
-  const int n = k.n >= 0 ? k.n : T.self;
-  const int pn = k.n >= 0 ? T.nbr_parent[F] : T.parent;
+  const int n = T.nbr[F];
+  const bool gh = tr_ntype(T, F) == T_GHOST;
+  const bool ok = !skip && n >= 0;
+  const V* zp = zero_page<V>();
   const int i = lane >> 3, j = lane & 7;
-  k.mv = c.fmask[size_t(n) * 512 + lane * 8 + KN];
+  k.mv = (ok ? c.fmask + size_t(n) * 512 + lane * 8 : reinterpret_cast<const uint8_t*>(zp))[ok ? KN : 0];
   const int noct = oct ^ 4;
   const int ox = noct & 1, oy = (noct >> 1) & 1, oz = (noct >> 2) & 1;
-  k.pz = load_s_raw(XO + size_t(pn) * 512 + split_cell(ox * 4 + (i >> 1), oy * 4 + (j >> 1), oz * 4 + (KN >> 1)));
-  k.xo = load_s_raw(XD + size_t(n) * 512 + split_cell(i, j, KN));
+  k.pz = load_s_raw(ok ? XO + size_t(T.nbr_parent[F]) * 512 +
+                             split_cell(ox * 4 + (i >> 1), oy * 4 + (j >> 1), oz * 4 + (KN >> 1))
+                       : zp);
+  k.xo = load_s_raw(ok && !gh ? XD + size_t(n) * 512 + split_cell(i, j, KN) : zp);
   return k;
 }
-// khalo_prolong's value from the raw loads
-template <class V>
-__device__ __forceinline__ float khalo_prolong_value(KHaloRaw<V> k, float pcoef) {
-  if (k.n < 0) return 0.f;
+// khalo_prolong's value from the raw loads (a missing neighbour's zero mask
+// gives 0)
+template <int F, class V>
+__device__ __forceinline__ float khalo_prolong_value(const TileRec& T, KHaloRaw<V> k, float pcoef) {
+  const bool gh = tr_ntype(T, F) == T_GHOST;
   const float pz = single_f<V>(k.pz);
-  const float xo = k.gh ? 0.f : single_f<V>(k.xo);
-  return (k.mv & M_INT) ? vround<V>(k.gh ? pz : __builtin_fmaf(pcoef, pz, xo)) : 0.f;
+  const float xo = single_f<V>(k.xo);
+  return (k.mv & M_INT) ? vround<V>(gh ? pz : __builtin_fmaf(pcoef, pz, xo)) : 0.f;
 }
 template <class V> struct RawQuadS { typename RawV<V>::S2 h0, h1; };
 template <class V>
@@ -1522,39 +1507,31 @@ __device__ __forceinline__ void tile_gs_prolong_eager(const Ctx& c, const TileRe
   const size_t base = size_t(T.self) * 512 + lane * 8;
   const int oct = tr_octant(T);
   const bool own_ghost = tr_type(T) == T_GHOST;
-  // ---- loads
+  const V* zp = zero_page<V>();
+  // ---- loads, none in a branch (values that do not exist come from the zero page)
   const int32_t* dp = (flags & F_CHECK_DONE) ? &c.ps[T.prob].done : reinterpret_cast<const int32_t*>(c.fmask);
   const int32_t done = *dp;
   const uint64_t mown = load_ct8(c.fmask + base);
   const int ppl = ((oct & 1) * 4 + (i >> 1)) * 8 + (((oct >> 1) & 1) * 4 + (j >> 1));  // parent pencil
   RawQuadS<V> rpq = load_quad_split_raw(XO + size_t(T.parent) * 512, ppl, ((oct >> 2) & 1) * 4);
-  RawPencil<V> rown{}, rrhs{}, rh{};
-  RawQuadS<V> rhpq{};
-  KHaloRaw<V> km{}, kp{};
-  uint64_t hw = 0;
-  int hn = -1, hpl = 0, hppl = 0, dst = 0;
-  bool hgh = false;
-  if (!own_ghost) {
-    rown = load_pencil_split_raw(XD + size_t(T.self) * 512, lane);
-    rrhs = load_pencil_split_raw(vc_rhs<V>(c) + size_t(T.self) * 512, lane);
-    // halo ring of lanes 0..31 as in tile_gs_prolong; every lane loads
-    const int g = (lane >> 3) & 3, t = lane & 7;
-    const int f = (g == 0) ? 0 : (g == 1) ? 3 : (g == 2) ? 1 : 4;
-    hn = lane < 32 ? pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4])) : -1;
-    const int pn = pick4(g, uni(T.nbr_parent[0]), uni(T.nbr_parent[3]), uni(T.nbr_parent[1]), uni(T.nbr_parent[4]));
-    const int ni = (g == 0) ? 7 : (g == 1) ? 0 : t, nj = (g == 2) ? 7 : (g == 3) ? 0 : t;  // pencil in the neighbour
-    dst = (g == 0) ? (t + 1) : (g == 1) ? (91 + t) : (g == 2) ? ((t + 1) * 10) : ((t + 1) * 10 + 9);
-    const int noct = oct ^ (1 << (f % 3));
-    hpl = ni * 8 + nj;
-    hppl = ((noct & 1) * 4 + (ni >> 1)) * 8 + (((noct >> 1) & 1) * 4 + (nj >> 1));
-    hgh = tr_ntype(T, f) == T_GHOST;
-    const int ln = hn >= 0 ? hn : T.self, lpn = hn >= 0 ? pn : T.parent;
-    hw = load_ct8(c.fmask + size_t(ln) * 512 + hpl * 8);
-    rh = load_pencil_split_raw(XD + size_t(ln) * 512, hpl);
-    rhpq = load_quad_split_raw(XO + size_t(lpn) * 512, hppl, ((noct >> 2) & 1) * 4);
-    km = khalo_prolong_issue<2, 7>(c, T, XD, XO, oct, lane);
-    kp = khalo_prolong_issue<5, 0>(c, T, XD, XO, oct, lane);
-  }
+  RawPencil<V> rown = load_pencil_split_raw(own_ghost ? zp : XD + size_t(T.self) * 512, lane);
+  RawPencil<V> rrhs = load_pencil_split_raw(own_ghost ? zp : vc_rhs<V>(c) + size_t(T.self) * 512, lane);
+  // halo ring of lanes 0..31 as in tile_gs_prolong; every lane loads
+  const int g = (lane >> 3) & 3, t = lane & 7;
+  const int f = ring_face(g);
+  const int hn = pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
+  const int pn = pick4(g, uni(T.nbr_parent[0]), uni(T.nbr_parent[3]), uni(T.nbr_parent[1]), uni(T.nbr_parent[4]));
+  const int ni = pick4(g, 7, 0, t, t), nj = pick4(g, t, t, 7, 0);  // pencil in the neighbour
+  const int noct = oct ^ (1 << (f % 3));
+  const int hpl = ni * 8 + nj;
+  const int hppl = ((noct & 1) * 4 + (ni >> 1)) * 8 + (((noct >> 1) & 1) * 4 + (nj >> 1));
+  const bool hgh = tr_ntype(T, f) == T_GHOST;
+  const bool hok = !own_ghost && lane < 32 && hn >= 0;
+  const uint64_t hw = load_ct8(hok ? c.fmask + size_t(hn) * 512 + hpl * 8 : reinterpret_cast<const uint8_t*>(zp));
+  RawPencil<V> rh = load_pencil_split_raw(hok && !hgh ? XD + size_t(hn) * 512 : zp, hpl);
+  RawQuadS<V> rhpq = load_quad_split_raw(hok ? XO + size_t(pn) * 512 : zp, hppl, ((noct >> 2) & 1) * 4);
+  KHaloRaw<V> km = khalo_prolong_issue<2, 7>(c, T, XD, XO, oct, lane, own_ghost);
+  KHaloRaw<V> kp = khalo_prolong_issue<5, 0>(c, T, XD, XO, oct, lane, own_ghost);
   // ---- one wait, then the values
   hold_all(rpq.h0);
   hold_all(rpq.h1);
@@ -1569,8 +1546,8 @@ __device__ __forceinline__ void tile_gs_prolong_eager(const Ctx& c, const TileRe
   hold_all(kp.mv);
   hold_all(kp.pz);
   hold_all(kp.xo);
-  float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), xb = xa;
-  if (!own_ghost) pencil_split_f(rown, lane, xa, xb);
+  float4 xa, xb;
+  pencil_split_f(rown, lane, xa, xb);
   float4 oa, ob;
   prolong_pencil<V>(xa, xb, quad_split_f(rpq, ppl), mown, own_ghost, pcoef, oa, ob);
   if (own_ghost) {  // GHOST tile: its prolongated values only
@@ -1586,20 +1563,19 @@ __device__ __forceinline__ void tile_gs_prolong_eager(const Ctx& c, const TileRe
   L.xs[0][pc] = oa;
   L.xs[1][pc] = ob;
   if (lane < 32) {
-    float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;  // x = 0 outside the grid (GMGSolver.cu:120-122)
-    if (hn >= 0) {
-      float4 ya = make_float4(0.f, 0.f, 0.f, 0.f), yb = ya;
-      if (!hgh) pencil_split_f(rh, hpl, ya, yb);
-      prolong_pencil<V>(ya, yb, quad_split_f(rhpq, hppl), hw, hgh, pcoef, va, vb);
-    }
-    L.xs[0][dst] = va;
-    L.xs[1][dst] = vb;
+    // x = 0 outside the grid (GMGSolver.cu:120-122): a missing neighbour's
+    // zero masks make every value 0
+    float4 ya, yb, va, vb;
+    pencil_split_f(rh, hpl, ya, yb);
+    prolong_pencil<V>(ya, yb, quad_split_f(rhpq, hppl), hw, hgh, pcoef, va, vb);
+    L.xs[0][ring_dst(g, t)] = va;
+    L.xs[1][ring_dst(g, t)] = vb;
   }
-  S.xkm = khalo_prolong_value(km, pcoef);
-  S.xkp = khalo_prolong_value(kp, pcoef);
+  S.xkm = khalo_prolong_value<2>(T, km, pcoef);
+  S.xkp = khalo_prolong_value<5>(T, kp, pcoef);
   S.h = T.h;
   S.b_int = T.bc[6];
-  for (int f = 0; f < 6; f++) S.b_f[f] = T.bc[f];
+  for (int e = 0; e < 6; e++) S.b_f[e] = T.bc[e];
   float4 ba, bb;
   pencil_split_f(rrhs, lane, ba, bb);
   const float b[8] = {ba.x, ba.y, ba.z, ba.w, bb.x, bb.y, bb.z, bb.w};
