@@ -78,30 +78,40 @@ __global__ __launch_bounds__(64) void k_gs_pair(Ctx c, const TileRec* __restrict
   const V* R = vc_rhs<V>(c);
   const int self = T.self;
   const float h = T.h;
-  // ---- round 1: the face neighbours' records (lanes 0..5) and the tile's own data
-  if (lane < 6) {
-    const int nbv = pick4(lane & 3, uni(T.nbr[0]), uni(T.nbr[1]), uni(T.nbr[2]), uni(T.nbr[3]));
-    const int nb = lane < 4 ? nbv : (lane == 4 ? uni(T.nbr[4]) : uni(T.nbr[5]));
-    const bool act = nb >= 0 && (tr_ntype(T, lane) & (T_LEAF | T_NONLEAF));
-    const TileRec* rp = c.tiles + (act ? nb : self);
-    PairNbr pn;
+  const V* zp = zero_page<V>();
+  // ---- round 1: every load that needs only the tile's record, all issued
+  // before the first wait.  Values that do not exist or count as 0 (outside
+  // the grid, ghosts in the down-sweep, lanes past the ring) are read from the
+  // zero page; values only an inactive neighbour's update would read come
+  // from the zero page too and are not used.
+  // the face neighbours' records (lanes 0..5 keep theirs)
+  const int lf = lane < 6 ? lane : 5;
+  const int nbv = pick4(lane & 3, uni(T.nbr[0]), uni(T.nbr[1]), uni(T.nbr[2]), uni(T.nbr[3]));
+  const int nb = lane < 4 ? nbv : (lane == 4 ? uni(T.nbr[4]) : uni(T.nbr[5]));
+  const bool nact = nb >= 0 && (tr_ntype(T, lf) & (T_LEAF | T_NONLEAF));
+  const TileRec* nrec = c.tiles + (nact ? nb : self);
+  PairNbr pn;
 #pragma unroll
-    for (int e = 0; e < 7; e++) pn.bc[e] = rp->bc[e];
+  for (int e = 0; e < 7; e++) pn.bc[e] = nrec->bc[e];
 #pragma unroll
-    for (int e = 0; e < 6; e++) pn.nbr[e] = rp->nbr[e];
-    pn.ntypes = rp->ntypes;
-    L.nb[lane] = pn;
-  }
+  for (int e = 0; e < 6; e++) pn.nbr[e] = nrec->nbr[e];
+  pn.ntypes = nrec->ntypes;
   const int32_t* dp = (flags & F_CHECK_DONE) ? &c.ps[T.prob].done : reinterpret_cast<const int32_t*>(c.fmask);
   const int32_t done = *dp;
   const V* own = XI + size_t(self) * 512;
-  const float4 xa = quad_f<V>(load_q_raw(own + (c1 << 8) + lane * 4));  // colour c1 (old)
-  const float4 xb = quad_f<V>(load_q_raw(own + (c2 << 8) + lane * 4));  // colour c2 (old)
-  const float4 ba = quad_f<V>(load_q_raw(R + size_t(self) * 512 + (c1 << 8) + lane * 4));
-  const float4 bb = quad_f<V>(load_q_raw(R + size_t(self) * 512 + (c2 << 8) + lane * 4));
+  typename RawV<V>::Q qxa = load_q_raw(own + (c1 << 8) + lane * 4);  // colour c1 (old)
+  typename RawV<V>::Q qxb = load_q_raw(own + (c2 << 8) + lane * 4);  // colour c2 (old)
+  typename RawV<V>::Q qba = load_q_raw(R + size_t(self) * 512 + (c1 << 8) + lane * 4);
+  typename RawV<V>::Q qbb = load_q_raw(R + size_t(self) * 512 + (c2 << 8) + lane * 4);
   const uint64_t mask = load_ct8(c.fmask + size_t(self) * 512 + lane * 8);
-  L.q[(i + 1) * 10 + (j + 1)] = xb;
-
+  // the r pencil of a LEAF tile, for the r.z partial (DOT)
+  float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1v = r0;
+  if (DOT) {
+    const float* rr = tr_type(T) == T_LEAF ? c.r + size_t(self) * 512 : zero_page<float>();
+    const float4* rpn = reinterpret_cast<const float4*>(rr + lane * 8);
+    r0 = rpn[0];
+    r1v = rpn[1];
+  }
   // halo ring (lanes 0..31): the neighbour's face pencil, both colours, and,
   // where that neighbour is smoothed, what its colour-c1 update reads
   const int g = (lane >> 3) & 3, t = lane & 7;
@@ -112,20 +122,15 @@ __global__ __launch_bounds__(64) void k_gs_pair(Ctx c, const TileRec* __restrict
   const int rn = pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
   const uint8_t rty = tr_ntype(T, rf);
   const bool rok = lane < 32 && pair_readable(rn, rty, ghost_zero);
-  const bool ract = lane < 32 && rn >= 0 && (rty & (T_LEAF | T_NONLEAF));
-  const int rt = rok ? rn : self;  // read at the tile itself when the values are dropped
-  const V* RX = (rok && rty == T_GHOST) ? XG : XI;
-  float4 rc1 = quad_f<V>(load_q_raw(RX + size_t(rt) * 512 + (c1 << 8) + rpl * 4));
-  float4 rc2 = quad_f<V>(load_q_raw(RX + size_t(rt) * 512 + (c2 << 8) + rpl * 4));
-  const float4 rb = quad_f<V>(load_q_raw(R + size_t(rt) * 512 + (c1 << 8) + rpl * 4));
-  const uint64_t rm = load_ct8(c.fmask + size_t(rt) * 512 + rpl * 8);
+  const bool ract = lane < 32 && rn >= 0 && (rty & (T_LEAF | T_NONLEAF));  // implies rok
+  const V* RT = rok ? (rty == T_GHOST ? XG : XI) + size_t(rn) * 512 : zp;
+  const V* RB = rok ? R + size_t(rn) * 512 : zp;
+  typename RawV<V>::Q qrc1 = load_q_raw(rok ? RT + (c1 << 8) + rpl * 4 : zp);
+  typename RawV<V>::Q qrc2 = load_q_raw(rok ? RT + (c2 << 8) + rpl * 4 : zp);
+  typename RawV<V>::Q qrb = load_q_raw(rok ? RB + (c1 << 8) + rpl * 4 : zp);
+  const uint64_t rm = load_ct8(rok ? c.fmask + size_t(rn) * 512 + rpl * 8 : reinterpret_cast<const uint8_t*>(zp));
   const int dpl = ring_deep(g, t);
-  const float4 rdeep = quad_f<V>(load_q_raw(RX + size_t(rt) * 512 + (c2 << 8) + dpl * 4));
-  if (!rok) {
-    rc1 = make_float4(0.f, 0.f, 0.f, 0.f);
-    rc2 = rc1;
-  }
-  if (lane < 32) L.q[ring_slot(g, t)] = rc2;
+  typename RawV<V>::Q qrdeep = load_q_raw(rok ? RT + (c2 << 8) + dpl * 4 : zp);
 
   // z planes (every lane): cell (i,j,7) of the z- neighbour and (i,j,0) of
   // the z+ one (old values); the lane whose own cell next to a plane has
@@ -136,19 +141,39 @@ __global__ __launch_bounds__(64) void k_gs_pair(Ctx c, const TileRec* __restrict
   const bool actm = nzm >= 0 && (tzm & (T_LEAF | T_NONLEAF)), actp = nzp >= 0 && (tzp & (T_LEAF | T_NONLEAF));
   const bool upd_m = ((i + j) & 1) == c2;  // own (i,j,0) is colour c2: the z- plane cell (i,j,7) is colour c1
   const bool upd_p = !upd_m;              // own (i,j,7) is colour c2: the z+ plane cell (i,j,0) is colour c1
-  const int tm = okm ? nzm : self, tp = okp ? nzp : self;
-  const V* XM = (okm && tzm == T_GHOST) ? XG : XI;
-  const V* XP = (okp && tzp == T_GHOST) ? XG : XI;
-  float vm = single_f<V>(load_s_raw(XM + size_t(tm) * 512 + split_cell(i, j, 7)));
-  float vp = single_f<V>(load_s_raw(XP + size_t(tp) * 512 + split_cell(i, j, 0)));
-  // the updating lane's row data: deeper cell, rhs, face mask
-  const int tu = upd_m ? tm : tp;
+  const V* XM = okm ? (tzm == T_GHOST ? XG : XI) + size_t(nzm) * 512 : zp;
+  const V* XP = okp ? (tzp == T_GHOST ? XG : XI) + size_t(nzp) * 512 : zp;
+  typename RawV<V>::S svm = load_s_raw(okm ? XM + split_cell(i, j, 7) : zp);
+  typename RawV<V>::S svp = load_s_raw(okp ? XP + split_cell(i, j, 0) : zp);
+  // the updating lane's row data: deeper cell, rhs, face mask (used when that
+  // plane's tile is smoothed, which implies it is readable)
+  const bool oku = upd_m ? okm : okp;
+  const int tu = upd_m ? nzm : nzp;
   const int ku = upd_m ? 7 : 0, kd = upd_m ? 6 : 1;
-  const float zdeep = single_f<V>(load_s_raw((upd_m ? XM : XP) + size_t(tu) * 512 + split_cell(i, j, kd)));
-  const float zb = single_f<V>(load_s_raw(R + size_t(tu) * 512 + split_cell(i, j, ku)));
-  const uint32_t zmask = c.fmask[size_t(tu) * 512 + lane * 8 + ku];
-  if (!okm) vm = 0.f;
-  if (!okp) vp = 0.f;
+  typename RawV<V>::S szdeep = load_s_raw(oku ? (upd_m ? XM : XP) + split_cell(i, j, kd) : zp);
+  typename RawV<V>::S szb = load_s_raw(oku ? R + size_t(tu) * 512 + split_cell(i, j, ku) : zp);
+  const uint32_t zmask = *(oku ? c.fmask + size_t(tu) * 512 + lane * 8 + ku : reinterpret_cast<const uint8_t*>(zp));
+
+  // ---- one wait, then round 1's values into LDS
+  hold_all(qxa);
+  hold_all(qxb);
+  hold_all(qba);
+  hold_all(qbb);
+  hold_all(qrc1);
+  hold_all(qrc2);
+  hold_all(qrb);
+  hold_all(qrdeep);
+  hold_all(svm);
+  hold_all(svp);
+  hold_all(szdeep);
+  hold_all(szb);
+  const float4 xa = quad_f<V>(qxa), xb = quad_f<V>(qxb), ba = quad_f<V>(qba), bb = quad_f<V>(qbb);
+  const float4 rc1 = quad_f<V>(qrc1), rc2 = quad_f<V>(qrc2), rb = quad_f<V>(qrb), rdeep = quad_f<V>(qrdeep);
+  const float vm = single_f<V>(svm), vp = single_f<V>(svp);
+  const float zdeep = single_f<V>(szdeep), zb = single_f<V>(szb);
+  if (lane < 6) L.nb[lane] = pn;
+  L.q[(i + 1) * 10 + (j + 1)] = xb;
+  if (lane < 32) L.q[ring_slot(g, t)] = rc2;
   L.pl[0][i + 1][j + 1] = vm;
   L.pl[1][i + 1][j + 1] = vp;
   wave_sync();
@@ -169,8 +194,8 @@ __global__ __launch_bounds__(64) void k_gs_pair(Ctx c, const TileRec* __restrict
     // the edge pencil: across y (x rings) it is (ri, 7 - ...) etc.
     const int ei = g < 2 ? ri : (t == 0 ? 7 : 0);
     const int ej = g < 2 ? (t == 0 ? 7 : 0) : rj;
-    float4 ev = quad_f<V>(load_q_raw((eok && ety == T_GHOST ? XG : XI) + size_t(eok ? en : self) * 512 + (c2 << 8) +
-                                     (ei * 8 + ej) * 4));
+    typename RawV<V>::Q qev =
+        load_q_raw(eok ? (ety == T_GHOST ? XG : XI) + size_t(en) * 512 + (c2 << 8) + (ei * 8 + ej) * 4 : zp);
     // the ring pencil's k end: its colour-c1 cells at even k (p = 0) need
     // cell (ri,rj,-1) = (ri,rj,7) of its z- neighbour, else (ri,rj,8) = (ri,rj,0) of its z+ one
     const bool rp = ((c1 + ri + rj) & 1) != 0;
@@ -178,8 +203,8 @@ __global__ __launch_bounds__(64) void k_gs_pair(Ctx c, const TileRec* __restrict
     const int kn = rnb.nbr[kf];
     const uint8_t kty = uint8_t((rnb.ntypes >> (4 * kf)) & 0xf);
     const bool kok = lane < 32 && ract && pair_readable(kn, kty, ghost_zero);
-    float kv = single_f<V>(load_s_raw((kok && kty == T_GHOST ? XG : XI) + size_t(kok ? kn : self) * 512 +
-                                      split_cell(ri, rj, rp ? 0 : 7)));
+    typename RawV<V>::S skv =
+        load_s_raw(kok ? (kty == T_GHOST ? XG : XI) + size_t(kn) * 512 + split_cell(ri, rj, rp ? 0 : 7) : zp);
     // plane edges: lanes 0..31 fetch one edge cell of each plane
     //   q = 0: (-1, t) from the plane tile's x- neighbour, cell (7,t,K); 1: (8, t) from x+, (0,t,K);
     //   2: (t, -1) from y-, (t,7,K); 3: (t, 8) from y+, (t,0,K)
@@ -192,14 +217,16 @@ __global__ __launch_bounds__(64) void k_gs_pair(Ctx c, const TileRec* __restrict
     const uint8_t emt = uint8_t((zmn.ntypes >> (4 * pf)) & 0xf), ept = uint8_t((zpn.ntypes >> (4 * pf)) & 0xf);
     const bool emok = lane < 32 && actm && pair_readable(emn, emt, ghost_zero);
     const bool epok = lane < 32 && actp && pair_readable(epn, ept, ghost_zero);
-    float emv = single_f<V>(load_s_raw((emok && emt == T_GHOST ? XG : XI) + size_t(emok ? emn : self) * 512 +
-                                       split_cell(pi, pj, 7)));
-    float epv = single_f<V>(load_s_raw((epok && ept == T_GHOST ? XG : XI) + size_t(epok ? epn : self) * 512 +
-                                       split_cell(pi, pj, 0)));
-    if (!eok) ev = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (!kok) kv = 0.f;
-    if (!emok) emv = 0.f;
-    if (!epok) epv = 0.f;
+    typename RawV<V>::S semv =
+        load_s_raw(emok ? (emt == T_GHOST ? XG : XI) + size_t(emn) * 512 + split_cell(pi, pj, 7) : zp);
+    typename RawV<V>::S sepv =
+        load_s_raw(epok ? (ept == T_GHOST ? XG : XI) + size_t(epn) * 512 + split_cell(pi, pj, 0) : zp);
+    hold_all(qev);
+    hold_all(skv);
+    hold_all(semv);
+    hold_all(sepv);
+    const float4 ev = quad_f<V>(qev);
+    const float kv = single_f<V>(skv), emv = single_f<V>(semv), epv = single_f<V>(sepv);
     if (end) L.corner[g][t == 0 ? 0 : 1] = ev;
     if (lane < 32) {
       const int a = q == 0 ? 0 : (q == 1 ? 9 : t + 1), b2 = q == 2 ? 0 : (q == 3 ? 9 : t + 1);
@@ -271,9 +298,7 @@ __global__ __launch_bounds__(64) void k_gs_pair(Ctx c, const TileRec* __restrict
     store4(tile + (c2 << 8) + lane * 4, n2);
     if (DOT && tr_type(T) == T_LEAF) {
       // z final: colour c2 as just stored, colour c1 from pass 1; r.z over the
-      // lane's pencil in cell order, as k_cg_dot_rz sums it
-      const float4* rpn = reinterpret_cast<const float4*>(c.r + size_t(self) * 512 + lane * 8);
-      const float4 r0 = rpn[0], r1v = rpn[1];
+      // lane's pencil in cell order, as k_cg_dot_rz sums it (r loaded in round 1)
       const float zc[4] = {vround<V>(n2.x), vround<V>(n2.y), vround<V>(n2.z), vround<V>(n2.w)};
       const float zo[4] = {x1.x, x1.y, x1.z, x1.w};
       const float rv[8] = {r0.x, r0.y, r0.z, r0.w, r1v.x, r1v.y, r1v.z, r1v.w};
