@@ -376,11 +376,11 @@ __device__ __forceinline__ void store_quad_split(V* __restrict__ tile, int pl, i
 // a tile has been issued; a widening inside the branch that loads a value would
 // make that branch wait for its load.  hold() puts an empty asm the compiler
 // cannot see through on the raw bits, so that it cannot move the widening up.
-// A device array of zeros as large as one tile of FP32 values: loads of values
+// A device array of zeros as large as one tile of FP64 values: loads of values
 // that are 0 by definition read it (L2-resident) instead of being skipped.
 // Nothing writes it; it is not declared const so that it sits in global
 // memory like the arrays it stands in for (one kind of load for both).
-__device__ __align__(16) float kZeroPage[512] = {};
+__device__ __align__(16) float kZeroPage[1024] = {};
 template <class V> __device__ __forceinline__ const V* zero_page() { return reinterpret_cast<const V*>(kZeroPage); }
 
 template <class V> struct RawV;
@@ -2875,41 +2875,45 @@ __global__ __launch_bounds__(64) void k_apply64(Ctx c, const TileRec* __restrict
                                                 double* __restrict__ AX) {
   const int flags = 0;
   TILE_KERNEL_PROLOGUE
+  // the tile's pencils and its halo ring, index (i+1)*10 + (j+1), cells 0..7
+  __shared__ double xs[100][8];
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t self = size_t(T.self) * 512;
   const size_t base = self + lane * 8;
   const double h = double(T.h);
   const double ooh = 1.0 / h;
   const uint8_t to = tr_type(T);
+  const double* zp = zero_page<double>();
+  // ---- loads, none in a branch: own pencil, one ring pencil (lanes 0..31;
+  // x = 0 outside the grid), the two k halos, the face masks
+  double own[8], ring[8];
+  load8d(X + base, own);
+  const int g = (lane >> 3) & 3, t = lane & 7;
+  const int rn = pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
+  const bool rok = lane < 32 && rn >= 0;
+  load8d(rok ? X + size_t(rn) * 512 + ring_src(g, t) * 8 : zp, ring);
+  const double xkm = *(T.nbr[2] >= 0 ? X + size_t(T.nbr[2]) * 512 + lane * 8 + 7 : zp);
+  const double xkp = *(T.nbr[5] >= 0 ? X + size_t(T.nbr[5]) * 512 + lane * 8 : zp);
+  const uint64_t m = load_ct8(c.fmask + base);
   double bf[6];
 #pragma unroll
   for (int f = 0; f < 6; f++) bf[f] = base_coeff64(to, T.nbr[f] < 0 ? to : tr_ntype(T, f), ooh);
   const double bint = base_coeff64(to, to, ooh);
-  double own[8], xim[8], xip[8], xjm[8], xjp[8];
-  load8d(X + base, own);
-  // pencil offsets of the four i/j neighbours (-1: outside the grid, x = 0)
-  auto nb_pencil = [&](bool inside, int own_off, int f, int nb_off, double v[8]) {
-    const int nt = T.nbr[f];
-    if (inside) load8d(X + self + own_off * 8, v);
-    else if (nt >= 0) load8d(X + size_t(nt) * 512 + nb_off * 8, v);
-    else {
+  const int pc = (i + 1) * 10 + (j + 1);
 #pragma unroll
-      for (int k = 0; k < 8; k++) v[k] = 0.0;
-    }
-  };
-  nb_pencil(i > 0, lane - 8, 0, 56 + j, xim);
-  nb_pencil(i < 7, lane + 8, 3, j, xip);
-  nb_pencil(j > 0, lane - 1, 1, i * 8 + 7, xjm);
-  nb_pencil(j < 7, lane + 1, 4, i * 8, xjp);
-  const double xkm = T.nbr[2] >= 0 ? X[size_t(T.nbr[2]) * 512 + lane * 8 + 7] : 0.0;
-  const double xkp = T.nbr[5] >= 0 ? X[size_t(T.nbr[5]) * 512 + lane * 8] : 0.0;
-  const uint64_t m = load_ct8(c.fmask + base);
+  for (int k = 0; k < 8; k++) xs[pc][k] = own[k];
+  if (lane < 32) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) xs[ring_dst(g, t)][k] = ring[k];
+  }
+  wave_sync();
   const double bim = (i == 0) ? bf[0] : bint, bip = (i == 7) ? bf[3] : bint;
   const double bjm = (j == 0) ? bf[1] : bint, bjp = (j == 7) ? bf[4] : bint;
   double out[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    const double xn[6] = {xim[k], xip[k], xjm[k], xjp[k], k == 0 ? xkm : own[k - 1], k == 7 ? xkp : own[k + 1]};
+    const double xn[6] = {xs[pc - 10][k], xs[pc + 10][k], xs[pc - 1][k], xs[pc + 1][k], k == 0 ? xkm : own[k - 1],
+                          k == 7 ? xkp : own[k + 1]};
     const double bc[6] = {bim, bip, bjm, bjp, k == 0 ? bf[2] : bint, k == 7 ? bf[5] : bint};
     const uint32_t mk = ct_at(m, k);
     double sum = 0.0;
