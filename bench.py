@@ -744,6 +744,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--eager-max", type=int, default=-1,
                     help="levels of at most this many tiles use the one-wait prolongation pass (-1: the library default)")
+    ap.add_argument("--fused16-max", type=int, default=-1,
+                    help="16-bit levels of at most this many tiles run the first down-sweep iteration in one launch "
+                         "(-1: the library default)")
     ap.add_argument("--batch", type=int, default=256,
                     help="problems in the batched form measured beside the plain solve (0: skip)")
     ap.add_argument("--batch-reps", type=int, default=3)
@@ -799,8 +802,8 @@ def main():
     t0 = time.perf_counter()
     P = ca.Problem(scene.tree)
     t_create = time.perf_counter() - t0
-    if args.pair_sweeps >= 0 or args.eager_max >= 0:
-        P.set_level_forms(args.eager_max, -1, args.pair_sweeps)
+    if args.pair_sweeps >= 0 or args.eager_max >= 0 or args.fused16_max >= 0:
+        P.set_level_forms(args.eager_max, args.fused16_max, args.pair_sweeps)
     if args.kernels_only:
         if args.kernels_batch > 0:
             # the same kernels at batch scale, past the last-level cache: the
@@ -937,8 +940,8 @@ def main():
         t0 = time.perf_counter()
         B = ca.Problem.batch([sc.tree for sc in scenes])
         t_pool = time.perf_counter() - t0
-        if args.pair_sweeps >= 0 or args.eager_max >= 0:
-            B.set_level_forms(args.eager_max, -1, args.pair_sweeps)
+        if args.pair_sweeps >= 0 or args.eager_max >= 0 or args.fused16_max >= 0:
+            B.set_level_forms(args.eager_max, args.fused16_max, args.pair_sweeps)
         note(f"pool of {args.batch} problems ({B.n} tiles) created in {t_pool:.2f} s")
         # FP32 storage is also refined in FP64, so that every pressure meets
         # 1e-6 of its FP64 residual like the mixed forms'
