@@ -2047,22 +2047,33 @@ __global__ __launch_bounds__(64) void k_cg_dot_rz(Ctx c, const TileRec* __restri
   if (lane == 0) c.part[T.leaf_slot] = a;
 }
 
-// p = z + beta p on INTERIOR cells of LEAF tiles (GMGSolver.cu:642-653), then
-// the new p copied into the tile's ghost children (their x is the parent's
-// INTERIOR value, PoissonGrid.cu:585-605), ready for the operator.
+// The previous iteration's x += alpha p (deferred from k_cg_update_xr, which
+// then reads neither x nor p; the same fma on the same values, one iteration
+// later, and k_cg_finish_x applies the last one), then p = z + beta p on
+// INTERIOR cells of LEAF tiles (GMGSolver.cu:587-609 and 642-653); the new p
+// copied into the tile's ghost children (their x is the parent's INTERIOR
+// value, PoissonGrid.cu:585-605), ready for the operator.
 template <class V>
 __global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
   TILE_KERNEL_PROLOGUE
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t base = size_t(T.self) * 512 + lane * 8;
-  float pv[8], zv[8];
+  float pv[8], zv[8], xv[8];
   load8(c.p + base, pv);
   load8_split(vc_z<V>(c) + size_t(T.self) * 512, lane, zv);
+  load8(c.x + base, xv);
   const uint64_t ct = load_ct8(c.ctype + base);
   if (prob_done(c, T, flags)) return;
-  const float beta = float(c.ps[T.prob].beta);
+  const ProbState& ps = c.ps[T.prob];
+  const float beta = float(ps.beta);
+  const float alpha = float(ps.alpha);
+  const bool upd_x = ps.iter > 0;  // an earlier iteration left its x update pending
   for (int k = 0; k < 8; k++)
-    if (ct_at(ct, k) & C_INT) pv[k] = __builtin_fmaf(beta, pv[k], zv[k]);
+    if (ct_at(ct, k) & C_INT) {
+      if (upd_x) xv[k] = __builtin_fmaf(alpha, pv[k], xv[k]);
+      pv[k] = __builtin_fmaf(beta, pv[k], zv[k]);
+    }
+  if (upd_x) store8(c.x + base, xv);
   store8(c.p + base, pv);
   if (!tr_has_gchild(T)) return;
   __shared__ float Lp[512];
@@ -2146,34 +2157,47 @@ __global__ __launch_bounds__(64) void k_ghost_gather(Ctx c, const TileRec* __res
   if (lane == 0) c.part[T.leaf_slot] = a;
 }
 
-// x += alpha p, r -= alpha A p on INTERIOR cells of LEAF tiles
-// (GMGSolver.cu:587-609); r is also the next V-cycle's rhs; partials of |r|^2.
+// r -= alpha A p on INTERIOR cells of LEAF tiles (GMGSolver.cu:587-609); r is
+// also the next V-cycle's rhs; partials of |r|^2.  The iteration's x += alpha p
+// is left to the next k_cg_update_p (or k_cg_finish_x).
 template <class V>
 __global__ __launch_bounds__(64) void k_cg_update_xr(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
   TILE_KERNEL_PROLOGUE
   const int lane = lane_id();
   const size_t base = size_t(T.self) * 512 + lane * 8;
-  float xv[8], rv[8], pv[8], av[8];
-  load8(c.x + base, xv);
+  float rv[8], av[8];
   load8(c.r + base, rv);
-  load8(c.p + base, pv);
   load8(c.ap + base, av);
   const uint64_t ct = load_ct8(c.ctype + base);
   if (prob_done(c, T, flags)) return;
   const float alpha = float(c.ps[T.prob].alpha);
-  for (int k = 0; k < 8; k++) {
-    if (ct_at(ct, k) & C_INT) {
-      xv[k] = __builtin_fmaf(alpha, pv[k], xv[k]);
-      rv[k] = __builtin_fmaf(-alpha, av[k], rv[k]);
-    }
-  }
-  store8(c.x + base, xv);
+  for (int k = 0; k < 8; k++)
+    if (ct_at(ct, k) & C_INT) rv[k] = __builtin_fmaf(-alpha, av[k], rv[k]);
   store8(c.r + base, rv);
   store8_split(vc_rhs<V>(c) + size_t(T.self) * 512, lane, rv);
   double a = 0.0;
   for (int k = 0; k < 8; k++) a += double(rv[k]) * double(rv[k]);
   a = wave_sum(a);
   if (lane == 0) c.part[T.leaf_slot] = a;
+}
+
+// The last iteration's pending x += alpha p on INTERIOR cells of LEAF tiles,
+// once after a problem's iterations end (every problem of the list, done or
+// not; none when no iteration ran).
+__global__ __launch_bounds__(64) void k_cg_finish_x(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
+  TILE_KERNEL_PROLOGUE
+  const int lane = lane_id();
+  const size_t base = size_t(T.self) * 512 + lane * 8;
+  float pv[8], xv[8];
+  load8(c.p + base, pv);
+  load8(c.x + base, xv);
+  const uint64_t ct = load_ct8(c.ctype + base);
+  const ProbState& ps = c.ps[T.prob];
+  if (ps.iter <= 0) return;
+  const float alpha = float(ps.alpha);
+  for (int k = 0; k < 8; k++)
+    if (ct_at(ct, k) & C_INT) xv[k] = __builtin_fmaf(alpha, pv[k], xv[k]);
+  store8(c.x + base, xv);
 }
 
 // Sum of a problem's per-tile partials in the fixed order (lane q adds

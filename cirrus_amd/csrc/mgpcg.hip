@@ -1248,6 +1248,8 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
     have_prev = true;
     slot = 1 - slot;
   }
+  // the last iteration's x update (k_cg_update_p applies each earlier one)
+  k_cg_finish_x<<<grid_for(P.leaf.n, P.swz), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.swz);
   if (f16s) {  // back to the caller's scale: x and the residual r
     k_scale_leaf<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.x, -1);
     k_scale_leaf<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, -1);

@@ -591,6 +591,8 @@ void server_collect(cg_server& S, int k) {
     I.res.seconds = std::chrono::duration<double>(now - I.t_submit).count();
     S.active.erase(std::find(S.active.begin(), S.active.end(), q));
     S.dirty = true;
+    // the last iteration's x update (k_cg_update_p applies each earlier one)
+    k_cg_finish_x<<<grid_for(I.leaf.n, kSwizzle), 64, 0, S.stream>>>(S.view.ctx(), I.L(I.leaf), I.leaf.n, kSwizzle);
     if (S.refine) {
       // the step's recursive residual, relative to the request's b
       I.bb_b = s.bb_b;
