@@ -2018,6 +2018,19 @@ __global__ __launch_bounds__(64) void k_perturb_rhs(Ctx c, const TileRec* __rest
   store8(B + base, b);
 }
 
+// Staged problem states to their slots: src[i] -> ps[map[i]], i < n (one
+// workgroup per state).
+__global__ __launch_bounds__(64) void k_ps_scatter(ProbState* __restrict__ ps, const ProbState* __restrict__ src,
+                                                   const int32_t* __restrict__ map, int n) {
+  static_assert(sizeof(ProbState) % 16 == 0, "ProbState is copied in 16-byte words");
+  constexpr int W = sizeof(ProbState) / 16;
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const int4* from = reinterpret_cast<const int4*>(src + i);
+  int4* to = reinterpret_cast<int4*>(ps + map[i]);
+  for (int k = threadIdx.x; k < W; k += blockDim.x) to[k] = from[k];
+}
+
 // Copy runs of tile records into an assembled worklist: run q copies
 // counts[q] records from srcs[q] to dst + offs[q] (grid-stride over records).
 __global__ __launch_bounds__(256) void k_gather_recs(const TileRec* const* __restrict__ srcs, const int32_t* __restrict__ offs,
@@ -3080,15 +3093,17 @@ __global__ __launch_bounds__(64) void k_ir_accumulate(Ctx c, const TileRec* __re
   for (int k = 0; k < 8; k++) xd[k] = xd[k] + double(xf[k]);
   store8d(X64 + base, xd);
 }
-// One problem's |r64|^2 (its leaf-slot partials) and |b|^2 (ProbState::bb_b)
-// into out[0], out[1].
+// Per problem q (workgroup -> problem through pmap, else the identity): its
+// |r64|^2 (the leaf-slot partials) and |b|^2 (ProbState::bb_b) into
+// out[2q], out[2q + 1].
 __global__ __launch_bounds__(FIN_THREADS) void k_ir_verdict(Ctx c, double* __restrict__ out) {
   __shared__ double lds[FIN_CHUNK];
-  const ProbState* ps = c.ps + blockIdx.x;
+  const int q = c.pmap ? c.pmap[blockIdx.x] : int(blockIdx.x);
+  const ProbState* ps = c.ps + q;
   const double s = block_sum_partials(c.part, ps->leaf_begin, ps->leaf_count, lds);
   if (threadIdx.x == 0) {
-    out[2 * blockIdx.x] = s;
-    out[2 * blockIdx.x + 1] = ps->bb_b;
+    out[2 * q] = s;
+    out[2 * q + 1] = ps->bb_b;
   }
 }
 // Per-problem totals of the leaf-slot partials, in the fixed order.

@@ -2447,7 +2447,7 @@ cg_status cg_server_info(const cg_server* s, int64_t* capacity_tiles, int64_t* f
     for (auto& kv : s->tiles_alloc.free_) f += kv.second;
     *free_tiles = f;
   }
-  if (num_instances) *num_instances = s->max_probs - int32_t(s->free_slots.size());
+  if (num_instances) *num_instances = s->nres;
   return CG_OK;
 }
 

@@ -2,7 +2,7 @@
 //
 // Instances are resident hierarchies: each admitted tree gets a slab of tiles,
 // leaf slots and ghost slots in device arrays sized from an HBM budget, its own
-// tile records on the device and a pinned staging buffer.  A request is a right
+// tile records in a device arena and a pinned staging buffer.  A request is a right
 // hand side for an instance (optionally warm-started from the instance's
 // current pressure).  Every cg_server_step enqueues one CG iteration over the
 // instances whose request is running -- their worklists assembled on the device
@@ -17,6 +17,14 @@
 // A d = float(r) with the MGPCG until its residual is below half the tolerance,
 // adds d to x, and checks |b - A x| in FP64; a request whose check fails runs
 // another such step (kMaxRefineSteps at most) before it is reported.
+//
+// The per-request work around the iterations (a start's residual and state, a
+// finish's last x update and FP64 check, a passed check's x) runs once per step
+// for all the requests that start, finish or pass in that step, over worklists
+// assembled like the active set's.  A release hands the instance's memory back
+// once the stream has passed it (an event), so it neither waits for the device
+// nor frees device memory: records and coarse slots live in server-wide arenas
+// and pinned staging buffers are kept for later admissions.
 #pragma once
 
 #include <deque>
@@ -36,13 +44,16 @@ struct Instance {
   int max_level = 0, C = -1;
   bool rz_ok = false;    // every LEAF tile is on a grid-wide level >= 1
   std::vector<int> ext;  // internal tile i -> caller index
-  TileRec* d_recs = nullptr;
+  TileRec* d_recs = nullptr;  // the instance's records: rec_count records of the server's arena from rec_begin
+  int64_t rec_begin = 0;
+  int rec_count = 0;
   DList leaf, gather, applyl, ghosts, all;
   std::vector<DList> actg, lvlg;
-  CoarseSlot* d_slots = nullptr;
+  CoarseSlot* d_slots = nullptr;  // the slot's part of the server's coarse-slot arena
   CoarsePlan plan;
   float* h_buf = nullptr;  // pinned: rhs on the way in, pressure on the way out
-  hipEvent_t ev = nullptr;
+  size_t h_bytes = 0;      // its capacity (a kept buffer may be larger than the instance needs)
+  hipEvent_t ev = nullptr;      // the slot's events (the server's)
   hipEvent_t ev_ver = nullptr;  // the FP64 check's result has arrived
   int state = INST_IDLE;
   int steps = 0;        // refinement steps run for the current request
@@ -58,16 +69,41 @@ struct Instance {
   const TileRec* L(const DList& d) const { return d_recs + d.off; }
 };
 
+// A released instance's resources, handed back once the stream passes `ev`.
+struct Released {
+  hipEvent_t ev = nullptr;
+  int slot = -1;
+  int64_t tile_begin = 0, leaf_begin = 0, ghost_begin = 0, rec_begin = 0;
+  int ntiles = 0, nleaf = 0, nghost = 0, rec_count = 0;
+  float* h_buf = nullptr;
+  size_t h_bytes = 0;
+};
+
+// Worklists of a group of instances (the requests that start, finish or pass
+// their FP64 check in one step), assembled in the server's group arena: leaf
+// over all of them, leaf_full / ghosts / applyl over the first nfull.
+struct Group {
+  int n = 0, nfull = 0;
+  DList leaf, leaf_full, ghosts, applyl;
+};
+enum : int { GRP_START = 0, GRP_FINISH = 1, GRP_PASSED = 2, GRP_REGIONS = 3 };
+
 }  // namespace
 
 struct cg_server {
   cg_params prm{};
   int refine = 0;  // every request is an FP64 iterative refinement
   VParams v;
-  int max_probs = 0;
+  int max_probs = 0;  // resident instances at most
+  int nslots = 0;     // problem slots: max_probs and room for releases the stream has not yet passed
+  int nres = 0;       // resident instances
   int64_t cap_tiles = 0, cap_leaf = 0, cap_ghost = 0;
-  RangeAlloc tiles_alloc, leaf_alloc, ghost_alloc;
+  RangeAlloc tiles_alloc, leaf_alloc, ghost_alloc, rec_alloc;
   std::vector<int> free_slots;
+  std::vector<Released> pending;  // releases the stream has not yet passed
+  std::vector<hipEvent_t> ev_free;
+  std::multimap<size_t, float*> hbuf_cache;  // pinned staging buffers of released instances, by size
+  std::vector<hipEvent_t> slot_ev, slot_ev_ver;
   std::vector<std::unique_ptr<Instance>> inst;  // by slot
   std::vector<int> active;                      // slots whose request runs, assembly order
   std::vector<int> snap_active[2];              // active set of the iteration each snapshot follows
@@ -97,21 +133,30 @@ struct cg_server {
   int32_t* d_dflag = nullptr;
   int32_t* d_dflag_init = nullptr;
   int32_t* d_pmap = nullptr;
-  int32_t* d_newmap = nullptr;
   double* d_part = nullptr;
   double* d_part2 = nullptr;
   TileRec* d_asm = nullptr;
   int64_t asm_cap = 0;
+  TileRec* d_recs = nullptr;  // arena of the instances' own records
+  CoarseSlot* d_cslots = nullptr;  // arena of the instances' coarse slots (CL_MAX_TILES per slot)
+  TileRec* d_gasm = nullptr;  // group worklists
+  int64_t gasm_cap = 0;
+  // group tables (runs, map, staged problem states): pinned, per step parity and
+  // region; on the device one region, reused in stream order
+  char* h_gtab[2] = {nullptr, nullptr};
+  char* d_gtab = nullptr;
+  size_t gstride = 0, goff_map = 0, goff_ps = 0;
+  hipEvent_t ev_gtab[2][GRP_REGIONS] = {};
+  bool gtab_used[2][GRP_REGIONS] = {};
   CoarsePlan* d_plans = nullptr;
   const TileRec** d_srcs = nullptr;
   int32_t* d_offs = nullptr;
   int32_t* d_counts = nullptr;
   int runs_cap = 0;
   hipStream_t stream = nullptr;
-  // pinned host memory: ProbState snapshots, per-slot ProbState staging, per-step small tables
+  // pinned host memory: ProbState snapshots, per-step small tables
   ProbState* h_snap[2] = {nullptr, nullptr};
   hipEvent_t ev_snap[2] = {nullptr, nullptr};
-  ProbState* h_ps_stage = nullptr;
   char* h_tab[2] = {nullptr, nullptr};
   size_t tab_cap = 0;
   ~cg_server();
@@ -119,35 +164,44 @@ struct cg_server {
 
 cg_server::~cg_server() {
   if (stream) (void)hipStreamSynchronize(stream);
-  for (auto& I : inst) {
-    if (!I) continue;
-    if (I->d_recs) (void)hipFree(I->d_recs);
-    if (I->d_slots) (void)hipFree(I->d_slots);
-    if (I->h_buf) (void)hipHostFree(I->h_buf);
-    if (I->ev) (void)hipEventDestroy(I->ev);
-    if (I->ev_ver) (void)hipEventDestroy(I->ev_ver);
+  for (auto& I : inst)
+    if (I && I->h_buf) (void)hipHostFree(I->h_buf);
+  for (auto& R : pending) {
+    if (R.h_buf) (void)hipHostFree(R.h_buf);
+    if (R.ev) (void)hipEventDestroy(R.ev);
   }
+  for (auto& kv : hbuf_cache) (void)hipHostFree(kv.second);
+  for (auto e : ev_free) (void)hipEventDestroy(e);
+  for (auto e : slot_ev) if (e) (void)hipEventDestroy(e);
+  for (auto e : slot_ev_ver) if (e) (void)hipEventDestroy(e);
+  for (auto& row : ev_gtab)
+    for (auto e : row) if (e) (void)hipEventDestroy(e);
   if (h_ver) (void)hipHostFree(h_ver);
-  void* ptrs[] = {d_ctype, d_fmask, d_diag, d_zb, d_b, d_x64, d_r64, d_ver, d_zh, d_rhsvh, d_zbh, d_gsum, d_ps, d_dflag, d_dflag_init, d_pmap, d_newmap, d_part, d_part2, d_asm,
-                  d_plans, (void*)d_srcs, d_offs, d_counts};
+  void* ptrs[] = {d_ctype, d_fmask, d_diag, d_zb, d_b, d_x64, d_r64, d_ver, d_zh, d_rhsvh, d_zbh, d_gsum, d_ps, d_dflag,
+                  d_dflag_init, d_pmap, d_part, d_part2, d_asm, d_recs, d_cslots, d_gasm, d_gtab, d_plans,
+                  (void*)d_srcs, d_offs, d_counts};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
   for (auto e : ev_snap) if (e) (void)hipEventDestroy(e);
   for (auto q : h_snap) if (q) (void)hipHostFree(q);
   for (auto q : h_tab) if (q) (void)hipHostFree(q);
-  if (h_ps_stage) (void)hipHostFree(h_ps_stage);
+  for (auto q : h_gtab) if (q) (void)hipHostFree(q);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
 namespace {
 
 // Device bytes one tile costs: 6 channels, the V-cycle scratch, the kept rhs
-// and the colour sweep's diagonal (FP32), cell types and face masks, ghost row sums, dot partials and room
-// for its records in the assembled worklists; BF16 storage adds three bfloat16
-// vectors, verify_f64 two FP64 vectors.
+// and the colour sweep's diagonal (FP32), cell types and face masks, ghost row
+// sums, dot partials and room for its records: the instance's own lists, the
+// assembled active worklists and the group worklists; BF16 storage adds three
+// bfloat16 vectors, verify_f64 two FP64 vectors.
 constexpr int kRecKindsPerTile = 5;  // leaf, gather, applyl, actg, lvlg: at most 5 records per tile
-constexpr int64_t kBytesPerTile =
-    (CG_NUM_CHANNELS + 3) * 512 * 4 + 2 * 512 + 64 * 4 + 2 * 8 + kRecKindsPerTile * int64_t(sizeof(TileRec));
+// An instance's own lists (leaf, gather, applyl, ghosts, all, actg, lvlg) hold
+// 3 leaf + ghost + 3 n <= 6 n records; a group's (leaf, ghosts, applyl) <= 2 n.
+constexpr int kOwnRecsPerTile = 6, kGroupRecsPerTile = 2;
+constexpr int64_t kBytesPerTile = (CG_NUM_CHANNELS + 3) * 512 * 4 + 2 * 512 + 64 * 4 + 2 * 8 +
+                                  (kRecKindsPerTile + kOwnRecsPerTile + kGroupRecsPerTile) * int64_t(sizeof(TileRec));
 
 void server_create(const cg_server_config* cfg, cg_server& S) {
   if (!cfg || cfg->budget_bytes <= 0 || cfg->max_problems <= 0) fail(CG_ERR_INVALID, "cg_server_create: bad configuration");
@@ -162,6 +216,7 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   S.v.ooa = prm.one_over_alpha;
   S.v.pcoef = prm.prolong_coeff;
   S.max_probs = cfg->max_problems;
+  S.nslots = S.max_probs + std::max(16, S.max_probs / 8);
   S.refine = cfg->refine ? 1 : 0;
   const bool f64 = prm.verify_f64 || S.refine;
   const int64_t per_tile = kBytesPerTile + (prm.storage == CG_STORE_BF16 ? 3 * 512 * int64_t(sizeof(bf16)) : 0) +
@@ -173,8 +228,11 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   S.tiles_alloc.init(S.cap_tiles);
   S.leaf_alloc.init(S.cap_leaf);
   S.ghost_alloc.init(S.cap_ghost);
-  S.inst.resize(S.max_probs);
-  for (int q = S.max_probs - 1; q >= 0; q--) S.free_slots.push_back(q);
+  S.rec_alloc.init(S.cap_tiles * kOwnRecsPerTile);
+  S.inst.resize(S.nslots);
+  S.slot_ev.assign(S.nslots, nullptr);
+  S.slot_ev_ver.assign(S.nslots, nullptr);
+  for (int q = S.nslots - 1; q >= 0; q--) S.free_slots.push_back(q);
 
   HIPC(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
   const size_t cells = size_t(S.cap_tiles) * 512;
@@ -191,8 +249,8 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
     HIPC(hipMalloc(&S.d_r64, cells * sizeof(double)));
     HIPC(hipMemsetAsync(S.d_x64, 0, cells * sizeof(double), S.stream));
     HIPC(hipMemsetAsync(S.d_r64, 0, cells * sizeof(double), S.stream));
-    HIPC(hipMalloc(&S.d_ver, 2 * sizeof(double) * S.max_probs));
-    HIPC(hipHostMalloc(&S.h_ver, 2 * sizeof(double) * S.max_probs, hipHostMallocDefault));
+    HIPC(hipMalloc(&S.d_ver, 2 * sizeof(double) * S.nslots));
+    HIPC(hipHostMalloc(&S.h_ver, 2 * sizeof(double) * S.nslots, hipHostMallocDefault));
   }
   if (S.prm.storage == CG_STORE_BF16) {
     bf16** bufs[3] = {&S.d_zh, &S.d_rhsvh, &S.d_zbh};
@@ -202,27 +260,40 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
     }
   }
   HIPC(hipMalloc(&S.d_gsum, size_t(S.cap_ghost) * 64 * sizeof(float)));
-  HIPC(hipMalloc(&S.d_ps, sizeof(ProbState) * S.max_probs));
-  HIPC(hipMemsetAsync(S.d_ps, 0, sizeof(ProbState) * S.max_probs, S.stream));
-  HIPC(hipMalloc(&S.d_dflag, sizeof(int32_t) * S.max_probs));
-  HIPC(hipMalloc(&S.d_dflag_init, sizeof(int32_t) * S.max_probs));
+  HIPC(hipMalloc(&S.d_ps, sizeof(ProbState) * S.nslots));
+  HIPC(hipMemsetAsync(S.d_ps, 0, sizeof(ProbState) * S.nslots, S.stream));
+  HIPC(hipMalloc(&S.d_dflag, sizeof(int32_t) * S.nslots));
+  HIPC(hipMalloc(&S.d_dflag_init, sizeof(int32_t) * S.nslots));
   HIPC(hipMalloc(&S.d_pmap, sizeof(int32_t) * S.max_probs));
-  HIPC(hipMalloc(&S.d_newmap, sizeof(int32_t) * S.max_probs));
   HIPC(hipMalloc(&S.d_part, sizeof(double) * S.cap_leaf));
   HIPC(hipMalloc(&S.d_part2, sizeof(double) * S.cap_leaf));
   S.asm_cap = S.cap_tiles * kRecKindsPerTile;
   HIPC(hipMalloc(&S.d_asm, sizeof(TileRec) * S.asm_cap));
+  HIPC(hipMalloc(&S.d_recs, sizeof(TileRec) * S.cap_tiles * kOwnRecsPerTile));
+  HIPC(hipMalloc(&S.d_cslots, sizeof(CoarseSlot) * CL_MAX_TILES * S.nslots));
+  S.gasm_cap = S.cap_tiles * kGroupRecsPerTile;
+  HIPC(hipMalloc(&S.d_gasm, sizeof(TileRec) * S.gasm_cap));
+  {
+    const size_t R = size_t(3) * S.nslots;  // runs: leaf, ghosts and applyl of each instance
+    S.goff_map = R * (sizeof(TileRec*) + 8);
+    S.goff_ps = (S.goff_map + sizeof(int32_t) * S.nslots + 63) & ~size_t(63);
+    S.gstride = (S.goff_ps + sizeof(ProbState) * S.nslots + 255) & ~size_t(255);
+    for (int k = 0; k < 2; k++) {
+      HIPC(hipHostMalloc(&S.h_gtab[k], S.gstride * GRP_REGIONS, hipHostMallocDefault));
+      for (int g = 0; g < GRP_REGIONS; g++) HIPC(hipEventCreateWithFlags(&S.ev_gtab[k][g], hipEventDisableTiming));
+    }
+    HIPC(hipMalloc(&S.d_gtab, S.gstride));
+  }
   HIPC(hipMalloc(&S.d_plans, sizeof(CoarsePlan) * S.max_probs));
   S.runs_cap = S.max_probs * (3 + 2 * 32);
   HIPC(hipMalloc(&S.d_srcs, sizeof(TileRec*) * S.runs_cap));
   HIPC(hipMalloc(&S.d_offs, sizeof(int32_t) * S.runs_cap));
   HIPC(hipMalloc(&S.d_counts, sizeof(int32_t) * S.runs_cap));
   for (int k = 0; k < 2; k++) {
-    HIPC(hipHostMalloc(&S.h_snap[k], sizeof(ProbState) * S.max_probs, hipHostMallocDefault));
+    HIPC(hipHostMalloc(&S.h_snap[k], sizeof(ProbState) * S.nslots, hipHostMallocDefault));
     HIPC(hipEventCreateWithFlags(&S.ev_snap[k], hipEventDisableTiming));
   }
-  HIPC(hipHostMalloc(&S.h_ps_stage, sizeof(ProbState) * S.max_probs, hipHostMallocDefault));
-  // per-step tables: runs (srcs, offs, counts), active map, new map, coarse plans
+  // per-step tables: runs (srcs, offs, counts), active map, coarse plans
   S.tab_cap = size_t(S.runs_cap) * (sizeof(TileRec*) + 8) + size_t(S.max_probs) * (8 + sizeof(CoarsePlan)) + 4096;
   for (int k = 0; k < 2; k++) HIPC(hipHostMalloc(&S.h_tab[k], S.tab_cap, hipHostMallocDefault));
 
@@ -251,16 +322,58 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   HIPC(hipStreamSynchronize(S.stream));  // the zero fills above are done before any request
 }
 
+// Releases whose point the stream has passed (all of them, waiting, if `wait`)
+// give their slot, ranges and staging buffer back.
+void reap_releases(cg_server& S, bool wait) {
+  for (size_t i = 0; i < S.pending.size();) {
+    Released& R = S.pending[i];
+    if (wait) {
+      HIPC(hipEventSynchronize(R.ev));
+    } else {
+      const hipError_t e = hipEventQuery(R.ev);
+      if (e == hipErrorNotReady) { i++; continue; }
+      HIPC(e);
+    }
+    if (!S.tiles_alloc.release(R.tile_begin, R.ntiles) || !S.leaf_alloc.release(R.leaf_begin, R.nleaf) ||
+        !S.ghost_alloc.release(R.ghost_begin, R.nghost) || !S.rec_alloc.release(R.rec_begin, R.rec_count))
+      fail(CG_ERR_STATE, "cg_server_release: the instance's ranges were already free");
+    S.free_slots.push_back(R.slot);
+    if (R.h_buf) {
+      if (S.hbuf_cache.size() < size_t(S.max_probs)) S.hbuf_cache.emplace(R.h_bytes, R.h_buf);
+      else HIPC(hipHostFree(R.h_buf));
+    }
+    S.ev_free.push_back(R.ev);
+    S.pending.erase(S.pending.begin() + i);
+  }
+}
+
+// A range of `n` from `A`; when none is free and releases are pending, after
+// waiting for them.
+int64_t alloc_range(cg_server& S, RangeAlloc& A, int64_t n) {
+  int64_t at = A.alloc(n);
+  if (at < 0 && !S.pending.empty()) {
+    reap_releases(S, true);
+    at = A.alloc(n);
+  }
+  return at;
+}
+
+void admit_upload(cg_server& S, const cg_tree_desc* d, const TreeBuild& B, const std::vector<TileRec>& W,
+                  const std::vector<CoarseSlot>& cslots, Instance& inst, int slot, int64_t tb, int64_t rb);
+
 void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
   if (!d || !out) fail(CG_ERR_INVALID, "cg_server_admit: null argument");
   if (d->num_tiles <= 0) fail(CG_ERR_INVALID, "cg_server_admit: empty tree");
+  if (!S.pending.empty()) reap_releases(S, false);
+  if (S.nres >= S.max_probs) fail(CG_ERR_BUSY, "cg_server_admit: no free problem slot");
+  if (S.free_slots.empty()) reap_releases(S, true);
   if (S.free_slots.empty()) fail(CG_ERR_BUSY, "cg_server_admit: no free problem slot");
   const int n = d->num_tiles;
-  const int64_t tb = S.tiles_alloc.alloc(n);
+  const int64_t tb = alloc_range(S, S.tiles_alloc, n);
   if (tb < 0) fail(CG_ERR_BUSY, "cg_server_admit: HBM budget exhausted");
   // leaf and ghost counts are known only after the build: reserve the tile count
   // first, trim after
-  const int64_t lb = S.leaf_alloc.alloc(n), gb = S.ghost_alloc.alloc(n);
+  const int64_t lb = alloc_range(S, S.leaf_alloc, n), gb = alloc_range(S, S.ghost_alloc, n);
   if (lb < 0 || gb < 0) {
     S.tiles_alloc.release(tb, n);
     if (lb >= 0) S.leaf_alloc.release(lb, n);
@@ -268,6 +381,7 @@ void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
     fail(CG_ERR_BUSY, "cg_server_admit: HBM budget exhausted");
   }
   const int slot = S.free_slots.back();
+  S.free_slots.pop_back();
   auto I = std::make_unique<Instance>();
   TreeBuild B;
   try {
@@ -276,11 +390,11 @@ void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
     S.tiles_alloc.release(tb, n);
     S.leaf_alloc.release(lb, n);
     S.ghost_alloc.release(gb, n);
+    S.free_slots.push_back(slot);
     throw;
   }
   S.leaf_alloc.release(lb + B.nleaf, n - B.nleaf);
   S.ghost_alloc.release(gb + B.nghost, n - B.nghost);
-  S.free_slots.pop_back();
   I->slot = slot;
   I->tile_begin = tb;
   I->leaf_begin = lb;
@@ -316,13 +430,48 @@ void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
     I->actg[l] = add_list([&](int i) { return B.lev[i] == l && B.ty[i] != GHOST && grid_level(i); });
     I->lvlg[l] = add_list([&](int i) { return B.lev[i] == l && grid_level(i); });
   }
-  HIPC(hipMalloc(&I->d_recs, sizeof(TileRec) * W.size()));
+  std::vector<CoarseSlot> cslots;
+  if (I->C >= 0) coarse_plan(B.tiles, int(tb), I->C, slot, I->plan, cslots);
+  const bool fits = W.size() <= size_t(n) * kOwnRecsPerTile && cslots.size() <= size_t(CL_MAX_TILES);
+  const int64_t rb = fits ? alloc_range(S, S.rec_alloc, int64_t(W.size())) : -2;
+  auto undo = [&] {
+    S.tiles_alloc.release(tb, n);
+    S.leaf_alloc.release(lb, B.nleaf);
+    S.ghost_alloc.release(gb, B.nghost);
+    if (rb >= 0) S.rec_alloc.release(rb, int64_t(W.size()));
+    S.free_slots.push_back(slot);
+  };
+  if (rb < 0) {
+    undo();
+    if (rb == -2) fail(CG_ERR_INVALID, "cg_server_admit: more records or coarse slots than the tree allows");
+    fail(CG_ERR_BUSY, "cg_server_admit: HBM budget exhausted");
+  }
+  try {
+    admit_upload(S, d, B, W, cslots, *I, slot, tb, rb);
+  } catch (...) {
+    undo();
+    if (I->h_buf) (void)hipHostFree(I->h_buf);
+    throw;
+  }
+  *out = slot;
+  S.inst[slot] = std::move(I);
+  S.nres++;
+}
+
+// The admitted instance's records, coarse slots and cell types to the device,
+// its face masks, diagonal and zeroed vectors (on the server's stream), its
+// staging buffer and events.
+void admit_upload(cg_server& S, const cg_tree_desc* d, const TreeBuild& B, const std::vector<TileRec>& W,
+                  const std::vector<CoarseSlot>& cslots, Instance& inst, int slot, int64_t tb, int64_t rb) {
+  Instance* I = &inst;
+  const int n = d->num_tiles;
+  I->rec_begin = rb;
+  I->rec_count = int(W.size());
+  I->d_recs = S.d_recs + rb;
   HIPC(hipMemcpy(I->d_recs, W.data(), sizeof(TileRec) * W.size(), hipMemcpyHostToDevice));
   if (I->C >= 0) {
-    std::vector<CoarseSlot> slots;
-    coarse_plan(B.tiles, int(tb), I->C, slot, I->plan, slots);
-    HIPC(hipMalloc(&I->d_slots, sizeof(CoarseSlot) * slots.size()));
-    HIPC(hipMemcpy(I->d_slots, slots.data(), sizeof(CoarseSlot) * slots.size(), hipMemcpyHostToDevice));
+    I->d_slots = S.d_cslots + size_t(slot) * CL_MAX_TILES;
+    HIPC(hipMemcpy(I->d_slots, cslots.data(), sizeof(CoarseSlot) * cslots.size(), hipMemcpyHostToDevice));
     I->plan.slots = I->d_slots;
   }
   // cell types in internal order; x = 0 so that a first warm start is a cold one
@@ -336,41 +485,62 @@ void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
   HIPC(hipMemsetAsync(S.d_ch[CG_CH_X] + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float), S.stream));
   HIPC(hipMemsetAsync(S.d_b + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float), S.stream));
   if (S.d_x64) HIPC(hipMemsetAsync(S.d_x64 + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(double), S.stream));
-  HIPC(hipHostMalloc(&I->h_buf, size_t(n) * 512 * sizeof(float), hipHostMallocDefault));
-  HIPC(hipEventCreateWithFlags(&I->ev, hipEventDisableTiming));
-  HIPC(hipEventCreateWithFlags(&I->ev_ver, hipEventDisableTiming));
-  *out = slot;
-  S.inst[slot] = std::move(I);
+  const size_t need = size_t(n) * 512 * sizeof(float);
+  auto kept = S.hbuf_cache.lower_bound(need);
+  if (kept != S.hbuf_cache.end() && kept->first <= 2 * need) {
+    I->h_bytes = kept->first;
+    I->h_buf = kept->second;
+    S.hbuf_cache.erase(kept);
+  } else {
+    HIPC(hipHostMalloc(&I->h_buf, need, hipHostMallocDefault));
+    I->h_bytes = need;
+  }
+  if (!S.slot_ev[slot]) HIPC(hipEventCreateWithFlags(&S.slot_ev[slot], hipEventDisableTiming));
+  if (!S.slot_ev_ver[slot]) HIPC(hipEventCreateWithFlags(&S.slot_ev_ver[slot], hipEventDisableTiming));
+  I->ev = S.slot_ev[slot];
+  I->ev_ver = S.slot_ev_ver[slot];
 }
 
 Instance& instance(cg_server& S, int32_t id) {
-  if (id < 0 || id >= S.max_probs || !S.inst[id]) fail(CG_ERR_INVALID, "unknown instance");
+  if (id < 0 || id >= S.nslots || !S.inst[id]) fail(CG_ERR_INVALID, "unknown instance");
   return *S.inst[id];
 }
 
+// The instance leaves at once; its slot, ranges and staging buffer come back
+// once the stream has passed this point (no kernel may still read its slab).
 void server_release(cg_server& S, int32_t id) {
   Instance& I = instance(S, id);
   if (I.state == INST_QUEUED || I.state == INST_ACTIVE || I.state == INST_VERIFY)
     fail(CG_ERR_STATE, "cg_server_release: the instance has a request running");
-  HIPC(hipStreamSynchronize(S.stream));  // no kernel may still read its slab
+  Released R;
+  if (S.ev_free.empty()) {
+    HIPC(hipEventCreateWithFlags(&R.ev, hipEventDisableTiming));
+  } else {
+    R.ev = S.ev_free.back();
+    S.ev_free.pop_back();
+  }
+  const hipError_t e = hipEventRecord(R.ev, S.stream);
+  if (e != hipSuccess) {
+    S.ev_free.push_back(R.ev);
+    HIPC(e);
+  }
   if (I.state == INST_DONE)
     for (auto it = S.done_q.begin(); it != S.done_q.end(); ++it)
       if (*it == id) { S.done_q.erase(it); break; }
-  if (!S.tiles_alloc.release(I.tile_begin, I.ntiles) || !S.leaf_alloc.release(I.leaf_begin, I.nleaf) ||
-      !S.ghost_alloc.release(I.ghost_begin, I.nghost))
-    fail(CG_ERR_STATE, "cg_server_release: the instance's ranges were already free");
-  HIPC(hipFree(I.d_recs));
-  I.d_recs = nullptr;
-  if (I.d_slots) HIPC(hipFree(I.d_slots));
-  I.d_slots = nullptr;
-  HIPC(hipHostFree(I.h_buf));
-  I.h_buf = nullptr;
-  HIPC(hipEventDestroy(I.ev));
-  I.ev = nullptr;
-  HIPC(hipEventDestroy(I.ev_ver));
-  I.ev_ver = nullptr;
+  R.slot = id;
+  R.tile_begin = I.tile_begin;
+  R.leaf_begin = I.leaf_begin;
+  R.ghost_begin = I.ghost_begin;
+  R.rec_begin = I.rec_begin;
+  R.ntiles = I.ntiles;
+  R.nleaf = I.nleaf;
+  R.nghost = I.nghost;
+  R.rec_count = I.rec_count;
+  R.h_buf = I.h_buf;
+  R.h_bytes = I.h_bytes;
+  S.pending.push_back(R);
   S.inst[id].reset();
-  S.free_slots.push_back(id);
+  S.nres--;
 }
 
 void server_submit(cg_server& S, int32_t id, const float* rhs, int32_t flags, int64_t tag) {
@@ -389,47 +559,108 @@ void server_submit(cg_server& S, int32_t id, const float* rhs, int32_t flags, in
   I.state = INST_QUEUED;
 }
 
-// A Ctx whose ProbState array starts at the instance's slot, for one-workgroup
-// per-problem reductions.
-Ctx one_problem(cg_server& S, const Instance& I) {
-  Ctx cv = S.view.ctx();
-  cv.ps = S.d_ps + I.slot;
-  cv.pmap = nullptr;
-  return cv;
+const TileRec* GL(const cg_server& S, const DList& d) { return S.d_gasm + d.off; }
+const int32_t* gmap(const cg_server& S) { return reinterpret_cast<const int32_t*>(S.d_gtab + S.goff_map); }
+// The kernels' Ctx for one workgroup per group member (workgroup a -> slot map[a]).
+Ctx group_ctx(cg_server& S) {
+  Ctx c = S.view.ctx();
+  c.pmap = gmap(S);
+  return c;
 }
 
-// r64 = b - A x64 on the instance's LEAF tiles in FP64 (re-centred for pure
-// Neumann) with |r64|^2 partials; write_r: float(r64) into the r channel.
-void instance_residual64(cg_server& S, Instance& I, bool write_r) {
+// A group's worklists: each listed instance's runs copied from its own lists
+// into the group arena (the leaf runs of every instance, the ghost and operator
+// runs of the first nfull), with the slots as the group map and, for a start,
+// the instances' new problem states scattered to their slots.  The host table
+// of (step parity k, region) is reused only after its last upload has run.
+Group group_lists(cg_server& S, int k, int region, const std::vector<int>& slots, int nfull, const ProbState* ps) {
+  Group G;
+  G.n = int(slots.size());
+  G.nfull = nfull;
+  const hipStream_t st = S.stream;
+  if (S.gtab_used[k][region]) HIPC(hipEventSynchronize(S.ev_gtab[k][region]));
+  char* tab = S.h_gtab[k] + size_t(region) * S.gstride;
+  const int R = 3 * S.nslots;
+  auto** srcs = reinterpret_cast<const TileRec**>(tab);
+  auto* offs = reinterpret_cast<int32_t*>(tab + sizeof(TileRec*) * R);
+  auto* cnts = offs + R;
+  auto* map = reinterpret_cast<int32_t*>(tab + S.goff_map);
+  int nr = 0;
+  int64_t pos = 0;
+  auto kind = [&](int a0, int a1, auto get) {
+    DList out;
+    out.off = int(pos);
+    for (int a = a0; a < a1; a++) {
+      const Instance& I = *S.inst[slots[a]];
+      const DList& dl = get(I);
+      if (dl.n == 0) continue;
+      srcs[nr] = I.L(dl);
+      offs[nr] = int32_t(pos);
+      cnts[nr] = dl.n;
+      nr++;
+      pos += dl.n;
+    }
+    out.n = int(pos) - out.off;
+    return out;
+  };
+  G.leaf_full = kind(0, nfull, [](const Instance& I) -> const DList& { return I.leaf; });
+  const DList more = kind(nfull, G.n, [](const Instance& I) -> const DList& { return I.leaf; });
+  G.leaf = DList{G.leaf_full.off, G.leaf_full.n + more.n};
+  G.ghosts = kind(0, nfull, [](const Instance& I) -> const DList& { return I.ghosts; });
+  G.applyl = kind(0, nfull, [](const Instance& I) -> const DList& { return I.applyl; });
+  if (pos > S.gasm_cap) fail(CG_ERR_BUSY, "cg_server_step: group worklists exceed their reserved space");
+  for (int a = 0; a < G.n; a++) map[a] = slots[a];
+  size_t bytes = S.goff_map + sizeof(int32_t) * G.n;
+  if (ps) {
+    std::memcpy(tab + S.goff_ps, ps, sizeof(ProbState) * G.n);
+    bytes = S.goff_ps + sizeof(ProbState) * G.n;
+  }
+  HIPC(hipMemcpyAsync(S.d_gtab, tab, bytes, hipMemcpyHostToDevice, st));
+  HIPC(hipEventRecord(S.ev_gtab[k][region], st));
+  S.gtab_used[k][region] = true;
+  if (ps && G.n)
+    k_ps_scatter<<<unsigned(G.n), 64, 0, st>>>(S.d_ps, reinterpret_cast<const ProbState*>(S.d_gtab + S.goff_ps), gmap(S), G.n);
+  if (nr)
+    k_gather_recs<<<dim3(64, unsigned(std::min(nr, 65535))), 256, 0, st>>>(
+        reinterpret_cast<const TileRec* const*>(S.d_gtab), reinterpret_cast<const int32_t*>(S.d_gtab + sizeof(TileRec*) * R),
+        reinterpret_cast<const int32_t*>(S.d_gtab + sizeof(TileRec*) * R + sizeof(int32_t) * R), nr, S.d_gasm);
+  HIPC(hipGetLastError());
+  return G;
+}
+
+// r64 = b - A x64 on the LEAF tiles of the group's first nfull instances in
+// FP64 (re-centred for pure Neumann) with |r64|^2 partials; write_r:
+// float(r64) into the r channel.
+void group_residual64(cg_server& S, const Group& G, bool write_r) {
   const hipStream_t st = S.stream;
   const Ctx c = S.view.ctx();
-  const unsigned gl = grid_for(I.leaf.n, kSwizzle);
-  if (I.ghosts.n) k_propagate_ghosts64<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, S.d_x64);
-  k_apply64<<<I.applyl.n, 64, 0, st>>>(c, I.L(I.applyl), I.applyl.n, S.d_x64, S.d_r64);
-  if (I.ghosts.n) k_accumulate_ghosts64<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, S.d_r64);
+  const unsigned gl = grid_for(G.leaf_full.n, kSwizzle);
+  if (G.ghosts.n) k_propagate_ghosts64<<<G.ghosts.n, 64, 0, st>>>(c, GL(S, G.ghosts), G.ghosts.n, S.d_x64);
+  k_apply64<<<G.applyl.n, 64, 0, st>>>(c, GL(S, G.applyl), G.applyl.n, S.d_x64, S.d_r64);
+  if (G.ghosts.n) k_accumulate_ghosts64<<<G.ghosts.n, 64, 0, st>>>(c, GL(S, G.ghosts), G.ghosts.n, S.d_r64);
   if (S.prm.pure_neumann) {
-    k_ir_residual<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, S.d_r64, 1);
-    k_finalize<<<1, FIN_THREADS, 0, st>>>(one_problem(S, I), FIN_MEAN, 0);
-    k_ir_recenter<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_r64, write_r ? 1 : 0);
+    k_ir_residual<<<gl, 64, 0, st>>>(c, GL(S, G.leaf_full), G.leaf_full.n, S.d_b, S.d_r64, 1);
+    k_finalize<<<unsigned(G.nfull), FIN_THREADS, 0, st>>>(group_ctx(S), FIN_MEAN, 0);
+    k_ir_recenter<<<gl, 64, 0, st>>>(c, GL(S, G.leaf_full), G.leaf_full.n, S.d_r64, write_r ? 1 : 0);
   } else {
-    k_ir_residual<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, S.d_r64, write_r ? 0 : 2);
+    k_ir_residual<<<gl, 64, 0, st>>>(c, GL(S, G.leaf_full), G.leaf_full.n, S.d_b, S.d_r64, write_r ? 0 : 2);
   }
 }
 
 // Start queued requests: ProbState, rhs upload, x0 (zero for a cold start),
-// r0 = b - A x0 on the instance's own lists, |b|^2.
+// r0 = b - A x0, |b|^2 -- one group for all of them.
 void server_start_queued(cg_server& S, int k) {
-  std::vector<int> fresh;
-  for (int q = 0; q < S.max_probs; q++)
-    if (S.inst[q] && S.inst[q]->state == INST_QUEUED) fresh.push_back(q);
+  std::vector<int> fresh, cont;  // refinement: requests that continue one are put last (no residual)
+  for (int q = 0; q < S.nslots; q++)
+    if (S.inst[q] && S.inst[q]->state == INST_QUEUED) (S.refine && S.inst[q]->ir_cont ? cont : fresh).push_back(q);
+  const int nfull = int(fresh.size());
+  fresh.insert(fresh.end(), cont.begin(), cont.end());
   if (fresh.empty()) return;
   const hipStream_t st = S.stream;
-  int32_t* hmap = reinterpret_cast<int32_t*>(S.h_tab[k] + S.tab_cap - 2048);  // tail of this step's table
-  const int nf = std::min<int>(int(fresh.size()), 2048 / 4);
-  Ctx c = S.view.ctx();
-  for (int f = 0; f < nf; f++) {
+  std::vector<ProbState> pss(fresh.size());
+  for (size_t f = 0; f < fresh.size(); f++) {
     Instance& I = *S.inst[fresh[f]];
-    ProbState& ps = S.h_ps_stage[I.slot];
+    ProbState& ps = pss[f];
     std::memset(&ps, 0, sizeof(ps));
     const double tol2 = S.prm.rel_tol * S.prm.rel_tol;
     // refinement: the inner solve stops below half the tolerance of |b|
@@ -444,39 +675,41 @@ void server_start_queued(cg_server& S, int k) {
     ps.leaf_count = I.nleaf;
     ps.pure_neumann = S.prm.pure_neumann;
     ps.check_stride = S.prm.sync_stride;
-    HIPC(hipMemcpyAsync(S.d_ps + I.slot, &ps, sizeof(ps), hipMemcpyHostToDevice, st));
     if (!I.dev_rhs && !I.ir_cont)
       HIPC(hipMemcpyAsync(S.d_b + size_t(I.tile_begin) * 512, I.h_buf, size_t(I.ntiles) * 512 * sizeof(float),
                           hipMemcpyHostToDevice, st));
-    hmap[f] = I.slot;
     I.state = INST_ACTIVE;
     S.active.push_back(I.slot);
     S.dirty = true;
-    if (S.refine) {
-      if (!I.ir_cont) {  // r = b - A x64 in FP64 (x64 = 0 for a cold start); |b|^2 and the threshold floor
-        if (!I.warm) HIPC(hipMemsetAsync(S.d_x64 + size_t(I.tile_begin) * 512, 0, size_t(I.ntiles) * 512 * sizeof(double), st));
-        instance_residual64(S, I, true);
-        k_ir_norm<<<grid_for(I.leaf.n, kSwizzle), 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b);
-        k_finalize<<<1, FIN_THREADS, 0, st>>>(one_problem(S, I), FIN_IRBB, 0);
-      }
-      // the step solves A d = float(r) from d = 0
-      if (S.prm.storage == CG_STORE_BF16) k_cg_init<bf16><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, 0);
-      else k_cg_init<float><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, 0);
-      continue;
+    if (S.refine) {  // r = b - A x64 in FP64 (x64 = 0 for a cold start)
+      if (!I.ir_cont && !I.warm)
+        HIPC(hipMemsetAsync(S.d_x64 + size_t(I.tile_begin) * 512, 0, size_t(I.ntiles) * 512 * sizeof(double), st));
+    } else if (!I.warm) {
+      k_fill_tiles<float><<<I.all.n, 64, 0, st>>>(I.L(I.all), I.all.n, S.d_ch[CG_CH_X], 0.f);
     }
-    float* X = S.d_ch[CG_CH_X];
-    if (!I.warm) k_fill_tiles<float><<<I.all.n, 64, 0, st>>>(I.L(I.all), I.all.n, X, 0.f);
-    if (I.ghosts.n) k_propagate_ghosts<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, X, X);
-    k_apply<<<I.applyl.n, 64, 0, st>>>(c, I.L(I.applyl), I.applyl.n, X, c.ap, -1, 0, 0, 0);
-    if (I.ghosts.n) k_accumulate_ghosts<<<I.ghosts.n, 64, 0, st>>>(c, I.L(I.ghosts), I.ghosts.n, c.ap);
-    if (S.prm.storage == CG_STORE_BF16) k_cg_init_warm<bf16><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, 0);
-    else k_cg_init_warm<float><<<I.leaf.n, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_b, 0);
   }
-  HIPC(hipMemcpyAsync(S.d_newmap, hmap, sizeof(int32_t) * nf, hipMemcpyHostToDevice, st));
-  Ctx ci = c;
-  ci.pmap = S.d_newmap;
+  const Group G = group_lists(S, k, GRP_START, fresh, nfull, pss.data());
+  const Ctx c = S.view.ctx();
+  if (S.refine) {
+    if (G.nfull) {  // |b|^2 and the threshold floor
+      group_residual64(S, G, true);
+      k_ir_norm<<<grid_for(G.leaf_full.n, kSwizzle), 64, 0, st>>>(c, GL(S, G.leaf_full), G.leaf_full.n, S.d_b);
+      k_finalize<<<unsigned(G.nfull), FIN_THREADS, 0, st>>>(group_ctx(S), FIN_IRBB, 0);
+    }
+    // the step solves A d = float(r) from d = 0
+    if (S.prm.storage == CG_STORE_BF16) k_cg_init<bf16><<<G.leaf.n, 64, 0, st>>>(c, GL(S, G.leaf), G.leaf.n, 0);
+    else k_cg_init<float><<<G.leaf.n, 64, 0, st>>>(c, GL(S, G.leaf), G.leaf.n, 0);
+  } else {
+    float* X = S.d_ch[CG_CH_X];
+    if (G.ghosts.n) k_propagate_ghosts<<<G.ghosts.n, 64, 0, st>>>(c, GL(S, G.ghosts), G.ghosts.n, X, X);
+    k_apply<<<G.applyl.n, 64, 0, st>>>(c, GL(S, G.applyl), G.applyl.n, X, c.ap, -1, 0, 0, 0);
+    if (G.ghosts.n) k_accumulate_ghosts<<<G.ghosts.n, 64, 0, st>>>(c, GL(S, G.ghosts), G.ghosts.n, c.ap);
+    if (S.prm.storage == CG_STORE_BF16) k_cg_init_warm<bf16><<<G.leaf.n, 64, 0, st>>>(c, GL(S, G.leaf), G.leaf.n, S.d_b, 0);
+    else k_cg_init_warm<float><<<G.leaf.n, 64, 0, st>>>(c, GL(S, G.leaf), G.leaf.n, S.d_b, 0);
+  }
+  Ctx ci = group_ctx(S);
   ci.dflag = S.d_dflag_init;
-  k_finalize<<<unsigned(nf), FIN_THREADS, 0, st>>>(ci, FIN_BB, 0);
+  k_finalize<<<unsigned(G.n), FIN_THREADS, 0, st>>>(ci, FIN_BB, 0);
   HIPC(hipGetLastError());
 }
 
@@ -557,29 +790,37 @@ void finish_request(cg_server& S, Instance& I) {
   S.done_q.push_back(I.slot);
 }
 
-// FP64 check of a finished request: x64 = x (verify_f64) or x64 += d (a
-// refinement step, whose r = float(b - A x64) is left ready for another step),
-// then |b - A x64|^2 and |b|^2 to the host.
-void enqueue_verify(cg_server& S, Instance& I) {
+// FP64 check of the finished requests of a group: x64 = x (verify_f64) or
+// x64 += d (a refinement step, whose r = float(b - A x64) is left ready for
+// another step), then |b - A x64|^2 and |b|^2 to the host.
+void enqueue_verify(cg_server& S, const Group& G, const std::vector<int>& slots) {
   const hipStream_t st = S.stream;
   const Ctx c = S.view.ctx();
-  const unsigned gl = grid_for(I.leaf.n, kSwizzle);
-  if (S.refine) k_ir_accumulate<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_x64, 0);
-  else k_ir_load_x<<<gl, 64, 0, st>>>(c, I.L(I.leaf), I.leaf.n, S.d_ch[CG_CH_X], S.d_x64);
-  instance_residual64(S, I, S.refine != 0);
-  k_ir_verdict<<<1, FIN_THREADS, 0, st>>>(one_problem(S, I), S.d_ver + 2 * I.slot);
-  HIPC(hipMemcpyAsync(S.h_ver + 2 * I.slot, S.d_ver + 2 * I.slot, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPC(hipEventRecord(I.ev_ver, st));
-  I.state = INST_VERIFY;
+  const unsigned gl = grid_for(G.leaf.n, kSwizzle);
+  if (S.refine) k_ir_accumulate<<<gl, 64, 0, st>>>(c, GL(S, G.leaf), G.leaf.n, S.d_x64, 0);
+  else k_ir_load_x<<<gl, 64, 0, st>>>(c, GL(S, G.leaf), G.leaf.n, S.d_ch[CG_CH_X], S.d_x64);
+  group_residual64(S, G, S.refine != 0);
+  k_ir_verdict<<<unsigned(G.n), FIN_THREADS, 0, st>>>(group_ctx(S), S.d_ver);
+  // one copy over the group's slot range (the other slots' entries in it are
+  // rewritten with the values their own checks left)
+  const int lo = *std::min_element(slots.begin(), slots.end()), hi = *std::max_element(slots.begin(), slots.end());
+  HIPC(hipMemcpyAsync(S.h_ver + 2 * lo, S.d_ver + 2 * lo, 2 * sizeof(double) * (hi - lo + 1), hipMemcpyDeviceToHost, st));
+  for (int q : slots) {
+    Instance& I = *S.inst[q];
+    HIPC(hipEventRecord(I.ev_ver, st));
+    I.state = INST_VERIFY;
+  }
 }
 
-// Finished requests of the iteration snapshot k: results, pressure download.
+// Finished requests of the iteration snapshot k: results, the last x update,
+// the FP64 checks (one group), pressure download.
 void server_collect(cg_server& S, int k) {
   if (!S.snap_pending[k]) return;
   HIPC(hipEventSynchronize(S.ev_snap[k]));
   S.snap_pending[k] = false;
   const ProbState* st = S.h_snap[k];
   const auto now = std::chrono::steady_clock::now();
+  std::vector<int> check, rest;  // to the FP64 check; reported now (b = 0, or no check)
   for (int q : S.snap_active[k]) {
     if (!S.inst[q]) continue;
     Instance& I = *S.inst[q];
@@ -591,8 +832,6 @@ void server_collect(cg_server& S, int k) {
     I.res.seconds = std::chrono::duration<double>(now - I.t_submit).count();
     S.active.erase(std::find(S.active.begin(), S.active.end(), q));
     S.dirty = true;
-    // the last iteration's x update (k_cg_update_p applies each earlier one)
-    k_cg_finish_x<<<grid_for(I.leaf.n, kSwizzle), 64, 0, S.stream>>>(S.view.ctx(), I.L(I.leaf), I.leaf.n, kSwizzle);
     if (S.refine) {
       // the step's recursive residual, relative to the request's b
       I.bb_b = s.bb_b;
@@ -600,35 +839,49 @@ void server_collect(cg_server& S, int k) {
       I.res.bb = s.bb_b;
       I.res.rr = s.rr;
       I.res.rel_res = s.bb_b > 0 ? (S.prm.sync_stride > 0 ? std::sqrt(s.rr / s.bb_b) : -1.0) : 0.0;
-      if (s.bb_b > 0) {
-        enqueue_verify(S, I);
-      } else {  // b = 0: x = 0
-        HIPC(hipMemsetAsync(S.d_x64 + size_t(I.tile_begin) * 512, 0, size_t(I.ntiles) * 512 * sizeof(double), S.stream));
-        HIPC(hipMemsetAsync(S.d_ch[CG_CH_X] + size_t(I.tile_begin) * 512, 0, size_t(I.ntiles) * 512 * sizeof(float), S.stream));
-        I.res.converged = 1;
-        I.res.rr = 0.0;
-        I.res.rel_res_f64 = 0.0;
-        finish_request(S, I);
-      }
-      continue;
-    }
-    I.res.bb = s.bb;
-    I.res.rr = s.rr;
-    I.res.rel_res = s.bb > 0 ? (S.prm.sync_stride > 0 ? std::sqrt(s.rr / s.bb) : -1.0) : 0.0;
-    if (S.prm.verify_f64 && s.bb > 0) {
-      enqueue_verify(S, I);
+      (s.bb_b > 0 ? check : rest).push_back(q);
     } else {
-      if (S.prm.verify_f64) I.res.rel_res_f64 = 0.0;  // b = 0: x = 0 is exact
-      finish_request(S, I);
+      I.res.bb = s.bb;
+      I.res.rr = s.rr;
+      I.res.rel_res = s.bb > 0 ? (S.prm.sync_stride > 0 ? std::sqrt(s.rr / s.bb) : -1.0) : 0.0;
+      (S.prm.verify_f64 && s.bb > 0 ? check : rest).push_back(q);
     }
   }
+  const hipStream_t stm = S.stream;
+  const Ctx c = S.view.ctx();
+  // the last iteration's x update (k_cg_update_p applies each earlier one)
+  for (int q : rest) {
+    const Instance& I = *S.inst[q];
+    k_cg_finish_x<<<grid_for(I.leaf.n, kSwizzle), 64, 0, stm>>>(c, I.L(I.leaf), I.leaf.n, kSwizzle);
+  }
+  if (!check.empty()) {
+    const Group G = group_lists(S, k, GRP_FINISH, check, int(check.size()), nullptr);
+    k_cg_finish_x<<<grid_for(G.leaf.n, kSwizzle), 64, 0, stm>>>(c, GL(S, G.leaf), G.leaf.n, kSwizzle);
+    enqueue_verify(S, G, check);
+  }
+  for (int q : rest) {
+    Instance& I = *S.inst[q];
+    if (S.refine) {  // b = 0: x = 0
+      HIPC(hipMemsetAsync(S.d_x64 + size_t(I.tile_begin) * 512, 0, size_t(I.ntiles) * 512 * sizeof(double), stm));
+      HIPC(hipMemsetAsync(S.d_ch[CG_CH_X] + size_t(I.tile_begin) * 512, 0, size_t(I.ntiles) * 512 * sizeof(float), stm));
+      I.res.converged = 1;
+      I.res.rr = 0.0;
+      I.res.rel_res_f64 = 0.0;
+    } else if (S.prm.verify_f64) {
+      I.res.rel_res_f64 = 0.0;  // b = 0: x = 0 is exact
+    }
+    finish_request(S, I);
+  }
+  HIPC(hipGetLastError());
 }
 
 // Requests whose FP64 check has arrived.  verify_f64: reported with the
 // checked residual.  refine: reported once |b - A x|^2 < rel_tol^2 |b|^2 (x =
-// float(x64) then), or after kMaxRefineSteps steps; otherwise another step.
-void server_verified(cg_server& S) {
-  for (int q = 0; q < S.max_probs; q++) {
+// float(x64) then, one group), or after kMaxRefineSteps steps; otherwise
+// another step.
+void server_verified(cg_server& S, int k) {
+  std::vector<int> passed;
+  for (int q = 0; q < S.nslots; q++) {
     if (!S.inst[q] || S.inst[q]->state != INST_VERIFY) continue;
     Instance& I = *S.inst[q];
     const hipError_t e = hipEventQuery(I.ev_ver);
@@ -646,14 +899,18 @@ void server_verified(cg_server& S) {
     if (pass || I.steps >= kMaxRefineSteps || !I.res.converged) {
       I.res.converged = pass ? 1 : 0;
       I.res.rr = rr;
-      k_ir_accumulate<<<grid_for(I.leaf.n, kSwizzle), 64, 0, S.stream>>>(S.view.ctx(), I.L(I.leaf), I.leaf.n, S.d_x64, 1);
-      finish_request(S, I);
+      passed.push_back(q);
     } else {
       I.ir_cont = 1;
       I.iters_acc = I.res.iters;
       I.state = INST_QUEUED;
     }
   }
+  if (passed.empty()) return;
+  const Group G = group_lists(S, k, GRP_PASSED, passed, 0, nullptr);
+  k_ir_accumulate<<<grid_for(G.leaf.n, kSwizzle), 64, 0, S.stream>>>(S.view.ctx(), GL(S, G.leaf), G.leaf.n, S.d_x64, 1);
+  HIPC(hipGetLastError());
+  for (int q : passed) finish_request(S, *S.inst[q]);
 }
 
 // One step: an iteration over the current active set, then (one iteration
@@ -662,6 +919,7 @@ void server_verified(cg_server& S) {
 void server_step(cg_server& S, int32_t* n_active) {
   const int k = int(S.step_no & 1);
   cg_problem& P = S.view;
+  if (!S.pending.empty()) reap_releases(S, false);
   if (!S.active.empty() && !S.dirty) {
     enqueue_cg_iteration(P, S.v, S.prm.pure_neumann, S.prm.storage);
     HIPC(hipGetLastError());
@@ -673,7 +931,7 @@ void server_step(cg_server& S, int32_t* n_active) {
     S.snap_pending[k] = true;
   }
   server_collect(S, 1 - k);
-  if (S.prm.verify_f64 || S.refine) server_verified(S);
+  if (S.prm.verify_f64 || S.refine) server_verified(S, k);
   server_start_queued(S, k);
   if (S.dirty) server_assemble(S, k);
   S.step_no++;
