@@ -200,6 +200,42 @@ def run_served(ca, solver_kw, scenes, duration=60.0, lam=2000.0, max_resident=25
     }
 
 
+def served_light_sweep(ca, solver_kw, scenes, args, budget, capacity):
+    """The served_mixed form below capacity: one request window per offered
+    rate of --serve-rate-light (fractions of `capacity`, served_mixed's
+    completion rate, when below 1.5).  Returns the record of the highest rate
+    whose p99 after the first quarter of its window stays within
+    --light-p99-ms (else the lowest rate), with every point under "sweep"."""
+    rates = []
+    for tok in str(args.serve_rate_light).split(","):
+        v = float(tok)
+        if v <= 0:
+            continue
+        if v < 1.5:
+            if not capacity:
+                continue
+            v *= capacity
+        rates.append(v)
+    if not rates or args.serve_seconds_light <= 0:
+        return None
+    points = []
+    for lam in sorted(rates):
+        f = run_served(ca, solver_kw, scenes, duration=args.serve_seconds_light, lam=lam,
+                       max_resident=args.serve_instances, budget=budget, storage=1, refine=True)
+        points.append(f)
+        note(f"served_mixed_light: {lam:.0f}/s offered, p50 {f['p50_ms_after_first_quarter']:.1f} ms, "
+             f"p99 {f['p99_ms_after_first_quarter']:.1f} ms after the first quarter")
+    ok = [f for f in points if f["p99_ms_after_first_quarter"] is not None
+          and f["p99_ms_after_first_quarter"] <= args.light_p99_ms]
+    best = dict(ok[-1] if ok else points[0])
+    best["p99_bound_ms"] = args.light_p99_ms
+    best["p99_bound_met"] = bool(ok)
+    best["sweep"] = [{k: f[k] for k in ("offered_per_s", "solves_per_s", "completed", "unfinished", "p50_ms",
+                                        "p99_ms", "p50_ms_after_first_quarter", "p99_ms_after_first_quarter",
+                                        "fp64_over_tol")} for f in points]
+    return best
+
+
 def run_batched(ca, scenes, B, reps, solver_kw, use_graph=True, storage=0, refine=False, tune=False, t_build=0.0,
                 oracle_sample=False):
     """Solve nprob independent problems together; returns the form's record.
@@ -754,11 +790,15 @@ def main():
                     help="length of the Poisson request window of the served_mixed form (0: skip the served forms)")
     ap.add_argument("--serve-seconds-fp32", type=float, default=10.0,
                     help="length of the request window of the FP32 served form")
-    ap.add_argument("--serve-seconds-light", type=float, default=20.0,
-                    help="length of the request window of the served_mixed_light form")
+    ap.add_argument("--serve-seconds-light", type=float, default=15.0,
+                    help="length of each request window of the served_mixed_light sweep")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
-    ap.add_argument("--serve-rate-light", type=float, default=0.0,
-                    help="offered rate of the served_mixed_light form (0: 0.7 of what served_mixed completed per s)")
+    ap.add_argument("--serve-rate-light", default="0.45,0.55,0.65",
+                    help="offered rates of the served_mixed_light sweep: comma list, as fractions of what served_mixed "
+                         "completed per s when below 1.5, else requests/s")
+    ap.add_argument("--light-p99-ms", type=float, default=50.0,
+                    help="served_mixed_light reports the highest swept rate whose p99 (after the first quarter) "
+                         "stays within this bound")
     ap.add_argument("--serve-instances", type=int, default=256, help="most instances resident at once")
     ap.add_argument("--serve-scenes", type=int, default=256,
                     help="lognormal scenes the served instances draw their octrees from")
@@ -984,10 +1024,16 @@ def main():
                 storage, lam, dur = 0, args.serve_rate, args.serve_seconds_fp32
             elif name == "served_mixed":  # the serving configuration as stated
                 storage, lam, dur = 1, args.serve_rate, args.serve_seconds
-            else:  # below capacity: 0.7 of what served_mixed completed per second
-                storage, dur = 1, args.serve_seconds_light
-                lam = args.serve_rate_light if args.serve_rate_light > 0 else \
-                    (0.7 * capacity if capacity else 0.0)
+            else:  # below capacity: a sweep of offered rates
+                f = served_light_sweep(ca, solver_kw, sscenes, args, budget, capacity)
+                if f is None:
+                    continue
+                if base:
+                    f["vs_plain_solves_per_s"] = f["solves_per_s"] / base["solves_per_s"]
+                forms[name] = f
+                note(f"{name}: p99 within {args.light_p99_ms:g} ms up to {f['offered_per_s']:.0f}/s offered "
+                     f"(p50 {f['p50_ms_after_first_quarter']:.1f} ms, p99 {f['p99_ms_after_first_quarter']:.1f} ms)")
+                continue
             if lam <= 0 or dur <= 0:
                 continue
             try:

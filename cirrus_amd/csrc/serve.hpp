@@ -108,6 +108,12 @@ struct cg_server {
   std::vector<int> active;                      // slots whose request runs, assembly order
   std::vector<int> snap_active[2];              // active set of the iteration each snapshot follows
   bool dirty = false;
+  // the iteration over an assembled active set is captured in a graph on its
+  // second step and replayed while the set stays; replaced graphs are destroyed
+  // once the stream has passed their last launch
+  int64_t asm_ver = 0, graph_ver = -1, last_iter_ver = -1;
+  hipGraphExec_t graph = nullptr;
+  std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired;
   std::deque<int> done_q;                       // slots with a finished request, in completion order
   int64_t step_no = 0;
   bool snap_pending[2] = {false, false};
@@ -164,6 +170,11 @@ struct cg_server {
 
 cg_server::~cg_server() {
   if (stream) (void)hipStreamSynchronize(stream);
+  if (graph) (void)hipGraphExecDestroy(graph);
+  for (auto& r : retired) {
+    (void)hipGraphExecDestroy(r.first);
+    (void)hipEventDestroy(r.second);
+  }
   for (auto& I : inst)
     if (I && I->h_buf) (void)hipHostFree(I->h_buf);
   for (auto& R : pending) {
@@ -777,6 +788,7 @@ void server_assemble(cg_server& S, int k) {
   if (nr) k_gather_recs<<<dim3(64, unsigned(std::min(nr, 65535))), 256, 0, st>>>(S.d_srcs, S.d_offs, S.d_counts, nr, S.d_asm);
   HIPC(hipGetLastError());
   S.dirty = false;
+  S.asm_ver++;
 }
 
 // A request is reported: its pressure goes to the pinned buffer (unless it stays
@@ -916,12 +928,53 @@ void server_verified(cg_server& S, int k) {
 // One step: an iteration over the current active set, then (one iteration
 // behind, so the device keeps working) finished requests, new requests and a
 // reassembly of the worklists for the next step.
+// One CG iteration over the assembled active set: launched kernel by kernel on
+// a set's first step, captured on its second, replayed after that.
+void server_iteration(cg_server& S) {
+  cg_problem& P = S.view;
+  for (size_t i = 0; i < S.retired.size();) {
+    const hipError_t e = hipEventQuery(S.retired[i].second);
+    if (e == hipErrorNotReady) { i++; continue; }
+    HIPC(e);
+    HIPC(hipGraphExecDestroy(S.retired[i].first));
+    S.ev_free.push_back(S.retired[i].second);
+    S.retired.erase(S.retired.begin() + i);
+  }
+  if (S.graph && S.graph_ver == S.asm_ver) {
+    HIPC(hipGraphLaunch(S.graph, S.stream));
+  } else if (S.last_iter_ver == S.asm_ver) {
+    if (S.graph) {  // every launch of the old graph is ahead of this event
+      hipEvent_t ev;
+      if (S.ev_free.empty()) {
+        HIPC(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      } else {
+        ev = S.ev_free.back();
+        S.ev_free.pop_back();
+      }
+      S.retired.emplace_back(S.graph, ev);
+      S.graph = nullptr;
+      HIPC(hipEventRecord(ev, S.stream));
+    }
+    hipGraph_t g;
+    HIPC(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
+    enqueue_cg_iteration(P, S.v, S.prm.pure_neumann, S.prm.storage);
+    HIPC(hipStreamEndCapture(S.stream, &g));
+    const hipError_t e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPC(e);
+    S.graph_ver = S.asm_ver;
+    HIPC(hipGraphLaunch(S.graph, S.stream));
+  } else {
+    enqueue_cg_iteration(P, S.v, S.prm.pure_neumann, S.prm.storage);
+  }
+  S.last_iter_ver = S.asm_ver;
+}
+
 void server_step(cg_server& S, int32_t* n_active) {
   const int k = int(S.step_no & 1);
-  cg_problem& P = S.view;
   if (!S.pending.empty()) reap_releases(S, false);
   if (!S.active.empty() && !S.dirty) {
-    enqueue_cg_iteration(P, S.v, S.prm.pure_neumann, S.prm.storage);
+    server_iteration(S);
     HIPC(hipGetLastError());
     int maxslot = 0;
     for (int q : S.active) maxslot = std::max(maxslot, q + 1);
