@@ -359,8 +359,11 @@ cg_status cg_calc_cell_types_from_leafs(const cg_tree_desc* desc, uint8_t* cell_
  * running (continuous batching).  A finished request is read with
  * cg_server_next_result; each result equals the same request solved alone.
  * Warm start: x0 = the instance's last pressure, r0 = b - A x0, tolerance
- * relative to |b| (a cold start uses x0 = 0, as GMGSolver::solve).  One handle
- * must not be used from two threads at once. */
+ * relative to |b| (a cold start uses x0 = 0, as GMGSolver::solve).
+ * cg_server_release returns at once: the instance's memory and slot are
+ * reused once the device has passed the point of release (an admission that
+ * needs them waits for that point).  One handle must not be used from two
+ * threads at once. */
 typedef struct cg_server cg_server;
 typedef struct {
   int64_t budget_bytes;      /* device memory for channels, cell types and worklists */
