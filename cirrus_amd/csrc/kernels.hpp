@@ -42,7 +42,7 @@ struct alignas(16) TileRec {
   int32_t gslot;      // GHOST: slot in the 64-value ghost buffer; -1 otherwise
   int32_t self;       // the tile's index
   float h;            // cell size of the tile's level
-  uint32_t meta;      // level | type << 8 | octant << 16 | has_gchild << 24
+  uint32_t meta;      // level | type << 8 | octant << 16 | has_gchild << 24 | all_interior << 25
   uint32_t ntypes;    // type of face neighbour f in bits 4f..4f+3 (0: none)
   float bc[7];        // face coefficients of the smoothing view (subtree at the
                       // tile's own level): faces x-,y-,z-,x+,y+,z+, then inside
@@ -56,7 +56,9 @@ static_assert(sizeof(TileRec) == 176, "TileRec is 176 bytes");
 __host__ __device__ __forceinline__ int tr_level(const TileRec& T) { return int(T.meta & 0xff); }
 __host__ __device__ __forceinline__ uint8_t tr_type(const TileRec& T) { return uint8_t((T.meta >> 8) & 0xff); }
 __host__ __device__ __forceinline__ int tr_octant(const TileRec& T) { return int((T.meta >> 16) & 0xff); }
-__host__ __device__ __forceinline__ bool tr_has_gchild(const TileRec& T) { return (T.meta >> 24) != 0; }
+__host__ __device__ __forceinline__ bool tr_has_gchild(const TileRec& T) { return ((T.meta >> 24) & 1) != 0; }
+// every one of the tile's 512 cells is INTERIOR (its cell types need not be read)
+__host__ __device__ __forceinline__ bool tr_all_int(const TileRec& T) { return ((T.meta >> 25) & 1) != 0; }
 __host__ __device__ __forceinline__ uint8_t tr_ntype(const TileRec& T, int f) { return uint8_t((T.ntypes >> (4 * f)) & 0xf); }
 
 struct alignas(16) ProbState {
@@ -247,6 +249,11 @@ __device__ __forceinline__ void store2(f16* __restrict__ p, float2 a) { *reinter
 
 __device__ __forceinline__ uint64_t load_ct8(const uint8_t* __restrict__ p) {
   return *reinterpret_cast<const uint64_t*>(p);
+}
+// The lane's 8 cell types, or eight INTERIOR bytes without a load when the
+// record says every cell of the tile is INTERIOR.
+__device__ __forceinline__ uint64_t tile_ct8(const TileRec& T, const uint8_t* __restrict__ p) {
+  return tr_all_int(T) ? 0x0101010101010101ull * C_INT : load_ct8(p);
 }
 
 // A wavefront-uniform value (keeps record fields in scalar registers).
@@ -2075,7 +2082,7 @@ __global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const TileRec* __rest
   load8(c.p + base, pv);
   load8_split(vc_z<V>(c) + size_t(T.self) * 512, lane, zv);
   load8(c.x + base, xv);
-  const uint64_t ct = load_ct8(c.ctype + base);
+  const uint64_t ct = tile_ct8(T, c.ctype + base);
   if (prob_done(c, T, flags)) return;
   const ProbState& ps = c.ps[T.prob];
   const float beta = float(ps.beta);
@@ -2181,7 +2188,7 @@ __global__ __launch_bounds__(64) void k_cg_update_xr(Ctx c, const TileRec* __res
   float rv[8], av[8];
   load8(c.r + base, rv);
   load8(c.ap + base, av);
-  const uint64_t ct = load_ct8(c.ctype + base);
+  const uint64_t ct = tile_ct8(T, c.ctype + base);
   if (prob_done(c, T, flags)) return;
   const float alpha = float(c.ps[T.prob].alpha);
   for (int k = 0; k < 8; k++)
@@ -2204,7 +2211,7 @@ __global__ __launch_bounds__(64) void k_cg_finish_x(Ctx c, const TileRec* __rest
   float pv[8], xv[8];
   load8(c.p + base, pv);
   load8(c.x + base, xv);
-  const uint64_t ct = load_ct8(c.ctype + base);
+  const uint64_t ct = tile_ct8(T, c.ctype + base);
   const ProbState& ps = c.ps[T.prob];
   if (ps.iter <= 0) return;
   const float alpha = float(ps.alpha);

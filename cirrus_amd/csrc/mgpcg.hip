@@ -319,6 +319,10 @@ void build_tree(const cg_tree_desc* d, int prob, int base, int leaf_base, int gh
     }
     // smoothing-view coefficients (subtree at the tile's own level)
     view_coeffs(T, l, T.bc);
+    const uint8_t* ct = d->cell_type + size_t(B.ext[i]) * 512;
+    bool all_int = true;
+    for (int k = 0; k < 512 && all_int; k++) all_int = ct[k] == C_INT;
+    if (all_int) T.meta |= 1u << 25;
   }
   for (int i = 0; i < n; i++) {
     TileRec& T = B.tiles[i];
