@@ -42,7 +42,8 @@ struct alignas(16) TileRec {
   int32_t gslot;      // GHOST: slot in the 64-value ghost buffer; -1 otherwise
   int32_t self;       // the tile's index
   float h;            // cell size of the tile's level
-  uint32_t meta;      // level | type << 8 | octant << 16 | has_gchild << 24 | all_interior << 25
+  uint32_t meta;      // level | type << 8 | octant << 16 | has_gchild << 24 | all_interior << 25 |
+                      // inert_candidate << 26
   uint32_t ntypes;    // type of face neighbour f in bits 4f..4f+3 (0: none)
   float bc[7];        // face coefficients of the smoothing view (subtree at the
                       // tile's own level): faces x-,y-,z-,x+,y+,z+, then inside
@@ -59,6 +60,10 @@ __host__ __device__ __forceinline__ int tr_octant(const TileRec& T) { return int
 __host__ __device__ __forceinline__ bool tr_has_gchild(const TileRec& T) { return ((T.meta >> 24) & 1) != 0; }
 // every one of the tile's 512 cells is INTERIOR (its cell types need not be read)
 __host__ __device__ __forceinline__ bool tr_all_int(const TileRec& T) { return ((T.meta >> 25) & 1) != 0; }
+// a LEAF tile without ghost children and without an INTERIOR cell: when its
+// right-hand side is zero as well (decided per solve), every CG iteration
+// leaves each of its values as it found them (see k_cg_init)
+__host__ __device__ __forceinline__ bool tr_inert_cand(const TileRec& T) { return ((T.meta >> 26) & 1) != 0; }
 __host__ __device__ __forceinline__ uint8_t tr_ntype(const TileRec& T, int f) { return uint8_t((T.ntypes >> (4 * f)) & 0xf); }
 
 struct alignas(16) ProbState {
@@ -103,6 +108,7 @@ struct Ctx {
   double* part2; // second partial stream (mean counts)
   const float* diag;  // per cell, colour-split order: the V-cycle colour sweep's diagonal D (k_diag)
   const TileRec* tiles;  // every tile's record by tile index (null where the handle has none)
+  uint8_t* inert = nullptr;  // per tile: 1 while the current solve leaves the tile untouched (k_cg_init)
 };
 
 // The V-cycle vectors of the storage type V.
@@ -114,6 +120,10 @@ template <> __device__ __forceinline__ float* vc_rhs<float>(const Ctx& c) { retu
 template <> __device__ __forceinline__ bf16* vc_rhs<bf16>(const Ctx& c) { return c.rhsvh; }
 template <> __device__ __forceinline__ f16* vc_z<f16>(const Ctx& c) { return c.zf; }
 template <> __device__ __forceinline__ f16* vc_rhs<f16>(const Ctx& c) { return c.rhsvf; }
+template <class V> __device__ __forceinline__ V* vc_zb(const Ctx& c);
+template <> __device__ __forceinline__ float* vc_zb<float>(const Ctx& c) { return c.zb; }
+template <> __device__ __forceinline__ bf16* vc_zb<bf16>(const Ctx& c) { return c.zbh; }
+template <> __device__ __forceinline__ f16* vc_zb<f16>(const Ctx& c) { return c.zbf; }
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -302,6 +312,7 @@ enum : int {
   F_SWIZZLE = 16,    // XCD-aware tile order
   F_CHECK_DONE = 32, // skip tiles of converged problems
   F_DOT_RZ = 64,     // last colour pass of a level's up-sweep: also the r.z partial of each LEAF tile
+  F_INERT = 128,     // a CG iteration's launch: tiles the solve's init found inert are skipped
 };
 
 __device__ __forceinline__ bool prob_done(const Ctx& c, const TileRec& T, int flags) {
@@ -1210,11 +1221,16 @@ __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, 
   int w = work_index(flags & F_SWIZZLE);         \
   if (w >= n) return;                            \
   const TileRec T = list[w];
+// In a CG iteration's launch, a tile that k_cg_init found inert exits: the
+// launch would write the values its arrays already hold.
+#define SKIP_INERT \
+  if ((flags & F_INERT) && tr_inert_cand(T) && c.inert[T.self]) return;
 
 template <class V, bool DOT = false>
 __global__ __launch_bounds__(64) void k_gs_color(Ctx c, const TileRec* __restrict__ list, int n, int level, V* X,
                                                  const V* F, int color, int flags) {
   TILE_KERNEL_PROLOGUE
+  SKIP_INERT
   __shared__ GsLDS L;
   tile_gs<V, DOT>(c, T, level, X, F, color, flags, L);
 }
@@ -1369,6 +1385,7 @@ __device__ __forceinline__ float first_red(float b, float d) {
 template <class V>
 __global__ __launch_bounds__(64) void k_gs_first(Ctx c, const TileRec* __restrict__ list, int n, V* X, int flags) {
   TILE_KERNEL_PROLOGUE
+  SKIP_INERT
   __shared__ GsLDS L;
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const bool ghost_zero = flags & F_GHOST_ZERO;
@@ -1596,6 +1613,7 @@ template <class V>
 __global__ __launch_bounds__(64) void k_gs_prolong_eager(Ctx c, const TileRec* __restrict__ list, int n, const V* XD,
                                                          V* XO, float pcoef, int color, int flags) {
   TILE_KERNEL_PROLOGUE
+  SKIP_INERT
   __shared__ TileLDS L;
   tile_gs_prolong_eager(c, T, XD, XO, pcoef, color, flags, L);
 }
@@ -1603,6 +1621,7 @@ template <class V>
 __global__ __launch_bounds__(64) void k_gs_prolong(Ctx c, const TileRec* __restrict__ list, int n, const V* XD,
                                                    V* XO, float pcoef, int color, int flags) {
   TILE_KERNEL_PROLOGUE
+  SKIP_INERT
   __shared__ TileLDS L;
   tile_gs_prolong(c, T, XD, XO, pcoef, color, flags, L);
 }
@@ -1610,6 +1629,7 @@ template <class V>
 __global__ __launch_bounds__(64) void k_jacobi(Ctx c, const TileRec* __restrict__ list, int n, int level, const V* X,
                                                const V* XG, V* Y, const V* F, float omega, int flags) {
   TILE_KERNEL_PROLOGUE
+  SKIP_INERT
   __shared__ TileLDS L;
   tile_jacobi(c, T, level, X, XG, Y, F, omega, flags, L);
 }
@@ -1617,6 +1637,7 @@ template <class V, bool EAGER = false>
 __global__ __launch_bounds__(64) void k_restrict(Ctx c, const TileRec* __restrict__ list, int n, int level, const V* X,
                                                  const V* F, V* CO, float one_over_alpha, int flags) {
   TILE_KERNEL_PROLOGUE
+  SKIP_INERT
   __shared__ TileLDS L;
   __shared__ float res[512];
   tile_restrict<V, EAGER>(c, T, level, X, F, CO, one_over_alpha, flags, L, res);
@@ -1625,6 +1646,7 @@ template <class V>
 __global__ __launch_bounds__(64) void k_prolong(Ctx c, const TileRec* __restrict__ list, int n, int level,
                                                 const V* XC, const V* XF, V* XO, V* XG, float pcoef, int flags) {
   TILE_KERNEL_PROLOGUE
+  SKIP_INERT
   tile_prolong(c, T, level, XC, XF, XO, XG, pcoef, flags);
 }
 
@@ -1874,6 +1896,30 @@ __global__ void k_empty(int* p) {
 // CG kernels (rotated loop: V-cycle, gamma/beta, p, A p + alpha, x/r + check).
 // Kernels write per-tile FP64 partials; k_finalize reduces them per problem.
 // ---------------------------------------------------------------------------
+// Per solve, for a candidate tile (LEAF, no ghost child, no INTERIOR cell,
+// tr_inert_cand): it is inert when its initial residual is +0 in every cell.
+// The residual then stays +0 (A p is 0 off INTERIOR cells), and so does every
+// value the iterations would write there: the smoothers and the prolongation
+// leave 0 on cells that are not INTERIOR, the residual and the restriction of
+// zeros are 0, the operator writes 0, every dot partial is 0 (the slot holds
+// the +0 written below) and x keeps its value (the updates are masked).  So
+// its V-cycle vectors, A p and its octant block of the parent's V-cycle rhs
+// (which only its restriction writes) are set to those values here, and the
+// iteration kernels skip it (SKIP_INERT).
+template <class V>
+__device__ __forceinline__ void mark_inert(const Ctx& c, const TileRec& T, int lane, const float rv[8]) {
+  bool zero = true;
+  for (int k = 0; k < 8; k++) zero = zero && __float_as_uint(rv[k]) == 0u;
+  const bool inert = __all(zero);
+  if (lane == 0) c.inert[T.self] = inert ? 1 : 0;
+  if (!inert) return;
+  const float z8[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  store8(c.ap + size_t(T.self) * 512 + lane * 8, z8);
+  store8_split(vc_z<V>(c) + size_t(T.self) * 512, lane, z8);
+  store8_split(vc_zb<V>(c) + size_t(T.self) * 512, lane, z8);
+  if (T.parent >= 0) vc_rhs<V>(c)[parent_cell_split(T, lane)] = from_f<V>(0.f);
+}
+
 // Init: x = 0, p = 0 on LEAF tiles, V-cycle rhs = b; partials of |b|^2
 // (GMGSolver.cu:537-555).
 template <class V>
@@ -1891,6 +1937,7 @@ __global__ __launch_bounds__(64) void k_cg_init(Ctx c, const TileRec* __restrict
   for (int k = 0; k < 8; k++) a += double(rv[k]) * double(rv[k]);
   a = wave_sum(a);
   if (lane == 0) c.part[T.leaf_slot] = a;
+  if (c.inert && tr_inert_cand(T)) mark_inert<V>(c, T, lane, rv);
 }
 
 // Warm-started init: r = b - A x0 on LEAF tiles (A x0 in AP, formed by the
@@ -1914,6 +1961,7 @@ __global__ __launch_bounds__(64) void k_cg_init_warm(Ctx c, const TileRec* __res
   for (int k = 0; k < 8; k++) a += double(bv[k]) * double(bv[k]);
   a = wave_sum(a);
   if (lane == 0) c.part[T.leaf_slot] = a;
+  if (c.inert && tr_inert_cand(T)) mark_inert<V>(c, T, lane, rv);
 }
 
 // Pool build on the device: out[k] = tiles[ids[k]] (the worklists, records
@@ -2055,6 +2103,7 @@ __global__ __launch_bounds__(256) void k_gather_recs(const TileRec* const* __res
 template <class V>
 __global__ __launch_bounds__(64) void k_cg_dot_rz(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
   TILE_KERNEL_PROLOGUE
+  SKIP_INERT
   const int lane = lane_id();
   const size_t base = size_t(T.self) * 512 + lane * 8;
   float rv[8], zv[8];
@@ -2076,6 +2125,7 @@ __global__ __launch_bounds__(64) void k_cg_dot_rz(Ctx c, const TileRec* __restri
 template <class V>
 __global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
   TILE_KERNEL_PROLOGUE
+  SKIP_INERT
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
   const size_t base = size_t(T.self) * 512 + lane * 8;
   float pv[8], zv[8], xv[8];
@@ -2119,6 +2169,7 @@ __global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const TileRec* __rest
 __global__ __launch_bounds__(64) void k_apply(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ X,
                                               float* __restrict__ AX, int sl, int diag, int dot, int flags) {
   TILE_KERNEL_PROLOGUE
+  SKIP_INERT
   __shared__ TileLDS L;
   __shared__ float gl[512];
   Stencil S;
@@ -2183,6 +2234,7 @@ __global__ __launch_bounds__(64) void k_ghost_gather(Ctx c, const TileRec* __res
 template <class V>
 __global__ __launch_bounds__(64) void k_cg_update_xr(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
   TILE_KERNEL_PROLOGUE
+  SKIP_INERT
   const int lane = lane_id();
   const size_t base = size_t(T.self) * 512 + lane * 8;
   float rv[8], av[8];
@@ -2206,6 +2258,7 @@ __global__ __launch_bounds__(64) void k_cg_update_xr(Ctx c, const TileRec* __res
 // not; none when no iteration ran).
 __global__ __launch_bounds__(64) void k_cg_finish_x(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
   TILE_KERNEL_PROLOGUE
+  SKIP_INERT
   const int lane = lane_id();
   const size_t base = size_t(T.self) * 512 + lane * 8;
   float pv[8], xv[8];

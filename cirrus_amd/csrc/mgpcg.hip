@@ -103,6 +103,7 @@ struct cg_problem {
 
   // device
   uint8_t* d_ctype = nullptr;
+  uint8_t* d_inert = nullptr;  // per tile: the current solve leaves it untouched (k_cg_init)
   uint8_t* d_fmask = nullptr;   // face masks, then (same allocation) the colour-split masks d_fsplit
   uint32_t* d_fsplit = nullptr;
   float* d_diag = nullptr;  // the colour sweep's diagonal per cell (k_diag), for k_gs_first
@@ -171,6 +172,7 @@ struct cg_problem {
     c.part = d_part;
     c.part2 = d_part2;
     c.tiles = gs_pair ? d_tiles : nullptr;
+    c.inert = d_inert;
     return c;
   }
   Ctx ctx() const { return ctx(d_ch[CG_CH_Z], d_ch[CG_CH_R]); }
@@ -185,8 +187,8 @@ cg_problem::~cg_problem() {
   if (h_pin) (void)hipHostFree(h_pin);
   if (h_ps) (void)hipHostFree(h_ps);
   if (ev_ps) (void)hipEventDestroy(ev_ps);
-  void* ptrs[] = {d_ctype, d_fmask, d_diag, d_zb, d_zh, d_rhsvh, d_zbh, d_x64, d_r64, d_bsave, d_psum, d_gsum, d_ps, d_dflag,
-                  d_plans, d_part, d_part2, d_scalar, d_lists, d_tiles, d_tmp_list, d_coarse, d_amgc, d_amgct, d_ext};
+  void* ptrs[] = {d_ctype, d_inert, d_fmask, d_diag, d_zb, d_zh, d_rhsvh, d_zbh, d_x64, d_r64, d_bsave, d_psum, d_gsum, d_ps,
+                  d_dflag, d_plans, d_part, d_part2, d_scalar, d_lists, d_tiles, d_tmp_list, d_coarse, d_amgc, d_amgct, d_ext};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   for (auto q : d_ch) if (q) (void)hipFree(q);
   for (auto q : d_api) if (q) (void)hipFree(q);
@@ -320,9 +322,13 @@ void build_tree(const cg_tree_desc* d, int prob, int base, int leaf_base, int gh
     // smoothing-view coefficients (subtree at the tile's own level)
     view_coeffs(T, l, T.bc);
     const uint8_t* ct = d->cell_type + size_t(B.ext[i]) * 512;
-    bool all_int = true;
-    for (int k = 0; k < 512 && all_int; k++) all_int = ct[k] == C_INT;
+    bool all_int = true, no_int = true;
+    for (int k = 0; k < 512; k++) {
+      all_int = all_int && ct[k] == C_INT;
+      no_int = no_int && !(ct[k] & C_INT);
+    }
     if (all_int) T.meta |= 1u << 25;
+    if (no_int && my == LEAF && !tr_has_gchild(T)) T.meta |= 1u << 26;
   }
   for (int i = 0; i < n; i++) {
     TileRec& T = B.tiles[i];
@@ -607,6 +613,8 @@ void build_pool(const cg_tree_desc* descs, int nprob, cg_problem& P) {
   }
   HIPC(hipMalloc(&P.d_zb, cells * sizeof(float)));
   HIPC(hipMemsetAsync(P.d_zb, 0, cells * sizeof(float), P.stream));
+  HIPC(hipMalloc(&P.d_inert, size_t(n)));
+  HIPC(hipMemsetAsync(P.d_inert, 0, size_t(n), P.stream));
   size_t g64 = size_t(std::max(1, ghost_total)) * 64 * sizeof(float);
   HIPC(hipMalloc(&P.d_gsum, g64));
   HIPC(hipMemsetAsync(P.d_gsum, 0, g64, P.stream));
@@ -743,7 +751,8 @@ struct VParams {
   int nu = 1, coarsest = 10, smoother = CG_SMOOTH_RBGS;
   float omega = 2.f / 3.f, ooa = 1.f, pcoef = 1.f;
   int check_done = 1;
-  int amg = 0;  // the preconditioner is VCycleAMG (cg_amg_solve)
+  int amg = 0;    // the preconditioner is VCycleAMG (cg_amg_solve)
+  int inert = 0;  // a CG iteration's V-cycle: skip the tiles the solve's init found inert (F_INERT)
 };
 
 // Workgroups of a looped colour sweep (tile block 8): as many as the device
@@ -850,7 +859,7 @@ constexpr int kProlongEagerMax = 16384;
 template <class V>
 void enqueue_vcycle(cg_problem& P, V* Z, V* ZO, const V* R, const VParams& v, int last_flags = 0) {
   const Ctx c = P.ctx();
-  const int base = F_VCYCLE | P.swz | (v.check_done ? F_CHECK_DONE : 0);
+  const int base = F_VCYCLE | P.swz | (v.check_done ? F_CHECK_DONE : 0) | (v.inert ? F_INERT : 0);
   const int L = P.max_level;  // a problem's levels above its coarse_C use the grid-wide lists actg / lvlg
   V* ZB = pool_zb<V>(P);
   for (int l = L; l >= 1; l--) {
@@ -1031,17 +1040,21 @@ void enqueue_amg_vcycle(cg_problem& P, const VParams& v) {
 // starts with the V-cycle): [re-centre r], z = M^-1 r, gamma/beta, p, A p and
 // alpha, x/r and the convergence test.
 template <class V>
-void enqueue_cg_iteration(cg_problem& P, const VParams& v, int pure_neumann) {
+void enqueue_cg_iteration(cg_problem& P, const VParams& vp, int pure_neumann) {
   const Ctx c = P.ctx();
-  const int fl = P.swz | F_CHECK_DONE;
+  // tiles k_cg_init found inert are skipped (not with AMG, whose V-cycle
+  // kernels do not take part; not in the server, whose init does not mark them)
+  VParams v = vp;
+  v.inert = !v.amg && c.inert != nullptr;
+  const int fl = P.swz | F_CHECK_DONE | (v.inert ? F_INERT : 0);
   const unsigned np = unsigned(P.nprob);  // one finalize workgroup per problem
   // one problem and no problem map: its partials are slots 0 .. leaf tiles - 1
   const bool one = P.nprob == 1 && !c.pmap && P.leaf_count.size() == 1 && P.leaf_begin.size() == 1;
   const int rb = one ? P.leaf_begin[0] : 0, rc = one ? P.leaf_count[0] : -1;
   if (pure_neumann) {
-    k_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, fl);
+    k_mean<<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, fl & ~F_INERT);
     k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_MEAN, fl);
-    k_subtract_mean<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, 1, fl);
+    k_subtract_mean<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, 1, fl & ~F_INERT);
   }
   // RB-GS: down-sweep in the scratch buffer, z assembled in z by the up-sweep
   V* Z = pool_z<V>(P);
@@ -1190,7 +1203,8 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
   set_prob_state(P, prm.rel_tol, prm.max_iters, prm.pure_neumann, prm.sync_stride, tol_q, maxit_q, thr_abs_q);
   int max_iters = prm.max_iters;
   if (maxit_q) for (int q = 0; q < P.nprob; q++) max_iters = std::max(max_iters, maxit_q[q]);
-  const Ctx c = P.ctx();
+  Ctx c = P.ctx();
+  if (amg) c.inert = nullptr;  // the AMG V-cycle skips no tile
   const bool f16s = prm.storage == CG_STORE_F16;
   if (f16s) {  // float16 storage: solve for 2^k b, k from |b|^2 per problem (FIN_SCALE)
     k_ir_norm<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r);
@@ -1253,7 +1267,8 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
     slot = 1 - slot;
   }
   // the last iteration's x update (k_cg_update_p applies each earlier one)
-  k_cg_finish_x<<<grid_for(P.leaf.n, P.swz), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, P.swz);
+  k_cg_finish_x<<<grid_for(P.leaf.n, P.swz), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n,
+                                                                 P.swz | (c.inert ? F_INERT : 0));
   if (f16s) {  // back to the caller's scale: x and the residual r
     k_scale_leaf<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.x, -1);
     k_scale_leaf<<<grid_for(P.leaf.n, kSwizzle), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, c.r, -1);
