@@ -43,7 +43,7 @@ struct alignas(16) TileRec {
   int32_t self;       // the tile's index
   float h;            // cell size of the tile's level
   uint32_t meta;      // level | type << 8 | octant << 16 | has_gchild << 24 | all_interior << 25 |
-                      // inert_candidate << 26
+                      // inert_candidate << 26 | all_open << 27
   uint32_t ntypes;    // type of face neighbour f in bits 4f..4f+3 (0: none)
   float bc[7];        // face coefficients of the smoothing view (subtree at the
                       // tile's own level): faces x-,y-,z-,x+,y+,z+, then inside
@@ -64,6 +64,9 @@ __host__ __device__ __forceinline__ bool tr_all_int(const TileRec& T) { return (
 // right-hand side is zero as well (decided per solve), every CG iteration
 // leaves each of its values as it found them (see k_cg_init)
 __host__ __device__ __forceinline__ bool tr_inert_cand(const TileRec& T) { return ((T.meta >> 26) & 1) != 0; }
+// every cell INTERIOR and no face cut by a NEUMANN cell: each face mask byte is
+// M_OPEN_INT (the masks need not be read)
+__host__ __device__ __forceinline__ bool tr_all_open(const TileRec& T) { return ((T.meta >> 27) & 1) != 0; }
 __host__ __device__ __forceinline__ uint8_t tr_ntype(const TileRec& T, int f) { return uint8_t((T.ntypes >> (4 * f)) & 0xf); }
 
 struct alignas(16) ProbState {
@@ -400,6 +403,18 @@ __device__ __forceinline__ void store_quad_split(V* __restrict__ tile, int pl, i
 // memory like the arrays it stands in for (one kind of load for both).
 __device__ __align__(16) float kZeroPage[1024] = {};
 template <class V> __device__ __forceinline__ const V* zero_page() { return reinterpret_cast<const V*>(kZeroPage); }
+// The face masks of a tile whose masks are all M_OPEN_INT (tr_all_open), in
+// both layouts (512 cell bytes; 128 colour-split words), read in place of the
+// tile's own: a load that stays in the caches instead of one from HBM.
+#define CG_R8(x) x, x, x, x, x, x, x, x
+__device__ __align__(16) uint32_t kOpenPage[128] = {CG_R8(CG_R8(0x7f7f7f7fu)), CG_R8(CG_R8(0x7f7f7f7fu))};
+#undef CG_R8
+__device__ __forceinline__ const uint8_t* own_masks(const Ctx& c, const TileRec& T) {
+  return tr_all_open(T) ? reinterpret_cast<const uint8_t*>(kOpenPage) : c.fmask + size_t(T.self) * 512;
+}
+__device__ __forceinline__ const uint32_t* own_split_masks(const Ctx& c, const TileRec& T) {
+  return tr_all_open(T) ? kOpenPage : c.fsplit + size_t(T.self) * 128;
+}
 
 template <class V> struct RawV;
 template <> struct RawV<float> { typedef float4 Q; typedef float S; typedef float2 S2; };
@@ -555,7 +570,7 @@ __device__ __forceinline__ StageRaw<V> stage_issue(const Ctx& c, const TileRec& 
   StageRaw<V> r;
   const bool own_zero = zero_all || (ghost_zero && own_ghost);
   r.own = load_pencil_any_raw<SPLIT>(own_zero ? zp : (own_ghost ? XG : X) + size_t(T.self) * 512, lane);
-  r.m = load_ct8(c.fmask + size_t(T.self) * 512 + lane * 8);
+  r.m = load_ct8(own_masks(c, T) + lane * 8);
   {
     const int g = (lane >> 3) & 3, t = lane & 7;
     const int f = ring_face(g);
@@ -880,7 +895,7 @@ __device__ __forceinline__ GsRaw<V> gs_issue(const Ctx& c, const TileRec& T, con
   g.done = *dp;
   g.xc = load_q_raw(tile + hc + lane * 4);
   g.xo = load_q_raw(tile + ho + lane * 4);
-  g.mask = c.fsplit[size_t(T.self) * 128 + color * 64 + lane];
+  g.mask = own_split_masks(c, T)[color * 64 + lane];
   if (DOT) {
     // r of the lane's pencil on LEAF tiles (zeros otherwise)
     const float* rp = tr_type(T) == T_LEAF ? c.r + size_t(T.self) * 512 : zero_page<float>();
@@ -1156,10 +1171,9 @@ template <class V>
 __device__ __forceinline__ void tile_gs_prolong(const Ctx& c, const TileRec& T, const V* XD, V* XO, float pcoef,
                                                 int color, int flags, TileLDS& L) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
-  const size_t base = size_t(T.self) * 512 + lane * 8;
   const int oct = tr_octant(T);
   const bool own_ghost = tr_type(T) == T_GHOST;
-  const uint64_t mown = load_ct8(c.fmask + base);
+  const uint64_t mown = load_ct8(own_masks(c, T) + lane * 8);
   const float4 pq = parent_quad(XO, T.parent, oct, i, j);
   float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), xb = xa;
   if (!own_ghost) load_pencil_split(XD + size_t(T.self) * 512, lane, xa, xb);
@@ -1393,7 +1407,7 @@ __global__ __launch_bounds__(64) void k_gs_first(Ctx c, const TileRec* __restric
   // ---- loads, all issued before the first wait
   const int32_t* dp = (flags & F_CHECK_DONE) ? &c.ps[T.prob].done : reinterpret_cast<const int32_t*>(c.fmask);
   const int32_t done = *dp;
-  const uint64_t mask = load_ct8(c.fmask + size_t(T.self) * 512 + lane * 8);
+  const uint64_t mask = load_ct8(own_masks(c, T) + lane * 8);
   typename RawV<V>::Q rb0 = load_q_raw(R + size_t(T.self) * 512 + lane * 4);        // colour-0 rhs quad
   typename RawV<V>::Q rb1 = load_q_raw(R + size_t(T.self) * 512 + 256 + lane * 4);  // colour-1 rhs quad
   // halo ring of the colour-1 pass: lanes 0..31, the colour-0 quad of one neighbour pencil
@@ -1528,14 +1542,13 @@ template <class V>
 __device__ __forceinline__ void tile_gs_prolong_eager(const Ctx& c, const TileRec& T, const V* XD, V* XO, float pcoef,
                                                       int color, int flags, TileLDS& L) {
   const int lane = lane_id(), i = lane >> 3, j = lane & 7;
-  const size_t base = size_t(T.self) * 512 + lane * 8;
   const int oct = tr_octant(T);
   const bool own_ghost = tr_type(T) == T_GHOST;
   const V* zp = zero_page<V>();
   // ---- loads, none in a branch (values that do not exist come from the zero page)
   const int32_t* dp = (flags & F_CHECK_DONE) ? &c.ps[T.prob].done : reinterpret_cast<const int32_t*>(c.fmask);
   const int32_t done = *dp;
-  const uint64_t mown = load_ct8(c.fmask + base);
+  const uint64_t mown = load_ct8(own_masks(c, T) + lane * 8);
   const int ppl = ((oct & 1) * 4 + (i >> 1)) * 8 + (((oct >> 1) & 1) * 4 + (j >> 1));  // parent pencil
   RawQuadS<V> rpq = load_quad_split_raw(XO + size_t(T.parent) * 512, ppl, ((oct >> 2) & 1) * 4);
   RawPencil<V> rown = load_pencil_split_raw(own_ghost ? zp : XD + size_t(T.self) * 512, lane);
@@ -2991,7 +3004,7 @@ __global__ __launch_bounds__(64) void k_apply64(Ctx c, const TileRec* __restrict
   load8d(rok ? X + size_t(rn) * 512 + ring_src(g, t) * 8 : zp, ring);
   const double xkm = *(T.nbr[2] >= 0 ? X + size_t(T.nbr[2]) * 512 + lane * 8 + 7 : zp);
   const double xkp = *(T.nbr[5] >= 0 ? X + size_t(T.nbr[5]) * 512 + lane * 8 : zp);
-  const uint64_t m = load_ct8(c.fmask + base);
+  const uint64_t m = load_ct8(own_masks(c, T) + lane * 8);
   double bf[6];
 #pragma unroll
   for (int f = 0; f < 6; f++) bf[f] = base_coeff64(to, T.nbr[f] < 0 ? to : tr_ntype(T, f), ooh);

@@ -283,10 +283,12 @@ void build_tree(const cg_tree_desc* d, int prob, int base, int leaf_base, int gh
     std::memset(&T, 0, sizeof(T));
     const int l = lev[i];
     const uint8_t my = ty[i];
+    int nbl[6];  // face neighbours, local indices
     for (int f = 0; f < 6; f++) {
       int q[3] = {cd[i][0], cd[i][1], cd[i][2]};
       q[f % 3] += (f < 3) ? -1 : 1;
       const int nb = find(l, q[0], q[1], q[2]);
+      nbl[f] = nb;
       T.nbr[f] = nb < 0 ? -1 : base + nb;
       T.ntypes |= uint32_t(nb >= 0 ? ty[nb] : 0) << (4 * f);
     }
@@ -329,6 +331,22 @@ void build_tree(const cg_tree_desc* d, int prob, int base, int leaf_base, int gh
     }
     if (all_int) T.meta |= 1u << 25;
     if (no_int && my == LEAF && !tr_has_gchild(T)) T.meta |= 1u << 26;
+    // every face mask M_OPEN_INT (as k_face_mask forms them): all cells
+    // INTERIOR and no NEUMANN cell across a tile face (no neighbour: open)
+    bool open = all_int;
+    for (int f = 0; f < 6 && open; f++) {
+      if (nbl[f] < 0) continue;
+      const uint8_t* nct = d->cell_type + size_t(B.ext[nbl[f]]) * 512;
+      const int a = f % 3, layer = f < 3 ? 7 : 0;
+      for (int u = 0; u < 64 && open; u++) {
+        int ijk[3];
+        ijk[a] = layer;
+        ijk[(a + 1) % 3] = u >> 3;
+        ijk[(a + 2) % 3] = u & 7;
+        if (nct[ijk[0] * 64 + ijk[1] * 8 + ijk[2]] & C_NEU) open = false;
+      }
+    }
+    if (open) T.meta |= 1u << 27;
   }
   for (int i = 0; i < n; i++) {
     TileRec& T = B.tiles[i];
