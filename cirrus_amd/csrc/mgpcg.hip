@@ -1060,8 +1060,8 @@ void enqueue_amg_vcycle(cg_problem& P, const VParams& v) {
 template <class V>
 void enqueue_cg_iteration(cg_problem& P, const VParams& vp, int pure_neumann) {
   const Ctx c = P.ctx();
-  // tiles k_cg_init found inert are skipped (not with AMG, whose V-cycle
-  // kernels do not take part; not in the server, whose init does not mark them)
+  // tiles the solve's (or the request's) init found inert are skipped (not with
+  // AMG, whose V-cycle kernels do not take part)
   VParams v = vp;
   v.inert = !v.amg && c.inert != nullptr;
   const int fl = P.swz | F_CHECK_DONE | (v.inert ? F_INERT : 0);

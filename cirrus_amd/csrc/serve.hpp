@@ -121,6 +121,7 @@ struct cg_server {
   // device
   cg_problem view;  // the kernels' view of the assembled active set (owner = false)
   uint8_t* d_ctype = nullptr;
+  uint8_t* d_inert = nullptr;   // per tile: the running request leaves it untouched (k_cg_init)
   uint8_t* d_fmask = nullptr;   // face masks, then (same allocation) the colour-split masks d_fsplit
   uint32_t* d_fsplit = nullptr;
   float* d_diag = nullptr;
@@ -188,7 +189,7 @@ cg_server::~cg_server() {
   for (auto& row : ev_gtab)
     for (auto e : row) if (e) (void)hipEventDestroy(e);
   if (h_ver) (void)hipHostFree(h_ver);
-  void* ptrs[] = {d_ctype, d_fmask, d_diag, d_zb, d_b, d_x64, d_r64, d_ver, d_zh, d_rhsvh, d_zbh, d_gsum, d_ps, d_dflag,
+  void* ptrs[] = {d_ctype, d_inert, d_fmask, d_diag, d_zb, d_b, d_x64, d_r64, d_ver, d_zh, d_rhsvh, d_zbh, d_gsum, d_ps, d_dflag,
                   d_dflag_init, d_pmap, d_part, d_part2, d_asm, d_recs, d_cslots, d_gasm, d_gtab, d_plans,
                   (void*)d_srcs, d_offs, d_counts};
   for (void* q : ptrs) if (q) (void)hipFree(q);
@@ -203,7 +204,7 @@ cg_server::~cg_server() {
 namespace {
 
 // Device bytes one tile costs: 6 channels, the V-cycle scratch, the kept rhs
-// and the colour sweep's diagonal (FP32), cell types and face masks, ghost row
+// and the colour sweep's diagonal (FP32), cell types, inert flags and face masks, ghost row
 // sums, dot partials and room for its records: the instance's own lists, the
 // assembled active worklists and the group worklists; BF16 storage adds three
 // bfloat16 vectors, verify_f64 two FP64 vectors.
@@ -211,7 +212,7 @@ constexpr int kRecKindsPerTile = 5;  // leaf, gather, applyl, actg, lvlg: at mos
 // An instance's own lists (leaf, gather, applyl, ghosts, all, actg, lvlg) hold
 // 3 leaf + ghost + 3 n <= 6 n records; a group's (leaf, ghosts, applyl) <= 2 n.
 constexpr int kOwnRecsPerTile = 6, kGroupRecsPerTile = 2;
-constexpr int64_t kBytesPerTile = (CG_NUM_CHANNELS + 3) * 512 * 4 + 2 * 512 + 64 * 4 + 2 * 8 +
+constexpr int64_t kBytesPerTile = (CG_NUM_CHANNELS + 3) * 512 * 4 + 2 * 512 + 1 + 64 * 4 + 2 * 8 +
                                   (kRecKindsPerTile + kOwnRecsPerTile + kGroupRecsPerTile) * int64_t(sizeof(TileRec));
 
 void server_create(const cg_server_config* cfg, cg_server& S) {
@@ -248,6 +249,8 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   HIPC(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
   const size_t cells = size_t(S.cap_tiles) * 512;
   HIPC(hipMalloc(&S.d_ctype, cells));
+  HIPC(hipMalloc(&S.d_inert, size_t(S.cap_tiles)));
+  HIPC(hipMemsetAsync(S.d_inert, 0, size_t(S.cap_tiles), S.stream));
   HIPC(hipMalloc(&S.d_fmask, 2 * cells));
   S.d_fsplit = reinterpret_cast<uint32_t*>(S.d_fmask + cells);
   HIPC(hipMalloc(&S.d_diag, cells * sizeof(float)));
@@ -313,6 +316,7 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   P.owner = false;
   P.stream = S.stream;
   P.d_ctype = S.d_ctype;
+  P.d_inert = S.d_inert;
   P.d_fmask = S.d_fmask;
   P.d_fsplit = S.d_fsplit;
   P.d_diag = S.d_diag;
@@ -864,11 +868,11 @@ void server_collect(cg_server& S, int k) {
   // the last iteration's x update (k_cg_update_p applies each earlier one)
   for (int q : rest) {
     const Instance& I = *S.inst[q];
-    k_cg_finish_x<<<grid_for(I.leaf.n, kSwizzle), 64, 0, stm>>>(c, I.L(I.leaf), I.leaf.n, kSwizzle);
+    k_cg_finish_x<<<grid_for(I.leaf.n, kSwizzle), 64, 0, stm>>>(c, I.L(I.leaf), I.leaf.n, kSwizzle | F_INERT);
   }
   if (!check.empty()) {
     const Group G = group_lists(S, k, GRP_FINISH, check, int(check.size()), nullptr);
-    k_cg_finish_x<<<grid_for(G.leaf.n, kSwizzle), 64, 0, stm>>>(c, GL(S, G.leaf), G.leaf.n, kSwizzle);
+    k_cg_finish_x<<<grid_for(G.leaf.n, kSwizzle), 64, 0, stm>>>(c, GL(S, G.leaf), G.leaf.n, kSwizzle | F_INERT);
     enqueue_verify(S, G, check);
   }
   for (int q : rest) {
