@@ -273,3 +273,53 @@ def test_refined_warm_sequences_same_alone_and_in_the_mix():
         assert np.array_equal(x1[m], x2[m])
     # the warm requests need fewer iterations than the cold one
     assert seq_alone[1][2]["iters"] < seq_alone[0][2]["iters"]
+
+
+def test_admission_churn_results_unchanged():
+    """Instances admitted, served and released in a stream (released memory and
+    slots come back once the stream has passed the release, and new instances
+    reuse them while other requests run; requests start and finish in groups):
+    every cold refined request still equals the pool's refinement of the same
+    problem alone, bit for bit."""
+    scs = scenes()
+    ref = {}
+    for q, sc in enumerate(scs):
+        P = ca.Problem(sc.tree)
+        P.set(ca.CH_B, sc.rhs)
+        r = ca.GMGSolver().solve_refined(P, 1e-6, 0.0, 5, 1000, KW["level_iters"], KW["coarsest_iters"],
+                                         storage=ca.STORE_BF16)[0]
+        ref[q] = ((r["iters"], r["rr"], r["bb"]), P.get(ca.CH_X))
+    S = ca.Server(BUDGET, 3, refine=True, storage=ca.STORE_BF16, **KW)
+    rng = np.random.default_rng(11)
+    live = {}  # instance -> scene
+    running = {}  # tag -> (instance, scene)
+    done = 0
+    tag = 0
+    for _ in range(20000):
+        while len(live) < 3 and tag < 40:
+            q = int(rng.integers(len(scs)))
+            i = S.admit(scs[q].tree)
+            live[i] = q
+            S.submit(i, scs[q].rhs, tag=tag)
+            running[tag] = (i, q)
+            tag += 1
+        for _ in range(int(rng.integers(1, 4))):
+            S.step()
+        while True:
+            r = S.next_result()
+            if r is None:
+                break
+            t, inst, res, x = r
+            i, q = running.pop(t)
+            assert inst == i
+            want, xw = ref[q]
+            assert (res["iters"], res["rr"], res["bb"]) == want, (t, q, res)
+            m = scs[q].tree.type == po.LEAF
+            assert np.array_equal(x[m], xw[m]), (t, q)
+            S.release(i)
+            del live[i]
+            done += 1
+        if tag >= 40 and not running:
+            break
+    assert done == 40
+    assert S.info()["instances"] == 0
