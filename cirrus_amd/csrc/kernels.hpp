@@ -637,6 +637,110 @@ __device__ __forceinline__ void stage_tile(const Ctx& c, const TileRec& T, const
   stage_commit<SPLIT>(r, T, bcv, L, S);
 }
 
+// The CG operator's staging, where p is not stored on GHOST tiles: a GHOST
+// cell's value is its parent cell's if that cell is INTERIOR, else 0 (the
+// copy into ghost tiles the reference makes before the operator,
+// PoissonGrid.cu:585-605), read from the parent (a LEAF tile one level up)
+// wherever the tile itself or a face neighbour in its halo is a GHOST.  FP32,
+// natural layout; every load unconditional as in stage_issue.
+struct StagePG {
+  float4 own0, own1, halo0, halo1;
+  uint32_t own_ct, halo_ct, km_ct, kp_ct;
+  float km, kp;
+  uint64_t m;
+  bool halo_gh;  // the lane's ring pencil is a GHOST's (from its parent)
+};
+// Cell of `parent` that holds cell (i, j, 2 k4 .. 2 k4 + 1) of its child with
+// octant o (the pencil's four parent cells start at k4 = 0).
+__device__ __forceinline__ size_t pg_cell(int parent, int o, int i, int j, int k4) {
+  return size_t(parent) * 512 +
+         size_t((((o & 1) * 4 + (i >> 1)) * 8 + (((o >> 1) & 1) * 4 + (j >> 1))) * 8 + ((o >> 2) & 1) * 4 + k4);
+}
+__device__ __forceinline__ StagePG stage_issue_pg(const Ctx& c, const TileRec& T, const float* __restrict__ X) {
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const bool own_ghost = tr_type(T) == T_GHOST;
+  const int oct = tr_octant(T);
+  const float* zp = zero_page<float>();
+  const uint32_t* zw = reinterpret_cast<const uint32_t*>(kZeroPage);
+  StagePG r;
+  // own pencil, or its four parent cells and their types
+  const size_t po = pg_cell(own_ghost ? T.parent : 0, oct, i, j, 0);
+  const float* own = own_ghost ? X + po : X + size_t(T.self) * 512 + lane * 8;
+  r.own0 = *reinterpret_cast<const float4*>(own);
+  r.own1 = *reinterpret_cast<const float4*>(own_ghost ? zp : own + 4);
+  r.own_ct = *(own_ghost ? reinterpret_cast<const uint32_t*>(c.ctype + po) : zw);
+  r.m = load_ct8(own_masks(c, T) + lane * 8);
+  {  // ring: lanes 0..31, one neighbour pencil each (x = 0 outside the grid)
+    const int g = (lane >> 3) & 3, t = lane & 7;
+    const int f = ring_face(g);
+    const int n = pick4(g, uni(T.nbr[0]), uni(T.nbr[3]), uni(T.nbr[1]), uni(T.nbr[4]));
+    const int npar = pick4(g, uni(T.nbr_parent[0]), uni(T.nbr_parent[3]), uni(T.nbr_parent[1]), uni(T.nbr_parent[4]));
+    const bool ok = lane < 32 && n >= 0;
+    const bool gh = ok && tr_ntype(T, f) == T_GHOST;
+    const int sp = ring_src(g, t);                        // the neighbour's pencil si * 8 + sj
+    const int no = oct ^ (g < 2 ? 1 : 2);                 // its octant: across x or y
+    const size_t pn = pg_cell(gh ? npar : 0, no, sp >> 3, sp & 7, 0);
+    const float* h = !ok ? zp : (gh ? X + pn : X + size_t(n) * 512 + sp * 8);
+    r.halo0 = *reinterpret_cast<const float4*>(h);
+    r.halo1 = *reinterpret_cast<const float4*>((ok && !gh) ? h + 4 : zp);
+    r.halo_ct = *(gh ? reinterpret_cast<const uint32_t*>(c.ctype + pn) : zw);
+    r.halo_gh = gh;
+  }
+  // k halos: cell (i,j,7) of the z- neighbour, (i,j,0) of the z+ neighbour
+  const int nm = T.nbr[2], np = T.nbr[5];
+  const bool ghm = nm >= 0 && tr_ntype(T, 2) == T_GHOST, ghp = np >= 0 && tr_ntype(T, 5) == T_GHOST;
+  const size_t pm = pg_cell(ghm ? T.nbr_parent[2] : 0, oct ^ 4, i, j, 3);
+  const size_t pp = pg_cell(ghp ? T.nbr_parent[5] : 0, oct ^ 4, i, j, 0);
+  r.km = *(nm < 0 ? zp : (ghm ? X + pm : X + size_t(nm) * 512 + lane * 8 + 7));
+  r.kp = *(np < 0 ? zp : (ghp ? X + pp : X + size_t(np) * 512 + lane * 8));
+  r.km_ct = *(ghm ? c.ctype + pm : reinterpret_cast<const uint8_t*>(zw));
+  r.kp_ct = *(ghp ? c.ctype + pp : reinterpret_cast<const uint8_t*>(zw));
+  return r;
+}
+// A ghost pencil from its four parent cells: cells 2m and 2m + 1 take parent
+// value m if that cell is INTERIOR, else 0.
+__device__ __forceinline__ void pg_expand(float4 q, uint32_t ct, float4& a, float4& b) {
+  const float v0 = (ct & C_INT) ? q.x : 0.f, v1 = ((ct >> 8) & C_INT) ? q.y : 0.f;
+  const float v2 = ((ct >> 16) & C_INT) ? q.z : 0.f, v3 = ((ct >> 24) & C_INT) ? q.w : 0.f;
+  a = make_float4(v0, v0, v1, v1);
+  b = make_float4(v2, v2, v3, v3);
+}
+__device__ __forceinline__ void stage_commit_pg(StagePG& r, const TileRec& T, const float bcv[7], TileLDS& L,
+                                                Stencil& S) {
+  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  hold_all(r.own0);
+  hold_all(r.own1);
+  hold_all(r.halo0);
+  hold_all(r.halo1);
+  hold_all(r.km);
+  hold_all(r.kp);
+  hold_all(r.own_ct);
+  hold_all(r.halo_ct);
+  hold_all(r.km_ct);
+  hold_all(r.kp_ct);
+  float4 a = r.own0, b = r.own1;
+  if (tr_type(T) == T_GHOST) pg_expand(r.own0, r.own_ct, a, b);
+  S.own[0] = a.x; S.own[1] = a.y; S.own[2] = a.z; S.own[3] = a.w;
+  S.own[4] = b.x; S.own[5] = b.y; S.own[6] = b.z; S.own[7] = b.w;
+  S.m = r.m;
+  const int pc = (i + 1) * 10 + (j + 1);
+  L.xs[0][pc] = a;
+  L.xs[1][pc] = b;
+  if (lane < 32) {
+    const int g = lane >> 3, t = lane & 7;
+    float4 v0 = r.halo0, v1 = r.halo1;
+    if (r.halo_gh) pg_expand(r.halo0, r.halo_ct, v0, v1);
+    L.xs[0][ring_dst(g, t)] = v0;
+    L.xs[1][ring_dst(g, t)] = v1;
+  }
+  const bool ghm = T.nbr[2] >= 0 && tr_ntype(T, 2) == T_GHOST, ghp = T.nbr[5] >= 0 && tr_ntype(T, 5) == T_GHOST;
+  S.xkm = (ghm && !(r.km_ct & C_INT)) ? 0.f : r.km;
+  S.xkp = (ghp && !(r.kp_ct & C_INT)) ? 0.f : r.kp;
+  S.h = T.h;
+  S.b_int = bcv[6];
+  for (int f = 0; f < 6; f++) S.b_f[f] = bcv[f];
+}
+
 // Face coefficients of tile T seen with the subtree view at level sl
 // (GMGSolver.cu:26-39): 1/h^2 and (2/h/3)/h as the reference forms them; an
 // absent neighbour counts as the tile's own type (GMGSolver.cu:118-122).
@@ -2132,14 +2236,14 @@ __global__ __launch_bounds__(64) void k_cg_dot_rz(Ctx c, const TileRec* __restri
 // The previous iteration's x += alpha p (deferred from k_cg_update_xr, which
 // then reads neither x nor p; the same fma on the same values, one iteration
 // later, and k_cg_finish_x applies the last one), then p = z + beta p on
-// INTERIOR cells of LEAF tiles (GMGSolver.cu:587-609 and 642-653); the new p
-// copied into the tile's ghost children (their x is the parent's INTERIOR
-// value, PoissonGrid.cu:585-605), ready for the operator.
+// INTERIOR cells of LEAF tiles (GMGSolver.cu:587-609 and 642-653).  The ghost
+// children's p (the parent's INTERIOR values, PoissonGrid.cu:585-605) is not
+// stored: the operator reads it from here (k_apply<true>).
 template <class V>
 __global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const TileRec* __restrict__ list, int n, int flags) {
   TILE_KERNEL_PROLOGUE
   SKIP_INERT
-  const int lane = lane_id(), i = lane >> 3, j = lane & 7;
+  const int lane = lane_id();
   const size_t base = size_t(T.self) * 512 + lane * 8;
   float pv[8], zv[8], xv[8];
   load8(c.p + base, pv);
@@ -2158,20 +2262,6 @@ __global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const TileRec* __rest
     }
   if (upd_x) store8(c.x + base, xv);
   store8(c.p + base, pv);
-  if (!tr_has_gchild(T)) return;
-  __shared__ float Lp[512];
-  __shared__ uint8_t Lc[512];
-  for (int k = 0; k < 8; k++) { Lp[lane * 8 + k] = pv[k]; Lc[lane * 8 + k] = ct_at(ct, k); }
-  wave_sync();
-  for (int o = 0; o < 8; o++) {
-    int g = T.gch[o];
-    if (g < 0) continue;
-    int ox = o & 1, oy = (o >> 1) & 1, oz = (o >> 2) & 1;
-    int src = ((ox * 4 + (i >> 1)) * 8 + (oy * 4 + (j >> 1))) * 8 + oz * 4;
-    float gv[8];
-    for (int k = 0; k < 8; k++) gv[k] = (Lc[src + (k >> 1)] & C_INT) ? Lp[src + (k >> 1)] : 0.f;
-    store8(c.p + size_t(g) * 512 + lane * 8, gv);
-  }
 }
 
 // A p on LEAF and GHOST tiles with the subtree view sl.  In CG mode (dot=1)
@@ -2179,6 +2269,9 @@ __global__ __launch_bounds__(64) void k_cg_update_p(Ctx c, const TileRec* __rest
 // partial; GHOST tiles write the 2x2x2 sums of their rows for the parent
 // (AccumulateToParents128 with all cell types, PoissonGrid.cu:664-710).  In
 // channel mode (dot=0) every listed tile writes its rows to AX.
+// PG (the CG iteration's operator on p): GHOST values from their parents
+// (stage_issue_pg), as p is not copied into ghost tiles.
+template <bool PG = false>
 __global__ __launch_bounds__(64) void k_apply(Ctx c, const TileRec* __restrict__ list, int n, const float* __restrict__ X,
                                               float* __restrict__ AX, int sl, int diag, int dot, int flags) {
   TILE_KERNEL_PROLOGUE
@@ -2188,7 +2281,12 @@ __global__ __launch_bounds__(64) void k_apply(Ctx c, const TileRec* __restrict__
   Stencil S;
   float bcv[7];
   view_coeffs(T, sl, bcv);
-  stage_tile<false>(c, T, X, X, false, false, bcv, L, S);
+  if (PG) {
+    StagePG r = stage_issue_pg(c, T, X);
+    stage_commit_pg(r, T, bcv, L, S);
+  } else {
+    stage_tile<false>(c, T, X, X, false, false, bcv, L, S);
+  }
   if (prob_done(c, T, flags)) return;
   wave_sync();
   float ax[8], dg[8];

@@ -1084,7 +1084,7 @@ void enqueue_cg_iteration(cg_problem& P, const VParams& vp, int pure_neumann) {
   if (!rz_fused) k_cg_dot_rz<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
   k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_RZ, fl, rb, rc);
   k_cg_update_p<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
-  k_apply<<<grid_for(P.applyl.n, fl), 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, c.p, c.ap, -1, 0, 1, fl);
+  k_apply<true><<<grid_for(P.applyl.n, fl), 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, c.p, c.ap, -1, 0, 1, fl);
   if (P.gather.n) k_ghost_gather<<<grid_for(P.gather.n, fl), 64, 0, P.stream>>>(c, P.L(P.gather), P.gather.n, fl);
   k_finalize<<<np, FIN_THREADS, 0, P.stream>>>(c, FIN_PAP, fl, rb, rc);
   k_cg_update_xr<V><<<grid_for(P.leaf.n, fl), 64, 0, P.stream>>>(c, P.L(P.leaf), P.leaf.n, fl);
@@ -2339,7 +2339,8 @@ cg_status cg_benchmark_kernel(cg_problem* p, int32_t kernel, int32_t level, int3
         }
         case CG_KERNEL_APPLY:
           nt = P.applyl.n;
-          k_apply<<<grid_for(P.applyl.n, base), 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, c.p, c.ap, -1, 0, 1, base);
+          k_apply<true><<<grid_for(P.applyl.n, base), 64, 0, P.stream>>>(c, P.L(P.applyl), P.applyl.n, c.p, c.ap, -1, 0, 1,
+                                                                        base);
           break;
         case CG_KERNEL_RESTRICT: {
           const DList& al = P.lvl[level];
