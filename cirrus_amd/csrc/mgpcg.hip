@@ -323,28 +323,39 @@ void build_tree(const cg_tree_desc* d, int prob, int base, int leaf_base, int gh
     }
     // smoothing-view coefficients (subtree at the tile's own level)
     view_coeffs(T, l, T.bc);
-    const uint8_t* ct = d->cell_type + size_t(B.ext[i]) * 512;
+    // the cell types as 64 words of 8 cells (cell index i*64 + j*8 + k)
+    constexpr uint64_t kOnes = 0x0101010101010101ull;
+    auto words = [&](int t) {
+      std::array<uint64_t, 64> w;
+      std::memcpy(w.data(), d->cell_type + size_t(B.ext[t]) * 512, 512);
+      return w;
+    };
+    const auto ct = words(i);
     bool all_int = true, no_int = true;
-    for (int k = 0; k < 512; k++) {
-      all_int = all_int && ct[k] == C_INT;
-      no_int = no_int && !(ct[k] & C_INT);
+    for (int q = 0; q < 64; q++) {
+      all_int = all_int && ct[q] == kOnes * C_INT;
+      no_int = no_int && (ct[q] & (kOnes * C_INT)) == 0;
     }
     if (all_int) T.meta |= 1u << 25;
     if (no_int && my == LEAF && !tr_has_gchild(T)) T.meta |= 1u << 26;
     // every face mask M_OPEN_INT (as k_face_mask forms them): all cells
-    // INTERIOR and no NEUMANN cell across a tile face (no neighbour: open)
+    // INTERIOR and no NEUMANN cell across a tile face (no neighbour: open).
+    // The neighbour's face layer: x faces i = 7 / 0 (words 56..63 / 0..7),
+    // y faces j = 7 / 0 (word i*8 + 7 / i*8), z faces k = 7 / 0 (byte 7 / 0 of
+    // every word).
     bool open = all_int;
     for (int f = 0; f < 6 && open; f++) {
       if (nbl[f] < 0) continue;
-      const uint8_t* nct = d->cell_type + size_t(B.ext[nbl[f]]) * 512;
-      const int a = f % 3, layer = f < 3 ? 7 : 0;
-      for (int u = 0; u < 64 && open; u++) {
-        int ijk[3];
-        ijk[a] = layer;
-        ijk[(a + 1) % 3] = u >> 3;
-        ijk[(a + 2) % 3] = u & 7;
-        if (nct[ijk[0] * 64 + ijk[1] * 8 + ijk[2]] & C_NEU) open = false;
+      const auto nw = words(nbl[f]);
+      const bool minus = f < 3;
+      uint64_t neu = 0;
+      for (int u = 0; u < 8; u++) {
+        if (f % 3 == 0) neu |= nw[(minus ? 56 : 0) + u];
+        else if (f % 3 == 1) neu |= nw[u * 8 + (minus ? 7 : 0)];
       }
+      if (f % 3 == 2)
+        for (int q = 0; q < 64; q++) neu |= (nw[q] >> (minus ? 56 : 0)) & 0xffu;
+      if (neu & (kOnes * C_NEU)) open = false;
     }
     if (open) T.meta |= 1u << 27;
   }

@@ -480,26 +480,9 @@ void admit_upload(cg_server& S, const cg_tree_desc* d, const TreeBuild& B, const
                   const std::vector<CoarseSlot>& cslots, Instance& inst, int slot, int64_t tb, int64_t rb) {
   Instance* I = &inst;
   const int n = d->num_tiles;
-  I->rec_begin = rb;
-  I->rec_count = int(W.size());
-  I->d_recs = S.d_recs + rb;
-  HIPC(hipMemcpy(I->d_recs, W.data(), sizeof(TileRec) * W.size(), hipMemcpyHostToDevice));
-  if (I->C >= 0) {
-    I->d_slots = S.d_cslots + size_t(slot) * CL_MAX_TILES;
-    HIPC(hipMemcpy(I->d_slots, cslots.data(), sizeof(CoarseSlot) * cslots.size(), hipMemcpyHostToDevice));
-    I->plan.slots = I->d_slots;
-  }
-  // cell types in internal order; x = 0 so that a first warm start is a cold one
-  std::vector<uint8_t> ct(size_t(n) * 512);
-  for (int i = 0; i < n; i++) std::memcpy(&ct[size_t(i) * 512], d->cell_type + size_t(B.ext[i]) * 512, 512);
-  HIPC(hipMemcpy(S.d_ctype + size_t(tb) * 512, ct.data(), ct.size(), hipMemcpyHostToDevice));
-  // face masks on the server's stream, ahead of the instance's first request
-  k_face_mask<<<I->all.n, 64, 0, S.stream>>>(S.d_ctype, S.d_fmask, S.d_fsplit, I->L(I->all), I->all.n);
-  k_diag<<<I->all.n, 64, 0, S.stream>>>(S.view.ctx(), I->L(I->all), I->all.n, S.d_diag);
-  HIPC(hipGetLastError());
-  HIPC(hipMemsetAsync(S.d_ch[CG_CH_X] + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float), S.stream));
-  HIPC(hipMemsetAsync(S.d_b + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float), S.stream));
-  if (S.d_x64) HIPC(hipMemsetAsync(S.d_x64 + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(double), S.stream));
+  // the staging buffer first: the uploads below go through it (pinned memory,
+  // copied by DMA; each copy is complete when hipMemcpy returns, so the
+  // buffer is free again for the first request's right-hand side)
   const size_t need = size_t(n) * 512 * sizeof(float);
   auto kept = S.hbuf_cache.lower_bound(need);
   if (kept != S.hbuf_cache.end() && kept->first <= 2 * need) {
@@ -510,6 +493,31 @@ void admit_upload(cg_server& S, const cg_tree_desc* d, const TreeBuild& B, const
     HIPC(hipHostMalloc(&I->h_buf, need, hipHostMallocDefault));
     I->h_bytes = need;
   }
+  char* stage = reinterpret_cast<char*>(I->h_buf);
+  I->rec_begin = rb;
+  I->rec_count = int(W.size());
+  I->d_recs = S.d_recs + rb;
+  // records (at most 6 per tile, 1,056 B) then cell types (512 B per tile) fit
+  // the 2,048 B per tile of the staging buffer
+  const size_t rec_bytes = sizeof(TileRec) * W.size(), ct_off = (rec_bytes + 255) & ~size_t(255);
+  std::memcpy(stage, W.data(), rec_bytes);
+  HIPC(hipMemcpy(I->d_recs, stage, rec_bytes, hipMemcpyHostToDevice));
+  if (I->C >= 0) {
+    I->d_slots = S.d_cslots + size_t(slot) * CL_MAX_TILES;
+    HIPC(hipMemcpy(I->d_slots, cslots.data(), sizeof(CoarseSlot) * cslots.size(), hipMemcpyHostToDevice));
+    I->plan.slots = I->d_slots;
+  }
+  // cell types in internal order; x = 0 so that a first warm start is a cold one
+  uint8_t* ct = reinterpret_cast<uint8_t*>(stage + ct_off);
+  for (int i = 0; i < n; i++) std::memcpy(ct + size_t(i) * 512, d->cell_type + size_t(B.ext[i]) * 512, 512);
+  HIPC(hipMemcpy(S.d_ctype + size_t(tb) * 512, ct, size_t(n) * 512, hipMemcpyHostToDevice));
+  // face masks on the server's stream, ahead of the instance's first request
+  k_face_mask<<<I->all.n, 64, 0, S.stream>>>(S.d_ctype, S.d_fmask, S.d_fsplit, I->L(I->all), I->all.n);
+  k_diag<<<I->all.n, 64, 0, S.stream>>>(S.view.ctx(), I->L(I->all), I->all.n, S.d_diag);
+  HIPC(hipGetLastError());
+  HIPC(hipMemsetAsync(S.d_ch[CG_CH_X] + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float), S.stream));
+  HIPC(hipMemsetAsync(S.d_b + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(float), S.stream));
+  if (S.d_x64) HIPC(hipMemsetAsync(S.d_x64 + size_t(tb) * 512, 0, size_t(n) * 512 * sizeof(double), S.stream));
   if (!S.slot_ev[slot]) HIPC(hipEventCreateWithFlags(&S.slot_ev[slot], hipEventDisableTiming));
   if (!S.slot_ev_ver[slot]) HIPC(hipEventCreateWithFlags(&S.slot_ev_ver[slot], hipEventDisableTiming));
   I->ev = S.slot_ev[slot];
