@@ -103,6 +103,12 @@ struct cg_server {
   std::vector<Released> pending;  // releases the stream has not yet passed
   std::vector<hipEvent_t> ev_free;
   std::multimap<size_t, float*> hbuf_cache;  // pinned staging buffers of released instances, by size
+  // pinned ring for admission uploads (records, coarse slots, cell types),
+  // copied on the stream; a segment is reused once its event has passed
+  char* h_ring = nullptr;
+  size_t ring_cap = 0, ring_head = 0;
+  struct RingSeg { size_t begin, end; hipEvent_t ev; };
+  std::deque<RingSeg> ring_busy;
   std::vector<hipEvent_t> slot_ev, slot_ev_ver;
   std::vector<std::unique_ptr<Instance>> inst;  // by slot
   std::vector<int> active;                      // slots whose request runs, assembly order
@@ -183,6 +189,8 @@ cg_server::~cg_server() {
     if (R.ev) (void)hipEventDestroy(R.ev);
   }
   for (auto& kv : hbuf_cache) (void)hipHostFree(kv.second);
+  for (auto& g : ring_busy) (void)hipEventDestroy(g.ev);
+  if (h_ring) (void)hipHostFree(h_ring);
   for (auto e : ev_free) (void)hipEventDestroy(e);
   for (auto e : slot_ev) if (e) (void)hipEventDestroy(e);
   for (auto e : slot_ev_ver) if (e) (void)hipEventDestroy(e);
@@ -310,6 +318,8 @@ void server_create(const cg_server_config* cfg, cg_server& S) {
   // per-step tables: runs (srcs, offs, counts), active map, coarse plans
   S.tab_cap = size_t(S.runs_cap) * (sizeof(TileRec*) + 8) + size_t(S.max_probs) * (8 + sizeof(CoarsePlan)) + 4096;
   for (int k = 0; k < 2; k++) HIPC(hipHostMalloc(&S.h_tab[k], S.tab_cap, hipHostMallocDefault));
+  S.ring_cap = size_t(64) << 20;
+  HIPC(hipHostMalloc(&S.h_ring, S.ring_cap, hipHostMallocDefault));
 
   // the kernels' view: shares the server's arrays
   cg_problem& P = S.view;
@@ -473,6 +483,37 @@ void server_admit(cg_server& S, const cg_tree_desc* d, int32_t* out) {
   S.nres++;
 }
 
+// `bytes` of the admission ring (nullptr if it never fits): the oldest
+// segments in the way are waited for (they precede everything enqueued since).
+char* ring_take(cg_server& S, size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  if (bytes > S.ring_cap) return nullptr;
+  size_t at = S.ring_head;
+  if (at + bytes > S.ring_cap) at = 0;
+  auto overlaps = [&](const cg_server::RingSeg& g) { return g.begin < at + bytes && at < g.end; };
+  while (!S.ring_busy.empty() && std::any_of(S.ring_busy.begin(), S.ring_busy.end(), overlaps)) {
+    HIPC(hipEventSynchronize(S.ring_busy.front().ev));
+    S.ev_free.push_back(S.ring_busy.front().ev);
+    S.ring_busy.pop_front();
+  }
+  S.ring_head = at + bytes;
+  return S.h_ring + at;
+}
+// The segment [p, p + bytes) of the ring is read by copies enqueued so far.
+void ring_release_after_copies(cg_server& S, const char* p, size_t bytes) {
+  cg_server::RingSeg g;
+  g.begin = size_t(p - S.h_ring);
+  g.end = g.begin + ((bytes + 255) & ~size_t(255));
+  if (S.ev_free.empty()) {
+    HIPC(hipEventCreateWithFlags(&g.ev, hipEventDisableTiming));
+  } else {
+    g.ev = S.ev_free.back();
+    S.ev_free.pop_back();
+  }
+  HIPC(hipEventRecord(g.ev, S.stream));
+  S.ring_busy.push_back(g);
+}
+
 // The admitted instance's records, coarse slots and cell types to the device,
 // its face masks, diagonal and zeroed vectors (on the server's stream), its
 // staging buffer and events.
@@ -480,9 +521,7 @@ void admit_upload(cg_server& S, const cg_tree_desc* d, const TreeBuild& B, const
                   const std::vector<CoarseSlot>& cslots, Instance& inst, int slot, int64_t tb, int64_t rb) {
   Instance* I = &inst;
   const int n = d->num_tiles;
-  // the staging buffer first: the uploads below go through it (pinned memory,
-  // copied by DMA; each copy is complete when hipMemcpy returns, so the
-  // buffer is free again for the first request's right-hand side)
+  // the instance's pinned staging buffer (right-hand sides in, pressures out)
   const size_t need = size_t(n) * 512 * sizeof(float);
   auto kept = S.hbuf_cache.lower_bound(need);
   if (kept != S.hbuf_cache.end() && kept->first <= 2 * need) {
@@ -493,24 +532,36 @@ void admit_upload(cg_server& S, const cg_tree_desc* d, const TreeBuild& B, const
     HIPC(hipHostMalloc(&I->h_buf, need, hipHostMallocDefault));
     I->h_bytes = need;
   }
-  char* stage = reinterpret_cast<char*>(I->h_buf);
   I->rec_begin = rb;
   I->rec_count = int(W.size());
   I->d_recs = S.d_recs + rb;
-  // records (at most 6 per tile, 1,056 B) then cell types (512 B per tile) fit
-  // the 2,048 B per tile of the staging buffer
-  const size_t rec_bytes = sizeof(TileRec) * W.size(), ct_off = (rec_bytes + 255) & ~size_t(255);
+  // records, coarse slots and cell types through the admission ring, copied on
+  // the stream (the host does not wait for a device that is busy with other
+  // requests); a tree too large for the ring goes through its own staging
+  // buffer with synchronous copies
+  const size_t rec_bytes = sizeof(TileRec) * W.size(), ct_bytes = size_t(n) * 512;
+  const size_t slot_bytes = sizeof(CoarseSlot) * cslots.size();
+  const size_t o_ct = (rec_bytes + 255) & ~size_t(255), o_sl = o_ct + ((ct_bytes + 255) & ~size_t(255));
+  const size_t total = o_sl + slot_bytes;
+  char* ring = ring_take(S, total);
+  char* stage = ring ? ring : reinterpret_cast<char*>(I->h_buf);
+  if (!ring && total > I->h_bytes) fail(CG_ERR_INVALID, "cg_server_admit: staging buffer too small");
   std::memcpy(stage, W.data(), rec_bytes);
-  HIPC(hipMemcpy(I->d_recs, stage, rec_bytes, hipMemcpyHostToDevice));
+  for (int i = 0; i < n; i++) std::memcpy(stage + o_ct + size_t(i) * 512, d->cell_type + size_t(B.ext[i]) * 512, 512);
+  if (slot_bytes) std::memcpy(stage + o_sl, cslots.data(), slot_bytes);
+  auto up = [&](void* dst, size_t off, size_t bytes) {
+    if (ring) HIPC(hipMemcpyAsync(dst, stage + off, bytes, hipMemcpyHostToDevice, S.stream));
+    else HIPC(hipMemcpy(dst, stage + off, bytes, hipMemcpyHostToDevice));
+  };
+  up(I->d_recs, 0, rec_bytes);
   if (I->C >= 0) {
     I->d_slots = S.d_cslots + size_t(slot) * CL_MAX_TILES;
-    HIPC(hipMemcpy(I->d_slots, cslots.data(), sizeof(CoarseSlot) * cslots.size(), hipMemcpyHostToDevice));
+    up(I->d_slots, o_sl, slot_bytes);
     I->plan.slots = I->d_slots;
   }
   // cell types in internal order; x = 0 so that a first warm start is a cold one
-  uint8_t* ct = reinterpret_cast<uint8_t*>(stage + ct_off);
-  for (int i = 0; i < n; i++) std::memcpy(ct + size_t(i) * 512, d->cell_type + size_t(B.ext[i]) * 512, 512);
-  HIPC(hipMemcpy(S.d_ctype + size_t(tb) * 512, ct, size_t(n) * 512, hipMemcpyHostToDevice));
+  up(S.d_ctype + size_t(tb) * 512, o_ct, ct_bytes);
+  if (ring) ring_release_after_copies(S, ring, total);
   // face masks on the server's stream, ahead of the instance's first request
   k_face_mask<<<I->all.n, 64, 0, S.stream>>>(S.d_ctype, S.d_fmask, S.d_fsplit, I->L(I->all), I->all.n);
   k_diag<<<I->all.n, 64, 0, S.stream>>>(S.view.ctx(), I->L(I->all), I->all.n, S.d_diag);
