@@ -793,7 +793,7 @@ def main():
     ap.add_argument("--serve-seconds-light", type=float, default=15.0,
                     help="length of each request window of the served_mixed_light sweep")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
-    ap.add_argument("--serve-rate-light", default="0.45,0.55,0.65",
+    ap.add_argument("--serve-rate-light", default="0.5,0.65,0.8",
                     help="offered rates of the served_mixed_light sweep: comma list, as fractions of what served_mixed "
                          "completed per s when below 1.5, else requests/s")
     ap.add_argument("--light-p99-ms", type=float, default=50.0,
