@@ -51,8 +51,8 @@ PEAK_HBM_GBS = 8000.0
 def plain_drop_in(ca, items, solver_kw, reps=1):
     """The plain FP32 drop-in on the given problems: one handle per problem,
     solved one after another from x = 0 (as GMGSolver::solve starts,
-    src/GMGSolver.cu:537) to 1e-6 of the recursive residual; one warm-up pass
-    captures each handle's graph.  Returns solves/s and the latency spread."""
+    src/GMGSolver.cu:537) to 1e-6 of the recursive residual, after one warm-up
+    pass.  Returns solves/s and the latency spread."""
     S = ca.GMGSolver(1.0, 1.0)
     probs = [ca.Problem(t) for t, _ in items]
     lat = []
@@ -236,7 +236,7 @@ def served_light_sweep(ca, solver_kw, scenes, args, budget, capacity):
     return best
 
 
-def run_batched(ca, scenes, B, reps, solver_kw, use_graph=True, storage=0, refine=False, tune=False, t_build=0.0,
+def run_batched(ca, scenes, B, reps, solver_kw, use_graph=None, storage=0, refine=False, tune=False, t_build=0.0,
                 oracle_sample=False):
     """Solve nprob independent problems together; returns the form's record.
     storage: V-cycle storage (0 FP32, 1 BF16).  refine: FP64 iterative
@@ -271,7 +271,7 @@ def run_batched(ca, scenes, B, reps, solver_kw, use_graph=True, storage=0, refin
                                 smoother=solver_kw["smoother"], use_graph=use_graph, storage=storage, verify_f64=True)
         return time.perf_counter() - t, res
 
-    once()  # warm-up (graph capture)
+    once()  # warm-up
     times, res = [], None
     for _ in range(reps):
         dt, res = once()
@@ -777,7 +777,9 @@ def main():
                     help="one-launch red-black iterations on the measured handles: 1 on, 0 off, -1 the default")
     ap.add_argument("--no-oracle-check", action="store_true",
                     help="skip the FP64-oracle errors of the mixed-precision results (host solves after the timing)")
-    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="always replay one CG iteration's hipGraph (default: the library chooses per handle; "
+                         "results identical)")
     ap.add_argument("--eager-max", type=int, default=-1,
                     help="levels of at most this many tiles use the one-wait prolongation pass (-1: the library default)")
     ap.add_argument("--fused16-max", type=int, default=-1,
@@ -860,7 +862,7 @@ def main():
 
     def step():
         return S.solve(P, False, 1000, 1e-6, solver_kw["level_iters"], solver_kw["coarsest_iters"], 1, False,
-                       smoother=solver_kw["smoother"], use_graph=not args.no_graph)
+                       smoother=solver_kw["smoother"], use_graph=True if args.graph else None)
 
     # the solve overwrites b with the residual: re-upload b before each solve,
     # outside the timed call (a host->device copy is not part of the solve)
@@ -916,7 +918,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": desc, "tiles": P.n, "leaf_tiles": P.n_leaf, "max_level": L,
-                   "leaf_cells": P.n_leaf * 512, "iterations": iters, "graph": not args.no_graph},
+                   "leaf_cells": P.n_leaf * 512, "iterations": iters, "graph": args.graph},
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "build_s": t_build,
@@ -975,7 +977,7 @@ def main():
         if "plain_drop_in" in wanted:
             base = plain_drop_in(ca, items, solver_kw)
             forms["plain_drop_in_batch_problems"] = dict(base, workload="the batched forms' problems, FP32, one handle "
-                                                         "each, solved one after another from x = 0 (graph replay)")
+                                                         "each, solved one after another from x = 0")
             note(f"plain drop-in on the batch problems: {base['solves_per_s']:.1f} solves/s")
         t0 = time.perf_counter()
         B = ca.Problem.batch([sc.tree for sc in scenes])
@@ -989,7 +991,7 @@ def main():
                                             ("batched_mixed_f16", 2, True, False), ("batched_autotuned", 0, False, True)]:
             if name not in wanted:
                 continue
-            f = run_batched(ca, scenes, B, args.batch_reps, solver_kw, use_graph=not args.no_graph, storage=storage,
+            f = run_batched(ca, scenes, B, args.batch_reps, solver_kw, use_graph=True if args.graph else None, storage=storage,
                             refine=refine, tune=tune, t_build=t_build_b, oracle_sample=name == "batched_mixed")
             chk = f.pop("_check")
             if chk:

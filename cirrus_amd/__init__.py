@@ -545,6 +545,11 @@ def ReCenterLeafVoxels(prob, channel):
     _check(lib().cg_recenter_leaf_voxels(prob.handle, channel), "ReCenterLeafVoxels")
 
 
+def _graph_mode(use_graph):
+    """cg_params.use_graph from a Python value: None -> -1 (chosen by the handle)."""
+    return -1 if use_graph is None else int(bool(use_graph))
+
+
 def default_params(**kw):
     p = Params()
     lib().cg_params_default(ctypes.byref(p))
@@ -566,12 +571,12 @@ class AMGSolver:
         _check(lib().cg_amg_prepare(prob.handle, self.one_over_alpha), "AMGSolver.prepareTypesAndCoeffs")
 
     def solve(self, prob, verbose=False, max_iters=1000, relative_tolerance=1e-6, level_iters=1, coarsest_iters=10,
-              sync_stride=1, is_pure_neumann=False, use_graph=True):
+              sync_stride=1, is_pure_neumann=False, use_graph=None):
         """Returns (iters, relative residual) of the first problem like the reference."""
         p = default_params(max_iters=max_iters, rel_tol=relative_tolerance, level_iters=level_iters,
                            coarsest_iters=coarsest_iters, sync_stride=int(sync_stride),
                            pure_neumann=int(is_pure_neumann), one_over_alpha=self.one_over_alpha,
-                           prolong_coeff=self.prolong_coeff, use_graph=int(use_graph))
+                           prolong_coeff=self.prolong_coeff, use_graph=_graph_mode(use_graph))
         res = (Result * prob.num_problems)()
         _check(lib().cg_amg_solve(prob.handle, ctypes.byref(p), res), "AMGSolver.solve")
         self.last_batch = [r.as_dict() for r in res]
@@ -597,7 +602,7 @@ class GMGSolver:
         self.prolong_coeff = float(prolong_coeff)
 
     def solve(self, prob, verbose=False, max_iters=1000, relative_tolerance=1e-6, level_iters=1, coarsest_iters=10,
-              sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, use_graph=True,
+              sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, use_graph=None,
               storage=STORE_F32, verify_f64=False):
         """Returns (iters, relative residual) like the reference; the full result
         is kept in ``self.last`` (with verify_f64, ``rel_res_f64`` is the FP64
@@ -606,7 +611,7 @@ class GMGSolver:
                            coarsest_iters=coarsest_iters, smoother=smoother, omega=omega,
                            sync_stride=int(sync_stride), pure_neumann=int(is_pure_neumann),
                            one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff,
-                           use_graph=int(use_graph), storage=int(storage), verify_f64=int(verify_f64))
+                           use_graph=_graph_mode(use_graph), storage=int(storage), verify_f64=int(verify_f64))
         r = Result()
         _check(lib().cg_solve(prob.handle, ctypes.byref(p), ctypes.byref(r)), "GMGSolver.solve")
         self.last = r.as_dict()
@@ -620,7 +625,7 @@ class GMGSolver:
                "GMGSolver.VCycle")
 
     def solve_batch(self, prob, max_iters=1000, relative_tolerance=1e-6, level_iters=1, coarsest_iters=10,
-                    sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, use_graph=True,
+                    sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS, omega=2.0 / 3.0, use_graph=None,
                     storage=STORE_F32, verify_f64=False):
         """All problems of a batched :class:`Problem`; returns one result dict
         per problem (iters, converged, rel_res, bb, rr, seconds, rel_res_f64)."""
@@ -628,7 +633,7 @@ class GMGSolver:
                            coarsest_iters=coarsest_iters, smoother=smoother, omega=omega,
                            sync_stride=int(sync_stride), pure_neumann=int(is_pure_neumann),
                            one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff,
-                           use_graph=int(use_graph), storage=int(storage), verify_f64=int(verify_f64))
+                           use_graph=_graph_mode(use_graph), storage=int(storage), verify_f64=int(verify_f64))
         res = (Result * prob.num_problems)()
         _check(lib().cg_solve_batch(prob.handle, ctypes.byref(p), res), "GMGSolver.solve_batch")
         self.last_batch = [r.as_dict() for r in res]
@@ -636,7 +641,7 @@ class GMGSolver:
 
     def solve_refined(self, prob, relative_tolerance=1e-8, inner_tolerance=1e-4, max_outer=20, max_iters=1000,
                       level_iters=1, coarsest_iters=10, sync_stride=1, is_pure_neumann=False, smoother=SMOOTH_RBGS,
-                      omega=2.0 / 3.0, use_graph=True, storage=STORE_BF16):
+                      omega=2.0 / 3.0, use_graph=None, storage=STORE_BF16):
         """Mixed-precision form: iterative refinement with FP64 x and residual
         around the MGPCG (cg_solve_refined); one result dict per problem
         (outer, iters, converged, rel_res, bb, rr, seconds).  The FP64 x is
@@ -645,7 +650,7 @@ class GMGSolver:
                            coarsest_iters=coarsest_iters, smoother=smoother, omega=omega,
                            sync_stride=int(sync_stride), pure_neumann=int(is_pure_neumann),
                            one_over_alpha=self.one_over_alpha, prolong_coeff=self.prolong_coeff,
-                           use_graph=int(use_graph), storage=int(storage))
+                           use_graph=_graph_mode(use_graph), storage=int(storage))
         rp = RefineParams()
         lib().cg_refine_params_default(ctypes.byref(rp))
         rp.rel_tol, rp.inner_rel_tol, rp.max_outer = float(relative_tolerance), float(inner_tolerance), int(max_outer)

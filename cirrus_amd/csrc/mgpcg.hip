@@ -1253,7 +1253,12 @@ void solve(cg_problem& P, const cg_params& prm, cg_result* res, const double* to
   v.amg = amg ? 1 : 0;
   std::snprintf(keybuf, sizeof(keybuf), "%d/%d/%d/%a/%a/%a/%d/%d/%d/%d", v.nu, v.coarsest, v.smoother, double(v.omega),
                 double(v.ooa), double(v.pcoef), prm.pure_neumann, prm.storage, P.swz, v.amg);
-  const bool use_graph = prm.use_graph != 0;
+  // -1: replay where launches dominate an iteration (the AMG V-cycle; handles
+  // of fewer than kStreamMinTiles tiles), stream launches otherwise, which
+  // measured faster on larger single problems (a graph node costs less when
+  // empty, more once its kernel moves data)
+  constexpr int kStreamMinTiles = 4096;
+  const bool use_graph = prm.use_graph > 0 || (prm.use_graph < 0 && (amg || P.n < kStreamMinTiles));
   if (use_graph && P.graph_key != keybuf) {
     if (P.graph) HIPC(hipGraphExecDestroy(P.graph));
     P.graph = nullptr;
@@ -1829,7 +1834,7 @@ void cg_params_default(cg_params* p) {
   p->pure_neumann = 0;
   p->one_over_alpha = 1.0f;
   p->prolong_coeff = 1.0f;
-  p->use_graph = 1;
+  p->use_graph = -1;  // by the handle: replay for AMG and small handles, stream launches for large ones
   p->storage = CG_STORE_F32;
   p->verify_f64 = 0;
 }

@@ -87,7 +87,10 @@ typedef struct {
   int32_t pure_neumann;     /* re-centre r before each V-cycle (src/GMGSolver.cu:556,621) */
   float one_over_alpha;     /* restriction scale (1) */
   float prolong_coeff;      /* prolongation scale (1) */
-  int32_t use_graph;        /* capture one CG iteration in a hipGraph and replay it (1) */
+  int32_t use_graph;        /* 1: capture one CG iteration in a hipGraph and replay it; 0: launch each
+                               kernel on the handle's stream; -1: replay for the AMG preconditioner and
+                               handles of fewer than 4,096 tiles, stream launches otherwise (-1;
+                               results identical in every mode) */
   int32_t storage;          /* CG_STORE_F32 (0, the reference), CG_STORE_BF16 or CG_STORE_F16 */
   int32_t verify_f64;       /* recompute |b - A x| / |b| in FP64 at the end (cg_result.rel_res_f64) (0) */
 } cg_params;
