@@ -918,7 +918,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": desc, "tiles": P.n, "leaf_tiles": P.n_leaf, "max_level": L,
-                   "leaf_cells": P.n_leaf * 512, "iterations": iters, "graph": args.graph},
+                   "leaf_cells": P.n_leaf * 512, "iterations": iters,
+                   "launch": "hipGraph replay" if args.graph else "chosen per handle (stream launches at this size)"},
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "build_s": t_build,
